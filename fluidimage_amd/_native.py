@@ -1,0 +1,153 @@
+"""ctypes binding of libpiv_hip.so (the C ABI declared in include/piv_hip.h).
+
+The library is built in-tree (``fluidimage_amd/libpiv_hip.so``) by
+``fluidimage_amd.build``. There is no fallback: if the library cannot be
+loaded every compute entry point raises ``NativeLibraryError``.
+"""
+
+import ctypes
+import os
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libpiv_hip.so"
+
+PIV_OK = 0
+PIV_U8, PIV_U16, PIV_F32 = 0, 1, 2
+PIV_SUBPIX = {"2d_gaussian2": 0, "centroid": 1, "no_subpix": 2}
+PIV_WINFLAG_INVALID = 1
+
+WIN_OK = 0
+WIN_PEAK_BOUNDARY = 1
+WIN_PEAK_NAN = 2
+WIN_SUBPIX_BOUNDARY = 3
+WIN_SUBPIX_WRONG = 4
+WIN_BAD_CROP = 5
+WIN_NO_PEAK = 6
+
+EXPLANATIONS = {
+    WIN_PEAK_BOUNDARY: "Correlation peak touching boundary.",
+    WIN_PEAK_NAN: "Correlation peak touching nan.",
+    WIN_SUBPIX_BOUNDARY: "close boundary",
+    WIN_SUBPIX_WRONG: "wrong subpix",
+    WIN_BAD_CROP: "Bad im_crop shape.",
+    WIN_NO_PEAK: "No valid correlation value in the search box.",
+}
+
+
+class NativeLibraryError(RuntimeError):
+    pass
+
+
+class PivImage(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("dtype", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("pitch", ctypes.c_int64),
+        ("minval", ctypes.c_void_p),
+    ]
+
+
+class PivWindow(ctypes.Structure):
+    _fields_ = [
+        ("image0", ctypes.c_int32),
+        ("image1", ctypes.c_int32),
+        ("y0", ctypes.c_int32),
+        ("x0", ctypes.c_int32),
+        ("y1", ctypes.c_int32),
+        ("x1", ctypes.c_int32),
+        ("dx_in", ctypes.c_int32),
+        ("dy_in", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class PivResults(ctypes.Structure):
+    _fields_ = [
+        ("dx", ctypes.c_void_p),
+        ("dy", ctypes.c_void_p),
+        ("correl_max", ctypes.c_void_p),
+        ("err", ctypes.c_void_p),
+        ("ix", ctypes.c_void_p),
+        ("iy", ctypes.c_void_p),
+        ("norm", ctypes.c_void_p),
+        ("sec", ctypes.c_void_p),
+        ("nsec", ctypes.c_void_p),
+        ("sec_sub", ctypes.c_void_p),
+    ]
+
+
+# exported symbol -> (restype, argtypes); mirrors include/piv_hip.h
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+SIGNATURES = {
+    "piv_version": (ctypes.c_char_p, []),
+    "piv_last_error": (ctypes.c_char_p, []),
+    "piv_shape_supported": (ctypes.c_int, [_I, _I]),
+    "piv_plan_create": (
+        ctypes.c_int,
+        [ctypes.POINTER(_P), _I, _I, ctypes.c_double, _P, _P, _I, _I, _I, _I],
+    ),
+    "piv_plan_destroy": (ctypes.c_int, [_P]),
+    "piv_images_min": (ctypes.c_int, [_P, _I, _P, _P]),
+    "piv_correlate_batch": (
+        ctypes.c_int,
+        [_P, _I, _P, _P, _I, _I, _P, ctypes.POINTER(PivResults), _P],
+    ),
+    "piv_peaks_from_maps": (ctypes.c_int, [_P, _P, _P, _I, _I, ctypes.POINTER(PivResults), _P]),
+    "piv_subpix_at": (ctypes.c_int, [_P, _P, _P, _P, _I, _P, _P, _P, _P]),
+    "piv_make_windows": (
+        ctypes.c_int,
+        [_I, _I, _P, _P, _P, _P, _I, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P],
+    ),
+    "piv_fix": (
+        ctypes.c_int,
+        [_P, _P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P,
+         _P, _P],
+    ),
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Load (once) and return the ctypes library handle."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else Path(os.environ.get("PIV_HIP_LIB", LIB_PATH))
+    if not p.exists():
+        raise NativeLibraryError(
+            f"{p} not found: build it with `python -m fluidimage_amd.build` "
+            "(the HIP library is required; there is no CPU path)"
+        )
+    try:
+        lib = ctypes.CDLL(str(p))
+    except OSError as exc:
+        raise NativeLibraryError(f"cannot load {p}: {exc}") from exc
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status, what=""):
+    if status != PIV_OK:
+        msg = load().piv_last_error().decode()
+        raise RuntimeError(f"{what} failed (status {status}): {msg}")
+
+
+def stream_ptr(stream=None):
+    """Raw hipStream_t of a torch stream (current stream when None)."""
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)

@@ -1,0 +1,328 @@
+// Device-side building blocks of the PIV correlation path (gfx950).
+//
+//   * window load from HBM with the reference's zero padding and min offset
+//     (src/fluidimage/works/piv/singlepass.py:196-208,391-407);
+//   * peak search with the reference's tie rule, error rules and second
+//     chance (src/fluidimage/calcul/correl.py:88-164,226-275);
+//   * 2d_gaussian2 / centroid sub-pixel fit (src/fluidimage/calcul/subpix.py:16-218).
+//
+// A window is served by a group of T lanes (T a power of two dividing the
+// 64-lane wavefront), so every cross-lane reduction is a __shfl_xor within
+// the group and never needs a workgroup barrier.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/piv_hip.h"
+
+namespace fia {
+
+constexpr int kMaxSquares = 8;  // 1 second-chance square + up to 7 secondary peaks
+
+constexpr int lanes_for(int n) {
+  int t = 8;
+  while (t < n) t *= 2;
+  return t;
+}
+
+template <int DT>
+__device__ __forceinline__ float fetch_px(const void* data, int64_t idx) {
+  if constexpr (DT == PIV_U8) return float(static_cast<const uint8_t*>(data)[idx]);
+  else if constexpr (DT == PIV_U16) return float(static_cast<const uint16_t*>(data)[idx]);
+  else return static_cast<const float*>(data)[idx];
+}
+
+template <class V>
+__device__ __forceinline__ V group_sum(V v, int T) {
+  for (int off = T / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, T);
+  return v;
+}
+
+// numpy bounds of a[start:stop] on an axis of length n
+__device__ __forceinline__ void py_slice(int start, int stop, int n, int& lo, int& hi) {
+  if (start < 0) {
+    start += n;
+    if (start < 0) start = 0;
+  } else if (start > n) {
+    start = n;
+  }
+  if (stop < 0) {
+    stop += n;
+    if (stop < 0) stop = 0;
+  } else if (stop > n) {
+    stop = n;
+  }
+  lo = start;
+  hi = stop;
+}
+
+struct PlanArgs {
+  const uint8_t* mask;  // where_large_displacement (n0*n1 bytes) or nullptr
+  int b0s, b0e, b1s, b1e;
+  int subpix_method;
+  int nsubpix;
+  int particle_radius;
+  int nb_peaks;
+  int has_dmax;
+};
+
+// Peak search state of one window: the LDS map plus the NaN squares
+// written so far (the reference NaNs (2r+1)^2 squares into the map before a
+// second attempt and between secondary peaks, correl.py:234-237,256-259).
+template <int N0, int N1, int T>
+struct Search {
+  const float* m;
+  const uint8_t* mask;
+  int b0s, b0e, b1s, b1e;
+  int sy0[kMaxSquares], sy1[kMaxSquares], sx0[kMaxSquares], sx1[kMaxSquares];
+  int nsq;
+
+  __device__ __forceinline__ bool valid(int i0, int i1) const {
+    float v = m[i0 * N1 + i1];
+    if (v != v) return false;
+    if (mask != nullptr && mask[i0 * N1 + i1]) return false;
+    bool out = true;
+#pragma unroll
+    for (int k = 0; k < kMaxSquares; ++k)
+      if (k < nsq && i0 >= sy0[k] && i0 < sy1[k] && i1 >= sx0[k] && i1 < sx1[k]) out = false;
+    return out;
+  }
+
+  __device__ __forceinline__ void add_square(int i0, int i1, int r) {
+    int a0, a1, c0, c1;
+    py_slice(i0 - r, i0 + r + 1, N0, a0, a1);
+    py_slice(i1 - r, i1 + r + 1, N1, c0, c1);
+    // constant-index stores keep the square table in registers
+#pragma unroll
+    for (int k = 0; k < kMaxSquares; ++k) {
+      if (k == nsq) {
+        sy0[k] = a0;
+        sy1[k] = a1;
+        sx0[k] = c0;
+        sx1[k] = c1;
+      }
+    }
+    if (nsq < kMaxSquares) ++nsq;
+  }
+
+  // nan_indices_max (correl.py:88-132): last maximal valid element of the
+  // box in row-major order; returns the PIV_WIN_* code of the peak.
+  __device__ int find(int lane, int& oi0, int& oi1) const {
+    float best = 0.f;
+    int blin = -1;
+    for (int c = b1s + lane; c < b1e; c += T) {
+      for (int r = b0s; r < b0e; ++r) {
+        if (!valid(r, c)) continue;
+        float v = m[r * N1 + c];
+        int lin = r * N1 + c;
+        if (blin < 0 || v > best || (v == best && lin > blin)) {
+          best = v;
+          blin = lin;
+        }
+      }
+    }
+    for (int off = T / 2; off > 0; off >>= 1) {
+      float ob = __shfl_xor(best, off, T);
+      int ol = __shfl_xor(blin, off, T);
+      if (ol >= 0 && (blin < 0 || ob > best || (ob == best && ol > blin))) {
+        best = ob;
+        blin = ol;
+      }
+    }
+    if (blin < 0) {
+      oi0 = oi1 = 0;
+      return PIV_WIN_NO_PEAK;
+    }
+    int i0 = blin / N1, i1 = blin % N1;
+    oi0 = i0;
+    oi1 = i1;
+    if (i0 == 0 || i0 == N0 - 1 || i1 == 0 || i1 == N1 - 1) return PIV_WIN_PEAK_BOUNDARY;
+    for (int d0 = -1; d0 <= 1; ++d0)
+      for (int d1 = -1; d1 <= 1; ++d1)
+        if (!valid(i0 + d0, i1 + d1)) return PIV_WIN_PEAK_NAN;
+    return PIV_WIN_OK;
+  }
+};
+
+// correl_max = correl[iy, ix] / norm with numpy-2 float32 promotion
+// (correl.py:153-157); 0 when norm == 0.
+__device__ __forceinline__ float peak_value(float v, double norm) {
+  return norm == 0.0 ? 0.f : v / float(norm);
+}
+
+// compute_subpix_2d_gaussian2 (subpix.py:16-61): float32 normalisation of
+// the 3x3 crop, then the log-quadratic fit.
+template <int N1>
+__device__ void gaussian2(const float* m, int ix, int iy, double& dpx, double& dpy) {
+  float c[9];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) c[3 * a + b] = m[(iy - 1 + a) * N1 + (ix - 1 + b)];
+  float mn = c[0];
+  for (int k = 1; k < 9; ++k) mn = fminf(mn, c[k]);
+  for (int k = 0; k < 9; ++k) c[k] -= mn;
+  float mx = c[0];
+  for (int k = 1; k < 9; ++k) mx = fmaxf(mx, c[k]);
+  for (int k = 0; k < 9; ++k) c[k] /= mx;
+  double c10 = 0, c01 = 0, c11 = 0, c20 = 0, c02 = 0;
+  for (int a = 0; a < 3; ++a) {
+    for (int b = 0; b < 3; ++b) {
+      float v = c[3 * a + b];
+      if (v == 0.f) v = 1e-8f;
+      double l = log(double(v));
+      int x = b - 1, y = a - 1;
+      c10 += x * l;
+      c01 += y * l;
+      c11 += x * y * l;
+      c20 += (3 * x * x - 2) * l;
+      c02 += (3 * y * y - 2) * l;
+    }
+  }
+  c10 /= 6;
+  c01 /= 6;
+  c11 /= 4;
+  c20 /= 6;
+  c02 /= 6;
+  double den = 4 * c20 * c02 - c11 * c11;
+  dpx = (c11 * c01 - 2 * c10 * c02) / den;
+  dpy = (c11 * c10 - 2 * c01 * c20) / den;
+}
+
+template <int N1>
+__device__ void centroid(const float* m, int ix, int iy, int n, double& dpx, double& dpy) {
+  double s = 0, sx = 0, sy = 0;
+  for (int a = -n; a <= n; ++a)
+    for (int b = -n; b <= n; ++b) {
+      double v = m[(iy + a) * N1 + (ix + b)];
+      s += v;
+      sx += b * v;
+      sy += a * v;
+    }
+  dpx = sx / s;
+  dpy = sy / s;
+}
+
+// SubPix.compute_subpix (subpix.py:107-218) -> PIV_WIN_* code
+template <int N0, int N1>
+__device__ int subpix(const float* m, int method, int nsub, int ix, int iy, double& dpx,
+                      double& dpy) {
+  dpx = dpy = 0.0;
+  if (iy - nsub < 0 || iy + nsub + 1 > N0 || ix - nsub < 0 || ix + nsub + 1 > N1)
+    return PIV_WIN_SUBPIX_BOUNDARY;
+  double thr = 2.0 * (0.5 + nsub) * (0.5 + nsub);
+  if (method == PIV_SUBPIX_NONE) return PIV_WIN_OK;
+  if (method == PIV_SUBPIX_2D_GAUSSIAN2) {
+    gaussian2<N1>(m, ix, iy, dpx, dpy);
+    if (dpx * dpx + dpy * dpy > thr) method = PIV_SUBPIX_CENTROID;
+    else return PIV_WIN_OK;
+  }
+  centroid<N1>(m, ix, iy, nsub, dpx, dpy);
+  if (dpx * dpx + dpy * dpy > thr) return PIV_WIN_SUBPIX_WRONG;
+  return PIV_WIN_OK;
+}
+
+// Everything that follows the correlation map of one window: peak search
+// with second chance, secondary peaks, sub-pixel fit and the result
+// records. `m` is the LDS map (reference layout), `win` the window index.
+// Returns the second-chance square (rows [*q0, *q1), cols [*q2, *q3)) when
+// one was written, for the map side effect with displacement_max None.
+template <int N0, int N1, int T>
+__device__ void finish_window(const float* m, const uint8_t* mask, const PlanArgs& pa, int lane,
+                              int win, double norm, int apply_subpix, int dx_in, int dy_in,
+                              const piv_results& res, int* sq) {
+  constexpr int iy0 = N0 / 2 - 1, ix0 = N1 / 2 - 1;
+  Search<N0, N1, T> s;
+  s.m = m;
+  s.mask = mask;
+  s.b0s = pa.b0s;
+  s.b0e = pa.b0e;
+  s.b1s = pa.b1s;
+  s.b1e = pa.b1e;
+  s.nsq = 0;
+  sq[0] = sq[1] = sq[2] = sq[3] = 0;
+  int i0, i1;
+  int code = s.find(lane, i0, i1);
+  float cmax = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : peak_value(m[i0 * N1 + i1], norm);
+  if (code != PIV_WIN_OK && code != PIV_WIN_NO_PEAK) {
+    // second chance (correl.py:231-246)
+    s.add_square(i0, i1, pa.particle_radius);
+    sq[0] = s.sy0[0];
+    sq[1] = s.sy1[0];
+    sq[2] = s.sx0[0];
+    sq[3] = s.sx1[0];
+    int j0, j1;
+    int code2 = s.find(lane, j0, j1);
+    if (code2 == PIV_WIN_OK) {
+      i0 = j0;
+      i1 = j1;
+      cmax = peak_value(m[i0 * N1 + i1], norm);
+      code = PIV_WIN_OK;
+    } else {
+      code = code2;
+    }
+  }
+  const bool writer = lane == 0;
+  if (code != PIV_WIN_OK) {
+    if (writer) {
+      float dx = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : float(ix0 - i1);
+      float dy = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : float(iy0 - i0);
+      res.dx[win] = dx + float(dx_in);
+      res.dy[win] = dy + float(dy_in);
+      res.correl_max[win] = cmax;
+      res.err[win] = code;
+      if (res.ix) res.ix[win] = i1;
+      if (res.iy) res.iy[win] = i0;
+      if (res.nsec) res.nsec[win] = 0;
+    }
+    return;
+  }
+  // secondary peaks (correl.py:250-270)
+  int nsec = 0;
+  if (pa.nb_peaks > 1) {
+    int c0 = i0, c1 = i1;
+    for (int k = 0; k < pa.nb_peaks - 1; ++k) {
+      s.add_square(c0, c1, pa.particle_radius);
+      int j0, j1;
+      int c = s.find(lane, j0, j1);
+      if (c != PIV_WIN_OK) break;
+      c0 = j0;
+      c1 = j1;
+      if (writer) {
+        size_t o = (size_t(win) * (pa.nb_peaks - 1) + k);
+        res.sec[3 * o + 0] = float(ix0 - j1);
+        res.sec[3 * o + 1] = float(iy0 - j0);
+        res.sec[3 * o + 2] = peak_value(m[j0 * N1 + j1], norm);
+        if (res.sec_sub) {
+          double px, py;
+          int e = subpix<N0, N1>(m, pa.subpix_method, pa.nsubpix, j1, j0, px, py);
+          res.sec_sub[2 * o + 0] = e ? __builtin_nanf("") : float(ix0 - (j1 + px));
+          res.sec_sub[2 * o + 1] = e ? __builtin_nanf("") : float(iy0 - (j0 + py));
+        }
+      }
+      ++nsec;
+    }
+  }
+  if (writer) {
+    float dx = float(ix0 - i1), dy = float(iy0 - i0);
+    int err = PIV_WIN_OK;
+    if (apply_subpix) {
+      double px, py;
+      int e = subpix<N0, N1>(m, pa.subpix_method, pa.nsubpix, i1, i0, px, py);
+      if (e == PIV_WIN_OK) {
+        dx = float(ix0 - (i1 + px));
+        dy = float(iy0 - (i0 + py));
+      } else {
+        err = e;
+      }
+    }
+    res.dx[win] = dx + float(dx_in);
+    res.dy[win] = dy + float(dy_in);
+    res.correl_max[win] = cmax;
+    res.err[win] = err;
+    if (res.ix) res.ix[win] = i1;
+    if (res.iy) res.iy[win] = i0;
+    if (res.nsec) res.nsec[win] = nsec;
+  }
+}
+
+}  // namespace fia

@@ -1,0 +1,753 @@
+// libpiv_hip: MI355X (gfx950) kernels and C ABI of the PIV correlation path.
+//
+// Fused window kernel (one group of T lanes per interrogation window,
+// several windows per workgroup):
+//   HBM crop of both images (zero padding + min offset)
+//   -> spatial mean removal (== the reference's DC-bin zeroing)
+//   -> one complex 2-D FFT of z = im0' + i im1' (column pass in registers,
+//      LDS transpose, row pass in registers)
+//   -> split F0 / F1 with Z(-k), half-spectrum energies, cross spectrum
+//      conj(F0) F1
+//   -> inverse 2-D FFT (rows, LDS transpose, columns)
+//   -> map in the reference layout (after _like_fftshift) in LDS
+//   -> masked peak search, second chance, secondary peaks, sub-pixel fit.
+// No global scratch: every intermediate stays in registers or LDS.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/piv_hip.h"
+#include "fft_reg.hpp"
+#include "piv_device.hpp"
+
+using namespace fia;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(PIV_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return PIV_OK;
+}
+
+// -------------------------------------------------------------------------
+// fused correlation kernel
+// -------------------------------------------------------------------------
+template <int N0, int N1>
+struct Shape {
+  static constexpr int T = lanes_for(N0 > N1 ? N0 : N1);
+  static constexpr int NN = N0 * N1;
+};
+
+// rotated LDS layout of the complex N0 x N1 buffer: element (r, c) lives at
+// r * N1 + (c + r) mod N1, so both the column pass (lanes over c) and the
+// row pass (lanes over r) touch distinct banks
+template <int N1>
+__device__ __forceinline__ int rot(int c, int r) {
+  int k = c + r;
+  while (k >= N1) k -= N1;
+  return k;
+}
+
+template <int N0, int N1, int DT, int WPB>
+__global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correlate(
+    const piv_image* __restrict__ images, const piv_window* __restrict__ windows, int nwin,
+    int apply_subpix, float* __restrict__ maps, piv_results res, PlanArgs pa) {
+  constexpr int T = Shape<N0, N1>::T;
+  constexpr int NN = N0 * N1;
+  __shared__ float2 smem[WPB * NN];
+  __shared__ uint8_t smask[NN];
+
+  const int lane = threadIdx.x & (T - 1);
+  const int wl = threadIdx.x / T;
+  const int win = blockIdx.x * WPB + wl;
+  if (pa.mask != nullptr)
+    for (int k = threadIdx.x; k < NN; k += T * WPB) smask[k] = pa.mask[k];
+
+  float2* buf = smem + wl * NN;
+  piv_window w;
+  const bool live = win < nwin;
+  if (live) {
+    w = windows[win];
+  } else {
+    w.flags = PIV_WINFLAG_INVALID;
+    w.dx_in = w.dy_in = 0;
+  }
+  const bool compute = live && !(w.flags & PIV_WINFLAG_INVALID);
+
+  // ---- 1. crop (column `lane` of both windows) -------------------------
+  float2 z[N0];
+  float s0 = 0.f, s1 = 0.f;
+  if (compute && lane < N1) {
+    const piv_image im0 = images[w.image0];
+    const piv_image im1 = images[w.image1];
+    const float m0 = im0.minval ? *im0.minval : 0.f;
+    const float m1 = im1.minval ? *im1.minval : 0.f;
+    const int xa = w.x0 + lane, xb = w.x1 + lane;
+    const bool ina = xa >= 0 && xa < im0.width;
+    const bool inb = xb >= 0 && xb < im1.width;
+#pragma unroll
+    for (int r = 0; r < N0; ++r) {
+      const int ya = w.y0 + r, yb = w.y1 + r;
+      float a = 0.f, b = 0.f;
+      if (ina && ya >= 0 && ya < im0.height)
+        a = fetch_px<DT>(im0.data, int64_t(ya) * im0.pitch + xa) - m0;
+      if (inb && yb >= 0 && yb < im1.height)
+        b = fetch_px<DT>(im1.data, int64_t(yb) * im1.pitch + xb) - m1;
+      z[r] = make_float2(a, b);
+      s0 += a;
+      s1 += b;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < N0; ++r) z[r] = make_float2(0.f, 0.f);
+  }
+  // ---- 2. mean removal ---------------------------------------------------
+  s0 = group_sum(s0, T);
+  s1 = group_sum(s1, T);
+  if (lane < N1) {
+    const float mean0 = s0 / float(NN), mean1 = s1 / float(NN);
+#pragma unroll
+    for (int r = 0; r < N0; ++r) {
+      z[r].x -= mean0;
+      z[r].y -= mean1;
+    }
+  }
+  // ---- 3. forward 2-D FFT of z = im0' + i im1' -------------------------
+  fft<N0, -1>(z);
+  if (lane < N1) {
+#pragma unroll
+    for (int r = 0; r < N0; ++r) buf[r * N1 + rot<N1>(lane, r)] = z[r];
+  }
+  __syncthreads();
+  float2 y[N1];
+  const int r = lane;  // row owned in the row pass
+  if (r < N0) {
+#pragma unroll
+    for (int c = 0; c < N1; ++c) y[c] = buf[r * N1 + rot<N1>(c, r)];
+  } else {
+#pragma unroll
+    for (int c = 0; c < N1; ++c) y[c] = make_float2(0.f, 0.f);
+  }
+  fft<N1, -1>(y);
+  __syncthreads();
+  if (r < N0) {
+#pragma unroll
+    for (int c = 0; c < N1; ++c) buf[r * N1 + rot<N1>(c, r)] = y[c];
+  }
+  __syncthreads();
+  // ---- 4. F0, F1 from Z(k), Z(-k); energies; cross spectrum ---------------
+  double e0 = 0.0, e1 = 0.0;
+  if (r < N0) {
+    const int pr = r == 0 ? 0 : N0 - r;
+#pragma unroll
+    for (int c = 0; c < N1; ++c) {
+      const int cm = c == 0 ? 0 : N1 - c;
+      const float2 zm = buf[pr * N1 + rot<N1>(cm, pr)];
+      const float2 a = y[c];
+      const float2 b = make_float2(zm.x, -zm.y);
+      float2 f0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+      const float2 d = make_float2(a.x - b.x, a.y - b.y);
+      float2 f1 = make_float2(0.5f * d.y, -0.5f * d.x);
+      if (r == 0 && c == 0) {
+        f0 = make_float2(0.f, 0.f);
+        f1 = make_float2(0.f, 0.f);
+      }
+      if (c <= N1 / 2) {
+        const double wgt = (c == 0 || c == N1 / 2) ? 1.0 : 2.0;
+        e0 += wgt * (double(f0.x) * f0.x + double(f0.y) * f0.y);
+        e1 += wgt * (double(f1.x) * f1.x + double(f1.y) * f1.y);
+      }
+      // conj(f0) * f1
+      y[c] = make_float2(f0.x * f1.x + f0.y * f1.y, f0.x * f1.y - f0.y * f1.x);
+    }
+  }
+  e0 = group_sum(e0, T);
+  e1 = group_sum(e1, T);
+  const double norm = sqrt(e0 * e1);
+  // ---- 5. inverse 2-D FFT (rows, then columns) ------------------------
+  fft<N1, +1>(y);
+  __syncthreads();
+  if (r < N0) {
+#pragma unroll
+    for (int c = 0; c < N1; ++c) buf[r * N1 + rot<N1>(c, r)] = y[c];
+  }
+  __syncthreads();
+  if (lane < N1) {
+#pragma unroll
+    for (int rr = 0; rr < N0; ++rr) z[rr] = buf[rr * N1 + rot<N1>(lane, rr)];
+  }
+  fft<N0, +1>(z);
+  __syncthreads();
+  // ---- 6. map in the reference layout: arr[a0, a1] -> out[g(a0), g(a1)],
+  //         g(a) = (n - 1 - a + n // 2) mod n  (inverse of _like_fftshift)
+  float* mapf = reinterpret_cast<float*>(buf);
+  if (lane < N1) {
+    const int g1 = (N1 - 1 - lane + N1 / 2) % N1;
+#pragma unroll
+    for (int a0 = 0; a0 < N0; ++a0) {
+      const int g0 = (N0 - 1 - a0 + N0 / 2) % N0;
+      mapf[g0 * N1 + g1] = z[a0].x;
+    }
+  }
+  __syncthreads();
+  // ---- 7. peak search, sub-pixel fit, records ---------------------------
+  int sq[4] = {0, 0, 0, 0};
+  if (compute) {
+    if (lane == 0 && res.norm) res.norm[win] = norm;
+    finish_window<N0, N1, T>(mapf, pa.mask ? smask : nullptr, pa, lane, win, norm,
+                             apply_subpix, w.dx_in, w.dy_in, res, sq);
+  } else if (live && lane == 0) {
+    const float nan = __builtin_nanf("");
+    res.dx[win] = nan;
+    res.dy[win] = nan;
+    res.correl_max[win] = nan;
+    res.err[win] = PIV_WIN_BAD_CROP;
+    if (res.norm) res.norm[win] = nan;
+    if (res.nsec) res.nsec[win] = 0;
+  }
+  if (maps != nullptr && live) {
+    float* out = maps + size_t(win) * NN;
+    const bool side = !pa.has_dmax;
+    for (int k = lane; k < NN; k += T) {
+      float v = compute ? mapf[k] : __builtin_nanf("");
+      if (side) {
+        const int i0 = k / N1, i1 = k % N1;
+        if (i0 >= sq[0] && i0 < sq[1] && i1 >= sq[2] && i1 < sq[3]) v = __builtin_nanf("");
+      }
+      out[k] = v;
+    }
+  }
+}
+
+// peak search (+ sub-pixel) on maps given by the caller
+template <int N0, int N1, int WPB>
+__global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_peaks(float* __restrict__ maps,
+                                                                 const double* __restrict__ norms,
+                                                                 int nwin, int apply_subpix,
+                                                                 piv_results res, PlanArgs pa) {
+  constexpr int T = Shape<N0, N1>::T;
+  constexpr int NN = N0 * N1;
+  __shared__ float smem[WPB * NN];
+  __shared__ uint8_t smask[NN];
+  const int lane = threadIdx.x & (T - 1);
+  const int wl = threadIdx.x / T;
+  const int win = blockIdx.x * WPB + wl;
+  if (pa.mask != nullptr)
+    for (int k = threadIdx.x; k < NN; k += T * WPB) smask[k] = pa.mask[k];
+  float* m = smem + wl * NN;
+  const bool live = win < nwin;
+  if (live)
+    for (int k = lane; k < NN; k += T) m[k] = maps[size_t(win) * NN + k];
+  __syncthreads();
+  if (!live) return;
+  int sq[4];
+  const double norm = norms[win];
+  finish_window<N0, N1, T>(m, pa.mask ? smask : nullptr, pa, lane, win, norm, apply_subpix, 0, 0,
+                           res, sq);
+  if (!pa.has_dmax) {
+    for (int k = lane; k < NN; k += T) {
+      const int i0 = k / N1, i1 = k % N1;
+      if (i0 >= sq[0] && i0 < sq[1] && i1 >= sq[2] && i1 < sq[3])
+        maps[size_t(win) * NN + k] = __builtin_nanf("");
+    }
+  }
+}
+
+template <int N0, int N1>
+__global__ void k_subpix_at(const float* __restrict__ maps, const int32_t* __restrict__ ix,
+                            const int32_t* __restrict__ iy, int nwin, float* out_x,
+                            float* out_y, int32_t* err, PlanArgs pa) {
+  const int win = blockIdx.x * blockDim.x + threadIdx.x;
+  if (win >= nwin) return;
+  double px, py;
+  const int x = ix[win], y = iy[win];
+  const int e = subpix<N0, N1>(maps + size_t(win) * N0 * N1, pa.subpix_method, pa.nsubpix, x, y,
+                               px, py);
+  err[win] = e;
+  out_x[win] = e ? float(x) : float(x + px);
+  out_y[win] = e ? float(y) : float(y + py);
+}
+
+// -------------------------------------------------------------------------
+// shape dispatch
+// -------------------------------------------------------------------------
+struct Launchers {
+  int n0, n1;
+  void (*correlate[3])(dim3, dim3, hipStream_t, const piv_image*, const piv_window*, int, int,
+                       float*, piv_results, PlanArgs);
+  void (*peaks)(dim3, dim3, hipStream_t, float*, const double*, int, int, piv_results, PlanArgs);
+  void (*subpix)(dim3, dim3, hipStream_t, const float*, const int32_t*, const int32_t*, int,
+                 float*, float*, int32_t*, PlanArgs);
+  int wpb;
+  int lanes;
+};
+
+template <int N0, int N1>
+constexpr int default_wpb() {
+  constexpr int T = Shape<N0, N1>::T;
+  int w = 256 / T;
+  while (w > 1 && w * N0 * N1 * 8 > 65536) --w;
+  return w;
+}
+
+template <int N0, int N1, int DT>
+void launch_correlate(dim3 g, dim3 b, hipStream_t s, const piv_image* im, const piv_window* wi,
+                      int n, int sp, float* maps, piv_results r, PlanArgs pa) {
+  hipLaunchKernelGGL((k_correlate<N0, N1, DT, default_wpb<N0, N1>()>), g, b, 0, s, im, wi, n, sp,
+                     maps, r, pa);
+}
+
+template <int N0, int N1>
+void launch_peaks(dim3 g, dim3 b, hipStream_t s, float* maps, const double* norms, int n, int sp,
+                  piv_results r, PlanArgs pa) {
+  hipLaunchKernelGGL((k_peaks<N0, N1, default_wpb<N0, N1>()>), g, b, 0, s, maps, norms, n, sp, r,
+                     pa);
+}
+
+template <int N0, int N1>
+void launch_subpix(dim3 g, dim3 b, hipStream_t s, const float* maps, const int32_t* ix,
+                   const int32_t* iy, int n, float* ox, float* oy, int32_t* e, PlanArgs pa) {
+  hipLaunchKernelGGL((k_subpix_at<N0, N1>), g, b, 0, s, maps, ix, iy, n, ox, oy, e, pa);
+}
+
+template <int N0, int N1>
+constexpr Launchers make_launchers() {
+  return Launchers{N0,
+                   N1,
+                   {&launch_correlate<N0, N1, PIV_U8>, &launch_correlate<N0, N1, PIV_U16>,
+                    &launch_correlate<N0, N1, PIV_F32>},
+                   &launch_peaks<N0, N1>,
+                   &launch_subpix<N0, N1>,
+                   default_wpb<N0, N1>(),
+                   Shape<N0, N1>::T};
+}
+
+const Launchers kShapes[] = {
+    make_launchers<12, 12>(), make_launchers<16, 16>(), make_launchers<20, 20>(),
+    make_launchers<24, 24>(), make_launchers<32, 32>(), make_launchers<40, 40>(),
+    make_launchers<48, 48>(), make_launchers<64, 64>(), make_launchers<64, 32>(),
+    make_launchers<32, 64>(),
+};
+
+const Launchers* find_shape(int n0, int n1) {
+  for (const Launchers& l : kShapes)
+    if (l.n0 == n0 && l.n1 == n1) return &l;
+  return nullptr;
+}
+
+// -------------------------------------------------------------------------
+// image minimum (im - im.min(), singlepass.py:206)
+// -------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t float_key(float f) {
+  uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+  uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(b);
+}
+
+__global__ void k_min_init(uint32_t* keys, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = 0xffffffffu;
+}
+
+__global__ __launch_bounds__(256) void k_min_reduce(const piv_image* __restrict__ images,
+                                                    uint32_t* keys) {
+  const piv_image im = images[blockIdx.y];
+  const int64_t npx = int64_t(im.height) * im.width;
+  float m = INFINITY;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < npx;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t y = i / im.width, x = i - y * im.width;
+    const int64_t idx = y * im.pitch + x;
+    float v;
+    if (im.dtype == PIV_U8) v = fetch_px<PIV_U8>(im.data, idx);
+    else if (im.dtype == PIV_U16) v = fetch_px<PIV_U16>(im.data, idx);
+    else v = fetch_px<PIV_F32>(im.data, idx);
+    m = fminf(m, v);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 64));
+  __shared__ float part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = fminf(fminf(part[0], part[1]), fminf(part[2], part[3]));
+    atomicMin(&keys[blockIdx.y], float_key(b));
+  }
+}
+
+__global__ void k_min_final(uint32_t* keys, float* out, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = key_float(keys[i]);
+}
+
+// -------------------------------------------------------------------------
+// WorkFIX (fix.py:66-133) and mean_neighbors (mean_neighbors.py:5-155)
+// -------------------------------------------------------------------------
+__global__ void k_fix_mark(float* dx, float* dy, const float* cmax, const int32_t* err, int n,
+                           float correl_min, float dmax2, int use_dmax, int32_t* flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int f = 0;
+  float x = dx[i], y = dy[i];
+  if (err[i] != PIV_WIN_OK) f |= 1;
+  if (cmax[i] < correl_min) f |= 2;
+  // delta2 test on the values after the previous NaN assignments
+  float xx = f ? __builtin_nanf("") : x, yy = f ? __builtin_nanf("") : y;
+  if (use_dmax && xx * xx + yy * yy > dmax2) f |= 4;
+  if (f) {
+    dx[i] = __builtin_nanf("");
+    dy[i] = __builtin_nanf("");
+  }
+  flags[i] = f;
+}
+
+// sum over the valid neighbours with the reference's order of additions
+// (mean_neighbors.py:15-115); NaN vectors count as 0 with weight 0
+__device__ __forceinline__ void neigh_sum(const float* f, const float* g, int ny, int nx, int i0,
+                                          int i1, float& sa, float& sb) {
+  auto A = [&](int p, int q) {
+    const float x = f[p * nx + q], y = g[p * nx + q];
+    return (x != x || y != y) ? 0.f : x;
+  };
+  auto B = [&](int p, int q) {
+    const float x = f[p * nx + q], y = g[p * nx + q];
+    return (x != x || y != y) ? 0.f : 1.f;
+  };
+  const int n0 = ny, n1 = nx;
+  if (i0 == 0 && i1 == 0) {
+    sa = A(1, 0) + A(1, 1) + A(0, 1);
+    sb = B(1, 0) + B(1, 1) + B(0, 1);
+  } else if (i0 == 0 && i1 == n1 - 1) {
+    sa = A(1, i1) + A(1, i1 - 1) + A(0, i1 - 1);
+    sb = B(1, i1) + B(1, i1 - 1) + B(0, i1 - 1);
+  } else if (i0 == n0 - 1 && i1 == 0) {
+    sa = A(i0 - 1, 0) + A(i0 - 1, 1) + A(i0, 1);
+    sb = B(i0 - 1, 0) + B(i0 - 1, 1) + B(i0, 1);
+  } else if (i0 == n0 - 1 && i1 == n1 - 1) {
+    sa = A(i0 - 1, i1) + A(i0 - 1, i1 - 1) + A(i0, i1 - 1);
+    sb = B(i0 - 1, i1) + B(i0 - 1, i1 - 1) + B(i0, i1 - 1);
+  } else if (i0 == 0 || i0 == n0 - 1) {
+    const int o = i0 == 0 ? 1 : -1;
+    sa = A(i0, i1 - 1) + A(i0, i1 + 1);
+    sb = B(i0, i1 - 1) + B(i0, i1 + 1);
+    for (int idx = -1; idx < 2; ++idx) {
+      sa += A(i0 + o, i1 - idx);
+      sb += B(i0 + o, i1 - idx);
+    }
+  } else if (i1 == 0 || i1 == n1 - 1) {
+    const int o = i1 == 0 ? 1 : -1;
+    sa = A(i0 - 1, i1) + A(i0 + 1, i1);
+    sb = B(i0 - 1, i1) + B(i0 + 1, i1);
+    for (int idx = -1; idx < 2; ++idx) {
+      sa += A(i0 + idx, i1 + o);
+      sb += B(i0 + idx, i1 + o);
+    }
+  } else {
+    sa = A(i0, i1 - 1) + A(i0, i1 + 1);
+    sb = B(i0, i1 - 1) + B(i0, i1 + 1);
+    for (int idx = -1; idx < 2; ++idx) {
+      sa += A(i0 - 1, i1 - idx) + A(i0 + 1, i1 - idx);
+      sb += B(i0 - 1, i1 - idx) + B(i0 + 1, i1 - idx);
+    }
+  }
+}
+
+// neighbour means, squared differences and the threshold flag; reads the
+// fields as left by k_fix_mark and writes nothing they are read from
+__global__ void k_fix_neigh(const float* dx, const float* dy, int ny, int nx, float thr2,
+                            int use_thr, int32_t* flags, float* diff2, float* mean_x,
+                            float* mean_y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ny * nx) return;
+  const int i0 = i / nx, i1 = i % nx;
+  const float x = dx[i], y = dy[i];
+  float mx = __builtin_nanf(""), my = __builtin_nanf("");
+  if (!(x != x || y != y) && ny > 1 && nx > 1) {
+    float sx, sy, cx, cy;
+    neigh_sum(dx, dy, ny, nx, i0, i1, sx, cx);
+    neigh_sum(dy, dx, ny, nx, i0, i1, sy, cy);
+    if (cx != 0.f) {
+      mx = sx / cx;
+      my = sy / cy;
+    }
+  }
+  const float ddx = mx - x, ddy = my - y;
+  const float d2 = ddx * ddx + ddy * ddy;
+  if (diff2) diff2[i] = d2;
+  if (mean_x) mean_x[i] = mx;
+  if (mean_y) mean_y[i] = my;
+  if (use_thr && d2 > thr2) flags[i] |= 8;
+}
+
+__global__ void k_fix_apply(float* dx, float* dy, const int32_t* flags, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i] & 8) {
+    dx[i] = __builtin_nanf("");
+    dy[i] = __builtin_nanf("");
+  }
+}
+
+// -------------------------------------------------------------------------
+// window records of one pass (singlepass.py:210-260 first pass,
+// :752-753 predictor rounding, :787-870 later passes, :391-407 crop bounds)
+// -------------------------------------------------------------------------
+struct CropGeom {
+  int s0y, s0x, e0y, e0x;  // _start_for_crop0 / _stop_for_crop0
+  int s1y, s1x, e1y, e1x;  // _start_for_crop1 / _stop_for_crop1
+};
+
+__global__ void k_make_windows(int ny, int nx, const int32_t* __restrict__ iyvecs,
+                               const int32_t* __restrict__ ixvecs,
+                               const double* __restrict__ dxa, const double* __restrict__ dya,
+                               int height, int width, int npad, CropGeom g, int image0,
+                               int image1, piv_window* __restrict__ out, double* xs, double* ys,
+                               int32_t* dxin_out, int32_t* dyin_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ny * nx) return;
+  const int gx = ixvecs[i % nx], gy = iyvecs[i / nx];
+  int dxi = 0, dyi = 0;
+  int ixs0 = gx, iys0 = gy, ixs1 = gx, iys1 = gy;
+  if (dxa != nullptr) {
+    // np.round (half to even) then astype(int32)
+    dxi = int(rint(dxa[i]));
+    dyi = int(rint(dya[i]));
+    ixs0 = gx - (dxi >> 1);  // floor division by 2
+    iys0 = gy - (dyi >> 1);
+    ixs1 = ixs0 + dxi;
+    iys1 = iys0 + dyi;
+    const bool out1 = ixs1 > width || ixs1 < 0 || iys1 > height || iys1 < 0;
+    const bool out0 = ixs0 > width || ixs0 < 0 || iys0 > height || iys0 < 0;
+    if (out1) {
+      ixs0 = gx - dxi;
+      iys0 = gy - dyi;
+      ixs1 = gx;
+      iys1 = gy;
+    } else if (out0) {
+      ixs0 = gx;
+      iys0 = gy;
+      ixs1 = gx + dxi;
+      iys1 = gy + dyi;
+    }
+  }
+  if (xs) xs[i] = dxa != nullptr ? (double(ixs0) + double(ixs1)) / 2.0 : double(gx);
+  if (ys) ys[i] = dxa != nullptr ? (double(iys0) + double(iys1)) / 2.0 : double(gy);
+  if (dxin_out) dxin_out[i] = dxi;
+  if (dyin_out) dyin_out[i] = dyi;
+  // crop rows [c - start, c + stop) of the padded image must lie in
+  // [0, H + 2 npad], else the slice has the wrong shape ("Bad im_crop shape.")
+  const int hp = height + 2 * npad, wp = width + 2 * npad;
+  const int py0 = iys0 + npad, px0 = ixs0 + npad, py1 = iys1 + npad, px1 = ixs1 + npad;
+  const bool ok = py0 - g.s0y >= 0 && py0 + g.e0y <= hp && px0 - g.s0x >= 0 &&
+                  px0 + g.e0x <= wp && py1 - g.s1y >= 0 && py1 + g.e1y <= hp &&
+                  px1 - g.s1x >= 0 && px1 + g.e1x <= wp;
+  piv_window w;
+  w.image0 = image0;
+  w.image1 = image1;
+  w.y0 = iys0 - g.s0y;
+  w.x0 = ixs0 - g.s0x;
+  w.y1 = iys1 - g.s1y;
+  w.x1 = ixs1 - g.s1x;
+  w.dx_in = dxi;
+  w.dy_in = dyi;
+  w.flags = ok ? 0 : PIV_WINFLAG_INVALID;
+  w.reserved = 0;
+  out[i] = w;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+struct piv_plan {
+  int n0, n1;
+  const Launchers* l;
+  uint8_t* d_mask;
+  PlanArgs pa;
+};
+
+extern "C" {
+
+const char* piv_version(void) { return "fluidimage_amd piv_hip 0.1 (gfx950)"; }
+
+const char* piv_last_error(void) { return g_last_error.c_str(); }
+
+int piv_shape_supported(int32_t n0, int32_t n1) { return find_shape(n0, n1) != nullptr; }
+
+int piv_plan_create(piv_plan** plan, int32_t n0, int32_t n1, double displacement_max,
+                    const uint8_t* mask, const int32_t* box, int32_t subpix_method,
+                    int32_t nsubpix, int32_t particle_radius, int32_t nb_peaks_to_search) {
+  if (plan == nullptr) return fail(PIV_EINVAL, "plan is NULL");
+  *plan = nullptr;
+  const Launchers* l = find_shape(n0, n1);
+  if (l == nullptr)
+    return fail(PIV_EUNSUPPORTED,
+                "window shape " + std::to_string(n0) + "x" + std::to_string(n1) +
+                    " is not built into libpiv_hip");
+  if (subpix_method < 0 || subpix_method > PIV_SUBPIX_NONE)
+    return fail(PIV_EINVAL, "bad subpix method");
+  if (nsubpix < 1 || 2 * nsubpix + 1 > (n0 < n1 ? n0 : n1))
+    return fail(PIV_EINVAL, "bad nsubpix");
+  if (nb_peaks_to_search < 1 || nb_peaks_to_search > kMaxSquares - 1)
+    return fail(PIV_EUNSUPPORTED, "nb_peaks_to_search must be in [1, 7]");
+  if (particle_radius < 0) return fail(PIV_EINVAL, "bad particle_radius");
+  piv_plan* p = new piv_plan();
+  p->n0 = n0;
+  p->n1 = n1;
+  p->l = l;
+  p->d_mask = nullptr;
+  p->pa.mask = nullptr;
+  p->pa.has_dmax = displacement_max >= 0.0;
+  if (p->pa.has_dmax) {
+    if (mask == nullptr || box == nullptr) {
+      delete p;
+      return fail(PIV_EINVAL, "mask and box are required with displacement_max");
+    }
+    if (hipMalloc(&p->d_mask, size_t(n0) * n1) != hipSuccess ||
+        hipMemcpy(p->d_mask, mask, size_t(n0) * n1, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(p->d_mask);
+      delete p;
+      return fail(PIV_EHIP, "cannot upload the search mask");
+    }
+    p->pa.mask = p->d_mask;
+    p->pa.b0s = box[0];
+    p->pa.b0e = box[1];
+    p->pa.b1s = box[2];
+    p->pa.b1e = box[3];
+    if (box[0] < 0 || box[1] > n0 || box[2] < 0 || box[3] > n1) {
+      (void)hipFree(p->d_mask);
+      delete p;
+      return fail(PIV_EINVAL, "search box outside the window");
+    }
+  } else {
+    p->pa.b0s = 0;
+    p->pa.b0e = n0;
+    p->pa.b1s = 0;
+    p->pa.b1e = n1;
+  }
+  p->pa.subpix_method = subpix_method;
+  p->pa.nsubpix = nsubpix;
+  p->pa.particle_radius = particle_radius;
+  p->pa.nb_peaks = nb_peaks_to_search;
+  *plan = p;
+  return PIV_OK;
+}
+
+int piv_plan_destroy(piv_plan* plan) {
+  if (plan == nullptr) return PIV_OK;
+  if (plan->d_mask) (void)hipFree(plan->d_mask);
+  delete plan;
+  return PIV_OK;
+}
+
+int piv_images_min(const piv_image* images, int32_t nimages, float* out_min, void* stream) {
+  if (nimages <= 0) return PIV_OK;
+  if (images == nullptr || out_min == nullptr) return fail(PIV_EINVAL, "NULL argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // keys live in the output buffer (same size, reinterpreted)
+  uint32_t* keys = reinterpret_cast<uint32_t*>(out_min);
+  hipLaunchKernelGGL(k_min_init, dim3((nimages + 255) / 256), dim3(256), 0, s, keys, nimages);
+  hipLaunchKernelGGL(k_min_reduce, dim3(256, nimages), dim3(256), 0, s, images, keys);
+  hipLaunchKernelGGL(k_min_final, dim3((nimages + 255) / 256), dim3(256), 0, s, keys, out_min,
+                     nimages);
+  return check_launch("piv_images_min");
+}
+
+int piv_correlate_batch(piv_plan* plan, int32_t dtype, const piv_image* images,
+                           const piv_window* windows, int32_t nwin, int32_t apply_subpix,
+                           float* maps, const piv_results* res, void* stream) {
+  if (plan == nullptr || res == nullptr) return fail(PIV_EINVAL, "NULL plan or results");
+  if (nwin <= 0) return PIV_OK;
+  if (dtype < 0 || dtype > PIV_F32) return fail(PIV_EINVAL, "bad dtype");
+  if (images == nullptr || windows == nullptr || res->dx == nullptr || res->dy == nullptr ||
+      res->correl_max == nullptr || res->err == nullptr)
+    return fail(PIV_EINVAL, "NULL image, window or result array");
+  if (plan->pa.nb_peaks > 1 && (res->sec == nullptr || res->nsec == nullptr))
+    return fail(PIV_EINVAL, "secondary peak arrays required when nb_peaks_to_search > 1");
+  const Launchers* l = plan->l;
+  const int wpb = l->wpb;
+  dim3 grid((nwin + wpb - 1) / wpb), block(l->lanes * wpb);
+  l->correlate[dtype](grid, block, static_cast<hipStream_t>(stream), images, windows, nwin,
+                      apply_subpix, maps, *res, plan->pa);
+  return check_launch("piv_correlate_batch");
+}
+
+int piv_peaks_from_maps(piv_plan* plan, float* maps, const double* norms, int32_t nwin,
+                        int32_t apply_subpix, const piv_results* res, void* stream) {
+  if (plan == nullptr || res == nullptr) return fail(PIV_EINVAL, "NULL plan or results");
+  if (nwin <= 0) return PIV_OK;
+  if (maps == nullptr || norms == nullptr || res->dx == nullptr || res->dy == nullptr ||
+      res->correl_max == nullptr || res->err == nullptr)
+    return fail(PIV_EINVAL, "NULL map, norm or result array");
+  if (plan->pa.nb_peaks > 1 && (res->sec == nullptr || res->nsec == nullptr))
+    return fail(PIV_EINVAL, "secondary peak arrays required when nb_peaks_to_search > 1");
+  const Launchers* l = plan->l;
+  dim3 grid((nwin + l->wpb - 1) / l->wpb), block(l->lanes * l->wpb);
+  l->peaks(grid, block, static_cast<hipStream_t>(stream), maps, norms, nwin, apply_subpix, *res,
+           plan->pa);
+  return check_launch("piv_peaks_from_maps");
+}
+
+int piv_subpix_at(piv_plan* plan, const float* maps, const int32_t* ix, const int32_t* iy,
+                  int32_t nwin, float* out_x, float* out_y, int32_t* err, void* stream) {
+  if (plan == nullptr) return fail(PIV_EINVAL, "NULL plan");
+  if (nwin <= 0) return PIV_OK;
+  if (!maps || !ix || !iy || !out_x || !out_y || !err) return fail(PIV_EINVAL, "NULL argument");
+  plan->l->subpix(dim3((nwin + 63) / 64), dim3(64), static_cast<hipStream_t>(stream), maps, ix,
+                  iy, nwin, out_x, out_y, err, plan->pa);
+  return check_launch("piv_subpix_at");
+}
+
+int piv_fix(float* dx, float* dy, const float* correl_max, const int32_t* err, int32_t ny,
+            int32_t nx, float correl_min, float displacement_max, float threshold,
+            int32_t* flags, float* diff2, float* mean_x, float* mean_y, void* stream) {
+  const int n = ny * nx;
+  if (n <= 0) return PIV_OK;
+  if (!dx || !dy || !correl_max || !err || !flags) return fail(PIV_EINVAL, "NULL argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = (n + 255) / 256;
+  const float dmax2 = float(double(displacement_max) * displacement_max);
+  hipLaunchKernelGGL(k_fix_mark, dim3(nb), dim3(256), 0, s, dx, dy, correl_max, err, n,
+                     correl_min, dmax2, displacement_max > 0.f, flags);
+  const float thr2 = float(double(threshold) * threshold);
+  hipLaunchKernelGGL(k_fix_neigh, dim3(nb), dim3(256), 0, s, dx, dy, ny, nx, thr2,
+                     threshold >= 0.f, flags, diff2, mean_x, mean_y);
+  if (threshold >= 0.f)
+    hipLaunchKernelGGL(k_fix_apply, dim3(nb), dim3(256), 0, s, dx, dy, flags, n);
+  return check_launch("piv_fix");
+}
+
+int piv_make_windows(int32_t ny, int32_t nx, const int32_t* iyvecs, const int32_t* ixvecs,
+                     const double* dx_approx, const double* dy_approx, int32_t height,
+                     int32_t width, int32_t npad, const int32_t* crop, int32_t image0,
+                     int32_t image1, piv_window* windows, double* xs, double* ys,
+                     int32_t* dx_in, int32_t* dy_in, void* stream) {
+  const int n = ny * nx;
+  if (n <= 0) return PIV_OK;
+  if (!iyvecs || !ixvecs || !crop || !windows) return fail(PIV_EINVAL, "NULL argument");
+  if ((dx_approx == nullptr) != (dy_approx == nullptr))
+    return fail(PIV_EINVAL, "dx_approx and dy_approx must both be given or both NULL");
+  CropGeom g{crop[0], crop[1], crop[2], crop[3], crop[4], crop[5], crop[6], crop[7]};
+  hipLaunchKernelGGL(k_make_windows, dim3((n + 255) / 256), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), ny, nx, iyvecs, ixvecs, dx_approx,
+                     dy_approx, height, width, npad, g, image0, image1, windows, xs, ys, dx_in,
+                     dy_in);
+  return check_launch("piv_make_windows");
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+/*
+ * piv_hip.h -- C ABI of the MI355X (gfx950) PIV correlation library
+ * (libpiv_hip.so). Plain pointers and sizes only; every buffer argument is a
+ * device pointer unless stated otherwise, every call is asynchronous on the
+ * caller's HIP stream and returns a status code (PIV_OK == 0). The message
+ * of the last failure on the calling thread is piv_last_error().
+ *
+ * Reference interfaces replaced (paths in the fluidimage 0.5.3 tree):
+ *   CorrelFFTW.__init__ / _finalize_init    src/fluidimage/calcul/correl.py:172-200,518-549,643-646
+ *   CorrelFFTW.__call__                     src/fluidimage/calcul/correl.py:648-662
+ *   compute_displacements_from_correl       src/fluidimage/calcul/correl.py:226-275,561-568
+ *   apply_subpix / SubPix.compute_subpix    src/fluidimage/calcul/correl.py:277-281, subpix.py:107-218
+ *   BaseWorkPIV._pad_images + _loop_vectors src/fluidimage/works/piv/singlepass.py:196-208,262-361
+ *   WorkFIX.calcul                          src/fluidimage/works/piv/fix.py:66-133
+ *   mean_neighbors_xy                       src/fluidimage/calcul/mean_neighbors.py:128-155
+ *   _calcul_positions_vectors_subimages     src/fluidimage/works/piv/singlepass.py:210-260,787-870
+ *
+ * Not thread-safe per plan: a plan may be used by one host thread at a time
+ * (the reference's FFTW plans have the same restriction, fft.py:218-234).
+ */
+#ifndef FLUIDIMAGE_AMD_PIV_HIP_H
+#define FLUIDIMAGE_AMD_PIV_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define PIV_OK 0
+#define PIV_EINVAL 1       /* bad argument */
+#define PIV_EUNSUPPORTED 2 /* window shape or option not built into the library */
+#define PIV_EHIP 3         /* HIP runtime error */
+
+/* ---- image element types ---- */
+#define PIV_U8 0
+#define PIV_U16 1
+#define PIV_F32 2
+
+/* ---- sub-pixel methods (SubPix.methods, subpix.py:73) ---- */
+#define PIV_SUBPIX_2D_GAUSSIAN2 0
+#define PIV_SUBPIX_CENTROID 1
+#define PIV_SUBPIX_NONE 2
+
+/* ---- per-window error codes (strings of correl.py:128,130,
+ *      subpix.py:154,215, singlepass.py:310) ---- */
+#define PIV_WIN_OK 0
+#define PIV_WIN_PEAK_BOUNDARY 1 /* "Correlation peak touching boundary." */
+#define PIV_WIN_PEAK_NAN 2      /* "Correlation peak touching nan." */
+#define PIV_WIN_SUBPIX_BOUNDARY 3 /* "close boundary" */
+#define PIV_WIN_SUBPIX_WRONG 4  /* "wrong subpix" */
+#define PIV_WIN_BAD_CROP 5      /* "Bad im_crop shape." */
+#define PIV_WIN_NO_PEAK 6       /* no finite value in the search box */
+
+/* window flag bits */
+#define PIV_WINFLAG_INVALID 1 /* crop outside the padded image: no compute */
+
+/* one image (device memory); rows are `pitch` elements apart */
+typedef struct piv_image {
+  const void* data;
+  int32_t dtype;   /* PIV_U8 / PIV_U16 / PIV_F32 */
+  int32_t height;
+  int32_t width;
+  int32_t reserved;
+  int64_t pitch;
+  const float* minval; /* device scalar subtracted from in-image pixels
+                          (im - im.min(), singlepass.py:206), or NULL */
+} piv_image;
+
+/* one interrogation window: top-left corners of the two crops in image
+ * coordinates (negative or beyond the image reads as the zero padding) */
+typedef struct piv_window {
+  int32_t image0, image1; /* indices into the piv_image array */
+  int32_t y0, x0;
+  int32_t y1, x1;
+  int32_t dx_in, dy_in;   /* integer predictor added to the result */
+  int32_t flags;          /* PIV_WINFLAG_* */
+  int32_t reserved;
+} piv_window;
+
+/* per-window outputs (device arrays of length nwin; any may be NULL except
+ * dx, dy, correl_max, err). Secondary peaks (nb_peaks_to_search > 1) are
+ * stored as [nwin][nb_peaks - 1] triples (dx, dy, correl) in `sec`, their
+ * count in `nsec`, and their sub-pixel refinement (dx, dy, NaN on error)
+ * in `sec_sub`. */
+typedef struct piv_results {
+  float* dx;
+  float* dy;
+  float* correl_max;
+  int32_t* err;
+  int32_t* ix;
+  int32_t* iy;
+  double* norm;
+  float* sec;
+  int32_t* nsec;
+  float* sec_sub;
+} piv_results;
+
+typedef struct piv_plan piv_plan;
+
+/* Version string of the library. */
+const char* piv_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* piv_last_error(void);
+/* 1 if the library has a kernel for windows of n0 x n1 (rows x cols). */
+int piv_shape_supported(int32_t n0, int32_t n1);
+
+/* Create a correlation plan for n0 x n1 windows (CorrelFFTW.__init__).
+ * displacement_max < 0 means None. `mask` (host, n0*n1 bytes, may be NULL
+ * when displacement_max < 0) is where_large_displacement, `box` (host, 4
+ * ints) the search box (i0_start, i0_stop, i1_start, i1_stop). */
+int piv_plan_create(piv_plan** plan, int32_t n0, int32_t n1,
+                    double displacement_max, const uint8_t* mask,
+                    const int32_t* box, int32_t subpix_method,
+                    int32_t nsubpix, int32_t particle_radius,
+                    int32_t nb_peaks_to_search);
+int piv_plan_destroy(piv_plan* plan);
+
+/* out_min[i] = min over image i (device float per image). `images` is a
+ * device array of nimages descriptors. */
+int piv_images_min(const piv_image* images, int32_t nimages, float* out_min,
+                   void* stream);
+
+/* Fused crop -> mean removal -> FFT cross-correlation -> normalisation ->
+ * masked peak search (second chance, secondary peaks) -> optional sub-pixel
+ * fit, for nwin windows. `images` and `windows` are device arrays; every
+ * image of the call has element type `dtype`. `maps`
+ * (device, nwin*n0*n1 float32, may be NULL) receives the correlation maps in
+ * the reference layout (after _like_fftshift). */
+int piv_correlate_batch(piv_plan* plan, int32_t dtype, const piv_image* images,
+                        const piv_window* windows, int32_t nwin,
+                        int32_t apply_subpix, float* maps,
+                        const piv_results* res, void* stream);
+
+/* Peak search (+ optional sub-pixel fit) on existing maps (nwin*n0*n1,
+ * reference layout) with their norms. With displacement_max None the
+ * second-chance NaN square is written into `maps`, as the reference does. */
+int piv_peaks_from_maps(piv_plan* plan, float* maps, const double* norms,
+                        int32_t nwin, int32_t apply_subpix,
+                        const piv_results* res, void* stream);
+
+/* Sub-pixel position at integer indices (ix, iy) of each map:
+ * out_x/out_y = ix + deplx / iy + deply, err = PIV_WIN_* . */
+int piv_subpix_at(piv_plan* plan, const float* maps, const int32_t* ix,
+                  const int32_t* iy, int32_t nwin, float* out_x, float* out_y,
+                  int32_t* err, void* stream);
+
+/* Window records of one pass on a ny x nx vector grid at (iyvecs[i],
+ * ixvecs[j]) (BaseWorkPIV._prepare_with_image grid, singlepass.py:108-141).
+ * First pass: dx_approx == dy_approx == NULL (singlepass.py:210-260). Later
+ * passes: the float64 predictor on the grid, rounded half-to-even
+ * (singlepass.py:752-753) and split between the two crops with the
+ * reference's out-of-image fallbacks (singlepass.py:787-870). `crop` (host,
+ * 8 ints): _start_for_crop0 (y, x), _stop_for_crop0 (y, x),
+ * _start_for_crop1 (y, x), _stop_for_crop1 (y, x). Windows whose crop leaves
+ * the padded image get PIV_WINFLAG_INVALID. xs/ys (float64 vector
+ * positions) and dx_in/dy_in (int32 rounded predictor) may be NULL. */
+int piv_make_windows(int32_t ny, int32_t nx, const int32_t* iyvecs,
+                     const int32_t* ixvecs, const double* dx_approx,
+                     const double* dy_approx, int32_t height, int32_t width,
+                     int32_t npad, const int32_t* crop, int32_t image0,
+                     int32_t image1, piv_window* windows, double* xs,
+                     double* ys, int32_t* dx_in, int32_t* dy_in, void* stream);
+
+/* WorkFIX on a (ny, nx) vector grid (fix.py:66-133): NaN the vectors with
+ * err != 0, correl_max < correl_min, |d|^2 > displacement_max^2 (if > 0),
+ * then the 8-neighbour mean test against threshold (if >= 0). flags receives
+ * bit 0: piv error, 1: correl < correl_min, 2: delta2 < displacement_max2,
+ * 3: diff neighbour too large; diff2 the squared difference to the
+ * neighbour mean; mean_x / mean_y the neighbour means (may be NULL). */
+int piv_fix(float* dx, float* dy, const float* correl_max, const int32_t* err,
+            int32_t ny, int32_t nx, float correl_min, float displacement_max,
+            float threshold, int32_t* flags, float* diff2, float* mean_x,
+            float* mean_y, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDIMAGE_AMD_PIV_HIP_H */

@@ -1,0 +1,52 @@
+"""The C-ABI library loads and exports every entry point of include/piv_hip.h
+(CPU only: no compute call)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from fluidimage_amd import _native
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "piv_hip.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(piv_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "piv_correlate_batch" in names and "piv_plan_create" in names
+    assert set(names) == set(_native.SIGNATURES), "ctypes signatures drift from the header"
+
+
+def test_library_exports_every_symbol():
+    lib = _native.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts():
+    assert ctypes.sizeof(_native.PivImage) == 40
+    assert ctypes.sizeof(_native.PivWindow) == 40
+    assert ctypes.sizeof(_native.PivResults) == 80
+
+
+def test_plan_validation_without_gpu():
+    lib = _native.load()
+    assert lib.piv_shape_supported(32, 32) == 1
+    assert lib.piv_shape_supported(33, 32) == 0
+    plan = ctypes.c_void_p()
+    # unsupported shape: status PIV_EUNSUPPORTED with a message
+    st = lib.piv_plan_create(ctypes.byref(plan), 33, 33, -1.0, None, None, 0, 1, 3, 1)
+    assert st == 2 and b"not built" in lib.piv_last_error()
+    # displacement_max None needs no device memory: the plan can be made
+    st = lib.piv_plan_create(ctypes.byref(plan), 32, 32, -1.0, None, None, 0, 1, 3, 1)
+    assert st == 0
+    assert lib.piv_plan_destroy(plan) == 0
+    st = lib.piv_plan_create(ctypes.byref(plan), 32, 32, -1.0, None, None, 0, 1, 3, 9)
+    assert st == 2

@@ -1,0 +1,140 @@
+"""GPU parity of the correlation path against the CPU oracle.
+
+Bars: maps within 1e-5 of their largest magnitude (float32 FFT vs FP64),
+norms within 1e-6 relative, integer peaks identical except windows whose two
+best candidates are within 2e-5 of the map scale in the FP64 oracle (listed
+and checked to be one of the tied candidates), correl_max within 1e-5
+relative, sub-pixel displacements within 1e-3 px.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(12, 12), (16, 16), (20, 20), (24, 24), (32, 32), (40, 40), (48, 48), (64, 64),
+          (64, 32), (32, 64)]
+
+
+def _correl(shape, **kw):
+    from fluidimage_amd.calcul.correl import CorrelHIP
+
+    return CorrelHIP(shape, shape, **kw)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_call_map_and_norm_match_oracle(shape):
+    from oracle import correl as oc
+
+    print("case: __call__ map", shape, flush=True)
+    rng = np.random.default_rng(11)
+    c = _correl(shape)
+    for trial in range(3):
+        im0 = (rng.random(shape) * 255).astype(np.float32)
+        im1 = (rng.random(shape) * 255).astype(np.float32)
+        m, norm = c(im0, im1)
+        m64, n64 = oc.correl_fft(im0, im1, "f64")
+        assert m.dtype == np.float32 and m.shape == shape
+        assert np.abs(m - m64).max() <= 1e-5 * np.abs(m64).max()
+        assert norm == pytest.approx(float(n64), rel=1e-6)
+
+
+@pytest.mark.parametrize("shape", [(32, 32), (64, 32), (48, 48), (24, 24)])
+def test_known_answers(shape):
+    # src/fluidimage/calcul/test_correl.py:75-109: zero shift within 1e-3,
+    # autocorrelation max 1, shift (3.3, 5.8) within 0.8 px
+    from tests.helpers import shifted_pair
+
+    print("case: known answers", shape, flush=True)
+    rng = np.random.default_rng(12)
+    im0, im1 = shifted_pair(rng, shape, (3.3, 5.8))
+    for dmax in ("50%", None):
+        c = _correl(shape, displacement_max=dmax, method_subpix="2d_gaussian2")
+        m, norm = c(im0, im0)
+        dx, dy, cmax, _ = c.compute_displacements_from_correl(m, norm=norm)
+        assert np.allclose([dx, dy], [0, 0], atol=1e-3)
+        assert np.allclose(cmax, 1.0, rtol=1e-5)
+        m, norm = c(im0, im1)
+        dx, dy, cmax, _ = c.compute_displacements_from_correl(m, norm=norm)
+        assert np.allclose([dx, dy], [3.3, 5.8], atol=0.8)
+        sx, sy = c.apply_subpix(dx, dy, m)
+        assert np.allclose([sx, sy], [3.3, 5.8], atol=0.2)
+
+
+def _planted_maps(rng, shape, n):
+    maps = []
+    n0, n1 = shape
+    for k in range(n):
+        m = rng.normal(size=shape).astype(np.float32)
+        kind = k % 5
+        if kind == 0:  # peak on the border
+            m[0, rng.integers(n1)] = 50.0
+        elif kind == 1:  # exact ties
+            m[n0 // 2, n1 // 2] = m[n0 // 2 + 2, n1 // 2 - 1] = 30.0
+        elif kind == 2:  # peak next to the border column
+            m[rng.integers(n0), n1 - 2] = 40.0
+        elif kind == 3:  # NaN values in the map
+            m[n0 // 2, :] = np.nan
+        maps.append(m)
+    return maps
+
+
+@pytest.mark.parametrize("dmax", [None, 5, "30%"])
+@pytest.mark.parametrize("nb_peaks", [1, 3])
+def test_peaks_from_maps_match_oracle(dmax, nb_peaks):
+    from oracle import correl as oc
+
+    print("case: peaks_from_maps", dmax, nb_peaks, flush=True)
+    shape = (24, 24)
+    rng = np.random.default_rng(13)
+    c = _correl(shape, displacement_max=dmax, nb_peaks_to_search=nb_peaks, particle_radius=2)
+    o = oc.OracleCorrelFFT(shape, displacement_max=dmax, nb_peaks_to_search=nb_peaks,
+                           particle_radius=2)
+    for m in _planted_maps(rng, shape, 40):
+        norm = float(np.nansum(m * m)) + 1.0
+        mg = m.copy()
+        mo = m.copy()
+        try:
+            got = c.compute_displacements_from_correl(mg, norm)
+            got_err = None
+        except Exception as e:  # PIVError
+            got, got_err = e.results, e.explanation
+        try:
+            exp = o.compute_displacements_from_correl(mo, norm)
+            exp_err = None
+        except oc.PIVError as e:
+            exp, exp_err = e.results, e.explanation
+        assert got_err == exp_err
+        assert got[0] == exp[0] and got[1] == exp[1]
+        assert np.float32(got[2]) == np.float32(exp[2])
+        if got_err is None and nb_peaks > 1:
+            assert [(a, b) for a, b, _ in got[3]] == [(a, b) for a, b, _ in exp[3]]
+            assert np.allclose([v for *_, v in got[3]], [v for *_, v in exp[3]], rtol=1e-6)
+        # the in-place second-chance square of displacement_max None
+        assert np.array_equal(np.isnan(mg), np.isnan(mo))
+
+
+def test_apply_subpix_matches_oracle():
+    from oracle import correl as oc
+
+    shape = (32, 32)
+    rng = np.random.default_rng(14)
+    c = _correl(shape, method_subpix="2d_gaussian2")
+    o = oc.OracleCorrelFFT(shape, method_subpix="2d_gaussian2")
+    for _ in range(30):
+        m = rng.random(shape).astype(np.float32)
+        iy, ix = rng.integers(1, 31, 2)
+        m[iy, ix] = 3.0
+        dx, dy = c.compute_displacement_from_indices(ix, iy)
+        try:
+            got = c.apply_subpix(dx, dy, m)
+        except Exception as e:
+            got = e.explanation
+        try:
+            exp = o.apply_subpix(dx, dy, m)
+        except Exception as e:
+            exp = e.explanation
+        if isinstance(exp, str):
+            assert got == exp
+        else:
+            assert np.allclose(got, exp, atol=1e-4)
