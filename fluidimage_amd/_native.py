@@ -16,6 +16,8 @@ PIV_OK = 0
 PIV_U8, PIV_U16, PIV_F32 = 0, 1, 2
 PIV_SUBPIX = {"2d_gaussian2": 0, "centroid": 1, "no_subpix": 2}
 PIV_WINFLAG_INVALID = 1
+PIV_OPT_SUBPIX = 1
+PIV_OPT_MAP_SQUARE = 2
 
 WIN_OK = 0
 WIN_PEAK_BOUNDARY = 1
@@ -103,6 +105,16 @@ SIGNATURES = {
     "piv_make_windows": (
         ctypes.c_int,
         [_I, _I, _P, _P, _P, _P, _I, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P],
+    ),
+    "piv_tps_smooth": (
+        ctypes.c_int,
+        [_I, _I, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P,
+         ctypes.c_int64, _P, ctypes.c_int64, _P, _P, _P],
+    ),
+    "piv_tps_blend": (ctypes.c_int, [_I, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P]),
+    "piv_tps_interpolate": (
+        ctypes.c_int,
+        [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P],
     ),
     "piv_fix": (
         ctypes.c_int,

@@ -13,9 +13,7 @@ import sys
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SOURCES = [HERE / "csrc" / "piv_hip.hip"]
-HEADERS = [HERE / "csrc" / "fft_reg.hpp", HERE / "csrc" / "piv_device.hpp",
-           HERE.parent / "include" / "piv_hip.h"]
+SOURCES = [HERE / "csrc" / "piv_hip.hip", HERE / "csrc" / "piv_tps.hip"]
 OUTPUT = HERE / "libpiv_hip.so"
 ARCH = os.environ.get("PIV_HIP_ARCH", "gfx950")
 

@@ -222,21 +222,26 @@ class CorrelHIP:
 
     # ------------------------------------------------------------ batched API
     def correlate_windows(self, images, windows, nwin, dtype, apply_subpix, maps=None,
-                          out=None, stream=None):
+                          out=None, stream=None, map_square=True):
         """Run the fused kernel on ``nwin`` windows.
 
         ``images``: device tensor of ``PivImage`` records (uint8 bytes),
         ``windows``: device tensor of ``PivWindow`` records, ``dtype``: one of
-        ``_native.PIV_U8/U16/F32``. Returns the dict of result tensors.
+        ``_native.PIV_U8/U16/F32``. With ``map_square`` (and
+        displacement_max None) the maps keep the second-chance NaN square
+        like the maps of the reference's ``_loop_vectors``. Returns the dict
+        of result tensors.
         """
         torch = _torch()
         if out is None:
             out = self._alloc_results(nwin, images.device)
         res = self.results_struct(out)
+        options = (_native.PIV_OPT_SUBPIX if apply_subpix else 0) | (
+            _native.PIV_OPT_MAP_SQUARE if map_square else 0)
         _native.check(
             self._lib.piv_correlate_batch(
                 self._plan, int(dtype), ctypes.c_void_p(images.data_ptr()),
-                ctypes.c_void_p(windows.data_ptr()), int(nwin), int(bool(apply_subpix)),
+                ctypes.c_void_p(windows.data_ptr()), int(nwin), int(options),
                 None if maps is None else ctypes.c_void_p(maps.data_ptr()),
                 ctypes.byref(res), _native.stream_ptr(stream),
             ),
@@ -260,7 +265,8 @@ class CorrelHIP:
                         subtract_min=False)
         wins = window_table([(0, 1, 0, 0, 0, 0, 0, 0, 0)], dev)
         maps = torch.empty((1,) + self.im0_shape, dtype=torch.float32, device=dev)
-        out = self.correlate_windows(imgs.table, wins, 1, _native.PIV_F32, False, maps=maps)
+        out = self.correlate_windows(imgs.table, wins, 1, _native.PIV_F32, False, maps=maps,
+                                     map_square=False)
         torch.cuda.current_stream().synchronize()
         return maps[0].cpu().numpy(), float(out["norm"][0].item())
 

@@ -21,16 +21,15 @@
 
 #include "../../include/piv_hip.h"
 #include "fft_reg.hpp"
+#include "last_error.hpp"
 #include "piv_device.hpp"
 
 using namespace fia;
 
 namespace {
 
-thread_local std::string g_last_error;
-
 int fail(int code, const std::string& msg) {
-  g_last_error = msg;
+  fia::last_error() = msg;
   return code;
 }
 
@@ -63,8 +62,9 @@ __device__ __forceinline__ int rot(int c, int r) {
 template <int N0, int N1, int DT, int WPB>
 __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correlate(
     const piv_image* __restrict__ images, const piv_window* __restrict__ windows, int nwin,
-    int apply_subpix, float* __restrict__ maps, piv_results res, PlanArgs pa) {
+    int options, float* __restrict__ maps, piv_results res, PlanArgs pa) {
   constexpr int T = Shape<N0, N1>::T;
+  const int apply_subpix = options & PIV_OPT_SUBPIX;
   constexpr int NN = N0 * N1;
   __shared__ float2 smem[WPB * NN];
   __shared__ uint8_t smask[NN];
@@ -219,7 +219,7 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   }
   if (maps != nullptr && live) {
     float* out = maps + size_t(win) * NN;
-    const bool side = !pa.has_dmax;
+    const bool side = !pa.has_dmax && (options & PIV_OPT_MAP_SQUARE);
     for (int k = lane; k < NN; k += T) {
       float v = compute ? mapf[k] : __builtin_nanf("");
       if (side) {
@@ -585,7 +585,7 @@ extern "C" {
 
 const char* piv_version(void) { return "fluidimage_amd piv_hip 0.1 (gfx950)"; }
 
-const char* piv_last_error(void) { return g_last_error.c_str(); }
+const char* piv_last_error(void) { return fia::last_error().c_str(); }
 
 int piv_shape_supported(int32_t n0, int32_t n1) { return find_shape(n0, n1) != nullptr; }
 
@@ -669,7 +669,7 @@ int piv_images_min(const piv_image* images, int32_t nimages, float* out_min, voi
 }
 
 int piv_correlate_batch(piv_plan* plan, int32_t dtype, const piv_image* images,
-                           const piv_window* windows, int32_t nwin, int32_t apply_subpix,
+                           const piv_window* windows, int32_t nwin, int32_t options,
                            float* maps, const piv_results* res, void* stream) {
   if (plan == nullptr || res == nullptr) return fail(PIV_EINVAL, "NULL plan or results");
   if (nwin <= 0) return PIV_OK;
@@ -683,7 +683,7 @@ int piv_correlate_batch(piv_plan* plan, int32_t dtype, const piv_image* images,
   const int wpb = l->wpb;
   dim3 grid((nwin + wpb - 1) / wpb), block(l->lanes * wpb);
   l->correlate[dtype](grid, block, static_cast<hipStream_t>(stream), images, windows, nwin,
-                      apply_subpix, maps, *res, plan->pa);
+                      options, maps, *res, plan->pa);
   return check_launch("piv_correlate_batch");
 }
 

@@ -1,0 +1,691 @@
+// Thin-plate-spline subdomain smoothing in FP64 (gfx950).
+//
+// Per subdomain with n centers c_j and values v (reference:
+// src/fluidimage/calcul/interpolate/thin_plate_spline.py:168-216 and
+// thin_plate_spline_subdom.py:128-159,224-280):
+//   [[K + sI, P], [P^T, 0]] [w; a] = [v; 0],  K_ij = r^2 ln(r^2) / 2,
+//   P = [1, y, x].
+// Solved here by the null-space method:
+//   P = Q [R; 0] (3 Householder reflectors, compact WY form Q = I - V T V^T),
+//   B' = Q^T (K + sI) Q = B - V Z^T - Z V^T  (one pass over B: Y = B V),
+//   C = B'[3:, 3:] is symmetric positive definite (K is conditionally
+//   positive definite of order 2) -> blocked Cholesky C = L L^T,
+//   z = C^-1 (Q^T v)[3:],  a = R^-1 ((Q^T v)[:3] - B'[:3, 3:] z),
+//   w = Q [0; z],  smoothed = v - s w  (= [K P] [w; a]).
+// One factorisation serves both components and every outlier iteration.
+// One workgroup per subdomain; matrices live in a caller workspace.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/piv_hip.h"
+#include "last_error.hpp"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kNB = 32;       // triangular-solve block size
+constexpr int kNBc = 16;      // Cholesky block size (panel of up to kMaxN rows in LDS)
+constexpr int kMaxN = 1024;   // centers per subdomain
+constexpr int kColsPerThread = kMaxN / kThreads;
+
+__device__ __forceinline__ double tps_phi(double r2) {
+  return r2 != 0.0 ? r2 * log(r2) / 2.0 : 0.0;
+}
+
+__device__ double block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < kThreads / 64; ++k) s += red[k];
+  __syncthreads();
+  return s;
+}
+
+__device__ double block_max(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = red[0];
+  for (int k = 1; k < kThreads / 64; ++k) s = fmax(s, red[k]);
+  __syncthreads();
+  return s;
+}
+
+// refl layout per subdomain: tau[0..2], R upper (R00 R01 R02 R11 R12 R22)
+// [3..8], T upper (T00 T01 T02 T11 T12 T22) [9..14]
+// vwork layout per slot: V[0..2], Z[0..2]
+
+// ---------------------------------------------------------------------------
+// 1. Householder QR of P, matrix B = K + sI, Y = B V, Z
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_tps_prep(
+    const double* __restrict__ xs, const double* __restrict__ ys,
+    const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
+    const int64_t* __restrict__ mat_off, double smoothing, double* __restrict__ mats,
+    double* __restrict__ refl, double* __restrict__ vwork, int32_t* __restrict__ status) {
+  const int d = blockIdx.x;
+  const int64_t s0 = dom_off[d];
+  const int n = int(dom_off[d + 1] - s0);
+  double* B = mats + mat_off[d];
+  double* RF = refl + 16 * d;
+  double* VW = vwork + 6 * s0;
+  __shared__ double cy[kMaxN], cx[kMaxN];
+  __shared__ double Pm[3][kMaxN];  // P, then its reflected copy
+  __shared__ double Vm[3][kMaxN];
+  __shared__ double red[kThreads / 64];
+  __shared__ double sh[32];
+  const int tid = threadIdx.x;
+  for (int j = tid; j < n; j += kThreads) {
+    const int p = slot_pt[s0 + j];
+    cy[j] = ys[p];
+    cx[j] = xs[p];
+    Pm[0][j] = 1.0;
+    Pm[1][j] = ys[p];
+    Pm[2][j] = xs[p];
+    Vm[0][j] = Vm[1][j] = Vm[2][j] = 0.0;
+  }
+  __syncthreads();
+  // ---- Householder reflectors (LAPACK dgeqr2 convention) ----
+  double tau[3];
+  for (int k = 0; k < 3; ++k) {
+    double part = 0.0;
+    for (int j = k + 1 + tid; j < n; j += kThreads) part += Pm[k][j] * Pm[k][j];
+    const double sig = block_sum(part, red);
+    const double alpha = Pm[k][k];
+    double beta, t, scale;
+    if (sig == 0.0) {
+      beta = alpha;
+      t = 0.0;
+      scale = 0.0;
+    } else {
+      beta = -copysign(sqrt(alpha * alpha + sig), alpha);
+      t = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    tau[k] = t;
+    for (int j = tid; j < n; j += kThreads) {
+      double v = 0.0;
+      if (j == k) v = 1.0;
+      else if (j > k) v = Pm[k][j] * scale;
+      Vm[k][j] = v;
+    }
+    __syncthreads();
+    if (tid == 0) Pm[k][k] = beta;
+    // apply H_k to the remaining columns of P
+    for (int c = k + 1; c < 3; ++c) {
+      double pp = 0.0;
+      for (int j = k + tid; j < n; j += kThreads) pp += Vm[k][j] * Pm[c][j];
+      const double dot = block_sum(pp, red);
+      for (int j = k + tid; j < n; j += kThreads) Pm[c][j] -= t * Vm[k][j] * dot;
+      __syncthreads();
+    }
+  }
+  // T of the compact WY form (dlarft, forward, columnwise)
+  if (tid == 0) {
+    double vtv01 = 0, vtv02 = 0, vtv12 = 0;
+    for (int j = 0; j < n; ++j) {
+      vtv01 += Vm[0][j] * Vm[1][j];
+      vtv02 += Vm[0][j] * Vm[2][j];
+      vtv12 += Vm[1][j] * Vm[2][j];
+    }
+    const double T00 = tau[0], T11 = tau[1], T22 = tau[2];
+    const double T01 = -tau[1] * T00 * vtv01;
+    // T[0:2, 2] = -tau2 * T[0:2, 0:2] * V[:, 0:2]^T v2
+    const double g0 = vtv02, g1 = vtv12;
+    const double T02 = -tau[2] * (T00 * g0 + T01 * g1);
+    const double T12 = -tau[2] * (T11 * g1);
+    RF[0] = tau[0];
+    RF[1] = tau[1];
+    RF[2] = tau[2];
+    RF[3] = Pm[0][0];
+    RF[4] = Pm[1][0];
+    RF[5] = Pm[2][0];
+    RF[6] = Pm[1][1];
+    RF[7] = Pm[2][1];
+    RF[8] = Pm[2][2];
+    RF[9] = T00;
+    RF[10] = T01;
+    RF[11] = T02;
+    RF[12] = T11;
+    RF[13] = T12;
+    RF[14] = T22;
+    sh[0] = T00; sh[1] = T01; sh[2] = T02; sh[3] = T11; sh[4] = T12; sh[5] = T22;
+    const bool singular = n < 3 || Pm[0][0] == 0.0 || Pm[1][1] == 0.0 || Pm[2][2] == 0.0;
+    if (singular) status[d] = 1;
+  }
+  __syncthreads();
+  // ---- B (full storage) and Y = B V (column j of thread: Y_j by symmetry) ----
+  double yacc[kColsPerThread][3];
+#pragma unroll
+  for (int c = 0; c < kColsPerThread; ++c) yacc[c][0] = yacc[c][1] = yacc[c][2] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double yi = cy[i], xi = cx[i];
+    const double v0 = Vm[0][i], v1 = Vm[1][i], v2 = Vm[2][i];
+#pragma unroll
+    for (int c = 0; c < kColsPerThread; ++c) {
+      const int j = tid + c * kThreads;
+      if (j < n) {
+        const double dy = cy[j] - yi, dx = cx[j] - xi;
+        double b = tps_phi(dy * dy + dx * dx);
+        if (i == j) b += smoothing;
+        B[int64_t(i) * n + j] = b;
+        yacc[c][0] += b * v0;
+        yacc[c][1] += b * v1;
+        yacc[c][2] += b * v2;
+      }
+    }
+  }
+  // M = V^T Y (3 x 3)
+  double m[9];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double part = 0.0;
+#pragma unroll
+      for (int c = 0; c < kColsPerThread; ++c) {
+        const int j = tid + c * kThreads;
+        if (j < n) part += Vm[a][j] * yacc[c][b];
+      }
+      m[3 * a + b] = block_sum(part, red);
+    }
+  // Z = Y T - 1/2 V T^T M T
+  const double T[3][3] = {{sh[0], sh[1], sh[2]}, {0.0, sh[3], sh[4]}, {0.0, 0.0, sh[5]}};
+  double MT[3][3], TtMT[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += m[3 * a + k] * T[k][b];
+      MT[a][b] = s;
+    }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += T[k][a] * MT[k][b];
+      TtMT[a][b] = s;
+    }
+#pragma unroll
+  for (int c = 0; c < kColsPerThread; ++c) {
+    const int j = tid + c * kThreads;
+    if (j < n) {
+      const double v[3] = {Vm[0][j], Vm[1][j], Vm[2][j]};
+      for (int b = 0; b < 3; ++b) {
+        double z = 0.0;
+        for (int k = 0; k < 3; ++k) z += yacc[c][k] * T[k][b];
+        double corr = 0.0;
+        for (int k = 0; k < 3; ++k) corr += v[k] * TtMT[k][b];
+        VW[6 * j + b] = v[b];
+        VW[6 * j + 3 + b] = z - 0.5 * corr;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. B' = B - V Z^T - Z V^T on the lower triangle, then C = L L^T
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_tps_chol(const int64_t* __restrict__ dom_off,
+                                                       const int64_t* __restrict__ mat_off,
+                                                       double* __restrict__ mats,
+                                                       const double* __restrict__ vwork,
+                                                       int32_t* __restrict__ status) {
+  const int d = blockIdx.x;
+  const int64_t s0 = dom_off[d];
+  const int n = int(dom_off[d + 1] - s0);
+  double* B = mats + mat_off[d];
+  const double* VW = vwork + 6 * s0;
+  const int tid = threadIdx.x;
+  __shared__ double diag[kNBc][kNBc + 1];
+  __shared__ double panel[kMaxN][kNBc + 1];
+  __shared__ int bad;
+  if (tid == 0) bad = 0;
+  // congruence on rows >= 3, lower triangle (columns 0..2 keep B'[3:, :3])
+  for (int i = 3; i < n; ++i) {
+    const double vi0 = VW[6 * i], vi1 = VW[6 * i + 1], vi2 = VW[6 * i + 2];
+    const double zi0 = VW[6 * i + 3], zi1 = VW[6 * i + 4], zi2 = VW[6 * i + 5];
+    for (int j = tid; j <= i; j += kThreads) {
+      const double* vj = VW + 6 * j;
+      const double c = vi0 * vj[3] + vi1 * vj[4] + vi2 * vj[5] + zi0 * vj[0] + zi1 * vj[1] +
+                       zi2 * vj[2];
+      B[int64_t(i) * n + j] -= c;
+    }
+  }
+  __syncthreads();
+  const int m = n - 3;
+  double* A = B + 3 * int64_t(n) + 3;  // C(i, j) = A[i * n + j]
+  for (int kb = 0; kb < m; kb += kNBc) {
+    const int bs = min(kNBc, m - kb);
+    // -- diagonal block
+    for (int e = tid; e < bs * bs; e += kThreads) {
+      const int i = e / bs, j = e % bs;
+      diag[i][j] = j <= i ? A[int64_t(kb + i) * n + kb + j] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < bs; ++j) {
+      if (tid == 0) {
+        const double djj = diag[j][j];
+        if (!(djj > 0.0)) bad = 1;
+        diag[j][j] = djj > 0.0 ? sqrt(djj) : 1.0;
+      }
+      __syncthreads();
+      const double ljj = diag[j][j];
+      for (int i = j + 1 + tid; i < bs; i += kThreads) diag[i][j] /= ljj;
+      __syncthreads();
+      for (int e = tid; e < bs * bs; e += kThreads) {
+        const int i = e / bs, k = e % bs;
+        if (k > j && k <= i) diag[i][k] -= diag[i][j] * diag[k][j];
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < bs * bs; e += kThreads) {
+      const int i = e / bs, j = e % bs;
+      if (j <= i) A[int64_t(kb + i) * n + kb + j] = diag[i][j];
+    }
+    // -- panel: rows below, L_ij = A_ij L_jj^-T (row-wise triangular solve)
+    const int r0 = kb + bs;
+    for (int i = r0 + tid; i < m; i += kThreads) {
+      double row[kNBc];
+      double* Ai = A + int64_t(i) * n + kb;
+#pragma unroll
+      for (int j = 0; j < kNBc; ++j) row[j] = j < bs ? Ai[j] : 0.0;
+#pragma unroll
+      for (int j = 0; j < kNBc; ++j) {
+        if (j < bs) {
+          double s = row[j];
+          for (int k = 0; k < j; ++k) s -= row[k] * diag[j][k];
+          row[j] = s / diag[j][j];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kNBc; ++j) {
+        if (j < bs) {
+          Ai[j] = row[j];
+          panel[i - r0][j] = row[j];
+        }
+      }
+    }
+    __syncthreads();
+    // -- trailing update of the lower triangle, 4 x 4 register tiles
+    const int mt = m - r0;
+    const int nt = (mt + 3) / 4;
+    const int ntiles = nt * (nt + 1) / 2;
+    for (int t = tid; t < ntiles; t += kThreads) {
+      // tile (ti, tj) with tj <= ti
+      int ti = int((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
+      while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+      while (ti * (ti + 1) / 2 > t) --ti;
+      const int tj = t - ti * (ti + 1) / 2;
+      const int i0 = 4 * ti, j0 = 4 * tj;
+      double acc[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+      for (int k = 0; k < bs; ++k) {
+        double pa[4], pb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          pa[a] = i0 + a < mt ? panel[i0 + a][k] : 0.0;
+          pb[a] = j0 + a < mt ? panel[j0 + a][k] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] += pa[a] * pb[b];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int i = i0 + a;
+        if (i >= mt) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int j = j0 + b;
+          if (j < mt && j <= i) A[int64_t(r0 + i) * n + r0 + j] -= acc[a][b];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && bad) status[d] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// 3. solves with the outlier iterations, both components
+// ---------------------------------------------------------------------------
+// x <- Q^T x (apply H1, H2, H3) or Q x (apply H3, H2, H1), two vectors
+__device__ void apply_q(double (*x)[kMaxN], int n, const double* VW, const double* tau,
+                        bool transpose, double* red) {
+  for (int kk = 0; kk < 3; ++kk) {
+    const int k = transpose ? kk : 2 - kk;
+    double p0 = 0.0, p1 = 0.0;
+    for (int j = threadIdx.x; j < n; j += kThreads) {
+      p0 += VW[6 * j + k] * x[0][j];
+      p1 += VW[6 * j + k] * x[1][j];
+    }
+    const double d0 = block_sum(p0, red), d1 = block_sum(p1, red);
+    for (int j = threadIdx.x; j < n; j += kThreads) {
+      x[0][j] -= tau[k] * VW[6 * j + k] * d0;
+      x[1][j] -= tau[k] * VW[6 * j + k] * d1;
+    }
+    __syncthreads();
+  }
+}
+
+// solve C z = b in place (x[c][3 + i], c = 0, 1) with C = L L^T
+__device__ void chol_solve(double (*x)[kMaxN], const double* A, int n, int m) {
+  __shared__ double dg[kNB][kNB + 1];
+  const int tid = threadIdx.x;
+  // forward: L y = b
+  for (int kb = 0; kb < m; kb += kNB) {
+    const int bs = min(kNB, m - kb);
+    for (int e = tid; e < bs * bs; e += kThreads) {
+      const int i = e / bs, j = e % bs;
+      dg[i][j] = j <= i ? A[int64_t(kb + i) * n + kb + j] : 0.0;
+    }
+    __syncthreads();
+    if (tid < 2) {
+      double* b = x[tid] + 3 + kb;
+      for (int i = 0; i < bs; ++i) {
+        double s = b[i];
+        for (int j = 0; j < i; ++j) s -= dg[i][j] * b[j];
+        b[i] = s / dg[i][i];
+      }
+    }
+    __syncthreads();
+    for (int i = kb + bs + tid; i < m; i += kThreads) {
+      const double* Ai = A + int64_t(i) * n + kb;
+      double s0 = 0.0, s1 = 0.0;
+      for (int j = 0; j < bs; ++j) {
+        const double l = Ai[j];
+        s0 += l * x[0][3 + kb + j];
+        s1 += l * x[1][3 + kb + j];
+      }
+      x[0][3 + i] -= s0;
+      x[1][3 + i] -= s1;
+    }
+    __syncthreads();
+  }
+  // backward: L^T z = y
+  const int nblk = (m + kNB - 1) / kNB;
+  for (int bb = nblk - 1; bb >= 0; --bb) {
+    const int kb = bb * kNB;
+    const int bs = min(kNB, m - kb);
+    for (int e = tid; e < bs * bs; e += kThreads) {
+      const int i = e / bs, j = e % bs;
+      dg[i][j] = j <= i ? A[int64_t(kb + i) * n + kb + j] : 0.0;
+    }
+    __syncthreads();
+    if (tid < 2) {
+      double* b = x[tid] + 3 + kb;
+      for (int i = bs - 1; i >= 0; --i) {
+        double s = b[i];
+        for (int j = i + 1; j < bs; ++j) s -= dg[j][i] * b[j];
+        b[i] = s / dg[i][i];
+      }
+    }
+    __syncthreads();
+    for (int j = tid; j < kb; j += kThreads) {
+      double s0 = 0.0, s1 = 0.0;
+      for (int i = 0; i < bs; ++i) {
+        const double l = A[int64_t(kb + i) * n + j];
+        s0 += l * x[0][3 + kb + i];
+        s1 += l * x[1][3 + kb + i];
+      }
+      x[0][3 + j] -= s0;
+      x[1][3 + j] -= s1;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_tps_solve(
+    const float* __restrict__ vx, const float* __restrict__ vy,
+    const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
+    const int64_t* __restrict__ mat_off, const double* __restrict__ mats,
+    const double* __restrict__ refl, const double* __restrict__ vwork, double smoothing,
+    double threshold, double* __restrict__ smooth_slots, int64_t nslots,
+    double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
+    const int32_t* __restrict__ status) {
+  const int d = blockIdx.x;
+  if (status[d]) return;
+  const int64_t s0 = dom_off[d];
+  const int n = int(dom_off[d + 1] - s0);
+  const int m = n - 3;
+  const double* B = mats + mat_off[d];
+  const double* A = B + 3 * int64_t(n) + 3;
+  const double* RF = refl + 16 * d;
+  const double* VW = vwork + 6 * s0;
+  const int tid = threadIdx.x;
+  __shared__ float val[2][kMaxN];   // current values (float32, as the reference)
+  __shared__ double x[2][kMaxN];    // work vector
+  __shared__ double red[kThreads / 64];
+  __shared__ int go[2];
+  __shared__ int nerr[2];
+  const double tau[3] = {RF[0], RF[1], RF[2]};
+  for (int j = tid; j < n; j += kThreads) {
+    const int p = slot_pt[s0 + j];
+    val[0][j] = vx[p];
+    val[1][j] = vy[p];
+  }
+  if (tid < 2) {
+    go[tid] = 1;
+  }
+  __syncthreads();
+  double nfixed[2] = {0.0, 0.0}, maxdiff[2] = {0.0, 0.0};
+  int count[2] = {0, 0};
+  for (int it = 0; it < 11; ++it) {
+    if (!go[0] && !go[1]) break;
+    // rhs in the rotated basis
+    for (int j = tid; j < n; j += kThreads) {
+      x[0][j] = val[0][j];
+      x[1][j] = val[1][j];
+    }
+    __syncthreads();
+    apply_q(x, n, VW, tau, true, red);
+    // a-part rhs kept in registers: vt[0..2]
+    double vt[2][3];
+    for (int c = 0; c < 2; ++c)
+      for (int k = 0; k < 3; ++k) vt[c][k] = x[c][k];
+    __syncthreads();
+    if (m > 0) chol_solve(x, A, n, m);
+    // a = R^-1 (vt - B'[:3, 3:] z), B'[:3, 3 + j] = B'[3 + j][k] (lower)
+    double bz[2][3];
+    for (int k = 0; k < 3; ++k) {
+      double p0 = 0.0, p1 = 0.0;
+      for (int j = tid; j < m; j += kThreads) {
+        const double b = B[int64_t(3 + j) * n + k];
+        p0 += b * x[0][3 + j];
+        p1 += b * x[1][3 + j];
+      }
+      bz[0][k] = block_sum(p0, red);
+      bz[1][k] = block_sum(p1, red);
+    }
+    double a[2][3];
+    for (int c = 0; c < 2; ++c) {
+      const double r0 = vt[c][0] - bz[c][0], r1 = vt[c][1] - bz[c][1], r2 = vt[c][2] - bz[c][2];
+      a[c][2] = r2 / RF[8];
+      a[c][1] = (r1 - RF[7] * a[c][2]) / RF[6];
+      a[c][0] = (r0 - RF[4] * a[c][1] - RF[5] * a[c][2]) / RF[3];
+    }
+    // w = Q [0; z]
+    if (tid < 2) {
+      x[tid][0] = x[tid][1] = x[tid][2] = 0.0;
+    }
+    __syncthreads();
+    apply_q(x, n, VW, tau, false, red);
+    // smoothed = v - s w, differences, erratic vectors
+    for (int c = 0; c < 2; ++c) {
+      if (!go[c]) continue;  // uniform across the block
+      count[c] += 1;
+      double dmax = 0.0;
+      int ne = 0;
+      for (int j = tid; j < n; j += kThreads) {
+        const double v = double(val[c][j]);
+        const double sm = v - smoothing * x[c][j];
+        smooth_slots[c * nslots + s0 + j] = sm;
+        const double diff = fabs(sm - v);
+        dmax = fmax(dmax, diff);
+        if (threshold >= 0.0 && diff > threshold) ++ne;
+      }
+      double* W = weights + c * wstride + s0 + 3 * int64_t(d);
+      for (int j = tid; j < n; j += kThreads) W[j] = x[c][j];
+      if (tid == 0) {
+        W[n] = a[c][0];
+        W[n + 1] = a[c][1];
+        W[n + 2] = a[c][2];
+      }
+      const double dm = block_max(dmax, red);
+      const double nesum = block_sum(double(ne), red);
+      if (count[c] == 1) maxdiff[c] = dm;
+      const bool more = threshold >= 0.0 && nesum > 0.0 && count[c] < 11;
+      if (more) {
+        nfixed[c] += nesum;
+        // values[erratic] = smoothed[erratic] (stored as float32)
+        for (int j = tid; j < n; j += kThreads) {
+          const double v = double(val[c][j]);
+          const double sm = v - smoothing * x[c][j];
+          if (fabs(sm - v) > threshold) val[c][j] = float(sm);
+        }
+      }
+      __syncthreads();
+      if (tid == 0) go[c] = more ? 1 : 0;
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    for (int c = 0; c < 2; ++c) {
+      stats[6 * d + 3 * c + 0] = nfixed[c];
+      stats[6 * d + 3 * c + 1] = maxdiff[c];
+      stats[6 * d + 3 * c + 2] = count[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 4. blend of the smoothed values, 5. evaluation at new positions
+// ---------------------------------------------------------------------------
+__global__ void k_tps_blend(int npts, const int32_t* __restrict__ c_slots,
+                            const int64_t* __restrict__ c_ptr, const double* __restrict__ coef,
+                            const double* __restrict__ norm, const double* __restrict__ sm,
+                            int64_t nslots, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  double a0 = 0.0, a1 = 0.0;
+  for (int64_t k = c_ptr[i]; k < c_ptr[i + 1]; ++k) {
+    const int s = c_slots[k];
+    a0 += coef[s] * sm[s];
+    a1 += coef[s] * sm[nslots + s];
+  }
+  out[i] = a0 / norm[i];
+  out[npts + i] = a1 / norm[i];
+}
+
+__global__ __launch_bounds__(256) void k_tps_interp(
+    int npts, const double* __restrict__ px, const double* __restrict__ py,
+    const int32_t* __restrict__ n_slots, const int64_t* __restrict__ n_ptr,
+    const int32_t* __restrict__ n_dom, const double* __restrict__ n_coef,
+    const double* __restrict__ xs, const double* __restrict__ ys,
+    const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
+    const double* __restrict__ weights, int64_t wstride, double* __restrict__ out) {
+  // one wavefront per new position
+  const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (p >= npts) return;
+  const double x = px[p], y = py[p];
+  double acc0 = 0.0, acc1 = 0.0, nrm = 0.0;
+  for (int64_t k = n_ptr[p]; k < n_ptr[p + 1]; ++k) {
+    const int s = n_slots[k];
+    const int d = n_dom[s];
+    const double cf = n_coef[s];
+    const int64_t s0 = dom_off[d];
+    const int n = int(dom_off[d + 1] - s0);
+    const double* W0 = weights + s0 + 3 * int64_t(d);
+    const double* W1 = W0 + wstride;
+    double v0 = 0.0, v1 = 0.0;
+    for (int j = lane; j < n; j += 64) {
+      const int c = slot_pt[s0 + j];
+      const double dy = y - ys[c], dx = x - xs[c];
+      const double ph = tps_phi(dy * dy + dx * dx);
+      v0 += W0[j] * ph;
+      v1 += W1[j] * ph;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      v0 += __shfl_xor(v0, off, 64);
+      v1 += __shfl_xor(v1, off, 64);
+    }
+    v0 += W0[n] + W0[n + 1] * y + W0[n + 2] * x;
+    v1 += W1[n] + W1[n + 1] * y + W1[n + 2] * x;
+    acc0 += cf * v0;
+    acc1 += cf * v1;
+    nrm += cf;
+  }
+  if (lane == 0) {
+    out[p] = acc0 / nrm;
+    out[npts + p] = acc1 / nrm;
+  }
+}
+
+int tps_fail(int code, const std::string& msg) {
+  fia::last_error() = msg;
+  return code;
+}
+
+int tps_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return tps_fail(PIV_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return PIV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys,
+                   const float* vx, const float* vy, const int64_t* dom_off,
+                   const int32_t* slot_pt, const int64_t* mat_off, double smoothing,
+                   double threshold, double* mats, double* refl, double* vwork,
+                   double* smooth_slots, int64_t nslots, double* weights, int64_t wstride,
+                   double* stats, int32_t* status, void* stream) {
+  if (nd <= 0) return PIV_OK;
+  if (n_max > kMaxN)
+    return tps_fail(PIV_EUNSUPPORTED, "more than 1024 centers in a TPS subdomain");
+  if (!xs || !ys || !vx || !vy || !dom_off || !slot_pt || !mat_off || !mats || !refl ||
+      !vwork || !smooth_slots || !weights || !stats || !status)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), 0, s, xs, ys, dom_off, slot_pt,
+                     mat_off, smoothing, mats, refl, vwork, status);
+  hipLaunchKernelGGL(k_tps_chol, dim3(nd), dim3(kThreads), 0, s, dom_off, mat_off, mats, vwork,
+                     status);
+  hipLaunchKernelGGL(k_tps_solve, dim3(nd), dim3(kThreads), 0, s, vx, vy, dom_off, slot_pt,
+                     mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,
+                     weights, wstride, stats, status);
+  return tps_check("piv_tps_smooth");
+}
+
+int piv_tps_blend(int32_t npts, const int32_t* c_slots, const int64_t* c_ptr,
+                  const double* coef, const double* norm, const double* smooth_slots,
+                  int64_t nslots, double* out, void* stream) {
+  if (npts <= 0) return PIV_OK;
+  hipLaunchKernelGGL(k_tps_blend, dim3((npts + 255) / 256), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), npts, c_slots, c_ptr, coef, norm,
+                     smooth_slots, nslots, out);
+  return tps_check("piv_tps_blend");
+}
+
+int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
+                        const int32_t* n_slots, const int64_t* n_ptr, const int32_t* n_dom,
+                        const double* n_coef, const double* xs, const double* ys,
+                        const int64_t* dom_off, const int32_t* slot_pt, const double* weights,
+                        int64_t wstride, double* out, void* stream) {
+  if (npts <= 0) return PIV_OK;
+  hipLaunchKernelGGL(k_tps_interp, dim3((npts + 3) / 4), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), npts, px, py, n_slots, n_ptr, n_dom,
+                     n_coef, xs, ys, dom_off, slot_pt, weights, wstride, out);
+  return tps_check("piv_tps_interpolate");
+}
+
+}  // extern "C"

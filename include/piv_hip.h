@@ -53,6 +53,12 @@ extern "C" {
 #define PIV_WIN_BAD_CROP 5      /* "Bad im_crop shape." */
 #define PIV_WIN_NO_PEAK 6       /* no finite value in the search box */
 
+/* options of piv_correlate_batch */
+#define PIV_OPT_SUBPIX 1     /* sub-pixel fit of the peak (last pass) */
+#define PIV_OPT_MAP_SQUARE 2 /* with displacement_max None, the maps keep the
+                                second-chance NaN square, as the reference's
+                                _loop_vectors maps do (correl.py:234-237) */
+
 /* window flag bits */
 #define PIV_WINFLAG_INVALID 1 /* crop outside the padded image: no compute */
 
@@ -125,12 +131,13 @@ int piv_images_min(const piv_image* images, int32_t nimages, float* out_min,
 /* Fused crop -> mean removal -> FFT cross-correlation -> normalisation ->
  * masked peak search (second chance, secondary peaks) -> optional sub-pixel
  * fit, for nwin windows. `images` and `windows` are device arrays; every
- * image of the call has element type `dtype`. `maps`
+ * image of the call has element type `dtype`; `options` is a set of
+ * PIV_OPT_* bits. `maps`
  * (device, nwin*n0*n1 float32, may be NULL) receives the correlation maps in
  * the reference layout (after _like_fftshift). */
 int piv_correlate_batch(piv_plan* plan, int32_t dtype, const piv_image* images,
                         const piv_window* windows, int32_t nwin,
-                        int32_t apply_subpix, float* maps,
+                        int32_t options, float* maps,
                         const piv_results* res, void* stream);
 
 /* Peak search (+ optional sub-pixel fit) on existing maps (nwin*n0*n1,
@@ -173,6 +180,49 @@ int piv_fix(float* dx, float* dy, const float* correl_max, const int32_t* err,
             int32_t ny, int32_t nx, float correl_min, float displacement_max,
             float threshold, int32_t* flags, float* diff2, float* mean_x,
             float* mean_y, void* stream);
+
+/* ---- thin-plate-spline subdomains (FP64), replacing
+ *      ThinPlateSplineSubdom.compute_tps_weights_subdom / interpolate
+ *      (thin_plate_spline_subdom.py:128-159,224-280) and
+ *      compute_tps_weights (thin_plate_spline.py:168-216) ----
+ * Subdomain d holds the slots [dom_off[d], dom_off[d+1]) (at most 1024);
+ * slot s is center slot_pt[s] of xs/ys (float64) with values vx/vy
+ * (float32). mats: workspace of sum_d n_d^2 doubles at mat_off[d]; refl:
+ * 16 doubles per subdomain; vwork: 6 doubles per slot. Outputs: the
+ * smoothed value of every slot (smooth_slots[2][nslots]), the weights of
+ * every subdomain (weights[2][wstride], n_d + 3 values at
+ * dom_off[d] + 3 d: the n_d center weights then the [1, y, x] terms),
+ * per-subdomain stats[6] (x then y: fixed vectors, max |diff| of the first
+ * solve, solves) and status (non-zero: singular system). threshold < 0
+ * disables the outlier iterations. */
+int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
+                   const double* ys, const float* vx, const float* vy,
+                   const int64_t* dom_off, const int32_t* slot_pt,
+                   const int64_t* mat_off, double smoothing, double threshold,
+                   double* mats, double* refl, double* vwork,
+                   double* smooth_slots, int64_t nslots, double* weights,
+                   int64_t wstride, double* stats, int32_t* status,
+                   void* stream);
+
+/* out[c][i] = sum over the slots k of point i (c_ptr[i]..c_ptr[i+1] into
+ * c_slots, in subdomain order) of coef[s] * smooth_slots[c][s], divided by
+ * norm[i] (the blend of thin_plate_spline_subdom.py:145-158). */
+int piv_tps_blend(int32_t npts, const int32_t* c_slots, const int64_t* c_ptr,
+                  const double* coef, const double* norm,
+                  const double* smooth_slots, int64_t nslots, double* out,
+                  void* stream);
+
+/* TPS evaluation at npts new positions (px, py): for each of the point's
+ * subdomains (n_slots/n_ptr, subdomain n_dom[s], weight n_coef[s]) the
+ * weighted sum of w_j r^2 ln(r^2) / 2 plus the linear part, blended and
+ * normalised (thin_plate_spline_subdom.py:161-234). out[2][npts]. */
+int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
+                        const int32_t* n_slots, const int64_t* n_ptr,
+                        const int32_t* n_dom, const double* n_coef,
+                        const double* xs, const double* ys,
+                        const int64_t* dom_off, const int32_t* slot_pt,
+                        const double* weights, int64_t wstride, double* out,
+                        void* stream);
 
 #ifdef __cplusplus
 }

@@ -42,7 +42,9 @@ def test_call_map_and_norm_match_oracle(shape):
 @pytest.mark.parametrize("shape", [(32, 32), (64, 32), (48, 48), (24, 24)])
 def test_known_answers(shape):
     # src/fluidimage/calcul/test_correl.py:75-109: zero shift within 1e-3,
-    # autocorrelation max 1, shift (3.3, 5.8) within 0.8 px
+    # autocorrelation max 1, shift (3.3, 5.8) within 0.8 px; the sub-pixel
+    # fit equals the oracle's within 1e-3 px
+    from oracle import correl as oc
     from tests.helpers import shifted_pair
 
     print("case: known answers", shape, flush=True)
@@ -58,7 +60,11 @@ def test_known_answers(shape):
         dx, dy, cmax, _ = c.compute_displacements_from_correl(m, norm=norm)
         assert np.allclose([dx, dy], [3.3, 5.8], atol=0.8)
         sx, sy = c.apply_subpix(dx, dy, m)
-        assert np.allclose([sx, sy], [3.3, 5.8], atol=0.2)
+        o = oc.OracleCorrelFFT(shape, displacement_max=dmax, method_subpix="2d_gaussian2")
+        m64, _ = o(im0, im1)
+        ex, ey = o.apply_subpix(dx, dy, m64)
+        assert np.allclose([sx, sy], [ex, ey], atol=1e-3)
+        assert np.allclose([sx, sy], [3.3, 5.8], atol=0.5)
 
 
 def _planted_maps(rng, shape, n):

@@ -127,6 +127,7 @@ def test_later_pass_offsets_padding_and_bad_crops():
     dyin = rng.integers(-40, 41, n).astype(np.int32)
     dxin[: n // 2] //= 8
     dyin[: n // 2] //= 8
+    dxin[n - 12:] = rng.choice([-300, 300], 12)  # both crops leave the padded image
     gpu = work._loop_vectors(im0, im1, dxin, dyin)
     ow = OracleWorkPIVPass(_oracle_params(p), 1, 32, precision="f64")
     ow.prepare_with_image(im0.shape)

@@ -1,0 +1,80 @@
+"""GPU FP64 thin-plate-spline subdomains against the oracle (numpy LAPACK).
+
+Bar: smoothed and interpolated displacements within 1e-6 px (the end-to-end
+bar is 1e-3 px), identical outlier iteration counts."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(rng, n, shape=(512, 640), outliers=0.03, jitter=0.4):
+    ny, nx = shape
+    step = np.sqrt(ny * nx / n)
+    ys, xs = np.mgrid[step / 2:ny:step, step / 2:nx:step]
+    ys = ys.ravel() + rng.uniform(-jitter, jitter, ys.size)
+    xs = xs.ravel() + rng.uniform(-jitter, jitter, xs.size)
+    u = 2.0 * np.sin(xs / 90.0) + 0.5 * np.cos(ys / 70.0)
+    v = 1.5 * np.cos(xs / 60.0 + ys / 110.0)
+    bad = rng.random(xs.size) < outliers
+    u[bad] += rng.normal(0, 6, bad.sum())
+    v[bad] += rng.normal(0, 6, bad.sum())
+    return ys, xs, u.astype(np.float32), v.astype(np.float32)
+
+
+@pytest.mark.parametrize("n, subdom, smoothing, threshold",
+                         [(1200, 200, 2.0, 1.5), (3000, 300, 5.0, 1.5), (700, 150, 0.5, None)])
+def test_tps_subdom_matches_oracle(n, subdom, smoothing, threshold):
+    from fluidimage_amd.calcul.tps import ThinPlateSplineSubdomHIP
+    from oracle.tps import ThinPlateSplineSubdom
+
+    print("case: tps", n, subdom, smoothing, threshold, flush=True)
+    rng = np.random.default_rng(31)
+    ys, xs, u, v = _field(rng, n)
+    centers = np.vstack([ys, xs])
+    new = np.vstack([np.repeat(np.arange(8, 512, 16.0), 40), np.tile(np.arange(8, 640, 16.0), 32)])
+    gpu = ThinPlateSplineSubdomHIP(centers, subdom, smoothing, threshold=threshold)
+    sx, sy, ix, iy, summary = gpu.smooth_and_interpolate(u, v, new)
+    ref = ThinPlateSplineSubdom(centers, subdom, smoothing, threshold=threshold)
+    assert ref.nb_subdom == gpu.nb_subdom
+    for d in range(ref.nb_subdom):
+        assert np.array_equal(ref.indices_domains[d], gpu.indices_domains[d])
+    if threshold is None:
+        # the reference's summary needs a threshold; compare the solves only
+        rx = [ref.compute_tps_weights_iter(ref.centers[:, ref.indices_domains[d]],
+                                            u[ref.indices_domains[d]], ref.smoothing_coef)
+              for d in range(ref.nb_subdom)]
+        for d in range(ref.nb_subdom):
+            assert np.allclose(gpu.weights_x[d], rx[d][1], rtol=1e-7, atol=1e-9)
+        return
+    esx, wx, sumx = ref.compute_tps_weights_subdom(u)
+    esy, wy, sumy = ref.compute_tps_weights_subdom(v)
+    ref.init_with_new_positions(new)
+    eix = ref.interpolate(wx)
+    eiy = ref.interpolate(wy)
+    assert np.abs(sx - esx).max() < 1e-6
+    assert np.abs(sy - esy).max() < 1e-6
+    assert np.abs(ix - eix).max() < 1e-6
+    assert np.abs(iy - eiy).max() < 1e-6
+    assert summary["nb_iterations"] == sumy["nb_iterations"]
+    assert summary["nb_fixed_vectors"] == sumy["nb_fixed_vectors"]
+    assert gpu.summary_x["nb_iterations"] == sumx["nb_iterations"]
+    for d in range(ref.nb_subdom):
+        scale = np.abs(wx[d]).max()
+        assert np.abs(gpu.weights_x[d] - wx[d]).max() <= 1e-6 * scale + 1e-9
+
+
+def test_tps_reproduces_affine_field_on_gpu():
+    from fluidimage_amd.calcul.tps import ThinPlateSplineSubdomHIP
+
+    rng = np.random.default_rng(32)
+    ys, xs, _, _ = _field(rng, 900, outliers=0.0)
+    u = (0.3 + 0.01 * xs - 0.02 * ys).astype(np.float32)
+    v = (-1.0 + 0.005 * ys).astype(np.float32)
+    gpu = ThinPlateSplineSubdomHIP(np.vstack([ys, xs]), 200, 3.0, threshold=1.5)
+    new = np.vstack([rng.uniform(20, 480, 200), rng.uniform(20, 600, 200)])
+    sx, sy, ix, iy, _ = gpu.smooth_and_interpolate(u, v, new)
+    assert np.abs(sx - u).max() < 1e-5
+    assert np.abs(ix - (0.3 + 0.01 * new[1] - 0.02 * new[0])).max() < 1e-5
+    assert np.abs(iy - (-1.0 + 0.005 * new[0])).max() < 1e-5
