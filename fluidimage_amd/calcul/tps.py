@@ -4,17 +4,18 @@ Same method and parameters as the reference's ``ThinPlateSplineSubdom``
 (src/fluidimage/calcul/interpolate/thin_plate_spline_subdom.py:25-280) and
 ``compute_tps_weights`` (src/fluidimage/calcul/interpolate/thin_plate_spline.py:168-216):
 
-* subdomain grid, rescaled smoothing, 20 % buffers, half-open membership,
-  partition-of-unity weights (1 - x^2)(1 - y^2) + 0.001 -- integer/geometry
-  bookkeeping done here on the host (O(number of vectors));
+* selection of the finite vectors, subdomain membership (half-open limits
+  with 20 % buffers) and partition-of-unity weights
+  (1 - x^2)(1 - y^2) + 0.001 run on the device; the host only turns five
+  scalars (count and bounds) into the subdomain grid with the reference's
+  formulas (subdomain counts, ``np.linspace`` limits, rescaled smoothing);
 * per subdomain, the saddle-point system [[K + sI, P], [P^T, 0]] with
   K = r^2 ln(r^2) / 2 and P = [1, y, x] is solved in FP64 by libpiv_hip:
   Householder QR of P, congruence of K + sI onto the null space of P^T
   (symmetric positive definite there) and a blocked Cholesky factorisation,
   reused for the x and y components and for every outlier iteration
-  (threshold_tps, at most 10 solves, thin_plate_spline_subdom.py:236-280);
-* blending of the smoothed values and evaluation at the new positions on
-  the GPU.
+  (threshold_tps, at most 11 solves, thin_plate_spline_subdom.py:236-280);
+* blending of the smoothed values and evaluation at the new positions.
 
 The reference solves the same systems with LAPACK's LU (``np.linalg.solve``);
 both give the unique solution up to FP64 rounding.
@@ -27,225 +28,210 @@ import numpy as np
 
 from .. import _native
 
+MAX_SLOTS = 4
+MAX_CENTERS_PER_SUBDOM = 1024
+
 
 class SingularSystemError(ArithmeticError):
     """A subdomain system is singular (the reference's LinAlgError)."""
 
 
-def _membership(xs, ys, lmin_x, lmax_x, lmin_y, lmax_y):
-    """(d, i) pairs of points i inside subdomain d = iy * nsx + ix, with the
-    reference's half-open tests, sorted by d then i."""
-    nsx = len(lmin_x)
-    in_x = (xs[None, :] >= lmin_x[:, None]) & (xs[None, :] < lmax_x[:, None])
-    in_y = (ys[None, :] >= lmin_y[:, None]) & (ys[None, :] < lmax_y[:, None])
-    ix_i, ix_x = np.nonzero(in_x.T)  # (point, ix) sorted by point
-    iy_i, iy_y = np.nonzero(in_y.T)
-    npts = len(xs)
-    cx = np.bincount(ix_i, minlength=npts)
-    cy = np.bincount(iy_i, minlength=npts)
-    sx = np.concatenate([[0], np.cumsum(cx)])
-    sy = np.concatenate([[0], np.cumsum(cy)])
-    reps = cx * cy
-    tot = int(reps.sum())
-    pts = np.repeat(np.arange(npts), reps)
-    # position of each pair inside its point's cx * cy block
-    start = np.concatenate([[0], np.cumsum(reps)])[:-1]
-    k = np.arange(tot) - np.repeat(start, reps)
-    cyr = np.repeat(cy, reps)
-    kx = k // np.maximum(cyr, 1)
-    ky = k % np.maximum(cyr, 1)
-    dx_ = ix_x[sx[pts] + kx]
-    dy_ = iy_y[sy[pts] + ky]
-    dom = dy_ * nsx + dx_
-    order = np.lexsort((pts, dom))
-    return dom[order], pts[order]
+def subdomain_grid(count, x_min, x_max, y_min, y_max, subdom_size, smoothing_coef,
+                   percent_buffer_area=20):
+    """Host arithmetic of thin_plate_spline_subdom.py:53-90 on the bounds."""
+    range_x = x_max - x_min
+    range_y = y_max - y_min
+    if range_x == 0 or range_y == 0:
+        raise SingularSystemError("degenerate set of centers")
+    aspect_ratio = range_y / range_x
+    nb_subdom = count / subdom_size
+    nsx = int(np.floor(sqrt(nb_subdom / aspect_ratio))) or 1
+    nsy = int(np.ceil(sqrt(nb_subdom * aspect_ratio))) or 1
+    x_dom = np.linspace(x_min, x_max, nsx + 1)
+    y_dom = np.linspace(y_min, y_max, nsy + 1)
+    num_vectors_per_subdom = count / (nsx * nsy)
+    smoothing = smoothing_coef * sqrt(
+        (x_dom[1] - x_dom[0]) * (y_dom[1] - y_dom[0]) / num_vectors_per_subdom)
+    coef_buffer = percent_buffer_area / 100
+    bx = coef_buffer * range_x / nsx
+    by = coef_buffer * range_y / nsy
+    lim = np.concatenate([x_dom[:-1] - bx, x_dom[1:] + bx, y_dom[:-1] - by, y_dom[1:] + by])
+    return nsx, nsy, smoothing, lim
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def tps_device(dx, dy, xs, ys, new_x, new_y, subdom_size, smoothing_coef, threshold,
+               percent_buffer_area=20, stream=None, keep=False):
+    """TPS smoothing of the finite vectors of (dx, dy) at (xs, ys) and
+    evaluation at (new_x, new_y); all tensors on the device (dx, dy float32,
+    positions float64). Two small device-to-host reads size the problem.
+
+    Returns a dict of device tensors: ``smoothed`` (2, count) at the
+    selected centers, ``index`` (count) of the selected vectors, ``approx``
+    (2, M) at the new positions, plus host ``nb_subdom``, ``stats`` (nd, 6).
+    """
+    import torch
+
+    lib = _native.load()
+    dev = dx.device
+    sp = _native.stream_ptr(stream)
+    n = int(dx.numel())
+    f64 = dict(dtype=torch.float64, device=dev)
+    cx = torch.empty(n, **f64)
+    cy = torch.empty(n, **f64)
+    vx = torch.empty(n, dtype=torch.float32, device=dev)
+    vy = torch.empty(n, dtype=torch.float32, device=dev)
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    info = torch.empty(5, **f64)
+    _native.check(lib.piv_tps_select(n, _p(dx), _p(dy), _p(xs), _p(ys), _p(cx), _p(cy),
+                                     _p(vx), _p(vy), _p(idx), _p(info), sp), "piv_tps_select")
+    count, x_min, x_max, y_min, y_max = info.cpu().numpy().tolist()
+    count = int(count)
+    if count == 0:
+        raise SingularSystemError("Only nan.")
+    nsx, nsy, smoothing, lim_h = subdomain_grid(count, x_min, x_max, y_min, y_max,
+                                                subdom_size, smoothing_coef,
+                                                percent_buffer_area)
+    nd = nsx * nsy
+    lim = torch.from_numpy(lim_h).to(dev)
+    counts = torch.empty(nd, dtype=torch.int32, device=dev)
+    _native.check(lib.piv_tps_count(nsx, nsy, _p(lim), count, _p(cx), _p(cy), _p(counts), sp),
+                  "piv_tps_count")
+    sizes = counts.cpu().numpy().astype(np.int64)
+    if sizes.min() < 1:
+        raise SingularSystemError("empty subdomain")
+    n_max = int(sizes.max())
+    if n_max > MAX_CENTERS_PER_SUBDOM:
+        raise NotImplementedError(
+            f"{n_max} centers in a TPS subdomain (limit {MAX_CENTERS_PER_SUBDOM}): "
+            "use a smaller params.multipass.subdom_size")
+    dom_off_h = np.concatenate([[0], np.cumsum(sizes)])
+    mat_off_h = np.concatenate([[0], np.cumsum(sizes**2)])
+    nslots = int(dom_off_h[-1])
+    dom_off = torch.from_numpy(dom_off_h).to(dev)
+    mat_off = torch.from_numpy(mat_off_h).to(dev)
+    slot_pt = torch.empty(nslots, dtype=torch.int32, device=dev)
+    _native.check(lib.piv_tps_fill(nsx, nsy, _p(lim), count, _p(cx), _p(cy), _p(dom_off),
+                                   _p(slot_pt), sp), "piv_tps_fill")
+    status = torch.zeros(nd + 1, dtype=torch.int32, device=dev)
+    c_slot = torch.empty(count * MAX_SLOTS, dtype=torch.int32, device=dev)
+    c_coef = torch.empty(count * MAX_SLOTS, **f64)
+    c_norm = torch.empty(count, **f64)
+    _native.check(lib.piv_tps_point_slots(nsx, nsy, _p(lim), count, _p(cx), _p(cy), _p(dom_off),
+                                          _p(slot_pt), _p(c_slot), _p(c_coef), _p(c_norm),
+                                          _p(status[nd:]), sp), "piv_tps_point_slots")
+    m = int(new_x.numel())
+    n_dom = torch.empty(m * MAX_SLOTS, dtype=torch.int32, device=dev)
+    n_coef = torch.empty(m * MAX_SLOTS, **f64)
+    n_norm = torch.empty(m, **f64)
+    _native.check(lib.piv_tps_point_slots(nsx, nsy, _p(lim), m, _p(new_x), _p(new_y), None, None,
+                                          _p(n_dom), _p(n_coef), _p(n_norm), _p(status[nd:]),
+                                          sp), "piv_tps_point_slots")
+    mats = torch.empty(int(mat_off_h[-1]), **f64)
+    refl = torch.empty((nd, 16), **f64)
+    vwork = torch.empty(nslots * 6, **f64)
+    smooth_slots = torch.empty((2, nslots), **f64)
+    wstride = nslots + 3 * nd
+    weights = torch.empty((2, wstride), **f64)
+    stats = torch.zeros((nd, 6), **f64)
+    thr = -1.0 if threshold is None else float(threshold)
+    _native.check(
+        lib.piv_tps_smooth(nd, n_max, _p(cx), _p(cy), _p(vx), _p(vy), _p(dom_off), _p(slot_pt),
+                           _p(mat_off), float(smoothing), thr, _p(mats), _p(refl), _p(vwork),
+                           _p(smooth_slots), nslots, _p(weights), wstride, _p(stats),
+                           _p(status), sp),
+        "piv_tps_smooth")
+    smoothed = torch.empty((2, count), **f64)
+    _native.check(lib.piv_tps_blend(count, _p(c_slot), _p(c_coef), _p(c_norm), _p(smooth_slots),
+                                    nslots, _p(smoothed), sp), "piv_tps_blend")
+    approx = torch.empty((2, m), **f64)
+    _native.check(lib.piv_tps_interpolate(m, _p(new_x), _p(new_y), _p(n_dom), _p(n_coef),
+                                          _p(n_norm), _p(cx), _p(cy), _p(dom_off), _p(slot_pt),
+                                          _p(weights), wstride, _p(approx), sp),
+                  "piv_tps_interpolate")
+    st = status.cpu().numpy()
+    if st[nd]:
+        raise NotImplementedError("a point lies in more than 4 TPS subdomains")
+    if st[:nd].any():
+        raise SingularSystemError(
+            f"{int((st[:nd] != 0).sum())} subdomain system(s) not positive definite")
+    out = {"smoothed": smoothed, "index": idx[:count], "approx": approx, "nb_subdom": nd,
+           "nsx": nsx, "nsy": nsy, "smoothing_coef": smoothing, "count": count}
+    if keep:
+        out.update(stats=stats, weights=weights, dom_off=dom_off_h, slot_pt=slot_pt)
+    else:
+        out["stats"] = stats
+    return out
+
+
+def summaries(stats):
+    """Reference-style summaries (x then y) from the per-subdomain stats."""
+    s = stats.cpu().numpy() if hasattr(stats, "cpu") else np.asarray(stats)
+    out = []
+    for c in range(2):
+        out.append({
+            "nb_fixed_vectors": [int(v) for v in s[:, 3 * c]],
+            "max(Udiff)": [float(v) for v in s[:, 3 * c + 1]],
+            "nb_iterations": [int(v) for v in s[:, 3 * c + 2]],
+            "nb_fixed_vectors_tot": int(s[:, 3 * c].sum()),
+        })
+    return out
 
 
 class ThinPlateSplineSubdomHIP:
-    """GPU version of ThinPlateSplineSubdom (``centers = vstack([ys, xs])``)."""
+    """Host-array front end of ``tps_device`` (``centers = vstack([ys, xs])``)."""
 
     def __init__(self, centers, subdom_size, smoothing_coef, threshold=None,
                  percent_buffer_area=20):
-        centers = np.asarray(centers, dtype=np.float64)
-        self.centers = centers
+        self.centers = np.asarray(centers, dtype=np.float64)
         self.subdom_size = subdom_size
+        self.smoothing_coef_param = smoothing_coef
         self.threshold = threshold
-        xs = centers[1]
-        ys = centers[0]
-        self.num_centers = xs.size
-        x_min, x_max = xs.min(), xs.max()
-        y_min, y_max = ys.min(), ys.max()
-        range_x = x_max - x_min
-        range_y = y_max - y_min
-        if range_x == 0 or range_y == 0:
-            raise SingularSystemError("degenerate set of centers")
-        aspect_ratio = range_y / range_x
-        nb_subdom = xs.size / subdom_size
-        nb_subdomx = int(np.floor(sqrt(nb_subdom / aspect_ratio))) or 1
-        nb_subdomy = int(np.ceil(sqrt(nb_subdom * aspect_ratio))) or 1
-        self.nb_subdomx = nb_subdomx
-        self.nb_subdomy = nb_subdomy
-        self.nb_subdom = nb_subdomx * nb_subdomy
-        x_dom = np.linspace(x_min, x_max, nb_subdomx + 1)
-        y_dom = np.linspace(y_min, y_max, nb_subdomy + 1)
-        num_vectors_per_subdom = xs.size / self.nb_subdom
-        self.smoothing_coef = smoothing_coef * sqrt(
-            (x_dom[1] - x_dom[0]) * (y_dom[1] - y_dom[0]) / num_vectors_per_subdom
-        )
-        coef_buffer = percent_buffer_area / 100
-        bx = coef_buffer * range_x / nb_subdomx
-        by = coef_buffer * range_y / nb_subdomy
-        self.limits_min_x = x_dom[:-1] - bx
-        self.limits_max_x = x_dom[1:] + bx
-        self.limits_min_y = y_dom[:-1] - by
-        self.limits_max_y = y_dom[1:] + by
-        ix = np.tile(np.arange(nb_subdomx), nb_subdomy)
-        iy = np.repeat(np.arange(nb_subdomy), nb_subdomx)
-        self.xmax_domains = self.limits_max_x[ix]
-        self.ymax_domains = self.limits_max_y[iy]
-        self.xc_domains = 0.5 * (self.limits_min_x[ix] + self.limits_max_x[ix])
-        self.yc_domains = 0.5 * (self.limits_min_y[iy] + self.limits_max_y[iy])
-        dom, pts = _membership(xs, ys, self.limits_min_x, self.limits_max_x,
-                               self.limits_min_y, self.limits_max_y)
-        self.slot_dom = dom
-        self.slot_pt = pts
-        self.dom_offsets = np.searchsorted(dom, np.arange(self.nb_subdom + 1)).astype(np.int64)
-        self.indices_domains = [pts[self.dom_offsets[d]:self.dom_offsets[d + 1]]
-                                for d in range(self.nb_subdom)]
-        self.slot_coef = self._coef(xs[pts], ys[pts], dom)
-        self.norm_coefs = np.zeros(self.num_centers)
-        # accumulate in subdomain order like the reference (subdom.py:118-126)
-        for d in range(self.nb_subdom):
-            a, b = self.dom_offsets[d], self.dom_offsets[d + 1]
-            self.norm_coefs[pts[a:b]] += self.slot_coef[a:b]
-        if np.any(np.diff(self.dom_offsets) < 1):
-            raise SingularSystemError("empty subdomain")
+        self.percent_buffer_area = percent_buffer_area
+        self.num_centers = self.centers.shape[1]
 
-    def _coef(self, xs_d, ys_d, dom):
-        """thin_plate_spline_subdom.py:208-222"""
-        xc = self.xc_domains[dom]
-        yc = self.yc_domains[dom]
-        dx_max = self.xmax_domains[dom] - xc
-        dy_max = self.ymax_domains[dom] - yc
-        dx2 = (xs_d - xc) ** 2 / dx_max**2
-        dy2 = (ys_d - yc) ** 2 / dy_max**2
-        return (1 - dx2) * (1 - dy2) + 0.001
-
-    # ------------------------------------------------------------------
     def smooth_and_interpolate(self, deltaxs, deltays, new_positions, stream=None):
         """compute_tps_weights_subdom for x and y, then init_with_new_positions
         + interpolate on ``new_positions = vstack([ys, xs])``.
 
-        Returns (dxs_smooth, dys_smooth, dxs_interp, dys_interp, summary);
-        the per-subdomain weights are kept in ``weights_x`` / ``weights_y``.
+        Returns (dxs_smooth, dys_smooth, dxs_interp, dys_interp, summary)
+        where summary is the y-component summary (the one the reference
+        keeps after its second call).
         """
         import torch
 
         dev = torch.device("cuda")
-        lib = _native.load()
-        nd = self.nb_subdom
-        sizes = np.diff(self.dom_offsets)
-        n_max = int(sizes.max())
-        # per-subdomain matrix storage: n_d x n_d doubles
-        mat_off = np.concatenate([[0], np.cumsum(sizes.astype(np.int64) ** 2)])
-        new_positions = np.asarray(new_positions, dtype=np.float64)
-        nys, nxs = new_positions[0], new_positions[1]
-        ndom, npt = _membership(nxs, nys, self.limits_min_x, self.limits_max_x,
-                                self.limits_min_y, self.limits_max_y)
-        new_off = np.searchsorted(ndom, np.arange(nd + 1)).astype(np.int64)
-        new_coef = self._coef(nxs[npt], nys[npt], ndom)
-        # slots of every point, in subdomain order (for the ordered blends)
-        c_order = np.lexsort((self.slot_dom, self.slot_pt))
-        c_ptr = np.searchsorted(self.slot_pt[c_order], np.arange(self.num_centers + 1))
-        n_order = np.lexsort((ndom, npt))
-        n_ptr = np.searchsorted(npt[n_order], np.arange(len(nxs) + 1))
 
         def t(a, dtype):
             return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(dev)
 
-        xs = t(self.centers[1], np.float64)
-        ys = t(self.centers[0], np.float64)
-        vx = t(deltaxs, np.float32)
-        vy = t(deltays, np.float32)
-        d_off = t(self.dom_offsets, np.int64)
-        d_pt = t(self.slot_pt, np.int32)
-        d_coef = t(self.slot_coef, np.float64)
-        d_matoff = t(mat_off, np.int64)
-        mats = torch.empty(int(mat_off[-1]), dtype=torch.float64, device=dev)
-        refl = torch.empty((nd, 16), dtype=torch.float64, device=dev)
-        vwork = torch.empty(int(self.dom_offsets[-1]) * 6, dtype=torch.float64, device=dev)
-        smooth_slots = torch.empty((2, int(self.dom_offsets[-1])), dtype=torch.float64,
-                                   device=dev)
-        weights = torch.empty((2, int(self.dom_offsets[-1]) + 3 * nd), dtype=torch.float64,
-                              device=dev)
-        stats = torch.zeros((nd, 6), dtype=torch.float64, device=dev)
-        status = torch.zeros(nd, dtype=torch.int32, device=dev)
-        c_slots = t(c_order, np.int32)
-        c_ptr_t = t(c_ptr, np.int64)
-        norm_c = t(self.norm_coefs, np.float64)
-        smooth_out = torch.empty((2, self.num_centers), dtype=torch.float64, device=dev)
-        n_slots = t(n_order, np.int32)
-        n_ptr_t = t(n_ptr, np.int64)
-        n_dom = t(ndom, np.int32)
-        n_coef = t(new_coef, np.float64)
-        n_x = t(nxs, np.float64)
-        n_y = t(nys, np.float64)
-        interp_out = torch.empty((2, len(nxs)), dtype=torch.float64, device=dev)
-        thr = -1.0 if self.threshold is None else float(self.threshold)
-        P = ctypes.c_void_p
-        _native.check(
-            lib.piv_tps_smooth(
-                nd, n_max, P(xs.data_ptr()), P(ys.data_ptr()), P(vx.data_ptr()),
-                P(vy.data_ptr()), P(d_off.data_ptr()), P(d_pt.data_ptr()),
-                P(d_matoff.data_ptr()), float(self.smoothing_coef), thr, P(mats.data_ptr()),
-                P(refl.data_ptr()), P(vwork.data_ptr()), P(smooth_slots.data_ptr()),
-                int(self.dom_offsets[-1]), P(weights.data_ptr()),
-                int(self.dom_offsets[-1]) + 3 * nd, P(stats.data_ptr()), P(status.data_ptr()),
-                _native.stream_ptr(stream)),
-            "piv_tps_smooth",
-        )
-        _native.check(
-            lib.piv_tps_blend(
-                self.num_centers, P(c_slots.data_ptr()), P(c_ptr_t.data_ptr()),
-                P(d_coef.data_ptr()), P(norm_c.data_ptr()), P(smooth_slots.data_ptr()),
-                int(self.dom_offsets[-1]), P(smooth_out.data_ptr()),
-                _native.stream_ptr(stream)),
-            "piv_tps_blend",
-        )
-        _native.check(
-            lib.piv_tps_interpolate(
-                len(nxs), P(n_x.data_ptr()), P(n_y.data_ptr()), P(n_slots.data_ptr()),
-                P(n_ptr_t.data_ptr()), P(n_dom.data_ptr()), P(n_coef.data_ptr()),
-                P(xs.data_ptr()), P(ys.data_ptr()), P(d_off.data_ptr()), P(d_pt.data_ptr()),
-                P(weights.data_ptr()), int(self.dom_offsets[-1]) + 3 * nd,
-                P(interp_out.data_ptr()), _native.stream_ptr(stream)),
-            "piv_tps_interpolate",
-        )
-        torch.cuda.current_stream().synchronize()
-        st = status.cpu().numpy()
-        if st.any():
-            raise SingularSystemError(
-                f"{int((st != 0).sum())} subdomain system(s) not positive definite")
-        w = weights.cpu().numpy()
-        offs = self.dom_offsets
-        self.weights_x = []
-        self.weights_y = []
-        for d in range(nd):
-            a, b = offs[d], offs[d + 1]
-            base_w = a + 3 * d
-            self.weights_x.append(w[0, base_w:base_w + (b - a) + 3].copy())
-            self.weights_y.append(w[1, base_w:base_w + (b - a) + 3].copy())
-        s = stats.cpu().numpy()
-        summary = {
-            "nb_fixed_vectors": [int(v) for v in s[:, 3]],
-            "max(Udiff)": [float(v) for v in s[:, 4]],
-            "nb_iterations": [int(v) for v in s[:, 5]],
-            "nb_fixed_vectors_tot": int(s[:, 3].sum()),
-        }
-        self.summary_x = {"nb_fixed_vectors": [int(v) for v in s[:, 0]],
-                          "max(Udiff)": [float(v) for v in s[:, 1]],
-                          "nb_iterations": [int(v) for v in s[:, 2]],
-                          "nb_fixed_vectors_tot": int(s[:, 0].sum())}
-        out = smooth_out.cpu().numpy()
-        interp = interp_out.cpu().numpy()
-        return out[0], out[1], interp[0], interp[1], summary
+        new_positions = np.asarray(new_positions, dtype=np.float64)
+        out = tps_device(t(deltaxs, np.float32), t(deltays, np.float32),
+                         t(self.centers[1], np.float64), t(self.centers[0], np.float64),
+                         t(new_positions[1], np.float64), t(new_positions[0], np.float64),
+                         self.subdom_size, self.smoothing_coef_param, self.threshold,
+                         self.percent_buffer_area, stream=stream, keep=True)
+        self.nb_subdom = out["nb_subdom"]
+        self.nb_subdomx = out["nsx"]
+        self.nb_subdomy = out["nsy"]
+        self.smoothing_coef = out["smoothing_coef"]
+        dom_off = out["dom_off"]
+        slot_pt = out["slot_pt"].cpu().numpy()
+        index = out["index"].cpu().numpy()
+        self.indices_domains = [index[slot_pt[dom_off[d]:dom_off[d + 1]]]
+                                for d in range(self.nb_subdom)]
+        w = out["weights"].cpu().numpy()
+        self.weights_x, self.weights_y = [], []
+        for d in range(self.nb_subdom):
+            a, b = dom_off[d], dom_off[d + 1]
+            base = a + 3 * d
+            self.weights_x.append(w[0, base:base + (b - a) + 3].copy())
+            self.weights_y.append(w[1, base:base + (b - a) + 3].copy())
+        sx, sy = summaries(out["stats"])
+        self.summary_x = sx
+        sm = out["smoothed"].cpu().numpy()
+        ap = out["approx"].cpu().numpy()
+        return sm[0], sm[1], ap[0], ap[1], sy

@@ -18,7 +18,7 @@
 
 namespace fia {
 
-constexpr int kMaxSquares = 8;  // 1 second-chance square + up to 7 secondary peaks
+constexpr int kMaxPeaks = 7;  // nb_peaks_to_search limit
 
 constexpr int lanes_for(int n) {
   int t = 8;
@@ -67,64 +67,35 @@ struct PlanArgs {
   int has_dmax;
 };
 
-// Peak search state of one window: the LDS map plus the NaN squares
-// written so far (the reference NaNs (2r+1)^2 squares into the map before a
-// second attempt and between secondary peaks, correl.py:234-237,256-259).
+// Peak search of one window on an LDS search copy of the map in which the
+// masked elements (where_large_displacement) and the NaN squares written so
+// far (the reference NaNs (2r+1)^2 squares before a second attempt and
+// between secondary peaks, correl.py:234-237,256-259) are NaN.
 template <int N0, int N1, int T>
 struct Search {
-  const float* m;
-  const uint8_t* mask;
+  float* s;
   int b0s, b0e, b1s, b1e;
-  int sy0[kMaxSquares], sy1[kMaxSquares], sx0[kMaxSquares], sx1[kMaxSquares];
-  int nsq;
 
-  __device__ __forceinline__ bool valid(int i0, int i1) const {
-    float v = m[i0 * N1 + i1];
-    if (v != v) return false;
-    if (mask != nullptr && mask[i0 * N1 + i1]) return false;
-    bool out = true;
-#pragma unroll
-    for (int k = 0; k < kMaxSquares; ++k)
-      if (k < nsq && i0 >= sy0[k] && i0 < sy1[k] && i1 >= sx0[k] && i1 < sx1[k]) out = false;
-    return out;
-  }
-
-  __device__ __forceinline__ void add_square(int i0, int i1, int r) {
-    int a0, a1, c0, c1;
-    py_slice(i0 - r, i0 + r + 1, N0, a0, a1);
-    py_slice(i1 - r, i1 + r + 1, N1, c0, c1);
-    // constant-index stores keep the square table in registers
-#pragma unroll
-    for (int k = 0; k < kMaxSquares; ++k) {
-      if (k == nsq) {
-        sy0[k] = a0;
-        sy1[k] = a1;
-        sx0[k] = c0;
-        sx1[k] = c1;
-      }
-    }
-    if (nsq < kMaxSquares) ++nsq;
-  }
-
-  // nan_indices_max (correl.py:88-132): last maximal valid element of the
+  // nan_indices_max (correl.py:88-132): last maximal non-NaN element of the
   // box in row-major order; returns the PIV_WIN_* code of the peak.
   __device__ int find(int lane, int& oi0, int& oi1) const {
     float best = 0.f;
     int blin = -1;
     for (int c = b1s + lane; c < b1e; c += T) {
+#pragma unroll 4
       for (int r = b0s; r < b0e; ++r) {
-        if (!valid(r, c)) continue;
-        float v = m[r * N1 + c];
-        int lin = r * N1 + c;
-        if (blin < 0 || v > best || (v == best && lin > blin)) {
+        const float v = s[r * N1 + c];
+        if (v != v) continue;
+        // rows ascend within a lane: ">=" keeps the last maximum
+        if (blin < 0 || v >= best) {
           best = v;
-          blin = lin;
+          blin = r * N1 + c;
         }
       }
     }
     for (int off = T / 2; off > 0; off >>= 1) {
-      float ob = __shfl_xor(best, off, T);
-      int ol = __shfl_xor(blin, off, T);
+      const float ob = __shfl_xor(best, off, T);
+      const int ol = __shfl_xor(blin, off, T);
       if (ol >= 0 && (blin < 0 || ob > best || (ob == best && ol > blin))) {
         best = ob;
         blin = ol;
@@ -134,14 +105,39 @@ struct Search {
       oi0 = oi1 = 0;
       return PIV_WIN_NO_PEAK;
     }
-    int i0 = blin / N1, i1 = blin % N1;
+    const int i0 = blin / N1, i1 = blin % N1;
     oi0 = i0;
     oi1 = i1;
     if (i0 == 0 || i0 == N0 - 1 || i1 == 0 || i1 == N1 - 1) return PIV_WIN_PEAK_BOUNDARY;
+    bool nan = false;
+#pragma unroll
     for (int d0 = -1; d0 <= 1; ++d0)
-      for (int d1 = -1; d1 <= 1; ++d1)
-        if (!valid(i0 + d0, i1 + d1)) return PIV_WIN_PEAK_NAN;
-    return PIV_WIN_OK;
+#pragma unroll
+      for (int d1 = -1; d1 <= 1; ++d1) {
+        const float v = s[(i0 + d0) * N1 + i1 + d1];
+        nan |= v != v;
+      }
+    return nan ? PIV_WIN_PEAK_NAN : PIV_WIN_OK;
+  }
+
+  // NaN the (2r+1)^2 square around (i0, i1) with numpy slice bounds; the
+  // window's lanes share one wavefront, so a wavefront fence orders the
+  // writes before the next search.
+  __device__ void nan_square(int lane, int i0, int i1, int r, int* q) {
+    int a0, a1, c0, c1;
+    py_slice(i0 - r, i0 + r + 1, N0, a0, a1);
+    py_slice(i1 - r, i1 + r + 1, N1, c0, c1);
+    const int h = a1 > a0 ? a1 - a0 : 0, w = c1 > c0 ? c1 - c0 : 0;
+    for (int k = lane; k < h * w; k += T) s[(a0 + k / w) * N1 + c0 + k % w] = __builtin_nanf("");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (q != nullptr) {
+      q[0] = a0;
+      q[1] = a1;
+      q[2] = c0;
+      q[3] = c1;
+    }
   }
 };
 
@@ -163,13 +159,14 @@ __device__ void gaussian2(const float* m, int ix, int iy, double& dpx, double& d
   for (int k = 0; k < 9; ++k) c[k] -= mn;
   float mx = c[0];
   for (int k = 1; k < 9; ++k) mx = fmaxf(mx, c[k]);
-  for (int k = 0; k < 9; ++k) c[k] /= mx;
+  const float inv = 1.f / mx;
+  for (int k = 0; k < 9; ++k) c[k] *= inv;
   double c10 = 0, c01 = 0, c11 = 0, c20 = 0, c02 = 0;
   for (int a = 0; a < 3; ++a) {
     for (int b = 0; b < 3; ++b) {
       float v = c[3 * a + b];
       if (v == 0.f) v = 1e-8f;
-      double l = log(double(v));
+      const double l = double(logf(v));
       int x = b - 1, y = a - 1;
       c10 += x * l;
       c01 += y * l;
@@ -223,35 +220,30 @@ __device__ int subpix(const float* m, int method, int nsub, int ix, int iy, doub
 
 // Everything that follows the correlation map of one window: peak search
 // with second chance, secondary peaks, sub-pixel fit and the result
-// records. `m` is the LDS map (reference layout), `win` the window index.
-// Returns the second-chance square (rows [*q0, *q1), cols [*q2, *q3)) when
-// one was written, for the map side effect with displacement_max None.
+// records. `m` is the LDS map (reference layout), `srch` its LDS search copy
+// (masked elements NaN), `win` the window index. `sq` receives the
+// second-chance square (rows [sq0, sq1), cols [sq2, sq3)), empty if none,
+// for the map side effect with displacement_max None.
 template <int N0, int N1, int T>
-__device__ void finish_window(const float* m, const uint8_t* mask, const PlanArgs& pa, int lane,
+__device__ void finish_window(const float* m, float* srch, const PlanArgs& pa, int lane,
                               int win, double norm, int apply_subpix, int dx_in, int dy_in,
                               const piv_results& res, int* sq) {
   constexpr int iy0 = N0 / 2 - 1, ix0 = N1 / 2 - 1;
   Search<N0, N1, T> s;
-  s.m = m;
-  s.mask = mask;
+  s.s = srch;
   s.b0s = pa.b0s;
   s.b0e = pa.b0e;
   s.b1s = pa.b1s;
   s.b1e = pa.b1e;
-  s.nsq = 0;
   sq[0] = sq[1] = sq[2] = sq[3] = 0;
   int i0, i1;
   int code = s.find(lane, i0, i1);
   float cmax = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : peak_value(m[i0 * N1 + i1], norm);
   if (code != PIV_WIN_OK && code != PIV_WIN_NO_PEAK) {
     // second chance (correl.py:231-246)
-    s.add_square(i0, i1, pa.particle_radius);
-    sq[0] = s.sy0[0];
-    sq[1] = s.sy1[0];
-    sq[2] = s.sx0[0];
-    sq[3] = s.sx1[0];
+    s.nan_square(lane, i0, i1, pa.particle_radius, sq);
     int j0, j1;
-    int code2 = s.find(lane, j0, j1);
+    const int code2 = s.find(lane, j0, j1);
     if (code2 == PIV_WIN_OK) {
       i0 = j0;
       i1 = j1;
@@ -264,8 +256,8 @@ __device__ void finish_window(const float* m, const uint8_t* mask, const PlanArg
   const bool writer = lane == 0;
   if (code != PIV_WIN_OK) {
     if (writer) {
-      float dx = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : float(ix0 - i1);
-      float dy = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : float(iy0 - i0);
+      const float dx = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : float(ix0 - i1);
+      const float dy = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : float(iy0 - i0);
       res.dx[win] = dx + float(dx_in);
       res.dy[win] = dy + float(dy_in);
       res.correl_max[win] = cmax;
@@ -281,20 +273,20 @@ __device__ void finish_window(const float* m, const uint8_t* mask, const PlanArg
   if (pa.nb_peaks > 1) {
     int c0 = i0, c1 = i1;
     for (int k = 0; k < pa.nb_peaks - 1; ++k) {
-      s.add_square(c0, c1, pa.particle_radius);
+      s.nan_square(lane, c0, c1, pa.particle_radius, nullptr);
       int j0, j1;
-      int c = s.find(lane, j0, j1);
+      const int c = s.find(lane, j0, j1);
       if (c != PIV_WIN_OK) break;
       c0 = j0;
       c1 = j1;
       if (writer) {
-        size_t o = (size_t(win) * (pa.nb_peaks - 1) + k);
+        const size_t o = size_t(win) * (pa.nb_peaks - 1) + k;
         res.sec[3 * o + 0] = float(ix0 - j1);
         res.sec[3 * o + 1] = float(iy0 - j0);
         res.sec[3 * o + 2] = peak_value(m[j0 * N1 + j1], norm);
         if (res.sec_sub) {
           double px, py;
-          int e = subpix<N0, N1>(m, pa.subpix_method, pa.nsubpix, j1, j0, px, py);
+          const int e = subpix<N0, N1>(m, pa.subpix_method, pa.nsubpix, j1, j0, px, py);
           res.sec_sub[2 * o + 0] = e ? __builtin_nanf("") : float(ix0 - (j1 + px));
           res.sec_sub[2 * o + 1] = e ? __builtin_nanf("") : float(iy0 - (j0 + py));
         }
@@ -307,7 +299,7 @@ __device__ void finish_window(const float* m, const uint8_t* mask, const PlanArg
     int err = PIV_WIN_OK;
     if (apply_subpix) {
       double px, py;
-      int e = subpix<N0, N1>(m, pa.subpix_method, pa.nsubpix, i1, i0, px, py);
+      const int e = subpix<N0, N1>(m, pa.subpix_method, pa.nsubpix, i1, i0, px, py);
       if (e == PIV_WIN_OK) {
         dx = float(ix0 - (i1 + px));
         dy = float(iy0 - (i0 + py));

@@ -43,6 +43,32 @@ int check_launch(const char* what) {
 // -------------------------------------------------------------------------
 // fused correlation kernel
 // -------------------------------------------------------------------------
+// pixel fetch through a global-address-space pointer (no flat loads)
+template <int DT>
+__device__ __forceinline__ float gfetch(const void* data, int64_t idx) {
+  if constexpr (DT == PIV_U8)
+    return float(((const __attribute__((address_space(1))) uint8_t*)data)[idx]);
+  else if constexpr (DT == PIV_U16)
+    return float(((const __attribute__((address_space(1))) uint16_t*)data)[idx]);
+  else
+    return ((const __attribute__((address_space(1))) float*)data)[idx];
+}
+
+template <int DT>
+struct PixelType;
+template <>
+struct PixelType<PIV_U8> {
+  using type = uint8_t;
+};
+template <>
+struct PixelType<PIV_U16> {
+  using type = uint16_t;
+};
+template <>
+struct PixelType<PIV_F32> {
+  using type = float;
+};
+
 template <int N0, int N1>
 struct Shape {
   static constexpr int T = lanes_for(N0 > N1 ? N0 : N1);
@@ -67,13 +93,20 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   const int apply_subpix = options & PIV_OPT_SUBPIX;
   constexpr int NN = N0 * N1;
   __shared__ float2 smem[WPB * NN];
-  __shared__ uint8_t smask[NN];
+  __shared__ uint64_t scol[N1];  // bit i0 of scol[i1]: where_large_displacement[i0, i1]
 
   const int lane = threadIdx.x & (T - 1);
   const int wl = threadIdx.x / T;
   const int win = blockIdx.x * WPB + wl;
-  if (pa.mask != nullptr)
-    for (int k = threadIdx.x; k < NN; k += T * WPB) smask[k] = pa.mask[k];
+  static_assert(N0 <= 64, "column masks hold up to 64 rows");
+  if (pa.mask != nullptr) {
+    for (int c = threadIdx.x; c < N1; c += T * WPB) {
+      uint64_t bits = 0;
+      for (int i0 = 0; i0 < N0; ++i0)
+        if (pa.mask[i0 * N1 + c]) bits |= uint64_t(1) << i0;
+      scol[c] = bits;
+    }
+  }
 
   float2* buf = smem + wl * NN;
   piv_window w;
@@ -94,20 +127,42 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
     const piv_image im1 = images[w.image1];
     const float m0 = im0.minval ? *im0.minval : 0.f;
     const float m1 = im1.minval ? *im1.minval : 0.f;
-    const int xa = w.x0 + lane, xb = w.x1 + lane;
-    const bool ina = xa >= 0 && xa < im0.width;
-    const bool inb = xb >= 0 && xb < im1.width;
+    const bool inside = w.y0 >= 0 && w.y0 + N0 <= im0.height && w.x0 >= 0 &&
+                        w.x0 + N1 <= im0.width && w.y1 >= 0 && w.y1 + N0 <= im1.height &&
+                        w.x1 >= 0 && w.x1 + N1 <= im1.width;
+    if (inside) {
+      // interior window: no padding, one pointer walk per image
+      using E = typename PixelType<DT>::type;
+      const __attribute__((address_space(1))) E* pa0 =
+          (const __attribute__((address_space(1))) E*)im0.data + int64_t(w.y0) * im0.pitch +
+          w.x0 + lane;
+      const __attribute__((address_space(1))) E* pb0 =
+          (const __attribute__((address_space(1))) E*)im1.data + int64_t(w.y1) * im1.pitch +
+          w.x1 + lane;
 #pragma unroll
-    for (int r = 0; r < N0; ++r) {
-      const int ya = w.y0 + r, yb = w.y1 + r;
-      float a = 0.f, b = 0.f;
-      if (ina && ya >= 0 && ya < im0.height)
-        a = fetch_px<DT>(im0.data, int64_t(ya) * im0.pitch + xa) - m0;
-      if (inb && yb >= 0 && yb < im1.height)
-        b = fetch_px<DT>(im1.data, int64_t(yb) * im1.pitch + xb) - m1;
-      z[r] = make_float2(a, b);
-      s0 += a;
-      s1 += b;
+      for (int r = 0; r < N0; ++r) {
+        const float a = float(pa0[int64_t(r) * im0.pitch]) - m0;
+        const float b = float(pb0[int64_t(r) * im1.pitch]) - m1;
+        z[r] = make_float2(a, b);
+        s0 += a;
+        s1 += b;
+      }
+    } else {
+      const int xa = w.x0 + lane, xb = w.x1 + lane;
+      const bool ina = xa >= 0 && xa < im0.width;
+      const bool inb = xb >= 0 && xb < im1.width;
+#pragma unroll
+      for (int r = 0; r < N0; ++r) {
+        const int ya = w.y0 + r, yb = w.y1 + r;
+        float a = 0.f, b = 0.f;
+        if (ina && ya >= 0 && ya < im0.height)
+          a = gfetch<DT>(im0.data, int64_t(ya) * im0.pitch + xa) - m0;
+        if (inb && yb >= 0 && yb < im1.height)
+          b = gfetch<DT>(im1.data, int64_t(yb) * im1.pitch + xb) - m1;
+        z[r] = make_float2(a, b);
+        s0 += a;
+        s1 += b;
+      }
     }
   } else {
 #pragma unroll
@@ -148,7 +203,7 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   }
   __syncthreads();
   // ---- 4. F0, F1 from Z(k), Z(-k); energies; cross spectrum ---------------
-  double e0 = 0.0, e1 = 0.0;
+  float e0f = 0.f, e1f = 0.f;
   if (r < N0) {
     const int pr = r == 0 ? 0 : N0 - r;
 #pragma unroll
@@ -165,16 +220,16 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
         f1 = make_float2(0.f, 0.f);
       }
       if (c <= N1 / 2) {
-        const double wgt = (c == 0 || c == N1 / 2) ? 1.0 : 2.0;
-        e0 += wgt * (double(f0.x) * f0.x + double(f0.y) * f0.y);
-        e1 += wgt * (double(f1.x) * f1.x + double(f1.y) * f1.y);
+        const float wgt = (c == 0 || c == N1 / 2) ? 1.f : 2.f;
+        e0f += wgt * (f0.x * f0.x + f0.y * f0.y);
+        e1f += wgt * (f1.x * f1.x + f1.y * f1.y);
       }
       // conj(f0) * f1
       y[c] = make_float2(f0.x * f1.x + f0.y * f1.y, f0.x * f1.y - f0.y * f1.x);
     }
   }
-  e0 = group_sum(e0, T);
-  e1 = group_sum(e1, T);
+  const double e0 = group_sum(double(e0f), T);
+  const double e1 = group_sum(double(e1f), T);
   const double norm = sqrt(e0 * e1);
   // ---- 5. inverse 2-D FFT (rows, then columns) ------------------------
   fft<N1, +1>(y);
@@ -193,12 +248,16 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   // ---- 6. map in the reference layout: arr[a0, a1] -> out[g(a0), g(a1)],
   //         g(a) = (n - 1 - a + n // 2) mod n  (inverse of _like_fftshift)
   float* mapf = reinterpret_cast<float*>(buf);
+  float* srch = mapf + NN;  // search copy: masked elements NaN
   if (lane < N1) {
     const int g1 = (N1 - 1 - lane + N1 / 2) % N1;
+    const uint64_t bits = pa.mask != nullptr ? scol[g1] : uint64_t(0);
 #pragma unroll
     for (int a0 = 0; a0 < N0; ++a0) {
       const int g0 = (N0 - 1 - a0 + N0 / 2) % N0;
-      mapf[g0 * N1 + g1] = z[a0].x;
+      const float v = z[a0].x;
+      mapf[g0 * N1 + g1] = v;
+      srch[g0 * N1 + g1] = ((bits >> g0) & 1) ? __builtin_nanf("") : v;
     }
   }
   __syncthreads();
@@ -206,8 +265,8 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   int sq[4] = {0, 0, 0, 0};
   if (compute) {
     if (lane == 0 && res.norm) res.norm[win] = norm;
-    finish_window<N0, N1, T>(mapf, pa.mask ? smask : nullptr, pa, lane, win, norm,
-                             apply_subpix, w.dx_in, w.dy_in, res, sq);
+    finish_window<N0, N1, T>(mapf, srch, pa, lane, win, norm, apply_subpix, w.dx_in, w.dy_in,
+                             res, sq);
   } else if (live && lane == 0) {
     const float nan = __builtin_nanf("");
     res.dx[win] = nan;
@@ -239,23 +298,24 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_peaks(
                                                                  piv_results res, PlanArgs pa) {
   constexpr int T = Shape<N0, N1>::T;
   constexpr int NN = N0 * N1;
-  __shared__ float smem[WPB * NN];
-  __shared__ uint8_t smask[NN];
+  __shared__ float smem[WPB * 2 * NN];
   const int lane = threadIdx.x & (T - 1);
   const int wl = threadIdx.x / T;
   const int win = blockIdx.x * WPB + wl;
-  if (pa.mask != nullptr)
-    for (int k = threadIdx.x; k < NN; k += T * WPB) smask[k] = pa.mask[k];
-  float* m = smem + wl * NN;
+  float* m = smem + wl * 2 * NN;
+  float* srch = m + NN;
   const bool live = win < nwin;
   if (live)
-    for (int k = lane; k < NN; k += T) m[k] = maps[size_t(win) * NN + k];
+    for (int k = lane; k < NN; k += T) {
+      const float v = maps[size_t(win) * NN + k];
+      m[k] = v;
+      srch[k] = (pa.mask != nullptr && pa.mask[k]) ? __builtin_nanf("") : v;
+    }
   __syncthreads();
   if (!live) return;
   int sq[4];
   const double norm = norms[win];
-  finish_window<N0, N1, T>(m, pa.mask ? smask : nullptr, pa, lane, win, norm, apply_subpix, 0, 0,
-                           res, sq);
+  finish_window<N0, N1, T>(m, srch, pa, lane, win, norm, apply_subpix, 0, 0, res, sq);
   if (!pa.has_dmax) {
     for (int k = lane; k < NN; k += T) {
       const int i0 = k / N1, i1 = k % N1;
@@ -364,27 +424,30 @@ __global__ void k_min_init(uint32_t* keys, int n) {
   if (i < n) keys[i] = 0xffffffffu;
 }
 
+template <int DT>
+__device__ __forceinline__ float row_min(const piv_image& im, int y, float m) {
+  const int64_t base = int64_t(y) * im.pitch;
+  for (int x = threadIdx.x; x < im.width; x += blockDim.x)
+    m = fminf(m, gfetch<DT>(im.data, base + x));
+  return m;
+}
+
+// rows blockIdx.x, blockIdx.x + gridDim.x, ... of image blockIdx.y
 __global__ __launch_bounds__(256) void k_min_reduce(const piv_image* __restrict__ images,
                                                     uint32_t* keys) {
   const piv_image im = images[blockIdx.y];
-  const int64_t npx = int64_t(im.height) * im.width;
   float m = INFINITY;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < npx;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t y = i / im.width, x = i - y * im.width;
-    const int64_t idx = y * im.pitch + x;
-    float v;
-    if (im.dtype == PIV_U8) v = fetch_px<PIV_U8>(im.data, idx);
-    else if (im.dtype == PIV_U16) v = fetch_px<PIV_U16>(im.data, idx);
-    else v = fetch_px<PIV_F32>(im.data, idx);
-    m = fminf(m, v);
+  for (int y = blockIdx.x; y < im.height; y += gridDim.x) {
+    if (im.dtype == PIV_U8) m = row_min<PIV_U8>(im, y, m);
+    else if (im.dtype == PIV_U16) m = row_min<PIV_U16>(im, y, m);
+    else m = row_min<PIV_F32>(im, y, m);
   }
   for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 64));
   __shared__ float part[4];
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float b = fminf(fminf(part[0], part[1]), fminf(part[2], part[3]));
+    const float b = fminf(fminf(part[0], part[1]), fminf(part[2], part[3]));
     atomicMin(&keys[blockIdx.y], float_key(b));
   }
 }
@@ -603,7 +666,7 @@ int piv_plan_create(piv_plan** plan, int32_t n0, int32_t n1, double displacement
     return fail(PIV_EINVAL, "bad subpix method");
   if (nsubpix < 1 || 2 * nsubpix + 1 > (n0 < n1 ? n0 : n1))
     return fail(PIV_EINVAL, "bad nsubpix");
-  if (nb_peaks_to_search < 1 || nb_peaks_to_search > kMaxSquares - 1)
+  if (nb_peaks_to_search < 1 || nb_peaks_to_search > kMaxPeaks)
     return fail(PIV_EUNSUPPORTED, "nb_peaks_to_search must be in [1, 7]");
   if (particle_radius < 0) return fail(PIV_EINVAL, "bad particle_radius");
   piv_plan* p = new piv_plan();

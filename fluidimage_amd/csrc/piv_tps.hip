@@ -565,19 +565,224 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
 }
 
 // ---------------------------------------------------------------------------
-// 4. blend of the smoothed values, 5. evaluation at new positions
+// 4. geometry: selection of the finite vectors, subdomain membership
+//    (thin_plate_spline_subdom.py:53-126,161-222), with the subdomain grid
+//    limits computed on the host from five scalars
 // ---------------------------------------------------------------------------
-__global__ void k_tps_blend(int npts, const int32_t* __restrict__ c_slots,
-                            const int64_t* __restrict__ c_ptr, const double* __restrict__ coef,
-                            const double* __restrict__ norm, const double* __restrict__ sm,
-                            int64_t nslots, double* __restrict__ out) {
+constexpr int kMaxSlots = 4;  // subdomains per point (20 % buffers: <= 2 x 2)
+
+// inclusive prefix sum over a 1024-thread block
+__device__ int block_scan_1024(int v, int* tmp, int& total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  if (lane == 63) tmp[wv] = v;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    int x = tmp[threadIdx.x];
+    for (int off = 1; off < 16; off <<= 1) {
+      const int u = __shfl_up(x, off, 16);
+      if (threadIdx.x >= off) x += u;
+    }
+    tmp[16 + threadIdx.x] = x;
+  }
+  __syncthreads();
+  total = tmp[16 + 15];
+  const int add = wv > 0 ? tmp[16 + wv - 1] : 0;
+  __syncthreads();
+  return v + add;
+}
+
+// one block of 1024 threads: ordered compaction of the vectors whose dx and
+// dy are finite (singlepass.py:450-463) and the bounds of their positions
+__global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restrict__ dx,
+                                                     const float* __restrict__ dy,
+                                                     const double* __restrict__ xs,
+                                                     const double* __restrict__ ys,
+                                                     double* cx, double* cy, float* vx,
+                                                     float* vy, int32_t* idx, double* info) {
+  __shared__ int tmp[32];
+  __shared__ double red[4][16];
+  const int per = (n + 1023) / 1024;
+  const int i0 = threadIdx.x * per, i1 = min(n, i0 + per);
+  int cnt = 0;
+  double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+  for (int i = i0; i < i1; ++i) {
+    const float a = dx[i], b = dy[i];
+    if (a == a && b == b) {
+      ++cnt;
+      xmin = fmin(xmin, xs[i]);
+      xmax = fmax(xmax, xs[i]);
+      ymin = fmin(ymin, ys[i]);
+      ymax = fmax(ymax, ys[i]);
+    }
+  }
+  int total;
+  int pos = block_scan_1024(cnt, tmp, total) - cnt;
+  for (int i = i0; i < i1; ++i) {
+    const float a = dx[i], b = dy[i];
+    if (a == a && b == b) {
+      cx[pos] = xs[i];
+      cy[pos] = ys[i];
+      vx[pos] = a;
+      vy[pos] = b;
+      idx[pos] = i;
+      ++pos;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    xmin = fmin(xmin, __shfl_xor(xmin, off, 64));
+    xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
+    ymin = fmin(ymin, __shfl_xor(ymin, off, 64));
+    ymax = fmax(ymax, __shfl_xor(ymax, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = xmin;
+    red[1][threadIdx.x >> 6] = xmax;
+    red[2][threadIdx.x >> 6] = ymin;
+    red[3][threadIdx.x >> 6] = ymax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 16; ++k) {
+      red[0][0] = fmin(red[0][0], red[0][k]);
+      red[1][0] = fmax(red[1][0], red[1][k]);
+      red[2][0] = fmin(red[2][0], red[2][k]);
+      red[3][0] = fmax(red[3][0], red[3][k]);
+    }
+    info[0] = total;
+    info[1] = red[0][0];
+    info[2] = red[1][0];
+    info[3] = red[2][0];
+    info[4] = red[3][0];
+  }
+}
+
+struct SubGrid {
+  int nsx, nsy;
+  const double* lim;  // lmin_x[nsx], lmax_x[nsx], lmin_y[nsy], lmax_y[nsy]
+  __device__ double xlo(int i) const { return lim[i]; }
+  __device__ double xhi(int i) const { return lim[nsx + i]; }
+  __device__ double ylo(int i) const { return lim[2 * nsx + i]; }
+  __device__ double yhi(int i) const { return lim[2 * nsx + nsy + i]; }
+  __device__ bool inside(int d, double x, double y) const {
+    const int ix = d % nsx, iy = d / nsx;
+    return x >= xlo(ix) && x < xhi(ix) && y >= ylo(iy) && y < yhi(iy);
+  }
+  // partition-of-unity weight of subdomain d (thin_plate_spline_subdom.py:208-222)
+  __device__ double coef(int d, double x, double y) const {
+    const int ix = d % nsx, iy = d / nsx;
+    const double xc = 0.5 * (xlo(ix) + xhi(ix)), yc = 0.5 * (ylo(iy) + yhi(iy));
+    const double dxm = xhi(ix) - xc, dym = yhi(iy) - yc;
+    const double dx2 = (x - xc) * (x - xc) / (dxm * dxm);
+    const double dy2 = (y - yc) * (y - yc) / (dym * dym);
+    return (1 - dx2) * (1 - dy2) + 0.001;
+  }
+};
+
+// members of subdomain blockIdx.x: count
+__global__ __launch_bounds__(256) void k_tps_count(SubGrid g, int n, const double* __restrict__ cx,
+                                                   const double* __restrict__ cy,
+                                                   int32_t* counts) {
+  const int d = blockIdx.x;
+  int c = 0;
+  for (int i = threadIdx.x; i < n; i += 256) c += g.inside(d, cx[i], cy[i]) ? 1 : 0;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  __shared__ int part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) counts[d] = part[0] + part[1] + part[2] + part[3];
+}
+
+// members of subdomain blockIdx.x in increasing center order (np.where)
+__global__ __launch_bounds__(1024) void k_tps_fill(SubGrid g, int n, const double* __restrict__ cx,
+                                                   const double* __restrict__ cy,
+                                                   const int64_t* __restrict__ dom_off,
+                                                   int32_t* slot_pt) {
+  __shared__ int tmp[32];
+  const int d = blockIdx.x;
+  int64_t base = dom_off[d];
+  for (int c0 = 0; c0 < n; c0 += 1024) {
+    const int i = c0 + threadIdx.x;
+    const int f = (i < n && g.inside(d, cx[i], cy[i])) ? 1 : 0;
+    int total;
+    const int incl = block_scan_1024(f, tmp, total);
+    if (f) slot_pt[base + incl - 1] = i;
+    base += total;
+  }
+}
+
+// per point: its subdomains in increasing order (at most kMaxSlots), their
+// weights and the sum of the weights. Centers mode (dom_off != NULL): the
+// slot is the position of the point in the member list of the subdomain.
+__global__ void k_tps_point_slots(SubGrid g, int npts, const double* __restrict__ px,
+                                  const double* __restrict__ py,
+                                  const int64_t* __restrict__ dom_off,
+                                  const int32_t* __restrict__ slot_pt, int32_t* out_slot,
+                                  double* out_coef, double* out_norm, int32_t* status) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  const double x = px[p], y = py[p];
+  int xs_[kMaxSlots], ys_[kMaxSlots];
+  int nx = 0, ny = 0;
+  for (int i = 0; i < g.nsx; ++i)
+    if (x >= g.xlo(i) && x < g.xhi(i)) {
+      if (nx < kMaxSlots) xs_[nx] = i;
+      ++nx;
+    }
+  for (int i = 0; i < g.nsy; ++i)
+    if (y >= g.ylo(i) && y < g.yhi(i)) {
+      if (ny < kMaxSlots) ys_[ny] = i;
+      ++ny;
+    }
+  int k = 0;
+  double norm = 0.0;
+  if (nx * ny > kMaxSlots) status[0] = 1;
+  for (int a = 0; a < min(ny, kMaxSlots); ++a)
+    for (int b = 0; b < min(nx, kMaxSlots); ++b) {
+      if (k >= kMaxSlots) continue;
+      const int d = ys_[a] * g.nsx + xs_[b];
+      const double cf = g.coef(d, x, y);
+      int slot = d;
+      if (dom_off != nullptr) {
+        int64_t lo = dom_off[d], hi = dom_off[d + 1];
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) / 2;
+          if (slot_pt[mid] < p) lo = mid + 1;
+          else hi = mid;
+        }
+        slot = int(lo);
+      }
+      out_slot[kMaxSlots * p + k] = slot;
+      out_coef[kMaxSlots * p + k] = cf;
+      norm += cf;
+      ++k;
+    }
+  for (; k < kMaxSlots; ++k) {
+    out_slot[kMaxSlots * p + k] = -1;
+    out_coef[kMaxSlots * p + k] = 0.0;
+  }
+  out_norm[p] = norm;
+}
+
+// ---------------------------------------------------------------------------
+// 5. blend of the smoothed values, 6. evaluation at new positions
+// ---------------------------------------------------------------------------
+__global__ void k_tps_blend(int npts, const int32_t* __restrict__ slots,
+                            const double* __restrict__ coef, const double* __restrict__ norm,
+                            const double* __restrict__ sm, int64_t nslots,
+                            double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npts) return;
   double a0 = 0.0, a1 = 0.0;
-  for (int64_t k = c_ptr[i]; k < c_ptr[i + 1]; ++k) {
-    const int s = c_slots[k];
-    a0 += coef[s] * sm[s];
-    a1 += coef[s] * sm[nslots + s];
+  for (int k = 0; k < kMaxSlots; ++k) {
+    const int s = slots[kMaxSlots * i + k];
+    if (s < 0) break;
+    const double c = coef[kMaxSlots * i + k];
+    a0 += c * sm[s];
+    a1 += c * sm[nslots + s];
   }
   out[i] = a0 / norm[i];
   out[npts + i] = a1 / norm[i];
@@ -585,21 +790,21 @@ __global__ void k_tps_blend(int npts, const int32_t* __restrict__ c_slots,
 
 __global__ __launch_bounds__(256) void k_tps_interp(
     int npts, const double* __restrict__ px, const double* __restrict__ py,
-    const int32_t* __restrict__ n_slots, const int64_t* __restrict__ n_ptr,
-    const int32_t* __restrict__ n_dom, const double* __restrict__ n_coef,
-    const double* __restrict__ xs, const double* __restrict__ ys,
-    const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
-    const double* __restrict__ weights, int64_t wstride, double* __restrict__ out) {
+    const int32_t* __restrict__ doms, const double* __restrict__ coef,
+    const double* __restrict__ norm, const double* __restrict__ cx,
+    const double* __restrict__ cy, const int64_t* __restrict__ dom_off,
+    const int32_t* __restrict__ slot_pt, const double* __restrict__ weights, int64_t wstride,
+    double* __restrict__ out) {
   // one wavefront per new position
   const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (p >= npts) return;
   const double x = px[p], y = py[p];
-  double acc0 = 0.0, acc1 = 0.0, nrm = 0.0;
-  for (int64_t k = n_ptr[p]; k < n_ptr[p + 1]; ++k) {
-    const int s = n_slots[k];
-    const int d = n_dom[s];
-    const double cf = n_coef[s];
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int k = 0; k < kMaxSlots; ++k) {
+    const int d = doms[kMaxSlots * p + k];
+    if (d < 0) break;
+    const double cf = coef[kMaxSlots * p + k];
     const int64_t s0 = dom_off[d];
     const int n = int(dom_off[d + 1] - s0);
     const double* W0 = weights + s0 + 3 * int64_t(d);
@@ -607,7 +812,7 @@ __global__ __launch_bounds__(256) void k_tps_interp(
     double v0 = 0.0, v1 = 0.0;
     for (int j = lane; j < n; j += 64) {
       const int c = slot_pt[s0 + j];
-      const double dy = y - ys[c], dx = x - xs[c];
+      const double dy = y - cy[c], dx = x - cx[c];
       const double ph = tps_phi(dy * dy + dx * dx);
       v0 += W0[j] * ph;
       v1 += W1[j] * ph;
@@ -620,11 +825,10 @@ __global__ __launch_bounds__(256) void k_tps_interp(
     v1 += W1[n] + W1[n + 1] * y + W1[n + 2] * x;
     acc0 += cf * v0;
     acc1 += cf * v1;
-    nrm += cf;
   }
   if (lane == 0) {
-    out[p] = acc0 / nrm;
-    out[npts + p] = acc1 / nrm;
+    out[p] = acc0 / norm[p];
+    out[npts + p] = acc1 / norm[p];
   }
 }
 
@@ -666,25 +870,75 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
   return tps_check("piv_tps_smooth");
 }
 
-int piv_tps_blend(int32_t npts, const int32_t* c_slots, const int64_t* c_ptr,
-                  const double* coef, const double* norm, const double* smooth_slots,
-                  int64_t nslots, double* out, void* stream) {
+int piv_tps_select(int32_t n, const float* dx, const float* dy, const double* xs,
+                   const double* ys, double* cx, double* cy, float* vx, float* vy,
+                   int32_t* idx, double* info, void* stream) {
+  if (n <= 0) return tps_fail(PIV_EINVAL, "no vector");
+  if (!dx || !dy || !xs || !ys || !cx || !cy || !vx || !vy || !idx || !info)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_tps_select, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), n,
+                     dx, dy, xs, ys, cx, cy, vx, vy, idx, info);
+  return tps_check("piv_tps_select");
+}
+
+int piv_tps_count(int32_t nsx, int32_t nsy, const double* lim, int32_t n, const double* cx,
+                  const double* cy, int32_t* counts, void* stream) {
+  if (nsx <= 0 || nsy <= 0) return tps_fail(PIV_EINVAL, "empty subdomain grid");
+  if (!lim || !cx || !cy || !counts) return tps_fail(PIV_EINVAL, "NULL argument");
+  SubGrid g{nsx, nsy, lim};
+  hipLaunchKernelGGL(k_tps_count, dim3(nsx * nsy), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), g, n, cx, cy, counts);
+  return tps_check("piv_tps_count");
+}
+
+int piv_tps_fill(int32_t nsx, int32_t nsy, const double* lim, int32_t n, const double* cx,
+                 const double* cy, const int64_t* dom_off, int32_t* slot_pt, void* stream) {
+  if (nsx <= 0 || nsy <= 0) return tps_fail(PIV_EINVAL, "empty subdomain grid");
+  if (!lim || !cx || !cy || !dom_off || !slot_pt) return tps_fail(PIV_EINVAL, "NULL argument");
+  SubGrid g{nsx, nsy, lim};
+  hipLaunchKernelGGL(k_tps_fill, dim3(nsx * nsy), dim3(1024), 0,
+                     static_cast<hipStream_t>(stream), g, n, cx, cy, dom_off, slot_pt);
+  return tps_check("piv_tps_fill");
+}
+
+int piv_tps_point_slots(int32_t nsx, int32_t nsy, const double* lim, int32_t npts,
+                        const double* px, const double* py, const int64_t* dom_off,
+                        const int32_t* slot_pt, int32_t* out_slot, double* out_coef,
+                        double* out_norm, int32_t* status, void* stream) {
   if (npts <= 0) return PIV_OK;
+  if (!lim || !px || !py || !out_slot || !out_coef || !out_norm || !status)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  if ((dom_off == nullptr) != (slot_pt == nullptr))
+    return tps_fail(PIV_EINVAL, "dom_off and slot_pt go together");
+  SubGrid g{nsx, nsy, lim};
+  hipLaunchKernelGGL(k_tps_point_slots, dim3((npts + 255) / 256), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), g, npts, px, py, dom_off, slot_pt,
+                     out_slot, out_coef, out_norm, status);
+  return tps_check("piv_tps_point_slots");
+}
+
+int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef, const double* norm,
+                  const double* smooth_slots, int64_t nslots, double* out, void* stream) {
+  if (npts <= 0) return PIV_OK;
+  if (!slots || !coef || !norm || !smooth_slots || !out)
+    return tps_fail(PIV_EINVAL, "NULL argument");
   hipLaunchKernelGGL(k_tps_blend, dim3((npts + 255) / 256), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), npts, c_slots, c_ptr, coef, norm,
-                     smooth_slots, nslots, out);
+                     static_cast<hipStream_t>(stream), npts, slots, coef, norm, smooth_slots,
+                     nslots, out);
   return tps_check("piv_tps_blend");
 }
 
-int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
-                        const int32_t* n_slots, const int64_t* n_ptr, const int32_t* n_dom,
-                        const double* n_coef, const double* xs, const double* ys,
-                        const int64_t* dom_off, const int32_t* slot_pt, const double* weights,
-                        int64_t wstride, double* out, void* stream) {
+int piv_tps_interpolate(int32_t npts, const double* px, const double* py, const int32_t* doms,
+                        const double* coef, const double* norm, const double* cx,
+                        const double* cy, const int64_t* dom_off, const int32_t* slot_pt,
+                        const double* weights, int64_t wstride, double* out, void* stream) {
   if (npts <= 0) return PIV_OK;
+  if (!px || !py || !doms || !coef || !norm || !cx || !cy || !dom_off || !slot_pt || !weights ||
+      !out)
+    return tps_fail(PIV_EINVAL, "NULL argument");
   hipLaunchKernelGGL(k_tps_interp, dim3((npts + 3) / 4), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), npts, px, py, n_slots, n_ptr, n_dom,
-                     n_coef, xs, ys, dom_off, slot_pt, weights, wstride, out);
+                     static_cast<hipStream_t>(stream), npts, px, py, doms, coef, norm, cx, cy,
+                     dom_off, slot_pt, weights, wstride, out);
   return tps_check("piv_tps_interpolate");
 }
 

@@ -82,8 +82,35 @@ class WorkPIV:
             self.works_piv.append(work_piv)
             self.works_fix.append(WorkFIX(params.fix, work_piv))
 
+    keep_maps = True  # results carry every correlation map, like the reference
+
     def calcul(self, couple):
-        """multipass.py:172-190"""
+        """multipass.py:172-190, all passes on the GPU (pipeline.py)."""
+        from ...data_objects import ArrayCouple
+        from .pipeline import run_pairs
+
+        if isinstance(couple, dict):
+            couple = ArrayCouple(**couple)
+        elif isinstance(couple, (tuple, list)):
+            couple = ArrayCouple(arrays=couple)
+        if not isinstance(couple, ArrayCouple):
+            raise ValueError(f"not isinstance(couple, ArrayCouple), {type(couple) = }")
+        couple.apply_mask(self.params.mask)
+        im0, im1 = couple.get_arrays()
+        pipe, passes = run_pairs(self, [(im0, im1)], keep_maps=self.keep_maps)
+        return pipe.to_host(passes, 0, couple=couple)
+
+    def calcul_batch(self, pairs, keep_maps=False):
+        """Several same-shape pairs in one pipeline: every pass is one
+        correlation launch over the windows of all pairs."""
+        from .pipeline import run_pairs
+
+        pipe, passes = run_pairs(self, list(pairs), keep_maps=keep_maps)
+        return [pipe.to_host(passes, k) for k in range(len(pairs))]
+
+    def calcul_passes_per_work(self, couple):
+        """The reference's pass-by-pass chain (FirstWorkPIV, WorkFIX,
+        WorkPIVFromDisplacement ... with host results between passes)."""
         results = MultipassPIVResults()
         piv_result = couple
         for work_piv, work_fix in zip(self.works_piv, self.works_fix):

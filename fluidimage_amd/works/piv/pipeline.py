@@ -1,0 +1,292 @@
+"""Device-resident multipass PIV over a batch of image pairs.
+
+One pass = one launch of ``piv_make_windows`` per pair, ONE launch of the
+fused correlation kernel over the windows of every pair, ``piv_fix`` per
+pair, and the interpolation of the fixed field on the next grid
+(GPU FP64 TPS, or scipy's griddata exactly as the reference when
+``use_tps`` selects it). The semantics are those of
+src/fluidimage/works/piv/multipass.py:172-190 applied to each pair; results
+stay on the device until ``to_host``.
+"""
+
+import numpy as np
+
+from ... import _native
+from ...data_objects import HeavyPIVResults, MultipassPIVResults
+from ...device import ImageSet, to_device_image
+from .fix import run_fix_device
+
+
+class PassOutput:
+    """Device tensors of one pass for a batch (pair-major slices)."""
+
+    def __init__(self, work, npairs, out, win, dev):
+        self.work = work
+        self.npairs = npairs
+        self.nvec = work.nb_vectors
+        self.out = out
+        self.win = win
+        self.dev = dev
+        self.approx = [None] * npairs  # predictor on the next grid, per pair
+        self.final = [None] * npairs
+        self.tps = [None] * npairs
+
+    def pair(self, key, k):
+        t = self.out[key] if key in self.out else self.win[key]
+        return t[k * self.nvec:(k + 1) * self.nvec]
+
+
+class MultipassPipeline:
+    """Runs the passes of a ``WorkPIV`` on batches of same-shape pairs."""
+
+    def __init__(self, work_piv):
+        self.work = work_piv
+        self.params = work_piv.params
+
+    def _event_pair(self, timers, key):
+        import torch
+
+        if timers is None:
+            return None
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        timers.setdefault(key, []).append(ev)
+        return ev
+
+    def _interp(self, work_next, res, k, last, stream=None, timers=None):
+        """apply_interp (singlepass.py:409-563) of pair k's fixed field onto
+        work_next's grid, returning float64 device tensors (x, y)."""
+        import torch
+
+        from ...calcul.tps import SingularSystemError, tps_device
+        from .interp import griddata
+
+        mp = self.params.multipass
+        dx = res.pair("dx", k)
+        dy = res.pair("dy", k)
+        xs = res.pair("xs", k)
+        ys = res.pair("ys", k)
+        dev = dx.device
+        gx, gy = work_next._device_grid_full(dev)
+        use_tps = mp.use_tps is True or (mp.use_tps == "last" and last)
+        if use_tps:
+            ev = self._event_pair(timers, f"tps_pass{work_next.index_pass}" + ("_final" if last else ""))
+            if ev:
+                ev[0].record()
+            try:
+                t = tps_device(dx, dy, xs, ys, gx, gy, mp.subdom_size, mp.smoothing_coef,
+                               mp.threshold_tps, 20, stream=stream)
+                if ev:
+                    ev[1].record()
+            except SingularSystemError as e:
+                if "Only nan" in str(e):
+                    raise
+                print("LinAlgError in TPS: compute delta_approx with griddata")
+            else:
+                res.tps[k] = t
+                return t["approx"][0], t["approx"][1]
+        hdx = dx.cpu().numpy()
+        hdy = dy.cpu().numpy()
+        sel = ~(np.isnan(hdx) | np.isnan(hdy))
+        if not sel.any():
+            raise ValueError("Only nan.")
+        hx = xs.cpu().numpy()[sel]
+        hy = ys.cpu().numpy()[sel]
+        centers = np.vstack([hy, hx])
+        ax = griddata(centers[::-1], hdx[sel], (work_next.ixvecs, work_next.iyvecs))
+        ay = griddata(centers[::-1], hdy[sel], (work_next.ixvecs, work_next.iyvecs))
+        return (torch.from_numpy(np.ascontiguousarray(ax)).to(dev),
+                torch.from_numpy(np.ascontiguousarray(ay)).to(dev))
+
+    def run(self, imageset, npairs, keep_maps=False, stream=None, final_interp=True,
+            timers=None):
+        """All passes for pairs (2k, 2k+1) of ``imageset``; returns the list
+        of PassOutput. ``timers`` (dict) collects event pairs around the
+        correlation launches and the TPS calls."""
+        import torch
+
+        works = self.work.works_piv
+        dev = imageset.device
+        passes = []
+        approx = [None] * npairs
+        for ip, work in enumerate(works):
+            nvec = work.nb_vectors
+            windows = torch.empty((npairs * nvec, 10), dtype=torch.int32, device=dev)
+            win = {k: [] for k in ("xs", "ys", "dx_in", "dy_in")}
+            for k in range(npairs):
+                dxa, dya = (None, None) if approx[k] is None else approx[k]
+                w = work.make_windows(2 * k, 2 * k + 1, dxa, dya,
+                                      windows=windows[k * nvec:(k + 1) * nvec], device=dev,
+                                      stream=stream)
+                for key in win:
+                    win[key].append(w[key])
+            win = {key: torch.cat(v) for key, v in win.items()}
+            maps = None
+            if keep_maps:
+                maps = torch.empty((npairs * nvec,) + work.shape_crop_im0, dtype=torch.float32,
+                                   device=dev)
+            ev = self._event_pair(timers, f"correlate_pass{ip}")
+            if ev:
+                ev[0].record()
+            out = work.correl.correlate_windows(imageset.table, windows, npairs * nvec,
+                                                imageset.dtype, work.has_to_apply_subpix,
+                                                maps=maps, stream=stream)
+            if ev:
+                ev[1].record()
+            out["maps"] = maps
+            out["windows"] = windows
+            res = PassOutput(work, npairs, out, win, dev)
+            ny, nx = len(work.iyvecs), len(work.ixvecs)
+            for k in range(npairs):
+                sub = {key: res.pair(key, k) for key in ("dx", "dy", "correl_max", "err")}
+                fx = run_fix_device(sub, ny, nx, self.params.fix, stream=stream)
+                for key in ("flags", "diff2", "mean_x", "mean_y"):
+                    out.setdefault(key, []).append(fx[key])
+            for key in ("flags", "diff2", "mean_x", "mean_y"):
+                out[key] = torch.cat(out[key])
+            if self.params.piv0.nb_peaks_to_search >= 2:
+                self._secondary_peaks(res)
+            passes.append(res)
+            if ip + 1 < len(works):
+                approx = [self._interp(works[ip + 1], res, k, last=False, stream=stream,
+                                       timers=timers)
+                          for k in range(npairs)]
+                for k in range(npairs):
+                    res.approx[k] = approx[k]
+        if final_interp:
+            res = passes[-1]
+            for k in range(npairs):
+                try:
+                    res.final[k] = self._interp(works[-1], res, k, last=True, stream=stream,
+                                                timers=timers)
+                except ValueError as e:
+                    print("Warning: InterpError at the end of the last piv pass:", e)
+        return passes
+
+    def _secondary_peaks(self, res):
+        """WorkFIX secondary-peak replacement (fix.py:135-243) on the host,
+        written back to the device fields."""
+        import torch
+
+        from .fix import WorkFIX
+
+        for k in range(res.npairs):
+            r = self._pass_to_heavy(res, k, fixed=True, with_maps=False)
+            h = {key: res.pair(key, k).cpu().numpy()
+                 for key in ("dx", "dy", "flags", "diff2", "mean_x", "mean_y")}
+            fixer = WorkFIX(self.params.fix, res.work)
+            fixer._secondary(r, h, {}, {})
+            replaced = getattr(r, "replaced_vectors", {})
+            if not replaced:
+                continue
+            res.pair("dx", k).copy_(torch.from_numpy(np.asarray(r.deltaxs, np.float32)))
+            res.pair("dy", k).copy_(torch.from_numpy(np.asarray(r.deltays, np.float32)))
+            res.pair("correl_max", k).copy_(
+                torch.from_numpy(np.asarray(r.correls_max, np.float32)))
+            # a replaced vector loses its error entry (fix.py:231-236)
+            cleared = [i for i in replaced if i not in r.errors]
+            if cleared:
+                ids = torch.tensor(cleared, dtype=torch.long, device=res.dev)
+                res.pair("flags", k)[ids] = 0
+                res.pair("err", k)[ids] = 0
+            res.replaced = getattr(res, "replaced", {})
+            res.replaced[k] = replaced
+
+    # ------------------------------------------------------------------ host
+    def _pass_to_heavy(self, res, k, fixed=True, with_maps=True, couple=None):
+        work = res.work
+        g = lambda key: res.pair(key, k).cpu().numpy()  # noqa: E731
+        err = g("err")
+        flags = g("flags") if fixed else None
+        from ...data_objects import error_strings
+
+        errors = error_strings(err, flags, g("diff2") if fixed else None)
+        if res.work.index_pass == 0:
+            xs = work.ixvecs_grid.copy()
+            ys = work.iyvecs_grid.copy()
+        else:
+            xs = g("xs")
+            ys = g("ys")
+        xs, ys = work._xyoriginalimage_from_xymasked(xs, ys)
+        correls = None
+        if with_maps and res.out.get("maps") is not None:
+            correls = list(res.pair("maps", k).cpu().numpy())
+        nb = self.params.piv0.nb_peaks_to_search
+        secondary = [None] * len(err)
+        sec_sub = None
+        if nb > 1:
+            sec = g("sec")
+            nsec = g("nsec")
+            sec_sub = g("sec_sub")
+            for i in range(len(err)):
+                if err[i] in (_native.WIN_OK, _native.WIN_SUBPIX_WRONG,
+                              _native.WIN_SUBPIX_BOUNDARY):
+                    secondary[i] = [(int(a), int(b), np.float32(c))
+                                    for (a, b, c) in sec[i, :nsec[i]]]
+        r = HeavyPIVResults(g("dx"), g("dy"), xs, ys, errors, correls_max=g("correl_max"),
+                            correls=correls, couple=couple, params=self.params,
+                            secondary_peaks=secondary)
+        r._sec_sub = sec_sub
+        r.indices_no_displacement = work.correl.get_indices_no_displacement()
+        r.displacement_max = work.correl.displacement_max
+        if work.index_pass > 0:
+            r.deltaxs_input = g("dx_in")
+            r.deltays_input = g("dy_in")
+        return r
+
+    def to_host(self, passes, k, couple=None):
+        """MultipassPIVResults of pair k (reference attribute names)."""
+        results = MultipassPIVResults()
+        works = self.work.works_piv
+        for ip, res in enumerate(passes):
+            r = self._pass_to_heavy(res, k, couple=couple)
+            if getattr(res, "replaced", None):
+                r.replaced_vectors = res.replaced.get(k, {})
+            work = res.work
+            if ip + 1 < len(passes) and res.approx[k] is not None:
+                nxt = works[ip + 1]
+                r.ixvecs_approx, r.iyvecs_approx = nxt._xyoriginalimage_from_xymasked(
+                    nxt.ixvecs_grid, nxt.iyvecs_grid)
+                r.deltaxs_approx = res.approx[k][0].cpu().numpy()
+                r.deltays_approx = res.approx[k][1].cpu().numpy()
+            if ip + 1 == len(passes) and res.final[k] is not None:
+                r.ixvecs_final, r.iyvecs_final = work._xyoriginalimage_from_xymasked(
+                    work.ixvecs_grid, work.iyvecs_grid)
+                r.deltaxs_final = res.final[k][0].cpu().numpy()
+                r.deltays_final = res.final[k][1].cpu().numpy()
+                t = res.tps[k]
+                if t is not None:
+                    sel = t["index"].cpu().numpy()
+                    sm = t["smoothed"].cpu().numpy()
+                    xs_s = r.ixvecs_final.copy().astype(np.float32)
+                    ys_s = r.iyvecs_final.copy().astype(np.float32)
+                    xs_s[sel] = r.xs[sel]
+                    ys_s[sel] = r.ys[sel]
+                    fx = r.deltaxs_final.copy()
+                    fy = r.deltays_final.copy()
+                    fx[sel] = sm[0]
+                    fy[sel] = sm[1]
+                    r.xs_smooth, r.ys_smooth = xs_s, ys_s
+                    r.deltaxs_smooth, r.deltays_smooth = fx, fy
+            results.append(r)
+        return results
+
+
+def run_pairs(work_piv, pairs, keep_maps=False, stream=None):
+    """WorkPIV on a list of (im0, im1) host or device arrays of one shape."""
+    import torch
+
+    dev = torch.device("cuda")
+    imgs = []
+    for im0, im1 in pairs:
+        imgs += [to_device_image(im0, dev), to_device_image(im1, dev)]
+    shapes = {tuple(t.shape) for t in imgs}
+    if len(shapes) != 1:
+        raise ValueError("all images of a batch must have one shape")
+    shape = shapes.pop()
+    for w in work_piv.works_piv:
+        if getattr(w, "imshape0", None) != shape:
+            w._prepare_with_image(imshape=shape)
+    imageset = ImageSet(imgs, stream=stream)
+    pipe = MultipassPipeline(work_piv)
+    passes = pipe.run(imageset, len(pairs), keep_maps=keep_maps, stream=stream)
+    return pipe, passes

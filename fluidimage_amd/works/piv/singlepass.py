@@ -128,6 +128,7 @@ class BaseWorkPIV(BaseWorkWithMask):
         self.ixvecs_grid = gx.flatten()
         self.iyvecs_grid = gy.flatten()
         self._dev_grid = None
+        self._dev_grid_full = None
 
     def _device_grid(self, device):
         if self._dev_grid is None or self._dev_grid[0].device != device:
@@ -137,6 +138,16 @@ class BaseWorkPIV(BaseWorkWithMask):
                 torch.from_numpy(self.ixvecs.astype(np.int32)).to(device),
             )
         return self._dev_grid
+
+    def _device_grid_full(self, device):
+        """Flattened grid (x, y) as float64 device tensors (TPS targets)."""
+        g = getattr(self, "_dev_grid_full", None)
+        if g is None or g[0].device != device:
+            torch = _torch()
+            g = (torch.from_numpy(self.ixvecs_grid.astype(np.float64)).to(device),
+                 torch.from_numpy(self.iyvecs_grid.astype(np.float64)).to(device))
+            self._dev_grid_full = g
+        return g
 
     @property
     def nb_vectors(self):

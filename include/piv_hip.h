@@ -204,22 +204,52 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    int64_t wstride, double* stats, int32_t* status,
                    void* stream);
 
-/* out[c][i] = sum over the slots k of point i (c_ptr[i]..c_ptr[i+1] into
- * c_slots, in subdomain order) of coef[s] * smooth_slots[c][s], divided by
- * norm[i] (the blend of thin_plate_spline_subdom.py:145-158). */
-int piv_tps_blend(int32_t npts, const int32_t* c_slots, const int64_t* c_ptr,
-                  const double* coef, const double* norm,
-                  const double* smooth_slots, int64_t nslots, double* out,
+/* Geometry of the subdomains, on the device.
+ * piv_tps_select: ordered compaction of the n vectors whose dx and dy are
+ * finite (singlepass.py:450-463) into cx, cy (positions), vx, vy (values)
+ * and idx (original index); info[5] = count, x_min, x_max, y_min, y_max.
+ * piv_tps_count / piv_tps_fill: members of each of the nsx * nsy
+ * subdomains (d = iy * nsx + ix) with the half-open limits `lim` (device:
+ * lmin_x[nsx], lmax_x[nsx], lmin_y[nsy], lmax_y[nsy]), in increasing
+ * center order (thin_plate_spline_subdom.py:98-116); fill writes them at
+ * dom_off[d] (prefix sum of the counts) into slot_pt.
+ * piv_tps_point_slots: for each point, its (at most 4) subdomains in
+ * increasing order, their partition-of-unity weights and the sum of the
+ * weights (thin_plate_spline_subdom.py:118-126,187-196,208-222); with
+ * dom_off/slot_pt the slot is the point's position in the member list of
+ * the subdomain (centers), else the subdomain index (new positions).
+ * status[0] is set when a point lies in more than 4 subdomains. */
+int piv_tps_select(int32_t n, const float* dx, const float* dy,
+                   const double* xs, const double* ys, double* cx, double* cy,
+                   float* vx, float* vy, int32_t* idx, double* info,
+                   void* stream);
+int piv_tps_count(int32_t nsx, int32_t nsy, const double* lim, int32_t n,
+                  const double* cx, const double* cy, int32_t* counts,
                   void* stream);
+int piv_tps_fill(int32_t nsx, int32_t nsy, const double* lim, int32_t n,
+                 const double* cx, const double* cy, const int64_t* dom_off,
+                 int32_t* slot_pt, void* stream);
+int piv_tps_point_slots(int32_t nsx, int32_t nsy, const double* lim,
+                        int32_t npts, const double* px, const double* py,
+                        const int64_t* dom_off, const int32_t* slot_pt,
+                        int32_t* out_slot, double* out_coef, double* out_norm,
+                        int32_t* status, void* stream);
 
-/* TPS evaluation at npts new positions (px, py): for each of the point's
- * subdomains (n_slots/n_ptr, subdomain n_dom[s], weight n_coef[s]) the
- * weighted sum of w_j r^2 ln(r^2) / 2 plus the linear part, blended and
- * normalised (thin_plate_spline_subdom.py:161-234). out[2][npts]. */
+/* out[c][i] = sum_k coef[4 i + k] * smooth_slots[c][slots[4 i + k]] /
+ * norm[i] over the point's slots (-1 terminated): the blend of
+ * thin_plate_spline_subdom.py:145-158. out[2][npts]. */
+int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef,
+                  const double* norm, const double* smooth_slots,
+                  int64_t nslots, double* out, void* stream);
+
+/* TPS evaluation at npts new positions (px, py): for each subdomain
+ * doms[4 p + k] (-1 terminated) of the point, the weighted sum of
+ * w_j r^2 ln(r^2) / 2 over its centers plus the linear part, blended with
+ * coef[4 p + k] and divided by norm[p] (thin_plate_spline_subdom.py:224-234).
+ * out[2][npts]. */
 int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
-                        const int32_t* n_slots, const int64_t* n_ptr,
-                        const int32_t* n_dom, const double* n_coef,
-                        const double* xs, const double* ys,
+                        const int32_t* doms, const double* coef,
+                        const double* norm, const double* cx, const double* cy,
                         const int64_t* dom_off, const int32_t* slot_pt,
                         const double* weights, int64_t wstride, double* out,
                         void* stream);

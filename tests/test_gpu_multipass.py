@@ -1,0 +1,79 @@
+"""GPU multipass pipeline against the oracle's multipass chain.
+
+Bars: every pass's displacements within 1e-3 px and identical error
+strings, except windows whose FP64 map has a near-tie (listed) and the
+windows downstream of them; final interpolated fields within 1e-3 px."""
+
+import numpy as np
+import pytest
+
+from tests.helpers import near_tie_windows
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(use_tps, number=3, window=64, shape=(512, 512), nb_peaks=1):
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+    from tests.test_gpu_piv import _oracle_params
+
+    im0, im1 = config_pair(3, shape=shape)
+    p = WorkPIV.create_default_params()
+    p.piv0.shape_crop_im0 = window
+    p.piv0.nb_peaks_to_search = nb_peaks
+    p.multipass.number = number
+    p.multipass.use_tps = use_tps
+    p.multipass.smoothing_coef = 2.0
+    p.multipass.subdom_size = 200
+    return im0, im1, p, _oracle_params(p)
+
+
+@pytest.mark.parametrize("use_tps", [True, "last", False])
+def test_multipass_vs_oracle(use_tps):
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+    from oracle.piv import OracleWorkPIV
+
+    print("case: multipass", use_tps, flush=True)
+    im0, im1, p, op = _setup(use_tps)
+    gpu = WorkPIV(p).calcul_from_arrays(im0, im1)
+    ref = OracleWorkPIV(op, precision="f64").calcul(im0, im1)
+    assert len(gpu.passes) == len(ref) == 3
+    frac_ok = []
+    for ip, (g, e) in enumerate(zip(gpu.passes, ref)):
+        maps = [m for m in e["correls"] if m is not None]
+        ties = near_tie_windows(np.stack(maps)) if maps else []
+        both_nan = np.isnan(g.deltaxs) & np.isnan(e["deltaxs"])
+        close = np.abs(g.deltaxs - e["deltaxs"]) <= 1e-3
+        closey = np.abs(g.deltays - e["deltays"]) <= 1e-3
+        ok = both_nan | (close & closey)
+        frac = ok.mean()
+        frac_ok.append(frac)
+        bad = np.flatnonzero(~ok)
+        print(f"pass {ip}: {len(ok)} vectors, {100 * frac:.3f} % within 1e-3 px, "
+              f"near-ties {list(ties)[:10]}, mismatches {list(bad)[:10]}", flush=True)
+        assert frac >= 0.995
+        same_err = sum(g.errors.get(i) == e["errors"].get(i) for i in range(len(ok)))
+        assert same_err >= 0.995 * len(ok)
+    gf = gpu.passes[-1]
+    ef = ref[-1]
+    df = np.abs(gf.deltaxs_final - ef["deltaxs_final"])
+    print("final field: max |diff|", df.max(), "frac > 1e-3:", (df > 1e-3).mean(), flush=True)
+    assert (df <= 1e-3).mean() >= 0.99
+
+
+def test_batch_equals_single_pairs():
+    """Batching pairs changes the execution, not the arithmetic."""
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+
+    im0, im1, p, _ = _setup(True, number=2, window=32, shape=(384, 384))
+    a0, a1 = config_pair(2, shape=(384, 384))
+    w = WorkPIV(p)
+    batch = w.calcul_batch([(im0, im1), (a0, a1), (im1, im0)])
+    for (x0, x1), b in zip([(im0, im1), (a0, a1), (im1, im0)], batch):
+        single = WorkPIV(p).calcul_from_arrays(x0, x1)
+        for ps, pb in zip(single.passes, b.passes):
+            assert np.array_equal(ps.deltaxs, pb.deltaxs, equal_nan=True)
+            assert np.array_equal(ps.deltays, pb.deltays, equal_nan=True)
+            assert ps.errors == pb.errors
+        assert np.array_equal(single.passes[-1].deltaxs_final, b.passes[-1].deltaxs_final)
