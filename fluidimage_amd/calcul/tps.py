@@ -29,7 +29,7 @@ import numpy as np
 from .. import _native
 
 MAX_SLOTS = 4
-MAX_CENTERS_PER_SUBDOM = 1024
+MAX_CENTERS_PER_SUBDOM = 2048
 
 
 class SingularSystemError(ArithmeticError):

@@ -25,10 +25,9 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kNB = 32;       // triangular-solve block size
-constexpr int kNBc = 16;      // Cholesky block size (panel of up to kMaxN rows in LDS)
-constexpr int kMaxN = 1024;   // centers per subdomain
-constexpr int kColsPerThread = kMaxN / kThreads;
+constexpr int kNB = 32;       // Cholesky / triangular-solve block size
+constexpr int kMaxN = 2048;   // centers per subdomain
+constexpr int kColsPerThread = (kMaxN + kThreads - 1) / kThreads;
 
 __device__ __forceinline__ double tps_phi(double r2) {
   return r2 != 0.0 ? r2 * log(r2) / 2.0 : 0.0;
@@ -225,129 +224,183 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
 }
 
 // ---------------------------------------------------------------------------
-// 2. B' = B - V Z^T - Z V^T on the lower triangle, then C = L L^T
+// 2. B' = B - V Z^T - Z V^T applied on the fly, C = B'[3:, 3:] = L L^T by a
+//    left-looking blocked Cholesky (block columns of kNB): each block column
+//    is read once, updated with the finished panels to its left (read back
+//    from global memory), its diagonal block factorised and inverted by one
+//    wavefront, and its panel multiplied by the inverse. The inverse of
+//    every diagonal block is kept, transposed, in the unused upper triangle
+//    of that block (its diagonal is 1 / L_ii) for the triangular solves.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_tps_chol(const int64_t* __restrict__ dom_off,
-                                                       const int64_t* __restrict__ mat_off,
-                                                       double* __restrict__ mats,
-                                                       const double* __restrict__ vwork,
-                                                       int32_t* __restrict__ status) {
+__device__ __forceinline__ double wz_corr(const double* VW, int i, int j) {
+  const double* a = VW + 6 * i;
+  const double* b = VW + 6 * j;
+  return a[0] * b[3] + a[1] * b[4] + a[2] * b[5] + a[3] * b[0] + a[4] * b[1] + a[5] * b[2];
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kCholThreads = 512;  // one row of the block column per thread
+
+__global__ __launch_bounds__(kCholThreads) void k_tps_chol(const int64_t* __restrict__ dom_off,
+                                                           const int64_t* __restrict__ mat_off,
+                                                           double* __restrict__ mats,
+                                                           const double* __restrict__ vwork,
+                                                           int32_t* __restrict__ status) {
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
   double* B = mats + mat_off[d];
   const double* VW = vwork + 6 * s0;
   const int tid = threadIdx.x;
-  __shared__ double diag[kNBc][kNBc + 1];
-  __shared__ double panel[kMaxN][kNBc + 1];
+  __shared__ __align__(16) double ljk[2][kNB][kNB];  // double-buffered L(J, K)^T blocks
+  __shared__ double dg[kNB][kNB + 1];      // L_JJ
+  __shared__ double inv[kNB][kNB + 1];     // inv(L_JJ)
+  __shared__ double rdiag[kNB];            // 1 / L_ii
   __shared__ int bad;
   if (tid == 0) bad = 0;
-  // congruence on rows >= 3, lower triangle (columns 0..2 keep B'[3:, :3])
-  for (int i = 3; i < n; ++i) {
-    const double vi0 = VW[6 * i], vi1 = VW[6 * i + 1], vi2 = VW[6 * i + 2];
-    const double zi0 = VW[6 * i + 3], zi1 = VW[6 * i + 4], zi2 = VW[6 * i + 5];
-    for (int j = tid; j <= i; j += kThreads) {
-      const double* vj = VW + 6 * j;
-      const double c = vi0 * vj[3] + vi1 * vj[4] + vi2 * vj[5] + zi0 * vj[0] + zi1 * vj[1] +
-                       zi2 * vj[2];
-      B[int64_t(i) * n + j] -= c;
-    }
+  // B'[3:, :3] (kept for the solves)
+  for (int e = tid; e < (n - 3) * 3; e += kCholThreads) {
+    const int i = 3 + e / 3, k = e % 3;
+    B[int64_t(i) * n + k] -= wz_corr(VW, i, k);
   }
-  __syncthreads();
   const int m = n - 3;
-  double* A = B + 3 * int64_t(n) + 3;  // C(i, j) = A[i * n + j]
-  for (int kb = 0; kb < m; kb += kNBc) {
-    const int bs = min(kNBc, m - kb);
-    // -- diagonal block
-    for (int e = tid; e < bs * bs; e += kThreads) {
-      const int i = e / bs, j = e % bs;
-      diag[i][j] = j <= i ? A[int64_t(kb + i) * n + kb + j] : 0.0;
-    }
-    __syncthreads();
-    for (int j = 0; j < bs; ++j) {
-      if (tid == 0) {
-        const double djj = diag[j][j];
-        if (!(djj > 0.0)) bad = 1;
-        diag[j][j] = djj > 0.0 ? sqrt(djj) : 1.0;
+  double* A = B + 3 * int64_t(n) + 3;  // C(i, j) = A[i n + j]
+  const double* VC = VW + 18;          // V, Z rows of C's index space
+  const int nblk = (m + kNB - 1) / kNB;
+  for (int J = 0; J < nblk; ++J) {
+    const int r0 = J * kNB;
+    const int bs = min(kNB, m - r0);
+    const int nchunk = (m - r0 + kCholThreads - 1) / kCholThreads;
+    for (int ch = 0; ch < nchunk; ++ch) {
+      const int rr = r0 + ch * kCholThreads + tid;
+      const bool row = rr < m;
+      double acc[kNB];
+      {
+        const int ri = row ? rr : m - 1;
+        const double* Ar = A + int64_t(ri) * n + r0;
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+          const int cc = c < bs ? c : bs - 1;
+          const double v = Ar[cc] - wz_corr(VC, ri, r0 + cc);
+          acc[c] = (row && c < bs && r0 + c <= rr) ? v : 0.0;
+        }
+      }
+      // acc -= L(rr, 0:r0) L(J, 0:r0)^T, one 32-column block K at a time;
+      // L(J, K) sits transposed in LDS (ljk[k][c]) and the next block is
+      // fetched while the current one is used
+      if (J > 0) {
+        const double* Lr = A + int64_t(row ? rr : m - 1) * n;
+        // element e of the block: row c = e % kNB of L(J, K), column k = e / kNB
+        const int e0 = tid, e1 = tid + kCholThreads;  // kNB * kNB = 2 * 512
+        const int c0 = e0 % kNB, k0 = e0 / kNB, c1 = e1 % kNB, k1 = e1 / kNB;
+        const double* g0 = A + int64_t(r0 + (c0 < bs ? c0 : 0)) * n + k0;
+        const double* g1 = A + int64_t(r0 + (c1 < bs ? c1 : 0)) * n + k1;
+        __syncthreads();
+        ljk[0][k0][c0] = c0 < bs ? g0[0] : 0.0;
+        ljk[0][k1][c1] = c1 < bs ? g1[0] : 0.0;
+        __syncthreads();
+        for (int K = 0; K < J; ++K) {
+          const int cur = K & 1;
+          const bool more = K + 1 < J;
+          double t0 = 0.0, t1 = 0.0;
+          if (more) {
+            t0 = g0[(K + 1) * kNB];
+            t1 = g1[(K + 1) * kNB];
+          }
+          const double* lk = Lr + K * kNB;
+          // rows of A start at any 8-byte boundary: scalar loads
+          double l4[4] = {lk[0], lk[1], lk[2], lk[3]};
+#pragma unroll 1
+          for (int k = 0; k < kNB; k += 4) {
+            double n4[4];
+            const double* lnx = lk + (k + 4 < kNB ? k + 4 : k);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) n4[kk] = lnx[kk];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+              const double2* col = reinterpret_cast<const double2*>(&ljk[cur][k + kk][0]);
+#pragma unroll
+              for (int c = 0; c < kNB; c += 2) {
+                const double2 w = col[c / 2];
+                acc[c] -= l4[kk] * w.x;
+                acc[c + 1] -= l4[kk] * w.y;
+              }
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) l4[kk] = n4[kk];
+          }
+          if (more) {
+            ljk[cur ^ 1][k0][c0] = c0 < bs ? t0 : 0.0;
+            ljk[cur ^ 1][k1][c1] = c1 < bs ? t1 : 0.0;
+          }
+          __syncthreads();
+        }
+      }
+      if (ch == 0) {
+        // diagonal block: left-looking unblocked Cholesky in LDS by
+        // wavefront 0 (lane i owns row i), then inv(L_JJ) column by column
+        if (tid < bs) {
+#pragma unroll
+          for (int c = 0; c < kNB; ++c) dg[tid][c] = c <= tid ? acc[c] : 0.0;
+        }
+        __syncthreads();
+        if (tid < 64) {
+          const int lane = tid;
+          for (int j = 0; j < bs; ++j) {
+            double v = 0.0;
+            if (lane >= j && lane < bs) {
+              v = dg[lane][j];
+              for (int k = 0; k < j; ++k) v -= dg[lane][k] * dg[j][k];
+            }
+            const double djj = __shfl(v, j, 64);
+            const double ljj = djj > 0.0 ? sqrt(djj) : 1.0;
+            if (lane == 0 && !(djj > 0.0)) bad = 1;
+            if (lane == j) dg[j][j] = ljj;
+            else if (lane > j && lane < bs) dg[lane][j] = v / ljj;
+            wave_sync();
+          }
+          if (lane < bs) rdiag[lane] = 1.0 / dg[lane][lane];
+          wave_sync();
+          if (lane < bs) {
+            for (int i = 0; i < bs; ++i) {
+              double x = 0.0;
+              if (i == lane) {
+                x = rdiag[i];
+              } else if (i > lane) {
+                double sacc = 0.0;
+                for (int k = lane; k < i; ++k) sacc += dg[i][k] * inv[k][lane];
+                x = -sacc * rdiag[i];
+              }
+              inv[i][lane] = x;
+            }
+          }
+        }
+        __syncthreads();
+        for (int e = tid; e < bs * bs; e += kCholThreads) {
+          const int i = e / bs, c = e % bs;
+          // L_JJ (lower with diagonal), inv(L_JJ)^T (strictly upper)
+          A[int64_t(r0 + i) * n + r0 + c] = c <= i ? dg[i][c] : inv[c][i];
+        }
+      }
+      // panel rows: L(rr, J) = a_rr inv(L_JJ)^T
+      if (row && rr >= r0 + bs) {
+        double* dst = A + int64_t(rr) * n + r0;
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+          if (c >= bs) continue;
+          double sacc = 0.0;
+#pragma unroll
+          for (int k = 0; k <= c; ++k) sacc += acc[k] * inv[c][k];
+          dst[c] = sacc;
+        }
       }
       __syncthreads();
-      const double ljj = diag[j][j];
-      for (int i = j + 1 + tid; i < bs; i += kThreads) diag[i][j] /= ljj;
-      __syncthreads();
-      for (int e = tid; e < bs * bs; e += kThreads) {
-        const int i = e / bs, k = e % bs;
-        if (k > j && k <= i) diag[i][k] -= diag[i][j] * diag[k][j];
-      }
-      __syncthreads();
     }
-    for (int e = tid; e < bs * bs; e += kThreads) {
-      const int i = e / bs, j = e % bs;
-      if (j <= i) A[int64_t(kb + i) * n + kb + j] = diag[i][j];
-    }
-    // -- panel: rows below, L_ij = A_ij L_jj^-T (row-wise triangular solve)
-    const int r0 = kb + bs;
-    for (int i = r0 + tid; i < m; i += kThreads) {
-      double row[kNBc];
-      double* Ai = A + int64_t(i) * n + kb;
-#pragma unroll
-      for (int j = 0; j < kNBc; ++j) row[j] = j < bs ? Ai[j] : 0.0;
-#pragma unroll
-      for (int j = 0; j < kNBc; ++j) {
-        if (j < bs) {
-          double s = row[j];
-          for (int k = 0; k < j; ++k) s -= row[k] * diag[j][k];
-          row[j] = s / diag[j][j];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < kNBc; ++j) {
-        if (j < bs) {
-          Ai[j] = row[j];
-          panel[i - r0][j] = row[j];
-        }
-      }
-    }
-    __syncthreads();
-    // -- trailing update of the lower triangle, 4 x 4 register tiles
-    const int mt = m - r0;
-    const int nt = (mt + 3) / 4;
-    const int ntiles = nt * (nt + 1) / 2;
-    for (int t = tid; t < ntiles; t += kThreads) {
-      // tile (ti, tj) with tj <= ti
-      int ti = int((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
-      while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-      while (ti * (ti + 1) / 2 > t) --ti;
-      const int tj = t - ti * (ti + 1) / 2;
-      const int i0 = 4 * ti, j0 = 4 * tj;
-      double acc[4][4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-      for (int k = 0; k < bs; ++k) {
-        double pa[4], pb[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          pa[a] = i0 + a < mt ? panel[i0 + a][k] : 0.0;
-          pb[a] = j0 + a < mt ? panel[j0 + a][k] : 0.0;
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] += pa[a] * pb[b];
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int i = i0 + a;
-        if (i >= mt) continue;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int j = j0 + b;
-          if (j < mt && j <= i) A[int64_t(r0 + i) * n + r0 + j] -= acc[a][b];
-        }
-      }
-    }
-    __syncthreads();
   }
   if (tid == 0 && bad) status[d] = 1;
 }
@@ -374,34 +427,38 @@ __device__ void apply_q(double (*x)[kMaxN], int n, const double* VW, const doubl
   }
 }
 
-// solve C z = b in place (x[c][3 + i], c = 0, 1) with C = L L^T
+// solve C z = b in place (x[c][3 + i], c = 0, 1) with C = L L^T, using the
+// inverted diagonal blocks (upper triangle of each diagonal block, diagonal
+// 1 / L_ii): every block step is a small matrix-vector product
 __device__ void chol_solve(double (*x)[kMaxN], const double* A, int n, int m) {
-  __shared__ double dg[kNB][kNB + 1];
+  __shared__ double xb[2][kNB];
   const int tid = threadIdx.x;
+  const int nblk = (m + kNB - 1) / kNB;
   // forward: L y = b
-  for (int kb = 0; kb < m; kb += kNB) {
-    const int bs = min(kNB, m - kb);
-    for (int e = tid; e < bs * bs; e += kThreads) {
-      const int i = e / bs, j = e % bs;
-      dg[i][j] = j <= i ? A[int64_t(kb + i) * n + kb + j] : 0.0;
-    }
-    __syncthreads();
-    if (tid < 2) {
-      double* b = x[tid] + 3 + kb;
-      for (int i = 0; i < bs; ++i) {
-        double s = b[i];
-        for (int j = 0; j < i; ++j) s -= dg[i][j] * b[j];
-        b[i] = s / dg[i][i];
+  for (int J = 0; J < nblk; ++J) {
+    const int r0 = J * kNB;
+    const int bs = min(kNB, m - r0);
+    if (tid < 64) {
+      const int c = tid & 31, comp = tid >> 5;
+      if (c < bs) {
+        const double* b = x[comp] + 3 + r0;
+        double s = b[c] / A[int64_t(r0 + c) * n + r0 + c];
+        for (int k = 0; k < c; ++k) s += A[int64_t(r0 + k) * n + r0 + c] * b[k];
+        xb[comp][c] = s;
       }
     }
     __syncthreads();
-    for (int i = kb + bs + tid; i < m; i += kThreads) {
-      const double* Ai = A + int64_t(i) * n + kb;
+    if (tid < 64) {
+      const int c = tid & 31, comp = tid >> 5;
+      if (c < bs) x[comp][3 + r0 + c] = xb[comp][c];
+    }
+    for (int i = r0 + bs + tid; i < m; i += kThreads) {
+      const double* Ai = A + int64_t(i) * n + r0;
       double s0 = 0.0, s1 = 0.0;
-      for (int j = 0; j < bs; ++j) {
-        const double l = Ai[j];
-        s0 += l * x[0][3 + kb + j];
-        s1 += l * x[1][3 + kb + j];
+      for (int k = 0; k < bs; ++k) {
+        const double l = Ai[k];
+        s0 += l * xb[0][k];
+        s1 += l * xb[1][k];
       }
       x[0][3 + i] -= s0;
       x[1][3 + i] -= s1;
@@ -409,30 +466,30 @@ __device__ void chol_solve(double (*x)[kMaxN], const double* A, int n, int m) {
     __syncthreads();
   }
   // backward: L^T z = y
-  const int nblk = (m + kNB - 1) / kNB;
-  for (int bb = nblk - 1; bb >= 0; --bb) {
-    const int kb = bb * kNB;
-    const int bs = min(kNB, m - kb);
-    for (int e = tid; e < bs * bs; e += kThreads) {
-      const int i = e / bs, j = e % bs;
-      dg[i][j] = j <= i ? A[int64_t(kb + i) * n + kb + j] : 0.0;
-    }
-    __syncthreads();
-    if (tid < 2) {
-      double* b = x[tid] + 3 + kb;
-      for (int i = bs - 1; i >= 0; --i) {
-        double s = b[i];
-        for (int j = i + 1; j < bs; ++j) s -= dg[j][i] * b[j];
-        b[i] = s / dg[i][i];
+  for (int J = nblk - 1; J >= 0; --J) {
+    const int r0 = J * kNB;
+    const int bs = min(kNB, m - r0);
+    if (tid < 64) {
+      const int c = tid & 31, comp = tid >> 5;
+      if (c < bs) {
+        const double* y = x[comp] + 3 + r0;
+        const double* row = A + int64_t(r0 + c) * n + r0;
+        double s = y[c] / row[c];
+        for (int k = c + 1; k < bs; ++k) s += row[k] * y[k];
+        xb[comp][c] = s;
       }
     }
     __syncthreads();
-    for (int j = tid; j < kb; j += kThreads) {
+    if (tid < 64) {
+      const int c = tid & 31, comp = tid >> 5;
+      if (c < bs) x[comp][3 + r0 + c] = xb[comp][c];
+    }
+    for (int j = tid; j < r0; j += kThreads) {
       double s0 = 0.0, s1 = 0.0;
       for (int i = 0; i < bs; ++i) {
-        const double l = A[int64_t(kb + i) * n + j];
-        s0 += l * x[0][3 + kb + i];
-        s1 += l * x[1][3 + kb + i];
+        const double l = A[int64_t(r0 + i) * n + j];
+        s0 += l * xb[0][i];
+        s1 += l * xb[1][i];
       }
       x[0][3 + j] -= s0;
       x[1][3 + j] -= s1;
@@ -596,7 +653,8 @@ __device__ int block_scan_1024(int v, int* tmp, int& total) {
 }
 
 // one block of 1024 threads: ordered compaction of the vectors whose dx and
-// dy are finite (singlepass.py:450-463) and the bounds of their positions
+// dy are finite (singlepass.py:450-463) and the bounds of their positions;
+// consecutive threads read consecutive vectors, tile by tile
 __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restrict__ dx,
                                                      const float* __restrict__ dy,
                                                      const double* __restrict__ xs,
@@ -605,32 +663,33 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
                                                      float* vy, int32_t* idx, double* info) {
   __shared__ int tmp[32];
   __shared__ double red[4][16];
-  const int per = (n + 1023) / 1024;
-  const int i0 = threadIdx.x * per, i1 = min(n, i0 + per);
-  int cnt = 0;
   double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
-  for (int i = i0; i < i1; ++i) {
-    const float a = dx[i], b = dy[i];
-    if (a == a && b == b) {
-      ++cnt;
-      xmin = fmin(xmin, xs[i]);
-      xmax = fmax(xmax, xs[i]);
-      ymin = fmin(ymin, ys[i]);
-      ymax = fmax(ymax, ys[i]);
+  int base = 0;
+  for (int t0 = 0; t0 < n; t0 += 1024) {
+    const int i = t0 + threadIdx.x;
+    float a = 0.f, b = 0.f;
+    int f = 0;
+    if (i < n) {
+      a = dx[i];
+      b = dy[i];
+      f = (a == a && b == b) ? 1 : 0;
     }
-  }
-  int total;
-  int pos = block_scan_1024(cnt, tmp, total) - cnt;
-  for (int i = i0; i < i1; ++i) {
-    const float a = dx[i], b = dy[i];
-    if (a == a && b == b) {
-      cx[pos] = xs[i];
-      cy[pos] = ys[i];
+    int total;
+    const int incl = block_scan_1024(f, tmp, total);
+    if (f) {
+      const int pos = base + incl - 1;
+      const double x = xs[i], y = ys[i];
+      cx[pos] = x;
+      cy[pos] = y;
       vx[pos] = a;
       vy[pos] = b;
       idx[pos] = i;
-      ++pos;
+      xmin = fmin(xmin, x);
+      xmax = fmax(xmax, x);
+      ymin = fmin(ymin, y);
+      ymax = fmax(ymax, y);
     }
+    base += total;
   }
   for (int off = 32; off > 0; off >>= 1) {
     xmin = fmin(xmin, __shfl_xor(xmin, off, 64));
@@ -652,7 +711,7 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
       red[2][0] = fmin(red[2][0], red[2][k]);
       red[3][0] = fmax(red[3][0], red[3][k]);
     }
-    info[0] = total;
+    info[0] = base;
     info[1] = red[0][0];
     info[2] = red[1][0];
     info[3] = red[2][0];
@@ -855,14 +914,14 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
                    double* stats, int32_t* status, void* stream) {
   if (nd <= 0) return PIV_OK;
   if (n_max > kMaxN)
-    return tps_fail(PIV_EUNSUPPORTED, "more than 1024 centers in a TPS subdomain");
+    return tps_fail(PIV_EUNSUPPORTED, "more than 2048 centers in a TPS subdomain");
   if (!xs || !ys || !vx || !vy || !dom_off || !slot_pt || !mat_off || !mats || !refl ||
       !vwork || !smooth_slots || !weights || !stats || !status)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), 0, s, xs, ys, dom_off, slot_pt,
                      mat_off, smoothing, mats, refl, vwork, status);
-  hipLaunchKernelGGL(k_tps_chol, dim3(nd), dim3(kThreads), 0, s, dom_off, mat_off, mats, vwork,
+  hipLaunchKernelGGL(k_tps_chol, dim3(nd), dim3(kCholThreads), 0, s, dom_off, mat_off, mats, vwork,
                      status);
   hipLaunchKernelGGL(k_tps_solve, dim3(nd), dim3(kThreads), 0, s, vx, vy, dom_off, slot_pt,
                      mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,

@@ -185,7 +185,7 @@ int piv_fix(float* dx, float* dy, const float* correl_max, const int32_t* err,
  *      ThinPlateSplineSubdom.compute_tps_weights_subdom / interpolate
  *      (thin_plate_spline_subdom.py:128-159,224-280) and
  *      compute_tps_weights (thin_plate_spline.py:168-216) ----
- * Subdomain d holds the slots [dom_off[d], dom_off[d+1]) (at most 1024);
+ * Subdomain d holds the slots [dom_off[d], dom_off[d+1]) (at most 2048);
  * slot s is center slot_pt[s] of xs/ys (float64) with values vx/vy
  * (float32). mats: workspace of sum_d n_d^2 doubles at mat_off[d]; refl:
  * 16 doubles per subdomain; vwork: 6 doubles per slot. Outputs: the
