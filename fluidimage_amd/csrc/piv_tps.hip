@@ -238,6 +238,12 @@ __device__ __forceinline__ double wz_corr(const double* VW, int i, int j) {
   return a[0] * b[3] + a[1] * b[4] + a[2] * b[5] + a[3] * b[0] + a[4] * b[1] + a[5] * b[2];
 }
 
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -258,9 +264,7 @@ __global__ __launch_bounds__(kCholThreads) void k_tps_chol(const int64_t* __rest
   const double* VW = vwork + 6 * s0;
   const int tid = threadIdx.x;
   __shared__ __align__(16) double ljk[2][kNB][kNB];  // double-buffered L(J, K)^T blocks
-  __shared__ double dg[kNB][kNB + 1];      // L_JJ
   __shared__ double inv[kNB][kNB + 1];     // inv(L_JJ)
-  __shared__ double rdiag[kNB];            // 1 / L_ii
   __shared__ int bad;
   if (tid == 0) bad = 0;
   // B'[3:, :3] (kept for the solves)
@@ -295,6 +299,8 @@ __global__ __launch_bounds__(kCholThreads) void k_tps_chol(const int64_t* __rest
       // fetched while the current one is used
       if (J > 0) {
         const double* Lr = A + int64_t(row ? rr : m - 1) * n;
+        // waves without a row of the block column skip the arithmetic
+        const bool wave_active = r0 + ch * kCholThreads + (tid & ~63) < m;
         // element e of the block: row c = e % kNB of L(J, K), column k = e / kNB
         const int e0 = tid, e1 = tid + kCholThreads;  // kNB * kNB = 2 * 512
         const int c0 = e0 % kNB, k0 = e0 / kNB, c1 = e1 % kNB, k1 = e1 / kNB;
@@ -316,7 +322,7 @@ __global__ __launch_bounds__(kCholThreads) void k_tps_chol(const int64_t* __rest
           // rows of A start at any 8-byte boundary: scalar loads
           double l4[4] = {lk[0], lk[1], lk[2], lk[3]};
 #pragma unroll 1
-          for (int k = 0; k < kNB; k += 4) {
+          for (int k = 0; k < (wave_active ? kNB : 0); k += 4) {
             double n4[4];
             const double* lnx = lk + (k + 4 < kNB ? k + 4 : k);
 #pragma unroll
@@ -342,50 +348,55 @@ __global__ __launch_bounds__(kCholThreads) void k_tps_chol(const int64_t* __rest
         }
       }
       if (ch == 0) {
-        // diagonal block: left-looking unblocked Cholesky in LDS by
-        // wavefront 0 (lane i owns row i), then inv(L_JJ) column by column
-        if (tid < bs) {
-#pragma unroll
-          for (int c = 0; c < kNB; ++c) dg[tid][c] = c <= tid ? acc[c] : 0.0;
-        }
-        __syncthreads();
+        // diagonal block in the registers of wavefront 0 (lane i < 32 holds
+        // row i of the block in acc and row i of inv(L_JJ) in x):
+        // right-looking factorisation, each step also eliminating column j
+        // from the rows of inv(L_JJ) below j; entries of other lanes come
+        // through readlane (uniform values)
         if (tid < 64) {
           const int lane = tid;
-          for (int j = 0; j < bs; ++j) {
-            double v = 0.0;
-            if (lane >= j && lane < bs) {
-              v = dg[lane][j];
-              for (int k = 0; k < j; ++k) v -= dg[lane][k] * dg[j][k];
+          const bool drow = lane < kNB;
+          bool singular = false;
+          double x[kNB];
+#pragma unroll
+          for (int c = 0; c < kNB; ++c) x[c] = c == lane ? 1.0 : 0.0;
+#pragma unroll
+          for (int j = 0; j < kNB; ++j) {
+            double dj = readlane_d(acc[j], j);
+            if (j >= bs) dj = 1.0;
+            if (!(dj > 0.0)) {
+              singular = true;
+              dj = 1.0;
             }
-            const double djj = __shfl(v, j, 64);
-            const double ljj = djj > 0.0 ? sqrt(djj) : 1.0;
-            if (lane == 0 && !(djj > 0.0)) bad = 1;
-            if (lane == j) dg[j][j] = ljj;
-            else if (lane > j && lane < bs) dg[lane][j] = v / ljj;
-            wave_sync();
-          }
-          if (lane < bs) rdiag[lane] = 1.0 / dg[lane][lane];
-          wave_sync();
-          if (lane < bs) {
-            for (int i = 0; i < bs; ++i) {
-              double x = 0.0;
-              if (i == lane) {
-                x = rdiag[i];
-              } else if (i > lane) {
-                double sacc = 0.0;
-                for (int k = lane; k < i; ++k) sacc += dg[i][k] * inv[k][lane];
-                x = -sacc * rdiag[i];
+            const double rj = 1.0 / sqrt(dj);
+            if (drow) {
+              // column j of L_JJ (lane j: L_jj = dj rj), trailing update
+              acc[j] = j < bs ? acc[j] * rj : (lane == j ? 1.0 : 0.0);
+#pragma unroll
+              for (int c = j + 1; c < kNB; ++c) acc[c] -= acc[j] * readlane_d(acc[j], c);
+              // row j of inv(L_JJ) is final once scaled by 1 / L_jj
+              const double lij = lane > j ? acc[j] : 0.0;
+#pragma unroll
+              for (int c = 0; c <= j; ++c) {
+                const double xjc = readlane_d(x[c], j) * rj;
+                x[c] = lane == j ? xjc : x[c] - lij * xjc;
               }
-              inv[i][lane] = x;
+            }
+          }
+          if (singular && lane == 0) bad = 1;
+          if (lane < bs) {
+            // L_JJ in the lower triangle with the diagonal (row `lane`),
+            // inv(L_JJ)^T in the strictly upper one (column `lane`)
+            double* blk = A + int64_t(r0) * n + r0;
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) {
+              if (c <= lane) blk[int64_t(lane) * n + c] = acc[c];
+              if (c < lane) blk[int64_t(c) * n + lane] = x[c];
+              inv[lane][c] = x[c];
             }
           }
         }
         __syncthreads();
-        for (int e = tid; e < bs * bs; e += kCholThreads) {
-          const int i = e / bs, c = e % bs;
-          // L_JJ (lower with diagonal), inv(L_JJ)^T (strictly upper)
-          A[int64_t(r0 + i) * n + r0 + c] = c <= i ? dg[i][c] : inv[c][i];
-        }
       }
       // panel rows: L(rr, J) = a_rr inv(L_JJ)^T
       if (row && rr >= r0 + bs) {
