@@ -225,12 +225,18 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
 
 // ---------------------------------------------------------------------------
 // 2. B' = B - V Z^T - Z V^T applied on the fly, C = B'[3:, 3:] = L L^T by a
-//    left-looking blocked Cholesky (block columns of kNB): each block column
-//    is read once, updated with the finished panels to its left (read back
-//    from global memory), its diagonal block factorised and inverted by one
-//    wavefront, and its panel multiplied by the inverse. The inverse of
-//    every diagonal block is kept, transposed, in the unused upper triangle
-//    of that block (its diagonal is 1 / L_ii) for the triangular solves.
+//    left-looking blocked Cholesky (block columns of kNB = 32). For block
+//    column J (rows r0 = 32 J .. m):
+//    a) update: C(r0:m, J) -= L(r0:m, 0:r0) L(J, 0:r0)^T on the FP64 matrix
+//       cores (v_mfma_f64_16x16x4), one 16-row strip per wavefront at a time,
+//       operands read from L2, the result written back in place;
+//    b) the 32 x 32 diagonal block is factorised and inverted by wavefront 0
+//       in registers;
+//    c) panel: L(r0+32:m, J) = C(r0+32:m, J) inv(L_JJ)^T, again on the
+//       matrix cores with inv(L_JJ) in LDS.
+//    The inverse of every diagonal block is kept, transposed, in the unused
+//    upper triangle of that block (the diagonal keeps L_ii) for the
+//    triangular solves.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wz_corr(const double* VW, int i, int j) {
   const double* a = VW + 6 * i;
@@ -250,9 +256,18 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kCholThreads = 512;  // one row of the block column per thread
+typedef double dvec4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(kCholThreads) void k_tps_chol(const int64_t* __restrict__ dom_off,
+// D(16x16) += A(16x4) B(4x16): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[4 v + l / 16][l % 16], v = 0..3
+__device__ __forceinline__ dvec4 mfma_f64(double a, double b, dvec4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// NT threads per subdomain: 512 when every subdomain has a CU of its own,
+// 256 (two workgroups per CU) otherwise
+template <int NT>
+__global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom_off,
                                                            const int64_t* __restrict__ mat_off,
                                                            double* __restrict__ mats,
                                                            const double* __restrict__ vwork,
@@ -262,9 +277,11 @@ __global__ __launch_bounds__(kCholThreads) void k_tps_chol(const int64_t* __rest
   const int n = int(dom_off[d + 1] - s0);
   double* B = mats + mat_off[d];
   const double* VW = vwork + 6 * s0;
+  constexpr int kCholThreads = NT, kCholWaves = NT / 64;
   const int tid = threadIdx.x;
-  __shared__ __align__(16) double ljk[2][kNB][kNB];  // double-buffered L(J, K)^T blocks
-  __shared__ double inv[kNB][kNB + 1];     // inv(L_JJ)
+  const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;  // MFMA operand row / k group
+  __shared__ double inv[kNB][kNB + 1];       // inv(L_JJ)
   __shared__ int bad;
   if (tid == 0) bad = 0;
   // B'[3:, :3] (kept for the solves)
@@ -279,135 +296,136 @@ __global__ __launch_bounds__(kCholThreads) void k_tps_chol(const int64_t* __rest
   for (int J = 0; J < nblk; ++J) {
     const int r0 = J * kNB;
     const int bs = min(kNB, m - r0);
-    const int nchunk = (m - r0 + kCholThreads - 1) / kCholThreads;
-    for (int ch = 0; ch < nchunk; ++ch) {
-      const int rr = r0 + ch * kCholThreads + tid;
-      const bool row = rr < m;
+    const int nstrip = (m - r0 + 15) / 16;
+    // a) block-column update, strip by strip
+    for (int st = wave; st < nstrip; st += kCholWaves) {
+      const int row0 = r0 + 16 * st;
+      const int ra = min(row0 + li, m - 1);     // A operand row (clamped)
+      const int rb0 = min(r0 + li, m - 1);      // B operand rows: block J
+      const int rb1 = min(r0 + 16 + li, m - 1);
+      const double* pa = A + int64_t(ra) * n + 4 * lk;
+      const double* pb0 = A + int64_t(rb0) * n + 4 * lk;
+      const double* pb1 = A + int64_t(rb1) * n + 4 * lk;
+      dvec4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+      // k = k0 + 4 lk + q: the lane reads 4 consecutive columns per operand
+      for (int k0 = 0; k0 < r0; k0 += 16) {
+        double a4[4], b4[4], e4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a4[q] = pa[k0 + q];
+          b4[q] = pb0[k0 + q];
+          e4[q] = pb1[k0 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          c0 = mfma_f64(a4[q], b4[q], c0);
+          c1 = mfma_f64(a4[q], e4[q], c1);
+        }
+      }
+      // C(row, col) - wz - update, rows 4 v + lk, columns li (+16)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row0 + 4 * v + lk;
+        if (row < m) {
+          double* dst = A + int64_t(row) * n + r0;
+          const int col0 = li, col1 = 16 + li;
+          if (col0 < bs) dst[col0] = dst[col0] - wz_corr(VC, row, r0 + col0) - c0[v];
+          if (col1 < bs) dst[col1] = dst[col1] - wz_corr(VC, row, r0 + col1) - c1[v];
+        }
+      }
+    }
+    __syncthreads();
+    // b) diagonal block in the registers of wavefront 0 (lane i < 32 holds
+    //    row i of the block in acc and row i of inv(L_JJ) in x):
+    //    right-looking factorisation, each step also eliminating column j
+    //    from the rows of inv(L_JJ) below j; entries of other lanes come
+    //    through readlane (uniform values)
+    if (wave == 0) {
+      const bool drow = lane < kNB;
       double acc[kNB];
       {
-        const int ri = row ? rr : m - 1;
-        const double* Ar = A + int64_t(ri) * n + r0;
+        const double* src = A + int64_t(r0 + (lane < bs ? lane : 0)) * n + r0;
 #pragma unroll
         for (int c = 0; c < kNB; ++c) {
-          const int cc = c < bs ? c : bs - 1;
-          const double v = Ar[cc] - wz_corr(VC, ri, r0 + cc);
-          acc[c] = (row && c < bs && r0 + c <= rr) ? v : 0.0;
+          const double v = src[c < bs ? c : 0];
+          acc[c] = (lane < bs && c <= lane) ? v : 0.0;
         }
       }
-      // acc -= L(rr, 0:r0) L(J, 0:r0)^T, one 32-column block K at a time;
-      // L(J, K) sits transposed in LDS (ljk[k][c]) and the next block is
-      // fetched while the current one is used
-      if (J > 0) {
-        const double* Lr = A + int64_t(row ? rr : m - 1) * n;
-        // waves without a row of the block column skip the arithmetic
-        const bool wave_active = r0 + ch * kCholThreads + (tid & ~63) < m;
-        // element e of the block: row c = e % kNB of L(J, K), column k = e / kNB
-        const int e0 = tid, e1 = tid + kCholThreads;  // kNB * kNB = 2 * 512
-        const int c0 = e0 % kNB, k0 = e0 / kNB, c1 = e1 % kNB, k1 = e1 / kNB;
-        const double* g0 = A + int64_t(r0 + (c0 < bs ? c0 : 0)) * n + k0;
-        const double* g1 = A + int64_t(r0 + (c1 < bs ? c1 : 0)) * n + k1;
-        __syncthreads();
-        ljk[0][k0][c0] = c0 < bs ? g0[0] : 0.0;
-        ljk[0][k1][c1] = c1 < bs ? g1[0] : 0.0;
-        __syncthreads();
-        for (int K = 0; K < J; ++K) {
-          const int cur = K & 1;
-          const bool more = K + 1 < J;
-          double t0 = 0.0, t1 = 0.0;
-          if (more) {
-            t0 = g0[(K + 1) * kNB];
-            t1 = g1[(K + 1) * kNB];
+      bool singular = false;
+      double x[kNB];
+#pragma unroll
+      for (int c = 0; c < kNB; ++c) x[c] = c == lane ? 1.0 : 0.0;
+#pragma unroll
+      for (int j = 0; j < kNB; ++j) {
+        double dj = readlane_d(acc[j], j);
+        if (j >= bs) dj = 1.0;
+        if (!(dj > 0.0)) {
+          singular = true;
+          dj = 1.0;
+        }
+        const double rj = 1.0 / sqrt(dj);
+        if (drow) {
+          // column j of L_JJ (lane j: L_jj = dj rj), trailing update
+          acc[j] = j < bs ? acc[j] * rj : (lane == j ? 1.0 : 0.0);
+#pragma unroll
+          for (int c = j + 1; c < kNB; ++c) acc[c] -= acc[j] * readlane_d(acc[j], c);
+          // row j of inv(L_JJ) is final once scaled by 1 / L_jj
+          const double lij = lane > j ? acc[j] : 0.0;
+#pragma unroll
+          for (int c = 0; c <= j; ++c) {
+            const double xjc = readlane_d(x[c], j) * rj;
+            x[c] = lane == j ? xjc : x[c] - lij * xjc;
           }
-          const double* lk = Lr + K * kNB;
-          // rows of A start at any 8-byte boundary: scalar loads
-          double l4[4] = {lk[0], lk[1], lk[2], lk[3]};
-#pragma unroll 1
-          for (int k = 0; k < (wave_active ? kNB : 0); k += 4) {
-            double n4[4];
-            const double* lnx = lk + (k + 4 < kNB ? k + 4 : k);
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) n4[kk] = lnx[kk];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-              const double2* col = reinterpret_cast<const double2*>(&ljk[cur][k + kk][0]);
-#pragma unroll
-              for (int c = 0; c < kNB; c += 2) {
-                const double2 w = col[c / 2];
-                acc[c] -= l4[kk] * w.x;
-                acc[c + 1] -= l4[kk] * w.y;
-              }
-            }
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) l4[kk] = n4[kk];
-          }
-          if (more) {
-            ljk[cur ^ 1][k0][c0] = c0 < bs ? t0 : 0.0;
-            ljk[cur ^ 1][k1][c1] = c1 < bs ? t1 : 0.0;
-          }
-          __syncthreads();
         }
       }
-      if (ch == 0) {
-        // diagonal block in the registers of wavefront 0 (lane i < 32 holds
-        // row i of the block in acc and row i of inv(L_JJ) in x):
-        // right-looking factorisation, each step also eliminating column j
-        // from the rows of inv(L_JJ) below j; entries of other lanes come
-        // through readlane (uniform values)
-        if (tid < 64) {
-          const int lane = tid;
-          const bool drow = lane < kNB;
-          bool singular = false;
-          double x[kNB];
-#pragma unroll
-          for (int c = 0; c < kNB; ++c) x[c] = c == lane ? 1.0 : 0.0;
-#pragma unroll
-          for (int j = 0; j < kNB; ++j) {
-            double dj = readlane_d(acc[j], j);
-            if (j >= bs) dj = 1.0;
-            if (!(dj > 0.0)) {
-              singular = true;
-              dj = 1.0;
-            }
-            const double rj = 1.0 / sqrt(dj);
-            if (drow) {
-              // column j of L_JJ (lane j: L_jj = dj rj), trailing update
-              acc[j] = j < bs ? acc[j] * rj : (lane == j ? 1.0 : 0.0);
-#pragma unroll
-              for (int c = j + 1; c < kNB; ++c) acc[c] -= acc[j] * readlane_d(acc[j], c);
-              // row j of inv(L_JJ) is final once scaled by 1 / L_jj
-              const double lij = lane > j ? acc[j] : 0.0;
-#pragma unroll
-              for (int c = 0; c <= j; ++c) {
-                const double xjc = readlane_d(x[c], j) * rj;
-                x[c] = lane == j ? xjc : x[c] - lij * xjc;
-              }
-            }
-          }
-          if (singular && lane == 0) bad = 1;
-          if (lane < bs) {
-            // L_JJ in the lower triangle with the diagonal (row `lane`),
-            // inv(L_JJ)^T in the strictly upper one (column `lane`)
-            double* blk = A + int64_t(r0) * n + r0;
-#pragma unroll
-            for (int c = 0; c < kNB; ++c) {
-              if (c <= lane) blk[int64_t(lane) * n + c] = acc[c];
-              if (c < lane) blk[int64_t(c) * n + lane] = x[c];
-              inv[lane][c] = x[c];
-            }
-          }
-        }
-        __syncthreads();
-      }
-      // panel rows: L(rr, J) = a_rr inv(L_JJ)^T
-      if (row && rr >= r0 + bs) {
-        double* dst = A + int64_t(rr) * n + r0;
+      if (singular && lane == 0) bad = 1;
+      if (lane < bs) {
+        // L_JJ in the lower triangle with the diagonal (row `lane`),
+        // inv(L_JJ)^T in the strictly upper one (column `lane`)
+        double* blk = A + int64_t(r0) * n + r0;
 #pragma unroll
         for (int c = 0; c < kNB; ++c) {
-          if (c >= bs) continue;
-          double sacc = 0.0;
+          if (c <= lane) blk[int64_t(lane) * n + c] = acc[c];
+          if (c < lane) blk[int64_t(c) * n + lane] = x[c];
+        }
+      }
+      if (drow) {
 #pragma unroll
-          for (int k = 0; k <= c; ++k) sacc += acc[k] * inv[c][k];
-          dst[c] = sacc;
+        for (int c = 0; c < kNB; ++c) inv[lane][c] = x[c];
+      }
+    }
+    __syncthreads();
+    // c) panel rows below the diagonal block: L(rows, J) = C(rows, J) inv^T,
+    //    D[i][j] = sum_k C[i][k] inv[j][k] (k <= j), k = k0 + 4 lk + q
+    if (bs == kNB) {
+      for (int st = 2 + wave; st < nstrip; st += kCholWaves) {
+        const int row0 = r0 + 16 * st;
+        const int ra = min(row0 + li, m - 1);
+        const double* pa = A + int64_t(ra) * n + r0 + 4 * lk;
+        double a8[8];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) a8[4 * h + q] = pa[16 * h + q];
+        dvec4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int k = 16 * h + 4 * lk + q;
+            c0 = mfma_f64(a8[4 * h + q], inv[li][k], c0);
+            c1 = mfma_f64(a8[4 * h + q], inv[16 + li][k], c1);
+          }
+        // the strip's reads of C(rows, J) are complete before the writes
+        // (each strip is read and written by one wavefront only)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = row0 + 4 * v + lk;
+          if (row < m) {
+            double* dst = A + int64_t(row) * n + r0;
+            dst[li] = c0[v];
+            dst[16 + li] = c1[v];
+          }
         }
       }
       __syncthreads();
@@ -932,8 +950,12 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), 0, s, xs, ys, dom_off, slot_pt,
                      mat_off, smoothing, mats, refl, vwork, status);
-  hipLaunchKernelGGL(k_tps_chol, dim3(nd), dim3(kCholThreads), 0, s, dom_off, mat_off, mats, vwork,
-                     status);
+  if (nd <= 256)
+    hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), 0, s, dom_off, mat_off, mats, vwork,
+                       status);
+  else
+    hipLaunchKernelGGL(k_tps_chol<256>, dim3(nd), dim3(256), 0, s, dom_off, mat_off, mats, vwork,
+                       status);
   hipLaunchKernelGGL(k_tps_solve, dim3(nd), dim3(kThreads), 0, s, vx, vy, dom_off, slot_pt,
                      mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,
                      weights, wstride, stats, status);
