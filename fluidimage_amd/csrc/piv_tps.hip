@@ -457,71 +457,109 @@ __device__ void apply_q(double (*x)[kMaxN], int n, const double* VW, const doubl
 }
 
 // solve C z = b in place (x[c][3 + i], c = 0, 1) with C = L L^T, using the
-// inverted diagonal blocks (upper triangle of each diagonal block, diagonal
-// 1 / L_ii): every block step is a small matrix-vector product
+// inverted diagonal blocks (strict upper triangle of each diagonal block
+// holds inv(L_JJ)^T, the diagonal holds L_ii). Both sweeps are left-looking
+// so that every block step is one coalesced pass over a block row (forward)
+// or a block column (backward) of L, reduced across threads, followed by a
+// 32 x 32 triangular matrix-vector product out of LDS.
 __device__ void chol_solve(double (*x)[kMaxN], const double* A, int n, int m) {
-  __shared__ double xb[2][kNB];
+  static_assert(kThreads == 256 && kNB == 32, "thread layout of chol_solve");
+  __shared__ double dblk[kNB][kNB + 1];
+  __shared__ double tmp[2][kNB];
+  __shared__ double part[2][8][kNB];
   const int tid = threadIdx.x;
   const int nblk = (m + kNB - 1) / kNB;
-  // forward: L y = b
+  double* x0 = x[0] + 3;
+  double* x1 = x[1] + 3;
+  // forward: y_J = inv(L_JJ) (b_J - L(J, 0:r0) y(0:r0))
   for (int J = 0; J < nblk; ++J) {
     const int r0 = J * kNB;
     const int bs = min(kNB, m - r0);
-    if (tid < 64) {
-      const int c = tid & 31, comp = tid >> 5;
-      if (c < bs) {
-        const double* b = x[comp] + 3 + r0;
-        double s = b[c] / A[int64_t(r0 + c) * n + r0 + c];
-        for (int k = 0; k < c; ++k) s += A[int64_t(r0 + k) * n + r0 + c] * b[k];
-        xb[comp][c] = s;
+    for (int e = tid; e < kNB * kNB; e += kThreads) {
+      const int i = e / kNB, c = e % kNB;
+      dblk[i][c] = (i < bs && c < bs) ? A[int64_t(r0 + i) * n + r0 + c] : 0.0;
+    }
+    {
+      // thread = (row i, part p): columns 32 q + 4 p .. + 3 of row r0 + i
+      const int i = tid >> 3, p = tid & 7;
+      double s0 = 0.0, s1 = 0.0;
+      if (i < bs) {
+        const double* Lr = A + int64_t(r0 + i) * n + 4 * p;
+        for (int q = 0; q < J; ++q) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int k = 32 * q + 4 * p + u;
+            const double l = Lr[32 * q + u];
+            s0 += l * x0[k];
+            s1 += l * x1[k];
+          }
+        }
+      }
+#pragma unroll
+      for (int o = 4; o >= 1; o >>= 1) {
+        s0 += __shfl_xor(s0, o, 64);
+        s1 += __shfl_xor(s1, o, 64);
+      }
+      if (p == 0 && i < bs) {
+        tmp[0][i] = x0[r0 + i] - s0;
+        tmp[1][i] = x1[r0 + i] - s1;
       }
     }
     __syncthreads();
     if (tid < 64) {
       const int c = tid & 31, comp = tid >> 5;
-      if (c < bs) x[comp][3 + r0 + c] = xb[comp][c];
-    }
-    for (int i = r0 + bs + tid; i < m; i += kThreads) {
-      const double* Ai = A + int64_t(i) * n + r0;
-      double s0 = 0.0, s1 = 0.0;
-      for (int k = 0; k < bs; ++k) {
-        const double l = Ai[k];
-        s0 += l * xb[0][k];
-        s1 += l * xb[1][k];
+      if (c < bs) {
+        // y_c = tmp_c / L_cc + sum_{k < c} inv(L)[c][k] tmp_k
+        double sacc = tmp[comp][c] / dblk[c][c];
+#pragma unroll
+        for (int k = 0; k < kNB; ++k)
+          if (k < c) sacc += dblk[k][c] * tmp[comp][k];
+        (comp ? x1 : x0)[r0 + c] = sacc;
       }
-      x[0][3 + i] -= s0;
-      x[1][3 + i] -= s1;
     }
     __syncthreads();
   }
-  // backward: L^T z = y
+  // backward: z_J = inv(L_JJ)^T (y_J - L(r0+bs:m, J)^T z(r0+bs:m))
   for (int J = nblk - 1; J >= 0; --J) {
     const int r0 = J * kNB;
     const int bs = min(kNB, m - r0);
-    if (tid < 64) {
-      const int c = tid & 31, comp = tid >> 5;
+    for (int e = tid; e < kNB * kNB; e += kThreads) {
+      const int i = e / kNB, c = e % kNB;
+      dblk[i][c] = (i < bs && c < bs) ? A[int64_t(r0 + i) * n + r0 + c] : 0.0;
+    }
+    {
+      // thread = (column c, row group g): rows r0 + bs + g, + 8, ...
+      const int c = tid & 31, g = tid >> 5;
+      double s0 = 0.0, s1 = 0.0;
       if (c < bs) {
-        const double* y = x[comp] + 3 + r0;
-        const double* row = A + int64_t(r0 + c) * n + r0;
-        double s = y[c] / row[c];
-        for (int k = c + 1; k < bs; ++k) s += row[k] * y[k];
-        xb[comp][c] = s;
+        for (int i = r0 + bs + g; i < m; i += 8) {
+          const double l = A[int64_t(i) * n + r0 + c];
+          s0 += l * x0[i];
+          s1 += l * x1[i];
+        }
       }
+      part[0][g][c] = s0;
+      part[1][g][c] = s1;
     }
     __syncthreads();
     if (tid < 64) {
       const int c = tid & 31, comp = tid >> 5;
-      if (c < bs) x[comp][3 + r0 + c] = xb[comp][c];
+      double sacc = 0.0;
+#pragma unroll
+      for (int g = 0; g < 8; ++g) sacc += part[comp][g][c];
+      tmp[comp][c] = c < bs ? (comp ? x1 : x0)[r0 + c] - sacc : 0.0;
     }
-    for (int j = tid; j < r0; j += kThreads) {
-      double s0 = 0.0, s1 = 0.0;
-      for (int i = 0; i < bs; ++i) {
-        const double l = A[int64_t(r0 + i) * n + j];
-        s0 += l * xb[0][i];
-        s1 += l * xb[1][i];
+    __syncthreads();
+    if (tid < 64) {
+      const int c = tid & 31, comp = tid >> 5;
+      if (c < bs) {
+        // z_c = tmp_c / L_cc + sum_{k > c} inv(L)[k][c] tmp_k
+        double sacc = tmp[comp][c] / dblk[c][c];
+#pragma unroll
+        for (int k = 0; k < kNB; ++k)
+          if (k > c && k < bs) sacc += dblk[c][k] * tmp[comp][k];
+        (comp ? x1 : x0)[r0 + c] = sacc;
       }
-      x[0][3 + j] -= s0;
-      x[1][3 + j] -= s1;
     }
     __syncthreads();
   }
