@@ -1,20 +1,29 @@
 """Benchmark of the PIV path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multipass|single]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs P]
+                    [--workload multipass|single]
 
-Workload "multipass" (default): one 2048x2048 uint16 pair (12-bit, 0.03
-particles/px, 8 Fourier modes up to 6 px, 2 % of the 64 px windows replaced
-by noise), WorkPIV with 3 passes (64, 32, 16 px windows, 50 % overlap),
-ThinPlateSplineSubdom between passes and at the end (use_tps True,
-subdom_size 200, smoothing_coef 2.0, threshold_tps 1.5), WorkFIX after
-every pass, 2d_gaussian2 on the last pass, correlation maps kept (the
-reference's results carry them). One step = the whole multipass of the pair
-(image minima, window records, correlation, fix, FP64 TPS, final
+Workload "multipass" (default): P seeded 2048x2048 uint16 pairs (12-bit,
+0.03 particles/px, 8 Fourier modes up to 6 px, 2 % of the 64 px windows of
+im1 replaced by noise), WorkPIV with 3 passes (64, 32, 16 px windows, 50 %
+overlap), ThinPlateSplineSubdom between passes and at the end (use_tps True,
+subdom_size 200, smoothing_coef 2.0, threshold_tps 1.5), WorkFIX after every
+pass, 2d_gaussian2 on the last pass, correlation maps kept (the reference's
+results carry them). One step = the whole multipass of the P pairs as one
+batch (image minima, window records, one correlation launch per pass over
+the windows of every pair, fix, one batched FP64 TPS per pass, final
 interpolation); the images are resident in HBM before the timed region.
-value = final-pass vectors (65,025) per second.
+value = final-pass vectors (65,025 per pair) x P per second. The same
+pipeline on one pair (the plain per-pair path) is timed in the same run.
 
-Workload "single": the single-pass correlation of the same kind of pair
-(32x32 windows, displacement_max 8, 16,129 windows).
+Workload "single": the single-pass correlation of the same kind of pairs
+(32x32 windows, displacement_max 8, 16,129 windows per pair).
+
+The roofline object is the longest correlation launch of a step (the stage
+whose bytes are the algorithmic HBM figure: unique image bytes + maps +
+16 B peak records), timed as back-to-back relaunches between two HIP events
+on the launch stream. traffic comes from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+of that kernel when one exists for the workload.
 
 Prints one JSON line. --copy-calibration runs a plain 256 MiB device copy,
 the reference for scaling the FETCH_SIZE / WRITE_SIZE counters.
@@ -23,6 +32,7 @@ the reference for scaling the FETCH_SIZE / WRITE_SIZE counters.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -32,6 +42,11 @@ if ROOT not in sys.path:
 
 METRIC = "vectors/s, 2048² multipass PIV: batched vs plain HIP at <=1e-3 px; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 counter scaling measured with --copy-calibration (256 MiB copy):
+# FETCH_SIZE reads 131,085 KiB per dispatch (half the bytes moved),
+# WRITE_SIZE 262,144 KiB
+FETCH_SCALE = 2.0
+WRITE_SCALE = 1.0
 
 
 def parse():
@@ -39,8 +54,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--workload", default="multipass", choices=["multipass", "single"])
     ap.add_argument("--no-maps", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the one-pair (plain per-pair path) measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (5 times), the "
@@ -76,7 +94,7 @@ def multipass_params():
 
 def cpu_baseline_multipass(im0, im1):
     """The oracle (numpy/pocketfft float32 + LAPACK restatement of the
-    reference) on the whole pair, one thread."""
+    reference) on one pair, one thread."""
     from threadpoolctl import threadpool_limits
 
     from oracle.piv import OracleWorkPIV, default_params
@@ -95,7 +113,7 @@ def cpu_baseline_multipass(im0, im1):
         "unit": "vectors/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"the whole workload (one 2048x2048 pair, 3 passes + TPS, {n} final "
+        "sample": f"one pair of the workload (one 2048x2048 pair, 3 passes + TPS, {n} final "
         f"vectors), oracle float32 FFT + LAPACK TPS, {dt:.1f} s on 1 thread",
     }
 
@@ -116,15 +134,17 @@ def cpu_baseline_single(im0, im1):
         dt = time.perf_counter() - t0
     n = len(res["deltaxs"])
     return {"value": round(n / dt, 1), "unit": "vectors/s", "cores": 1, "kind": "port",
-            "sample": f"the whole workload ({n} windows), oracle float32, {dt:.1f} s on 1 thread"}
+            "sample": f"one pair of the workload ({n} windows), oracle float32, {dt:.1f} s "
+                      "on 1 thread"}
 
 
-def correl_bytes(work, imshape, itemsize, nwin, maps):
+def correl_bytes(work, imshape, itemsize, nwin, npairs, maps):
     """Algorithmic bytes of one correlation launch: unique image bytes
-    (2 H W b) + maps (nwin n0 n1 4, when written) + 16 B peak record."""
+    (2 H W b per pair) + maps (nwin n0 n1 4, when written) + 16 B peak
+    record per window."""
     H, W = imshape
     n0, n1 = work.shape_crop_im0
-    return 2 * H * W * itemsize + (nwin * n0 * n1 * 4 if maps else 0) + 16 * nwin
+    return 2 * H * W * itemsize * npairs + (nwin * n0 * n1 * 4 if maps else 0) + 16 * nwin
 
 
 def correl_flops(work, nwin):
@@ -133,6 +153,21 @@ def correl_flops(work, nwin):
     n0, n1 = work.shape_crop_im0
     n = n0 * n1
     return nwin * (7.5 * n * np.log2(n) + 14 * n0 * (n1 // 2 + 1))
+
+
+def traffic_from_profile(kernel_tag):
+    """HBM bytes per launch from the committed PMC summary of the kernel
+    (lines 'FETCH_SIZE_KiB_per_dispatch: X' and 'WRITE_SIZE_KiB_per_dispatch: Y'),
+    scaled by the copy calibration; None without a summary."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    f = re.search(r"FETCH_SIZE_KiB_per_dispatch:\s*([0-9.]+)", text)
+    w = re.search(r"WRITE_SIZE_KiB_per_dispatch:\s*([0-9.]+)", text)
+    if not (f and w):
+        return None
+    return int((float(f.group(1)) * FETCH_SCALE + float(w.group(1)) * WRITE_SCALE) * 1024)
 
 
 def timed(step, steps, warmup):
@@ -151,6 +186,61 @@ def timed(step, steps, warmup):
     return wall, avg
 
 
+def relaunch_ms(fn, reps=20):
+    """Average duration of fn's launch over reps back-to-back launches
+    between two events on the current stream (the launch stream)."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def make_pipeline(args, shape):
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+    from fluidimage_amd.works.piv.pipeline import MultipassPipeline
+
+    if args.workload == "multipass":
+        params = multipass_params()
+    else:
+        params = WorkPIV.create_default_params()
+        params.piv0.shape_crop_im0 = 32
+        params.piv0.displacement_max = 8
+    work = WorkPIV(params)
+    work._prepare_with_image(imshape=shape)
+    return work, MultipassPipeline(work)
+
+
+def run_bench(args, pairs, keep_maps, steps, warmup):
+    """Time `steps` pipeline steps over the pairs as one batch."""
+    import torch
+
+    from fluidimage_amd.device import ImageSet, to_device_image
+
+    dev = torch.device("cuda:0")
+    work, pipe = make_pipeline(args, pairs[0][0].shape)
+    ims = []
+    for a, b in pairs:
+        ims += [to_device_image(a, dev), to_device_image(b, dev)]
+    images = ImageSet(ims)
+    state = {}
+
+    def step(timers):
+        images.recompute_min()
+        state["passes"] = pipe.run(images, len(pairs), keep_maps=keep_maps, timers=timers,
+                                   final_interp=args.workload == "multipass")
+
+    wall, avg = timed(step, steps, warmup)
+    return work, images, state["passes"], wall / steps * 1e3, avg
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -162,71 +252,71 @@ def main():
         sys.exit(2)
     import torch
 
-    from fluidimage_amd.device import ImageSet, to_device_image
     from fluidimage_amd.synthetic import config_pair
-    from fluidimage_amd.works.piv.multipass import WorkPIV
-    from fluidimage_amd.works.piv.pipeline import MultipassPipeline
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
+    torch.cuda.set_device(0)
     keep_maps = not args.no_maps
     cfg = 3 if args.workload == "multipass" else 2
-    print(f"bench: generating the 2048x2048 uint16 pair (config {cfg})", file=sys.stderr,
+    npairs = max(1, args.pairs)
+    print(f"bench: generating {npairs} 2048x2048 uint16 pairs (config {cfg})", file=sys.stderr,
           flush=True)
-    im0, im1 = config_pair(cfg)
-    if args.workload == "multipass":
-        params = multipass_params()
-    else:
-        params = WorkPIV.create_default_params()
-        params.piv0.shape_crop_im0 = 32
-        params.piv0.displacement_max = 8
-    work = WorkPIV(params)
-    work._prepare_with_image(imshape=im0.shape)
-    images = ImageSet([to_device_image(im0, dev), to_device_image(im1, dev)])
-    pipe = MultipassPipeline(work)
-    state = {}
+    pairs = [config_pair(cfg, pair_idx=k) for k in range(npairs)]
+    im0 = pairs[0][0]
 
-    def step(timers):
-        images.recompute_min()
-        state["passes"] = pipe.run(images, 1, keep_maps=keep_maps, timers=timers,
-                                   final_interp=args.workload == "multipass")
-
-    print("bench: warmup + timed steps", file=sys.stderr, flush=True)
-    wall, avg = timed(step, args.steps, args.warmup)
-    ms_step = wall / args.steps * 1e3
+    print("bench: batched steps", file=sys.stderr, flush=True)
+    work, images, passes, ms_step, avg = run_bench(args, pairs, keep_maps, args.steps,
+                                                   args.warmup)
     last = work.works_piv[-1]
     nvec = last.nb_vectors
-    passes = state["passes"]
     nerr = int((passes[-1].out["err"] != 0).sum().item())
-    dom_key = max(avg, key=avg.get)
+
+    # correlation launches: event-bracketed per pass in the steps, and the
+    # longest one relaunched back to back for its clean average duration
     per_pass = {}
     for ip, w in enumerate(work.works_piv):
         key = f"correlate_pass{ip}"
         if key in avg:
-            b = correl_bytes(w, im0.shape, im0.dtype.itemsize, w.nb_vectors, keep_maps)
-            per_pass[key] = {
-                "window": list(w.shape_crop_im0),
-                "windows": int(w.nb_vectors),
-                "ms": round(avg[key], 4),
-                "GB/s": round(b / (avg[key] / 1e3) / 1e9, 1),
-                "valu_TFLOP/s": round(correl_flops(w, w.nb_vectors) / (avg[key] / 1e3) / 1e12,
-                                      2),
-            }
-    # roofline of the longest correlation launch (HBM-bound with maps)
-    ck = max(per_pass, key=lambda k: per_pass[k]["ms"])
-    cw = work.works_piv[int(ck[-1])]
-    cbytes = correl_bytes(cw, im0.shape, im0.dtype.itemsize, cw.nb_vectors, keep_maps)
-    achieved = cbytes / (avg[ck] / 1e3) / 1e9
+            nwin = w.nb_vectors * npairs
+            b = correl_bytes(w, im0.shape, im0.dtype.itemsize, nwin, npairs, keep_maps)
+            per_pass[key] = {"window": list(w.shape_crop_im0), "windows": int(nwin),
+                             "ms_in_step": round(avg[key], 4)}
+    ck = max(per_pass, key=lambda k: per_pass[k]["ms_in_step"])
+    cip = int(ck[len("correlate_pass"):])
+    cw = work.works_piv[cip]
+    po = passes[cip]
+    nwin = cw.nb_vectors * npairs
+    maps = po.out["maps"]
+
+    def relaunch():
+        cw.correl.correlate_windows(images.table, po.out["windows"], nwin, images.dtype,
+                                    cw.has_to_apply_subpix, maps=maps)
+
+    k_ms = relaunch_ms(relaunch)
+    cbytes = correl_bytes(cw, im0.shape, im0.dtype.itemsize, nwin, npairs, keep_maps)
+    achieved = cbytes / (k_ms / 1e3) / 1e9
+    n0, n1 = cw.shape_crop_im0
+    dt_tag = {1: "u8", 2: "u16", 4: "f32"}[im0.dtype.itemsize]
+    ktag = f"k_correlate_{n0}x{n1}_{dt_tag}_2048_{args.workload}_{npairs}pairs"
+    traffic = traffic_from_profile(ktag)
+    for key, v in per_pass.items():
+        w = work.works_piv[int(key[len("correlate_pass"):])]
+        b = correl_bytes(w, im0.shape, im0.dtype.itemsize, v["windows"], npairs, keep_maps)
+        v["GB/s_in_step"] = round(b / (v["ms_in_step"] / 1e3) / 1e9, 1)
+        v["valu_TFLOP/s_in_step"] = round(
+            correl_flops(w, v["windows"]) / (v["ms_in_step"] / 1e3) / 1e12, 2)
+
     if args.workload == "multipass":
-        desc = ("multipass PIV of one 2048x2048 uint16 pair: 64/32/16 px windows, 50% "
-                "overlap, WorkFIX, GPU FP64 ThinPlateSplineSubdom (subdom 200, smoothing 2.0) "
-                "between passes and at the end, 2d_gaussian2, maps ")
+        desc = ("multipass PIV of {p} 2048x2048 uint16 pairs as one batch: 64/32/16 px "
+                "windows, 50% overlap, WorkFIX, GPU FP64 ThinPlateSplineSubdom (subdom 200, "
+                "smoothing 2.0, threshold 1.5) between passes and at the end, 2d_gaussian2, "
+                "maps ").format(p=npairs)
     else:
-        desc = ("single-pass batched correlation of one 2048x2048 uint16 pair: 32x32 "
-                "windows 50% overlap, displacement_max 8, 2d_gaussian2, maps ")
+        desc = ("single-pass batched correlation of {p} 2048x2048 uint16 pairs: 32x32 "
+                "windows 50% overlap, displacement_max 8, 2d_gaussian2, maps ").format(p=npairs)
+    value = nvec * npairs / (ms_step / 1e3)
     line = {
         "metric": METRIC,
-        "value": round(nvec / (ms_step / 1e3), 1),
+        "value": round(value, 1),
         "unit": "vectors/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -239,12 +329,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": desc + ("materialised" if keep_maps else "off"),
-            "final_vectors": int(nvec),
-            "pairs": 1,
+            "pairs": npairs,
+            "final_vectors_per_pair": int(nvec),
             "final_window_errors": nerr,
-            "stage_ms": {k: round(v, 4) for k, v in sorted(avg.items())},
+            "stage_ms_in_step": {k: round(v, 4) for k, v in sorted(avg.items())},
             "correlation_passes": per_pass,
-            "dominant": dom_key,
         },
         "roofline": {
             "bound": "hbm",
@@ -252,18 +341,30 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": f"k_correlate<{cw.shape_crop_im0[0]},{cw.shape_crop_im0[1]},u16> ({ck})",
-            "kernel_ms": round(avg[ck], 4),
+            "traffic": traffic,
+            "kernel": f"k_correlate<{n0},{n1},{dt_tag}> ({ck}, {nwin} windows)",
+            "kernel_ms": round(k_ms, 4),
             "bytes_per_launch": int(cbytes),
+            "profile_tag": ktag,
         },
     }
+
+    if not args.no_plain and npairs > 1:
+        print("bench: plain per-pair steps", file=sys.stderr, flush=True)
+        _, _, _, ms1, _ = run_bench(args, pairs[:1], keep_maps, max(3, args.steps // 2),
+                                    args.warmup)
+        plain = nvec / (ms1 / 1e3)
+        line["config"]["plain_per_pair"] = {"ms_per_step": round(ms1, 3),
+                                            "vectors/s": round(plain, 1)}
+        line["config"]["batched_over_plain"] = round(value / plain, 2)
+
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
+        a, b = pairs[0]
         if args.workload == "multipass":
-            line["cpu_baseline"] = cpu_baseline_multipass(im0, im1)
+            line["cpu_baseline"] = cpu_baseline_multipass(a, b)
         else:
-            line["cpu_baseline"] = cpu_baseline_single(im0, im1)
+            line["cpu_baseline"] = cpu_baseline_single(a, b)
     print(json.dumps(line), flush=True)
 
 

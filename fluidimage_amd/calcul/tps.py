@@ -67,69 +67,168 @@ def tps_device(dx, dy, xs, ys, new_x, new_y, subdom_size, smoothing_coef, thresh
                percent_buffer_area=20, stream=None, keep=False):
     """TPS smoothing of the finite vectors of (dx, dy) at (xs, ys) and
     evaluation at (new_x, new_y); all tensors on the device (dx, dy float32,
-    positions float64). Two small device-to-host reads size the problem.
+    positions float64). Small device-to-host reads size the problem.
 
     Returns a dict of device tensors: ``smoothed`` (2, count) at the
     selected centers, ``index`` (count) of the selected vectors, ``approx``
     (2, M) at the new positions, plus host ``nb_subdom``, ``stats`` (nd, 6).
     """
+    out = tps_device_batch([(dx, dy, xs, ys)], new_x, new_y, subdom_size, smoothing_coef,
+                           threshold, percent_buffer_area, stream=stream, keep=keep)[0]
+    if isinstance(out, Exception):
+        raise out
+    return out
+
+
+class _Field:
+    """Host bookkeeping of one vector field of a batch."""
+
+    def __init__(self, k, n, base):
+        self.k = k
+        self.n = n          # vectors of the field
+        self.base = base    # position of its first center in the batch arrays
+        self.error = None
+
+
+def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshold,
+                     percent_buffer_area=20, stream=None, keep=False):
+    """``tps_device`` for several vector fields at once (one image pair
+    each): each field gets its own subdomain grid (thin_plate_spline_subdom.py:53-90),
+    and the subdomains of all fields go through one launch of every
+    per-subdomain kernel. ``fields`` is a list of (dx, dy, xs, ys) device
+    tensors; (new_x, new_y) are the new positions, the same for every field.
+
+    Returns one entry per field: the dict of ``tps_device`` or the
+    ``SingularSystemError`` of that field alone ("Only nan." when it has no
+    finite vector).
+    """
     import torch
 
     lib = _native.load()
-    dev = dx.device
+    dev = fields[0][0].device
     sp = _native.stream_ptr(stream)
-    n = int(dx.numel())
     f64 = dict(dtype=torch.float64, device=dev)
-    cx = torch.empty(n, **f64)
-    cy = torch.empty(n, **f64)
-    vx = torch.empty(n, dtype=torch.float32, device=dev)
-    vy = torch.empty(n, dtype=torch.float32, device=dev)
-    idx = torch.empty(n, dtype=torch.int32, device=dev)
-    info = torch.empty(5, **f64)
-    _native.check(lib.piv_tps_select(n, _p(dx), _p(dy), _p(xs), _p(ys), _p(cx), _p(cy),
-                                     _p(vx), _p(vy), _p(idx), _p(info), sp), "piv_tps_select")
-    count, x_min, x_max, y_min, y_max = info.cpu().numpy().tolist()
-    count = int(count)
-    if count == 0:
-        raise SingularSystemError("Only nan.")
-    nsx, nsy, smoothing, lim_h = subdomain_grid(count, x_min, x_max, y_min, y_max,
-                                                subdom_size, smoothing_coef,
-                                                percent_buffer_area)
-    nd = nsx * nsy
-    lim = torch.from_numpy(lim_h).to(dev)
-    counts = torch.empty(nd, dtype=torch.int32, device=dev)
-    _native.check(lib.piv_tps_count(nsx, nsy, _p(lim), count, _p(cx), _p(cy), _p(counts), sp),
-                  "piv_tps_count")
+    i32 = dict(dtype=torch.int32, device=dev)
+    thr = -1.0 if threshold is None else float(threshold)
+    flist = []
+    base = 0
+    for k, f in enumerate(fields):
+        flist.append(_Field(k, int(f[0].numel()), base))
+        base += flist[-1].n
+    ntot = base
+    cx = torch.empty(ntot, **f64)
+    cy = torch.empty(ntot, **f64)
+    vx = torch.empty(ntot, dtype=torch.float32, device=dev)
+    vy = torch.empty(ntot, dtype=torch.float32, device=dev)
+    idx = torch.empty(ntot, **i32)
+    info = torch.empty((len(fields), 5), **f64)
+    for f, (dx, dy, xs, ys) in zip(flist, fields):
+        b = f.base
+        if f.n == 0:
+            info[f.k].zero_()
+            continue
+        _native.check(lib.piv_tps_select(f.n, _p(dx), _p(dy), _p(xs), _p(ys), _p(cx[b:]),
+                                         _p(cy[b:]), _p(vx[b:]), _p(vy[b:]), _p(idx[b:]),
+                                         _p(info[f.k]), sp), "piv_tps_select")
+    info_h = info.cpu().numpy()
+    # subdomain grids (host scalars)
+    active = []
+    for f in flist:
+        count, x_min, x_max, y_min, y_max = info_h[f.k].tolist()
+        f.count = int(count)
+        if f.count == 0:
+            f.error = SingularSystemError("Only nan.")
+            continue
+        try:
+            f.nsx, f.nsy, f.smoothing, f.lim_h = subdomain_grid(
+                f.count, x_min, x_max, y_min, y_max, subdom_size, smoothing_coef,
+                percent_buffer_area)
+        except SingularSystemError as e:
+            f.error = e
+            continue
+        f.nd = f.nsx * f.nsy
+        active.append(f)
+    results = _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
+    out = []
+    for f in flist:
+        if f.error is not None:
+            out.append(f.error)
+        else:
+            r = results[f.k]
+            r["index"] = idx[f.base:f.base + f.count]
+            out.append(r)
+    return out
+
+
+def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep):
+    """Membership, FP64 solves, blend and evaluation for the fields whose
+    grid is valid; fields with an empty subdomain get their error and the
+    others are solved again without them."""
+    import torch
+
+    f64 = dict(dtype=torch.float64, device=dev)
+    i32 = dict(dtype=torch.int32, device=dev)
+    if not active:
+        return {}
+    nd_base = 0
+    for f in active:
+        f.nd_base = nd_base
+        nd_base += f.nd
+    nd = nd_base
+    lim = torch.from_numpy(np.concatenate([f.lim_h for f in active])).to(dev)
+    lim_off = np.concatenate([[0], np.cumsum([f.lim_h.size for f in active])])
+    counts = torch.empty(nd, **i32)
+    for j, f in enumerate(active):
+        _native.check(lib.piv_tps_count(f.nsx, f.nsy, _p(lim[lim_off[j]:]), f.count,
+                                        _p(cx[f.base:]), _p(cy[f.base:]),
+                                        _p(counts[f.nd_base:]), sp), "piv_tps_count")
     sizes = counts.cpu().numpy().astype(np.int64)
-    if sizes.min() < 1:
-        raise SingularSystemError("empty subdomain")
+    retry = False
+    for f in active:
+        sz = sizes[f.nd_base:f.nd_base + f.nd]
+        if sz.min() < 1:
+            f.error = SingularSystemError("empty subdomain")
+            retry = True
+        elif int(sz.max()) > MAX_CENTERS_PER_SUBDOM:
+            raise NotImplementedError(
+                f"{int(sz.max())} centers in a TPS subdomain (limit {MAX_CENTERS_PER_SUBDOM}): "
+                "use a smaller params.multipass.subdom_size")
+    if retry:
+        return _solve_active(lib, [f for f in active if f.error is None], cx, cy, vx, vy,
+                             new_x, new_y, thr, dev, sp, keep)
     n_max = int(sizes.max())
-    if n_max > MAX_CENTERS_PER_SUBDOM:
-        raise NotImplementedError(
-            f"{n_max} centers in a TPS subdomain (limit {MAX_CENTERS_PER_SUBDOM}): "
-            "use a smaller params.multipass.subdom_size")
     dom_off_h = np.concatenate([[0], np.cumsum(sizes)])
     mat_off_h = np.concatenate([[0], np.cumsum(sizes**2)])
     nslots = int(dom_off_h[-1])
     dom_off = torch.from_numpy(dom_off_h).to(dev)
     mat_off = torch.from_numpy(mat_off_h).to(dev)
-    slot_pt = torch.empty(nslots, dtype=torch.int32, device=dev)
-    _native.check(lib.piv_tps_fill(nsx, nsy, _p(lim), count, _p(cx), _p(cy), _p(dom_off),
-                                   _p(slot_pt), sp), "piv_tps_fill")
-    status = torch.zeros(nd + 1, dtype=torch.int32, device=dev)
-    c_slot = torch.empty(count * MAX_SLOTS, dtype=torch.int32, device=dev)
-    c_coef = torch.empty(count * MAX_SLOTS, **f64)
-    c_norm = torch.empty(count, **f64)
-    _native.check(lib.piv_tps_point_slots(nsx, nsy, _p(lim), count, _p(cx), _p(cy), _p(dom_off),
-                                          _p(slot_pt), _p(c_slot), _p(c_coef), _p(c_norm),
-                                          _p(status[nd:]), sp), "piv_tps_point_slots")
+    smoothing = torch.from_numpy(
+        np.concatenate([np.full(f.nd, f.smoothing) for f in active])).to(dev)
+    slot_pt = torch.empty(nslots, **i32)
+    status = torch.zeros(nd + 1, **i32)
+    ntot = int(cx.numel())
+    c_slot = torch.empty(ntot * MAX_SLOTS, **i32)
+    c_coef = torch.empty(ntot * MAX_SLOTS, **f64)
+    c_norm = torch.empty(ntot, **f64)
     m = int(new_x.numel())
-    n_dom = torch.empty(m * MAX_SLOTS, dtype=torch.int32, device=dev)
-    n_coef = torch.empty(m * MAX_SLOTS, **f64)
-    n_norm = torch.empty(m, **f64)
-    _native.check(lib.piv_tps_point_slots(nsx, nsy, _p(lim), m, _p(new_x), _p(new_y), None, None,
-                                          _p(n_dom), _p(n_coef), _p(n_norm), _p(status[nd:]),
-                                          sp), "piv_tps_point_slots")
+    na = len(active)
+    n_dom = torch.empty(na * m * MAX_SLOTS, **i32)
+    n_coef = torch.empty(na * m * MAX_SLOTS, **f64)
+    n_norm = torch.empty(na * m, **f64)
+    for j, f in enumerate(active):
+        lim_j = _p(lim[lim_off[j]:])
+        b = f.base
+        _native.check(lib.piv_tps_fill(f.nsx, f.nsy, lim_j, f.count, _p(cx[b:]), _p(cy[b:]),
+                                       _p(dom_off[f.nd_base:]), _p(slot_pt), b, sp),
+                      "piv_tps_fill")
+        _native.check(lib.piv_tps_point_slots(
+            f.nsx, f.nsy, lim_j, f.count, _p(cx[b:]), _p(cy[b:]), _p(dom_off[f.nd_base:]),
+            _p(slot_pt), b, 0, _p(c_slot[MAX_SLOTS * b:]), _p(c_coef[MAX_SLOTS * b:]),
+            _p(c_norm[b:]), _p(status[nd:]), sp), "piv_tps_point_slots")
+        _native.check(lib.piv_tps_point_slots(
+            f.nsx, f.nsy, lim_j, m, _p(new_x), _p(new_y), None, None, 0, f.nd_base,
+            _p(n_dom[MAX_SLOTS * m * j:]), _p(n_coef[MAX_SLOTS * m * j:]),
+            _p(n_norm[m * j:]), _p(status[nd:]), sp), "piv_tps_point_slots")
     mats = torch.empty(int(mat_off_h[-1]), **f64)
     refl = torch.empty((nd, 16), **f64)
     vwork = torch.empty(nslots * 6, **f64)
@@ -137,34 +236,48 @@ def tps_device(dx, dy, xs, ys, new_x, new_y, subdom_size, smoothing_coef, thresh
     wstride = nslots + 3 * nd
     weights = torch.empty((2, wstride), **f64)
     stats = torch.zeros((nd, 6), **f64)
-    thr = -1.0 if threshold is None else float(threshold)
     _native.check(
         lib.piv_tps_smooth(nd, n_max, _p(cx), _p(cy), _p(vx), _p(vy), _p(dom_off), _p(slot_pt),
-                           _p(mat_off), float(smoothing), thr, _p(mats), _p(refl), _p(vwork),
+                           _p(mat_off), _p(smoothing), thr, _p(mats), _p(refl), _p(vwork),
                            _p(smooth_slots), nslots, _p(weights), wstride, _p(stats),
                            _p(status), sp),
         "piv_tps_smooth")
-    smoothed = torch.empty((2, count), **f64)
-    _native.check(lib.piv_tps_blend(count, _p(c_slot), _p(c_coef), _p(c_norm), _p(smooth_slots),
-                                    nslots, _p(smoothed), sp), "piv_tps_blend")
-    approx = torch.empty((2, m), **f64)
-    _native.check(lib.piv_tps_interpolate(m, _p(new_x), _p(new_y), _p(n_dom), _p(n_coef),
+    smoothed = torch.empty((2, ntot), **f64)
+    for f in active:
+        b = f.base
+        sm = smoothed[:, b:b + f.count]
+        blend = torch.empty((2, f.count), **f64)
+        _native.check(lib.piv_tps_blend(f.count, _p(c_slot[MAX_SLOTS * b:]),
+                                        _p(c_coef[MAX_SLOTS * b:]), _p(c_norm[b:]),
+                                        _p(smooth_slots), nslots, _p(blend), sp),
+                      "piv_tps_blend")
+        sm.copy_(blend)
+    px = new_x.repeat(na)
+    py = new_y.repeat(na)
+    approx = torch.empty((2, na * m), **f64)
+    _native.check(lib.piv_tps_interpolate(na * m, _p(px), _p(py), _p(n_dom), _p(n_coef),
                                           _p(n_norm), _p(cx), _p(cy), _p(dom_off), _p(slot_pt),
                                           _p(weights), wstride, _p(approx), sp),
                   "piv_tps_interpolate")
     st = status.cpu().numpy()
     if st[nd]:
         raise NotImplementedError("a point lies in more than 4 TPS subdomains")
-    if st[:nd].any():
-        raise SingularSystemError(
-            f"{int((st[:nd] != 0).sum())} subdomain system(s) not positive definite")
-    out = {"smoothed": smoothed, "index": idx[:count], "approx": approx, "nb_subdom": nd,
-           "nsx": nsx, "nsy": nsy, "smoothing_coef": smoothing, "count": count}
-    if keep:
-        out.update(stats=stats, weights=weights, dom_off=dom_off_h, slot_pt=slot_pt)
-    else:
-        out["stats"] = stats
-    return out
+    results = {}
+    for j, f in enumerate(active):
+        bad = st[f.nd_base:f.nd_base + f.nd]
+        if bad.any():
+            f.error = SingularSystemError(
+                f"{int((bad != 0).sum())} subdomain system(s) not positive definite")
+            continue
+        r = {"smoothed": smoothed[:, f.base:f.base + f.count],
+             "approx": approx[:, m * j:m * (j + 1)],
+             "nb_subdom": f.nd, "nsx": f.nsx, "nsy": f.nsy, "smoothing_coef": f.smoothing,
+             "count": f.count, "stats": stats[f.nd_base:f.nd_base + f.nd]}
+        if keep:
+            # single-field use: subdomain offsets and center indices are the field's own
+            r.update(weights=weights, dom_off=dom_off_h, slot_pt=slot_pt)
+        results[f.k] = r
+    return results
 
 
 def summaries(stats):

@@ -65,7 +65,8 @@ __device__ double block_max(double v, double* red) {
 __global__ __launch_bounds__(kThreads) void k_tps_prep(
     const double* __restrict__ xs, const double* __restrict__ ys,
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
-    const int64_t* __restrict__ mat_off, double smoothing, double* __restrict__ mats,
+    const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
+    double* __restrict__ mats,
     double* __restrict__ refl, double* __restrict__ vwork, int32_t* __restrict__ status) {
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
       if (j < n) {
         const double dy = cy[j] - yi, dx = cx[j] - xi;
         double b = tps_phi(dy * dy + dx * dx);
-        if (i == j) b += smoothing;
+        if (i == j) b += smoothing_d[d];
         B[int64_t(i) * n + j] = b;
         yacc[c][0] += b * v0;
         yacc[c][1] += b * v1;
@@ -264,8 +265,6 @@ __device__ __forceinline__ dvec4 mfma_f64(double a, double b, dvec4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// NT threads per subdomain: 512 when every subdomain has a CU of its own,
-// 256 (two workgroups per CU) otherwise
 template <int NT>
 __global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom_off,
                                                            const int64_t* __restrict__ mat_off,
@@ -282,6 +281,8 @@ __global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom
   const int lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;  // MFMA operand row / k group
   __shared__ double inv[kNB][kNB + 1];       // inv(L_JJ)
+  __shared__ double lbuf[kNB][kNB + 1];      // L_JJ
+  __shared__ double rdiag[kNB];              // 1 / L_ii
   __shared__ int bad;
   if (tid == 0) bad = 0;
   // B'[3:, :3] (kept for the solves)
@@ -307,19 +308,30 @@ __global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom
       const double* pb0 = A + int64_t(rb0) * n + 4 * lk;
       const double* pb1 = A + int64_t(rb1) * n + 4 * lk;
       dvec4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
-      // k = k0 + 4 lk + q: the lane reads 4 consecutive columns per operand
-      for (int k0 = 0; k0 < r0; k0 += 16) {
-        double a4[4], b4[4], e4[4];
+      // k = k0 + 4 lk + q: the lane reads 4 consecutive columns per operand;
+      // the operands of step k0 + 32 are requested while step k0 runs
+      double op[3][12];
+      auto fetch = [&](double* o, int k0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          a4[q] = pa[k0 + q];
-          b4[q] = pb0[k0 + q];
-          e4[q] = pb1[k0 + q];
+          o[q] = pa[k0 + q];
+          o[4 + q] = pb0[k0 + q];
+          o[8 + q] = pb1[k0 + q];
+        }
+      };
+      if (r0 > 0) fetch(op[0], 0);
+      if (r0 > 16) fetch(op[1], 16);
+      for (int k0 = 0; k0 < r0; k0 += 16) {
+        if (k0 + 32 < r0) fetch(op[2], k0 + 32);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          c0 = mfma_f64(op[0][q], op[0][4 + q], c0);
+          c1 = mfma_f64(op[0][q], op[0][8 + q], c1);
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          c0 = mfma_f64(a4[q], b4[q], c0);
-          c1 = mfma_f64(a4[q], e4[q], c1);
+        for (int e = 0; e < 12; ++e) {
+          op[0][e] = op[1][e];
+          op[1][e] = op[2][e];
         }
       }
       // C(row, col) - wz - update, rows 4 v + lk, columns li (+16)
@@ -335,63 +347,64 @@ __global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom
       }
     }
     __syncthreads();
-    // b) diagonal block in the registers of wavefront 0 (lane i < 32 holds
-    //    row i of the block in acc and row i of inv(L_JJ) in x):
-    //    right-looking factorisation, each step also eliminating column j
-    //    from the rows of inv(L_JJ) below j; entries of other lanes come
-    //    through readlane (uniform values)
+    // b) diagonal block by wavefront 0: right-looking factorisation in
+    //    registers (lane i < 32 holds row i; entries of other lanes through
+    //    readlane), then column `lane` of inv(L_JJ) by forward substitution
+    //    against L_JJ in LDS
     if (wave == 0) {
       const bool drow = lane < kNB;
-      double acc[kNB];
+      bool singular = false;
       {
+        double acc[kNB];
         const double* src = A + int64_t(r0 + (lane < bs ? lane : 0)) * n + r0;
 #pragma unroll
         for (int c = 0; c < kNB; ++c) {
           const double v = src[c < bs ? c : 0];
           acc[c] = (lane < bs && c <= lane) ? v : 0.0;
         }
-      }
-      bool singular = false;
-      double x[kNB];
 #pragma unroll
-      for (int c = 0; c < kNB; ++c) x[c] = c == lane ? 1.0 : 0.0;
-#pragma unroll
-      for (int j = 0; j < kNB; ++j) {
-        double dj = readlane_d(acc[j], j);
-        if (j >= bs) dj = 1.0;
-        if (!(dj > 0.0)) {
-          singular = true;
-          dj = 1.0;
-        }
-        const double rj = 1.0 / sqrt(dj);
-        if (drow) {
-          // column j of L_JJ (lane j: L_jj = dj rj), trailing update
-          acc[j] = j < bs ? acc[j] * rj : (lane == j ? 1.0 : 0.0);
-#pragma unroll
-          for (int c = j + 1; c < kNB; ++c) acc[c] -= acc[j] * readlane_d(acc[j], c);
-          // row j of inv(L_JJ) is final once scaled by 1 / L_jj
-          const double lij = lane > j ? acc[j] : 0.0;
-#pragma unroll
-          for (int c = 0; c <= j; ++c) {
-            const double xjc = readlane_d(x[c], j) * rj;
-            x[c] = lane == j ? xjc : x[c] - lij * xjc;
+        for (int j = 0; j < kNB; ++j) {
+          double dj = readlane_d(acc[j], j);
+          if (j >= bs) dj = 1.0;
+          if (!(dj > 0.0)) {
+            singular = true;
+            dj = 1.0;
           }
+          const double rj = 1.0 / sqrt(dj);
+          if (drow) {
+            // column j of L_JJ (lane j: L_jj = dj rj), trailing update
+            acc[j] = j < bs ? acc[j] * rj : (lane == j ? 1.0 : 0.0);
+#pragma unroll
+            for (int c = j + 1; c < kNB; ++c) acc[c] -= acc[j] * readlane_d(acc[j], c);
+          }
+        }
+        if (drow) {
+#pragma unroll
+          for (int c = 0; c < kNB; ++c) lbuf[lane][c] = c <= lane ? acc[c] : 0.0;
         }
       }
       if (singular && lane == 0) bad = 1;
-      if (lane < bs) {
-        // L_JJ in the lower triangle with the diagonal (row `lane`),
-        // inv(L_JJ)^T in the strictly upper one (column `lane`)
-        double* blk = A + int64_t(r0) * n + r0;
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) {
-          if (c <= lane) blk[int64_t(lane) * n + c] = acc[c];
-          if (c < lane) blk[int64_t(c) * n + lane] = x[c];
-        }
-      }
+      wave_sync();
+      if (drow) rdiag[lane] = 1.0 / lbuf[lane][lane];
+      wave_sync();
       if (drow) {
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) inv[lane][c] = x[c];
+        // column `lane` of inv(L_JJ) into inv[.][lane]:
+        // x_i = (delta_{i, lane} - sum_{k < i} L_ik x_k) / L_ii, x_k = 0 for k < lane
+        for (int i = 0; i < kNB; ++i) {
+          double sx = i == lane ? 1.0 : 0.0;
+          int k = lane;
+          for (; k + 4 <= i; k += 4)
+            sx -= lbuf[i][k] * inv[k][lane] + lbuf[i][k + 1] * inv[k + 1][lane] +
+                  lbuf[i][k + 2] * inv[k + 2][lane] + lbuf[i][k + 3] * inv[k + 3][lane];
+          for (; k < i; ++k) sx -= lbuf[i][k] * inv[k][lane];
+          inv[i][lane] = i < lane ? 0.0 : sx * rdiag[i];
+        }
+        // L_JJ (lower with the diagonal) and inv(L_JJ)^T (strictly upper)
+        // into the matrix, row `lane`
+        if (lane < bs) {
+          double* blk = A + int64_t(r0 + lane) * n + r0;
+          for (int c = 0; c < bs; ++c) blk[c] = c <= lane ? lbuf[lane][c] : inv[c][lane];
+        }
       }
     }
     __syncthreads();
@@ -569,7 +582,8 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
     const float* __restrict__ vx, const float* __restrict__ vy,
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
     const int64_t* __restrict__ mat_off, const double* __restrict__ mats,
-    const double* __restrict__ refl, const double* __restrict__ vwork, double smoothing,
+    const double* __restrict__ refl, const double* __restrict__ vwork,
+    const double* __restrict__ smoothing_d,
     double threshold, double* __restrict__ smooth_slots, int64_t nslots,
     double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
     const int32_t* __restrict__ status) {
@@ -582,6 +596,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
   const double* A = B + 3 * int64_t(n) + 3;
   const double* RF = refl + 16 * d;
   const double* VW = vwork + 6 * s0;
+  const double smoothing = smoothing_d[d];
   const int tid = threadIdx.x;
   __shared__ float val[2][kMaxN];   // current values (float32, as the reference)
   __shared__ double x[2][kMaxN];    // work vector
@@ -826,7 +841,7 @@ __global__ __launch_bounds__(256) void k_tps_count(SubGrid g, int n, const doubl
 __global__ __launch_bounds__(1024) void k_tps_fill(SubGrid g, int n, const double* __restrict__ cx,
                                                    const double* __restrict__ cy,
                                                    const int64_t* __restrict__ dom_off,
-                                                   int32_t* slot_pt) {
+                                                   int32_t* slot_pt, int32_t center_base) {
   __shared__ int tmp[32];
   const int d = blockIdx.x;
   int64_t base = dom_off[d];
@@ -835,7 +850,7 @@ __global__ __launch_bounds__(1024) void k_tps_fill(SubGrid g, int n, const doubl
     const int f = (i < n && g.inside(d, cx[i], cy[i])) ? 1 : 0;
     int total;
     const int incl = block_scan_1024(f, tmp, total);
-    if (f) slot_pt[base + incl - 1] = i;
+    if (f) slot_pt[base + incl - 1] = center_base + i;
     base += total;
   }
 }
@@ -846,8 +861,9 @@ __global__ __launch_bounds__(1024) void k_tps_fill(SubGrid g, int n, const doubl
 __global__ void k_tps_point_slots(SubGrid g, int npts, const double* __restrict__ px,
                                   const double* __restrict__ py,
                                   const int64_t* __restrict__ dom_off,
-                                  const int32_t* __restrict__ slot_pt, int32_t* out_slot,
-                                  double* out_coef, double* out_norm, int32_t* status) {
+                                  const int32_t* __restrict__ slot_pt, int32_t center_base,
+                                  int32_t dom_base, int32_t* out_slot, double* out_coef,
+                                  double* out_norm, int32_t* status) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npts) return;
   const double x = px[p], y = py[p];
@@ -871,12 +887,13 @@ __global__ void k_tps_point_slots(SubGrid g, int npts, const double* __restrict_
       if (k >= kMaxSlots) continue;
       const int d = ys_[a] * g.nsx + xs_[b];
       const double cf = g.coef(d, x, y);
-      int slot = d;
+      int slot = dom_base + d;
       if (dom_off != nullptr) {
+        const int key = center_base + p;
         int64_t lo = dom_off[d], hi = dom_off[d + 1];
         while (lo < hi) {
           const int64_t mid = (lo + hi) / 2;
-          if (slot_pt[mid] < p) lo = mid + 1;
+          if (slot_pt[mid] < key) lo = mid + 1;
           else hi = mid;
         }
         slot = int(lo);
@@ -975,20 +992,21 @@ extern "C" {
 
 int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys,
                    const float* vx, const float* vy, const int64_t* dom_off,
-                   const int32_t* slot_pt, const int64_t* mat_off, double smoothing,
+                   const int32_t* slot_pt, const int64_t* mat_off, const double* smoothing,
                    double threshold, double* mats, double* refl, double* vwork,
                    double* smooth_slots, int64_t nslots, double* weights, int64_t wstride,
                    double* stats, int32_t* status, void* stream) {
   if (nd <= 0) return PIV_OK;
   if (n_max > kMaxN)
     return tps_fail(PIV_EUNSUPPORTED, "more than 2048 centers in a TPS subdomain");
-  if (!xs || !ys || !vx || !vy || !dom_off || !slot_pt || !mat_off || !mats || !refl ||
-      !vwork || !smooth_slots || !weights || !stats || !status)
+  if (!xs || !ys || !vx || !vy || !dom_off || !slot_pt || !mat_off || !smoothing || !mats ||
+      !refl || !vwork || !smooth_slots || !weights || !stats || !status)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), 0, s, xs, ys, dom_off, slot_pt,
                      mat_off, smoothing, mats, refl, vwork, status);
-  if (nd <= 256)
+  // few subdomains: 8 wavefronts each; many: 4, two workgroups per CU
+  if (nd <= 128)
     hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), 0, s, dom_off, mat_off, mats, vwork,
                        status);
   else
@@ -1022,19 +1040,21 @@ int piv_tps_count(int32_t nsx, int32_t nsy, const double* lim, int32_t n, const 
 }
 
 int piv_tps_fill(int32_t nsx, int32_t nsy, const double* lim, int32_t n, const double* cx,
-                 const double* cy, const int64_t* dom_off, int32_t* slot_pt, void* stream) {
+                 const double* cy, const int64_t* dom_off, int32_t* slot_pt,
+                 int32_t center_base, void* stream) {
   if (nsx <= 0 || nsy <= 0) return tps_fail(PIV_EINVAL, "empty subdomain grid");
   if (!lim || !cx || !cy || !dom_off || !slot_pt) return tps_fail(PIV_EINVAL, "NULL argument");
   SubGrid g{nsx, nsy, lim};
   hipLaunchKernelGGL(k_tps_fill, dim3(nsx * nsy), dim3(1024), 0,
-                     static_cast<hipStream_t>(stream), g, n, cx, cy, dom_off, slot_pt);
+                     static_cast<hipStream_t>(stream), g, n, cx, cy, dom_off, slot_pt, center_base);
   return tps_check("piv_tps_fill");
 }
 
 int piv_tps_point_slots(int32_t nsx, int32_t nsy, const double* lim, int32_t npts,
                         const double* px, const double* py, const int64_t* dom_off,
-                        const int32_t* slot_pt, int32_t* out_slot, double* out_coef,
-                        double* out_norm, int32_t* status, void* stream) {
+                        const int32_t* slot_pt, int32_t center_base, int32_t dom_base,
+                        int32_t* out_slot, double* out_coef, double* out_norm, int32_t* status,
+                        void* stream) {
   if (npts <= 0) return PIV_OK;
   if (!lim || !px || !py || !out_slot || !out_coef || !out_norm || !status)
     return tps_fail(PIV_EINVAL, "NULL argument");
@@ -1043,7 +1063,7 @@ int piv_tps_point_slots(int32_t nsx, int32_t nsy, const double* lim, int32_t npt
   SubGrid g{nsx, nsy, lim};
   hipLaunchKernelGGL(k_tps_point_slots, dim3((npts + 255) / 256), dim3(256), 0,
                      static_cast<hipStream_t>(stream), g, npts, px, py, dom_off, slot_pt,
-                     out_slot, out_coef, out_norm, status);
+                     center_base, dom_base, out_slot, out_coef, out_norm, status);
   return tps_check("piv_tps_point_slots");
 }
 

@@ -52,45 +52,55 @@ class MultipassPipeline:
         timers.setdefault(key, []).append(ev)
         return ev
 
-    def _interp(self, work_next, res, k, last, stream=None, timers=None):
-        """apply_interp (singlepass.py:409-563) of pair k's fixed field onto
-        work_next's grid, returning float64 device tensors (x, y)."""
-        import torch
-
-        from ...calcul.tps import SingularSystemError, tps_device
-        from .interp import griddata
+    def _interp(self, work_next, res, last, stream=None, timers=None):
+        """apply_interp (singlepass.py:409-563) of every pair's fixed field
+        onto work_next's grid, returning per pair the float64 device tensors
+        (x, y). The TPS of all pairs runs as one batch; a pair whose TPS
+        system is singular falls back to griddata, as in the reference."""
+        from ...calcul.tps import SingularSystemError, tps_device_batch
 
         mp = self.params.multipass
-        dx = res.pair("dx", k)
-        dy = res.pair("dy", k)
-        xs = res.pair("xs", k)
-        ys = res.pair("ys", k)
-        dev = dx.device
-        gx, gy = work_next._device_grid_full(dev)
         use_tps = mp.use_tps is True or (mp.use_tps == "last" and last)
+        out = [None] * res.npairs
         if use_tps:
+            dev = res.dev
+            gx, gy = work_next._device_grid_full(dev)
+            fields = [(res.pair("dx", k), res.pair("dy", k), res.pair("xs", k), res.pair("ys", k))
+                      for k in range(res.npairs)]
             ev = self._event_pair(timers, f"tps_pass{work_next.index_pass}" + ("_final" if last else ""))
             if ev:
                 ev[0].record()
-            try:
-                t = tps_device(dx, dy, xs, ys, gx, gy, mp.subdom_size, mp.smoothing_coef,
-                               mp.threshold_tps, 20, stream=stream)
-                if ev:
-                    ev[1].record()
-            except SingularSystemError as e:
-                if "Only nan" in str(e):
-                    raise
-                print("LinAlgError in TPS: compute delta_approx with griddata")
-            else:
+            tps = tps_device_batch(fields, gx, gy, mp.subdom_size, mp.smoothing_coef,
+                                   mp.threshold_tps, 20, stream=stream)
+            if ev:
+                ev[1].record()
+            for k, t in enumerate(tps):
+                if isinstance(t, SingularSystemError):
+                    if "Only nan" not in str(t):
+                        print("LinAlgError in TPS: compute delta_approx with griddata")
+                    continue  # griddata below (which raises "Only nan." itself)
                 res.tps[k] = t
-                return t["approx"][0], t["approx"][1]
+                out[k] = (t["approx"][0], t["approx"][1])
+        for k in range(res.npairs):
+            if out[k] is None:
+                out[k] = self._griddata(work_next, res, k)
+        return out
+
+    def _griddata(self, work_next, res, k):
+        import torch
+
+        from .interp import griddata
+
+        dx = res.pair("dx", k)
+        dy = res.pair("dy", k)
+        dev = dx.device
         hdx = dx.cpu().numpy()
         hdy = dy.cpu().numpy()
         sel = ~(np.isnan(hdx) | np.isnan(hdy))
         if not sel.any():
             raise ValueError("Only nan.")
-        hx = xs.cpu().numpy()[sel]
-        hy = ys.cpu().numpy()[sel]
+        hx = res.pair("xs", k).cpu().numpy()[sel]
+        hy = res.pair("ys", k).cpu().numpy()[sel]
         centers = np.vstack([hy, hx])
         ax = griddata(centers[::-1], hdx[sel], (work_next.ixvecs, work_next.iyvecs))
         ay = griddata(centers[::-1], hdy[sel], (work_next.ixvecs, work_next.iyvecs))
@@ -147,19 +157,19 @@ class MultipassPipeline:
                 self._secondary_peaks(res)
             passes.append(res)
             if ip + 1 < len(works):
-                approx = [self._interp(works[ip + 1], res, k, last=False, stream=stream,
-                                       timers=timers)
-                          for k in range(npairs)]
+                approx = self._interp(works[ip + 1], res, last=False, stream=stream,
+                                      timers=timers)
                 for k in range(npairs):
                     res.approx[k] = approx[k]
         if final_interp:
             res = passes[-1]
-            for k in range(npairs):
-                try:
-                    res.final[k] = self._interp(works[-1], res, k, last=True, stream=stream,
-                                                timers=timers)
-                except ValueError as e:
-                    print("Warning: InterpError at the end of the last piv pass:", e)
+            try:
+                final = self._interp(works[-1], res, last=True, stream=stream, timers=timers)
+            except ValueError as e:
+                print("Warning: InterpError at the end of the last piv pass:", e)
+            else:
+                for k in range(npairs):
+                    res.final[k] = final[k]
         return passes
 
     def _secondary_peaks(self, res):

@@ -193,12 +193,15 @@ int piv_fix(float* dx, float* dy, const float* correl_max, const int32_t* err,
  * every subdomain (weights[2][wstride], n_d + 3 values at
  * dom_off[d] + 3 d: the n_d center weights then the [1, y, x] terms),
  * per-subdomain stats[6] (x then y: fixed vectors, max |diff| of the first
- * solve, solves) and status (non-zero: singular system). threshold < 0
- * disables the outlier iterations. */
+ * solve, solves) and status (non-zero: singular system). `smoothing` is
+ * the device array of the (rescaled) smoothing coefficient of every
+ * subdomain, so that the subdomains of several vector fields, each with its
+ * own grid, go through one call. threshold < 0 disables the outlier
+ * iterations. */
 int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    const double* ys, const float* vx, const float* vy,
                    const int64_t* dom_off, const int32_t* slot_pt,
-                   const int64_t* mat_off, double smoothing, double threshold,
+                   const int64_t* mat_off, const double* smoothing, double threshold,
                    double* mats, double* refl, double* vwork,
                    double* smooth_slots, int64_t nslots, double* weights,
                    int64_t wstride, double* stats, int32_t* status,
@@ -212,13 +215,15 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
  * subdomains (d = iy * nsx + ix) with the half-open limits `lim` (device:
  * lmin_x[nsx], lmax_x[nsx], lmin_y[nsy], lmax_y[nsy]), in increasing
  * center order (thin_plate_spline_subdom.py:98-116); fill writes them at
- * dom_off[d] (prefix sum of the counts) into slot_pt.
+ * dom_off[d] (prefix sum of the counts) into slot_pt, offset by
+ * center_base (the position of cx[0] in the center arrays of a batch).
  * piv_tps_point_slots: for each point, its (at most 4) subdomains in
  * increasing order, their partition-of-unity weights and the sum of the
  * weights (thin_plate_spline_subdom.py:118-126,187-196,208-222); with
  * dom_off/slot_pt the slot is the point's position in the member list of
- * the subdomain (centers), else the subdomain index (new positions).
- * status[0] is set when a point lies in more than 4 subdomains. */
+ * the subdomain (centers: point p is center center_base + p), else the
+ * subdomain index plus dom_base (new positions). status[0] is set when a
+ * point lies in more than 4 subdomains. */
 int piv_tps_select(int32_t n, const float* dx, const float* dy,
                    const double* xs, const double* ys, double* cx, double* cy,
                    float* vx, float* vy, int32_t* idx, double* info,
@@ -228,10 +233,11 @@ int piv_tps_count(int32_t nsx, int32_t nsy, const double* lim, int32_t n,
                   void* stream);
 int piv_tps_fill(int32_t nsx, int32_t nsy, const double* lim, int32_t n,
                  const double* cx, const double* cy, const int64_t* dom_off,
-                 int32_t* slot_pt, void* stream);
+                 int32_t* slot_pt, int32_t center_base, void* stream);
 int piv_tps_point_slots(int32_t nsx, int32_t nsy, const double* lim,
                         int32_t npts, const double* px, const double* py,
                         const int64_t* dom_off, const int32_t* slot_pt,
+                        int32_t center_base, int32_t dom_base,
                         int32_t* out_slot, double* out_coef, double* out_norm,
                         int32_t* status, void* stream);
 

@@ -41,6 +41,8 @@ def default_params():
             "subdom_size": 200,
             "smoothing_coef": 5.0,
             "threshold_tps": 1.5,
+            # extension (no reference counterpart): None or "bicubic"
+            "image_deformation": None,
         },
         "mask": {"strcrop": None},
     }
@@ -197,17 +199,33 @@ class OracleWorkPIVPass:
             ixs_pad + stop[1] <= padshape[1]
         )
 
-    def loop_vectors(self, im0, im1, deltaxs_input=None, deltays_input=None, keep_maps=True):
-        """singlepass.py:262-361 with the FFTs batched over windows."""
+    def loop_vectors(self, im0, im1, deltaxs_input=None, deltays_input=None, keep_maps=True,
+                     deform=False):
+        """singlepass.py:262-361 with the FFTs batched over windows.
+
+        ``deform``: the later-pass windows are the bicubic deformed crops of
+        oracle/deform.py around the grid nodes, with the float predictor
+        (deltaxs_input, deltays_input) on the grid; the vector is the
+        residual plus the predictor (no reference counterpart)."""
         im0pad, im1pad = self.pad_images(im0, im1)
-        xs, ys, ixs0, iys0, ixs1, iys1 = self.positions(deltaxs_input, deltays_input)
+        if deform and deltaxs_input is not None:
+            xs = self.ixvecs_grid.astype(np.float64)
+            ys = self.iyvecs_grid.astype(np.float64)
+        else:
+            deform = False
+            xs, ys, ixs0, iys0, ixs1, iys1 = self.positions(deltaxs_input, deltays_input)
         nb_vec = len(xs)
         n0, n1 = self.shape_crop_im0
         m0, m1 = self.shape_crop_im1
-        valid = self.crop_valid(ixs0, iys0, self.shape_crop_im0, self._start_for_crop0,
-                                self._stop_for_crop0, im0pad.shape) & self.crop_valid(
-            ixs1, iys1, self.shape_crop_im1, self._start_for_crop1, self._stop_for_crop1,
-            im1pad.shape)
+        if deform:
+            if (m0, m1) != (n0, n1):
+                raise ValueError("image deformation needs shape_crop_im1 == shape_crop_im0")
+            valid = np.ones(nb_vec, dtype=bool)
+        else:
+            valid = self.crop_valid(ixs0, iys0, self.shape_crop_im0, self._start_for_crop0,
+                                    self._stop_for_crop0, im0pad.shape) & self.crop_valid(
+                ixs1, iys1, self.shape_crop_im1, self._start_for_crop1, self._stop_for_crop1,
+                im1pad.shape)
         deltaxs = np.empty(nb_vec, dtype=np.float32)
         deltays = np.empty_like(deltaxs)
         correls_max = np.empty_like(deltaxs)
@@ -218,7 +236,13 @@ class OracleWorkPIVPass:
         norms = np.full(nb_vec, np.nan)
         has_to_apply_subpix = self.index_pass == self.params["multipass"]["number"] - 1
         iv = np.flatnonzero(valid)
-        if iv.size:
+        if iv.size and deform:
+            from .deform import deformed_crops
+
+            c0, c1 = deformed_crops(im0, im1, self.iyvecs, self.ixvecs, deltaxs_input,
+                                    deltays_input, (n0, n1))
+            cmaps, cnorms = self.correl(c0.astype(np.float32), c1.astype(np.float32))
+        elif iv.size:
             r0 = np.arange(n0)[None, :, None]
             c0 = np.arange(n1)[None, None, :]
             r1 = np.arange(m0)[None, :, None]
@@ -255,7 +279,10 @@ class OracleWorkPIVPass:
             deltaxs[ivec] = deltays[ivec] = correls_max[ivec] = np.nan
             errors[ivec] = EXPLANATIONS[ERR_BAD_CROP]
             codes[ivec] = ERR_BAD_CROP
-        if deltaxs_input is not None:
+        if deform:
+            deltaxs[:] = (deltaxs.astype(np.float64) + deltaxs_input).astype(np.float32)
+            deltays[:] = (deltays.astype(np.float64) + deltays_input).astype(np.float32)
+        elif deltaxs_input is not None:
             deltaxs += deltaxs_input
             deltays += deltays_input
         xs, ys = self.xyoriginal_from_xymasked(xs, ys)
@@ -385,9 +412,14 @@ class OracleWorkPIV:
                 res = work.loop_vectors(im0, im1)
             else:
                 work.apply_interp(res)
-                dxin = np.round(res["deltaxs_approx"]).astype("int32")
-                dyin = np.round(res["deltays_approx"]).astype("int32")
-                res = work.loop_vectors(im0, im1, dxin, dyin)
+                if self.params["multipass"].get("image_deformation"):
+                    res = work.loop_vectors(im0, im1, np.asarray(res["deltaxs_approx"], float),
+                                            np.asarray(res["deltays_approx"], float),
+                                            deform=True)
+                else:
+                    dxin = np.round(res["deltaxs_approx"]).astype("int32")
+                    dyin = np.round(res["deltays_approx"]).astype("int32")
+                    res = work.loop_vectors(im0, im1, dxin, dyin)
             ny, nx = len(work.iyvecs), len(work.ixvecs)
             res = work_fix(res, self.params["fix"], ny, nx, p["nb_peaks_to_search"],
                            work.correl, res["deltaxs_input"], res["deltays_input"],

@@ -24,6 +24,38 @@ def test_header_declares_entry_points():
     assert set(names) == set(_native.SIGNATURES), "ctypes signatures drift from the header"
 
 
+def declared_prototypes():
+    """name -> list of parameter declarations, from the header text."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    protos = {}
+    for m in re.finditer(r"\b(piv_[a-z_0-9]+)\s*\(([^)]*)\)\s*;", text):
+        params = [p.strip() for p in m.group(2).split(",") if p.strip()]
+        protos[m.group(1)] = [] if params == ["void"] else params
+    return protos
+
+
+def _ctype_of(decl):
+    decl = decl.replace("const ", "")
+    if "*" in decl:
+        return "p"
+    kind = decl.split()[0]
+    return {"int32_t": "i32", "int64_t": "i64", "double": "f64", "float": "f32"}[kind]
+
+
+def test_ctypes_argument_lists_match_the_header():
+    codes = {ctypes.c_int32: "i32", ctypes.c_int64: "i64", ctypes.c_double: "f64",
+             ctypes.c_float: "f32"}
+    protos = declared_prototypes()
+    assert set(protos) == set(_native.SIGNATURES)
+    for name, params in protos.items():
+        argtypes = _native.SIGNATURES[name][1]
+        assert len(argtypes) == len(params), name
+        for decl, at in zip(params, argtypes):
+            want = _ctype_of(decl)
+            got = codes.get(at, "p")
+            assert want == got, (name, decl, at)
+
+
 def test_library_exports_every_symbol():
     lib = _native.load()
     for name in declared_functions():

@@ -266,3 +266,81 @@ def test_positions_from_displacement_fallbacks():
     _, _, ixs0, _, ixs1, _ = w.positions(dx2, np.zeros(n, np.int32))
     mid = n // 2
     assert ixs0[mid] - 8 == w.ixvecs_grid[mid] + 2 and ixs1[mid] - 8 == w.ixvecs_grid[mid] - 1
+
+
+# ---- image deformation (oracle/deform.py): closed-form pins -------------------------------
+
+
+def test_keys_weights_partition_and_quadratic_reproduction():
+    from oracle.deform import keys_weights
+
+    t = np.random.default_rng(0).random(200)
+    w = keys_weights(t)
+    assert np.allclose(w.sum(-1), 1.0, atol=1e-14)
+    taps = np.array([-1.0, 0.0, 1.0, 2.0])
+    for coefs in [(1.0, 0.0, 0.0), (0.3, -1.7, 0.0), (2.0, 0.5, -0.25)]:
+        poly = np.polynomial.Polynomial(coefs)
+        assert np.allclose((w * poly(taps)[None, :]).sum(-1), poly(t), atol=1e-12)
+    # a cubic is not reproduced (Keys a = -0.5 is third-order accurate, not exact)
+    cubic = np.polynomial.Polynomial((0, 0, 0, 1.0))
+    assert np.abs((w * cubic(taps)[None, :]).sum(-1) - cubic(t)).max() > 1e-3
+
+
+def test_bicubic_sample_integer_positions_and_quadratic_image():
+    from oracle.deform import bicubic_sample
+
+    rng = np.random.default_rng(1)
+    img = rng.random((20, 24)) * 100
+    yy, xx = np.mgrid[0:20, 0:24]
+    assert np.array_equal(bicubic_sample(img, yy.astype(float), xx.astype(float)), img)
+    quad = 3.0 + 0.5 * yy - 0.2 * xx + 0.01 * yy * xx + 0.03 * yy**2 - 0.02 * xx**2
+    py = rng.uniform(2, 16, 300)
+    px = rng.uniform(2, 20, 300)
+    expect = 3.0 + 0.5 * py - 0.2 * px + 0.01 * py * px + 0.03 * py**2 - 0.02 * px**2
+    assert np.allclose(bicubic_sample(quad, py, px), expect, atol=1e-10)
+    # zero outside the image
+    assert bicubic_sample(img, np.array([-5.0]), np.array([3.0]))[0] == 0.0
+
+
+def test_bilinear_field_reproduces_affine():
+    from oracle.deform import bilinear_field
+
+    iy = np.array([8, 16, 24, 32, 37])
+    ix = np.array([8, 16, 24, 31])
+    gy, gx = np.meshgrid(iy, ix, indexing="ij")
+    u = 0.5 + 0.1 * gx - 0.05 * gy
+    rng = np.random.default_rng(2)
+    y = rng.uniform(8, 37, 100)
+    x = rng.uniform(8, 31, 100)
+    assert np.allclose(bilinear_field(iy, ix, u, y, x), 0.5 + 0.1 * x - 0.05 * y, atol=1e-12)
+    # clamped (constant) outside the grid box
+    assert np.isclose(bilinear_field(iy, ix, u, np.array([0.0]), np.array([0.0]))[0], u[0, 0])
+
+
+def test_deformed_crops_zero_and_even_integer_fields():
+    from oracle.deform import deformed_crops
+
+    rng = np.random.default_rng(3)
+    im0 = rng.integers(0, 4000, (64, 72)).astype(np.uint16)
+    im1 = rng.integers(0, 4000, (64, 72)).astype(np.uint16)
+    iy = np.array([16, 24, 32, 40])
+    ix = np.array([16, 24, 32, 40, 48])
+    n = (16, 16)
+    zero = np.zeros((4, 5))
+    c0, c1 = deformed_crops(im0, im1, iy, ix, zero, zero, n)
+    a0 = im0.astype(float) - im0.min()
+    a1 = im1.astype(float) - im1.min()
+    k = 0
+    for y in iy:
+        for x in ix:
+            assert np.array_equal(c0[k], a0[y - 8:y + 8, x - 8:x + 8])
+            assert np.array_equal(c1[k], a1[y - 8:y + 8, x - 8:x + 8])
+            k += 1
+    # u = 2, v = -2: im0 sampled at (y + 1, x - 1), im1 at (y - 1, x + 1)
+    c0, c1 = deformed_crops(im0, im1, iy, ix, zero + 2.0, zero - 2.0, n)
+    k = 0
+    for y in iy:
+        for x in ix:
+            assert np.allclose(c0[k], a0[y - 7:y + 9, x - 9:x + 7], atol=1e-9)
+            assert np.allclose(c1[k], a1[y - 9:y + 7, x - 7:x + 9], atol=1e-9)
+            k += 1
