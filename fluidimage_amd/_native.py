@@ -111,11 +111,11 @@ SIGNATURES = {
         [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P, _P, _P, _P,
          ctypes.c_int64, _P, ctypes.c_int64, _P, _P, _P],
     ),
-    "piv_tps_select": (ctypes.c_int, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
-    "piv_tps_count": (ctypes.c_int, [_I, _I, _P, _I, _P, _P, _P, _P]),
-    "piv_tps_fill": (ctypes.c_int, [_I, _I, _P, _I, _P, _P, _P, _P, _I, _P]),
+    "piv_tps_select": (ctypes.c_int, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "piv_tps_count": (ctypes.c_int, [_I, _P, _P, _P, _P, _P, _P, _P]),
+    "piv_tps_fill": (ctypes.c_int, [_I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "piv_tps_point_slots": (
-        ctypes.c_int, [_I, _I, _P, _I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P]),
+        ctypes.c_int, [_I, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "piv_tps_blend": (ctypes.c_int, [_I, _P, _P, _P, _P, ctypes.c_int64, _P, _P]),
     "piv_tps_interpolate": (
         ctypes.c_int,

@@ -110,26 +110,28 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
     f64 = dict(dtype=torch.float64, device=dev)
     i32 = dict(dtype=torch.int32, device=dev)
     thr = -1.0 if threshold is None else float(threshold)
-    flist = []
-    base = 0
-    for k, f in enumerate(fields):
-        flist.append(_Field(k, int(f[0].numel()), base))
-        base += flist[-1].n
-    ntot = base
+    n = int(fields[0][0].numel())
+    if any(int(f[0].numel()) != n for f in fields):
+        raise ValueError("the fields of a TPS batch have one number of vectors")
+    if n == 0:
+        return [SingularSystemError("Only nan.") for _ in fields]
+    flist = [_Field(k, n, k * n) for k in range(len(fields))]
+    ntot = n * len(fields)
+
+    def stacked(i):
+        t = [f[i] for f in fields]
+        return t[0].contiguous() if len(t) == 1 else torch.cat(t)
+
+    dx, dy, xs, ys = (stacked(i) for i in range(4))
     cx = torch.empty(ntot, **f64)
     cy = torch.empty(ntot, **f64)
     vx = torch.empty(ntot, dtype=torch.float32, device=dev)
     vy = torch.empty(ntot, dtype=torch.float32, device=dev)
     idx = torch.empty(ntot, **i32)
     info = torch.empty((len(fields), 5), **f64)
-    for f, (dx, dy, xs, ys) in zip(flist, fields):
-        b = f.base
-        if f.n == 0:
-            info[f.k].zero_()
-            continue
-        _native.check(lib.piv_tps_select(f.n, _p(dx), _p(dy), _p(xs), _p(ys), _p(cx[b:]),
-                                         _p(cy[b:]), _p(vx[b:]), _p(vy[b:]), _p(idx[b:]),
-                                         _p(info[f.k]), sp), "piv_tps_select")
+    _native.check(lib.piv_tps_select(len(fields), n, _p(dx), _p(dy), _p(xs), _p(ys), _p(cx),
+                                     _p(cy), _p(vx), _p(vy), _p(idx), _p(info), sp),
+                  "piv_tps_select")
     info_h = info.cpu().numpy()
     # subdomain grids (host scalars)
     active = []
@@ -175,13 +177,17 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
         f.nd_base = nd_base
         nd_base += f.nd
     nd = nd_base
-    lim = torch.from_numpy(np.concatenate([f.lim_h for f in active])).to(dev)
+    lim_h = np.concatenate([f.lim_h for f in active])
     lim_off = np.concatenate([[0], np.cumsum([f.lim_h.size for f in active])])
+    lim = torch.from_numpy(lim_h).to(dev)
+    table = np.array([[f.nsx, f.nsy, f.count, f.base, f.nd_base, lim_off[j]]
+                      for j, f in enumerate(active)], dtype=np.int32)
+    ftab = torch.from_numpy(table).to(dev)
+    dom_field = torch.from_numpy(
+        np.concatenate([np.full(f.nd, j, np.int32) for j, f in enumerate(active)])).to(dev)
     counts = torch.empty(nd, **i32)
-    for j, f in enumerate(active):
-        _native.check(lib.piv_tps_count(f.nsx, f.nsy, _p(lim[lim_off[j]:]), f.count,
-                                        _p(cx[f.base:]), _p(cy[f.base:]),
-                                        _p(counts[f.nd_base:]), sp), "piv_tps_count")
+    _native.check(lib.piv_tps_count(nd, _p(ftab), _p(dom_field), _p(lim), _p(cx), _p(cy),
+                                    _p(counts), sp), "piv_tps_count")
     sizes = counts.cpu().numpy().astype(np.int64)
     retry = False
     for f in active:
@@ -215,20 +221,15 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
     n_dom = torch.empty(na * m * MAX_SLOTS, **i32)
     n_coef = torch.empty(na * m * MAX_SLOTS, **f64)
     n_norm = torch.empty(na * m, **f64)
-    for j, f in enumerate(active):
-        lim_j = _p(lim[lim_off[j]:])
-        b = f.base
-        _native.check(lib.piv_tps_fill(f.nsx, f.nsy, lim_j, f.count, _p(cx[b:]), _p(cy[b:]),
-                                       _p(dom_off[f.nd_base:]), _p(slot_pt), b, sp),
-                      "piv_tps_fill")
-        _native.check(lib.piv_tps_point_slots(
-            f.nsx, f.nsy, lim_j, f.count, _p(cx[b:]), _p(cy[b:]), _p(dom_off[f.nd_base:]),
-            _p(slot_pt), b, 0, _p(c_slot[MAX_SLOTS * b:]), _p(c_coef[MAX_SLOTS * b:]),
-            _p(c_norm[b:]), _p(status[nd:]), sp), "piv_tps_point_slots")
-        _native.check(lib.piv_tps_point_slots(
-            f.nsx, f.nsy, lim_j, m, _p(new_x), _p(new_y), None, None, 0, f.nd_base,
-            _p(n_dom[MAX_SLOTS * m * j:]), _p(n_coef[MAX_SLOTS * m * j:]),
-            _p(n_norm[m * j:]), _p(status[nd:]), sp), "piv_tps_point_slots")
+    _native.check(lib.piv_tps_fill(nd, _p(ftab), _p(dom_field), _p(lim), _p(cx), _p(cy),
+                                   _p(dom_off), _p(slot_pt), sp), "piv_tps_fill")
+    _native.check(lib.piv_tps_point_slots(
+        na, _p(ftab), max(f.count for f in active), _p(lim), _p(cx), _p(cy), _p(dom_off),
+        _p(slot_pt), _p(c_slot), _p(c_coef), _p(c_norm), _p(status[nd:]), sp),
+        "piv_tps_point_slots")
+    _native.check(lib.piv_tps_point_slots(
+        na, _p(ftab), m, _p(lim), _p(new_x), _p(new_y), None, None, _p(n_dom), _p(n_coef),
+        _p(n_norm), _p(status[nd:]), sp), "piv_tps_point_slots")
     mats = torch.empty(int(mat_off_h[-1]), **f64)
     refl = torch.empty((nd, 16), **f64)
     vwork = torch.empty(nslots * 6, **f64)

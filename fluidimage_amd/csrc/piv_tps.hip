@@ -743,6 +743,20 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
                                                      const double* __restrict__ ys,
                                                      double* cx, double* cy, float* vx,
                                                      float* vy, int32_t* idx, double* info) {
+  // field blockIdx.x: elements [f n, (f + 1) n) in and out
+  {
+    const int64_t o = int64_t(blockIdx.x) * n;
+    dx += o;
+    dy += o;
+    xs += o;
+    ys += o;
+    cx += o;
+    cy += o;
+    vx += o;
+    vy += o;
+    idx += o;
+    info += 5 * blockIdx.x;
+  }
   __shared__ int tmp[32];
   __shared__ double red[4][16];
   double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
@@ -823,28 +837,49 @@ struct SubGrid {
   }
 };
 
-// members of subdomain blockIdx.x: count
-__global__ __launch_bounds__(256) void k_tps_count(SubGrid g, int n, const double* __restrict__ cx,
-                                                   const double* __restrict__ cy,
+__device__ __forceinline__ SubGrid field_grid(const piv_tps_field& f, const double* lim) {
+  return SubGrid{f.nsx, f.nsy, lim + f.lim_off};
+}
+
+// members of (global) subdomain blockIdx.x: count
+__global__ __launch_bounds__(256) void k_tps_count(const piv_tps_field* __restrict__ fields,
+                                                   const int32_t* __restrict__ dom_field,
+                                                   const double* __restrict__ lim,
+                                                   const double* __restrict__ cx_all,
+                                                   const double* __restrict__ cy_all,
                                                    int32_t* counts) {
-  const int d = blockIdx.x;
+  const piv_tps_field f = fields[dom_field[blockIdx.x]];
+  const SubGrid g = field_grid(f, lim);
+  const int d = blockIdx.x - f.dom_base;
+  const int n = f.count;
+  const double* cx = cx_all + f.center_base;
+  const double* cy = cy_all + f.center_base;
   int c = 0;
   for (int i = threadIdx.x; i < n; i += 256) c += g.inside(d, cx[i], cy[i]) ? 1 : 0;
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
   __shared__ int part[4];
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
   __syncthreads();
-  if (threadIdx.x == 0) counts[d] = part[0] + part[1] + part[2] + part[3];
+  if (threadIdx.x == 0) counts[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-// members of subdomain blockIdx.x in increasing center order (np.where)
-__global__ __launch_bounds__(1024) void k_tps_fill(SubGrid g, int n, const double* __restrict__ cx,
-                                                   const double* __restrict__ cy,
+// members of (global) subdomain blockIdx.x in increasing center order (np.where)
+__global__ __launch_bounds__(1024) void k_tps_fill(const piv_tps_field* __restrict__ fields,
+                                                   const int32_t* __restrict__ dom_field,
+                                                   const double* __restrict__ lim,
+                                                   const double* __restrict__ cx_all,
+                                                   const double* __restrict__ cy_all,
                                                    const int64_t* __restrict__ dom_off,
-                                                   int32_t* slot_pt, int32_t center_base) {
+                                                   int32_t* slot_pt) {
   __shared__ int tmp[32];
-  const int d = blockIdx.x;
-  int64_t base = dom_off[d];
+  const piv_tps_field f = fields[dom_field[blockIdx.x]];
+  const SubGrid g = field_grid(f, lim);
+  const int d = blockIdx.x - f.dom_base;
+  const int n = f.count;
+  const int center_base = f.center_base;
+  const double* cx = cx_all + center_base;
+  const double* cy = cy_all + center_base;
+  int64_t base = dom_off[blockIdx.x];
   for (int c0 = 0; c0 < n; c0 += 1024) {
     const int i = c0 + threadIdx.x;
     const int f = (i < n && g.inside(d, cx[i], cy[i])) ? 1 : 0;
@@ -858,15 +893,29 @@ __global__ __launch_bounds__(1024) void k_tps_fill(SubGrid g, int n, const doubl
 // per point: its subdomains in increasing order (at most kMaxSlots), their
 // weights and the sum of the weights. Centers mode (dom_off != NULL): the
 // slot is the position of the point in the member list of the subdomain.
-__global__ void k_tps_point_slots(SubGrid g, int npts, const double* __restrict__ px,
-                                  const double* __restrict__ py,
+// field blockIdx.y; centers mode (dom_off != NULL): its centers
+// cx[center_base + p], p < count, outputs at center_base + p; new-position
+// mode: the m positions px[p], outputs at field * m + p
+__global__ void k_tps_point_slots(const piv_tps_field* __restrict__ fields,
+                                  const double* __restrict__ lim, int m,
+                                  const double* __restrict__ px_all,
+                                  const double* __restrict__ py_all,
                                   const int64_t* __restrict__ dom_off,
-                                  const int32_t* __restrict__ slot_pt, int32_t center_base,
-                                  int32_t dom_base, int32_t* out_slot, double* out_coef,
-                                  double* out_norm, int32_t* status) {
+                                  const int32_t* __restrict__ slot_pt, int32_t* out_slot,
+                                  double* out_coef, double* out_norm, int32_t* status) {
+  const piv_tps_field f = fields[blockIdx.y];
+  const SubGrid g = field_grid(f, lim);
+  const bool centers = dom_off != nullptr;
+  const int npts = centers ? f.count : m;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npts) return;
-  const double x = px[p], y = py[p];
+  const int64_t in = centers ? int64_t(f.center_base) + p : p;
+  const int64_t o = centers ? int64_t(f.center_base) + p : int64_t(blockIdx.y) * m + p;
+  const int center_base = f.center_base, dom_base = f.dom_base;
+  out_slot += kMaxSlots * o;
+  out_coef += kMaxSlots * o;
+  out_norm += o;
+  const double x = px_all[in], y = py_all[in];
   int xs_[kMaxSlots], ys_[kMaxSlots];
   int nx = 0, ny = 0;
   for (int i = 0; i < g.nsx; ++i)
@@ -888,9 +937,9 @@ __global__ void k_tps_point_slots(SubGrid g, int npts, const double* __restrict_
       const int d = ys_[a] * g.nsx + xs_[b];
       const double cf = g.coef(d, x, y);
       int slot = dom_base + d;
-      if (dom_off != nullptr) {
+      if (centers) {
         const int key = center_base + p;
-        int64_t lo = dom_off[d], hi = dom_off[d + 1];
+        int64_t lo = dom_off[dom_base + d], hi = dom_off[dom_base + d + 1];
         while (lo < hi) {
           const int64_t mid = (lo + hi) / 2;
           if (slot_pt[mid] < key) lo = mid + 1;
@@ -898,16 +947,16 @@ __global__ void k_tps_point_slots(SubGrid g, int npts, const double* __restrict_
         }
         slot = int(lo);
       }
-      out_slot[kMaxSlots * p + k] = slot;
-      out_coef[kMaxSlots * p + k] = cf;
+      out_slot[k] = slot;
+      out_coef[k] = cf;
       norm += cf;
       ++k;
     }
   for (; k < kMaxSlots; ++k) {
-    out_slot[kMaxSlots * p + k] = -1;
-    out_coef[kMaxSlots * p + k] = 0.0;
+    out_slot[k] = -1;
+    out_coef[k] = 0.0;
   }
-  out_norm[p] = norm;
+  out_norm[0] = norm;
 }
 
 // ---------------------------------------------------------------------------
@@ -1018,52 +1067,52 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
   return tps_check("piv_tps_smooth");
 }
 
-int piv_tps_select(int32_t n, const float* dx, const float* dy, const double* xs,
-                   const double* ys, double* cx, double* cy, float* vx, float* vy,
-                   int32_t* idx, double* info, void* stream) {
-  if (n <= 0) return tps_fail(PIV_EINVAL, "no vector");
+int piv_tps_select(int32_t nfields, int32_t n, const float* dx, const float* dy,
+                   const double* xs, const double* ys, double* cx, double* cy, float* vx,
+                   float* vy, int32_t* idx, double* info, void* stream) {
+  if (nfields <= 0 || n <= 0) return tps_fail(PIV_EINVAL, "no vector");
   if (!dx || !dy || !xs || !ys || !cx || !cy || !vx || !vy || !idx || !info)
     return tps_fail(PIV_EINVAL, "NULL argument");
-  hipLaunchKernelGGL(k_tps_select, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), n,
-                     dx, dy, xs, ys, cx, cy, vx, vy, idx, info);
+  hipLaunchKernelGGL(k_tps_select, dim3(nfields), dim3(1024), 0,
+                     static_cast<hipStream_t>(stream), n, dx, dy, xs, ys, cx, cy, vx, vy, idx,
+                     info);
   return tps_check("piv_tps_select");
 }
 
-int piv_tps_count(int32_t nsx, int32_t nsy, const double* lim, int32_t n, const double* cx,
-                  const double* cy, int32_t* counts, void* stream) {
-  if (nsx <= 0 || nsy <= 0) return tps_fail(PIV_EINVAL, "empty subdomain grid");
-  if (!lim || !cx || !cy || !counts) return tps_fail(PIV_EINVAL, "NULL argument");
-  SubGrid g{nsx, nsy, lim};
-  hipLaunchKernelGGL(k_tps_count, dim3(nsx * nsy), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), g, n, cx, cy, counts);
+int piv_tps_count(int32_t nd, const piv_tps_field* fields, const int32_t* dom_field,
+                  const double* lim, const double* cx, const double* cy, int32_t* counts,
+                  void* stream) {
+  if (nd <= 0) return tps_fail(PIV_EINVAL, "empty subdomain grid");
+  if (!fields || !dom_field || !lim || !cx || !cy || !counts)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_tps_count, dim3(nd), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     fields, dom_field, lim, cx, cy, counts);
   return tps_check("piv_tps_count");
 }
 
-int piv_tps_fill(int32_t nsx, int32_t nsy, const double* lim, int32_t n, const double* cx,
-                 const double* cy, const int64_t* dom_off, int32_t* slot_pt,
-                 int32_t center_base, void* stream) {
-  if (nsx <= 0 || nsy <= 0) return tps_fail(PIV_EINVAL, "empty subdomain grid");
-  if (!lim || !cx || !cy || !dom_off || !slot_pt) return tps_fail(PIV_EINVAL, "NULL argument");
-  SubGrid g{nsx, nsy, lim};
-  hipLaunchKernelGGL(k_tps_fill, dim3(nsx * nsy), dim3(1024), 0,
-                     static_cast<hipStream_t>(stream), g, n, cx, cy, dom_off, slot_pt, center_base);
+int piv_tps_fill(int32_t nd, const piv_tps_field* fields, const int32_t* dom_field,
+                 const double* lim, const double* cx, const double* cy, const int64_t* dom_off,
+                 int32_t* slot_pt, void* stream) {
+  if (nd <= 0) return tps_fail(PIV_EINVAL, "empty subdomain grid");
+  if (!fields || !dom_field || !lim || !cx || !cy || !dom_off || !slot_pt)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_tps_fill, dim3(nd), dim3(1024), 0, static_cast<hipStream_t>(stream),
+                     fields, dom_field, lim, cx, cy, dom_off, slot_pt);
   return tps_check("piv_tps_fill");
 }
 
-int piv_tps_point_slots(int32_t nsx, int32_t nsy, const double* lim, int32_t npts,
-                        const double* px, const double* py, const int64_t* dom_off,
-                        const int32_t* slot_pt, int32_t center_base, int32_t dom_base,
-                        int32_t* out_slot, double* out_coef, double* out_norm, int32_t* status,
-                        void* stream) {
-  if (npts <= 0) return PIV_OK;
-  if (!lim || !px || !py || !out_slot || !out_coef || !out_norm || !status)
+int piv_tps_point_slots(int32_t nfields, const piv_tps_field* fields, int32_t max_points,
+                        const double* lim, const double* px, const double* py,
+                        const int64_t* dom_off, const int32_t* slot_pt, int32_t* out_slot,
+                        double* out_coef, double* out_norm, int32_t* status, void* stream) {
+  if (nfields <= 0 || max_points <= 0) return PIV_OK;
+  if (!fields || !lim || !px || !py || !out_slot || !out_coef || !out_norm || !status)
     return tps_fail(PIV_EINVAL, "NULL argument");
   if ((dom_off == nullptr) != (slot_pt == nullptr))
     return tps_fail(PIV_EINVAL, "dom_off and slot_pt go together");
-  SubGrid g{nsx, nsy, lim};
-  hipLaunchKernelGGL(k_tps_point_slots, dim3((npts + 255) / 256), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), g, npts, px, py, dom_off, slot_pt,
-                     center_base, dom_base, out_slot, out_coef, out_norm, status);
+  hipLaunchKernelGGL(k_tps_point_slots, dim3((max_points + 255) / 256, nfields), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), fields, lim, max_points, px, py, dom_off,
+                     slot_pt, out_slot, out_coef, out_norm, status);
   return tps_check("piv_tps_point_slots");
 }
 

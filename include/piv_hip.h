@@ -207,39 +207,55 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    int64_t wstride, double* stats, int32_t* status,
                    void* stream);
 
-/* Geometry of the subdomains, on the device.
- * piv_tps_select: ordered compaction of the n vectors whose dx and dy are
- * finite (singlepass.py:450-463) into cx, cy (positions), vx, vy (values)
- * and idx (original index); info[5] = count, x_min, x_max, y_min, y_max.
- * piv_tps_count / piv_tps_fill: members of each of the nsx * nsy
- * subdomains (d = iy * nsx + ix) with the half-open limits `lim` (device:
- * lmin_x[nsx], lmax_x[nsx], lmin_y[nsy], lmax_y[nsy]), in increasing
- * center order (thin_plate_spline_subdom.py:98-116); fill writes them at
- * dom_off[d] (prefix sum of the counts) into slot_pt, offset by
- * center_base (the position of cx[0] in the center arrays of a batch).
+/* One vector field (one image pair) of a TPS batch: its subdomain grid
+ * (nsx x nsy, limits at lim + lim_off: lmin_x[nsx], lmax_x[nsx],
+ * lmin_y[nsy], lmax_y[nsy]), its number of selected centers, the position
+ * of its first center in the batch's center arrays and the index of its
+ * first subdomain. A device array of these records describes a batch. */
+typedef struct piv_tps_field {
+  int32_t nsx, nsy;
+  int32_t count;
+  int32_t center_base;
+  int32_t dom_base;
+  int32_t lim_off;
+} piv_tps_field;
+
+/* piv_tps_select: for each of nfields fields of n vectors (elements
+ * [f n, (f + 1) n) of dx, dy, xs, ys), ordered compaction of the vectors
+ * whose dx and dy are finite (singlepass.py:450-463) into cx, cy
+ * (positions), vx, vy (values) and idx (index in the field), starting at
+ * element f n; info[f][5] = count, x_min, x_max, y_min, y_max.
+ * piv_tps_count / piv_tps_fill: members of every subdomain d of the batch
+ * (field dom_field[d], local index d - dom_base = iy * nsx + ix) with the
+ * half-open limits of its field, in increasing center order
+ * (thin_plate_spline_subdom.py:98-116); fill writes the batch center
+ * indices at dom_off[d] (prefix sum of the counts) into slot_pt.
  * piv_tps_point_slots: for each point, its (at most 4) subdomains in
  * increasing order, their partition-of-unity weights and the sum of the
- * weights (thin_plate_spline_subdom.py:118-126,187-196,208-222); with
- * dom_off/slot_pt the slot is the point's position in the member list of
- * the subdomain (centers: point p is center center_base + p), else the
- * subdomain index plus dom_base (new positions). status[0] is set when a
- * point lies in more than 4 subdomains. */
-int piv_tps_select(int32_t n, const float* dx, const float* dy,
+ * weights (thin_plate_spline_subdom.py:118-126,187-196,208-222). With
+ * dom_off/slot_pt (centers): the points are the centers of each field
+ * (px = cx), outputs at the center's batch index, the slot being its
+ * position in the member list of the subdomain; without: the max_points
+ * new positions (px, py) shared by every field, outputs at
+ * f * max_points + p, the slot being the batch subdomain index.
+ * status[0] is set when a point lies in more than 4 subdomains. */
+int piv_tps_select(int32_t nfields, int32_t n, const float* dx, const float* dy,
                    const double* xs, const double* ys, double* cx, double* cy,
                    float* vx, float* vy, int32_t* idx, double* info,
                    void* stream);
-int piv_tps_count(int32_t nsx, int32_t nsy, const double* lim, int32_t n,
-                  const double* cx, const double* cy, int32_t* counts,
-                  void* stream);
-int piv_tps_fill(int32_t nsx, int32_t nsy, const double* lim, int32_t n,
-                 const double* cx, const double* cy, const int64_t* dom_off,
-                 int32_t* slot_pt, int32_t center_base, void* stream);
-int piv_tps_point_slots(int32_t nsx, int32_t nsy, const double* lim,
-                        int32_t npts, const double* px, const double* py,
-                        const int64_t* dom_off, const int32_t* slot_pt,
-                        int32_t center_base, int32_t dom_base,
-                        int32_t* out_slot, double* out_coef, double* out_norm,
-                        int32_t* status, void* stream);
+int piv_tps_count(int32_t nd, const piv_tps_field* fields,
+                  const int32_t* dom_field, const double* lim, const double* cx,
+                  const double* cy, int32_t* counts, void* stream);
+int piv_tps_fill(int32_t nd, const piv_tps_field* fields,
+                 const int32_t* dom_field, const double* lim, const double* cx,
+                 const double* cy, const int64_t* dom_off, int32_t* slot_pt,
+                 void* stream);
+int piv_tps_point_slots(int32_t nfields, const piv_tps_field* fields,
+                        int32_t max_points, const double* lim, const double* px,
+                        const double* py, const int64_t* dom_off,
+                        const int32_t* slot_pt, int32_t* out_slot,
+                        double* out_coef, double* out_norm, int32_t* status,
+                        void* stream);
 
 /* out[c][i] = sum_k coef[4 i + k] * smooth_slots[c][slots[4 i + k]] /
  * norm[i] over the point's slots (-1 terminated): the blend of
