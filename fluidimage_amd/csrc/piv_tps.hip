@@ -27,7 +27,6 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kNB = 32;       // Cholesky / triangular-solve block size
 constexpr int kMaxN = 2048;   // centers per subdomain
-constexpr int kColsPerThread = (kMaxN + kThreads - 1) / kThreads;
 
 __device__ __forceinline__ double tps_phi(double r2) {
   return r2 != 0.0 ? r2 * log(r2) / 2.0 : 0.0;
@@ -66,37 +65,42 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
     const double* __restrict__ xs, const double* __restrict__ ys,
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
     const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
-    double* __restrict__ mats,
-    double* __restrict__ refl, double* __restrict__ vwork, int32_t* __restrict__ status) {
+    double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
+    int32_t* __restrict__ status, int nm) {
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
   double* B = mats + mat_off[d];
   double* RF = refl + 16 * d;
   double* VW = vwork + 6 * s0;
-  __shared__ double cy[kMaxN], cx[kMaxN];
-  __shared__ double Pm[3][kMaxN];  // P, then its reflected copy
-  __shared__ double Vm[3][kMaxN];
+  // dynamic LDS, nm >= n doubles per array: cy, cx, P (3 columns, then
+  // their reflected copy), V (3 Householder vectors)
+  extern __shared__ double dyn[];
+  double* cy = dyn;
+  double* cx = dyn + nm;
+#define PM(k, j) dyn[(2 + (k)) * nm + (j)]
+#define VM(k, j) dyn[(5 + (k)) * nm + (j)]
   __shared__ double red[kThreads / 64];
-  __shared__ double sh[32];
+  __shared__ double sh[8];
   const int tid = threadIdx.x;
+  const double smoothing = smoothing_d[d];
   for (int j = tid; j < n; j += kThreads) {
     const int p = slot_pt[s0 + j];
     cy[j] = ys[p];
     cx[j] = xs[p];
-    Pm[0][j] = 1.0;
-    Pm[1][j] = ys[p];
-    Pm[2][j] = xs[p];
-    Vm[0][j] = Vm[1][j] = Vm[2][j] = 0.0;
+    PM(0, j) = 1.0;
+    PM(1, j) = ys[p];
+    PM(2, j) = xs[p];
+    VM(0, j) = VM(1, j) = VM(2, j) = 0.0;
   }
   __syncthreads();
   // ---- Householder reflectors (LAPACK dgeqr2 convention) ----
   double tau[3];
   for (int k = 0; k < 3; ++k) {
     double part = 0.0;
-    for (int j = k + 1 + tid; j < n; j += kThreads) part += Pm[k][j] * Pm[k][j];
+    for (int j = k + 1 + tid; j < n; j += kThreads) part += PM(k, j) * PM(k, j);
     const double sig = block_sum(part, red);
-    const double alpha = Pm[k][k];
+    const double alpha = PM(k, k);
     double beta, t, scale;
     if (sig == 0.0) {
       beta = alpha;
@@ -111,117 +115,131 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
     for (int j = tid; j < n; j += kThreads) {
       double v = 0.0;
       if (j == k) v = 1.0;
-      else if (j > k) v = Pm[k][j] * scale;
-      Vm[k][j] = v;
+      else if (j > k) v = PM(k, j) * scale;
+      VM(k, j) = v;
     }
     __syncthreads();
-    if (tid == 0) Pm[k][k] = beta;
+    if (tid == 0) PM(k, k) = beta;
     // apply H_k to the remaining columns of P
     for (int c = k + 1; c < 3; ++c) {
       double pp = 0.0;
-      for (int j = k + tid; j < n; j += kThreads) pp += Vm[k][j] * Pm[c][j];
+      for (int j = k + tid; j < n; j += kThreads) pp += VM(k, j) * PM(c, j);
       const double dot = block_sum(pp, red);
-      for (int j = k + tid; j < n; j += kThreads) Pm[c][j] -= t * Vm[k][j] * dot;
+      for (int j = k + tid; j < n; j += kThreads) PM(c, j) -= t * VM(k, j) * dot;
       __syncthreads();
     }
   }
   // T of the compact WY form (dlarft, forward, columnwise)
+  double p01 = 0.0, p02 = 0.0, p12 = 0.0;
+  for (int j = tid; j < n; j += kThreads) {
+    p01 += VM(0, j) * VM(1, j);
+    p02 += VM(0, j) * VM(2, j);
+    p12 += VM(1, j) * VM(2, j);
+  }
+  const double vtv01 = block_sum(p01, red);
+  const double vtv02 = block_sum(p02, red);
+  const double vtv12 = block_sum(p12, red);
+  const double T00 = tau[0], T11 = tau[1], T22 = tau[2];
+  const double T01 = -tau[1] * T00 * vtv01;
+  // T[0:2, 2] = -tau2 * T[0:2, 0:2] * V[:, 0:2]^T v2
+  const double T02 = -tau[2] * (T00 * vtv02 + T01 * vtv12);
+  const double T12 = -tau[2] * (T11 * vtv12);
   if (tid == 0) {
-    double vtv01 = 0, vtv02 = 0, vtv12 = 0;
-    for (int j = 0; j < n; ++j) {
-      vtv01 += Vm[0][j] * Vm[1][j];
-      vtv02 += Vm[0][j] * Vm[2][j];
-      vtv12 += Vm[1][j] * Vm[2][j];
-    }
-    const double T00 = tau[0], T11 = tau[1], T22 = tau[2];
-    const double T01 = -tau[1] * T00 * vtv01;
-    // T[0:2, 2] = -tau2 * T[0:2, 0:2] * V[:, 0:2]^T v2
-    const double g0 = vtv02, g1 = vtv12;
-    const double T02 = -tau[2] * (T00 * g0 + T01 * g1);
-    const double T12 = -tau[2] * (T11 * g1);
     RF[0] = tau[0];
     RF[1] = tau[1];
     RF[2] = tau[2];
-    RF[3] = Pm[0][0];
-    RF[4] = Pm[1][0];
-    RF[5] = Pm[2][0];
-    RF[6] = Pm[1][1];
-    RF[7] = Pm[2][1];
-    RF[8] = Pm[2][2];
+    RF[3] = PM(0, 0);
+    RF[4] = PM(1, 0);
+    RF[5] = PM(2, 0);
+    RF[6] = PM(1, 1);
+    RF[7] = PM(2, 1);
+    RF[8] = PM(2, 2);
     RF[9] = T00;
     RF[10] = T01;
     RF[11] = T02;
     RF[12] = T11;
     RF[13] = T12;
     RF[14] = T22;
-    sh[0] = T00; sh[1] = T01; sh[2] = T02; sh[3] = T11; sh[4] = T12; sh[5] = T22;
-    const bool singular = n < 3 || Pm[0][0] == 0.0 || Pm[1][1] == 0.0 || Pm[2][2] == 0.0;
+    const bool singular = n < 3 || PM(0, 0) == 0.0 || PM(1, 1) == 0.0 || PM(2, 2) == 0.0;
     if (singular) status[d] = 1;
   }
-  __syncthreads();
-  // ---- B (full storage) and Y = B V (column j of thread: Y_j by symmetry) ----
-  double yacc[kColsPerThread][3];
-#pragma unroll
-  for (int c = 0; c < kColsPerThread; ++c) yacc[c][0] = yacc[c][1] = yacc[c][2] = 0.0;
-  for (int i = 0; i < n; ++i) {
-    const double yi = cy[i], xi = cx[i];
-    const double v0 = Vm[0][i], v1 = Vm[1][i], v2 = Vm[2][i];
-#pragma unroll
-    for (int c = 0; c < kColsPerThread; ++c) {
-      const int j = tid + c * kThreads;
-      if (j < n) {
+  // ---- lower triangle of B = K + s I: row i by one wavefront, its lanes
+  //      over the columns j <= i (K is symmetric; only the lower triangle
+  //      is read afterwards) ----
+  {
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int i = wave; i < n; i += kThreads / 64) {
+      const double yi = cy[i], xi = cx[i];
+      double* Bi = B + int64_t(i) * n;
+      for (int j = lane; j <= i; j += 64) {
         const double dy = cy[j] - yi, dx = cx[j] - xi;
         double b = tps_phi(dy * dy + dx * dx);
-        if (i == j) b += smoothing_d[d];
-        B[int64_t(i) * n + j] = b;
-        yacc[c][0] += b * v0;
-        yacc[c][1] += b * v1;
-        yacc[c][2] += b * v2;
+        if (i == j) b += smoothing;
+        Bi[j] = b;
       }
     }
   }
-  // M = V^T Y (3 x 3)
-  double m[9];
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) {
-      double part = 0.0;
-#pragma unroll
-      for (int c = 0; c < kColsPerThread; ++c) {
-        const int j = tid + c * kThreads;
-        if (j < n) part += Vm[a][j] * yacc[c][b];
-      }
-      m[3 * a + b] = block_sum(part, red);
+  __syncthreads();
+  // ---- Y = B V (column j of thread, B read back from the lower triangle:
+  //      B[i][j] for i >= j down the column, B[j][i] along row j for i < j),
+  //      M = V^T Y, Z = Y T - 1/2 V T^T M T ----
+  double m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = tid; j < n; j += kThreads) {
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    const double* Bj = B + int64_t(j) * n;
+    for (int i = 0; i < j; ++i) {
+      const double b = Bj[i];
+      y0 += b * VM(0, i);
+      y1 += b * VM(1, i);
+      y2 += b * VM(2, i);
     }
-  // Z = Y T - 1/2 V T^T M T
-  const double T[3][3] = {{sh[0], sh[1], sh[2]}, {0.0, sh[3], sh[4]}, {0.0, 0.0, sh[5]}};
+    for (int i = j; i < n; ++i) {
+      const double b = B[int64_t(i) * n + j];
+      y0 += b * VM(0, i);
+      y1 += b * VM(1, i);
+      y2 += b * VM(2, i);
+    }
+    const double v0 = VM(0, j), v1 = VM(1, j), v2 = VM(2, j);
+    m9[0] += v0 * y0; m9[1] += v0 * y1; m9[2] += v0 * y2;
+    m9[3] += v1 * y0; m9[4] += v1 * y1; m9[5] += v1 * y2;
+    m9[6] += v2 * y0; m9[7] += v2 * y1; m9[8] += v2 * y2;
+    // keep Y in the Z slots until M is known
+    VW[6 * j + 0] = v0;
+    VW[6 * j + 1] = v1;
+    VW[6 * j + 2] = v2;
+    VW[6 * j + 3] = y0;
+    VW[6 * j + 4] = y1;
+    VW[6 * j + 5] = y2;
+  }
+  double m[9];
+  for (int e = 0; e < 9; ++e) m[e] = block_sum(m9[e], red);
+  const double T[3][3] = {{T00, T01, T02}, {0.0, T11, T12}, {0.0, 0.0, T22}};
   double MT[3][3], TtMT[3][3];
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b) {
-      double s = 0;
-      for (int k = 0; k < 3; ++k) s += m[3 * a + k] * T[k][b];
-      MT[a][b] = s;
+      double sacc = 0;
+      for (int k = 0; k < 3; ++k) sacc += m[3 * a + k] * T[k][b];
+      MT[a][b] = sacc;
     }
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b) {
-      double s = 0;
-      for (int k = 0; k < 3; ++k) s += T[k][a] * MT[k][b];
-      TtMT[a][b] = s;
+      double sacc = 0;
+      for (int k = 0; k < 3; ++k) sacc += T[k][a] * MT[k][b];
+      TtMT[a][b] = sacc;
     }
-#pragma unroll
-  for (int c = 0; c < kColsPerThread; ++c) {
-    const int j = tid + c * kThreads;
-    if (j < n) {
-      const double v[3] = {Vm[0][j], Vm[1][j], Vm[2][j]};
-      for (int b = 0; b < 3; ++b) {
-        double z = 0.0;
-        for (int k = 0; k < 3; ++k) z += yacc[c][k] * T[k][b];
-        double corr = 0.0;
-        for (int k = 0; k < 3; ++k) corr += v[k] * TtMT[k][b];
-        VW[6 * j + b] = v[b];
-        VW[6 * j + 3 + b] = z - 0.5 * corr;
-      }
+  for (int j = tid; j < n; j += kThreads) {
+    const double v[3] = {VW[6 * j + 0], VW[6 * j + 1], VW[6 * j + 2]};
+    const double y[3] = {VW[6 * j + 3], VW[6 * j + 4], VW[6 * j + 5]};
+    for (int b = 0; b < 3; ++b) {
+      double z = 0.0;
+      for (int k = 0; k < 3; ++k) z += y[k] * T[k][b];
+      double corr = 0.0;
+      for (int k = 0; k < 3; ++k) corr += v[k] * TtMT[k][b];
+      VW[6 * j + 3 + b] = z - 0.5 * corr;
     }
   }
+#undef PM
+#undef VM
 }
 
 // ---------------------------------------------------------------------------
@@ -451,19 +469,19 @@ __global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom
 // 3. solves with the outlier iterations, both components
 // ---------------------------------------------------------------------------
 // x <- Q^T x (apply H1, H2, H3) or Q x (apply H3, H2, H1), two vectors
-__device__ void apply_q(double (*x)[kMaxN], int n, const double* VW, const double* tau,
+__device__ void apply_q(double* x0, double* x1, int n, const double* VW, const double* tau,
                         bool transpose, double* red) {
   for (int kk = 0; kk < 3; ++kk) {
     const int k = transpose ? kk : 2 - kk;
     double p0 = 0.0, p1 = 0.0;
     for (int j = threadIdx.x; j < n; j += kThreads) {
-      p0 += VW[6 * j + k] * x[0][j];
-      p1 += VW[6 * j + k] * x[1][j];
+      p0 += VW[6 * j + k] * x0[j];
+      p1 += VW[6 * j + k] * x1[j];
     }
     const double d0 = block_sum(p0, red), d1 = block_sum(p1, red);
     for (int j = threadIdx.x; j < n; j += kThreads) {
-      x[0][j] -= tau[k] * VW[6 * j + k] * d0;
-      x[1][j] -= tau[k] * VW[6 * j + k] * d1;
+      x0[j] -= tau[k] * VW[6 * j + k] * d0;
+      x1[j] -= tau[k] * VW[6 * j + k] * d1;
     }
     __syncthreads();
   }
@@ -475,15 +493,15 @@ __device__ void apply_q(double (*x)[kMaxN], int n, const double* VW, const doubl
 // so that every block step is one coalesced pass over a block row (forward)
 // or a block column (backward) of L, reduced across threads, followed by a
 // 32 x 32 triangular matrix-vector product out of LDS.
-__device__ void chol_solve(double (*x)[kMaxN], const double* A, int n, int m) {
+__device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m) {
   static_assert(kThreads == 256 && kNB == 32, "thread layout of chol_solve");
   __shared__ double dblk[kNB][kNB + 1];
   __shared__ double tmp[2][kNB];
   __shared__ double part[2][8][kNB];
   const int tid = threadIdx.x;
   const int nblk = (m + kNB - 1) / kNB;
-  double* x0 = x[0] + 3;
-  double* x1 = x[1] + 3;
+  double* x0 = xa + 3;
+  double* x1 = xb + 3;
   // forward: y_J = inv(L_JJ) (b_J - L(J, 0:r0) y(0:r0))
   for (int J = 0; J < nblk; ++J) {
     const int r0 = J * kNB;
@@ -586,7 +604,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
     const double* __restrict__ smoothing_d,
     double threshold, double* __restrict__ smooth_slots, int64_t nslots,
     double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
-    const int32_t* __restrict__ status) {
+    const int32_t* __restrict__ status, int nm) {
   const int d = blockIdx.x;
   if (status[d]) return;
   const int64_t s0 = dom_off[d];
@@ -598,8 +616,12 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
   const double* VW = vwork + 6 * s0;
   const double smoothing = smoothing_d[d];
   const int tid = threadIdx.x;
-  __shared__ float val[2][kMaxN];   // current values (float32, as the reference)
-  __shared__ double x[2][kMaxN];    // work vector
+  // dynamic LDS (nm >= n, even): work vectors x[2][nm], then the current
+  // values val[2][nm] (float32, as the reference)
+  extern __shared__ double dyn[];
+  double* x[2] = {dyn, dyn + nm};
+  float* val[2] = {reinterpret_cast<float*>(dyn + 2 * nm),
+                   reinterpret_cast<float*>(dyn + 2 * nm) + nm};
   __shared__ double red[kThreads / 64];
   __shared__ int go[2];
   __shared__ int nerr[2];
@@ -623,13 +645,13 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
       x[1][j] = val[1][j];
     }
     __syncthreads();
-    apply_q(x, n, VW, tau, true, red);
+    apply_q(x[0], x[1], n, VW, tau, true, red);
     // a-part rhs kept in registers: vt[0..2]
     double vt[2][3];
     for (int c = 0; c < 2; ++c)
       for (int k = 0; k < 3; ++k) vt[c][k] = x[c][k];
     __syncthreads();
-    if (m > 0) chol_solve(x, A, n, m);
+    if (m > 0) chol_solve(x[0], x[1], A, n, m);
     // a = R^-1 (vt - B'[:3, 3:] z), B'[:3, 3 + j] = B'[3 + j][k] (lower)
     double bz[2][3];
     for (int k = 0; k < 3; ++k) {
@@ -654,7 +676,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
       x[tid][0] = x[tid][1] = x[tid][2] = 0.0;
     }
     __syncthreads();
-    apply_q(x, n, VW, tau, false, red);
+    apply_q(x[0], x[1], n, VW, tau, false, red);
     // smoothed = v - s w, differences, erratic vectors
     for (int c = 0; c < 2; ++c) {
       if (!go[c]) continue;  // uniform across the block
@@ -1052,8 +1074,9 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
       !refl || !vwork || !smooth_slots || !weights || !stats || !status)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), 0, s, xs, ys, dom_off, slot_pt,
-                     mat_off, smoothing, mats, refl, vwork, status);
+  hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), size_t(8) * n_max * sizeof(double),
+                     s, xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, status,
+                     int(n_max));
   // few subdomains: 8 wavefronts each; many: 4, two workgroups per CU
   if (nd <= 128)
     hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), 0, s, dom_off, mat_off, mats, vwork,
@@ -1061,9 +1084,11 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
   else
     hipLaunchKernelGGL(k_tps_chol<256>, dim3(nd), dim3(256), 0, s, dom_off, mat_off, mats, vwork,
                        status);
-  hipLaunchKernelGGL(k_tps_solve, dim3(nd), dim3(kThreads), 0, s, vx, vy, dom_off, slot_pt,
+  hipLaunchKernelGGL(k_tps_solve, dim3(nd), dim3(kThreads),
+                     size_t(3) * ((n_max + 1) & ~1) * sizeof(double), s, vx, vy, dom_off, slot_pt,
                      mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,
-                     weights, wstride, stats, status);
+                     weights, wstride, stats, status,
+                     (n_max + 1) & ~1);
   return tps_check("piv_tps_smooth");
 }
 
