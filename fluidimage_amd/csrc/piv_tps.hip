@@ -284,7 +284,7 @@ __device__ __forceinline__ dvec4 mfma_f64(double a, double b, dvec4 c) {
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom_off,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_tps_chol(const int64_t* __restrict__ dom_off,
                                                            const int64_t* __restrict__ mat_off,
                                                            double* __restrict__ mats,
                                                            const double* __restrict__ vwork,
@@ -301,6 +301,7 @@ __global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom
   __shared__ double inv[kNB][kNB + 1];       // inv(L_JJ)
   __shared__ double lbuf[kNB][kNB + 1];      // L_JJ
   __shared__ double rdiag[kNB];              // 1 / L_ii
+  __shared__ double tbuf[16][17];            // L21 X11
   __shared__ int bad;
   if (tid == 0) bad = 0;
   // B'[3:, :3] (kept for the solves)
@@ -316,70 +317,109 @@ __global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom
     const int r0 = J * kNB;
     const int bs = min(kNB, m - r0);
     const int nstrip = (m - r0 + 15) / 16;
-    // a) block-column update, strip by strip
-    for (int st = wave; st < nstrip; st += kCholWaves) {
-      const int row0 = r0 + 16 * st;
-      const int ra = min(row0 + li, m - 1);     // A operand row (clamped)
-      const int rb0 = min(r0 + li, m - 1);      // B operand rows: block J
+    // a) block-column update, two 16-row strips per wavefront at a time
+    //    (the block-row operand is shared by both strips)
+    for (int sp = wave; 2 * sp < nstrip; sp += kCholWaves) {
+      const int row0 = r0 + 32 * sp;
+      const int ra0 = min(row0 + li, m - 1);       // A operand rows (clamped)
+      const int ra1 = min(row0 + 16 + li, m - 1);
+      const int rb0 = min(r0 + li, m - 1);         // B operand rows: block J
       const int rb1 = min(r0 + 16 + li, m - 1);
-      const double* pa = A + int64_t(ra) * n + 4 * lk;
+      const double* pa0 = A + int64_t(ra0) * n + 4 * lk;
+      const double* pa1 = A + int64_t(ra1) * n + 4 * lk;
       const double* pb0 = A + int64_t(rb0) * n + 4 * lk;
       const double* pb1 = A + int64_t(rb1) * n + 4 * lk;
-      dvec4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+      dvec4 c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
+      // the rank-6 term V Z^T + Z V^T of B' as two more k-steps:
+      // a_i = [V_i, Z_i], b_j = [Z_j, V_j] (k < 6, zero beyond)
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) {
+        const int k = 4 * s2 + lk;
+        const bool kin = k < 6;
+        const int ka = kin ? k : 0, kb = kin ? (k + 3) % 6 : 0;
+        const double a0 = kin ? VC[6 * ra0 + ka] : 0.0;
+        const double a1 = kin ? VC[6 * ra1 + ka] : 0.0;
+        const double b0 = kin ? VC[6 * rb0 + kb] : 0.0;
+        const double b1 = kin ? VC[6 * rb1 + kb] : 0.0;
+        c00 = mfma_f64(a0, b0, c00);
+        c01 = mfma_f64(a0, b1, c01);
+        c10 = mfma_f64(a1, b0, c10);
+        c11 = mfma_f64(a1, b1, c11);
+      }
       // k = k0 + 4 lk + q: the lane reads 4 consecutive columns per operand;
-      // the operands of step k0 + 32 are requested while step k0 runs
-      double op[3][12];
+      // the operands of step k0 + 16 are requested while step k0 runs
+      double op[2][16];
       auto fetch = [&](double* o, int k0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          o[q] = pa[k0 + q];
-          o[4 + q] = pb0[k0 + q];
-          o[8 + q] = pb1[k0 + q];
+          o[q] = pa0[k0 + q];
+          o[4 + q] = pa1[k0 + q];
+          o[8 + q] = pb0[k0 + q];
+          o[12 + q] = pb1[k0 + q];
         }
       };
       if (r0 > 0) fetch(op[0], 0);
-      if (r0 > 16) fetch(op[1], 16);
-      for (int k0 = 0; k0 < r0; k0 += 16) {
-        if (k0 + 32 < r0) fetch(op[2], k0 + 32);
+      for (int k0 = 0; k0 < r0; k0 += 32) {
+        // two steps per iteration, ping-pong buffers
+        if (k0 + 16 < r0) fetch(op[1], k0 + 16);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          c0 = mfma_f64(op[0][q], op[0][4 + q], c0);
-          c1 = mfma_f64(op[0][q], op[0][8 + q], c1);
+          c00 = mfma_f64(op[0][q], op[0][8 + q], c00);
+          c01 = mfma_f64(op[0][q], op[0][12 + q], c01);
+          c10 = mfma_f64(op[0][4 + q], op[0][8 + q], c10);
+          c11 = mfma_f64(op[0][4 + q], op[0][12 + q], c11);
         }
+        if (k0 + 16 >= r0) break;
+        if (k0 + 32 < r0) fetch(op[0], k0 + 32);
 #pragma unroll
-        for (int e = 0; e < 12; ++e) {
-          op[0][e] = op[1][e];
-          op[1][e] = op[2][e];
+        for (int q = 0; q < 4; ++q) {
+          c00 = mfma_f64(op[1][q], op[1][8 + q], c00);
+          c01 = mfma_f64(op[1][q], op[1][12 + q], c01);
+          c10 = mfma_f64(op[1][4 + q], op[1][8 + q], c10);
+          c11 = mfma_f64(op[1][4 + q], op[1][12 + q], c11);
         }
       }
-      // C(row, col) - wz - update, rows 4 v + lk, columns li (+16)
+      // C(row, col) -= (wz + update), rows 4 v + lk (+16), columns li (+16)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = row0 + 4 * v + lk;
-        if (row < m) {
-          double* dst = A + int64_t(row) * n + r0;
-          const int col0 = li, col1 = 16 + li;
-          if (col0 < bs) dst[col0] = dst[col0] - wz_corr(VC, row, r0 + col0) - c0[v];
-          if (col1 < bs) dst[col1] = dst[col1] - wz_corr(VC, row, r0 + col1) - c1[v];
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = row0 + 16 * h + 4 * v + lk;
+          if (row < m) {
+            double* dst = A + int64_t(row) * n + r0;
+            const int col0 = li, col1 = 16 + li;
+            const double u0 = h ? c10[v] : c00[v];
+            const double u1 = h ? c11[v] : c01[v];
+            if (col0 < bs) {
+              const double r = dst[col0] - u0;
+              dst[col0] = r;
+              if (sp == 0) lbuf[16 * h + 4 * v + lk][col0] = r;  // diagonal block
+            }
+            if (col1 < bs) {
+              const double r = dst[col1] - u1;
+              dst[col1] = r;
+              if (sp == 0) lbuf[16 * h + 4 * v + lk][col1] = r;
+            }
+          }
         }
       }
     }
     __syncthreads();
-    // b) diagonal block by wavefront 0: right-looking factorisation in
-    //    registers (lane i < 32 holds row i; entries of other lanes through
-    //    readlane), then column `lane` of inv(L_JJ) by forward substitution
-    //    against L_JJ in LDS
+    // b) diagonal block by wavefront 0 (C_JJ is in lbuf, written there by
+    //    this wavefront in a): right-looking factorisation in registers
+    //    (lane i < 32 holds row i; entries of other lanes through readlane),
+    //    then inv(L_JJ) by 16 x 16 blocks: the two diagonal blocks by forward
+    //    substitution (lane c: column c of the upper one, lane 16 + c: column
+    //    c of the lower one), the off-diagonal block -X22 (L21 X11) on the
+    //    matrix cores
     if (wave == 0) {
       const bool drow = lane < kNB;
       bool singular = false;
       {
         double acc[kNB];
-        const double* src = A + int64_t(r0 + (lane < bs ? lane : 0)) * n + r0;
 #pragma unroll
-        for (int c = 0; c < kNB; ++c) {
-          const double v = src[c < bs ? c : 0];
-          acc[c] = (lane < bs && c <= lane) ? v : 0.0;
-        }
+        for (int c = 0; c < kNB; ++c)
+          acc[c] = (lane < bs && c <= lane && c < bs) ? lbuf[lane < kNB ? lane : 0][c] : 0.0;
 #pragma unroll
         for (int j = 0; j < kNB; ++j) {
           double dj = readlane_d(acc[j], j);
@@ -396,33 +436,50 @@ __global__ __launch_bounds__(NT) void k_tps_chol(const int64_t* __restrict__ dom
             for (int c = j + 1; c < kNB; ++c) acc[c] -= acc[j] * readlane_d(acc[j], c);
           }
         }
+        wave_sync();
         if (drow) {
 #pragma unroll
           for (int c = 0; c < kNB; ++c) lbuf[lane][c] = c <= lane ? acc[c] : 0.0;
+          rdiag[lane] = 1.0 / lbuf[lane][lane];  // written by this lane just above
         }
       }
       if (singular && lane == 0) bad = 1;
       wave_sync();
-      if (drow) rdiag[lane] = 1.0 / lbuf[lane][lane];
-      wave_sync();
       if (drow) {
-        // column `lane` of inv(L_JJ) into inv[.][lane]:
-        // x_i = (delta_{i, lane} - sum_{k < i} L_ik x_k) / L_ii, x_k = 0 for k < lane
-        for (int i = 0; i < kNB; ++i) {
-          double sx = i == lane ? 1.0 : 0.0;
-          int k = lane;
-          for (; k + 4 <= i; k += 4)
-            sx -= lbuf[i][k] * inv[k][lane] + lbuf[i][k + 1] * inv[k + 1][lane] +
-                  lbuf[i][k + 2] * inv[k + 2][lane] + lbuf[i][k + 3] * inv[k + 3][lane];
-          for (; k < i; ++k) sx -= lbuf[i][k] * inv[k][lane];
-          inv[i][lane] = i < lane ? 0.0 : sx * rdiag[i];
+        // X11 (o = 0) and X22 (o = 16): column c, x_i = (delta_ic -
+        // sum_{c <= k < i} L_ik x_k) / L_ii
+        const int o = lane < 16 ? 0 : 16, c = lane - o;
+#pragma unroll 1
+        for (int i = 0; i < 16; ++i) {
+          double sx = i == c ? 1.0 : 0.0;
+          for (int k = c; k < i; ++k) sx -= lbuf[o + i][o + k] * inv[o + k][o + c];
+          inv[o + i][o + c] = i < c ? 0.0 : sx * rdiag[o + i];
+          inv[o + c][16 - o + i] = 0.0;  // the strictly upper 16 x 16 block stays zero
         }
+      }
+      wave_sync();
+      {
+        // T = L21 X11, then X21 = -X22 T (16 x 16 x 16 each, k = 4 s + lk)
+        dvec4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+          t = mfma_f64(lbuf[16 + li][4 * s4 + lk], inv[4 * s4 + lk][li], t);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) tbuf[4 * v + lk][li] = t[v];
+        wave_sync();
+        dvec4 x21 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+          x21 = mfma_f64(inv[16 + li][16 + 4 * s4 + lk], tbuf[4 * s4 + lk][li], x21);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) inv[16 + 4 * v + lk][li] = -x21[v];
+      }
+      wave_sync();
+      if (lane < bs) {
         // L_JJ (lower with the diagonal) and inv(L_JJ)^T (strictly upper)
         // into the matrix, row `lane`
-        if (lane < bs) {
-          double* blk = A + int64_t(r0 + lane) * n + r0;
-          for (int c = 0; c < bs; ++c) blk[c] = c <= lane ? lbuf[lane][c] : inv[c][lane];
-        }
+        double* blk = A + int64_t(r0 + lane) * n + r0;
+        for (int c = 0; c < bs; ++c) blk[c] = c <= lane ? lbuf[lane][c] : inv[c][lane];
       }
     }
     __syncthreads();
