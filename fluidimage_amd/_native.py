@@ -83,6 +83,19 @@ class PivResults(ctypes.Structure):
     ]
 
 
+class PivDeformGrid(ctypes.Structure):
+    _fields_ = [
+        ("iyvecs", ctypes.c_void_p),
+        ("ixvecs", ctypes.c_void_p),
+        ("ny", ctypes.c_int32),
+        ("nx", ctypes.c_int32),
+        ("cell_y", ctypes.c_void_p),
+        ("t_y", ctypes.c_void_p),
+        ("cell_x", ctypes.c_void_p),
+        ("t_x", ctypes.c_void_p),
+    ]
+
+
 # exported symbol -> (restype, argtypes); mirrors include/piv_hip.h
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -121,6 +134,9 @@ SIGNATURES = {
         ctypes.c_int,
         [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P],
     ),
+    "piv_deform_windows": (
+        ctypes.c_int, [_I, _P, _I, ctypes.POINTER(PivDeformGrid), _P, _P, _I, _I, _P, _P, _P]),
+    "piv_add_predictor": (ctypes.c_int, [_I, _P, _P, _P, _P, _P]),
     "piv_fix": (
         ctypes.c_int,
         [_P, _P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P,

@@ -1,0 +1,179 @@
+// Image deformation between passes (gfx950): bicubic resampling of the
+// interrogation windows by the predictor field. This has no counterpart in
+// the reference, whose later passes shift the windows by the rounded
+// predictor (src/fluidimage/works/piv/singlepass.py:752-753,822-825).
+//
+// Method: central-difference window deformation. Window w of pair k is
+// centred on grid node (yc, xc); its pixel (r, c) is at y = yc - n0/2 + r,
+// x = xc - n1/2 + c (the reference's crop convention). The predictor (u, v)
+// at that pixel is the bilinear interpolation of the pair's field on the
+// pass grid (per-pixel cell index and fraction tables, constant outside the
+// grid's bounding box). The deformed crops are im0(y - v/2, x - u/2) and
+// im1(y + v/2, x + u/2), sampled by Keys cubic convolution (a = -0.5; R. Keys,
+// IEEE TASSP 29(6), 1981) on the image minus its minimum, zero outside the
+// image; positions, weights and sums are FP64, the crops are stored as
+// float32 window stacks that the fused correlation reads as two images.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/piv_hip.h"
+#include "last_error.hpp"
+
+namespace {
+
+constexpr int kDeformThreads = 256;
+
+template <int DT>
+__device__ __forceinline__ double pixel(const void* data, int64_t idx);
+template <>
+__device__ __forceinline__ double pixel<PIV_U8>(const void* data, int64_t idx) {
+  return double(static_cast<const uint8_t*>(data)[idx]);
+}
+template <>
+__device__ __forceinline__ double pixel<PIV_U16>(const void* data, int64_t idx) {
+  return double(static_cast<const uint16_t*>(data)[idx]);
+}
+template <>
+__device__ __forceinline__ double pixel<PIV_F32>(const void* data, int64_t idx) {
+  return double(static_cast<const float*>(data)[idx]);
+}
+
+// Keys weights of the taps at offsets -1, 0, 1, 2 for the fraction t in [0, 1)
+__device__ __forceinline__ void keys_weights(double t, double w[4]) {
+  constexpr double a = -0.5;
+  const double d0 = 1.0 + t, d1 = t, d2 = 1.0 - t, d3 = 2.0 - t;
+  // |d| <= 1: (a + 2) d^3 - (a + 3) d^2 + 1; 1 < |d| < 2: a d^3 - 5 a d^2 + 8 a d - 4 a
+  w[0] = ((a * d0 - 5.0 * a) * d0 + 8.0 * a) * d0 - 4.0 * a;
+  w[1] = ((a + 2.0) * d1 - (a + 3.0)) * d1 * d1 + 1.0;
+  w[2] = ((a + 2.0) * d2 - (a + 3.0)) * d2 * d2 + 1.0;
+  w[3] = ((a * d3 - 5.0 * a) * d3 + 8.0 * a) * d3 - 4.0 * a;
+}
+
+template <int DT>
+__device__ double bicubic(const piv_image& im, double m, double py, double px) {
+  const double fy = floor(py), fx = floor(px);
+  double wy[4], wx[4];
+  keys_weights(py - fy, wy);
+  keys_weights(px - fx, wx);
+  const int iy = int(fy) - 1, ix = int(fx) - 1;
+  double acc = 0.0;
+  for (int a = 0; a < 4; ++a) {
+    const int yy = iy + a;
+    if (yy < 0 || yy >= im.height) continue;
+    const int64_t row = int64_t(yy) * im.pitch;
+    double ra = 0.0;
+    for (int b = 0; b < 4; ++b) {
+      const int xx = ix + b;
+      if (xx < 0 || xx >= im.width) continue;
+      ra += wx[b] * (pixel<DT>(im.data, row + xx) - m);
+    }
+    acc += wy[a] * ra;
+  }
+  return acc;
+}
+
+// one workgroup per window; threads over its n0 x n1 pixels
+template <int DT>
+__global__ __launch_bounds__(kDeformThreads) void k_deform_windows(
+    const piv_image* __restrict__ images, piv_deform_grid g, const double* __restrict__ u,
+    const double* __restrict__ v, int n0, int n1, int nvec, float* __restrict__ stack0,
+    float* __restrict__ stack1) {
+  const int w = blockIdx.x;
+  const int pair = w / nvec, node = w - pair * nvec;
+  const int yc = g.iyvecs[node / g.nx], xc = g.ixvecs[node % g.nx];
+  const piv_image im0 = images[2 * pair], im1 = images[2 * pair + 1];
+  const double m0 = im0.minval ? double(*im0.minval) : 0.0;
+  const double m1 = im1.minval ? double(*im1.minval) : 0.0;
+  const double* U = u + int64_t(pair) * nvec;
+  const double* V = v + int64_t(pair) * nvec;
+  const int npx = n0 * n1;
+  float* out0 = stack0 + int64_t(w) * npx;
+  float* out1 = stack1 + int64_t(w) * npx;
+  for (int e = threadIdx.x; e < npx; e += kDeformThreads) {
+    const int r = e / n1, c = e - r * n1;
+    const int y = yc - n0 / 2 + r, x = xc - n1 / 2 + c;
+    // bilinear predictor at the pixel
+    const int jy = g.cell_y[y], jx = g.cell_x[x];
+    const double ty = g.t_y[y], tx = g.t_x[x];
+    const int jy1 = min(jy + 1, g.ny - 1), jx1 = min(jx + 1, g.nx - 1);
+    const int64_t i00 = int64_t(jy) * g.nx + jx, i01 = int64_t(jy) * g.nx + jx1;
+    const int64_t i10 = int64_t(jy1) * g.nx + jx, i11 = int64_t(jy1) * g.nx + jx1;
+    const double uu = (1.0 - ty) * ((1.0 - tx) * U[i00] + tx * U[i01]) +
+                      ty * ((1.0 - tx) * U[i10] + tx * U[i11]);
+    const double vv = (1.0 - ty) * ((1.0 - tx) * V[i00] + tx * V[i01]) +
+                      ty * ((1.0 - tx) * V[i10] + tx * V[i11]);
+    out0[e] = float(bicubic<DT>(im0, m0, y - 0.5 * vv, x - 0.5 * uu));
+    out1[e] = float(bicubic<DT>(im1, m1, y + 0.5 * vv, x + 0.5 * uu));
+  }
+}
+
+__global__ void k_add_predictor(int n, float* __restrict__ dx, float* __restrict__ dy,
+                                const double* __restrict__ u, const double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dx[i] = float(double(dx[i]) + u[i]);
+  dy[i] = float(double(dy[i]) + v[i]);
+}
+
+int deform_fail(int code, const std::string& msg) {
+  fia::last_error() = msg;
+  return code;
+}
+
+int deform_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return deform_fail(PIV_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return PIV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int piv_deform_windows(int32_t dtype, const piv_image* images, int32_t npairs,
+                       const piv_deform_grid* grid, const double* u, const double* v,
+                       int32_t n0, int32_t n1, float* stack0, float* stack1, void* stream) {
+  if (!images || !grid || !u || !v || !stack0 || !stack1)
+    return deform_fail(PIV_EINVAL, "NULL argument");
+  if (npairs <= 0 || n0 <= 0 || n1 <= 0 || grid->ny <= 0 || grid->nx <= 0)
+    return deform_fail(PIV_EINVAL, "empty deformation problem");
+  if (!grid->iyvecs || !grid->ixvecs || !grid->cell_y || !grid->t_y || !grid->cell_x ||
+      !grid->t_x)
+    return deform_fail(PIV_EINVAL, "NULL grid table");
+  const int nvec = grid->ny * grid->nx;
+  const int64_t nwin = int64_t(npairs) * nvec;
+  if (nwin > 0x7fffffff) return deform_fail(PIV_EINVAL, "too many windows");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 gr(static_cast<unsigned>(nwin)), bl(kDeformThreads);
+  switch (dtype) {
+    case PIV_U8:
+      hipLaunchKernelGGL(k_deform_windows<PIV_U8>, gr, bl, 0, s, images, *grid, u, v, n0, n1,
+                         nvec, stack0, stack1);
+      break;
+    case PIV_U16:
+      hipLaunchKernelGGL(k_deform_windows<PIV_U16>, gr, bl, 0, s, images, *grid, u, v, n0, n1,
+                         nvec, stack0, stack1);
+      break;
+    case PIV_F32:
+      hipLaunchKernelGGL(k_deform_windows<PIV_F32>, gr, bl, 0, s, images, *grid, u, v, n0, n1,
+                         nvec, stack0, stack1);
+      break;
+    default:
+      return deform_fail(PIV_EINVAL, "unknown image dtype");
+  }
+  return deform_check("piv_deform_windows");
+}
+
+int piv_add_predictor(int32_t n, float* dx, float* dy, const double* u, const double* v,
+                      void* stream) {
+  if (n <= 0) return PIV_OK;
+  if (!dx || !dy || !u || !v) return deform_fail(PIV_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_add_predictor, dim3((n + 255) / 256), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), n, dx, dy, u, v);
+  return deform_check("piv_add_predictor");
+}
+
+}  // extern "C"

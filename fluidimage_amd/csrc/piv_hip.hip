@@ -44,6 +44,17 @@ int check_launch(const char* what) {
 // fused correlation kernel
 // -------------------------------------------------------------------------
 // pixel fetch through a global-address-space pointer (no flat loads)
+// Workgroups are dispatched to the 8 XCDs of the chip round-robin
+// (workgroup b to XCD b % 8). Renumbering them so that each XCD runs one
+// contiguous range of logical blocks keeps neighbouring windows, which
+// share image rows, under the same L2. A bijection of [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int x = b % kXcd, j = b / kXcd;
+  const int q = nb / kXcd, r = nb % kXcd;
+  return x * q + min(x, r) + j;
+}
+
 template <int DT>
 __device__ __forceinline__ float gfetch(const void* data, int64_t idx) {
   if constexpr (DT == PIV_U8)
@@ -97,7 +108,7 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
 
   const int lane = threadIdx.x & (T - 1);
   const int wl = threadIdx.x / T;
-  const int win = blockIdx.x * WPB + wl;
+  const int win = xcd_block(blockIdx.x, gridDim.x) * WPB + wl;
   static_assert(N0 <= 64, "column masks hold up to 64 rows");
   if (pa.mask != nullptr) {
     for (int c = threadIdx.x; c < N1; c += T * WPB) {

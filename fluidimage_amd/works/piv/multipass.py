@@ -35,6 +35,8 @@ class WorkPIV:
                 "subdom_size": 200,
                 "smoothing_coef": 5.0,
                 "threshold_tps": 1.5,
+                # extension (no counterpart in the reference): None or "bicubic"
+                "image_deformation": None,
             },
         )
 

@@ -42,6 +42,7 @@ class MultipassPipeline:
     def __init__(self, work_piv):
         self.work = work_piv
         self.params = work_piv.params
+        self._deformed = {}
 
     def _event_pair(self, timers, key):
         import torch
@@ -114,22 +115,47 @@ class MultipassPipeline:
         correlation launches and the TPS calls."""
         import torch
 
+        from .deform import DeformedPass, add_predictor, check_method
+
         works = self.work.works_piv
         dev = imageset.device
         passes = []
         approx = [None] * npairs
+        deform = check_method(getattr(self.params.multipass, "image_deformation", None))
+        if deform and self.params.piv0.nb_peaks_to_search > 1:
+            raise NotImplementedError(
+                "image deformation with nb_peaks_to_search > 1 is not implemented")
         for ip, work in enumerate(works):
             nvec = work.nb_vectors
-            windows = torch.empty((npairs * nvec, 10), dtype=torch.int32, device=dev)
-            win = {k: [] for k in ("xs", "ys", "dx_in", "dy_in")}
-            for k in range(npairs):
-                dxa, dya = (None, None) if approx[k] is None else approx[k]
-                w = work.make_windows(2 * k, 2 * k + 1, dxa, dya,
-                                      windows=windows[k * nvec:(k + 1) * nvec], device=dev,
-                                      stream=stream)
-                for key in win:
-                    win[key].append(w[key])
-            win = {key: torch.cat(v) for key, v in win.items()}
+            deformed = deform is not None and ip > 0
+            table, dtype = imageset.table, imageset.dtype
+            if deformed:
+                # windows centred on the grid nodes, resampled by the predictor
+                u = torch.cat([approx[k][0] for k in range(npairs)])
+                v = torch.cat([approx[k][1] for k in range(npairs)])
+                dp = self._deformed.get(ip)
+                if dp is None:
+                    dp = self._deformed[ip] = DeformedPass(work, dev)
+                gx, gy = work._device_grid_full(dev)
+                win = {"xs": gx.repeat(npairs), "ys": gy.repeat(npairs), "dx_in": u, "dy_in": v}
+                ev = self._event_pair(timers, f"deform_pass{ip}")
+                if ev:
+                    ev[0].record()
+                stacks, windows = dp.run(imageset, npairs, u, v, stream=stream)
+                if ev:
+                    ev[1].record()
+                table, dtype = stacks.table, stacks.dtype
+            else:
+                windows = torch.empty((npairs * nvec, 10), dtype=torch.int32, device=dev)
+                win = {k: [] for k in ("xs", "ys", "dx_in", "dy_in")}
+                for k in range(npairs):
+                    dxa, dya = (None, None) if approx[k] is None else approx[k]
+                    w = work.make_windows(2 * k, 2 * k + 1, dxa, dya,
+                                          windows=windows[k * nvec:(k + 1) * nvec], device=dev,
+                                          stream=stream)
+                    for key in win:
+                        win[key].append(w[key])
+                win = {key: torch.cat(v) for key, v in win.items()}
             maps = None
             if keep_maps:
                 maps = torch.empty((npairs * nvec,) + work.shape_crop_im0, dtype=torch.float32,
@@ -137,11 +163,13 @@ class MultipassPipeline:
             ev = self._event_pair(timers, f"correlate_pass{ip}")
             if ev:
                 ev[0].record()
-            out = work.correl.correlate_windows(imageset.table, windows, npairs * nvec,
-                                                imageset.dtype, work.has_to_apply_subpix,
-                                                maps=maps, stream=stream)
+            out = work.correl.correlate_windows(table, windows, npairs * nvec, dtype,
+                                                work.has_to_apply_subpix, maps=maps,
+                                                stream=stream)
             if ev:
                 ev[1].record()
+            if deformed:
+                add_predictor(out["dx"], out["dy"], win["dx_in"], win["dy_in"], stream=stream)
             out["maps"] = maps
             out["windows"] = windows
             res = PassOutput(work, npairs, out, win, dev)

@@ -276,6 +276,40 @@ int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
                         const double* weights, int64_t wstride, double* out,
                         void* stream);
 
+/* ---- image deformation between passes (no counterpart in the reference,
+ *      whose later passes shift the windows by the rounded predictor,
+ *      singlepass.py:752-753,822-825) ----
+ * Pass grid: nodes iyvecs[ny] x ixvecs[nx] (row-major vectors); for every
+ * image row y and column x, the cell of the predictor's bilinear
+ * interpolation and the fraction in it (cell_y[y], t_y[y], cell_x[x],
+ * t_x[x]; clamped to the grid's bounding box). Device pointers. */
+typedef struct piv_deform_grid {
+  const int32_t* iyvecs;
+  const int32_t* ixvecs;
+  int32_t ny, nx;
+  const int32_t* cell_y;
+  const double* t_y;
+  const int32_t* cell_x;
+  const double* t_x;
+} piv_deform_grid;
+
+/* Deformed crops of the windows centred on the grid nodes for npairs pairs
+ * (images 2k, 2k + 1 of `images`, all of element type dtype, minima
+ * subtracted): im0(y - v/2, x - u/2) and im1(y + v/2, x + u/2) by Keys cubic
+ * convolution (a = -0.5), zero outside the image, with (u, v) (device,
+ * npairs x ny x nx float64) the predictor. stack0 / stack1 (device,
+ * npairs x ny x nx x n0 x n1 float32) receive the windows, window-major.
+ * `grid` is a host pointer to a record of device pointers. */
+int piv_deform_windows(int32_t dtype, const piv_image* images, int32_t npairs,
+                       const piv_deform_grid* grid, const double* u,
+                       const double* v, int32_t n0, int32_t n1, float* stack0,
+                       float* stack1, void* stream);
+
+/* dx[i] += u[i], dy[i] += v[i] (float32 results, float64 predictor): the
+ * residual of a deformed pass plus the predictor at the window centre. */
+int piv_add_predictor(int32_t n, float* dx, float* dy, const double* u,
+                      const double* v, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
