@@ -7,12 +7,13 @@ Workload "multipass" (default): P seeded 2048x2048 uint16 pairs (12-bit,
 0.03 particles/px, 8 Fourier modes up to 6 px, 2 % of the 64 px windows of
 im1 replaced by noise), WorkPIV with 3 passes (64, 32, 16 px windows, 50 %
 overlap), ThinPlateSplineSubdom between passes and at the end (use_tps True,
-subdom_size 200, smoothing_coef 2.0, threshold_tps 1.5), WorkFIX after every
-pass, 2d_gaussian2 on the last pass, correlation maps kept (the reference's
+subdom_size 200, smoothing_coef 2.0, threshold_tps 1.5), bicubic image
+deformation by the predictor in passes 2 and 3, WorkFIX after every pass,
+2d_gaussian2 on the last pass, correlation maps kept (the reference's
 results carry them). One step = the whole multipass of the P pairs as one
 batch (image minima, window records, one correlation launch per pass over
-the windows of every pair, fix, one batched FP64 TPS per pass, final
-interpolation); the images are resident in HBM before the timed region.
+the windows of every pair, deformed windows in passes 2 and 3, fix, one
+batched FP64 TPS per pass, final interpolation); the images are resident in HBM before the timed region.
 value = final-pass vectors (65,025 per pair) x P per second. The same
 pipeline on one pair (the plain per-pair path) is timed in the same run.
 
@@ -89,6 +90,7 @@ def multipass_params():
     p.multipass.subdom_size = 200
     p.multipass.smoothing_coef = 2.0
     p.multipass.threshold_tps = 1.5
+    p.multipass.image_deformation = "bicubic"
     return p
 
 
@@ -102,7 +104,7 @@ def cpu_baseline_multipass(im0, im1):
     p = default_params()
     p["piv0"]["shape_crop_im0"] = 64
     p["multipass"].update(number=3, use_tps=True, subdom_size=200, smoothing_coef=2.0,
-                          threshold_tps=1.5)
+                          threshold_tps=1.5, image_deformation="bicubic")
     with threadpool_limits(1):
         t0 = time.perf_counter()
         res = OracleWorkPIV(p, precision="f32").calcul(im0, im1)
@@ -308,7 +310,8 @@ def main():
     if args.workload == "multipass":
         desc = ("multipass PIV of {p} 2048x2048 uint16 pairs as one batch: 64/32/16 px "
                 "windows, 50% overlap, WorkFIX, GPU FP64 ThinPlateSplineSubdom (subdom 200, "
-                "smoothing 2.0, threshold 1.5) between passes and at the end, 2d_gaussian2, "
+                "smoothing 2.0, threshold 1.5) between passes and at the end, bicubic image "
+                "deformation in passes 2-3, 2d_gaussian2, "
                 "maps ").format(p=npairs)
     else:
         desc = ("single-pass batched correlation of {p} 2048x2048 uint16 pairs: 32x32 "

@@ -134,8 +134,8 @@ SIGNATURES = {
         ctypes.c_int,
         [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P],
     ),
-    "piv_deform_windows": (
-        ctypes.c_int, [_I, _P, _I, ctypes.POINTER(PivDeformGrid), _P, _P, _I, _I, _P, _P, _P]),
+    "piv_deform_images": (
+        ctypes.c_int, [_I, _P, _I, _I, _I, ctypes.POINTER(PivDeformGrid), _P, _P, _P, _P]),
     "piv_add_predictor": (ctypes.c_int, [_I, _P, _P, _P, _P, _P]),
     "piv_fix": (
         ctypes.c_int,

@@ -11,8 +11,9 @@
 // grid's bounding box). The deformed crops are im0(y - v/2, x - u/2) and
 // im1(y + v/2, x + u/2), sampled by Keys cubic convolution (a = -0.5; R. Keys,
 // IEEE TASSP 29(6), 1981) on the image minus its minimum, zero outside the
-// image; positions, weights and sums are FP64, the crops are stored as
-// float32 window stacks that the fused correlation reads as two images.
+// image; positions, weights and sums are FP64. The pixel's predictor does not
+// depend on the window, so the images are deformed once (float32) and the
+// windows are cropped from them by the fused correlation.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -74,38 +75,38 @@ __device__ double bicubic(const piv_image& im, double m, double py, double px) {
   return acc;
 }
 
-// one workgroup per window; threads over its n0 x n1 pixels
+// deformed images of pair blockIdx.z, row blockIdx.y: the predictor at a
+// pixel does not depend on the window holding it, so the deformed crop of a
+// window centred on a grid node is the crop of the deformed image there
 template <int DT>
-__global__ __launch_bounds__(kDeformThreads) void k_deform_windows(
+__global__ __launch_bounds__(kDeformThreads) void k_deform_images(
     const piv_image* __restrict__ images, piv_deform_grid g, const double* __restrict__ u,
-    const double* __restrict__ v, int n0, int n1, int nvec, float* __restrict__ stack0,
-    float* __restrict__ stack1) {
-  const int w = blockIdx.x;
-  const int pair = w / nvec, node = w - pair * nvec;
-  const int yc = g.iyvecs[node / g.nx], xc = g.ixvecs[node % g.nx];
+    const double* __restrict__ v, int nvec, float* __restrict__ out, int64_t out_stride) {
+  const int pair = blockIdx.z, y = blockIdx.y;
   const piv_image im0 = images[2 * pair], im1 = images[2 * pair + 1];
   const double m0 = im0.minval ? double(*im0.minval) : 0.0;
   const double m1 = im1.minval ? double(*im1.minval) : 0.0;
   const double* U = u + int64_t(pair) * nvec;
   const double* V = v + int64_t(pair) * nvec;
-  const int npx = n0 * n1;
-  float* out0 = stack0 + int64_t(w) * npx;
-  float* out1 = stack1 + int64_t(w) * npx;
-  for (int e = threadIdx.x; e < npx; e += kDeformThreads) {
-    const int r = e / n1, c = e - r * n1;
-    const int y = yc - n0 / 2 + r, x = xc - n1 / 2 + c;
+  const int W = im0.width;
+  float* out0 = out + int64_t(2 * pair) * out_stride + int64_t(y) * W;
+  float* out1 = out + int64_t(2 * pair + 1) * out_stride + int64_t(y) * W;
+  const int jy = g.cell_y[y];
+  const double ty = g.t_y[y];
+  const int jy1 = min(jy + 1, g.ny - 1);
+  for (int x = threadIdx.x; x < W; x += kDeformThreads) {
     // bilinear predictor at the pixel
-    const int jy = g.cell_y[y], jx = g.cell_x[x];
-    const double ty = g.t_y[y], tx = g.t_x[x];
-    const int jy1 = min(jy + 1, g.ny - 1), jx1 = min(jx + 1, g.nx - 1);
+    const int jx = g.cell_x[x];
+    const double tx = g.t_x[x];
+    const int jx1 = min(jx + 1, g.nx - 1);
     const int64_t i00 = int64_t(jy) * g.nx + jx, i01 = int64_t(jy) * g.nx + jx1;
     const int64_t i10 = int64_t(jy1) * g.nx + jx, i11 = int64_t(jy1) * g.nx + jx1;
     const double uu = (1.0 - ty) * ((1.0 - tx) * U[i00] + tx * U[i01]) +
                       ty * ((1.0 - tx) * U[i10] + tx * U[i11]);
     const double vv = (1.0 - ty) * ((1.0 - tx) * V[i00] + tx * V[i01]) +
                       ty * ((1.0 - tx) * V[i10] + tx * V[i11]);
-    out0[e] = float(bicubic<DT>(im0, m0, y - 0.5 * vv, x - 0.5 * uu));
-    out1[e] = float(bicubic<DT>(im1, m1, y + 0.5 * vv, x + 0.5 * uu));
+    out0[x] = float(bicubic<DT>(im0, m0, y - 0.5 * vv, x - 0.5 * uu));
+    out1[x] = float(bicubic<DT>(im1, m1, y + 0.5 * vv, x + 0.5 * uu));
   }
 }
 
@@ -133,38 +134,39 @@ int deform_check(const char* what) {
 
 extern "C" {
 
-int piv_deform_windows(int32_t dtype, const piv_image* images, int32_t npairs,
-                       const piv_deform_grid* grid, const double* u, const double* v,
-                       int32_t n0, int32_t n1, float* stack0, float* stack1, void* stream) {
-  if (!images || !grid || !u || !v || !stack0 || !stack1)
-    return deform_fail(PIV_EINVAL, "NULL argument");
-  if (npairs <= 0 || n0 <= 0 || n1 <= 0 || grid->ny <= 0 || grid->nx <= 0)
+int piv_deform_images(int32_t dtype, const piv_image* images, int32_t npairs,
+                      int32_t height, int32_t width, const piv_deform_grid* grid,
+                      const double* u, const double* v, float* out, void* stream) {
+  if (!images || !grid || !u || !v || !out) return deform_fail(PIV_EINVAL, "NULL argument");
+  if (npairs <= 0 || height <= 0 || width <= 0 || grid->ny <= 0 || grid->nx <= 0)
     return deform_fail(PIV_EINVAL, "empty deformation problem");
+  if (npairs > 65535 || height > 65535)
+    return deform_fail(PIV_EINVAL, "too many pairs or rows for one launch");
   if (!grid->iyvecs || !grid->ixvecs || !grid->cell_y || !grid->t_y || !grid->cell_x ||
       !grid->t_x)
     return deform_fail(PIV_EINVAL, "NULL grid table");
   const int nvec = grid->ny * grid->nx;
-  const int64_t nwin = int64_t(npairs) * nvec;
-  if (nwin > 0x7fffffff) return deform_fail(PIV_EINVAL, "too many windows");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const dim3 gr(static_cast<unsigned>(nwin)), bl(kDeformThreads);
+  const dim3 gr(1, static_cast<unsigned>(height), static_cast<unsigned>(npairs));
+  const dim3 bl(kDeformThreads);
+  const int64_t stride = int64_t(height) * width;
   switch (dtype) {
     case PIV_U8:
-      hipLaunchKernelGGL(k_deform_windows<PIV_U8>, gr, bl, 0, s, images, *grid, u, v, n0, n1,
-                         nvec, stack0, stack1);
+      hipLaunchKernelGGL(k_deform_images<PIV_U8>, gr, bl, 0, s, images, *grid, u, v, nvec, out,
+                         stride);
       break;
     case PIV_U16:
-      hipLaunchKernelGGL(k_deform_windows<PIV_U16>, gr, bl, 0, s, images, *grid, u, v, n0, n1,
-                         nvec, stack0, stack1);
+      hipLaunchKernelGGL(k_deform_images<PIV_U16>, gr, bl, 0, s, images, *grid, u, v, nvec, out,
+                         stride);
       break;
     case PIV_F32:
-      hipLaunchKernelGGL(k_deform_windows<PIV_F32>, gr, bl, 0, s, images, *grid, u, v, n0, n1,
-                         nvec, stack0, stack1);
+      hipLaunchKernelGGL(k_deform_images<PIV_F32>, gr, bl, 0, s, images, *grid, u, v, nvec, out,
+                         stride);
       break;
     default:
       return deform_fail(PIV_EINVAL, "unknown image dtype");
   }
-  return deform_check("piv_deform_windows");
+  return deform_check("piv_deform_images");
 }
 
 int piv_add_predictor(int32_t n, float* dx, float* dy, const double* u, const double* v,

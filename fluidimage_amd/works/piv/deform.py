@@ -2,13 +2,13 @@
 passes shift the windows by the rounded predictor,
 src/fluidimage/works/piv/singlepass.py:752-753,822-825).
 
-With ``params.multipass.image_deformation = "bicubic"``, pass k > 0 builds
-its interrogation windows by resampling both images with the predictor field
-interpolated to every pixel (``piv_deform_windows``: bilinear predictor on
-the pass grid, Keys cubic convolution, central differences im0(x - u/2),
-im1(x + u/2)), correlates the deformed windows with the fused kernel, and
-adds the predictor at the window centre to the residual. Windows are centred
-on the grid nodes, so the vector positions are the nodes themselves.
+With ``params.multipass.image_deformation = "bicubic"``, pass k > 0
+resamples both images with the predictor field interpolated to every pixel
+(``piv_deform_images``: bilinear predictor on the pass grid, Keys cubic
+convolution, central differences im0(x - u/2), im1(x + u/2)), correlates the
+windows centred on the grid nodes of the deformed images with the fused
+kernel, and adds the predictor at the window centre to the residual. The
+vector positions are the grid nodes themselves.
 """
 
 import ctypes
@@ -60,45 +60,25 @@ class DeformedPass:
             self.iyv.data_ptr(), self.ixv.data_ptr(), len(work.iyvecs), len(work.ixvecs),
             self.cell_y.data_ptr(), self.t_y.data_ptr(), self.cell_x.data_ptr(),
             self.t_x.data_ptr())
-        self._windows = {}
-
-    def windows(self, npairs, device):
-        """Window records reading window w of the two stacks (rows
-        [w n0, (w + 1) n0) of stack images 0 and 1)."""
-        import torch
-
-        if npairs not in self._windows:
-            n0 = self.work.shape_crop_im0[0]
-            nwin = npairs * self.work.nb_vectors
-            rec = np.zeros((nwin, 10), np.int32)
-            rec[:, 1] = 1
-            rec[:, 2] = rec[:, 4] = np.arange(nwin, dtype=np.int64) * n0
-            self._windows[npairs] = torch.from_numpy(rec).to(device)
-        return self._windows[npairs]
 
     def run(self, imageset, npairs, u, v, stream=None):
-        """Deformed windows of every pair; returns (stack ImageSet, window
-        records). u, v: float64 device tensors (npairs x nvec, pair-major)."""
+        """Deformed images of every pair (an ImageSet of 2 npairs float32
+        images, pair k at 2k, 2k + 1); u, v: float64 device tensors
+        (npairs x nvec, pair-major) of the predictor on this pass's grid."""
         import torch
 
         work = self.work
-        n0, n1 = work.shape_crop_im0
-        if tuple(work.shape_crop_im1) != (n0, n1):
-            raise ValueError("image deformation needs shape_crop_im1 == shape_crop_im0")
-        nwin = npairs * work.nb_vectors
+        H, W = work.imshape0
         dev = imageset.device
-        stack0 = torch.empty((nwin * n0, n1), dtype=torch.float32, device=dev)
-        stack1 = torch.empty_like(stack0)
+        out = torch.empty((2 * npairs, H, W), dtype=torch.float32, device=dev)
         lib = _native.load()
         _native.check(
-            lib.piv_deform_windows(int(imageset.dtype), ctypes.c_void_p(imageset.table.data_ptr()),
-                                   int(npairs), ctypes.byref(self.grid),
-                                   ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(v.data_ptr()),
-                                   int(n0), int(n1), ctypes.c_void_p(stack0.data_ptr()),
-                                   ctypes.c_void_p(stack1.data_ptr()), _native.stream_ptr(stream)),
-            "piv_deform_windows")
-        stacks = ImageSet([stack0, stack1], subtract_min=False)
-        return stacks, self.windows(npairs, dev)
+            lib.piv_deform_images(int(imageset.dtype), ctypes.c_void_p(imageset.table.data_ptr()),
+                                  int(npairs), int(H), int(W), ctypes.byref(self.grid),
+                                  ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(v.data_ptr()),
+                                  ctypes.c_void_p(out.data_ptr()), _native.stream_ptr(stream)),
+            "piv_deform_images")
+        return ImageSet([out[i] for i in range(2 * npairs)], subtract_min=False)
 
 
 def add_predictor(dx, dy, u, v, stream=None):

@@ -136,15 +136,20 @@ class MultipassPipeline:
                 dp = self._deformed.get(ip)
                 if dp is None:
                     dp = self._deformed[ip] = DeformedPass(work, dev)
-                gx, gy = work._device_grid_full(dev)
-                win = {"xs": gx.repeat(npairs), "ys": gy.repeat(npairs), "dx_in": u, "dy_in": v}
                 ev = self._event_pair(timers, f"deform_pass{ip}")
                 if ev:
                     ev[0].record()
-                stacks, windows = dp.run(imageset, npairs, u, v, stream=stream)
+                deformed_images = dp.run(imageset, npairs, u, v, stream=stream)
                 if ev:
                     ev[1].record()
-                table, dtype = stacks.table, stacks.dtype
+                table, dtype = deformed_images.table, deformed_images.dtype
+                windows = torch.empty((npairs * nvec, 10), dtype=torch.int32, device=dev)
+                for k in range(npairs):
+                    work.make_windows(2 * k, 2 * k + 1, None, None,
+                                      windows=windows[k * nvec:(k + 1) * nvec], device=dev,
+                                      stream=stream)
+                gx, gy = work._device_grid_full(dev)
+                win = {"xs": gx.repeat(npairs), "ys": gy.repeat(npairs), "dx_in": u, "dy_in": v}
             else:
                 windows = torch.empty((npairs * nvec, 10), dtype=torch.int32, device=dev)
                 win = {k: [] for k in ("xs", "ys", "dx_in", "dy_in")}

@@ -293,17 +293,19 @@ typedef struct piv_deform_grid {
   const double* t_x;
 } piv_deform_grid;
 
-/* Deformed crops of the windows centred on the grid nodes for npairs pairs
- * (images 2k, 2k + 1 of `images`, all of element type dtype, minima
- * subtracted): im0(y - v/2, x - u/2) and im1(y + v/2, x + u/2) by Keys cubic
- * convolution (a = -0.5), zero outside the image, with (u, v) (device,
- * npairs x ny x nx float64) the predictor. stack0 / stack1 (device,
- * npairs x ny x nx x n0 x n1 float32) receive the windows, window-major.
+/* Deformed images of npairs pairs (images 2k, 2k + 1 of `images`, all
+ * height x width of element type dtype, minima subtracted):
+ * im0(y - v/2, x - u/2) and im1(y + v/2, x + u/2) at every pixel (y, x) by
+ * Keys cubic convolution (a = -0.5), zero outside the image, with (u, v) the
+ * bilinear interpolation at the pixel of the predictor (device,
+ * npairs x ny x nx float64) on the grid. `out` (device float32,
+ * 2 npairs x height x width) receives them in image order; the deformed
+ * window of a grid node is the crop of these images around the node.
  * `grid` is a host pointer to a record of device pointers. */
-int piv_deform_windows(int32_t dtype, const piv_image* images, int32_t npairs,
-                       const piv_deform_grid* grid, const double* u,
-                       const double* v, int32_t n0, int32_t n1, float* stack0,
-                       float* stack1, void* stream);
+int piv_deform_images(int32_t dtype, const piv_image* images, int32_t npairs,
+                      int32_t height, int32_t width, const piv_deform_grid* grid,
+                      const double* u, const double* v, float* out,
+                      void* stream);
 
 /* dx[i] += u[i], dy[i] += v[i] (float32 results, float64 predictor): the
  * residual of a deformed pass plus the predictor at the window centre. */

@@ -49,11 +49,16 @@ def test_deformed_windows_match_oracle(window, dtype):
     dp = DeformedPass(w, dev)
     ut = torch.from_numpy(u.ravel().copy()).to(dev)
     vt = torch.from_numpy(v.ravel().copy()).to(dev)
-    stacks, _ = dp.run(images, 1, ut, vt)
+    deformed = dp.run(images, 1, ut, vt)
     torch.cuda.synchronize()
     n0, n1 = w.shape_crop_im0
-    g0 = stacks.images[0].cpu().numpy().reshape(ny * nx, n0, n1)
-    g1 = stacks.images[1].cpu().numpy().reshape(ny * nx, n0, n1)
+    d0 = deformed.images[0].cpu().numpy()
+    d1 = deformed.images[1].cpu().numpy()
+    # the deformed window of a node is the crop of the deformed images there
+    g0 = np.stack([d0[y - n0 // 2:y - n0 // 2 + n0, x - n1 // 2:x - n1 // 2 + n1]
+                   for y in w.iyvecs for x in w.ixvecs])
+    g1 = np.stack([d1[y - n0 // 2:y - n0 // 2 + n0, x - n1 // 2:x - n1 // 2 + n1]
+                   for y in w.iyvecs for x in w.ixvecs])
     e0, e1 = deformed_crops(im0, im1, w.iyvecs, w.ixvecs, u, v, (n0, n1))
     scale = max(1.0, float(np.abs(e0).max()))
     assert np.abs(g0 - e0).max() <= 2e-7 * scale + 1e-4
