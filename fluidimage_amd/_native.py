@@ -137,6 +137,12 @@ SIGNATURES = {
     "piv_deform_images": (
         ctypes.c_int, [_I, _P, _I, _I, _I, ctypes.POINTER(PivDeformGrid), _P, _P, _P, _P]),
     "piv_add_predictor": (ctypes.c_int, [_I, _P, _P, _P, _P, _P]),
+    "piv_refine_forward": (
+        ctypes.c_int,
+        [_I, _P, _I, ctypes.POINTER(PivDeformGrid), _I, _I, _P, _P, _P, _P, _P]),
+    "piv_refine_backward": (
+        ctypes.c_int,
+        [_I, _P, _I, ctypes.POINTER(PivDeformGrid), _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "piv_fix": (
         ctypes.c_int,
         [_P, _P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P,

@@ -312,6 +312,27 @@ int piv_deform_images(int32_t dtype, const piv_image* images, int32_t npairs,
 int piv_add_predictor(int32_t n, float* dx, float* dy, const double* u,
                       const double* v, void* stream);
 
+/* ---- differentiable refinement of the final field (no counterpart in the
+ *      reference, which stops after the last pass, multipass.py:185-188) ----
+ * Parameters (dx, dy): displacement at every node of `grid` (device,
+ * npairs x ny x nx float64). Field at a pixel: bilinear interpolation of
+ * the nodes. Window w of a pair: n0 x n1 pixels centred on node w, samples
+ * a = im0(p - d/2), b = im1(p + d/2) by Keys cubic convolution (images 2k,
+ * 2k + 1, minima subtracted, zero outside). loss[k] += sum over the windows
+ * of pair k of 1 - NCC(a, b) (caller zeroes loss); stats (nwin x 5 float64)
+ * keeps the window sums for the backward pass, which adds
+ * grad_out[k] * d loss[k] / d (dx, dy) into (gdx, gdy), recomputing the
+ * samples window by window. */
+int piv_refine_forward(int32_t dtype, const piv_image* images, int32_t npairs,
+                       const piv_deform_grid* grid, int32_t n0, int32_t n1,
+                       const double* dx, const double* dy, double* stats,
+                       double* loss, void* stream);
+int piv_refine_backward(int32_t dtype, const piv_image* images, int32_t npairs,
+                        const piv_deform_grid* grid, int32_t n0, int32_t n1,
+                        const double* dx, const double* dy, const double* stats,
+                        const double* grad_out, double* gdx, double* gdy,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -344,3 +344,57 @@ def test_deformed_crops_zero_and_even_integer_fields():
             assert np.allclose(c0[k], a0[y - 7:y + 9, x - 9:x + 7], atol=1e-9)
             assert np.allclose(c1[k], a1[y - 9:y + 7, x - 7:x + 9], atol=1e-9)
             k += 1
+
+
+# ---- refinement loss (oracle/refine.py): pins ----------------------------------------------
+
+
+def test_refine_loss_at_zero_is_one_minus_ncc_of_crops():
+    import torch
+
+    from oracle.refine import refine_loss
+
+    rng = np.random.default_rng(11)
+    im0 = rng.integers(0, 4000, (48, 56)).astype(np.uint16)
+    im1 = rng.integers(0, 4000, (48, 56)).astype(np.uint16)
+    iy = np.array([8, 16, 24, 32, 40])
+    ix = np.array([8, 16, 24, 32, 40, 48])
+    z = torch.zeros(iy.size * ix.size, dtype=torch.float64)
+    got = float(refine_loss(im0, im1, iy, ix, (16, 16), z, z))
+    a0 = im0.astype(float) - im0.min()
+    a1 = im1.astype(float) - im1.min()
+    expect = 0.0
+    for y in iy:
+        for x in ix:
+            a = a0[y - 8:y + 8, x - 8:x + 8].ravel()
+            b = a1[y - 8:y + 8, x - 8:x + 8].ravel()
+            expect += 1.0 - np.corrcoef(a, b)[0, 1]
+    assert np.isclose(got, expect, rtol=1e-12)
+
+
+def test_refine_loss_autograd_matches_central_differences():
+    import torch
+
+    from fluidimage_amd.synthetic import config_pair
+    from oracle.refine import refine_loss
+
+    im0, im1 = config_pair(3, shape=(72, 80))
+    iy = np.array([8, 16, 24, 32, 40, 48, 56, 63])
+    ix = np.array([8, 16, 24, 32, 40, 48, 56, 64, 71])
+    rng = np.random.default_rng(5)
+    d0 = rng.normal(0, 1.0, (2, iy.size * ix.size))
+    x = torch.tensor(d0[0], requires_grad=True)
+    y = torch.tensor(d0[1], requires_grad=True)
+    loss = refine_loss(im0, im1, iy, ix, (16, 16), x, y)
+    gx, gy = torch.autograd.grad(loss, (x, y))
+    h = 1e-3
+    for k in rng.choice(iy.size * ix.size, 6, replace=False):
+        for comp, g in ((0, gx), (1, gy)):
+            e = np.zeros_like(d0)
+            e[comp, k] = h
+            lp = float(refine_loss(im0, im1, iy, ix, (16, 16), torch.tensor(d0[0] + e[0]),
+                                   torch.tensor(d0[1] + e[1])))
+            lm = float(refine_loss(im0, im1, iy, ix, (16, 16), torch.tensor(d0[0] - e[0]),
+                                   torch.tensor(d0[1] - e[1])))
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - float(g[k])) <= 1e-4 * max(1.0, abs(fd)), (k, comp, fd, float(g[k]))
