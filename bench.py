@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=8)
-    ap.add_argument("--workload", default="multipass", choices=["multipass", "single"])
+    ap.add_argument("--workload", default="multipass", choices=["multipass", "single", "refine"])
     ap.add_argument("--no-maps", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the one-pair (plain per-pair path) measurement")
@@ -243,10 +243,86 @@ def run_bench(args, pairs, keep_maps, steps, warmup):
     return work, images, state["passes"], wall / steps * 1e3, avg
 
 
+def bench_refine(args):
+    """Differentiable refinement: P 1024x1024 float32 pairs, multipass of
+    32x32 windows (2 passes 64/32, TPS), then 20 gradient steps over the batch
+    (a step is kept per pair only when it lowers the pair's loss). value =
+    refined vectors x pairs / time of the 20 steps."""
+    import torch
+
+    from fluidimage_amd.device import ImageSet, to_device_image
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+    from fluidimage_amd.works.piv.pipeline import MultipassPipeline
+    from fluidimage_amd.works.piv.refine import RefineProblem, refine
+
+    npairs = max(1, args.pairs)
+    print(f"bench: generating {npairs} 1024x1024 float32 pairs", file=sys.stderr, flush=True)
+    pairs = [config_pair(5, pair_idx=k) for k in range(npairs)]
+    p = WorkPIV.create_default_params()
+    p.piv0.shape_crop_im0 = 64
+    p.multipass.number = 2
+    p.multipass.use_tps = True
+    p.multipass.smoothing_coef = 2.0
+    work = WorkPIV(p)
+    work._prepare_with_image(imshape=pairs[0][0].shape)
+    dev = torch.device("cuda:0")
+    ims = []
+    for a, b in pairs:
+        ims += [to_device_image(a, dev), to_device_image(b, dev)]
+    images = ImageSet(ims)
+    passes = MultipassPipeline(work).run(images, npairs)
+    last = work.works_piv[-1]
+    prob = RefineProblem(last, images, npairs)
+    dx0 = torch.stack([passes[-1].final[k][0] for k in range(npairs)])
+    dy0 = torch.stack([passes[-1].final[k][1] for k in range(npairs)])
+    for _ in range(args.warmup):
+        refine(prob, dx0, dy0, steps=1)
+    torch.cuda.synchronize()
+    reps = max(1, args.steps // 10)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, _, l0, l1, acc = refine(prob, dx0, dy0, steps=20)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    nvec = last.nb_vectors
+    line = {
+        "metric": "refined vectors/s (20 gradient steps, batched pairs)",
+        "value": round(nvec * npairs / dt, 1),
+        "unit": "vectors/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"differentiable refinement of {npairs} 1024x1024 float32 pairs: "
+                        "windowed NCC over Keys-warped 32x32 windows, bilinear node field, "
+                        "20 accept-if-better gradient steps",
+            "pairs": npairs,
+            "vectors_per_pair": int(nvec),
+            "mean_loss_before": round(float(l0.mean()), 4),
+            "mean_loss_after": round(float(l1.mean()), 4),
+            "mean_accepted_steps": round(float(acc.double().mean()), 2),
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "refine":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        bench_refine(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "

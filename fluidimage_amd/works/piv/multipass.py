@@ -35,8 +35,11 @@ class WorkPIV:
                 "subdom_size": 200,
                 "smoothing_coef": 5.0,
                 "threshold_tps": 1.5,
-                # extension (no counterpart in the reference): None or "bicubic"
+                # extensions (no counterpart in the reference): image
+                # deformation between passes (None or "bicubic") and gradient
+                # refinement steps after the last pass (0: none)
                 "image_deformation": None,
+                "refine_steps": 0,
             },
         )
 

@@ -29,6 +29,7 @@ class PassOutput:
         self.dev = dev
         self.approx = [None] * npairs  # predictor on the next grid, per pair
         self.final = [None] * npairs
+        self.refined = [None] * npairs
         self.tps = [None] * npairs
 
     def pair(self, key, k):
@@ -43,6 +44,7 @@ class MultipassPipeline:
         self.work = work_piv
         self.params = work_piv.params
         self._deformed = {}
+        self._refine_problem = {}
 
     def _event_pair(self, timers, key):
         import torch
@@ -203,7 +205,34 @@ class MultipassPipeline:
             else:
                 for k in range(npairs):
                     res.final[k] = final[k]
+                steps = int(getattr(self.params.multipass, "refine_steps", 0) or 0)
+                if steps > 0:
+                    self._refine(imageset, res, steps, timers)
         return passes
+
+    def _refine(self, imageset, res, steps, timers=None):
+        """Differentiable refinement (refine.py) of the final fields of all
+        pairs from the multipass result (the fallback: a step is kept only
+        when it lowers its pair's loss)."""
+        import torch
+
+        from .refine import RefineProblem, refine
+
+        key = id(imageset)
+        prob = self._refine_problem.get(key)
+        if prob is None or prob.npairs != res.npairs:
+            prob = RefineProblem(res.work, imageset, res.npairs)
+            self._refine_problem = {key: prob}
+        ev = self._event_pair(timers, "refine")
+        if ev:
+            ev[0].record()
+        dx0 = torch.stack([res.final[k][0] for k in range(res.npairs)])
+        dy0 = torch.stack([res.final[k][1] for k in range(res.npairs)])
+        dx, dy, l0, l1, acc = refine(prob, dx0, dy0, steps=steps)
+        if ev:
+            ev[1].record()
+        for k in range(res.npairs):
+            res.refined[k] = (dx[k], dy[k], float(l0[k]), float(l1[k]), int(acc[k]))
 
     def _secondary_peaks(self, res):
         """WorkFIX secondary-peak replacement (fix.py:135-243) on the host,
@@ -310,6 +339,12 @@ class MultipassPipeline:
                     fy[sel] = sm[1]
                     r.xs_smooth, r.ys_smooth = xs_s, ys_s
                     r.deltaxs_smooth, r.deltays_smooth = fx, fy
+                if res.refined[k] is not None:
+                    rx, ry, l0, l1, acc = res.refined[k]
+                    r.deltaxs_refined = rx.cpu().numpy()
+                    r.deltays_refined = ry.cpu().numpy()
+                    r.refine_loss = (l0, l1)
+                    r.refine_accepted_steps = acc
             results.append(r)
         return results
 

@@ -122,3 +122,24 @@ def test_refinement_keeps_only_improving_steps():
     # the returned field has the returned loss
     chk, _ = prob.forward(dx.contiguous(), dy.contiguous())
     assert torch.allclose(chk, l1, rtol=1e-12, atol=0)
+
+
+def test_workpiv_refine_steps_parameter():
+    """params.multipass.refine_steps runs the refinement after the last pass;
+    the refined field is stored beside the multipass one, never worse."""
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+
+    im0, im1 = config_pair(2, shape=(256, 256))
+    p = WorkPIV.create_default_params()
+    p.piv0.shape_crop_im0 = 32
+    p.multipass.number = 2
+    p.multipass.use_tps = True
+    p.multipass.refine_steps = 10
+    res = WorkPIV(p).calcul_from_arrays(im0, im1)
+    last = res.passes[-1]
+    l0, l1 = last.refine_loss
+    assert l1 <= l0
+    assert last.deltaxs_refined.shape == last.deltaxs_final.shape
+    if last.refine_accepted_steps == 0:
+        assert np.array_equal(last.deltaxs_refined, last.deltaxs_final)
