@@ -237,11 +237,12 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
     wstride = nslots + 3 * nd
     weights = torch.empty((2, wstride), **f64)
     stats = torch.zeros((nd, 6), **f64)
+    slot_xy = torch.empty((2, nslots), **f64)
     _native.check(
         lib.piv_tps_smooth(nd, n_max, _p(cx), _p(cy), _p(vx), _p(vy), _p(dom_off), _p(slot_pt),
                            _p(mat_off), _p(smoothing), thr, _p(mats), _p(refl), _p(vwork),
-                           _p(smooth_slots), nslots, _p(weights), wstride, _p(stats),
-                           _p(status), sp),
+                           _p(slot_xy), _p(smooth_slots), nslots, _p(weights), wstride,
+                           _p(stats), _p(status), sp),
         "piv_tps_smooth")
     smoothed = torch.empty((2, ntot), **f64)
     for f in active:
@@ -257,7 +258,7 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
     py = new_y.repeat(na)
     approx = torch.empty((2, na * m), **f64)
     _native.check(lib.piv_tps_interpolate(na * m, _p(px), _p(py), _p(n_dom), _p(n_coef),
-                                          _p(n_norm), _p(cx), _p(cy), _p(dom_off), _p(slot_pt),
+                                          _p(n_norm), _p(slot_xy), nslots, _p(dom_off),
                                           _p(weights), wstride, _p(approx), sp),
                   "piv_tps_interpolate")
     st = status.cpu().numpy()

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
     const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
     double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
-    int32_t* __restrict__ status, int nm) {
+    double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm) {
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
@@ -88,6 +88,8 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
     const int p = slot_pt[s0 + j];
     cy[j] = ys[p];
     cx[j] = xs[p];
+    slot_xy[s0 + j] = xs[p];  // center coordinates by slot, for the evaluation
+    slot_xy[nslots + s0 + j] = ys[p];
     PM(0, j) = 1.0;
     PM(1, j) = ys[p];
     PM(2, j) = xs[p];
@@ -1062,11 +1064,11 @@ __global__ void k_tps_blend(int npts, const int32_t* __restrict__ slots,
 __global__ __launch_bounds__(256) void k_tps_interp(
     int npts, const double* __restrict__ px, const double* __restrict__ py,
     const int32_t* __restrict__ doms, const double* __restrict__ coef,
-    const double* __restrict__ norm, const double* __restrict__ cx,
-    const double* __restrict__ cy, const int64_t* __restrict__ dom_off,
-    const int32_t* __restrict__ slot_pt, const double* __restrict__ weights, int64_t wstride,
+    const double* __restrict__ norm, const double* __restrict__ slot_xy, int64_t nslots,
+    const int64_t* __restrict__ dom_off, const double* __restrict__ weights, int64_t wstride,
     double* __restrict__ out) {
-  // one wavefront per new position
+  // one wavefront per new position; the centers of a subdomain are read by
+  // slot (coordinates and weights contiguous)
   const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (p >= npts) return;
@@ -1080,10 +1082,11 @@ __global__ __launch_bounds__(256) void k_tps_interp(
     const int n = int(dom_off[d + 1] - s0);
     const double* W0 = weights + s0 + 3 * int64_t(d);
     const double* W1 = W0 + wstride;
+    const double* SX = slot_xy + s0;
+    const double* SY = slot_xy + nslots + s0;
     double v0 = 0.0, v1 = 0.0;
     for (int j = lane; j < n; j += 64) {
-      const int c = slot_pt[s0 + j];
-      const double dy = y - cy[c], dx = x - cx[c];
+      const double dy = y - SY[j], dx = x - SX[j];
       const double ph = tps_phi(dy * dy + dx * dx);
       v0 += W0[j] * ph;
       v1 += W1[j] * ph;
@@ -1121,19 +1124,19 @@ extern "C" {
 int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys,
                    const float* vx, const float* vy, const int64_t* dom_off,
                    const int32_t* slot_pt, const int64_t* mat_off, const double* smoothing,
-                   double threshold, double* mats, double* refl, double* vwork,
+                   double threshold, double* mats, double* refl, double* vwork, double* slot_xy,
                    double* smooth_slots, int64_t nslots, double* weights, int64_t wstride,
                    double* stats, int32_t* status, void* stream) {
   if (nd <= 0) return PIV_OK;
   if (n_max > kMaxN)
     return tps_fail(PIV_EUNSUPPORTED, "more than 2048 centers in a TPS subdomain");
   if (!xs || !ys || !vx || !vy || !dom_off || !slot_pt || !mat_off || !smoothing || !mats ||
-      !refl || !vwork || !smooth_slots || !weights || !stats || !status)
+      !refl || !vwork || !slot_xy || !smooth_slots || !weights || !stats || !status)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), size_t(8) * n_max * sizeof(double),
-                     s, xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, status,
-                     int(n_max));
+                     s, xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy,
+                     nslots, status, int(n_max));
   // few subdomains: 8 wavefronts each; many: 4, two workgroups per CU
   if (nd <= 128)
     hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), 0, s, dom_off, mat_off, mats, vwork,
@@ -1210,16 +1213,15 @@ int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef, const 
 }
 
 int piv_tps_interpolate(int32_t npts, const double* px, const double* py, const int32_t* doms,
-                        const double* coef, const double* norm, const double* cx,
-                        const double* cy, const int64_t* dom_off, const int32_t* slot_pt,
-                        const double* weights, int64_t wstride, double* out, void* stream) {
+                        const double* coef, const double* norm, const double* slot_xy,
+                        int64_t nslots, const int64_t* dom_off, const double* weights,
+                        int64_t wstride, double* out, void* stream) {
   if (npts <= 0) return PIV_OK;
-  if (!px || !py || !doms || !coef || !norm || !cx || !cy || !dom_off || !slot_pt || !weights ||
-      !out)
+  if (!px || !py || !doms || !coef || !norm || !slot_xy || !dom_off || !weights || !out)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipLaunchKernelGGL(k_tps_interp, dim3((npts + 3) / 4), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), npts, px, py, doms, coef, norm, cx, cy,
-                     dom_off, slot_pt, weights, wstride, out);
+                     static_cast<hipStream_t>(stream), npts, px, py, doms, coef, norm, slot_xy,
+                     nslots, dom_off, weights, wstride, out);
   return tps_check("piv_tps_interpolate");
 }
 

@@ -188,7 +188,8 @@ int piv_fix(float* dx, float* dy, const float* correl_max, const int32_t* err,
  * Subdomain d holds the slots [dom_off[d], dom_off[d+1]) (at most 2048);
  * slot s is center slot_pt[s] of xs/ys (float64) with values vx/vy
  * (float32). mats: workspace of sum_d n_d^2 doubles at mat_off[d]; refl:
- * 16 doubles per subdomain; vwork: 6 doubles per slot. Outputs: the
+ * 16 doubles per subdomain; vwork: 6 doubles per slot; slot_xy receives the
+ * center coordinates by slot (x[nslots] then y[nslots]). Outputs: the
  * smoothed value of every slot (smooth_slots[2][nslots]), the weights of
  * every subdomain (weights[2][wstride], n_d + 3 values at
  * dom_off[d] + 3 d: the n_d center weights then the [1, y, x] terms),
@@ -202,7 +203,7 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    const double* ys, const float* vx, const float* vy,
                    const int64_t* dom_off, const int32_t* slot_pt,
                    const int64_t* mat_off, const double* smoothing, double threshold,
-                   double* mats, double* refl, double* vwork,
+                   double* mats, double* refl, double* vwork, double* slot_xy,
                    double* smooth_slots, int64_t nslots, double* weights,
                    int64_t wstride, double* stats, int32_t* status,
                    void* stream);
@@ -267,14 +268,14 @@ int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef,
 /* TPS evaluation at npts new positions (px, py): for each subdomain
  * doms[4 p + k] (-1 terminated) of the point, the weighted sum of
  * w_j r^2 ln(r^2) / 2 over its centers plus the linear part, blended with
- * coef[4 p + k] and divided by norm[p] (thin_plate_spline_subdom.py:224-234).
+ * coef[4 p + k] and divided by norm[p] (thin_plate_spline_subdom.py:224-234);
+ * slot_xy: the center coordinates by slot written by piv_tps_smooth.
  * out[2][npts]. */
 int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
                         const int32_t* doms, const double* coef,
-                        const double* norm, const double* cx, const double* cy,
-                        const int64_t* dom_off, const int32_t* slot_pt,
-                        const double* weights, int64_t wstride, double* out,
-                        void* stream);
+                        const double* norm, const double* slot_xy, int64_t nslots,
+                        const int64_t* dom_off, const double* weights,
+                        int64_t wstride, double* out, void* stream);
 
 /* ---- image deformation between passes (no counterpart in the reference,
  *      whose later passes shift the windows by the rounded predictor,
