@@ -302,12 +302,11 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
 }
 
 // peak search (+ sub-pixel) on maps given by the caller
-template <int N0, int N1, int WPB>
-__global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_peaks(float* __restrict__ maps,
-                                                                 const double* __restrict__ norms,
-                                                                 int nwin, int apply_subpix,
-                                                                 piv_results res, PlanArgs pa) {
-  constexpr int T = Shape<N0, N1>::T;
+template <int N0, int N1, int WPB, int T>
+__global__ __launch_bounds__(T * WPB) void k_peaks(float* __restrict__ maps,
+                                                   const double* __restrict__ norms,
+                                                   int nwin, int apply_subpix,
+                                                   piv_results res, PlanArgs pa) {
   constexpr int NN = N0 * N1;
   __shared__ float smem[WPB * 2 * NN];
   const int lane = threadIdx.x & (T - 1);
@@ -352,6 +351,246 @@ __global__ void k_subpix_at(const float* __restrict__ maps, const int32_t* __res
 }
 
 // -------------------------------------------------------------------------
+// large windows (128 x 128): LDS-staged FFTs, one workgroup per window.
+// A line of N = R1 R2 points is transformed in LDS in two register passes
+// (R1-point FFTs of the R2 interleaved subsequences, twiddles, R2-point
+// FFTs), which leaves frequency k = k1 + R1 k2 at position k1 R2 + k2; the
+// inverse consumes that order and returns the natural one. The spectrum
+// separation works on (k, -k) pairs, so it runs in place.
+// -------------------------------------------------------------------------
+constexpr int kBigThreads = 256;
+
+template <int N>
+struct BigFFT {
+  static constexpr int R1 = N == 128 ? 16 : 8;
+  static constexpr int R2 = N / R1;
+  static constexpr int P = N + 1;  // padded row stride (float2) of the LDS buffer
+  __device__ static int pos(int k) { return (k % R1) * R2 + k / R1; }
+  __device__ static int freq(int p) { return p / R2 + R1 * (p % R2); }
+};
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// forward line FFT in place: natural order in, position k1 R2 + k2 holds
+// frequency k1 + R1 k2; tw[m] = exp(-2 pi i m / N)
+template <int N>
+__device__ void big_fft_fwd(float2* L, int st, const float2* tw) {
+  constexpr int R1 = BigFFT<N>::R1, R2 = BigFFT<N>::R2;
+  for (int j2 = 0; j2 < R2; ++j2) {
+    float2 v[R1];
+#pragma unroll
+    for (int j1 = 0; j1 < R1; ++j1) v[j1] = L[(j1 * R2 + j2) * st];
+    fft<R1, -1>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) L[(k1 * R2 + j2) * st] = cmulf(v[k1], tw[j2 * k1]);
+  }
+  for (int k1 = 0; k1 < R1; ++k1) {
+    float2 v[R2];
+#pragma unroll
+    for (int j2 = 0; j2 < R2; ++j2) v[j2] = L[(k1 * R2 + j2) * st];
+    fft<R2, -1>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) L[(k1 * R2 + k2) * st] = v[k2];
+  }
+}
+
+// unnormalised inverse line FFT in place: permuted order in, natural out
+template <int N>
+__device__ void big_fft_inv(float2* L, int st, const float2* tw) {
+  constexpr int R1 = BigFFT<N>::R1, R2 = BigFFT<N>::R2;
+  for (int k1 = 0; k1 < R1; ++k1) {
+    float2 v[R2];
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k2] = L[(k1 * R2 + k2) * st];
+    fft<R2, +1>(v);
+#pragma unroll
+    for (int j2 = 0; j2 < R2; ++j2) {
+      const float2 w = tw[j2 * k1];
+      L[(k1 * R2 + j2) * st] = cmulf(v[j2], make_float2(w.x, -w.y));
+    }
+  }
+  for (int j2 = 0; j2 < R2; ++j2) {
+    float2 v[R1];
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) v[k1] = L[(k1 * R2 + j2) * st];
+    fft<R1, +1>(v);
+#pragma unroll
+    for (int j1 = 0; j1 < R1; ++j1) L[(j1 * R2 + j2) * st] = v[j1];
+  }
+}
+
+__device__ __forceinline__ double big_block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int k = 0; k < kBigThreads / 64; ++k) t += red[k];
+  return t;
+}
+
+template <int N, int DT>
+__global__ __launch_bounds__(kBigThreads) void k_correlate_big(
+    const piv_image* __restrict__ images, const piv_window* __restrict__ windows, int nwin,
+    int options, float* __restrict__ maps, piv_results res, PlanArgs pa) {
+  using F = BigFFT<N>;
+  constexpr int P = F::P, NN = N * N;
+  static_assert(size_t(N) * P * sizeof(float2) >= 2 * size_t(NN) * sizeof(float),
+                "the map and its search copy reuse the spectrum buffer");
+  __shared__ float2 Z[N * P];
+  __shared__ float2 tw[N];
+  __shared__ double red[kBigThreads / 64];
+  __shared__ int sqs[4];
+  const int tid = threadIdx.x;
+  const int win = xcd_block(blockIdx.x, gridDim.x);  // grid = nwin, one window per block
+  const piv_window w = windows[win];
+  if (w.flags & PIV_WINFLAG_INVALID) {  // uniform across the block
+    const float nan = __builtin_nanf("");
+    if (tid == 0) {
+      res.dx[win] = nan;
+      res.dy[win] = nan;
+      res.correl_max[win] = nan;
+      res.err[win] = PIV_WIN_BAD_CROP;
+      if (res.norm) res.norm[win] = nan;
+      if (res.nsec) res.nsec[win] = 0;
+    }
+    if (maps != nullptr)
+      for (int k = tid; k < NN; k += kBigThreads) maps[size_t(win) * NN + k] = nan;
+    return;
+  }
+  for (int m = tid; m < N; m += kBigThreads) {
+    double sn, cs;
+    sincospi(-2.0 * m / N, &sn, &cs);
+    tw[m] = make_float2(float(cs), float(sn));
+  }
+  // ---- 1. crop (zero outside the image) and mean removal -----------------
+  const piv_image im0 = images[w.image0];
+  const piv_image im1 = images[w.image1];
+  const float m0 = im0.minval ? *im0.minval : 0.f;
+  const float m1 = im1.minval ? *im1.minval : 0.f;
+  float s0 = 0.f, s1 = 0.f;
+  for (int e = tid; e < NN; e += kBigThreads) {
+    const int r = e / N, c = e % N;
+    const int ya = w.y0 + r, xa = w.x0 + c, yb = w.y1 + r, xb = w.x1 + c;
+    float a = 0.f, b = 0.f;
+    if (ya >= 0 && ya < im0.height && xa >= 0 && xa < im0.width)
+      a = fetch_px<DT>(im0.data, int64_t(ya) * im0.pitch + xa) - m0;
+    if (yb >= 0 && yb < im1.height && xb >= 0 && xb < im1.width)
+      b = fetch_px<DT>(im1.data, int64_t(yb) * im1.pitch + xb) - m1;
+    Z[r * P + c] = make_float2(a, b);
+    s0 += a;
+    s1 += b;
+  }
+  const float mean0 = float(big_block_sum(double(s0), red)) / float(NN);
+  const float mean1 = float(big_block_sum(double(s1), red)) / float(NN);
+  for (int e = tid; e < NN; e += kBigThreads) {
+    float2& z = Z[(e / N) * P + e % N];
+    z.x -= mean0;
+    z.y -= mean1;
+  }
+  __syncthreads();
+  // ---- 2. forward 2-D FFT of im0' + i im1' (columns, then rows) ----------
+  if (tid < N) big_fft_fwd<N>(Z + tid, P, tw);
+  __syncthreads();
+  if (tid < N) big_fft_fwd<N>(Z + tid * P, 1, tw);
+  __syncthreads();
+  // ---- 3. F0, F1 from Z(k), Z(-k); energies; cross spectrum (in place, by
+  //         (k, -k) pairs: cross(-k) = conj(cross(k))) ----------------------
+  double e0 = 0.0, e1 = 0.0;
+  for (int e = tid; e < NN; e += kBigThreads) {
+    const int pr = e / N, pc = e % N;
+    const int kr = F::freq(pr), kc = F::freq(pc);
+    const int mr = kr == 0 ? 0 : N - kr, mc = kc == 0 ? 0 : N - kc;
+    const int qr = F::pos(mr), qc = F::pos(mc);
+    const int e2 = qr * N + qc;
+    if (e2 < e) continue;
+    const float2 a = Z[pr * P + pc];
+    const float2 zm = Z[qr * P + qc];
+    const float2 b = make_float2(zm.x, -zm.y);
+    float2 f0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+    const float2 d = make_float2(a.x - b.x, a.y - b.y);
+    float2 f1 = make_float2(0.5f * d.y, -0.5f * d.x);
+    if (kr == 0 && kc == 0) {
+      f0 = make_float2(0.f, 0.f);
+      f1 = make_float2(0.f, 0.f);
+    }
+    const float p0 = f0.x * f0.x + f0.y * f0.y, p1 = f1.x * f1.x + f1.y * f1.y;
+    if (kc <= N / 2) {
+      const float wgt = (kc == 0 || kc == N / 2) ? 1.f : 2.f;
+      e0 += double(wgt * p0);
+      e1 += double(wgt * p1);
+    }
+    const float2 cr = make_float2(f0.x * f1.x + f0.y * f1.y, f0.x * f1.y - f0.y * f1.x);
+    Z[pr * P + pc] = cr;
+    if (e2 != e) {
+      if (mc <= N / 2) {
+        const float wgt = (mc == 0 || mc == N / 2) ? 1.f : 2.f;
+        e0 += double(wgt * p0);
+        e1 += double(wgt * p1);
+      }
+      Z[qr * P + qc] = make_float2(cr.x, -cr.y);
+    }
+  }
+  e0 = big_block_sum(e0, red);
+  e1 = big_block_sum(e1, red);
+  const double norm = sqrt(e0 * e1);
+  // ---- 4. inverse 2-D FFT (rows, then columns) -----------------------------
+  if (tid < N) big_fft_inv<N>(Z + tid * P, 1, tw);
+  __syncthreads();
+  if (tid < N) big_fft_inv<N>(Z + tid, P, tw);
+  __syncthreads();
+  // ---- 5. map in the reference layout (g(a) = (n - 1 - a + n // 2) mod n)
+  //         and its search copy, in the same LDS --------------------------
+  float vals[NN / kBigThreads];
+#pragma unroll
+  for (int i = 0; i < NN / kBigThreads; ++i) {
+    const int e = tid + i * kBigThreads;
+    vals[i] = Z[(e / N) * P + e % N].x;
+  }
+  __syncthreads();
+  float* mapf = reinterpret_cast<float*>(Z);
+  float* srch = mapf + NN;
+#pragma unroll
+  for (int i = 0; i < NN / kBigThreads; ++i) {
+    const int e = tid + i * kBigThreads;
+    const int g0 = (N - 1 - e / N + N / 2) % N, g1 = (N - 1 - e % N + N / 2) % N;
+    const int k = g0 * N + g1;
+    mapf[k] = vals[i];
+    srch[k] = (pa.mask != nullptr && pa.mask[k]) ? __builtin_nanf("") : vals[i];
+  }
+  __syncthreads();
+  // ---- 6. peak search and sub-pixel fit by wavefront 0 -------------------
+  if (tid < 64) {
+    int sq[4] = {0, 0, 0, 0};
+    if (tid == 0 && res.norm) res.norm[win] = norm;
+    finish_window<N, N, 64>(mapf, srch, pa, tid, win, norm, options & PIV_OPT_SUBPIX, w.dx_in,
+                            w.dy_in, res, sq);
+    if (tid == 0) {
+      sqs[0] = sq[0];
+      sqs[1] = sq[1];
+      sqs[2] = sq[2];
+      sqs[3] = sq[3];
+    }
+  }
+  __syncthreads();
+  if (maps != nullptr) {
+    float* out = maps + size_t(win) * NN;
+    const bool side = !pa.has_dmax && (options & PIV_OPT_MAP_SQUARE);
+    for (int k = tid; k < NN; k += kBigThreads) {
+      float v = mapf[k];
+      if (side) {
+        const int i0 = k / N, i1 = k % N;
+        if (i0 >= sqs[0] && i0 < sqs[1] && i1 >= sqs[2] && i1 < sqs[3]) v = __builtin_nanf("");
+      }
+      out[k] = v;
+    }
+  }
+}
+
+// -------------------------------------------------------------------------
 // shape dispatch
 // -------------------------------------------------------------------------
 struct Launchers {
@@ -361,8 +600,8 @@ struct Launchers {
   void (*peaks)(dim3, dim3, hipStream_t, float*, const double*, int, int, piv_results, PlanArgs);
   void (*subpix)(dim3, dim3, hipStream_t, const float*, const int32_t*, const int32_t*, int,
                  float*, float*, int32_t*, PlanArgs);
-  int wpb;
-  int lanes;
+  int cwpb, cthreads;  // correlation: windows and threads per workgroup
+  int pwpb, pthreads;  // peaks from maps: windows and threads per workgroup
 };
 
 template <int N0, int N1>
@@ -380,11 +619,32 @@ void launch_correlate(dim3 g, dim3 b, hipStream_t s, const piv_image* im, const 
                      maps, r, pa);
 }
 
+template <int N, int DT>
+void launch_correlate_big(dim3 g, dim3 b, hipStream_t s, const piv_image* im,
+                          const piv_window* wi, int n, int sp, float* maps, piv_results r,
+                          PlanArgs pa) {
+  hipLaunchKernelGGL((k_correlate_big<N, DT>), g, b, 0, s, im, wi, n, sp, maps, r, pa);
+}
+
+// the search of one window runs on at most one wavefront
+template <int N0, int N1>
+constexpr int search_lanes() {
+  return Shape<N0, N1>::T < 64 ? Shape<N0, N1>::T : 64;
+}
+
+template <int N0, int N1>
+constexpr int peaks_wpb() {
+  constexpr int T = search_lanes<N0, N1>();
+  int w = 256 / T;
+  while (w > 1 && w * N0 * N1 * 8 > 65536) --w;
+  return w;
+}
+
 template <int N0, int N1>
 void launch_peaks(dim3 g, dim3 b, hipStream_t s, float* maps, const double* norms, int n, int sp,
                   piv_results r, PlanArgs pa) {
-  hipLaunchKernelGGL((k_peaks<N0, N1, default_wpb<N0, N1>()>), g, b, 0, s, maps, norms, n, sp, r,
-                     pa);
+  hipLaunchKernelGGL((k_peaks<N0, N1, peaks_wpb<N0, N1>(), search_lanes<N0, N1>()>), g, b, 0, s,
+                     maps, norms, n, sp, r, pa);
 }
 
 template <int N0, int N1>
@@ -402,14 +662,30 @@ constexpr Launchers make_launchers() {
                    &launch_peaks<N0, N1>,
                    &launch_subpix<N0, N1>,
                    default_wpb<N0, N1>(),
-                   Shape<N0, N1>::T};
+                   Shape<N0, N1>::T * default_wpb<N0, N1>(),
+                   peaks_wpb<N0, N1>(),
+                   search_lanes<N0, N1>() * peaks_wpb<N0, N1>()};
+}
+
+template <int N>
+constexpr Launchers make_launchers_big() {
+  return Launchers{N,
+                   N,
+                   {&launch_correlate_big<N, PIV_U8>, &launch_correlate_big<N, PIV_U16>,
+                    &launch_correlate_big<N, PIV_F32>},
+                   &launch_peaks<N, N>,
+                   &launch_subpix<N, N>,
+                   1,
+                   kBigThreads,
+                   peaks_wpb<N, N>(),
+                   search_lanes<N, N>() * peaks_wpb<N, N>()};
 }
 
 const Launchers kShapes[] = {
     make_launchers<12, 12>(), make_launchers<16, 16>(), make_launchers<20, 20>(),
     make_launchers<24, 24>(), make_launchers<32, 32>(), make_launchers<40, 40>(),
     make_launchers<48, 48>(), make_launchers<64, 64>(), make_launchers<64, 32>(),
-    make_launchers<32, 64>(),
+    make_launchers<32, 64>(), make_launchers_big<128>(),
 };
 
 const Launchers* find_shape(int n0, int n1) {
@@ -754,8 +1030,7 @@ int piv_correlate_batch(piv_plan* plan, int32_t dtype, const piv_image* images,
   if (plan->pa.nb_peaks > 1 && (res->sec == nullptr || res->nsec == nullptr))
     return fail(PIV_EINVAL, "secondary peak arrays required when nb_peaks_to_search > 1");
   const Launchers* l = plan->l;
-  const int wpb = l->wpb;
-  dim3 grid((nwin + wpb - 1) / wpb), block(l->lanes * wpb);
+  dim3 grid((nwin + l->cwpb - 1) / l->cwpb), block(l->cthreads);
   l->correlate[dtype](grid, block, static_cast<hipStream_t>(stream), images, windows, nwin,
                       options, maps, *res, plan->pa);
   return check_launch("piv_correlate_batch");
@@ -771,7 +1046,7 @@ int piv_peaks_from_maps(piv_plan* plan, float* maps, const double* norms, int32_
   if (plan->pa.nb_peaks > 1 && (res->sec == nullptr || res->nsec == nullptr))
     return fail(PIV_EINVAL, "secondary peak arrays required when nb_peaks_to_search > 1");
   const Launchers* l = plan->l;
-  dim3 grid((nwin + l->wpb - 1) / l->wpb), block(l->lanes * l->wpb);
+  dim3 grid((nwin + l->pwpb - 1) / l->pwpb), block(l->pthreads);
   l->peaks(grid, block, static_cast<hipStream_t>(stream), maps, norms, nwin, apply_subpix, *res,
            plan->pa);
   return check_launch("piv_peaks_from_maps");

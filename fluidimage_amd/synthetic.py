@@ -178,13 +178,16 @@ def config_pair(config_id, pair_idx=0, shape=None):
     raise ValueError(f"unknown config {config_id}")
 
 
-def uneven_batch(npairs=32, seed_offset=0):
+UNEVEN_SIZES = [(1024, 1024), (1200, 1600), (2048, 2048), (2160, 2560)]
+
+
+def uneven_batch(npairs=32, seed_offset=0, sizes=UNEVEN_SIZES):
     """Config-4 style batch: random sizes, schedules and Oseen + shear fields.
 
-    Returns a list of (im0, im1, windows_schedule) with uint16 images.
+    Returns a list of (im0, im1, windows_schedule) with uint16 images
+    (``sizes`` overrides the image sizes, for smaller test cases).
     """
     rng = np.random.default_rng(SEED_BASE + 4000 + seed_offset)
-    sizes = [(1024, 1024), (1200, 1600), (2048, 2048), (2160, 2560)]
     schedules = [(128, 64, 32), (64, 32, 16), (48, 24)]
     out = []
     for _ in range(npairs):

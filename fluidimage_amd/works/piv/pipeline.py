@@ -368,3 +368,30 @@ def run_pairs(work_piv, pairs, keep_maps=False, stream=None):
     pipe = MultipassPipeline(work_piv)
     passes = pipe.run(imageset, len(pairs), keep_maps=keep_maps, stream=stream)
     return pipe, passes
+
+
+def run_uneven(items, params):
+    """Pairs of different shapes and window schedules (a list of
+    (im0, im1, schedule)): pairs sharing a shape and a schedule run as one
+    batch (one correlation launch per pass, one TPS launch per kernel).
+    ``params`` is a WorkPIV params tree whose piv0.shape_crop_im0 and
+    multipass.number / coeff_zoom are replaced by each schedule's.
+    Returns the MultipassPIVResults in the order of ``items``."""
+    from copy import deepcopy
+
+    from .multipass import WorkPIV
+
+    groups = {}
+    for i, (im0, im1, sched) in enumerate(items):
+        groups.setdefault((tuple(im0.shape), tuple(sched)), []).append(i)
+    out = [None] * len(items)
+    for (shape, sched), idx in groups.items():
+        p = deepcopy(params)
+        p.piv0.shape_crop_im0 = int(sched[0])
+        p.multipass.number = len(sched)
+        p.multipass.coeff_zoom = [sched[k] // sched[k + 1] for k in range(len(sched) - 1)]
+        work = WorkPIV(p)
+        res = work.calcul_batch([(items[i][0], items[i][1]) for i in idx])
+        for i, r in zip(idx, res):
+            out[i] = r
+    return out

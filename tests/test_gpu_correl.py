@@ -13,7 +13,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(12, 12), (16, 16), (20, 20), (24, 24), (32, 32), (40, 40), (48, 48), (64, 64),
-          (64, 32), (32, 64)]
+          (64, 32), (32, 64), (128, 128)]
 
 
 def _correl(shape, **kw):
@@ -39,7 +39,7 @@ def test_call_map_and_norm_match_oracle(shape):
         assert norm == pytest.approx(float(n64), rel=1e-6)
 
 
-@pytest.mark.parametrize("shape", [(32, 32), (64, 32), (48, 48), (24, 24)])
+@pytest.mark.parametrize("shape", [(32, 32), (64, 32), (48, 48), (24, 24), (128, 128)])
 def test_known_answers(shape):
     # src/fluidimage/calcul/test_correl.py:75-109: zero shift within 1e-3,
     # autocorrelation max 1, shift (3.3, 5.8) within 0.8 px; the sub-pixel
@@ -87,11 +87,11 @@ def _planted_maps(rng, shape, n):
 
 @pytest.mark.parametrize("dmax", [None, 5, "30%"])
 @pytest.mark.parametrize("nb_peaks", [1, 3])
-def test_peaks_from_maps_match_oracle(dmax, nb_peaks):
+@pytest.mark.parametrize("shape", [(24, 24), (128, 128)])
+def test_peaks_from_maps_match_oracle(dmax, nb_peaks, shape):
     from oracle import correl as oc
 
-    print("case: peaks_from_maps", dmax, nb_peaks, flush=True)
-    shape = (24, 24)
+    print("case: peaks_from_maps", dmax, nb_peaks, shape, flush=True)
     rng = np.random.default_rng(13)
     c = _correl(shape, displacement_max=dmax, nb_peaks_to_search=nb_peaks, particle_radius=2)
     o = oc.OracleCorrelFFT(shape, displacement_max=dmax, nb_peaks_to_search=nb_peaks,

@@ -83,6 +83,8 @@ def compare_pass(gpu, exp, correl, label):
         dict(config=2, shape=(512, 512), window=64, dmax="30%"),
         dict(config=2, shape=(300, 300), window=16, dmax=None),
         dict(config=5, shape=(256, 256), window=32, dmax=None),
+        dict(config=2, shape=(704, 640), window=128, dmax=None),
+        dict(config=2, shape=(640, 640), window=128, dmax="30%"),
     ],
 )
 def test_single_pass_vs_oracle(cfg):
