@@ -117,7 +117,7 @@ SIGNATURES = {
     "piv_subpix_at": (ctypes.c_int, [_P, _P, _P, _P, _I, _P, _P, _P, _P]),
     "piv_make_windows": (
         ctypes.c_int,
-        [_I, _I, _P, _P, _P, _P, _I, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P],
+        [_I, _I, _I, _P, _P, _P, _P, _I, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P],
     ),
     "piv_tps_smooth": (
         ctypes.c_int,
@@ -145,7 +145,7 @@ SIGNATURES = {
         [_I, _P, _I, ctypes.POINTER(PivDeformGrid), _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "piv_fix": (
         ctypes.c_int,
-        [_P, _P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P,
+        [_I, _P, _P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P,
          _P, _P],
     ),
 }

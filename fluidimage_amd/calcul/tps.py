@@ -63,6 +63,27 @@ def _p(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def _upload(arrays, device):
+    """Host numpy arrays to device tensors in one pinned, asynchronous copy
+    (8-byte aligned slices of one byte buffer)."""
+    import torch
+
+    offs, total = [], 0
+    for a in arrays:
+        offs.append(total)
+        total += (a.nbytes + 7) // 8 * 8
+    host = torch.empty(max(total, 8), dtype=torch.uint8, pin_memory=True)
+    hv = host.numpy()
+    for a, o in zip(arrays, offs):
+        hv[o:o + a.nbytes] = np.ascontiguousarray(a).view(np.uint8).ravel()
+    dev = host.to(device, non_blocking=True)
+    out = []
+    for a, o in zip(arrays, offs):
+        t = dev[o:o + a.nbytes].view(getattr(torch, str(a.dtype)))
+        out.append(t.view(a.shape))
+    return out
+
+
 def tps_device(dx, dy, xs, ys, new_x, new_y, subdom_size, smoothing_coef, threshold,
                percent_buffer_area=20, stream=None, keep=False):
     """TPS smoothing of the finite vectors of (dx, dy) at (xs, ys) and
@@ -179,12 +200,11 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
     nd = nd_base
     lim_h = np.concatenate([f.lim_h for f in active])
     lim_off = np.concatenate([[0], np.cumsum([f.lim_h.size for f in active])])
-    lim = torch.from_numpy(lim_h).to(dev)
     table = np.array([[f.nsx, f.nsy, f.count, f.base, f.nd_base, lim_off[j]]
                       for j, f in enumerate(active)], dtype=np.int32)
-    ftab = torch.from_numpy(table).to(dev)
-    dom_field = torch.from_numpy(
-        np.concatenate([np.full(f.nd, j, np.int32) for j, f in enumerate(active)])).to(dev)
+    dom_field_h = np.concatenate([np.full(f.nd, j, np.int32) for j, f in enumerate(active)])
+    smoothing_h = np.concatenate([np.full(f.nd, f.smoothing) for f in active])
+    lim, ftab, dom_field, smoothing = _upload([lim_h, table, dom_field_h, smoothing_h], dev)
     counts = torch.empty(nd, **i32)
     _native.check(lib.piv_tps_count(nd, _p(ftab), _p(dom_field), _p(lim), _p(cx), _p(cy),
                                     _p(counts), sp), "piv_tps_count")
@@ -206,10 +226,7 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
     dom_off_h = np.concatenate([[0], np.cumsum(sizes)])
     mat_off_h = np.concatenate([[0], np.cumsum(sizes**2)])
     nslots = int(dom_off_h[-1])
-    dom_off = torch.from_numpy(dom_off_h).to(dev)
-    mat_off = torch.from_numpy(mat_off_h).to(dev)
-    smoothing = torch.from_numpy(
-        np.concatenate([np.full(f.nd, f.smoothing) for f in active])).to(dev)
+    dom_off, mat_off = _upload([dom_off_h.astype(np.int64), mat_off_h.astype(np.int64)], dev)
     slot_pt = torch.empty(nslots, **i32)
     status = torch.zeros(nd + 1, **i32)
     ntot = int(cx.numel())

@@ -747,10 +747,12 @@ __global__ void k_min_final(uint32_t* keys, float* out, int n) {
 // -------------------------------------------------------------------------
 // WorkFIX (fix.py:66-133) and mean_neighbors (mean_neighbors.py:5-155)
 // -------------------------------------------------------------------------
+// pair blockIdx.y of a batch: its n vectors at offset blockIdx.y * n of every array
 __global__ void k_fix_mark(float* dx, float* dy, const float* cmax, const int32_t* err, int n,
                            float correl_min, float dmax2, int use_dmax, int32_t* flags) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  i += blockIdx.y * n;
   int f = 0;
   float x = dx[i], y = dy[i];
   if (err[i] != PIV_WIN_OK) f |= 1;
@@ -824,6 +826,15 @@ __global__ void k_fix_neigh(const float* dx, const float* dy, int ny, int nx, fl
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ny * nx) return;
   const int i0 = i / nx, i1 = i % nx;
+  {
+    const int64_t o = int64_t(blockIdx.y) * ny * nx;
+    dx += o;
+    dy += o;
+    flags += o;
+    if (diff2) diff2 += o;
+    if (mean_x) mean_x += o;
+    if (mean_y) mean_y += o;
+  }
   const float x = dx[i], y = dy[i];
   float mx = __builtin_nanf(""), my = __builtin_nanf("");
   if (!(x != x || y != y) && ny > 1 && nx > 1) {
@@ -846,6 +857,7 @@ __global__ void k_fix_neigh(const float* dx, const float* dy, int ny, int nx, fl
 __global__ void k_fix_apply(float* dx, float* dy, const int32_t* flags, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  i += blockIdx.y * n;
   if (flags[i] & 8) {
     dx[i] = __builtin_nanf("");
     dy[i] = __builtin_nanf("");
@@ -867,9 +879,14 @@ __global__ void k_make_windows(int ny, int nx, const int32_t* __restrict__ iyvec
                                int height, int width, int npad, CropGeom g, int image0,
                                int image1, piv_window* __restrict__ out, double* xs, double* ys,
                                int32_t* dxin_out, int32_t* dyin_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ny * nx) return;
   const int gx = ixvecs[i % nx], gy = iyvecs[i / nx];
+  // pair blockIdx.y of a batch: images image0 + 2 k, image1 + 2 k, its
+  // vectors at offset k ny nx of the predictor and output arrays
+  image0 += 2 * blockIdx.y;
+  image1 += 2 * blockIdx.y;
+  i += blockIdx.y * ny * nx;
   int dxi = 0, dyi = 0;
   int ixs0 = gx, iys0 = gy, ixs1 = gx, iys1 = gy;
   if (dxa != nullptr) {
@@ -1062,14 +1079,15 @@ int piv_subpix_at(piv_plan* plan, const float* maps, const int32_t* ix, const in
   return check_launch("piv_subpix_at");
 }
 
-int piv_fix(float* dx, float* dy, const float* correl_max, const int32_t* err, int32_t ny,
-            int32_t nx, float correl_min, float displacement_max, float threshold,
+int piv_fix(int32_t npairs, float* dx, float* dy, const float* correl_max, const int32_t* err,
+            int32_t ny, int32_t nx, float correl_min, float displacement_max, float threshold,
             int32_t* flags, float* diff2, float* mean_x, float* mean_y, void* stream) {
   const int n = ny * nx;
-  if (n <= 0) return PIV_OK;
+  if (n <= 0 || npairs <= 0) return PIV_OK;
+  if (npairs > 65535) return fail(PIV_EINVAL, "too many pairs for one launch");
   if (!dx || !dy || !correl_max || !err || !flags) return fail(PIV_EINVAL, "NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int nb = (n + 255) / 256;
+  const dim3 nb((n + 255) / 256, npairs);
   const float dmax2 = float(double(displacement_max) * displacement_max);
   hipLaunchKernelGGL(k_fix_mark, dim3(nb), dim3(256), 0, s, dx, dy, correl_max, err, n,
                      correl_min, dmax2, displacement_max > 0.f, flags);
@@ -1081,18 +1099,19 @@ int piv_fix(float* dx, float* dy, const float* correl_max, const int32_t* err, i
   return check_launch("piv_fix");
 }
 
-int piv_make_windows(int32_t ny, int32_t nx, const int32_t* iyvecs, const int32_t* ixvecs,
-                     const double* dx_approx, const double* dy_approx, int32_t height,
-                     int32_t width, int32_t npad, const int32_t* crop, int32_t image0,
-                     int32_t image1, piv_window* windows, double* xs, double* ys,
+int piv_make_windows(int32_t npairs, int32_t ny, int32_t nx, const int32_t* iyvecs,
+                     const int32_t* ixvecs, const double* dx_approx, const double* dy_approx,
+                     int32_t height, int32_t width, int32_t npad, const int32_t* crop,
+                     int32_t image0, int32_t image1, piv_window* windows, double* xs, double* ys,
                      int32_t* dx_in, int32_t* dy_in, void* stream) {
   const int n = ny * nx;
-  if (n <= 0) return PIV_OK;
+  if (n <= 0 || npairs <= 0) return PIV_OK;
+  if (npairs > 65535) return fail(PIV_EINVAL, "too many pairs for one launch");
   if (!iyvecs || !ixvecs || !crop || !windows) return fail(PIV_EINVAL, "NULL argument");
   if ((dx_approx == nullptr) != (dy_approx == nullptr))
     return fail(PIV_EINVAL, "dx_approx and dy_approx must both be given or both NULL");
   CropGeom g{crop[0], crop[1], crop[2], crop[3], crop[4], crop[5], crop[6], crop[7]};
-  hipLaunchKernelGGL(k_make_windows, dim3((n + 255) / 256), dim3(256), 0,
+  hipLaunchKernelGGL(k_make_windows, dim3((n + 255) / 256, npairs), dim3(256), 0,
                      static_cast<hipStream_t>(stream), ny, nx, iyvecs, ixvecs, dx_approx,
                      dy_approx, height, width, npad, g, image0, image1, windows, xs, ys, dx_in,
                      dy_in);

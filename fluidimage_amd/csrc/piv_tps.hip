@@ -305,6 +305,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   __shared__ double rdiag[kNB];              // 1 / L_ii
   __shared__ double tbuf[16][17];            // L21 X11
   __shared__ int bad;
+  __shared__ int next_pair;                  // look-ahead work counter
   if (tid == 0) bad = 0;
   // B'[3:, :3] (kept for the solves)
   for (int e = tid; e < (n - 3) * 3; e += kCholThreads) {
@@ -315,18 +316,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   double* A = B + 3 * int64_t(n) + 3;  // C(i, j) = A[i n + j]
   const double* VC = VW + 18;          // V, Z rows of C's index space
   const int nblk = (m + kNB - 1) / kNB;
-  for (int J = 0; J < nblk; ++J) {
-    const int r0 = J * kNB;
-    const int bs = min(kNB, m - r0);
-    const int nstrip = (m - r0 + 15) / 16;
-    // a) block-column update, two 16-row strips per wavefront at a time
-    //    (the block-row operand is shared by both strips)
-    for (int sp = wave; 2 * sp < nstrip; sp += kCholWaves) {
-      const int row0 = r0 + 32 * sp;
+  // C(r0t:m, Jt) -= [rank-6 term] + L(r0t:m, kb:ke) L(Jt, kb:ke)^T for the
+  // strip pair sp (rows r0t + 32 sp .. + 31) on the FP64 matrix cores; with
+  // keep_diag, pair 0 also leaves the updated diagonal block in lbuf
+  auto update_pair = [&](int Jt, int sp, int kb, int ke, bool with_wz, bool keep_diag) {
+    const int r0t = Jt * kNB;
+    const int bst = min(kNB, m - r0t);
+    {
+      const int row0 = r0t + 32 * sp;
       const int ra0 = min(row0 + li, m - 1);       // A operand rows (clamped)
       const int ra1 = min(row0 + 16 + li, m - 1);
-      const int rb0 = min(r0 + li, m - 1);         // B operand rows: block J
-      const int rb1 = min(r0 + 16 + li, m - 1);
+      const int rb0 = min(r0t + li, m - 1);        // B operand rows: block Jt
+      const int rb1 = min(r0t + 16 + li, m - 1);
       const double* pa0 = A + int64_t(ra0) * n + 4 * lk;
       const double* pa1 = A + int64_t(ra1) * n + 4 * lk;
       const double* pb0 = A + int64_t(rb0) * n + 4 * lk;
@@ -336,6 +337,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       // a_i = [V_i, Z_i], b_j = [Z_j, V_j] (k < 6, zero beyond)
 #pragma unroll
       for (int s2 = 0; s2 < 2; ++s2) {
+        if (!with_wz) break;
         const int k = 4 * s2 + lk;
         const bool kin = k < 6;
         const int ka = kin ? k : 0, kb = kin ? (k + 3) % 6 : 0;
@@ -360,10 +362,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
           o[12 + q] = pb1[k0 + q];
         }
       };
-      if (r0 > 0) fetch(op[0], 0);
-      for (int k0 = 0; k0 < r0; k0 += 32) {
+      if (kb < ke) fetch(op[0], kb);
+      for (int k0 = kb; k0 < ke; k0 += 32) {
         // two steps per iteration, ping-pong buffers
-        if (k0 + 16 < r0) fetch(op[1], k0 + 16);
+        if (k0 + 16 < ke) fetch(op[1], k0 + 16);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           c00 = mfma_f64(op[0][q], op[0][8 + q], c00);
@@ -371,8 +373,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
           c10 = mfma_f64(op[0][4 + q], op[0][8 + q], c10);
           c11 = mfma_f64(op[0][4 + q], op[0][12 + q], c11);
         }
-        if (k0 + 16 >= r0) break;
-        if (k0 + 32 < r0) fetch(op[0], k0 + 32);
+        if (k0 + 16 >= ke) break;
+        if (k0 + 32 < ke) fetch(op[0], k0 + 32);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           c00 = mfma_f64(op[1][q], op[1][8 + q], c00);
@@ -388,24 +390,36 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         for (int v = 0; v < 4; ++v) {
           const int row = row0 + 16 * h + 4 * v + lk;
           if (row < m) {
-            double* dst = A + int64_t(row) * n + r0;
+            double* dst = A + int64_t(row) * n + r0t;
             const int col0 = li, col1 = 16 + li;
             const double u0 = h ? c10[v] : c00[v];
             const double u1 = h ? c11[v] : c01[v];
-            if (col0 < bs) {
+            if (col0 < bst) {
               const double r = dst[col0] - u0;
               dst[col0] = r;
-              if (sp == 0) lbuf[16 * h + 4 * v + lk][col0] = r;  // diagonal block
+              if (keep_diag && sp == 0) lbuf[16 * h + 4 * v + lk][col0] = r;  // diagonal block
             }
-            if (col1 < bs) {
+            if (col1 < bst) {
               const double r = dst[col1] - u1;
               dst[col1] = r;
-              if (sp == 0) lbuf[16 * h + 4 * v + lk][col1] = r;
+              if (keep_diag && sp == 0) lbuf[16 * h + 4 * v + lk][col1] = r;
             }
           }
         }
       }
     }
+  };
+  auto npairs_of = [&](int Jt) { return ((m - Jt * kNB + 15) / 16 + 1) / 2; };
+  for (int J = 0; J < nblk; ++J) {
+    const int r0 = J * kNB;
+    const int bs = min(kNB, m - r0);
+    const int nstrip = (m - r0 + 15) / 16;
+    if (tid == 0) next_pair = 0;
+    // a) finish block column J: its rank-32 update with block J - 1 (the
+    //    earlier blocks were applied while the previous diagonal block was
+    //    factorised), or for J = 0 the rank-6 term alone
+    for (int sp = wave; sp < npairs_of(J); sp += kCholWaves)
+      update_pair(J, sp, J > 0 ? r0 - kNB : 0, r0, J == 0, true);
     __syncthreads();
     // b) diagonal block by wavefront 0 (C_JJ is in lbuf, written there by
     //    this wavefront in a): right-looking factorisation in registers
@@ -482,6 +496,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         // into the matrix, row `lane`
         double* blk = A + int64_t(r0 + lane) * n + r0;
         for (int c = 0; c < bs; ++c) blk[c] = c <= lane ? lbuf[lane][c] : inv[c][lane];
+      }
+    }
+    // look-ahead: blocks 0..J-1 (and the rank-6 term) applied to block column
+    // J + 1 by the other wavefronts, pairs of strips taken from a counter;
+    // wavefront 0 joins after the diagonal block
+    if (J + 1 < nblk) {
+      const int np = npairs_of(J + 1);
+      for (;;) {
+        int sp = 0;
+        if (lane == 0) sp = atomicAdd(&next_pair, 1);
+        sp = __builtin_amdgcn_readfirstlane(sp);
+        if (sp >= np) break;
+        update_pair(J + 1, sp, 0, r0, true, false);
       }
     }
     __syncthreads();

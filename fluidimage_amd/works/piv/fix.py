@@ -16,13 +16,14 @@ from ... import _native
 from ...data_objects import error_strings
 
 
-def run_fix_device(out, ny, nx, params_fix, stream=None):
+def run_fix_device(out, ny, nx, params_fix, stream=None, npairs=1):
     """Apply the NaN marking and neighbour test in place on device results
-    ``out`` (dx, dy, correl_max, err). Adds flags, diff2, mean_x, mean_y."""
+    ``out`` (dx, dy, correl_max, err; npairs pair-major grids of ny x nx).
+    Adds flags, diff2, mean_x, mean_y."""
     import torch
 
     dev = out["dx"].device
-    n = ny * nx
+    n = ny * nx * npairs
     flags = torch.empty(n, dtype=torch.int32, device=dev)
     diff2 = torch.empty(n, dtype=torch.float32, device=dev)
     mean_x = torch.empty(n, dtype=torch.float32, device=dev)
@@ -32,7 +33,8 @@ def run_fix_device(out, ny, nx, params_fix, stream=None):
     lib = _native.load()
     _native.check(
         lib.piv_fix(
-            ctypes.c_void_p(out["dx"].data_ptr()), ctypes.c_void_p(out["dy"].data_ptr()),
+            int(npairs), ctypes.c_void_p(out["dx"].data_ptr()),
+            ctypes.c_void_p(out["dy"].data_ptr()),
             ctypes.c_void_p(out["correl_max"].data_ptr()), ctypes.c_void_p(out["err"].data_ptr()),
             ny, nx, float(params_fix.correl_min), float(dmax) if dmax else 0.0,
             float(thr) if thr is not None else -1.0, ctypes.c_void_p(flags.data_ptr()),

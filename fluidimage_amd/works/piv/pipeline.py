@@ -1,8 +1,7 @@
 """Device-resident multipass PIV over a batch of image pairs.
 
-One pass = one launch of ``piv_make_windows`` per pair, ONE launch of the
-fused correlation kernel over the windows of every pair, ``piv_fix`` per
-pair, and the interpolation of the fixed field on the next grid
+One pass = one launch of ``piv_make_windows``, ONE launch of the fused
+correlation kernel over the windows of every pair, one ``piv_fix``, and the interpolation of the fixed field on the next grid
 (GPU FP64 TPS, or scipy's griddata exactly as the reference when
 ``use_tps`` selects it). The semantics are those of
 src/fluidimage/works/piv/multipass.py:172-190 applied to each pair; results
@@ -145,24 +144,18 @@ class MultipassPipeline:
                 if ev:
                     ev[1].record()
                 table, dtype = deformed_images.table, deformed_images.dtype
-                windows = torch.empty((npairs * nvec, 10), dtype=torch.int32, device=dev)
-                for k in range(npairs):
-                    work.make_windows(2 * k, 2 * k + 1, None, None,
-                                      windows=windows[k * nvec:(k + 1) * nvec], device=dev,
-                                      stream=stream)
+                windows = work.make_windows(0, 1, None, None, device=dev, stream=stream,
+                                            npairs=npairs)["windows"]
                 gx, gy = work._device_grid_full(dev)
                 win = {"xs": gx.repeat(npairs), "ys": gy.repeat(npairs), "dx_in": u, "dy_in": v}
             else:
-                windows = torch.empty((npairs * nvec, 10), dtype=torch.int32, device=dev)
-                win = {k: [] for k in ("xs", "ys", "dx_in", "dy_in")}
-                for k in range(npairs):
-                    dxa, dya = (None, None) if approx[k] is None else approx[k]
-                    w = work.make_windows(2 * k, 2 * k + 1, dxa, dya,
-                                          windows=windows[k * nvec:(k + 1) * nvec], device=dev,
-                                          stream=stream)
-                    for key in win:
-                        win[key].append(w[key])
-                win = {key: torch.cat(v) for key, v in win.items()}
+                dxa = dya = None
+                if approx[0] is not None:
+                    dxa = torch.cat([approx[k][0] for k in range(npairs)])
+                    dya = torch.cat([approx[k][1] for k in range(npairs)])
+                w = work.make_windows(0, 1, dxa, dya, device=dev, stream=stream, npairs=npairs)
+                windows = w["windows"]
+                win = {key: w[key] for key in ("xs", "ys", "dx_in", "dy_in")}
             maps = None
             if keep_maps:
                 maps = torch.empty((npairs * nvec,) + work.shape_crop_im0, dtype=torch.float32,
@@ -181,13 +174,7 @@ class MultipassPipeline:
             out["windows"] = windows
             res = PassOutput(work, npairs, out, win, dev)
             ny, nx = len(work.iyvecs), len(work.ixvecs)
-            for k in range(npairs):
-                sub = {key: res.pair(key, k) for key in ("dx", "dy", "correl_max", "err")}
-                fx = run_fix_device(sub, ny, nx, self.params.fix, stream=stream)
-                for key in ("flags", "diff2", "mean_x", "mean_y"):
-                    out.setdefault(key, []).append(fx[key])
-            for key in ("flags", "diff2", "mean_x", "mean_y"):
-                out[key] = torch.cat(out[key])
+            run_fix_device(out, ny, nx, self.params.fix, stream=stream, npairs=npairs)
             if self.params.piv0.nb_peaks_to_search >= 2:
                 self._secondary_peaks(res)
             passes.append(res)

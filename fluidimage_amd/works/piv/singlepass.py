@@ -155,15 +155,16 @@ class BaseWorkPIV(BaseWorkWithMask):
 
     # ---------------------------------------------------------------- GPU
     def make_windows(self, image0, image1, dx_approx=None, dy_approx=None, windows=None,
-                     device=None, stream=None):
+                     device=None, stream=None, npairs=1):
         """Window records of this pass for the image pair (image0, image1) of
-        an ImageSet; ``dx_approx``/``dy_approx`` (float64 device tensors on
-        this pass's grid) give the predictor of later passes."""
+        an ImageSet (and the next npairs - 1 pairs, images image0 + 2k,
+        image1 + 2k); ``dx_approx``/``dy_approx`` (float64 device tensors on
+        this pass's grid, pair-major) give the predictor of later passes."""
         torch = _torch()
         if device is None:
             device = torch.device("cuda")
         iyv, ixv = self._device_grid(device)
-        nvec = self.nb_vectors
+        nvec = self.nb_vectors * npairs
         if windows is None:
             windows = torch.empty((nvec, 10), dtype=torch.int32, device=device)
         xs = torch.empty(nvec, dtype=torch.float64, device=device)
@@ -175,7 +176,7 @@ class BaseWorkPIV(BaseWorkWithMask):
         lib = _native.load()
         _native.check(
             lib.piv_make_windows(
-                len(self.iyvecs), len(self.ixvecs), ctypes.c_void_p(iyv.data_ptr()),
+                int(npairs), len(self.iyvecs), len(self.ixvecs), ctypes.c_void_p(iyv.data_ptr()),
                 ctypes.c_void_p(ixv.data_ptr()),
                 None if dx_approx is None else ctypes.c_void_p(dx_approx.data_ptr()),
                 None if dy_approx is None else ctypes.c_void_p(dy_approx.data_ptr()),

@@ -162,8 +162,11 @@ int piv_subpix_at(piv_plan* plan, const float* maps, const int32_t* ix,
  * 8 ints): _start_for_crop0 (y, x), _stop_for_crop0 (y, x),
  * _start_for_crop1 (y, x), _stop_for_crop1 (y, x). Windows whose crop leaves
  * the padded image get PIV_WINFLAG_INVALID. xs/ys (float64 vector
- * positions) and dx_in/dy_in (int32 rounded predictor) may be NULL. */
-int piv_make_windows(int32_t ny, int32_t nx, const int32_t* iyvecs,
+ * positions) and dx_in/dy_in (int32 rounded predictor) may be NULL.
+ * npairs pairs of a batch in one call: pair k reads images image0 + 2k,
+ * image1 + 2k and its ny nx vectors at offset k ny nx of the predictor and
+ * output arrays. */
+int piv_make_windows(int32_t npairs, int32_t ny, int32_t nx, const int32_t* iyvecs,
                      const int32_t* ixvecs, const double* dx_approx,
                      const double* dy_approx, int32_t height, int32_t width,
                      int32_t npad, const int32_t* crop, int32_t image0,
@@ -175,8 +178,10 @@ int piv_make_windows(int32_t ny, int32_t nx, const int32_t* iyvecs,
  * then the 8-neighbour mean test against threshold (if >= 0). flags receives
  * bit 0: piv error, 1: correl < correl_min, 2: delta2 < displacement_max2,
  * 3: diff neighbour too large; diff2 the squared difference to the
- * neighbour mean; mean_x / mean_y the neighbour means (may be NULL). */
-int piv_fix(float* dx, float* dy, const float* correl_max, const int32_t* err,
+ * neighbour mean; mean_x / mean_y the neighbour means (may be NULL).
+ * npairs grids of a batch in one call, pair k at offset k ny nx of every
+ * array. */
+int piv_fix(int32_t npairs, float* dx, float* dy, const float* correl_max, const int32_t* err,
             int32_t ny, int32_t nx, float correl_min, float displacement_max,
             float threshold, int32_t* flags, float* diff2, float* mean_x,
             float* mean_y, void* stream);
