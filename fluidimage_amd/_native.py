@@ -132,7 +132,7 @@ SIGNATURES = {
     "piv_tps_blend": (ctypes.c_int, [_I, _P, _P, _P, _P, ctypes.c_int64, _P, _P]),
     "piv_tps_interpolate": (
         ctypes.c_int,
-        [_I, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P],
+        [_I, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P, _P],
     ),
     "piv_deform_images": (
         ctypes.c_int, [_I, _P, _I, _I, _I, ctypes.POINTER(PivDeformGrid), _P, _P, _P, _P]),

@@ -171,7 +171,8 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
             continue
         f.nd = f.nsx * f.nsy
         active.append(f)
-    results = _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
+    own = (xs, ys, idx, n)
+    results = _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep, own)
     out = []
     for f in flist:
         if f.error is not None:
@@ -183,7 +184,38 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
     return out
 
 
-def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep):
+def _at_centers(active, own, smoothed, new_x, new_y, dev):
+    """Positions of the evaluation that are centers: where the new position
+    q of field j is the position of its vector q and that vector is valid,
+    the evaluation equals the blended smoothed value of that vector
+    ([K P][w; a] = v - s w at the centers). Returns (approx prefilled there,
+    uint8 mask of those positions), or (None, None) when the new positions
+    are not the fields' own grid."""
+    import torch
+
+    xs, ys, idx, n = own
+    m = int(new_x.numel())
+    if m != n:
+        return None, None
+    na = len(active)
+    ks, counts = _upload([np.array([f.k for f in active], np.int64),
+                          np.array([f.count for f in active], np.int64)], dev)
+    r = torch.arange(na * n, device=dev)
+    j = torch.div(r, n, rounding_mode="floor")
+    o = r - j * n
+    src = ks[j] * n + o  # stacked valid-list position (field base k n + rank)
+    valid = o < counts[j]
+    tgt = torch.where(valid, j * m + idx[src].long(), na * m)
+    pos = torch.full((na * m + 1,), -1, dtype=torch.int64, device=dev)
+    pos.index_put_((tgt,), torch.where(valid, src, -1))
+    pos = pos[:na * m]
+    sel = ks[j] * n + o  # the field's own position of vector o
+    done = (pos >= 0) & (new_x.repeat(na) == xs[sel]) & (new_y.repeat(na) == ys[sel])
+    approx = torch.where(done, smoothed[:, pos.clamp(min=0)], 0.0)
+    return approx.contiguous(), done.to(torch.uint8)
+
+
+def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep, own):
     """Membership, FP64 solves, blend and evaluation for the fields whose
     grid is valid; fields with an empty subdomain get their error and the
     others are solved again without them."""
@@ -221,7 +253,7 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
                 "use a smaller params.multipass.subdom_size")
     if retry:
         return _solve_active(lib, [f for f in active if f.error is None], cx, cy, vx, vy,
-                             new_x, new_y, thr, dev, sp, keep)
+                             new_x, new_y, thr, dev, sp, keep, own)
     n_max = int(sizes.max())
     dom_off_h = np.concatenate([[0], np.cumsum(sizes)])
     mat_off_h = np.concatenate([[0], np.cumsum(sizes**2)])
@@ -273,10 +305,12 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep)
         sm.copy_(blend)
     px = new_x.repeat(na)
     py = new_y.repeat(na)
-    approx = torch.empty((2, na * m), **f64)
+    approx, skip = _at_centers(active, own, smoothed, new_x, new_y, dev)
+    if approx is None:
+        approx = torch.empty((2, na * m), **f64)
     _native.check(lib.piv_tps_interpolate(na * m, _p(px), _p(py), _p(n_dom), _p(n_coef),
                                           _p(n_norm), _p(slot_xy), nslots, _p(dom_off),
-                                          _p(weights), wstride, _p(approx), sp),
+                                          _p(weights), wstride, _p(skip), _p(approx), sp),
                   "piv_tps_interpolate")
     st = status.cpu().numpy()
     if st[nd]:

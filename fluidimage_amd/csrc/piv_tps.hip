@@ -27,6 +27,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kNB = 32;       // Cholesky / triangular-solve block size
 constexpr int kMaxN = 2048;   // centers per subdomain
+constexpr size_t kPrepFusedLds = 150 * 1024;  // dynamic LDS bound of the fused build
 
 __device__ __forceinline__ double tps_phi(double r2) {
   return r2 != 0.0 ? r2 * log(r2) / 2.0 : 0.0;
@@ -66,7 +67,9 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
     const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
     double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
-    double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm) {
+    double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm,
+    bool fused) {
+  constexpr int kWaves = kThreads / 64;
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
@@ -74,12 +77,14 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
   double* RF = refl + 16 * d;
   double* VW = vwork + 6 * s0;
   // dynamic LDS, nm >= n doubles per array: cy, cx, P (3 columns, then
-  // their reflected copy), V (3 Householder vectors)
+  // their reflected copy), V (3 Householder vectors) and, when fused, the
+  // per-wavefront sums of Y = B V (3 kWaves arrays)
   extern __shared__ double dyn[];
   double* cy = dyn;
   double* cx = dyn + nm;
 #define PM(k, j) dyn[(2 + (k)) * nm + (j)]
 #define VM(k, j) dyn[(5 + (k)) * nm + (j)]
+#define YC(w, c, j) dyn[(8 + 3 * (w) + (c)) * nm + (j)]
   __shared__ double red[kThreads / 64];
   __shared__ double sh[8];
   const int tid = threadIdx.x;
@@ -168,9 +173,48 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
   // ---- lower triangle of B = K + s I: row i by one wavefront, its lanes
   //      over the columns j <= i (K is symmetric; only the lower triangle
   //      is read afterwards) ----
-  {
-    const int lane = tid & 63, wave = tid >> 6;
-    for (int i = wave; i < n; i += kThreads / 64) {
+  double m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int lane = tid & 63, wave = tid >> 6;
+  if (fused) {
+    // Y = B V accumulated while B is built (B is not read back): row i adds
+    // sum_{j <= i} B_ij V_j to Y_i (wavefront reduction) and B_ij V_i to
+    // Y_j for j < i; each wavefront sums into its own copy YC(w, ., .), in
+    // row order, so the result does not depend on the wave schedule
+    for (int e = tid; e < 3 * kWaves * nm; e += kThreads) YC(0, 0, e) = 0.0;
+    __syncthreads();
+    for (int i = wave; i < n; i += kWaves) {
+      const double yi = cy[i], xi = cx[i];
+      const double vi0 = VM(0, i), vi1 = VM(1, i), vi2 = VM(2, i);
+      double* Bi = B + int64_t(i) * n;
+      double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+      for (int j = lane; j <= i; j += 64) {
+        const double dy = cy[j] - yi, dx = cx[j] - xi;
+        double b = tps_phi(dy * dy + dx * dx);
+        if (i == j) b += smoothing;
+        Bi[j] = b;
+        r0 += b * VM(0, j);
+        r1 += b * VM(1, j);
+        r2 += b * VM(2, j);
+        if (j < i) {
+          YC(wave, 0, j) += b * vi0;
+          YC(wave, 1, j) += b * vi1;
+          YC(wave, 2, j) += b * vi2;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        r0 += __shfl_xor(r0, off, 64);
+        r1 += __shfl_xor(r1, off, 64);
+        r2 += __shfl_xor(r2, off, 64);
+      }
+      if (lane == 0) {
+        YC(wave, 0, i) += r0;
+        YC(wave, 1, i) += r1;
+        YC(wave, 2, i) += r2;
+      }
+    }
+    __syncthreads();
+  } else {
+    for (int i = wave; i < n; i += kWaves) {
       const double yi = cy[i], xi = cx[i];
       double* Bi = B + int64_t(i) * n;
       for (int j = lane; j <= i; j += 64) {
@@ -180,26 +224,35 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
         Bi[j] = b;
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
-  // ---- Y = B V (column j of thread, B read back from the lower triangle:
-  //      B[i][j] for i >= j down the column, B[j][i] along row j for i < j),
-  //      M = V^T Y, Z = Y T - 1/2 V T^T M T ----
-  double m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // ---- Y = B V (fused: the wavefront copies summed; else column j of
+  //      thread, B read back from the lower triangle: B[i][j] for i >= j
+  //      down the column, B[j][i] along row j for i < j), M = V^T Y,
+  //      Z = Y T - 1/2 V T^T M T ----
   for (int j = tid; j < n; j += kThreads) {
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-    const double* Bj = B + int64_t(j) * n;
-    for (int i = 0; i < j; ++i) {
-      const double b = Bj[i];
-      y0 += b * VM(0, i);
-      y1 += b * VM(1, i);
-      y2 += b * VM(2, i);
-    }
-    for (int i = j; i < n; ++i) {
-      const double b = B[int64_t(i) * n + j];
-      y0 += b * VM(0, i);
-      y1 += b * VM(1, i);
-      y2 += b * VM(2, i);
+    if (fused) {
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        y0 += YC(w, 0, j);
+        y1 += YC(w, 1, j);
+        y2 += YC(w, 2, j);
+      }
+    } else {
+      const double* Bj = B + int64_t(j) * n;
+      for (int i = 0; i < j; ++i) {
+        const double b = Bj[i];
+        y0 += b * VM(0, i);
+        y1 += b * VM(1, i);
+        y2 += b * VM(2, i);
+      }
+      for (int i = j; i < n; ++i) {
+        const double b = B[int64_t(i) * n + j];
+        y0 += b * VM(0, i);
+        y1 += b * VM(1, i);
+        y2 += b * VM(2, i);
+      }
     }
     const double v0 = VM(0, j), v1 = VM(1, j), v2 = VM(2, j);
     m9[0] += v0 * y0; m9[1] += v0 * y1; m9[2] += v0 * y2;
@@ -242,6 +295,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
   }
 #undef PM
 #undef VM
+#undef YC
 }
 
 // ---------------------------------------------------------------------------
@@ -578,7 +632,31 @@ __device__ void apply_q(double* x0, double* x1, int n, const double* VW, const d
 // holds inv(L_JJ)^T, the diagonal holds L_ii). Both sweeps are left-looking
 // so that every block step is one coalesced pass over a block row (forward)
 // or a block column (backward) of L, reduced across threads, followed by a
-// 32 x 32 triangular matrix-vector product out of LDS.
+// 32 x 32 triangular matrix-vector product out of LDS. The sweeps are
+// latency-bound: every thread issues its loads of a step in batches
+// (kBatch doubles in flight) before it uses them, and the sums keep the
+// order of one element after the other.
+constexpr int kBatch = 16;
+
+// the diagonal block J of L into LDS (zero outside the bs x bs block)
+__device__ __forceinline__ void load_diag_block(double (*dblk)[kNB + 1], const double* A, int n,
+                                                int r0, int bs) {
+  double v[kNB * kNB / kThreads];
+#pragma unroll
+  for (int t = 0; t < kNB * kNB / kThreads; ++t) {
+    const int e = threadIdx.x + kThreads * t;
+    const int i = e / kNB, c = e % kNB;
+    const int ii = min(i, bs - 1), cc = min(c, bs - 1);
+    v[t] = A[int64_t(r0 + ii) * n + r0 + cc];
+  }
+#pragma unroll
+  for (int t = 0; t < kNB * kNB / kThreads; ++t) {
+    const int e = threadIdx.x + kThreads * t;
+    const int i = e / kNB, c = e % kNB;
+    dblk[i][c] = (i < bs && c < bs) ? v[t] : 0.0;
+  }
+}
+
 __device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m) {
   static_assert(kThreads == 256 && kNB == 32, "thread layout of chol_solve");
   __shared__ double dblk[kNB][kNB + 1];
@@ -592,23 +670,33 @@ __device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m
   for (int J = 0; J < nblk; ++J) {
     const int r0 = J * kNB;
     const int bs = min(kNB, m - r0);
-    for (int e = tid; e < kNB * kNB; e += kThreads) {
-      const int i = e / kNB, c = e % kNB;
-      dblk[i][c] = (i < bs && c < bs) ? A[int64_t(r0 + i) * n + r0 + c] : 0.0;
-    }
+    load_diag_block(dblk, A, n, r0, bs);
     {
-      // thread = (row i, part p): columns 32 q + 4 p .. + 3 of row r0 + i
+      // thread = (row i, part p): columns 32 q + 4 p .. + 3 of row r0 + i,
+      // kBatch / 4 blocks q per batch of loads
       const int i = tid >> 3, p = tid & 7;
       double s0 = 0.0, s1 = 0.0;
       if (i < bs) {
         const double* Lr = A + int64_t(r0 + i) * n + 4 * p;
-        for (int q = 0; q < J; ++q) {
+        constexpr int QB = kBatch / 4;
+        for (int q0 = 0; q0 < J; q0 += QB) {
+          double l[kBatch];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int k = 32 * q + 4 * p + u;
-            const double l = Lr[32 * q + u];
-            s0 += l * x0[k];
-            s1 += l * x1[k];
+          for (int qq = 0; qq < QB; ++qq) {
+            const int q = min(q0 + qq, J - 1);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) l[4 * qq + u] = Lr[32 * q + u];
+          }
+#pragma unroll
+          for (int qq = 0; qq < QB; ++qq) {
+            if (q0 + qq < J) {
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int k = 32 * (q0 + qq) + 4 * p + u;
+                s0 += l[4 * qq + u] * x0[k];
+                s1 += l[4 * qq + u] * x1[k];
+              }
+            }
           }
         }
       }
@@ -628,7 +716,7 @@ __device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m
       if (c < bs) {
         // y_c = tmp_c / L_cc + sum_{k < c} inv(L)[c][k] tmp_k
         double sacc = tmp[comp][c] / dblk[c][c];
-#pragma unroll
+#pragma unroll 4
         for (int k = 0; k < kNB; ++k)
           if (k < c) sacc += dblk[k][c] * tmp[comp][k];
         (comp ? x1 : x0)[r0 + c] = sacc;
@@ -640,19 +728,26 @@ __device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m
   for (int J = nblk - 1; J >= 0; --J) {
     const int r0 = J * kNB;
     const int bs = min(kNB, m - r0);
-    for (int e = tid; e < kNB * kNB; e += kThreads) {
-      const int i = e / kNB, c = e % kNB;
-      dblk[i][c] = (i < bs && c < bs) ? A[int64_t(r0 + i) * n + r0 + c] : 0.0;
-    }
+    load_diag_block(dblk, A, n, r0, bs);
     {
-      // thread = (column c, row group g): rows r0 + bs + g, + 8, ...
+      // thread = (column c, row group g): rows r0 + bs + g, + 8, ...,
+      // kBatch rows per batch of loads
       const int c = tid & 31, g = tid >> 5;
       double s0 = 0.0, s1 = 0.0;
       if (c < bs) {
-        for (int i = r0 + bs + g; i < m; i += 8) {
-          const double l = A[int64_t(i) * n + r0 + c];
-          s0 += l * x0[i];
-          s1 += l * x1[i];
+        const double* Lc = A + r0 + c;
+        for (int i0 = r0 + bs + g; i0 < m; i0 += 8 * kBatch) {
+          double l[kBatch];
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u) l[u] = Lc[int64_t(min(i0 + 8 * u, m - 1)) * n];
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u) {
+            const int i = i0 + 8 * u;
+            if (i < m) {
+              s0 += l[u] * x0[i];
+              s1 += l[u] * x1[i];
+            }
+          }
         }
       }
       part[0][g][c] = s0;
@@ -672,7 +767,7 @@ __device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m
       if (c < bs) {
         // z_c = tmp_c / L_cc + sum_{k > c} inv(L)[k][c] tmp_k
         double sacc = tmp[comp][c] / dblk[c][c];
-#pragma unroll
+#pragma unroll 4
         for (int k = 0; k < kNB; ++k)
           if (k > c && k < bs) sacc += dblk[c][k] * tmp[comp][k];
         (comp ? x1 : x0)[r0 + c] = sacc;
@@ -1093,12 +1188,13 @@ __global__ __launch_bounds__(256) void k_tps_interp(
     const int32_t* __restrict__ doms, const double* __restrict__ coef,
     const double* __restrict__ norm, const double* __restrict__ slot_xy, int64_t nslots,
     const int64_t* __restrict__ dom_off, const double* __restrict__ weights, int64_t wstride,
-    double* __restrict__ out) {
-  // one wavefront per new position; the centers of a subdomain are read by
-  // slot (coordinates and weights contiguous)
+    const uint8_t* __restrict__ skip, double* __restrict__ out) {
+  // one wavefront per new position, the lanes over the centers of a
+  // subdomain (read by slot: coordinates and weights contiguous); positions
+  // flagged in skip keep the value the caller put in out
   const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
-  if (p >= npts) return;
+  if (p >= npts || (skip != nullptr && skip[p])) return;
   const double x = px[p], y = py[p];
   double acc0 = 0.0, acc1 = 0.0;
   for (int k = 0; k < kMaxSlots; ++k) {
@@ -1161,9 +1257,13 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
       !refl || !vwork || !slot_xy || !smooth_slots || !weights || !stats || !status)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), size_t(8) * n_max * sizeof(double),
-                     s, xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy,
-                     nslots, status, int(n_max));
+  // Y = B V fused into the matrix build when its 12 per-wavefront arrays
+  // fit in LDS beside the 8 of the factorisation of P
+  const bool fused = size_t(20) * n_max * sizeof(double) <= kPrepFusedLds;
+  hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads),
+                     size_t(fused ? 20 : 8) * n_max * sizeof(double), s, xs, ys, dom_off,
+                     slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots, status,
+                     int(n_max), fused);
   // few subdomains: 8 wavefronts each; many: 4, two workgroups per CU
   if (nd <= 128)
     hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), 0, s, dom_off, mat_off, mats, vwork,
@@ -1242,13 +1342,13 @@ int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef, const 
 int piv_tps_interpolate(int32_t npts, const double* px, const double* py, const int32_t* doms,
                         const double* coef, const double* norm, const double* slot_xy,
                         int64_t nslots, const int64_t* dom_off, const double* weights,
-                        int64_t wstride, double* out, void* stream) {
+                        int64_t wstride, const uint8_t* skip, double* out, void* stream) {
   if (npts <= 0) return PIV_OK;
   if (!px || !py || !doms || !coef || !norm || !slot_xy || !dom_off || !weights || !out)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipLaunchKernelGGL(k_tps_interp, dim3((npts + 3) / 4), dim3(256), 0,
                      static_cast<hipStream_t>(stream), npts, px, py, doms, coef, norm, slot_xy,
-                     nslots, dom_off, weights, wstride, out);
+                     nslots, dom_off, weights, wstride, skip, out);
   return tps_check("piv_tps_interpolate");
 }
 

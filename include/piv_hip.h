@@ -275,12 +275,16 @@ int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef,
  * w_j r^2 ln(r^2) / 2 over its centers plus the linear part, blended with
  * coef[4 p + k] and divided by norm[p] (thin_plate_spline_subdom.py:224-234);
  * slot_xy: the center coordinates by slot written by piv_tps_smooth.
- * out[2][npts]. */
+ * out[2][npts]. skip (optional, one byte per position): positions with a
+ * nonzero byte are not evaluated and keep what out holds (the caller puts
+ * there the blended smoothed value of a position that is a center, which
+ * the evaluation equals: [K P][w; a] = v - s w at the centers). */
 int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
                         const int32_t* doms, const double* coef,
                         const double* norm, const double* slot_xy, int64_t nslots,
                         const int64_t* dom_off, const double* weights,
-                        int64_t wstride, double* out, void* stream);
+                        int64_t wstride, const uint8_t* skip, double* out,
+                        void* stream);
 
 /* ---- image deformation between passes (no counterpart in the reference,
  *      whose later passes shift the windows by the rounded predictor,
