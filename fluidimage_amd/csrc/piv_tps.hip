@@ -331,6 +331,23 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// 1 / x and 1 / sqrt(x) for positive normal x: the hardware estimates
+// refined by two Newton-Raphson steps (y <- y (2 - x y), y <- y (3 - x y^2) / 2)
+__device__ __forceinline__ double rcp_d(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
+__device__ __forceinline__ double rsqrt_d(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 
 // D(16x16) += A(16x4) B(4x16): lane l supplies A[l % 16][l / 16] and
@@ -357,6 +374,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   __shared__ double inv[kNB][kNB + 1];       // inv(L_JJ)
   __shared__ double lbuf[kNB][kNB + 1];      // L_JJ
   __shared__ double rdiag[kNB];              // 1 / L_ii
+  __shared__ double colj[kNB];               // pivot column of the diagonal factorisation
   __shared__ double tbuf[16][17];            // L21 X11
   __shared__ int bad;
   __shared__ int next_pair;                  // look-ahead work counter
@@ -386,6 +404,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       const double* pa1 = A + int64_t(ra1) * n + 4 * lk;
       const double* pb0 = A + int64_t(rb0) * n + 4 * lk;
       const double* pb1 = A + int64_t(rb1) * n + 4 * lk;
+      // the current values of the strip pair's block (rows 4 v + lk (+16),
+      // columns li (+16)), requested with the first operands: this
+      // wavefront alone writes them during the call
+      double cur[2][4][2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double* src = A + int64_t(min(row0 + 16 * h + 4 * v + lk, m - 1)) * n + r0t;
+          cur[h][v][0] = src[min(li, bst - 1)];
+          cur[h][v][1] = src[min(16 + li, bst - 1)];
+        }
       dvec4 c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
       // the rank-6 term V Z^T + Z V^T of B' as two more k-steps:
       // a_i = [V_i, Z_i], b_j = [Z_j, V_j] (k < 6, zero beyond)
@@ -449,12 +479,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             const double u0 = h ? c10[v] : c00[v];
             const double u1 = h ? c11[v] : c01[v];
             if (col0 < bst) {
-              const double r = dst[col0] - u0;
+              const double r = cur[h][v][0] - u0;
               dst[col0] = r;
               if (keep_diag && sp == 0) lbuf[16 * h + 4 * v + lk][col0] = r;  // diagonal block
             }
             if (col1 < bst) {
-              const double r = dst[col1] - u1;
+              const double r = cur[h][v][1] - u1;
               dst[col1] = r;
               if (keep_diag && sp == 0) lbuf[16 * h + 4 * v + lk][col1] = r;
             }
@@ -477,54 +507,72 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __syncthreads();
     // b) diagonal block by wavefront 0 (C_JJ is in lbuf, written there by
     //    this wavefront in a): right-looking factorisation in registers
-    //    (lane i < 32 holds row i; entries of other lanes through readlane),
-    //    then inv(L_JJ) by 16 x 16 blocks: the two diagonal blocks by forward
-    //    substitution (lane c: column c of the upper one, lane 16 + c: column
-    //    c of the lower one), the off-diagonal block -X22 (L21 X11) on the
-    //    matrix cores
+    //    (lane i < 32 holds row i; column j of the trailing matrix is
+    //    published in LDS at step j and read back by every lane: a_ic -=
+    //    (a_ij / a_jj) a_cj, l_ij = a_ij / sqrt(a_jj)), then inv(L_JJ) by
+    //    16 x 16 blocks: the two diagonal blocks by forward substitution
+    //    in registers (lane c: column c of the upper one, lane 16 + c:
+    //    column c of the lower one), the off-diagonal block -X22 (L21 X11)
+    //    on the matrix cores
     if (wave == 0) {
-      const bool drow = lane < kNB;
+      // the lane index through an opaque move: the lane-dependent selects
+      // below stay in this block instead of being hoisted out of the J loop
+      int dl;
+      __asm__ volatile("v_mov_b32 %0, %1" : "=v"(dl) : "v"(lane));
+      const bool drow = dl < kNB;
       bool singular = false;
       {
         double acc[kNB];
 #pragma unroll
         for (int c = 0; c < kNB; ++c)
-          acc[c] = (lane < bs && c <= lane && c < bs) ? lbuf[lane < kNB ? lane : 0][c] : 0.0;
+          acc[c] = (dl < bs && c <= dl && c < bs) ? lbuf[dl < kNB ? dl : 0][c] : 0.0;
 #pragma unroll
         for (int j = 0; j < kNB; ++j) {
-          double dj = readlane_d(acc[j], j);
+          if (drow) colj[dl] = acc[j];
+          wave_sync();
+          double dj = colj[j];
           if (j >= bs) dj = 1.0;
           if (!(dj > 0.0)) {
             singular = true;
             dj = 1.0;
           }
-          const double rj = 1.0 / sqrt(dj);
+          const double inv_d = rcp_d(dj);
+          const double rj = rsqrt_d(dj);
           if (drow) {
-            // column j of L_JJ (lane j: L_jj = dj rj), trailing update
-            acc[j] = j < bs ? acc[j] * rj : (lane == j ? 1.0 : 0.0);
+            // column j of L_JJ (lane j: L_jj = dj rj), trailing update,
+            // the next pivot column first
+            const double t = acc[j] * inv_d;
 #pragma unroll
-            for (int c = j + 1; c < kNB; ++c) acc[c] -= acc[j] * readlane_d(acc[j], c);
+            for (int c = j + 1; c < kNB; ++c) {
+              acc[c] -= t * colj[c];
+              if ((c - j) % 8 == 0) __asm__ volatile("" ::: "memory");  // 8 loads in flight
+            }
+            acc[j] = j < bs ? acc[j] * rj : (dl == j ? 1.0 : 0.0);
           }
+          wave_sync();
         }
-        wave_sync();
         if (drow) {
 #pragma unroll
-          for (int c = 0; c < kNB; ++c) lbuf[lane][c] = c <= lane ? acc[c] : 0.0;
-          rdiag[lane] = 1.0 / lbuf[lane][lane];  // written by this lane just above
+          for (int c = 0; c < kNB; ++c) lbuf[dl][c] = c <= dl ? acc[c] : 0.0;
+          rdiag[dl] = rcp_d(lbuf[dl][dl]);  // written by this lane just above
         }
       }
-      if (singular && lane == 0) bad = 1;
+      if (singular && dl == 0) bad = 1;
       wave_sync();
       if (drow) {
         // X11 (o = 0) and X22 (o = 16): column c, x_i = (delta_ic -
-        // sum_{c <= k < i} L_ik x_k) / L_ii
-        const int o = lane < 16 ? 0 : 16, c = lane - o;
-#pragma unroll 1
+        // sum_{k < i} L_ik x_k) / L_ii (x_k = 0 for k < c)
+        const int o = dl < 16 ? 0 : 16, c = dl - o;
+        double xc[16];
+#pragma unroll
         for (int i = 0; i < 16; ++i) {
           double sx = i == c ? 1.0 : 0.0;
-          for (int k = c; k < i; ++k) sx -= lbuf[o + i][o + k] * inv[o + k][o + c];
-          inv[o + i][o + c] = i < c ? 0.0 : sx * rdiag[o + i];
+#pragma unroll
+          for (int k = 0; k < i; ++k) sx -= lbuf[o + i][o + k] * xc[k];
+          xc[i] = i < c ? 0.0 : sx * rdiag[o + i];
+          inv[o + i][o + c] = xc[i];
           inv[o + c][16 - o + i] = 0.0;  // the strictly upper 16 x 16 block stays zero
+          __asm__ volatile("" ::: "memory");  // keep the row loads of lbuf in their step
         }
       }
       wave_sync();
@@ -545,11 +593,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         for (int v = 0; v < 4; ++v) inv[16 + 4 * v + lk][li] = -x21[v];
       }
       wave_sync();
-      if (lane < bs) {
+      if (dl < bs) {
         // L_JJ (lower with the diagonal) and inv(L_JJ)^T (strictly upper)
-        // into the matrix, row `lane`
-        double* blk = A + int64_t(r0 + lane) * n + r0;
-        for (int c = 0; c < bs; ++c) blk[c] = c <= lane ? lbuf[lane][c] : inv[c][lane];
+        // into the matrix, row `dl`
+        double* blk = A + int64_t(r0 + dl) * n + r0;
+        for (int c = 0; c < bs; ++c) blk[c] = c <= dl ? lbuf[dl][c] : inv[c][dl];
       }
     }
     // look-ahead: blocks 0..J-1 (and the rank-6 term) applied to block column
@@ -569,15 +617,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     // c) panel rows below the diagonal block: L(rows, J) = C(rows, J) inv^T,
     //    D[i][j] = sum_k C[i][k] inv[j][k] (k <= j), k = k0 + 4 lk + q
     if (bs == kNB) {
-      for (int st = 2 + wave; st < nstrip; st += kCholWaves) {
-        const int row0 = r0 + 16 * st;
-        const int ra = min(row0 + li, m - 1);
+      // each wavefront requests the next strip's rows before the matrix-core
+      // steps of the current one
+      auto load_strip = [&](double* a8, int st) {
+        const int ra = min(r0 + 16 * st + li, m - 1);
         const double* pa = A + int64_t(ra) * n + r0 + 4 * lk;
-        double a8[8];
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
           for (int q = 0; q < 4; ++q) a8[4 * h + q] = pa[16 * h + q];
+      };
+      double a8[8], nx8[8];
+      int st = 2 + wave;
+      if (st < nstrip) load_strip(a8, st);
+      for (; st < nstrip; st += kCholWaves) {
+        const int row0 = r0 + 16 * st;
+        if (st + kCholWaves < nstrip) load_strip(nx8, st + kCholWaves);
         dvec4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int h = 0; h < 2; ++h)
@@ -598,6 +653,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             dst[16 + li] = c1[v];
           }
         }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a8[e] = nx8[e];
       }
       __syncthreads();
     }
