@@ -374,7 +374,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   __shared__ double inv[kNB][kNB + 1];       // inv(L_JJ)
   __shared__ double lbuf[kNB][kNB + 1];      // L_JJ
   __shared__ double rdiag[kNB];              // 1 / L_ii
-  __shared__ double colj[kNB];               // pivot column of the diagonal factorisation
+  __shared__ double colj[64];                // pivot column of the diagonal factorisation
   __shared__ double tbuf[16][17];            // L21 X11
   __shared__ int bad;
   __shared__ int next_pair;                  // look-ahead work counter
@@ -528,9 +528,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
           acc[c] = (dl < bs && c <= dl && c < bs) ? lbuf[dl < kNB ? dl : 0][c] : 0.0;
 #pragma unroll
         for (int j = 0; j < kNB; ++j) {
-          if (drow) colj[dl] = acc[j];
+          // (lanes 32..63 hold zero rows and run along without branches)
+          colj[dl] = acc[j];
           wave_sync();
-          double dj = colj[j];
+          double col[kNB];  // the pivot column, read at once
+#pragma unroll
+          for (int c = j; c < kNB; ++c) col[c] = colj[c];
+          double dj = col[j];
           if (j >= bs) dj = 1.0;
           if (!(dj > 0.0)) {
             singular = true;
@@ -538,17 +542,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
           }
           const double inv_d = rcp_d(dj);
           const double rj = rsqrt_d(dj);
-          if (drow) {
-            // column j of L_JJ (lane j: L_jj = dj rj), trailing update,
-            // the next pivot column first
-            const double t = acc[j] * inv_d;
+          // column j of L_JJ (lane j: L_jj = dj rj), trailing update
+          const double t = acc[j] * inv_d;
 #pragma unroll
-            for (int c = j + 1; c < kNB; ++c) {
-              acc[c] -= t * colj[c];
-              if ((c - j) % 8 == 0) __asm__ volatile("" ::: "memory");  // 8 loads in flight
-            }
-            acc[j] = j < bs ? acc[j] * rj : (dl == j ? 1.0 : 0.0);
-          }
+          for (int c = j + 1; c < kNB; ++c) acc[c] -= t * col[c];
+          acc[j] = j < bs ? acc[j] * rj : (dl == j ? 1.0 : 0.0);
           wave_sync();
         }
         if (drow) {
