@@ -29,8 +29,55 @@ constexpr int kNB = 32;       // Cholesky / triangular-solve block size
 constexpr int kMaxN = 2048;   // centers per subdomain
 constexpr size_t kPrepFusedLds = 150 * 1024;  // dynamic LDS bound of the fused build
 
+// 1 / x and 1 / sqrt(x) for positive normal x: the hardware estimates
+// refined by two Newton-Raphson steps (y <- y (2 - x y), y <- y (3 - x y^2) / 2)
+__device__ __forceinline__ double rcp_d(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
+__device__ __forceinline__ double rsqrt_d(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+// ln x for positive normal x, to about 2 ulp: x = 2^e m with m in
+// [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m - 1) / (m + 1), |s| < 0.1716,
+// by the odd series s + s^3 / 3 + ... + s^19 / 19 (the first omitted term is
+// below 2.3e-17 relative); e ln 2 added in two parts (the high part has 42
+// significant bits, so e ln2_hi is exact for every exponent)
+__device__ __forceinline__ double ln_pos(double x) {
+  constexpr double kLn2Hi = 0x1.62e42fefa3800p-1, kLn2Lo = 0x1.ef35793c76730p-45;
+  int e = __builtin_amdgcn_frexp_exp(x);  // x = mant 2^e, mant in [1/2, 1)
+  double m = __builtin_amdgcn_frexp_mant(x);
+  if (m < 0.70710678118654752440) {
+    m = 2.0 * m;
+    e -= 1;
+  }
+  const double s = (m - 1.0) * rcp_d(m + 1.0);
+  const double z = s * s;
+  double p = 1.0 / 19.0;
+  p = fma(p, z, 1.0 / 17.0);
+  p = fma(p, z, 1.0 / 15.0);
+  p = fma(p, z, 1.0 / 13.0);
+  p = fma(p, z, 1.0 / 11.0);
+  p = fma(p, z, 1.0 / 9.0);
+  p = fma(p, z, 1.0 / 7.0);
+  p = fma(p, z, 1.0 / 5.0);
+  p = fma(p, z, 1.0 / 3.0);
+  const double t = 2.0 * s;
+  const double de = double(e);
+  return fma(de, kLn2Hi, fma(de, kLn2Lo, fma(t * z, p, t)));
+}
+
+// r^2 ln(r^2) / 2, zero at r = 0
 __device__ __forceinline__ double tps_phi(double r2) {
-  return r2 != 0.0 ? r2 * log(r2) / 2.0 : 0.0;
+  return r2 != 0.0 ? r2 * ln_pos(r2) * 0.5 : 0.0;
 }
 
 __device__ double block_sum(double v, double* red) {
@@ -329,23 +376,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// 1 / x and 1 / sqrt(x) for positive normal x: the hardware estimates
-// refined by two Newton-Raphson steps (y <- y (2 - x y), y <- y (3 - x y^2) / 2)
-__device__ __forceinline__ double rcp_d(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  y = fma(y, fma(-x, y, 1.0), y);
-  y = fma(y, fma(-x, y, 1.0), y);
-  return y;
-}
-
-__device__ __forceinline__ double rsqrt_d(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  const double h = 0.5 * x;
-  y = y * fma(-h * y, y, 1.5);
-  y = y * fma(-h * y, y, 1.5);
-  return y;
 }
 
 typedef double dvec4 __attribute__((ext_vector_type(4)));
