@@ -108,6 +108,7 @@ class _Field:
         self.k = k
         self.n = n          # vectors of the field
         self.base = base    # position of its first center in the batch arrays
+        self.sorted_y = 0   # centers sorted by y (membership scans a y range)
         self.error = None
 
 
@@ -149,7 +150,7 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
     vx = torch.empty(ntot, dtype=torch.float32, device=dev)
     vy = torch.empty(ntot, dtype=torch.float32, device=dev)
     idx = torch.empty(ntot, **i32)
-    info = torch.empty((len(fields), 5), **f64)
+    info = torch.empty((len(fields), 6), **f64)
     _native.check(lib.piv_tps_select(len(fields), n, _p(dx), _p(dy), _p(xs), _p(ys), _p(cx),
                                      _p(cy), _p(vx), _p(vy), _p(idx), _p(info), sp),
                   "piv_tps_select")
@@ -157,8 +158,9 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
     # subdomain grids (host scalars)
     active = []
     for f in flist:
-        count, x_min, x_max, y_min, y_max = info_h[f.k].tolist()
+        count, x_min, x_max, y_min, y_max, sorted_y = info_h[f.k].tolist()
         f.count = int(count)
+        f.sorted_y = int(sorted_y)
         if f.count == 0:
             f.error = SingularSystemError("Only nan.")
             continue
@@ -232,7 +234,7 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     nd = nd_base
     lim_h = np.concatenate([f.lim_h for f in active])
     lim_off = np.concatenate([[0], np.cumsum([f.lim_h.size for f in active])])
-    table = np.array([[f.nsx, f.nsy, f.count, f.base, f.nd_base, lim_off[j]]
+    table = np.array([[f.nsx, f.nsy, f.count, f.base, f.nd_base, lim_off[j], f.sorted_y]
                       for j, f in enumerate(active)], dtype=np.int32)
     dom_field_h = np.concatenate([np.full(f.nd, j, np.int32) for j, f in enumerate(active)])
     smoothing_h = np.concatenate([np.full(f.nd, f.smoothing) for f in active])

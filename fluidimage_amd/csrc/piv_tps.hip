@@ -1043,12 +1043,13 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
     vx += o;
     vy += o;
     idx += o;
-    info += 5 * blockIdx.x;
+    info += 6 * blockIdx.x;
   }
   __shared__ int tmp[32];
   __shared__ double red[4][16];
   double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
   int base = 0;
+  int unsorted = 0;  // some ys[i] > ys[i + 1]
   for (int t0 = 0; t0 < n; t0 += 1024) {
     const int i = t0 + threadIdx.x;
     float a = 0.f, b = 0.f;
@@ -1057,6 +1058,7 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
       a = dx[i];
       b = dy[i];
       f = (a == a && b == b) ? 1 : 0;
+      if (i + 1 < n && !(ys[i] <= ys[i + 1])) unsorted = 1;
     }
     int total;
     const int incl = block_scan_1024(f, tmp, total);
@@ -1075,6 +1077,7 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
     }
     base += total;
   }
+  unsorted = __syncthreads_or(unsorted);
   for (int off = 32; off > 0; off >>= 1) {
     xmin = fmin(xmin, __shfl_xor(xmin, off, 64));
     xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
@@ -1100,6 +1103,7 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
     info[2] = red[1][0];
     info[3] = red[2][0];
     info[4] = red[3][0];
+    info[5] = unsorted ? 0.0 : 1.0;
   }
 }
 
@@ -1129,6 +1133,38 @@ __device__ __forceinline__ SubGrid field_grid(const piv_tps_field& f, const doub
   return SubGrid{f.nsx, f.nsy, lim + f.lim_off};
 }
 
+// first index i in [0, n) with a[i] >= v for a nondecreasing array, by the
+// whole block: the block's samples a[t step] locate the bucket, then the
+// bucket's elements are counted chunk by chunk
+__device__ int block_lower_bound(const double* a, int n, double v) {
+  const int nt = blockDim.x;
+  const int step = (n + nt - 1) / nt;
+  const int i1 = threadIdx.x * step;
+  const int c = __syncthreads_count(i1 < n && a[i1] < v);  // samples below v
+  if (c == 0) return 0;
+  int lo = (c - 1) * step + 1;  // a[lo - 1] < v; the answer is in [lo, hi]
+  const int hi = min(c * step, n);
+  int below = 0;
+  for (int c0 = lo; c0 < hi; c0 += nt) {
+    const int i2 = c0 + threadIdx.x;
+    below += __syncthreads_count(i2 < hi && a[i2] < v);
+  }
+  return lo + below;
+}
+
+// the range of centers [first, last) of a field that may lie in subdomain d:
+// those with y in [ylo, yhi) when the centers are sorted by y, else all
+__device__ void member_range(const piv_tps_field& f, const SubGrid& g, int d, const double* cy,
+                             int& first, int& last) {
+  first = 0;
+  last = f.count;
+  if (f.sorted_y) {
+    const int iy = d / g.nsx;
+    first = block_lower_bound(cy, f.count, g.ylo(iy));
+    last = block_lower_bound(cy, f.count, g.yhi(iy));
+  }
+}
+
 // members of (global) subdomain blockIdx.x: count
 __global__ __launch_bounds__(256) void k_tps_count(const piv_tps_field* __restrict__ fields,
                                                    const int32_t* __restrict__ dom_field,
@@ -1139,11 +1175,12 @@ __global__ __launch_bounds__(256) void k_tps_count(const piv_tps_field* __restri
   const piv_tps_field f = fields[dom_field[blockIdx.x]];
   const SubGrid g = field_grid(f, lim);
   const int d = blockIdx.x - f.dom_base;
-  const int n = f.count;
   const double* cx = cx_all + f.center_base;
   const double* cy = cy_all + f.center_base;
+  int first, last;
+  member_range(f, g, d, cy, first, last);
   int c = 0;
-  for (int i = threadIdx.x; i < n; i += 256) c += g.inside(d, cx[i], cy[i]) ? 1 : 0;
+  for (int i = first + threadIdx.x; i < last; i += 256) c += g.inside(d, cx[i], cy[i]) ? 1 : 0;
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
   __shared__ int part[4];
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
@@ -1163,17 +1200,18 @@ __global__ __launch_bounds__(1024) void k_tps_fill(const piv_tps_field* __restri
   const piv_tps_field f = fields[dom_field[blockIdx.x]];
   const SubGrid g = field_grid(f, lim);
   const int d = blockIdx.x - f.dom_base;
-  const int n = f.count;
   const int center_base = f.center_base;
   const double* cx = cx_all + center_base;
   const double* cy = cy_all + center_base;
   int64_t base = dom_off[blockIdx.x];
-  for (int c0 = 0; c0 < n; c0 += 1024) {
+  int first, last;
+  member_range(f, g, d, cy, first, last);
+  for (int c0 = first; c0 < last; c0 += 1024) {
     const int i = c0 + threadIdx.x;
-    const int f = (i < n && g.inside(d, cx[i], cy[i])) ? 1 : 0;
+    const int in = (i < last && g.inside(d, cx[i], cy[i])) ? 1 : 0;
     int total;
-    const int incl = block_scan_1024(f, tmp, total);
-    if (f) slot_pt[base + incl - 1] = center_base + i;
+    const int incl = block_scan_1024(in, tmp, total);
+    if (in) slot_pt[base + incl - 1] = center_base + i;
     base += total;
   }
 }

@@ -224,13 +224,17 @@ typedef struct piv_tps_field {
   int32_t center_base;
   int32_t dom_base;
   int32_t lim_off;
+  int32_t sorted_y; /* nonzero: the centers are sorted by y (info[f][5] of
+                       piv_tps_select); membership then scans only the
+                       centers within a subdomain's y limits */
 } piv_tps_field;
 
 /* piv_tps_select: for each of nfields fields of n vectors (elements
  * [f n, (f + 1) n) of dx, dy, xs, ys), ordered compaction of the vectors
  * whose dx and dy are finite (singlepass.py:450-463) into cx, cy
  * (positions), vx, vy (values) and idx (index in the field), starting at
- * element f n; info[f][5] = count, x_min, x_max, y_min, y_max.
+ * element f n; info[f][6] = count, x_min, x_max, y_min, y_max, and 1.0
+ * when ys is nondecreasing over the field (0.0 otherwise).
  * piv_tps_count / piv_tps_fill: members of every subdomain d of the batch
  * (field dom_field[d], local index d - dom_base = iy * nsx + ix) with the
  * half-open limits of its field, in increasing center order
