@@ -3,7 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs P]
                     [--workload multipass|single]
 
-Workload "multipass" (default): P seeded 2048x2048 uint16 pairs (12-bit,
+Workload "multipass" (default): P (32 by default) seeded 2048x2048 uint16 pairs (12-bit,
 0.03 particles/px, 8 Fourier modes up to 6 px, 2 % of the 64 px windows of
 im1 replaced by noise), WorkPIV with 3 passes (64, 32, 16 px windows, 50 %
 overlap), ThinPlateSplineSubdom between passes and at the end (use_tps True,
@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--pairs", type=int, default=8)
+    ap.add_argument("--pairs", type=int, default=32)
     ap.add_argument("--workload", default="multipass", choices=["multipass", "single", "refine"])
     ap.add_argument("--no-maps", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
