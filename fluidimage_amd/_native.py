@@ -96,6 +96,15 @@ class PivDeformGrid(ctypes.Structure):
     ]
 
 
+class PivTpsLattice(ctypes.Structure):
+    _fields_ = [
+        ("table", ctypes.c_void_p),
+        ("step", ctypes.c_double),
+        ("a_max", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 # exported symbol -> (restype, argtypes); mirrors include/piv_hip.h
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -122,8 +131,9 @@ SIGNATURES = {
     "piv_tps_smooth": (
         ctypes.c_int,
         [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P, _P, _P, _P, _P,
-         ctypes.c_int64, _P, ctypes.c_int64, _P, _P, _P],
+         ctypes.c_int64, _P, ctypes.c_int64, _P, _P, _P, _P],
     ),
+    "piv_tps_phi_table": (ctypes.c_int, [ctypes.c_double, _I, _P, _P]),
     "piv_tps_select": (ctypes.c_int, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "piv_tps_count": (ctypes.c_int, [_I, _P, _P, _P, _P, _P, _P, _P]),
     "piv_tps_fill": (ctypes.c_int, [_I, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -132,7 +142,10 @@ SIGNATURES = {
     "piv_tps_blend": (ctypes.c_int, [_I, _P, _P, _P, _P, ctypes.c_int64, _P, _P]),
     "piv_tps_interpolate": (
         ctypes.c_int,
-        [_I, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P, _P],
+        [_I, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _I, _I, _P, _P, ctypes.c_int64, _P, _P,
+         _P, _P, _P],
+    ),
+    "piv_tps_interpolate_workspace": (ctypes.c_int64, [_I, _I]
     ),
     "piv_deform_images": (
         ctypes.c_int, [_I, _P, _I, _I, _I, ctypes.POINTER(PivDeformGrid), _P, _P, _P, _P]),

@@ -30,6 +30,7 @@ from .. import _native
 
 MAX_SLOTS = 4
 MAX_CENTERS_PER_SUBDOM = 2048
+MAX_LATTICE_A = 511  # largest lattice table: 512 x 512 doubles (2 MiB)
 
 
 class SingularSystemError(ArithmeticError):
@@ -63,6 +64,47 @@ def _p(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def lattice_step(*arrays):
+    """Largest step h = g / 2**k (g a positive integer, k <= 2) of which every
+    value of the host arrays is an integer multiple, or None. The PIV grids
+    (window centres w // 2 + i step) give the grid step, or its gcd with the
+    next pass's step."""
+    vals = np.concatenate([np.ravel(np.asarray(a, dtype=np.float64)) for a in arrays])
+    if vals.size == 0 or not np.all(np.isfinite(vals)):
+        return None
+    for k in range(3):
+        scaled = vals * (1 << k)
+        ints = np.rint(scaled)
+        if np.all(ints == scaled) and np.all(np.abs(ints) < 2**31):
+            g = int(np.gcd.reduce(np.abs(ints).astype(np.int64)))
+            return None if g == 0 else g / (1 << k)
+    return None
+
+
+def _lattice(lib, step, fields_active, dev, sp):
+    """Device table of the kernel function on the lattice of step ``step``,
+    large enough for any two points of one subdomain (centers and new
+    positions lie inside its limits); None when there is no step or the
+    table would exceed MAX_LATTICE_A."""
+    import torch
+
+    if not step:
+        return None, None
+    extent = 0.0
+    for f in fields_active:
+        nsx = f.nsx
+        lx = f.lim_h[nsx:2 * nsx] - f.lim_h[:nsx]
+        ly = f.lim_h[2 * nsx + f.nsy:] - f.lim_h[2 * nsx:2 * nsx + f.nsy]
+        extent = max(extent, float(lx.max()), float(ly.max()))
+    a_max = int(np.ceil(extent / step)) + 1
+    if a_max > MAX_LATTICE_A:
+        return None, None
+    table = torch.empty((a_max + 1) ** 2, dtype=torch.float64, device=dev)
+    _native.check(lib.piv_tps_phi_table(float(step), a_max, _p(table), sp), "piv_tps_phi_table")
+    lat = _native.PivTpsLattice(table.data_ptr(), float(step), a_max, 0)
+    return lat, table
+
+
 def _upload(arrays, device):
     """Host numpy arrays to device tensors in one pinned, asynchronous copy
     (8-byte aligned slices of one byte buffer)."""
@@ -85,7 +127,7 @@ def _upload(arrays, device):
 
 
 def tps_device(dx, dy, xs, ys, new_x, new_y, subdom_size, smoothing_coef, threshold,
-               percent_buffer_area=20, stream=None, keep=False):
+               percent_buffer_area=20, stream=None, keep=False, lattice=None):
     """TPS smoothing of the finite vectors of (dx, dy) at (xs, ys) and
     evaluation at (new_x, new_y); all tensors on the device (dx, dy float32,
     positions float64). Small device-to-host reads size the problem.
@@ -93,9 +135,12 @@ def tps_device(dx, dy, xs, ys, new_x, new_y, subdom_size, smoothing_coef, thresh
     Returns a dict of device tensors: ``smoothed`` (2, count) at the
     selected centers, ``index`` (count) of the selected vectors, ``approx``
     (2, M) at the new positions, plus host ``nb_subdom``, ``stats`` (nd, 6).
+    ``lattice``: a step of which the positions are multiples (see
+    ``lattice_step``), or None; it changes the speed, not the result.
     """
     out = tps_device_batch([(dx, dy, xs, ys)], new_x, new_y, subdom_size, smoothing_coef,
-                           threshold, percent_buffer_area, stream=stream, keep=keep)[0]
+                           threshold, percent_buffer_area, stream=stream, keep=keep,
+                           lattice=lattice)[0]
     if isinstance(out, Exception):
         raise out
     return out
@@ -113,7 +158,7 @@ class _Field:
 
 
 def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshold,
-                     percent_buffer_area=20, stream=None, keep=False):
+                     percent_buffer_area=20, stream=None, keep=False, lattice=None):
     """``tps_device`` for several vector fields at once (one image pair
     each): each field gets its own subdomain grid (thin_plate_spline_subdom.py:53-90),
     and the subdomains of all fields go through one launch of every
@@ -122,7 +167,7 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
 
     Returns one entry per field: the dict of ``tps_device`` or the
     ``SingularSystemError`` of that field alone ("Only nan." when it has no
-    finite vector).
+    finite vector). ``lattice``: as in ``tps_device``.
     """
     import torch
 
@@ -174,7 +219,8 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
         f.nd = f.nsx * f.nsy
         active.append(f)
     own = (xs, ys, idx, n)
-    results = _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep, own)
+    results = _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep, own,
+                            lattice)
     out = []
     for f in flist:
         if f.error is not None:
@@ -217,7 +263,8 @@ def _at_centers(active, own, smoothed, new_x, new_y, dev):
     return approx.contiguous(), done.to(torch.uint8)
 
 
-def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep, own):
+def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep, own,
+                  lattice=None):
     """Membership, FP64 solves, blend and evaluation for the fields whose
     grid is valid; fields with an empty subdomain get their error and the
     others are solved again without them."""
@@ -255,7 +302,7 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
                 "use a smaller params.multipass.subdom_size")
     if retry:
         return _solve_active(lib, [f for f in active if f.error is None], cx, cy, vx, vy,
-                             new_x, new_y, thr, dev, sp, keep, own)
+                             new_x, new_y, thr, dev, sp, keep, own, lattice)
     n_max = int(sizes.max())
     dom_off_h = np.concatenate([[0], np.cumsum(sizes)])
     mat_off_h = np.concatenate([[0], np.cumsum(sizes**2)])
@@ -289,11 +336,13 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     weights = torch.empty((2, wstride), **f64)
     stats = torch.zeros((nd, 6), **f64)
     slot_xy = torch.empty((2, nslots), **f64)
+    lat, lat_table = _lattice(lib, lattice, active, dev, sp)
+    lat_p = None if lat is None else ctypes.byref(lat)
     _native.check(
         lib.piv_tps_smooth(nd, n_max, _p(cx), _p(cy), _p(vx), _p(vy), _p(dom_off), _p(slot_pt),
                            _p(mat_off), _p(smoothing), thr, _p(mats), _p(refl), _p(vwork),
                            _p(slot_xy), _p(smooth_slots), nslots, _p(weights), wstride,
-                           _p(stats), _p(status), sp),
+                           _p(stats), _p(status), lat_p, sp),
         "piv_tps_smooth")
     smoothed = torch.empty((2, ntot), **f64)
     for f in active:
@@ -310,11 +359,15 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     approx, skip = _at_centers(active, own, smoothed, new_x, new_y, dev)
     if approx is None:
         approx = torch.empty((2, na * m), **f64)
+    wbytes = int(lib.piv_tps_interpolate_workspace(na * m, nd))
+    iwork = torch.empty(max(wbytes, 8), dtype=torch.uint8, device=dev)
     _native.check(lib.piv_tps_interpolate(na * m, _p(px), _p(py), _p(n_dom), _p(n_coef),
-                                          _p(n_norm), _p(slot_xy), nslots, _p(dom_off),
-                                          _p(weights), wstride, _p(skip), _p(approx), sp),
+                                          _p(n_norm), _p(slot_xy), nslots, nd, n_max,
+                                          _p(dom_off), _p(weights), wstride, _p(skip),
+                                          _p(approx), _p(iwork), lat_p, sp),
                   "piv_tps_interpolate")
-    st = status.cpu().numpy()
+    st = status.cpu().numpy()  # synchronises: the workspaces are free to go after this
+    del lat_table, iwork
     if st[nd]:
         raise NotImplementedError("a point lies in more than 4 TPS subdomains")
     results = {}

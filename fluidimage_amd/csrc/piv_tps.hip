@@ -27,7 +27,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kNB = 32;       // Cholesky / triangular-solve block size
 constexpr int kMaxN = 2048;   // centers per subdomain
-constexpr size_t kPrepFusedLds = 150 * 1024;  // dynamic LDS bound of the fused build
+constexpr int kMaxLatticeA = 4095;            // largest lattice table: 4096 x 4096
 
 // 1 / x and 1 / sqrt(x) for positive normal x: the hardware estimates
 // refined by two Newton-Raphson steps (y <- y (2 - x y), y <- y (3 - x y^2) / 2)
@@ -80,6 +80,52 @@ __device__ __forceinline__ double tps_phi(double r2) {
   return r2 != 0.0 ? r2 * ln_pos(r2) * 0.5 : 0.0;
 }
 
+// r^2 of a difference, one expression for every caller (the lattice table
+// and the direct evaluation round alike)
+__device__ __forceinline__ double tps_r2(double dx, double dy) { return fma(dy, dy, dx * dx); }
+
+// The kernel function at offsets that are multiples of a lattice step:
+// table[(a_max + 1) a + b] = tps_phi(tps_r2(a step, b step)) for
+// 0 <= a, b <= a_max. Where both |dx| and |dy| are exactly a step, b step
+// with a, b <= a_max, the table entry is the value the direct evaluation
+// gives (same operands, same expression; the signs drop out of the
+// squares); any other difference is evaluated directly.
+struct PhiLattice {
+  const double* table;
+  double step, inv_step;
+  int a_max;
+};
+
+__device__ __forceinline__ double tps_phi_at(double dx, double dy, const PhiLattice& lat) {
+  if (lat.table != nullptr) {
+    const double ax = fabs(dx), ay = fabs(dy);
+    const double fa = rint(ax * lat.inv_step), fb = rint(ay * lat.inv_step);
+    if (fa <= double(lat.a_max) && fb <= double(lat.a_max) && fa * lat.step == ax &&
+        fb * lat.step == ay)
+      return lat.table[int(fa) * (lat.a_max + 1) + int(fb)];
+  }
+  return tps_phi(tps_r2(dx, dy));
+}
+
+__global__ void k_tps_phi_table(double step, int a_max, double* __restrict__ table) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = a_max + 1;
+  if (e >= w * w) return;
+  const int a = e / w, b = e % w;
+  table[e] = tps_phi(tps_r2(double(a) * step, double(b) * step));
+}
+
+PhiLattice make_lattice(const piv_tps_lattice* l) {
+  PhiLattice out{nullptr, 0.0, 0.0, 0};
+  if (l != nullptr && l->table != nullptr && l->step > 0.0 && l->a_max >= 0) {
+    out.table = l->table;
+    out.step = l->step;
+    out.inv_step = 1.0 / l->step;
+    out.a_max = l->a_max;
+  }
+  return out;
+}
+
 __device__ double block_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   __syncthreads();
@@ -115,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
     const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
     double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
     double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm,
-    bool fused) {
+    PhiLattice lat) {
   constexpr int kWaves = kThreads / 64;
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
@@ -124,14 +170,12 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
   double* RF = refl + 16 * d;
   double* VW = vwork + 6 * s0;
   // dynamic LDS, nm >= n doubles per array: cy, cx, P (3 columns, then
-  // their reflected copy), V (3 Householder vectors) and, when fused, the
-  // per-wavefront sums of Y = B V (3 kWaves arrays)
+  // their reflected copy, then Y = B V), V (3 Householder vectors)
   extern __shared__ double dyn[];
   double* cy = dyn;
   double* cx = dyn + nm;
 #define PM(k, j) dyn[(2 + (k)) * nm + (j)]
 #define VM(k, j) dyn[(5 + (k)) * nm + (j)]
-#define YC(w, c, j) dyn[(8 + 3 * (w) + (c)) * nm + (j)]
   __shared__ double red[kThreads / 64];
   __shared__ double sh[8];
   const int tid = threadIdx.x;
@@ -217,90 +261,43 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
     const bool singular = n < 3 || PM(0, 0) == 0.0 || PM(1, 1) == 0.0 || PM(2, 2) == 0.0;
     if (singular) status[d] = 1;
   }
-  // ---- lower triangle of B = K + s I: row i by one wavefront, its lanes
-  //      over the columns j <= i (K is symmetric; only the lower triangle
-  //      is read afterwards) ----
+  // ---- lower triangle of B = K + s I and Y = B V: row i by one
+  //      wavefront, its lanes over all columns j (B_ij = B_ji bit for bit:
+  //      the kernel function sees the squares of the differences), so that
+  //      Y_i is a wavefront reduction and nothing is scattered; only the
+  //      lower triangle j <= i is stored (it is all the factorisation
+  //      reads). Y goes to the P slots (P is no longer used: R and the
+  //      reflectors are saved above) ----
   double m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int lane = tid & 63, wave = tid >> 6;
-  if (fused) {
-    // Y = B V accumulated while B is built (B is not read back): row i adds
-    // sum_{j <= i} B_ij V_j to Y_i (wavefront reduction) and B_ij V_i to
-    // Y_j for j < i; each wavefront sums into its own copy YC(w, ., .), in
-    // row order, so the result does not depend on the wave schedule
-    for (int e = tid; e < 3 * kWaves * nm; e += kThreads) YC(0, 0, e) = 0.0;
-    __syncthreads();
-    for (int i = wave; i < n; i += kWaves) {
-      const double yi = cy[i], xi = cx[i];
-      const double vi0 = VM(0, i), vi1 = VM(1, i), vi2 = VM(2, i);
-      double* Bi = B + int64_t(i) * n;
-      double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-      for (int j = lane; j <= i; j += 64) {
-        const double dy = cy[j] - yi, dx = cx[j] - xi;
-        double b = tps_phi(dy * dy + dx * dx);
-        if (i == j) b += smoothing;
-        Bi[j] = b;
-        r0 += b * VM(0, j);
-        r1 += b * VM(1, j);
-        r2 += b * VM(2, j);
-        if (j < i) {
-          YC(wave, 0, j) += b * vi0;
-          YC(wave, 1, j) += b * vi1;
-          YC(wave, 2, j) += b * vi2;
-        }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        r0 += __shfl_xor(r0, off, 64);
-        r1 += __shfl_xor(r1, off, 64);
-        r2 += __shfl_xor(r2, off, 64);
-      }
-      if (lane == 0) {
-        YC(wave, 0, i) += r0;
-        YC(wave, 1, i) += r1;
-        YC(wave, 2, i) += r2;
-      }
+  __syncthreads();  // thread 0 has read R out of P
+  for (int i = wave; i < n; i += kWaves) {
+    const double yi = cy[i], xi = cx[i];
+    double* Bi = B + int64_t(i) * n;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    for (int j = lane; j < n; j += 64) {
+      double b = tps_phi_at(cx[j] - xi, cy[j] - yi, lat);
+      if (i == j) b += smoothing;
+      if (j <= i) Bi[j] = b;
+      r0 += b * VM(0, j);
+      r1 += b * VM(1, j);
+      r2 += b * VM(2, j);
     }
-    __syncthreads();
-  } else {
-    for (int i = wave; i < n; i += kWaves) {
-      const double yi = cy[i], xi = cx[i];
-      double* Bi = B + int64_t(i) * n;
-      for (int j = lane; j <= i; j += 64) {
-        const double dy = cy[j] - yi, dx = cx[j] - xi;
-        double b = tps_phi(dy * dy + dx * dx);
-        if (i == j) b += smoothing;
-        Bi[j] = b;
-      }
+    for (int off = 32; off > 0; off >>= 1) {
+      r0 += __shfl_xor(r0, off, 64);
+      r1 += __shfl_xor(r1, off, 64);
+      r2 += __shfl_xor(r2, off, 64);
     }
-    __syncthreads();
+    if (lane == 0) {
+      PM(0, i) = r0;
+      PM(1, i) = r1;
+      PM(2, i) = r2;
+    }
   }
-  // ---- Y = B V (fused: the wavefront copies summed; else column j of
-  //      thread, B read back from the lower triangle: B[i][j] for i >= j
-  //      down the column, B[j][i] along row j for i < j), M = V^T Y,
-  //      Z = Y T - 1/2 V T^T M T ----
+  __syncthreads();
+  // ---- M = V^T Y, Z = Y T - 1/2 V T^T M T ----
   for (int j = tid; j < n; j += kThreads) {
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-    if (fused) {
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
-        y0 += YC(w, 0, j);
-        y1 += YC(w, 1, j);
-        y2 += YC(w, 2, j);
-      }
-    } else {
-      const double* Bj = B + int64_t(j) * n;
-      for (int i = 0; i < j; ++i) {
-        const double b = Bj[i];
-        y0 += b * VM(0, i);
-        y1 += b * VM(1, i);
-        y2 += b * VM(2, i);
-      }
-      for (int i = j; i < n; ++i) {
-        const double b = B[int64_t(i) * n + j];
-        y0 += b * VM(0, i);
-        y1 += b * VM(1, i);
-        y2 += b * VM(2, i);
-      }
-    }
+    const double y0 = PM(0, j), y1 = PM(1, j), y2 = PM(2, j);
     const double v0 = VM(0, j), v1 = VM(1, j), v2 = VM(2, j);
     m9[0] += v0 * y0; m9[1] += v0 * y1; m9[2] += v0 * y2;
     m9[3] += v1 * y0; m9[4] += v1 * y1; m9[5] += v1 * y2;
@@ -342,7 +339,6 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
   }
 #undef PM
 #undef VM
-#undef YC
 }
 
 // ---------------------------------------------------------------------------
@@ -1306,50 +1302,184 @@ __global__ void k_tps_blend(int npts, const int32_t* __restrict__ slots,
   out[npts + i] = a1 / norm[i];
 }
 
-__global__ __launch_bounds__(256) void k_tps_interp(
-    int npts, const double* __restrict__ px, const double* __restrict__ py,
-    const int32_t* __restrict__ doms, const double* __restrict__ coef,
-    const double* __restrict__ norm, const double* __restrict__ slot_xy, int64_t nslots,
-    const int64_t* __restrict__ dom_off, const double* __restrict__ weights, int64_t wstride,
-    const uint8_t* __restrict__ skip, double* __restrict__ out) {
-  // one wavefront per new position, the lanes over the centers of a
-  // subdomain (read by slot: coordinates and weights contiguous); positions
-  // flagged in skip keep the value the caller put in out
-  const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-  const int lane = threadIdx.x & 63;
+// Evaluation by subdomain: the (point, slot) pairs e = 4 p + k of the
+// points to evaluate are sorted by their subdomain (count, exclusive scan,
+// fill: a counting sort), then one workgroup per subdomain holds its
+// centers and weights in LDS and evaluates its points, one thread per
+// point, the partial values going to partial[c][e]; the last kernel blends
+// the partials of each point in slot order. Workspace (int32 then double,
+// offsets in piv_tps_interpolate_workspace): cnt[nd + 1], cursor[nd],
+// list[4 npts], partial[2][4 npts].
+__global__ void k_tps_list_zero(int nd, int32_t* __restrict__ cnt,
+                                int32_t* __restrict__ cursor) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= nd) cnt[i] = 0;
+  if (i < nd) cursor[i] = 0;
+}
+
+// ctr[key] += (number of pending lanes with this key) for every key of the
+// wavefront's pending lanes, one atomic per distinct key; each pending lane
+// gets its own position in [old ctr[key], new ctr[key]) (-1 if not pending).
+// Called by every lane of the wavefront that reached it.
+__device__ __forceinline__ int wave_aggregated_add(int32_t* ctr, int key, bool pending) {
+  int pos = -1;
+  const int lane = __lane_id();
+  for (;;) {
+    const unsigned long long act = __ballot(pending);
+    if (act == 0ull) break;
+    const int leader = __ffsll(act) - 1;
+    const int lk = __shfl(key, leader);
+    const bool mine = pending && key == lk;
+    const unsigned long long same = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[lk], int(__popcll(same)));
+    base = __shfl(base, leader);
+    if (mine) {
+      pos = base + int(__popcll(same & ((1ull << lane) - 1ull)));
+      pending = false;
+    }
+  }
+  return pos;
+}
+
+// one thread per point, its slots in turn (neighbouring points share their
+// subdomains, so a wavefront issues few atomics)
+__global__ void k_tps_list_count(int npts, const int32_t* __restrict__ doms,
+                                 const uint8_t* __restrict__ skip, int32_t* __restrict__ cnt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p < npts && !(skip != nullptr && skip[p < npts ? p : 0]);
+  for (int k = 0; k < kMaxSlots; ++k) {
+    const int d = live ? doms[int64_t(kMaxSlots) * p + k] : -1;
+    wave_aggregated_add(cnt, d, d >= 0);
+  }
+}
+
+// exclusive prefix sum of cnt[0..nd) in place (cnt[nd] = total), one block
+__global__ __launch_bounds__(1024) void k_tps_list_scan(int nd, int32_t* __restrict__ cnt) {
+  __shared__ int tmp[32];
+  int carry = 0;
+  for (int b = 0; b < nd; b += 1024) {
+    const int i = b + int(threadIdx.x);
+    const int v = i < nd ? cnt[i] : 0;
+    int total;
+    const int inc = block_scan_1024(v, tmp, total);
+    if (i < nd) cnt[i] = carry + inc - v;
+    carry += total;
+  }
+  if (threadIdx.x == 0) cnt[nd] = carry;
+}
+
+__global__ void k_tps_list_fill(int npts, const int32_t* __restrict__ doms,
+                                const uint8_t* __restrict__ skip,
+                                const int32_t* __restrict__ off, int32_t* __restrict__ cursor,
+                                int32_t* __restrict__ list) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p < npts && !(skip != nullptr && skip[p < npts ? p : 0]);
+  for (int k = 0; k < kMaxSlots; ++k) {
+    const int64_t e = int64_t(kMaxSlots) * p + k;
+    const int d = live ? doms[e] : -1;
+    const int pos = wave_aggregated_add(cursor, d, d >= 0);
+    if (pos >= 0) list[off[d] + pos] = int32_t(e);
+  }
+}
+
+// sum_j w_j phi(|p - c_j|) plus the linear part, for the points listed
+// for subdomain blockIdx.x; centers and weights in LDS (x, y, w0, w1 per
+// center), one point per thread, the centers in order
+__global__ __launch_bounds__(256) void k_tps_eval_dom(
+    int npts, int nmax, const int32_t* __restrict__ off, const int32_t* __restrict__ list,
+    const double* __restrict__ px, const double* __restrict__ py,
+    const double* __restrict__ slot_xy, int64_t nslots, const int64_t* __restrict__ dom_off,
+    const double* __restrict__ weights, int64_t wstride, double* __restrict__ partial,
+    PhiLattice lat) {
+  const int d = blockIdx.x;
+  const int e0 = off[d], e1 = off[d + 1];
+  if (e0 == e1) return;
+  const int64_t s0 = dom_off[d];
+  const int n = int(dom_off[d + 1] - s0);
+  const int64_t stride = int64_t(kMaxSlots) * npts;
+  if (n > nmax) {
+    // more centers than the caller's bound (the LDS holds nmax): NaN values
+    for (int t = e0 + int(threadIdx.x); t < e1; t += blockDim.x) {
+      partial[list[t]] = __builtin_nan("");
+      partial[stride + list[t]] = __builtin_nan("");
+    }
+    return;
+  }
+  const double* W0 = weights + s0 + 3 * int64_t(d);
+  const double* W1 = W0 + wstride;
+  extern __shared__ double2 cen[];  // (x, y) then (w0, w1) of center j at 2 j, 2 j + 1
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    cen[2 * j] = make_double2(slot_xy[s0 + j], slot_xy[nslots + s0 + j]);
+    cen[2 * j + 1] = make_double2(W0[j], W1[j]);
+  }
+  const double a0 = W0[n], a1 = W0[n + 1], a2 = W0[n + 2];
+  const double b0 = W1[n], b1 = W1[n + 1], b2 = W1[n + 2];
+  __syncthreads();
+  for (int t = e0 + int(threadIdx.x); t < e1; t += blockDim.x) {
+    const int e = list[t];
+    const int p = e / kMaxSlots;
+    const double x = px[p], y = py[p];
+    double v0 = 0.0, v1 = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double2 c = cen[2 * j];
+      const double2 w = cen[2 * j + 1];
+      const double ph = tps_phi_at(x - c.x, y - c.y, lat);
+      v0 += w.x * ph;
+      v1 += w.y * ph;
+    }
+    v0 += a0 + a1 * y + a2 * x;
+    v1 += b0 + b1 * y + b2 * x;
+    partial[e] = v0;
+    partial[stride + e] = v1;
+  }
+}
+
+__global__ void k_tps_blend_partials(int npts, const int32_t* __restrict__ doms,
+                                     const double* __restrict__ coef,
+                                     const double* __restrict__ norm,
+                                     const uint8_t* __restrict__ skip,
+                                     const double* __restrict__ partial,
+                                     double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npts || (skip != nullptr && skip[p])) return;
-  const double x = px[p], y = py[p];
+  const int64_t stride = int64_t(kMaxSlots) * npts;
   double acc0 = 0.0, acc1 = 0.0;
   for (int k = 0; k < kMaxSlots; ++k) {
-    const int d = doms[kMaxSlots * p + k];
-    if (d < 0) break;
-    const double cf = coef[kMaxSlots * p + k];
-    const int64_t s0 = dom_off[d];
-    const int n = int(dom_off[d + 1] - s0);
-    const double* W0 = weights + s0 + 3 * int64_t(d);
-    const double* W1 = W0 + wstride;
-    const double* SX = slot_xy + s0;
-    const double* SY = slot_xy + nslots + s0;
-    double v0 = 0.0, v1 = 0.0;
-    for (int j = lane; j < n; j += 64) {
-      const double dy = y - SY[j], dx = x - SX[j];
-      const double ph = tps_phi(dy * dy + dx * dx);
-      v0 += W0[j] * ph;
-      v1 += W1[j] * ph;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      v0 += __shfl_xor(v0, off, 64);
-      v1 += __shfl_xor(v1, off, 64);
-    }
-    v0 += W0[n] + W0[n + 1] * y + W0[n + 2] * x;
-    v1 += W1[n] + W1[n + 1] * y + W1[n + 2] * x;
-    acc0 += cf * v0;
-    acc1 += cf * v1;
+    const int64_t e = int64_t(kMaxSlots) * p + k;
+    if (doms[e] < 0) break;
+    const double cf = coef[e];
+    acc0 += cf * partial[e];
+    acc1 += cf * partial[stride + e];
   }
-  if (lane == 0) {
-    out[p] = acc0 / norm[p];
-    out[npts + p] = acc1 / norm[p];
-  }
+  out[p] = acc0 / norm[p];
+  out[npts + p] = acc1 / norm[p];
+}
+
+// workspace layout of piv_tps_interpolate
+struct InterpWork {
+  int32_t* cnt;
+  int32_t* cursor;
+  int32_t* list;
+  double* partial;
+  size_t bytes;
+};
+
+InterpWork interp_work(void* base, int npts, int nd) {
+  InterpWork w;
+  char* b = static_cast<char*>(base);
+  size_t o = 0;
+  auto take = [&](size_t n) {
+    const size_t at = o;
+    o += (n + 255) & ~size_t(255);
+    return b + at;
+  };
+  w.cnt = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (size_t(nd) + 1)));
+  w.cursor = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * size_t(nd)));
+  w.list = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * size_t(kMaxSlots) * npts));
+  w.partial = reinterpret_cast<double*>(take(sizeof(double) * 2 * size_t(kMaxSlots) * npts));
+  w.bytes = o;
+  return w;
 }
 
 int tps_fail(int code, const std::string& msg) {
@@ -1372,7 +1502,8 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
                    const int32_t* slot_pt, const int64_t* mat_off, const double* smoothing,
                    double threshold, double* mats, double* refl, double* vwork, double* slot_xy,
                    double* smooth_slots, int64_t nslots, double* weights, int64_t wstride,
-                   double* stats, int32_t* status, void* stream) {
+                   double* stats, int32_t* status, const piv_tps_lattice* lattice,
+                   void* stream) {
   if (nd <= 0) return PIV_OK;
   if (n_max > kMaxN)
     return tps_fail(PIV_EUNSUPPORTED, "more than 2048 centers in a TPS subdomain");
@@ -1380,13 +1511,9 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
       !refl || !vwork || !slot_xy || !smooth_slots || !weights || !stats || !status)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  // Y = B V fused into the matrix build when its 12 per-wavefront arrays
-  // fit in LDS beside the 8 of the factorisation of P
-  const bool fused = size_t(20) * n_max * sizeof(double) <= kPrepFusedLds;
-  hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads),
-                     size_t(fused ? 20 : 8) * n_max * sizeof(double), s, xs, ys, dom_off,
-                     slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots, status,
-                     int(n_max), fused);
+  hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), size_t(8) * n_max * sizeof(double), s,
+                     xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy,
+                     nslots, status, int(n_max), make_lattice(lattice));
   // few subdomains: 8 wavefronts each; many: 4, two workgroups per CU
   if (nd <= 128)
     hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), 0, s, dom_off, mat_off, mats, vwork,
@@ -1462,17 +1589,50 @@ int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef, const 
   return tps_check("piv_tps_blend");
 }
 
+int64_t piv_tps_interpolate_workspace(int32_t npts, int32_t nd) {
+  if (npts < 0 || nd < 0) return -1;
+  return int64_t(interp_work(nullptr, npts, nd).bytes);
+}
+
 int piv_tps_interpolate(int32_t npts, const double* px, const double* py, const int32_t* doms,
                         const double* coef, const double* norm, const double* slot_xy,
-                        int64_t nslots, const int64_t* dom_off, const double* weights,
-                        int64_t wstride, const uint8_t* skip, double* out, void* stream) {
-  if (npts <= 0) return PIV_OK;
-  if (!px || !py || !doms || !coef || !norm || !slot_xy || !dom_off || !weights || !out)
+                        int64_t nslots, int32_t nd, int32_t n_max, const int64_t* dom_off,
+                        const double* weights, int64_t wstride, const uint8_t* skip,
+                        double* out, void* work, const piv_tps_lattice* lattice,
+                        void* stream) {
+  if (npts <= 0 || nd <= 0) return PIV_OK;
+  if (n_max <= 0 || n_max > kMaxN)
+    return tps_fail(PIV_EINVAL, "n_max of piv_tps_interpolate must be in [1, 2048]");
+  if (int64_t(npts) * kMaxSlots > INT32_MAX)
+    return tps_fail(PIV_EUNSUPPORTED, "too many TPS evaluation points");
+  if (!px || !py || !doms || !coef || !norm || !slot_xy || !dom_off || !weights || !out ||
+      !work)
     return tps_fail(PIV_EINVAL, "NULL argument");
-  hipLaunchKernelGGL(k_tps_interp, dim3((npts + 3) / 4), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), npts, px, py, doms, coef, norm, slot_xy,
-                     nslots, dom_off, weights, wstride, skip, out);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const InterpWork w = interp_work(work, npts, nd);
+  const unsigned ge = unsigned((npts + 255) / 256);
+  hipLaunchKernelGGL(k_tps_list_zero, dim3((nd + 256) / 256), dim3(256), 0, s, int(nd), w.cnt,
+                     w.cursor);
+  hipLaunchKernelGGL(k_tps_list_count, dim3(ge), dim3(256), 0, s, int(npts), doms, skip, w.cnt);
+  hipLaunchKernelGGL(k_tps_list_scan, dim3(1), dim3(1024), 0, s, int(nd), w.cnt);
+  hipLaunchKernelGGL(k_tps_list_fill, dim3(ge), dim3(256), 0, s, int(npts), doms, skip, w.cnt,
+                     w.cursor, w.list);
+  hipLaunchKernelGGL(k_tps_eval_dom, dim3(nd), dim3(256), size_t(2) * n_max * sizeof(double2),
+                     s, int(npts), int(n_max), w.cnt, w.list, px, py, slot_xy, nslots, dom_off, weights,
+                     wstride, w.partial, make_lattice(lattice));
+  hipLaunchKernelGGL(k_tps_blend_partials, dim3((npts + 255) / 256), dim3(256), 0, s, int(npts),
+                     doms, coef, norm, skip, w.partial, out);
   return tps_check("piv_tps_interpolate");
+}
+
+int piv_tps_phi_table(double step, int32_t a_max, double* table, void* stream) {
+  if (!(step > 0.0) || a_max < 0 || a_max > kMaxLatticeA)
+    return tps_fail(PIV_EINVAL, "lattice step must be positive and a_max in [0, 4095]");
+  if (!table) return tps_fail(PIV_EINVAL, "NULL argument");
+  const int w = a_max + 1;
+  hipLaunchKernelGGL(k_tps_phi_table, dim3((w * w + 255) / 256), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), step, int(a_max), table);
+  return tps_check("piv_tps_phi_table");
 }
 
 }  // extern "C"

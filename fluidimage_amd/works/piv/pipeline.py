@@ -59,7 +59,7 @@ class MultipassPipeline:
         onto work_next's grid, returning per pair the float64 device tensors
         (x, y). The TPS of all pairs runs as one batch; a pair whose TPS
         system is singular falls back to griddata, as in the reference."""
-        from ...calcul.tps import SingularSystemError, tps_device_batch
+        from ...calcul.tps import SingularSystemError, lattice_step, tps_device_batch
 
         mp = self.params.multipass
         use_tps = mp.use_tps is True or (mp.use_tps == "last" and last)
@@ -72,8 +72,14 @@ class MultipassPipeline:
             ev = self._event_pair(timers, f"tps_pass{work_next.index_pass}" + ("_final" if last else ""))
             if ev:
                 ev[0].record()
+            # the centers of a pass sit on its grid (deformed passes; the first
+            # pass) and the new positions are the next grid: the common step of
+            # both grids is the lattice of the kernel-function table (positions
+            # off that lattice are evaluated directly)
+            step = lattice_step(res.work.ixvecs, res.work.iyvecs, work_next.ixvecs,
+                                work_next.iyvecs)
             tps = tps_device_batch(fields, gx, gy, mp.subdom_size, mp.smoothing_coef,
-                                   mp.threshold_tps, 20, stream=stream)
+                                   mp.threshold_tps, 20, stream=stream, lattice=step)
             if ev:
                 ev[1].record()
             for k, t in enumerate(tps):

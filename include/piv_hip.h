@@ -186,6 +186,18 @@ int piv_fix(int32_t npairs, float* dx, float* dy, const float* correl_max, const
             float threshold, int32_t* flags, float* diff2, float* mean_x,
             float* mean_y, void* stream);
 
+/* Optional lattice of the TPS positions (NULL, or table NULL: none): a
+ * table written by piv_tps_phi_table with this step and a_max. Point
+ * differences that are exact multiples (a step, b step) with a, b <= a_max
+ * read the kernel function from the table, every other difference is
+ * evaluated directly; both give the same value. */
+typedef struct piv_tps_lattice {
+  const double* table;
+  double step;
+  int32_t a_max;
+  int32_t reserved;
+} piv_tps_lattice;
+
 /* ---- thin-plate-spline subdomains (FP64), replacing
  *      ThinPlateSplineSubdom.compute_tps_weights_subdom / interpolate
  *      (thin_plate_spline_subdom.py:128-159,224-280) and
@@ -203,7 +215,7 @@ int piv_fix(int32_t npairs, float* dx, float* dy, const float* correl_max, const
  * the device array of the (rescaled) smoothing coefficient of every
  * subdomain, so that the subdomains of several vector fields, each with its
  * own grid, go through one call. threshold < 0 disables the outlier
- * iterations. */
+ * iterations. lattice: see piv_tps_lattice (NULL: direct evaluation). */
 int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    const double* ys, const float* vx, const float* vy,
                    const int64_t* dom_off, const int32_t* slot_pt,
@@ -211,7 +223,7 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    double* mats, double* refl, double* vwork, double* slot_xy,
                    double* smooth_slots, int64_t nslots, double* weights,
                    int64_t wstride, double* stats, int32_t* status,
-                   void* stream);
+                   const piv_tps_lattice* lattice, void* stream);
 
 /* One vector field (one image pair) of a TPS batch: its subdomain grid
  * (nsx x nsy, limits at lim + lim_off: lmin_x[nsx], lmax_x[nsx],
@@ -278,17 +290,32 @@ int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef,
  * doms[4 p + k] (-1 terminated) of the point, the weighted sum of
  * w_j r^2 ln(r^2) / 2 over its centers plus the linear part, blended with
  * coef[4 p + k] and divided by norm[p] (thin_plate_spline_subdom.py:224-234);
- * slot_xy: the center coordinates by slot written by piv_tps_smooth.
- * out[2][npts]. skip (optional, one byte per position): positions with a
- * nonzero byte are not evaluated and keep what out holds (the caller puts
- * there the blended smoothed value of a position that is a center, which
- * the evaluation equals: [K P][w; a] = v - s w at the centers). */
+ * slot_xy: the center coordinates by slot written by piv_tps_smooth; nd
+ * subdomains (dom_off[nd + 1]), none with more than n_max (<= 2048)
+ * centers. out[2][npts]. skip (optional, one byte per position): positions
+ * with a nonzero byte are not evaluated and keep what out holds (the caller
+ * puts there the blended smoothed value of a position that is a center,
+ * which the evaluation equals: [K P][w; a] = v - s w at the centers).
+ * work: device workspace of piv_tps_interpolate_workspace(npts, nd) bytes
+ * (the points are sorted by subdomain there, one workgroup evaluating the
+ * points of one subdomain). lattice: see piv_tps_lattice. */
 int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
                         const int32_t* doms, const double* coef,
                         const double* norm, const double* slot_xy, int64_t nslots,
-                        const int64_t* dom_off, const double* weights,
-                        int64_t wstride, const uint8_t* skip, double* out,
+                        int32_t nd, int32_t n_max, const int64_t* dom_off,
+                        const double* weights, int64_t wstride, const uint8_t* skip,
+                        double* out, void* work, const piv_tps_lattice* lattice,
                         void* stream);
+int64_t piv_tps_interpolate_workspace(int32_t npts, int32_t nd);
+
+/* Table of the kernel function r^2 ln(r^2) / 2 on a lattice of step
+ * `step`: table[(a_max + 1) a + b] at r^2 = (a step)^2 + (b step)^2,
+ * 0 <= a, b <= a_max (at most 4095), (a_max + 1)^2 doubles. No counterpart
+ * in the reference, which evaluates the logarithm for every pair of points
+ * (thin_plate_spline.py:81-94); here it serves piv_tps_smooth and
+ * piv_tps_interpolate when the positions lie on a lattice (the PIV grids
+ * of the passes), with results equal to the direct evaluation. */
+int piv_tps_phi_table(double step, int32_t a_max, double* table, void* stream);
 
 /* ---- image deformation between passes (no counterpart in the reference,
  *      whose later passes shift the windows by the rounded predictor,

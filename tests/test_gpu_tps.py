@@ -78,3 +78,49 @@ def test_tps_reproduces_affine_field_on_gpu():
     assert np.abs(sx - u).max() < 1e-5
     assert np.abs(ix - (0.3 + 0.01 * new[1] - 0.02 * new[0])).max() < 1e-5
     assert np.abs(iy - (-1.0 + 0.005 * new[0])).max() < 1e-5
+
+
+def _grid_field(rng, step, shape, holes):
+    ny, nx = shape
+    ys, xs = np.mgrid[step // 2:ny - step // 2 + 1:step, step // 2:nx - step // 2 + 1:step]
+    ys = ys.ravel().astype(np.float64)
+    xs = xs.ravel().astype(np.float64)
+    u = (2.0 * np.sin(xs / 90.0) + 0.5 * np.cos(ys / 70.0)).astype(np.float32)
+    v = (1.5 * np.cos(xs / 60.0 + ys / 110.0)).astype(np.float32)
+    bad = rng.random(xs.size) < 0.03
+    u[bad] += rng.normal(0, 6, bad.sum()).astype(np.float32)
+    nan = rng.random(xs.size) < holes
+    u[nan] = np.nan
+    v[nan] = np.nan
+    return xs, ys, u, v
+
+
+@pytest.mark.parametrize("step_c, step_n", [(8, 8), (16, 8), (12, 6)])
+def test_tps_lattice_table_is_bit_identical(step_c, step_n):
+    """Centers on a PIV grid (with NaN holes) and new positions on a finer
+    grid: the kernel-function table on the grids' common lattice gives the
+    same bits as the direct evaluation."""
+    import torch
+
+    from fluidimage_amd.calcul.tps import lattice_step, tps_device
+
+    print("case: tps lattice", step_c, step_n, flush=True)
+    rng = np.random.default_rng(41)
+    xs, ys, u, v = _grid_field(rng, step_c, (600, 760), holes=0.02)
+    ny, nx = 600, 760
+    gy, gx = np.mgrid[step_n // 2:ny - step_n // 2 + 1:step_n,
+                      step_n // 2:nx - step_n // 2 + 1:step_n]
+    nxs, nys = gx.ravel().astype(np.float64), gy.ravel().astype(np.float64)
+    step = lattice_step(xs, ys, nxs, nys)
+    assert step == np.gcd(step_c // 2, step_n // 2)
+    dev = torch.device("cuda")
+
+    def t(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    outs = [tps_device(t(u), t(v), t(xs), t(ys), t(nxs), t(nys), 200, 2.0, 1.5,
+                       lattice=lat) for lat in (None, step)]
+    a, b = outs
+    assert torch.equal(a["smoothed"], b["smoothed"])
+    assert torch.equal(a["approx"], b["approx"])
+    assert torch.equal(a["stats"], b["stats"])
