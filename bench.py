@@ -172,6 +172,16 @@ def traffic_from_profile(kernel_tag):
     return int((float(f.group(1)) * FETCH_SCALE + float(w.group(1)) * WRITE_SCALE) * 1024)
 
 
+def tuning_in_use():
+    """Launch tunables in effect (from the committed tuning table, if any)."""
+    from fluidimage_amd import tuning
+
+    t = tuning.current()
+    t["table"] = os.path.basename(str(tuning._native.TUNING_PATH)) if os.path.exists(
+        tuning._native.TUNING_PATH) else None
+    return t
+
+
 def timed(step, steps, warmup):
     import torch
 
@@ -413,6 +423,7 @@ def main():
             "final_window_errors": nerr,
             "stage_ms_in_step": {k: round(v, 4) for k, v in sorted(avg.items())},
             "correlation_passes": per_pass,
+            "tuning": tuning_in_use(),
         },
         "roofline": {
             "bound": "hbm",

@@ -11,6 +11,10 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libpiv_hip.so"
+ARCH = "gfx950"
+# launch tuning table written by ``python -m fluidimage_amd.tuning --write``
+# on the target GPU; applied at load (PIV_HIP_TUNING: another file, or "none")
+TUNING_PATH = HERE / f"tuning_{ARCH}.json"
 
 PIV_OK = 0
 PIV_U8, PIV_U16, PIV_F32 = 0, 1, 2
@@ -134,6 +138,9 @@ SIGNATURES = {
          ctypes.c_int64, _P, ctypes.c_int64, _P, _P, _P, _P],
     ),
     "piv_tps_phi_table": (ctypes.c_int, [ctypes.c_double, _I, _P, _P]),
+    "piv_tune_candidates": (ctypes.c_int, [_I, _I, _I, _P, _I]),
+    "piv_tune_get": (ctypes.c_int, [_I, _I, _I]),
+    "piv_tune_set": (ctypes.c_int, [_I, _I, _I, _I]),
     "piv_tps_select": (ctypes.c_int, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "piv_tps_count": (ctypes.c_int, [_I, _P, _P, _P, _P, _P, _P, _P]),
     "piv_tps_fill": (ctypes.c_int, [_I, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -185,14 +192,31 @@ def load(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    _apply_tuning(lib)
     if path is None:
         _lib = lib
     return lib
 
 
-def check(status, what=""):
+def _apply_tuning(lib):
+    """Launch parameters of the tuning table (JSON data), if there is one."""
+    import json
+
+    choice = os.environ.get("PIV_HIP_TUNING", str(TUNING_PATH))
+    if choice == "none" or not Path(choice).exists():
+        return
+    table = json.loads(Path(choice).read_text())
+    for key, wpb in table.get("correlate_wpb", {}).items():
+        n0, n1 = (int(v) for v in key.split("x"))
+        check(lib.piv_tune_set(1, n0, n1, int(wpb)), f"tuning table {choice}: {key}", lib)
+    if "tps_chol_threads" in table:
+        check(lib.piv_tune_set(2, 0, 0, int(table["tps_chol_threads"])),
+              f"tuning table {choice}: tps_chol_threads", lib)
+
+
+def check(status, what="", lib=None):
     if status != PIV_OK:
-        msg = load().piv_last_error().decode()
+        msg = (lib if lib is not None else load()).piv_last_error().decode()
         raise RuntimeError(f"{what} failed (status {status}): {msg}")
 
 

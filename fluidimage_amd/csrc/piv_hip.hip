@@ -14,6 +14,7 @@
 // No global scratch: every intermediate stays in registers or LDS.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,6 +24,7 @@
 #include "fft_reg.hpp"
 #include "last_error.hpp"
 #include "piv_device.hpp"
+#include "tuning.hpp"
 
 using namespace fia;
 
@@ -593,14 +595,27 @@ __global__ __launch_bounds__(kBigThreads) void k_correlate_big(
 // -------------------------------------------------------------------------
 // shape dispatch
 // -------------------------------------------------------------------------
+// A shape's fused correlation kernel comes in up to kMaxVariants launch
+// variants that differ only in the windows per workgroup (WPB): every
+// window is computed by the same T lanes with the same instructions, so the
+// variant changes the speed, not the results. The variant in use is a
+// tunable (piv_tune_set), picked per shape by the autotuner.
+constexpr int kMaxVariants = 4;
+
+using CorrelateFn = void (*)(dim3, dim3, hipStream_t, const piv_image*, const piv_window*, int,
+                             int, float*, piv_results, PlanArgs);
+
+struct CorrVariant {
+  int wpb, threads;  // windows and threads per workgroup (wpb 0: no variant)
+  CorrelateFn correlate[3];
+};
+
 struct Launchers {
   int n0, n1;
-  void (*correlate[3])(dim3, dim3, hipStream_t, const piv_image*, const piv_window*, int, int,
-                       float*, piv_results, PlanArgs);
+  CorrVariant var[kMaxVariants];  // var[0] is the default
   void (*peaks)(dim3, dim3, hipStream_t, float*, const double*, int, int, piv_results, PlanArgs);
   void (*subpix)(dim3, dim3, hipStream_t, const float*, const int32_t*, const int32_t*, int,
                  float*, float*, int32_t*, PlanArgs);
-  int cwpb, cthreads;  // correlation: windows and threads per workgroup
   int pwpb, pthreads;  // peaks from maps: windows and threads per workgroup
 };
 
@@ -612,11 +627,42 @@ constexpr int default_wpb() {
   return w;
 }
 
-template <int N0, int N1, int DT>
+// the other WPB candidates of a tuned shape: a quarter, half and twice the
+// default, within 1024 threads and 80 KB of LDS per workgroup (two
+// workgroups per CU); 0 when there is none (or it repeats another)
+template <int N0, int N1>
+constexpr int wpb_half() {
+  return default_wpb<N0, N1>() > 1 ? default_wpb<N0, N1>() / 2 : 0;
+}
+
+template <int N0, int N1>
+constexpr int wpb_quarter() {
+  return default_wpb<N0, N1>() >= 4 ? default_wpb<N0, N1>() / 4 : 0;
+}
+
+template <int N0, int N1>
+constexpr int wpb_double() {
+  constexpr int w = 2 * default_wpb<N0, N1>();
+  return (Shape<N0, N1>::T * w <= 1024 && w * N0 * N1 * 8 <= 80 * 1024) ? w : 0;
+}
+
+template <int N0, int N1, int DT, int WPB>
 void launch_correlate(dim3 g, dim3 b, hipStream_t s, const piv_image* im, const piv_window* wi,
                       int n, int sp, float* maps, piv_results r, PlanArgs pa) {
-  hipLaunchKernelGGL((k_correlate<N0, N1, DT, default_wpb<N0, N1>()>), g, b, 0, s, im, wi, n, sp,
-                     maps, r, pa);
+  hipLaunchKernelGGL((k_correlate<N0, N1, DT, WPB>), g, b, 0, s, im, wi, n, sp, maps, r, pa);
+}
+
+template <int N0, int N1, int WPB>
+constexpr CorrVariant make_variant() {
+  if constexpr (WPB == 0) {
+    return CorrVariant{0, 0, {nullptr, nullptr, nullptr}};
+  } else {
+    return CorrVariant{WPB,
+                       Shape<N0, N1>::T * WPB,
+                       {&launch_correlate<N0, N1, PIV_U8, WPB>,
+                        &launch_correlate<N0, N1, PIV_U16, WPB>,
+                        &launch_correlate<N0, N1, PIV_F32, WPB>}};
+  }
 }
 
 template <int N, int DT>
@@ -653,16 +699,18 @@ void launch_subpix(dim3 g, dim3 b, hipStream_t s, const float* maps, const int32
   hipLaunchKernelGGL((k_subpix_at<N0, N1>), g, b, 0, s, maps, ix, iy, n, ox, oy, e, pa);
 }
 
-template <int N0, int N1>
+// Tuned: the window sizes of the multipass schedules (16, 24, 32, 48, 64)
+// get the extra WPB variants; the others only the default
+template <int N0, int N1, bool Tuned = false>
 constexpr Launchers make_launchers() {
   return Launchers{N0,
                    N1,
-                   {&launch_correlate<N0, N1, PIV_U8>, &launch_correlate<N0, N1, PIV_U16>,
-                    &launch_correlate<N0, N1, PIV_F32>},
+                   {make_variant<N0, N1, default_wpb<N0, N1>()>(),
+                    make_variant<N0, N1, Tuned ? wpb_half<N0, N1>() : 0>(),
+                    make_variant<N0, N1, Tuned ? wpb_double<N0, N1>() : 0>(),
+                    make_variant<N0, N1, Tuned ? wpb_quarter<N0, N1>() : 0>()},
                    &launch_peaks<N0, N1>,
                    &launch_subpix<N0, N1>,
-                   default_wpb<N0, N1>(),
-                   Shape<N0, N1>::T * default_wpb<N0, N1>(),
                    peaks_wpb<N0, N1>(),
                    search_lanes<N0, N1>() * peaks_wpb<N0, N1>()};
 }
@@ -671,27 +719,42 @@ template <int N>
 constexpr Launchers make_launchers_big() {
   return Launchers{N,
                    N,
-                   {&launch_correlate_big<N, PIV_U8>, &launch_correlate_big<N, PIV_U16>,
-                    &launch_correlate_big<N, PIV_F32>},
+                   {CorrVariant{1,
+                                kBigThreads,
+                                {&launch_correlate_big<N, PIV_U8>, &launch_correlate_big<N, PIV_U16>,
+                                 &launch_correlate_big<N, PIV_F32>}},
+                    make_variant<N, N, 0>(), make_variant<N, N, 0>(), make_variant<N, N, 0>()},
                    &launch_peaks<N, N>,
                    &launch_subpix<N, N>,
-                   1,
-                   kBigThreads,
                    peaks_wpb<N, N>(),
                    search_lanes<N, N>() * peaks_wpb<N, N>()};
 }
 
 const Launchers kShapes[] = {
-    make_launchers<12, 12>(), make_launchers<16, 16>(), make_launchers<20, 20>(),
-    make_launchers<24, 24>(), make_launchers<32, 32>(), make_launchers<40, 40>(),
-    make_launchers<48, 48>(), make_launchers<64, 64>(), make_launchers<64, 32>(),
-    make_launchers<32, 64>(), make_launchers_big<128>(),
+    make_launchers<12, 12>(),       make_launchers<16, 16, true>(), make_launchers<20, 20>(),
+    make_launchers<24, 24, true>(), make_launchers<32, 32, true>(), make_launchers<40, 40>(),
+    make_launchers<48, 48, true>(), make_launchers<64, 64, true>(), make_launchers<64, 32>(),
+    make_launchers<32, 64>(),       make_launchers_big<128>(),
 };
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+// variant in use per shape (index into Launchers::var)
+std::atomic<int> g_variant[kNumShapes];
+
+int shape_index(int n0, int n1) {
+  for (int i = 0; i < kNumShapes; ++i)
+    if (kShapes[i].n0 == n0 && kShapes[i].n1 == n1) return i;
+  return -1;
+}
 
 const Launchers* find_shape(int n0, int n1) {
-  for (const Launchers& l : kShapes)
-    if (l.n0 == n0 && l.n1 == n1) return &l;
-  return nullptr;
+  const int i = shape_index(n0, n1);
+  return i < 0 ? nullptr : &kShapes[i];
+}
+
+const CorrVariant& variant_of(const Launchers* l) {
+  const int v = g_variant[l - kShapes].load(std::memory_order_relaxed);
+  return l->var[v];
 }
 
 // -------------------------------------------------------------------------
@@ -956,6 +1019,59 @@ const char* piv_last_error(void) { return fia::last_error().c_str(); }
 
 int piv_shape_supported(int32_t n0, int32_t n1) { return find_shape(n0, n1) != nullptr; }
 
+int piv_tune_candidates(int32_t param, int32_t n0, int32_t n1, int32_t* values, int32_t cap) {
+  if (values == nullptr && cap > 0) return -PIV_EINVAL;
+  int count = 0;
+  auto put = [&](int v) {
+    if (count < cap) values[count] = v;
+    ++count;
+  };
+  if (param == PIV_TUNE_CORRELATE_WPB) {
+    const Launchers* l = find_shape(n0, n1);
+    if (l == nullptr) return -fail(PIV_EUNSUPPORTED, "window shape not built");
+    for (const CorrVariant& v : l->var)
+      if (v.wpb > 0) put(v.wpb);
+    return count;
+  }
+  if (param == PIV_TUNE_TPS_CHOL_THREADS) {
+    put(0);
+    put(256);
+    put(512);
+    return count;
+  }
+  return -fail(PIV_EINVAL, "unknown tuning parameter");
+}
+
+int piv_tune_get(int32_t param, int32_t n0, int32_t n1) {
+  if (param == PIV_TUNE_CORRELATE_WPB) {
+    const Launchers* l = find_shape(n0, n1);
+    if (l == nullptr) return -fail(PIV_EUNSUPPORTED, "window shape not built");
+    return variant_of(l).wpb;
+  }
+  if (param == PIV_TUNE_TPS_CHOL_THREADS) return fia::tps_chol_threads();
+  return -fail(PIV_EINVAL, "unknown tuning parameter");
+}
+
+int piv_tune_set(int32_t param, int32_t n0, int32_t n1, int32_t value) {
+  if (param == PIV_TUNE_CORRELATE_WPB) {
+    const int i = shape_index(n0, n1);
+    if (i < 0) return fail(PIV_EUNSUPPORTED, "window shape not built");
+    for (int k = 0; k < kMaxVariants; ++k)
+      if (kShapes[i].var[k].wpb > 0 && kShapes[i].var[k].wpb == value) {
+        g_variant[i].store(k, std::memory_order_relaxed);
+        return PIV_OK;
+      }
+    return fail(PIV_EINVAL, "windows per workgroup not among the candidates of this shape");
+  }
+  if (param == PIV_TUNE_TPS_CHOL_THREADS) {
+    if (value != 0 && value != 256 && value != 512)
+      return fail(PIV_EINVAL, "TPS Cholesky threads must be 0, 256 or 512");
+    fia::set_tps_chol_threads(value);
+    return PIV_OK;
+  }
+  return fail(PIV_EINVAL, "unknown tuning parameter");
+}
+
 int piv_plan_create(piv_plan** plan, int32_t n0, int32_t n1, double displacement_max,
                     const uint8_t* mask, const int32_t* box, int32_t subpix_method,
                     int32_t nsubpix, int32_t particle_radius, int32_t nb_peaks_to_search) {
@@ -1046,10 +1162,10 @@ int piv_correlate_batch(piv_plan* plan, int32_t dtype, const piv_image* images,
     return fail(PIV_EINVAL, "NULL image, window or result array");
   if (plan->pa.nb_peaks > 1 && (res->sec == nullptr || res->nsec == nullptr))
     return fail(PIV_EINVAL, "secondary peak arrays required when nb_peaks_to_search > 1");
-  const Launchers* l = plan->l;
-  dim3 grid((nwin + l->cwpb - 1) / l->cwpb), block(l->cthreads);
-  l->correlate[dtype](grid, block, static_cast<hipStream_t>(stream), images, windows, nwin,
-                      options, maps, *res, plan->pa);
+  const CorrVariant& v = variant_of(plan->l);
+  dim3 grid((nwin + v.wpb - 1) / v.wpb), block(v.threads);
+  v.correlate[dtype](grid, block, static_cast<hipStream_t>(stream), images, windows, nwin,
+                     options, maps, *res, plan->pa);
   return check_launch("piv_correlate_batch");
 }
 

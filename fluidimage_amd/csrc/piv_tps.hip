@@ -19,8 +19,12 @@
 #include <cmath>
 #include <string>
 
+#include <algorithm>
+#include <atomic>
+
 #include "../../include/piv_hip.h"
 #include "last_error.hpp"
+#include "tuning.hpp"
 
 namespace {
 
@@ -126,13 +130,14 @@ PhiLattice make_lattice(const piv_tps_lattice* l) {
   return out;
 }
 
+template <int NT = kThreads>
 __device__ double block_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   double s = 0.0;
-  for (int k = 0; k < kThreads / 64; ++k) s += red[k];
+  for (int k = 0; k < NT / 64; ++k) s += red[k];
   __syncthreads();
   return s;
 }
@@ -155,14 +160,15 @@ __device__ double block_max(double v, double* red) {
 // ---------------------------------------------------------------------------
 // 1. Householder QR of P, matrix B = K + sI, Y = B V, Z
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_tps_prep(
+template <int NT>
+__global__ __launch_bounds__(NT) void k_tps_prep(
     const double* __restrict__ xs, const double* __restrict__ ys,
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
     const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
     double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
     double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm,
     PhiLattice lat) {
-  constexpr int kWaves = kThreads / 64;
+  constexpr int kWaves = NT / 64;
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
@@ -176,11 +182,11 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
   double* cx = dyn + nm;
 #define PM(k, j) dyn[(2 + (k)) * nm + (j)]
 #define VM(k, j) dyn[(5 + (k)) * nm + (j)]
-  __shared__ double red[kThreads / 64];
+  __shared__ double red[NT / 64];
   __shared__ double sh[8];
   const int tid = threadIdx.x;
   const double smoothing = smoothing_d[d];
-  for (int j = tid; j < n; j += kThreads) {
+  for (int j = tid; j < n; j += NT) {
     const int p = slot_pt[s0 + j];
     cy[j] = ys[p];
     cx[j] = xs[p];
@@ -196,8 +202,8 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
   double tau[3];
   for (int k = 0; k < 3; ++k) {
     double part = 0.0;
-    for (int j = k + 1 + tid; j < n; j += kThreads) part += PM(k, j) * PM(k, j);
-    const double sig = block_sum(part, red);
+    for (int j = k + 1 + tid; j < n; j += NT) part += PM(k, j) * PM(k, j);
+    const double sig = block_sum<NT>(part, red);
     const double alpha = PM(k, k);
     double beta, t, scale;
     if (sig == 0.0) {
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
       scale = 1.0 / (alpha - beta);
     }
     tau[k] = t;
-    for (int j = tid; j < n; j += kThreads) {
+    for (int j = tid; j < n; j += NT) {
       double v = 0.0;
       if (j == k) v = 1.0;
       else if (j > k) v = PM(k, j) * scale;
@@ -221,22 +227,22 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
     // apply H_k to the remaining columns of P
     for (int c = k + 1; c < 3; ++c) {
       double pp = 0.0;
-      for (int j = k + tid; j < n; j += kThreads) pp += VM(k, j) * PM(c, j);
-      const double dot = block_sum(pp, red);
-      for (int j = k + tid; j < n; j += kThreads) PM(c, j) -= t * VM(k, j) * dot;
+      for (int j = k + tid; j < n; j += NT) pp += VM(k, j) * PM(c, j);
+      const double dot = block_sum<NT>(pp, red);
+      for (int j = k + tid; j < n; j += NT) PM(c, j) -= t * VM(k, j) * dot;
       __syncthreads();
     }
   }
   // T of the compact WY form (dlarft, forward, columnwise)
   double p01 = 0.0, p02 = 0.0, p12 = 0.0;
-  for (int j = tid; j < n; j += kThreads) {
+  for (int j = tid; j < n; j += NT) {
     p01 += VM(0, j) * VM(1, j);
     p02 += VM(0, j) * VM(2, j);
     p12 += VM(1, j) * VM(2, j);
   }
-  const double vtv01 = block_sum(p01, red);
-  const double vtv02 = block_sum(p02, red);
-  const double vtv12 = block_sum(p12, red);
+  const double vtv01 = block_sum<NT>(p01, red);
+  const double vtv02 = block_sum<NT>(p02, red);
+  const double vtv12 = block_sum<NT>(p12, red);
   const double T00 = tau[0], T11 = tau[1], T22 = tau[2];
   const double T01 = -tau[1] * T00 * vtv01;
   // T[0:2, 2] = -tau2 * T[0:2, 0:2] * V[:, 0:2]^T v2
@@ -296,7 +302,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
   }
   __syncthreads();
   // ---- M = V^T Y, Z = Y T - 1/2 V T^T M T ----
-  for (int j = tid; j < n; j += kThreads) {
+  for (int j = tid; j < n; j += NT) {
     const double y0 = PM(0, j), y1 = PM(1, j), y2 = PM(2, j);
     const double v0 = VM(0, j), v1 = VM(1, j), v2 = VM(2, j);
     m9[0] += v0 * y0; m9[1] += v0 * y1; m9[2] += v0 * y2;
@@ -311,7 +317,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
     VW[6 * j + 5] = y2;
   }
   double m[9];
-  for (int e = 0; e < 9; ++e) m[e] = block_sum(m9[e], red);
+  for (int e = 0; e < 9; ++e) m[e] = block_sum<NT>(m9[e], red);
   const double T[3][3] = {{T00, T01, T02}, {0.0, T11, T12}, {0.0, 0.0, T22}};
   double MT[3][3], TtMT[3][3];
   for (int a = 0; a < 3; ++a)
@@ -326,7 +332,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_prep(
       for (int k = 0; k < 3; ++k) sacc += T[k][a] * MT[k][b];
       TtMT[a][b] = sacc;
     }
-  for (int j = tid; j < n; j += kThreads) {
+  for (int j = tid; j < n; j += NT) {
     const double v[3] = {VW[6 * j + 0], VW[6 * j + 1], VW[6 * j + 2]};
     const double y[3] = {VW[6 * j + 3], VW[6 * j + 4], VW[6 * j + 5]};
     for (int b = 0; b < 3; ++b) {
@@ -1384,8 +1390,9 @@ __global__ void k_tps_list_fill(int npts, const int32_t* __restrict__ doms,
 }
 
 // sum_j w_j phi(|p - c_j|) plus the linear part, for the points listed
-// for subdomain blockIdx.x; centers and weights in LDS (x, y, w0, w1 per
-// center), one point per thread, the centers in order
+// for subdomain blockIdx.x (the list split over gridDim.y workgroups);
+// centers and weights in LDS (x, y, w0, w1 per center), one point per
+// thread, the centers in order
 __global__ __launch_bounds__(256) void k_tps_eval_dom(
     int npts, int nmax, const int32_t* __restrict__ off, const int32_t* __restrict__ list,
     const double* __restrict__ px, const double* __restrict__ py,
@@ -1394,13 +1401,15 @@ __global__ __launch_bounds__(256) void k_tps_eval_dom(
     PhiLattice lat) {
   const int d = blockIdx.x;
   const int e0 = off[d], e1 = off[d + 1];
-  if (e0 == e1) return;
+  const int first = e0 + int(blockIdx.y * blockDim.x);
+  const int step = int(gridDim.y * blockDim.x);
+  if (first >= e1) return;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
   const int64_t stride = int64_t(kMaxSlots) * npts;
   if (n > nmax) {
     // more centers than the caller's bound (the LDS holds nmax): NaN values
-    for (int t = e0 + int(threadIdx.x); t < e1; t += blockDim.x) {
+    for (int t = first + int(threadIdx.x); t < e1; t += step) {
       partial[list[t]] = __builtin_nan("");
       partial[stride + list[t]] = __builtin_nan("");
     }
@@ -1416,7 +1425,7 @@ __global__ __launch_bounds__(256) void k_tps_eval_dom(
   const double a0 = W0[n], a1 = W0[n + 1], a2 = W0[n + 2];
   const double b0 = W1[n], b1 = W1[n + 1], b2 = W1[n + 2];
   __syncthreads();
-  for (int t = e0 + int(threadIdx.x); t < e1; t += blockDim.x) {
+  for (int t = first + int(threadIdx.x); t < e1; t += step) {
     const int e = list[t];
     const int p = e / kMaxSlots;
     const double x = px[p], y = py[p];
@@ -1482,6 +1491,23 @@ InterpWork interp_work(void* base, int npts, int nd) {
   return w;
 }
 
+std::atomic<int> g_chol_threads{0};
+
+// compute units of the current device (cached per device)
+int cu_count() {
+  static std::atomic<int> cached[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int v = cached[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    cached[dev].store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
+
 int tps_fail(int code, const std::string& msg) {
   fia::last_error() = msg;
   return code;
@@ -1494,6 +1520,11 @@ int tps_check(const char* what) {
 }
 
 }  // namespace
+
+namespace fia {
+int tps_chol_threads() { return g_chol_threads.load(std::memory_order_relaxed); }
+void set_tps_chol_threads(int threads) { g_chol_threads.store(threads, std::memory_order_relaxed); }
+}  // namespace fia
 
 extern "C" {
 
@@ -1511,11 +1542,26 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
       !refl || !vwork || !slot_xy || !smooth_slots || !weights || !stats || !status)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_tps_prep, dim3(nd), dim3(kThreads), size_t(8) * n_max * sizeof(double), s,
-                     xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy,
-                     nslots, status, int(n_max), make_lattice(lattice));
-  // few subdomains: 8 wavefronts each; many: 4, two workgroups per CU
-  if (nd <= 128)
+  // up to one subdomain per CU: 16 wavefronts per matrix build, else 4
+  const size_t prep_lds = size_t(8) * n_max * sizeof(double);
+  if (nd <= cu_count())
+    hipLaunchKernelGGL(k_tps_prep<1024>, dim3(nd), dim3(1024), prep_lds, s, xs, ys, dom_off,
+                       slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots, status,
+                       int(n_max), make_lattice(lattice));
+  else
+    hipLaunchKernelGGL(k_tps_prep<kThreads>, dim3(nd), dim3(kThreads), prep_lds, s, xs, ys,
+                       dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots,
+                       status, int(n_max), make_lattice(lattice));
+  // 8 wavefronts per subdomain (one workgroup per CU) or 4 (two per CU).
+  // Up to two subdomains per CU the launch is one round either way if the
+  // width fits: 8 wavefronts up to one subdomain per CU, 4 up to two; beyond,
+  // the tuned width (default 4)
+  const int cus = cu_count();
+  int ct = fia::tps_chol_threads();
+  if (nd <= cus) ct = 512;
+  else if (nd <= 2 * cus) ct = 256;
+  else if (ct == 0) ct = 256;
+  if (ct == 512)
     hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), 0, s, dom_off, mat_off, mats, vwork,
                        status);
   else
@@ -1617,7 +1663,10 @@ int piv_tps_interpolate(int32_t npts, const double* px, const double* py, const 
   hipLaunchKernelGGL(k_tps_list_scan, dim3(1), dim3(1024), 0, s, int(nd), w.cnt);
   hipLaunchKernelGGL(k_tps_list_fill, dim3(ge), dim3(256), 0, s, int(npts), doms, skip, w.cnt,
                      w.cursor, w.list);
-  hipLaunchKernelGGL(k_tps_eval_dom, dim3(nd), dim3(256), size_t(2) * n_max * sizeof(double2),
+  // about 8 workgroups per CU in all: few subdomains split their lists
+  const int splits = std::min(64, std::max(1, (8 * cu_count() + int(nd) - 1) / int(nd)));
+  hipLaunchKernelGGL(k_tps_eval_dom, dim3(nd, splits), dim3(256),
+                     size_t(2) * n_max * sizeof(double2),
                      s, int(npts), int(n_max), w.cnt, w.list, px, py, slot_xy, nslots, dom_off, weights,
                      wstride, w.partial, make_lattice(lattice));
   hipLaunchKernelGGL(k_tps_blend_partials, dim3((npts + 255) / 256), dim3(256), 0, s, int(npts),

@@ -82,3 +82,44 @@ def test_plan_validation_without_gpu():
     assert lib.piv_plan_destroy(plan) == 0
     st = lib.piv_plan_create(ctypes.byref(plan), 32, 32, -1.0, None, None, 0, 1, 3, 9)
     assert st == 2
+
+
+def test_tuning_interface_without_gpu():
+    """Launch tunables: candidates per window shape, get/set, rejected values
+    (host-side state of the library; nothing is launched)."""
+    from fluidimage_amd import tuning
+
+    for n in tuning.DEFAULT_SHAPES:
+        cands = tuning.candidates(tuning.CORRELATE_WPB, n, n)
+        assert len(cands) >= 2 and len(set(cands)) == len(cands), (n, cands)
+        before = tuning.get(tuning.CORRELATE_WPB, n, n)
+        assert before in cands
+        for c in cands:
+            tuning.set_value(tuning.CORRELATE_WPB, n, n, c)
+            assert tuning.get(tuning.CORRELATE_WPB, n, n) == c
+        tuning.set_value(tuning.CORRELATE_WPB, n, n, before)
+    with pytest.raises(RuntimeError, match="not among the candidates"):
+        tuning.set_value(tuning.CORRELATE_WPB, 32, 32, 7)
+    with pytest.raises(ValueError, match="not built"):
+        tuning.candidates(tuning.CORRELATE_WPB, 33, 33)
+    assert tuning.candidates(tuning.TPS_CHOL_THREADS) == [0, 256, 512]
+    with pytest.raises(RuntimeError, match="0, 256 or 512"):
+        tuning.set_value(tuning.TPS_CHOL_THREADS, 0, 0, 128)
+    lib = _native.load()
+    assert lib.piv_tune_get(99, 0, 0) < 0
+
+
+def test_tuning_table_is_valid_for_the_library():
+    """The committed tuning table (if any) names only built candidates."""
+    import json
+
+    from fluidimage_amd import tuning
+
+    path = _native.TUNING_PATH
+    if not path.exists():
+        pytest.skip("no tuning table")
+    table = json.loads(path.read_text())
+    for key, wpb in table["correlate_wpb"].items():
+        n0, n1 = (int(v) for v in key.split("x"))
+        assert wpb in tuning.candidates(tuning.CORRELATE_WPB, n0, n1), key
+    assert table["tps_chol_threads"] in tuning.candidates(tuning.TPS_CHOL_THREADS)
