@@ -60,6 +60,48 @@ def subdomain_grid(count, x_min, x_max, y_min, y_max, subdom_size, smoothing_coe
     return nsx, nsy, smoothing, lim
 
 
+def subdomain_grids(counts, x_min, x_max, y_min, y_max, subdom_size, smoothing_coef,
+                    percent_buffer_area=20):
+    """``subdomain_grid`` for many fields at once (arrays of per-field
+    scalars): fields with the same subdomain counts share one vectorised
+    ``np.linspace`` (the same arithmetic, element by element, as one call per
+    field). Returns one (nsx, nsy, smoothing, lim) tuple per field, or the
+    SingularSystemError of a degenerate field."""
+    counts = np.asarray(counts, dtype=np.float64)
+    x_min, x_max, y_min, y_max = (np.asarray(a, dtype=np.float64)
+                                  for a in (x_min, x_max, y_min, y_max))
+    range_x = x_max - x_min
+    range_y = y_max - y_min
+    bad = (range_x == 0) | (range_y == 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        aspect_ratio = range_y / range_x
+        nb_subdom = counts / subdom_size
+        nsx = np.floor(np.sqrt(nb_subdom / aspect_ratio))
+        nsy = np.ceil(np.sqrt(nb_subdom * aspect_ratio))
+    out = [SingularSystemError("degenerate set of centers") if b else None for b in bad]
+    ok = ~bad
+    nsx = np.where(ok & (nsx > 0), nsx, 1).astype(np.int64)
+    nsy = np.where(ok & (nsy > 0), nsy, 1).astype(np.int64)
+    coef_buffer = percent_buffer_area / 100
+    keys = {}
+    for k in np.flatnonzero(ok):
+        keys.setdefault((int(nsx[k]), int(nsy[k])), []).append(k)
+    for (sx, sy), ks in keys.items():
+        ks = np.asarray(ks)
+        x_dom = np.linspace(x_min[ks], x_max[ks], sx + 1, axis=1)
+        y_dom = np.linspace(y_min[ks], y_max[ks], sy + 1, axis=1)
+        num_vectors_per_subdom = counts[ks] / (sx * sy)
+        smoothing = smoothing_coef * np.sqrt(
+            (x_dom[:, 1] - x_dom[:, 0]) * (y_dom[:, 1] - y_dom[:, 0]) / num_vectors_per_subdom)
+        bx = (coef_buffer * range_x[ks] / sx)[:, None]
+        by = (coef_buffer * range_y[ks] / sy)[:, None]
+        lim = np.concatenate([x_dom[:, :-1] - bx, x_dom[:, 1:] + bx, y_dom[:, :-1] - by,
+                              y_dom[:, 1:] + by], axis=1)
+        for j, k in enumerate(ks):
+            out[k] = (sx, sy, float(smoothing[j]), lim[j])
+    return out
+
+
 def _p(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
@@ -158,7 +200,8 @@ class _Field:
 
 
 def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshold,
-                     percent_buffer_area=20, stream=None, keep=False, lattice=None):
+                     percent_buffer_area=20, stream=None, keep=False, lattice=None,
+                     defer=False):
     """``tps_device`` for several vector fields at once (one image pair
     each): each field gets its own subdomain grid (thin_plate_spline_subdom.py:53-90),
     and the subdomains of all fields go through one launch of every
@@ -167,7 +210,9 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
 
     Returns one entry per field: the dict of ``tps_device`` or the
     ``SingularSystemError`` of that field alone ("Only nan." when it has no
-    finite vector). ``lattice``: as in ``tps_device``.
+    finite vector). ``lattice``: as in ``tps_device``. With ``defer`` the
+    call does not wait for the solves: the check for singular systems is
+    left to ``deferred_error`` on each returned dict (its "pending" entry).
     """
     import torch
 
@@ -200,27 +245,24 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
                                      _p(cy), _p(vx), _p(vy), _p(idx), _p(info), sp),
                   "piv_tps_select")
     info_h = info.cpu().numpy()
-    # subdomain grids (host scalars)
+    # subdomain grids (host scalars, all fields at once)
+    grids = subdomain_grids(info_h[:, 0], info_h[:, 1], info_h[:, 2], info_h[:, 3],
+                            info_h[:, 4], subdom_size, smoothing_coef, percent_buffer_area)
     active = []
-    for f in flist:
-        count, x_min, x_max, y_min, y_max, sorted_y = info_h[f.k].tolist()
-        f.count = int(count)
-        f.sorted_y = int(sorted_y)
+    for f, g in zip(flist, grids):
+        f.count = int(info_h[f.k, 0])
+        f.sorted_y = int(info_h[f.k, 5])
         if f.count == 0:
             f.error = SingularSystemError("Only nan.")
-            continue
-        try:
-            f.nsx, f.nsy, f.smoothing, f.lim_h = subdomain_grid(
-                f.count, x_min, x_max, y_min, y_max, subdom_size, smoothing_coef,
-                percent_buffer_area)
-        except SingularSystemError as e:
-            f.error = e
-            continue
-        f.nd = f.nsx * f.nsy
-        active.append(f)
+        elif isinstance(g, SingularSystemError):
+            f.error = g
+        else:
+            f.nsx, f.nsy, f.smoothing, f.lim_h = g
+            f.nd = f.nsx * f.nsy
+            active.append(f)
     own = (xs, ys, idx, n)
     results = _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep, own,
-                            lattice)
+                            lattice, defer, stream)
     out = []
     for f in flist:
         if f.error is not None:
@@ -263,8 +305,41 @@ def _at_centers(active, own, smoothed, new_x, new_y, dev):
     return approx.contiguous(), done.to(torch.uint8)
 
 
+class PendingStatus:
+    """Device status array of one batched TPS call whose check is deferred:
+    one value per subdomain (nonzero: singular system), then the flag of a
+    point in more than 4 subdomains."""
+
+    def __init__(self, status, nd, keep_alive=()):
+        self.status = status
+        self.nd = nd
+        self._keep_alive = keep_alive
+        self._host = None
+
+    def host(self):
+        if self._host is None:
+            self._host = self.status.cpu().numpy()
+            self._keep_alive = ()
+        return self._host
+
+
+def deferred_error(entry):
+    """Check of a field solved with ``defer=True``: None, or the
+    SingularSystemError the eager call would have returned for it; raises
+    NotImplementedError when a point lay in more than 4 subdomains."""
+    pending, base, nd = entry["pending"]
+    st = pending.host()
+    if st[pending.nd]:
+        raise NotImplementedError("a point lies in more than 4 TPS subdomains")
+    bad = st[base:base + nd]
+    if bad.any():
+        return SingularSystemError(
+            f"{int((bad != 0).sum())} subdomain system(s) not positive definite")
+    return None
+
+
 def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep, own,
-                  lattice=None):
+                  lattice=None, defer=False, stream=None):
     """Membership, FP64 solves, blend and evaluation for the fields whose
     grid is valid; fields with an empty subdomain get their error and the
     others are solved again without them."""
@@ -302,7 +377,7 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
                 "use a smaller params.multipass.subdom_size")
     if retry:
         return _solve_active(lib, [f for f in active if f.error is None], cx, cy, vx, vy,
-                             new_x, new_y, thr, dev, sp, keep, own, lattice)
+                             new_x, new_y, thr, dev, sp, keep, own, lattice, defer, stream)
     n_max = int(sizes.max())
     dom_off_h = np.concatenate([[0], np.cumsum(sizes)])
     mat_off_h = np.concatenate([[0], np.cumsum(sizes**2)])
@@ -311,9 +386,11 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     slot_pt = torch.empty(nslots, **i32)
     status = torch.zeros(nd + 1, **i32)
     ntot = int(cx.numel())
-    c_slot = torch.empty(ntot * MAX_SLOTS, **i32)
-    c_coef = torch.empty(ntot * MAX_SLOTS, **f64)
-    c_norm = torch.empty(ntot, **f64)
+    # slots of every center of the batch (-1: none, so that the centers
+    # beyond a field's count blend to nothing in the one blend launch below)
+    c_slot = torch.full((ntot * MAX_SLOTS,), -1, **i32)
+    c_coef = torch.zeros(ntot * MAX_SLOTS, **f64)
+    c_norm = torch.ones(ntot, **f64)
     m = int(new_x.numel())
     na = len(active)
     n_dom = torch.empty(na * m * MAX_SLOTS, **i32)
@@ -345,15 +422,8 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
                            _p(stats), _p(status), lat_p, sp),
         "piv_tps_smooth")
     smoothed = torch.empty((2, ntot), **f64)
-    for f in active:
-        b = f.base
-        sm = smoothed[:, b:b + f.count]
-        blend = torch.empty((2, f.count), **f64)
-        _native.check(lib.piv_tps_blend(f.count, _p(c_slot[MAX_SLOTS * b:]),
-                                        _p(c_coef[MAX_SLOTS * b:]), _p(c_norm[b:]),
-                                        _p(smooth_slots), nslots, _p(blend), sp),
-                      "piv_tps_blend")
-        sm.copy_(blend)
+    _native.check(lib.piv_tps_blend(ntot, _p(c_slot), _p(c_coef), _p(c_norm), _p(smooth_slots),
+                                    nslots, _p(smoothed), sp), "piv_tps_blend")
     px = new_x.repeat(na)
     py = new_y.repeat(na)
     approx, skip = _at_centers(active, own, smoothed, new_x, new_y, dev)
@@ -366,21 +436,38 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
                                           _p(dom_off), _p(weights), wstride, _p(skip),
                                           _p(approx), _p(iwork), lat_p, sp),
                   "piv_tps_interpolate")
-    st = status.cpu().numpy()  # synchronises: the workspaces are free to go after this
-    del lat_table, iwork
-    if st[nd]:
-        raise NotImplementedError("a point lies in more than 4 TPS subdomains")
+    if defer:
+        # no synchronisation here: the caller checks the status later
+        # (deferred_error); the workspaces stay referenced until then when
+        # the launches are not on the current stream (on the current stream
+        # the allocator orders any reuse of their memory after the launches)
+        keep_alive = ()
+        if stream is not None and stream != torch.cuda.current_stream():
+            keep_alive = (mats, refl, vwork, smooth_slots, weights, stats, slot_xy, dom_off,
+                          mat_off, slot_pt, c_slot, c_coef, c_norm, n_dom, n_coef, n_norm,
+                          lat_table, iwork, lim, ftab, dom_field, smoothing, counts, px, py)
+        pending = PendingStatus(status, nd, keep_alive)
+        st = None
+    else:
+        pending = None
+        st = status.cpu().numpy()  # synchronises: the workspaces are free to go after this
+        del lat_table, iwork
+        if st[nd]:
+            raise NotImplementedError("a point lies in more than 4 TPS subdomains")
     results = {}
     for j, f in enumerate(active):
-        bad = st[f.nd_base:f.nd_base + f.nd]
-        if bad.any():
-            f.error = SingularSystemError(
-                f"{int((bad != 0).sum())} subdomain system(s) not positive definite")
-            continue
+        if st is not None:
+            bad = st[f.nd_base:f.nd_base + f.nd]
+            if bad.any():
+                f.error = SingularSystemError(
+                    f"{int((bad != 0).sum())} subdomain system(s) not positive definite")
+                continue
         r = {"smoothed": smoothed[:, f.base:f.base + f.count],
              "approx": approx[:, m * j:m * (j + 1)],
              "nb_subdom": f.nd, "nsx": f.nsx, "nsy": f.nsy, "smoothing_coef": f.smoothing,
              "count": f.count, "stats": stats[f.nd_base:f.nd_base + f.nd]}
+        if pending is not None:
+            r["pending"] = (pending, f.nd_base, f.nd)
         if keep:
             # single-field use: subdomain offsets and center indices are the field's own
             r.update(weights=weights, dom_off=dom_off_h, slot_pt=slot_pt)

@@ -44,6 +44,11 @@ class MultipassPipeline:
         self.params = work_piv.params
         self._deformed = {}
         self._refine_problem = {}
+        # the TPS calls do not wait for their singular-system check: it runs
+        # once at the end of ``run``, and a pair that failed it is run again
+        # alone with the checks in place (griddata fallback and all)
+        self.defer_checks = True
+        self._pending = []
 
     def _event_pair(self, timers, key):
         import torch
@@ -79,7 +84,8 @@ class MultipassPipeline:
             step = lattice_step(res.work.ixvecs, res.work.iyvecs, work_next.ixvecs,
                                 work_next.iyvecs)
             tps = tps_device_batch(fields, gx, gy, mp.subdom_size, mp.smoothing_coef,
-                                   mp.threshold_tps, 20, stream=stream, lattice=step)
+                                   mp.threshold_tps, 20, stream=stream, lattice=step,
+                                   defer=self.defer_checks)
             if ev:
                 ev[1].record()
             for k, t in enumerate(tps):
@@ -89,6 +95,8 @@ class MultipassPipeline:
                     continue  # griddata below (which raises "Only nan." itself)
                 res.tps[k] = t
                 out[k] = (t["approx"][0], t["approx"][1])
+                if "pending" in t:
+                    self._pending.append((k, t))
         for k in range(res.npairs):
             if out[k] is None:
                 out[k] = self._griddata(work_next, res, k)
@@ -128,6 +136,7 @@ class MultipassPipeline:
         dev = imageset.device
         passes = []
         approx = [None] * npairs
+        self._pending = []
         deform = check_method(getattr(self.params.multipass, "image_deformation", None))
         if deform and self.params.piv0.nb_peaks_to_search > 1:
             raise NotImplementedError(
@@ -201,7 +210,49 @@ class MultipassPipeline:
                 steps = int(getattr(self.params.multipass, "refine_steps", 0) or 0)
                 if steps > 0:
                     self._refine(imageset, res, steps, timers)
+        failed = self._check_deferred()
+        if failed:
+            self._rerun_pairs(imageset, passes, failed, keep_maps, stream, final_interp)
         return passes
+
+    def _check_deferred(self):
+        """Pairs whose deferred TPS check found a singular system (one
+        device-to-host read per TPS call of the run)."""
+        from ...calcul.tps import deferred_error
+
+        failed = {k for k, t in self._pending if deferred_error(t) is not None}
+        self._pending = []
+        return failed
+
+    def _rerun_pairs(self, imageset, passes, pairs, keep_maps, stream, final_interp):
+        """Run the given pairs again one by one with the TPS checks in place
+        and put their results into the batch's passes (a pair's results do
+        not depend on the other pairs of its batch)."""
+        import torch
+
+        for k in sorted(pairs):
+            sub = MultipassPipeline(self.work)
+            sub.defer_checks = False
+            single = ImageSet([imageset.images[2 * k], imageset.images[2 * k + 1]],
+                              stream=stream)
+            one_passes = sub.run(single, 1, keep_maps=keep_maps, stream=stream,
+                                 final_interp=final_interp)
+            for res, one in zip(passes, one_passes):
+                n = res.nvec
+                for store, src in ((res.out, one.out), (res.win, one.win)):
+                    for key, t in store.items():
+                        s1 = src.get(key)
+                        if (key == "windows" or not torch.is_tensor(t) or s1 is None
+                                or t.dim() == 0 or t.shape[0] != res.npairs * n):
+                            continue
+                        t[k * n:(k + 1) * n].copy_(s1)
+                res.approx[k] = one.approx[0]
+                res.final[k] = one.final[0]
+                res.refined[k] = one.refined[0]
+                res.tps[k] = one.tps[0]
+                if getattr(one, "replaced", None):
+                    res.replaced = getattr(res, "replaced", {})
+                    res.replaced[k] = one.replaced.get(0, {})
 
     def _refine(self, imageset, res, steps, timers=None):
         """Differentiable refinement (refine.py) of the final fields of all

@@ -77,3 +77,38 @@ def test_batch_equals_single_pairs():
             assert np.array_equal(ps.deltays, pb.deltays, equal_nan=True)
             assert ps.errors == pb.errors
         assert np.array_equal(single.passes[-1].deltaxs_final, b.passes[-1].deltaxs_final)
+
+
+def test_deferred_failure_reruns_the_pair(monkeypatch):
+    """A pair whose deferred TPS check fails is run again alone and its
+    results replace the batch's; forced here for pair 1 (whose system is
+    fine), so the spliced results equal the plain batch's bit for bit."""
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+    from fluidimage_amd.works.piv.pipeline import MultipassPipeline
+
+    print("case: deferred rerun", flush=True)
+    im0, im1, p, _ = _setup(True, number=2, window=32, shape=(384, 384))
+    a0, a1 = config_pair(2, shape=(384, 384))
+    pairs = [(im0, im1), (a0, a1), (im1, im0)]
+    plain = WorkPIV(p).calcul_batch(pairs)
+    calls = []
+    orig = MultipassPipeline._check_deferred
+
+    def forced(self):
+        failed = orig(self)
+        if not calls:
+            calls.append(1)
+            failed = failed | {1}
+        return failed
+
+    monkeypatch.setattr(MultipassPipeline, "_check_deferred", forced)
+    spliced = WorkPIV(p).calcul_batch(pairs)
+    assert calls == [1]
+    for a, b in zip(plain, spliced):
+        for pa, pb in zip(a.passes, b.passes):
+            assert np.array_equal(pa.deltaxs, pb.deltaxs, equal_nan=True)
+            assert np.array_equal(pa.deltays, pb.deltays, equal_nan=True)
+            assert pa.errors == pb.errors
+        assert np.array_equal(a.passes[-1].deltaxs_final, b.passes[-1].deltaxs_final)
+        assert np.array_equal(a.passes[-1].deltays_smooth, b.passes[-1].deltays_smooth)

@@ -124,3 +124,41 @@ def test_tps_lattice_table_is_bit_identical(step_c, step_n):
     assert torch.equal(a["smoothed"], b["smoothed"])
     assert torch.equal(a["approx"], b["approx"])
     assert torch.equal(a["stats"], b["stats"])
+
+
+def test_tps_deferred_check_matches_eager():
+    """defer=True leaves the singular-system check to deferred_error, which
+    reports what the eager call reports, field by field."""
+    import torch
+
+    from fluidimage_amd.calcul.tps import (SingularSystemError, deferred_error, lattice_step,
+                                           tps_device_batch)
+
+    print("case: tps deferred check", flush=True)
+    rng = np.random.default_rng(43)
+    fields = []
+    dev = torch.device("cuda")
+
+    def t(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    for bad in (False, True, False):
+        xs, ys, u, v = _grid_field(rng, 8, (640, 720), holes=0.0)
+        if bad:
+            # the top-left subdomain keeps two centers only (fewer than the
+            # three terms of P = [1, y, x]): a singular system
+            corner = (xs < 170) & (ys < 170)
+            keep = corner & (ys == 4) & (xs <= 12)
+            u[corner & ~keep] = np.nan
+            v[corner & ~keep] = np.nan
+        fields.append((t(u), t(v), t(xs), t(ys)))
+    gx, gy = fields[0][2], fields[0][3]
+    step = lattice_step(fields[0][2].cpu().numpy(), fields[0][3].cpu().numpy())
+    eager = tps_device_batch(fields, gx, gy, 200, 2.0, 1.5, lattice=step)
+    deferred = tps_device_batch(fields, gx, gy, 200, 2.0, 1.5, lattice=step, defer=True)
+    assert isinstance(eager[1], SingularSystemError)
+    assert isinstance(deferred_error(deferred[1]), SingularSystemError)
+    for k in (0, 2):
+        assert isinstance(eager[k], dict) and deferred_error(deferred[k]) is None
+        assert torch.equal(eager[k]["approx"], deferred[k]["approx"])
+        assert torch.equal(eager[k]["smoothed"], deferred[k]["smoothed"])
