@@ -87,14 +87,29 @@ __device__ double bicubic(const piv_image& im, double m, double py, double px) {
   return acc;
 }
 
-// deformed images of pair blockIdx.z, row blockIdx.y: the predictor at a
-// pixel does not depend on the window holding it, so the deformed crop of a
-// window centred on a grid node is the crop of the deformed image there
+// Workgroups are dispatched to the 8 XCDs of the chip round-robin
+// (workgroup b to XCD b % 8); renumbered so that each XCD deforms one
+// contiguous band of rows, the source rows that neighbouring output rows
+// share (the 4 taps plus the displacement) stay under one L2. A bijection
+// of [0, nb).
+__device__ __forceinline__ int xcd_band(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int x = b % kXcd, j = b / kXcd;
+  const int q = nb / kXcd, r = nb % kXcd;
+  return x * q + min(x, r) + j;
+}
+
+// deformed images of one pair and row (logical workgroup pair * height + y):
+// the predictor at a pixel does not depend on the window holding it, so the
+// deformed crop of a window centred on a grid node is the crop of the
+// deformed image there
 template <int DT>
 __global__ __launch_bounds__(kDeformThreads) void k_deform_images(
     const piv_image* __restrict__ images, piv_deform_grid g, const double* __restrict__ u,
-    const double* __restrict__ v, int nvec, float* __restrict__ out, int64_t out_stride) {
-  const int pair = blockIdx.z, y = blockIdx.y;
+    const double* __restrict__ v, int nvec, float* __restrict__ out, int64_t out_stride,
+    int height) {
+  const int lb = xcd_band(int(blockIdx.x), int(gridDim.x));
+  const int pair = lb / height, y = lb % height;
   const piv_image im0 = images[2 * pair], im1 = images[2 * pair + 1];
   const double m0 = im0.minval ? double(*im0.minval) : 0.0;
   const double m1 = im1.minval ? double(*im1.minval) : 0.0;
@@ -152,28 +167,28 @@ int piv_deform_images(int32_t dtype, const piv_image* images, int32_t npairs,
   if (!images || !grid || !u || !v || !out) return deform_fail(PIV_EINVAL, "NULL argument");
   if (npairs <= 0 || height <= 0 || width <= 0 || grid->ny <= 0 || grid->nx <= 0)
     return deform_fail(PIV_EINVAL, "empty deformation problem");
-  if (npairs > 65535 || height > 65535)
+  if (int64_t(npairs) * height > INT32_MAX)
     return deform_fail(PIV_EINVAL, "too many pairs or rows for one launch");
   if (!grid->iyvecs || !grid->ixvecs || !grid->cell_y || !grid->t_y || !grid->cell_x ||
       !grid->t_x)
     return deform_fail(PIV_EINVAL, "NULL grid table");
   const int nvec = grid->ny * grid->nx;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const dim3 gr(1, static_cast<unsigned>(height), static_cast<unsigned>(npairs));
+  const dim3 gr(static_cast<unsigned>(int64_t(height) * npairs));
   const dim3 bl(kDeformThreads);
   const int64_t stride = int64_t(height) * width;
   switch (dtype) {
     case PIV_U8:
       hipLaunchKernelGGL(k_deform_images<PIV_U8>, gr, bl, 0, s, images, *grid, u, v, nvec, out,
-                         stride);
+                         stride, int(height));
       break;
     case PIV_U16:
       hipLaunchKernelGGL(k_deform_images<PIV_U16>, gr, bl, 0, s, images, *grid, u, v, nvec, out,
-                         stride);
+                         stride, int(height));
       break;
     case PIV_F32:
       hipLaunchKernelGGL(k_deform_images<PIV_F32>, gr, bl, 0, s, images, *grid, u, v, nvec, out,
-                         stride);
+                         stride, int(height));
       break;
     default:
       return deform_fail(PIV_EINVAL, "unknown image dtype");

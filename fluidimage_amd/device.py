@@ -93,7 +93,12 @@ class ImageSet:
             host[i]["width"] = t.shape[1]
             host[i]["pitch"] = t.stride(0)
             host[i]["minval"] = self.mins.data_ptr() + 4 * i if subtract_min else 0
-        self.table = torch.from_numpy(host.view(np.uint8).copy()).to(device)
+        # pinned staging and an asynchronous copy on the launch stream: a
+        # pageable copy would make the host wait for all queued GPU work
+        staged = torch.empty(host.nbytes, dtype=torch.uint8, pin_memory=True)
+        staged.numpy()[:] = host.view(np.uint8)
+        with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream()):
+            self.table = staged.to(device, non_blocking=True)
         if subtract_min:
             lib = _native.load()
             _native.check(

@@ -70,7 +70,15 @@ class DeformedPass:
         work = self.work
         H, W = work.imshape0
         dev = imageset.device
-        out = torch.empty((2 * npairs, H, W), dtype=torch.float32, device=dev)
+        # the deformed images live until the pass's correlation has read
+        # them: the buffer and its image table are kept for the next call
+        key = (2 * npairs, H, W, dev)
+        if getattr(self, "_out_key", None) != key:
+            self._out = torch.empty((2 * npairs, H, W), dtype=torch.float32, device=dev)
+            self._out_set = ImageSet([self._out[i] for i in range(2 * npairs)],
+                                     subtract_min=False, stream=stream)
+            self._out_key = key
+        out = self._out
         lib = _native.load()
         _native.check(
             lib.piv_deform_images(int(imageset.dtype), ctypes.c_void_p(imageset.table.data_ptr()),
@@ -78,7 +86,7 @@ class DeformedPass:
                                   ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(v.data_ptr()),
                                   ctypes.c_void_p(out.data_ptr()), _native.stream_ptr(stream)),
             "piv_deform_images")
-        return ImageSet([out[i] for i in range(2 * npairs)], subtract_min=False)
+        return self._out_set
 
 
 def add_predictor(dx, dy, u, v, stream=None):
