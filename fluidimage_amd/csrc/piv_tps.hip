@@ -98,17 +98,30 @@ struct PhiLattice {
   const double* table;
   double step, inv_step;
   int a_max;
+  int lds_words;  // nonzero: the kernels copy the table (this many doubles) into LDS
 };
 
-__device__ __forceinline__ double tps_phi_at(double dx, double dy, const PhiLattice& lat) {
-  if (lat.table != nullptr) {
+constexpr int kLatticeLdsBytes = 16 * 1024;  // largest table staged in LDS
+
+// `table`: the lattice table or its LDS copy
+__device__ __forceinline__ double tps_phi_at(double dx, double dy, const PhiLattice& lat,
+                                            const double* table) {
+  if (table != nullptr) {
     const double ax = fabs(dx), ay = fabs(dy);
     const double fa = rint(ax * lat.inv_step), fb = rint(ay * lat.inv_step);
     if (fa <= double(lat.a_max) && fb <= double(lat.a_max) && fa * lat.step == ax &&
         fb * lat.step == ay)
-      return lat.table[int(fa) * (lat.a_max + 1) + int(fb)];
+      return table[int(fa) * (lat.a_max + 1) + int(fb)];
   }
   return tps_phi(tps_r2(dx, dy));
+}
+
+// the lattice table in LDS at `lds` when it was sized for it (else the
+// global table); call before a barrier of the whole workgroup
+__device__ __forceinline__ const double* stage_lattice(const PhiLattice& lat, double* lds) {
+  if (lat.table == nullptr || lat.lds_words == 0) return lat.table;
+  for (int e = int(threadIdx.x); e < lat.lds_words; e += int(blockDim.x)) lds[e] = lat.table[e];
+  return lds;
 }
 
 __global__ void k_tps_phi_table(double step, int a_max, double* __restrict__ table) {
@@ -120,12 +133,14 @@ __global__ void k_tps_phi_table(double step, int a_max, double* __restrict__ tab
 }
 
 PhiLattice make_lattice(const piv_tps_lattice* l) {
-  PhiLattice out{nullptr, 0.0, 0.0, 0};
+  PhiLattice out{nullptr, 0.0, 0.0, 0, 0};
   if (l != nullptr && l->table != nullptr && l->step > 0.0 && l->a_max >= 0) {
     out.table = l->table;
     out.step = l->step;
     out.inv_step = 1.0 / l->step;
     out.a_max = l->a_max;
+    const int words = (l->a_max + 1) * (l->a_max + 1);
+    if (size_t(words) * sizeof(double) <= size_t(kLatticeLdsBytes)) out.lds_words = words;
   }
   return out;
 }
@@ -176,7 +191,8 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   double* RF = refl + 16 * d;
   double* VW = vwork + 6 * s0;
   // dynamic LDS, nm >= n doubles per array: cy, cx, P (3 columns, then
-  // their reflected copy, then Y = B V), V (3 Householder vectors)
+  // their reflected copy, then Y = B V), V (3 Householder vectors); then
+  // the lattice table when it is staged
   extern __shared__ double dyn[];
   double* cy = dyn;
   double* cx = dyn + nm;
@@ -186,6 +202,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   __shared__ double sh[8];
   const int tid = threadIdx.x;
   const double smoothing = smoothing_d[d];
+  const double* ltab = stage_lattice(lat, dyn + 8 * nm);
   for (int j = tid; j < n; j += NT) {
     const int p = slot_pt[s0 + j];
     cy[j] = ys[p];
@@ -282,7 +299,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
     double* Bi = B + int64_t(i) * n;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
     for (int j = lane; j < n; j += 64) {
-      double b = tps_phi_at(cx[j] - xi, cy[j] - yi, lat);
+      double b = tps_phi_at(cx[j] - xi, cy[j] - yi, lat, ltab);
       if (i == j) b += smoothing;
       if (j <= i) Bi[j] = b;
       r0 += b * VM(0, j);
@@ -1418,6 +1435,7 @@ __global__ __launch_bounds__(256) void k_tps_eval_dom(
   const double* W0 = weights + s0 + 3 * int64_t(d);
   const double* W1 = W0 + wstride;
   extern __shared__ double2 cen[];  // (x, y) then (w0, w1) of center j at 2 j, 2 j + 1
+  const double* ltab = stage_lattice(lat, reinterpret_cast<double*>(cen + 2 * nmax));
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     cen[2 * j] = make_double2(slot_xy[s0 + j], slot_xy[nslots + s0 + j]);
     cen[2 * j + 1] = make_double2(W0[j], W1[j]);
@@ -1433,7 +1451,7 @@ __global__ __launch_bounds__(256) void k_tps_eval_dom(
     for (int j = 0; j < n; ++j) {
       const double2 c = cen[2 * j];
       const double2 w = cen[2 * j + 1];
-      const double ph = tps_phi_at(x - c.x, y - c.y, lat);
+      const double ph = tps_phi_at(x - c.x, y - c.y, lat, ltab);
       v0 += w.x * ph;
       v1 += w.y * ph;
     }
@@ -1543,15 +1561,16 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
   // up to one subdomain per CU: 16 wavefronts per matrix build, else 4
-  const size_t prep_lds = size_t(8) * n_max * sizeof(double);
+  const PhiLattice lat = make_lattice(lattice);
+  const size_t prep_lds = size_t(8) * n_max * sizeof(double) + size_t(lat.lds_words) * sizeof(double);
   if (nd <= cu_count())
     hipLaunchKernelGGL(k_tps_prep<1024>, dim3(nd), dim3(1024), prep_lds, s, xs, ys, dom_off,
                        slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots, status,
-                       int(n_max), make_lattice(lattice));
+                       int(n_max), lat);
   else
     hipLaunchKernelGGL(k_tps_prep<kThreads>, dim3(nd), dim3(kThreads), prep_lds, s, xs, ys,
                        dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots,
-                       status, int(n_max), make_lattice(lattice));
+                       status, int(n_max), lat);
   // 8 wavefronts per subdomain (one workgroup per CU) or 4 (two per CU).
   // Up to two subdomains per CU the launch is one round either way if the
   // width fits: 8 wavefronts up to one subdomain per CU, 4 up to two; beyond,
@@ -1665,10 +1684,11 @@ int piv_tps_interpolate(int32_t npts, const double* px, const double* py, const 
                      w.cursor, w.list);
   // about 8 workgroups per CU in all: few subdomains split their lists
   const int splits = std::min(64, std::max(1, (8 * cu_count() + int(nd) - 1) / int(nd)));
+  const PhiLattice lat = make_lattice(lattice);
   hipLaunchKernelGGL(k_tps_eval_dom, dim3(nd, splits), dim3(256),
-                     size_t(2) * n_max * sizeof(double2),
+                     size_t(2) * n_max * sizeof(double2) + size_t(lat.lds_words) * sizeof(double),
                      s, int(npts), int(n_max), w.cnt, w.list, px, py, slot_xy, nslots, dom_off, weights,
-                     wstride, w.partial, make_lattice(lattice));
+                     wstride, w.partial, lat);
   hipLaunchKernelGGL(k_tps_blend_partials, dim3((npts + 255) / 256), dim3(256), 0, s, int(npts),
                      doms, coef, norm, skip, w.partial, out);
   return tps_check("piv_tps_interpolate");
