@@ -22,6 +22,7 @@ PIV_SUBPIX = {"2d_gaussian2": 0, "centroid": 1, "no_subpix": 2}
 PIV_WINFLAG_INVALID = 1
 PIV_OPT_SUBPIX = 1
 PIV_OPT_MAP_SQUARE = 2
+PIV_DIRECT_PYTHRAN, PIV_DIRECT_PYCUDA = 0, 1
 
 WIN_OK = 0
 WIN_PEAK_BOUNDARY = 1
@@ -128,6 +129,11 @@ SIGNATURES = {
     ),
     "piv_peaks_from_maps": (ctypes.c_int, [_P, _P, _P, _I, _I, ctypes.POINTER(PivResults), _P]),
     "piv_subpix_at": (ctypes.c_int, [_P, _P, _P, _P, _I, _P, _P, _P, _P]),
+    "piv_correlate_direct": (ctypes.c_int, [_I, _P, _I, _I, _P, _I, _I, _I, _I, _P, _P, _P]),
+    "piv_peaks_any": (
+        ctypes.c_int, [_I, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I,
+                       ctypes.POINTER(PivResults), _P]),
+    "piv_subpix_any": (ctypes.c_int, [_I, _P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _P]),
     "piv_make_windows": (
         ctypes.c_int,
         [_I, _I, _I, _P, _P, _P, _P, _I, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P],

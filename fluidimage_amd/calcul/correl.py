@@ -349,12 +349,212 @@ class CorrelHIP:
 # alias under the plural name some callers use for this entry point
 CorrelHIP.compute_displacements_from_indices = CorrelHIP.compute_displacements_from_correl
 
+
+class CorrelDirectHIP:
+    """Direct (spatial) correlation on gfx950 behind the interface of the
+    reference's CorrelPythran / CorrelPyCuda (src/fluidimage/calcul/correl.py:
+    284-451): maps of (2 dmax + 1)^2 with the zero displacement at (dmax,
+    dmax), CorrelBase's peak search without mask (the second-chance square is
+    written into the caller's map) and sub-pixel fit. The windows may differ
+    in shape (im1 no larger than im0). Map, norm, search and sub-pixel fit run
+    on the GPU (piv_correlate_direct, piv_peaks_any, piv_subpix_any).
+
+    The reference's ``_finalize_init`` only binds its local
+    ``displacement_max`` when the parameter is None (correl.py:388-408), so an
+    explicit value raises there; here an explicit value is used as given."""
+
+    _tag = "direct"
+    _kind = "pythran"
+
+    def __init__(self, im0_shape, im1_shape=None, method_subpix="centroid", nsubpix=1,
+                 displacement_max=None, particle_radius=3, nb_peaks_to_search=1, mode=None):
+        im0_shape = tuple(int(n) for n in im0_shape)
+        im1_shape = im0_shape if im1_shape is None else tuple(int(n) for n in im1_shape)
+        self.mode = mode
+        self.subpix = SubPixParams(method=method_subpix, nsubpix=nsubpix)
+        self.im0_shape, self.im1_shape = im0_shape, im1_shape
+        dmax = parse_displacement_max(displacement_max, im0_shape)
+        if dmax is None:
+            dmax = self._default_displacement_max()
+        if dmax <= 0:
+            raise ValueError("displacement_max <= 0 : problem with images shapes?")
+        self.displacement_max = int(dmax)
+        self.ix0 = self.iy0 = self.displacement_max
+        self.particle_radius = particle_radius
+        self.nb_peaks_to_search = nb_peaks_to_search
+        if nb_peaks_to_search < 1:
+            raise ValueError("nb_peaks_to_search has to be >= 1")
+        self.start_stop_for_search0 = [0, None]
+        self.start_stop_for_search1 = [0, None]
+        self._lib = _native.load()
+        self._norm_code = (_native.PIV_DIRECT_PYCUDA if self._kind == "pycuda"
+                           else _native.PIV_DIRECT_PYTHRAN)
+
+    def _default_displacement_max(self):
+        """correl.py:388-408 (pythran) / 418-442 (pycuda)."""
+        if self.im0_shape == self.im1_shape:
+            return min(self.im0_shape) // 2 - (0 if self._kind == "pycuda" else 1)
+        return min(self.im0_shape[0] - self.im1_shape[0],
+                   self.im0_shape[1] - self.im1_shape[1]) // 2 - 1
+
+    @property
+    def map_shape(self):
+        n = 2 * self.displacement_max + 1
+        return (n, n)
+
+    def compute_displacement_from_indices(self, ix, iy):
+        return self.ix0 - ix, self.iy0 - iy
+
+    def compute_indices_from_displacement(self, dx, dy):
+        return self.ix0 - dx, self.iy0 - dy
+
+    def get_indices_no_displacement(self):
+        return self.iy0, self.ix0
+
+    _alloc_results = CorrelHIP._alloc_results
+    results_struct = staticmethod(CorrelHIP.results_struct)
+
+    # ------------------------------------------------------------ batched API
+    def correlate_stack(self, win0, win1, stream=None):
+        """Maps and norms of a stack of window pairs (device float32 tensors
+        (nwin, n0y, n0x) and (nwin, n1y, n1x)): returns (maps (nwin, 2 dmax +
+        1, 2 dmax + 1) float32, norms (nwin,) float64) on the device."""
+        torch = _torch()
+        nwin = int(win0.shape[0])
+        if tuple(win0.shape[1:]) != self.im0_shape or tuple(win1.shape[1:]) != self.im1_shape \
+                or int(win1.shape[0]) != nwin:
+            raise ValueError("window stacks do not have the shapes of the correlation")
+        win0 = win0.to(torch.float32).contiguous()
+        win1 = win1.to(torch.float32).contiguous()
+        maps = torch.empty((nwin,) + self.map_shape, dtype=torch.float32, device=win0.device)
+        norms = torch.empty(nwin, dtype=torch.float64, device=win0.device)
+        (n0y, n0x), (n1y, n1x) = self.im0_shape, self.im1_shape
+        _native.check(
+            self._lib.piv_correlate_direct(
+                nwin, ctypes.c_void_p(win0.data_ptr()), n0y, n0x,
+                ctypes.c_void_p(win1.data_ptr()), n1y, n1x, self.displacement_max,
+                self._norm_code, ctypes.c_void_p(maps.data_ptr()),
+                ctypes.c_void_p(norms.data_ptr()), _native.stream_ptr(stream)),
+            "piv_correlate_direct")
+        return maps, norms
+
+    def peaks_stack(self, maps, norms, apply_subpix=True, stream=None):
+        """Peak search (+ sub-pixel fit) of a stack of maps on the device
+        (the second-chance squares are written into ``maps``); returns the
+        dict of result tensors (dx, dy, correl_max, err, ix, iy ...)."""
+        nwin = int(maps.shape[0])
+        out = self._alloc_results(nwin, maps.device)
+        res = self.results_struct(out)
+        n0, n1 = self.map_shape
+        _native.check(
+            self._lib.piv_peaks_any(
+                nwin, ctypes.c_void_p(maps.data_ptr()), ctypes.c_void_p(norms.data_ptr()),
+                n0, n1, self.iy0, self.ix0, _native.PIV_SUBPIX[self.subpix.method],
+                self.subpix.n, int(self.particle_radius), int(self.nb_peaks_to_search),
+                int(bool(apply_subpix)), ctypes.byref(res), _native.stream_ptr(stream)),
+            "piv_peaks_any")
+        return out
+
+    # ------------------------------------------------------------ per-window API
+    def __call__(self, im0, im1):
+        """(correl, norm) of one pair of windows (correl_numpy / correl_pycuda)."""
+        torch = _torch()
+        dev = torch.device("cuda")
+        a0 = np.ascontiguousarray(im0, dtype=np.float32)
+        a1 = np.ascontiguousarray(im1, dtype=np.float32)
+        if a0.shape != self.im0_shape or a1.shape != self.im1_shape:
+            raise ValueError("image shapes differ from the correlation's shapes")
+        maps, norms = self.correlate_stack(torch.from_numpy(a0).to(dev)[None],
+                                           torch.from_numpy(a1).to(dev)[None])
+        torch.cuda.current_stream().synchronize()
+        return maps[0].cpu().numpy(), float(norms[0].item())
+
+    def compute_displacements_from_correl(self, correl, norm=1.0):
+        """CorrelBase.compute_displacements_from_correl (correl.py:226-275):
+        ``(dx, dy, correl_max, other_peaks)`` or ``PIVError``; the
+        second-chance NaN square is written into ``correl``."""
+        torch = _torch()
+        dev = torch.device("cuda")
+        arr = np.asarray(correl)
+        if arr.shape != self.map_shape:
+            raise ValueError("correl has not the map shape of the correlation")
+        maps = torch.from_numpy(np.ascontiguousarray(arr, np.float32)).to(dev)[None]
+        norms = torch.tensor([float(norm)], dtype=torch.float64, device=dev)
+        out = self.peaks_stack(maps, norms, apply_subpix=False)
+        torch.cuda.current_stream().synchronize()
+        host = {k: v.cpu().numpy() for k, v in out.items()}
+        if isinstance(correl, np.ndarray) and correl.flags.writeable:
+            back = maps[0].cpu().numpy()
+            nan = np.isnan(back) & ~np.isnan(correl)
+            if nan.any():
+                correl[nan] = np.nan
+        err = int(host["err"][0])
+        dx, dy, cmax = host["dx"][0], host["dy"][0], host["correl_max"][0]
+        if err != _native.WIN_OK:
+            e = PIVError(explanation=_native.EXPLANATIONS[err])
+            e.results = (int(dx) if dx == dx else dx, int(dy) if dy == dy else dy, cmax)
+            raise e
+        other = None
+        if self.nb_peaks_to_search > 1:
+            other = [(int(a), int(b), np.float32(c))
+                     for (a, b, c) in host["sec"][0, :int(host["nsec"][0])]]
+        return int(dx), int(dy), np.float32(cmax), other
+
+    def apply_subpix(self, dx, dy, correl):
+        """correl.py:277-281 on the direct map."""
+        torch = _torch()
+        dev = torch.device("cuda")
+        ix, iy = self.compute_indices_from_displacement(dx, dy)
+        arr = np.ascontiguousarray(np.asarray(correl), np.float32)
+        maps = torch.from_numpy(arr).to(dev)[None]
+        tix = torch.tensor([int(ix)], dtype=torch.int32, device=dev)
+        tiy = torch.tensor([int(iy)], dtype=torch.int32, device=dev)
+        ox = torch.empty(1, dtype=torch.float32, device=dev)
+        oy = torch.empty(1, dtype=torch.float32, device=dev)
+        er = torch.empty(1, dtype=torch.int32, device=dev)
+        n0, n1 = arr.shape
+        _native.check(
+            self._lib.piv_subpix_any(
+                1, ctypes.c_void_p(maps.data_ptr()), n0, n1, ctypes.c_void_p(tix.data_ptr()),
+                ctypes.c_void_p(tiy.data_ptr()), _native.PIV_SUBPIX[self.subpix.method],
+                self.subpix.n, ctypes.c_void_p(ox.data_ptr()), ctypes.c_void_p(oy.data_ptr()),
+                ctypes.c_void_p(er.data_ptr()), _native.stream_ptr()),
+            "piv_subpix_any")
+        torch.cuda.current_stream().synchronize()
+        err = int(er.item())
+        if err != _native.WIN_OK:
+            raise PIVError(explanation=_native.EXPLANATIONS[err],
+                           result_compute_subpix=(iy, ix))
+        return self.compute_displacement_from_indices(float(ox.item()), float(oy.item()))
+
+
+class CorrelPythranHIP(CorrelDirectHIP):
+    """CorrelPythran's direct correlation (correl_numpy, correl.py:284-412)."""
+
+    _tag = "pythran"
+    _kind = "pythran"
+
+
+class CorrelPyCudaHIP(CorrelDirectHIP):
+    """CorrelPyCuda's direct correlation (correl_pycuda.py:26-240,
+    correl.py:415-451): plain sums, norm times the window size."""
+
+    _tag = "pycuda"
+    _kind = "pycuda"
+
+
 # "fftw" (the reference's default key, singlepass.py:590) selects the same
-# FFT correlation, so reference parameter trees work unchanged
-correlation_classes = {CorrelHIP._tag: CorrelHIP, "fftw": CorrelHIP}
+# FFT correlation, so reference parameter trees work unchanged; "pythran" and
+# "pycuda" the direct correlation (per-window API)
+correlation_classes = {CorrelHIP._tag: CorrelHIP, "fftw": CorrelHIP,
+                       CorrelPythranHIP._tag: CorrelPythranHIP,
+                       CorrelPyCudaHIP._tag: CorrelPyCudaHIP}
 
 
 def register(registry):
-    """Insert the HIP class into a reference-style ``correlation_classes``."""
+    """Insert the HIP classes into a reference-style ``correlation_classes``
+    (the FFT class as "hip", the direct ones under their reference tags)."""
     registry[CorrelHIP._tag] = CorrelHIP
+    registry[CorrelPythranHIP._tag] = CorrelPythranHIP
+    registry[CorrelPyCudaHIP._tag] = CorrelPyCudaHIP
     return registry

@@ -71,25 +71,26 @@ struct PlanArgs {
 // masked elements (where_large_displacement) and the NaN squares written so
 // far (the reference NaNs (2r+1)^2 squares before a second attempt and
 // between secondary peaks, correl.py:234-237,256-259) are NaN.
-template <int N0, int N1, int T>
+template <int T>
 struct Search {
   float* s;
+  int n0, n1;  // map shape
   int b0s, b0e, b1s, b1e;
 
   // nan_indices_max (correl.py:88-132): last maximal non-NaN element of the
   // box in row-major order; returns the PIV_WIN_* code of the peak.
-  __device__ int find(int lane, int& oi0, int& oi1) const {
+  __device__ __forceinline__ int find(int lane, int& oi0, int& oi1) const {
     float best = 0.f;
     int blin = -1;
     for (int c = b1s + lane; c < b1e; c += T) {
 #pragma unroll 4
       for (int r = b0s; r < b0e; ++r) {
-        const float v = s[r * N1 + c];
+        const float v = s[r * n1 + c];
         if (v != v) continue;
         // rows ascend within a lane: ">=" keeps the last maximum
         if (blin < 0 || v >= best) {
           best = v;
-          blin = r * N1 + c;
+          blin = r * n1 + c;
         }
       }
     }
@@ -105,16 +106,16 @@ struct Search {
       oi0 = oi1 = 0;
       return PIV_WIN_NO_PEAK;
     }
-    const int i0 = blin / N1, i1 = blin % N1;
+    const int i0 = blin / n1, i1 = blin % n1;
     oi0 = i0;
     oi1 = i1;
-    if (i0 == 0 || i0 == N0 - 1 || i1 == 0 || i1 == N1 - 1) return PIV_WIN_PEAK_BOUNDARY;
+    if (i0 == 0 || i0 == n0 - 1 || i1 == 0 || i1 == n1 - 1) return PIV_WIN_PEAK_BOUNDARY;
     bool nan = false;
 #pragma unroll
     for (int d0 = -1; d0 <= 1; ++d0)
 #pragma unroll
       for (int d1 = -1; d1 <= 1; ++d1) {
-        const float v = s[(i0 + d0) * N1 + i1 + d1];
+        const float v = s[(i0 + d0) * n1 + i1 + d1];
         nan |= v != v;
       }
     return nan ? PIV_WIN_PEAK_NAN : PIV_WIN_OK;
@@ -123,12 +124,12 @@ struct Search {
   // NaN the (2r+1)^2 square around (i0, i1) with numpy slice bounds; the
   // window's lanes share one wavefront, so a wavefront fence orders the
   // writes before the next search.
-  __device__ void nan_square(int lane, int i0, int i1, int r, int* q) {
+  __device__ __forceinline__ void nan_square(int lane, int i0, int i1, int r, int* q) {
     int a0, a1, c0, c1;
-    py_slice(i0 - r, i0 + r + 1, N0, a0, a1);
-    py_slice(i1 - r, i1 + r + 1, N1, c0, c1);
+    py_slice(i0 - r, i0 + r + 1, n0, a0, a1);
+    py_slice(i1 - r, i1 + r + 1, n1, c0, c1);
     const int h = a1 > a0 ? a1 - a0 : 0, w = c1 > c0 ? c1 - c0 : 0;
-    for (int k = lane; k < h * w; k += T) s[(a0 + k / w) * N1 + c0 + k % w] = __builtin_nanf("");
+    for (int k = lane; k < h * w; k += T) s[(a0 + k / w) * n1 + c0 + k % w] = __builtin_nanf("");
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -149,11 +150,11 @@ __device__ __forceinline__ float peak_value(float v, double norm) {
 
 // compute_subpix_2d_gaussian2 (subpix.py:16-61): float32 normalisation of
 // the 3x3 crop, then the log-quadratic fit.
-template <int N1>
-__device__ void gaussian2(const float* m, int ix, int iy, double& dpx, double& dpy) {
+__device__ __forceinline__ void gaussian2(const float* m, int n1, int ix, int iy, double& dpx,
+                                          double& dpy) {
   float c[9];
   for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) c[3 * a + b] = m[(iy - 1 + a) * N1 + (ix - 1 + b)];
+    for (int b = 0; b < 3; ++b) c[3 * a + b] = m[(iy - 1 + a) * n1 + (ix - 1 + b)];
   float mn = c[0];
   for (int k = 1; k < 9; ++k) mn = fminf(mn, c[k]);
   for (int k = 0; k < 9; ++k) c[k] -= mn;
@@ -185,12 +186,12 @@ __device__ void gaussian2(const float* m, int ix, int iy, double& dpx, double& d
   dpy = (c11 * c10 - 2 * c01 * c20) / den;
 }
 
-template <int N1>
-__device__ void centroid(const float* m, int ix, int iy, int n, double& dpx, double& dpy) {
+__device__ __forceinline__ void centroid(const float* m, int n1, int ix, int iy, int n,
+                                         double& dpx, double& dpy) {
   double s = 0, sx = 0, sy = 0;
   for (int a = -n; a <= n; ++a)
     for (int b = -n; b <= n; ++b) {
-      double v = m[(iy + a) * N1 + (ix + b)];
+      double v = m[(iy + a) * n1 + (ix + b)];
       s += v;
       sx += b * v;
       sy += a * v;
@@ -199,21 +200,20 @@ __device__ void centroid(const float* m, int ix, int iy, int n, double& dpx, dou
   dpy = sy / s;
 }
 
-// SubPix.compute_subpix (subpix.py:107-218) -> PIV_WIN_* code
-template <int N0, int N1>
-__device__ int subpix(const float* m, int method, int nsub, int ix, int iy, double& dpx,
-                      double& dpy) {
+// SubPix.compute_subpix (subpix.py:107-218) -> PIV_WIN_* code; map n0 x n1
+__device__ __forceinline__ int subpix(const float* m, int n0, int n1, int method, int nsub,
+                                      int ix, int iy, double& dpx, double& dpy) {
   dpx = dpy = 0.0;
-  if (iy - nsub < 0 || iy + nsub + 1 > N0 || ix - nsub < 0 || ix + nsub + 1 > N1)
+  if (iy - nsub < 0 || iy + nsub + 1 > n0 || ix - nsub < 0 || ix + nsub + 1 > n1)
     return PIV_WIN_SUBPIX_BOUNDARY;
   double thr = 2.0 * (0.5 + nsub) * (0.5 + nsub);
   if (method == PIV_SUBPIX_NONE) return PIV_WIN_OK;
   if (method == PIV_SUBPIX_2D_GAUSSIAN2) {
-    gaussian2<N1>(m, ix, iy, dpx, dpy);
+    gaussian2(m, n1, ix, iy, dpx, dpy);
     if (dpx * dpx + dpy * dpy > thr) method = PIV_SUBPIX_CENTROID;
     else return PIV_WIN_OK;
   }
-  centroid<N1>(m, ix, iy, nsub, dpx, dpy);
+  centroid(m, n1, ix, iy, nsub, dpx, dpy);
   if (dpx * dpx + dpy * dpy > thr) return PIV_WIN_SUBPIX_WRONG;
   return PIV_WIN_OK;
 }
@@ -223,14 +223,19 @@ __device__ int subpix(const float* m, int method, int nsub, int ix, int iy, doub
 // records. `m` is the LDS map (reference layout), `srch` its LDS search copy
 // (masked elements NaN), `win` the window index. `sq` receives the
 // second-chance square (rows [sq0, sq1), cols [sq2, sq3)), empty if none,
-// for the map side effect with displacement_max None.
-template <int N0, int N1, int T>
-__device__ void finish_window(const float* m, float* srch, const PlanArgs& pa, int lane,
-                              int win, double norm, int apply_subpix, int dx_in, int dy_in,
-                              const piv_results& res, int* sq) {
-  constexpr int iy0 = N0 / 2 - 1, ix0 = N1 / 2 - 1;
-  Search<N0, N1, T> s;
+// for the map side effect with displacement_max None. The map is n0 x n1
+// with the zero displacement at (iy0, ix0): (n0 // 2 - 1, n1 // 2 - 1) for
+// the FFT maps (correl.py:189), (dmax, dmax) for the direct maps
+// (correl.py:409-410,444-445).
+template <int T>
+__device__ __forceinline__ void finish_window(const float* m, float* srch, int n0, int n1,
+                                              int iy0, int ix0, const PlanArgs& pa, int lane,
+                                              int win, double norm, int apply_subpix, int dx_in,
+                                              int dy_in, const piv_results& res, int* sq) {
+  Search<T> s;
   s.s = srch;
+  s.n0 = n0;
+  s.n1 = n1;
   s.b0s = pa.b0s;
   s.b0e = pa.b0e;
   s.b1s = pa.b1s;
@@ -238,7 +243,7 @@ __device__ void finish_window(const float* m, float* srch, const PlanArgs& pa, i
   sq[0] = sq[1] = sq[2] = sq[3] = 0;
   int i0, i1;
   int code = s.find(lane, i0, i1);
-  float cmax = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : peak_value(m[i0 * N1 + i1], norm);
+  float cmax = code == PIV_WIN_NO_PEAK ? __builtin_nanf("") : peak_value(m[i0 * n1 + i1], norm);
   if (code != PIV_WIN_OK && code != PIV_WIN_NO_PEAK) {
     // second chance (correl.py:231-246)
     s.nan_square(lane, i0, i1, pa.particle_radius, sq);
@@ -247,7 +252,7 @@ __device__ void finish_window(const float* m, float* srch, const PlanArgs& pa, i
     if (code2 == PIV_WIN_OK) {
       i0 = j0;
       i1 = j1;
-      cmax = peak_value(m[i0 * N1 + i1], norm);
+      cmax = peak_value(m[i0 * n1 + i1], norm);
       code = PIV_WIN_OK;
     } else {
       code = code2;
@@ -283,10 +288,10 @@ __device__ void finish_window(const float* m, float* srch, const PlanArgs& pa, i
         const size_t o = size_t(win) * (pa.nb_peaks - 1) + k;
         res.sec[3 * o + 0] = float(ix0 - j1);
         res.sec[3 * o + 1] = float(iy0 - j0);
-        res.sec[3 * o + 2] = peak_value(m[j0 * N1 + j1], norm);
+        res.sec[3 * o + 2] = peak_value(m[j0 * n1 + j1], norm);
         if (res.sec_sub) {
           double px, py;
-          const int e = subpix<N0, N1>(m, pa.subpix_method, pa.nsubpix, j1, j0, px, py);
+          const int e = subpix(m, n0, n1, pa.subpix_method, pa.nsubpix, j1, j0, px, py);
           res.sec_sub[2 * o + 0] = e ? __builtin_nanf("") : float(ix0 - (j1 + px));
           res.sec_sub[2 * o + 1] = e ? __builtin_nanf("") : float(iy0 - (j0 + py));
         }
@@ -299,7 +304,7 @@ __device__ void finish_window(const float* m, float* srch, const PlanArgs& pa, i
     int err = PIV_WIN_OK;
     if (apply_subpix) {
       double px, py;
-      const int e = subpix<N0, N1>(m, pa.subpix_method, pa.nsubpix, i1, i0, px, py);
+      const int e = subpix(m, n0, n1, pa.subpix_method, pa.nsubpix, i1, i0, px, py);
       if (e == PIV_WIN_OK) {
         dx = float(ix0 - (i1 + px));
         dy = float(iy0 - (i0 + py));

@@ -278,8 +278,8 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   int sq[4] = {0, 0, 0, 0};
   if (compute) {
     if (lane == 0 && res.norm) res.norm[win] = norm;
-    finish_window<N0, N1, T>(mapf, srch, pa, lane, win, norm, apply_subpix, w.dx_in, w.dy_in,
-                             res, sq);
+    finish_window<T>(mapf, srch, N0, N1, N0 / 2 - 1, N1 / 2 - 1, pa, lane, win, norm,
+                     apply_subpix, w.dx_in, w.dy_in, res, sq);
   } else if (live && lane == 0) {
     const float nan = __builtin_nanf("");
     res.dx[win] = nan;
@@ -327,7 +327,8 @@ __global__ __launch_bounds__(T * WPB) void k_peaks(float* __restrict__ maps,
   if (!live) return;
   int sq[4];
   const double norm = norms[win];
-  finish_window<N0, N1, T>(m, srch, pa, lane, win, norm, apply_subpix, 0, 0, res, sq);
+  finish_window<T>(m, srch, N0, N1, N0 / 2 - 1, N1 / 2 - 1, pa, lane, win, norm, apply_subpix,
+                   0, 0, res, sq);
   if (!pa.has_dmax) {
     for (int k = lane; k < NN; k += T) {
       const int i0 = k / N1, i1 = k % N1;
@@ -345,7 +346,7 @@ __global__ void k_subpix_at(const float* __restrict__ maps, const int32_t* __res
   if (win >= nwin) return;
   double px, py;
   const int x = ix[win], y = iy[win];
-  const int e = subpix<N0, N1>(maps + size_t(win) * N0 * N1, pa.subpix_method, pa.nsubpix, x, y,
+  const int e = subpix(maps + size_t(win) * N0 * N1, N0, N1, pa.subpix_method, pa.nsubpix, x, y,
                                px, py);
   err[win] = e;
   out_x[win] = e ? float(x) : float(x + px);
@@ -568,8 +569,8 @@ __global__ __launch_bounds__(kBigThreads) void k_correlate_big(
   if (tid < 64) {
     int sq[4] = {0, 0, 0, 0};
     if (tid == 0 && res.norm) res.norm[win] = norm;
-    finish_window<N, N, 64>(mapf, srch, pa, tid, win, norm, options & PIV_OPT_SUBPIX, w.dx_in,
-                            w.dy_in, res, sq);
+    finish_window<64>(mapf, srch, N, N, N / 2 - 1, N / 2 - 1, pa, tid, win, norm,
+                      options & PIV_OPT_SUBPIX, w.dx_in, w.dy_in, res, sq);
     if (tid == 0) {
       sqs[0] = sq[0];
       sqs[1] = sq[1];

@@ -73,6 +73,12 @@ class BaseWorkPIV(BaseWorkWithMask):
             raise ValueError(
                 "params.piv0.method_correl should be in " + str(list(correlation_classes.keys()))
             )
+        if not hasattr(correl_cls, "correlate_windows"):
+            raise NotImplementedError(
+                f"method_correl {self.params.piv0.method_correl!r}: the batched WorkPIV runs "
+                "the fused FFT correlation ('fftw' / 'hip'); the direct correlation is "
+                "available through its per-window and window-stack API "
+                f"({correl_cls.__name__})")
         self.correl = correl_cls(
             im0_shape=self.shape_crop_im0,
             im1_shape=self.shape_crop_im1,

@@ -173,6 +173,43 @@ int piv_subpix_at(piv_plan* plan, const float* maps, const int32_t* ix,
                   const int32_t* iy, int32_t nwin, float* out_x, float* out_y,
                   int32_t* err, void* stream);
 
+/* ---- direct (spatial) correlation, replacing CorrelPythran / CorrelPyCuda
+ *      (correl.py:284-451, correl_numpy; correl_pycuda.py:26-240) ----
+ * nwin pairs of windows im0 (nwin x n0y x n0x float32) and im1
+ * (nwin x n1y x n1x, no larger than im0); maps (nwin x (2 dmax + 1)^2,
+ * element [dmax + dy, dmax + dx]) hold the sum of im1[y, x] im0[y + oy, x + ox]
+ * over the reference's overlap for the displacement (dy, dx), and norms
+ * the norm the reference returns with it. normalisation
+ * PIV_DIRECT_PYTHRAN: sum / overlap area * size(im1), norm
+ * sqrt(sum im1^2 sum im0^2) (correl_numpy); PIV_DIRECT_PYCUDA: the plain
+ * sum, norm sqrt(sum im1^2 sum im0^2) size(im0) (correl_pycuda). Sums in
+ * FP64. At most 16384 window pixels in all and 2 dmax + 1 <= 127; a dmax
+ * whose overlap would be empty or leave a window is rejected. */
+#define PIV_DIRECT_PYTHRAN 0
+#define PIV_DIRECT_PYCUDA 1
+int piv_correlate_direct(int32_t nwin, const float* im0, int32_t n0y, int32_t n0x,
+                         const float* im1, int32_t n1y, int32_t n1x,
+                         int32_t displacement_max, int32_t normalisation, float* maps,
+                         double* norms, void* stream);
+
+/* Peak search with the second chance, secondary peaks and (optional)
+ * sub-pixel fit on nwin maps of any shape n0 x n1 (3..127) whose zero
+ * displacement is at (iy0, ix0): CorrelBase.compute_displacements_from_correl
+ * and apply_subpix (correl.py:226-281) for the maps of the direct
+ * correlation ((dmax, dmax), correl.py:409-410,444-445). There is no search
+ * mask: the second-chance NaN square is written into `maps`, as the
+ * reference writes it into the caller's array. */
+int piv_peaks_any(int32_t nwin, float* maps, const double* norms, int32_t n0,
+                  int32_t n1, int32_t iy0, int32_t ix0, int32_t subpix_method,
+                  int32_t nsubpix, int32_t particle_radius, int32_t nb_peaks,
+                  int32_t apply_subpix, const piv_results* res, void* stream);
+
+/* Sub-pixel position at (ix, iy) of each n0 x n1 map, as piv_subpix_at. */
+int piv_subpix_any(int32_t nwin, const float* maps, int32_t n0, int32_t n1,
+                   const int32_t* ix, const int32_t* iy, int32_t subpix_method,
+                   int32_t nsubpix, float* out_x, float* out_y, int32_t* err,
+                   void* stream);
+
 /* Window records of one pass on a ny x nx vector grid at (iyvecs[i],
  * ixvecs[j]) (BaseWorkPIV._prepare_with_image grid, singlepass.py:108-141).
  * First pass: dx_approx == dy_approx == NULL (singlepass.py:210-260). Later

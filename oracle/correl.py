@@ -374,3 +374,86 @@ def brute_force_correl(im0, im1):
             shifted = np.roll(np.roll(x1, -dy, axis=0), -dx, axis=1)
             out[i0, i1] = n0 * n1 * np.sum(x0 * shifted)
     return out
+
+
+def default_displacement_max_direct(im0_shape, im1_shape, kind):
+    """displacement_max of CorrelPythran / CorrelPyCuda when the parameter is
+    None (src/fluidimage/calcul/correl.py:388-408, 418-442): min(shape) // 2
+    - 1 (pythran) or min(shape) // 2 (pycuda) for equal shapes, else
+    min(n0 - n1 per axis) // 2 - 1."""
+    if tuple(im0_shape) == tuple(im1_shape):
+        return min(im0_shape) // 2 - (1 if kind == "pythran" else 0)
+    return min(im0_shape[0] - im1_shape[0], im0_shape[1] - im1_shape[1]) // 2 - 1
+
+
+def _direct_overlap(n0, n1, d):
+    """(length, start in im1, start in im0) of the overlap on one axis for
+    the displacement d (correl.py:327-333 for d <= 0, :341-344 for d > 0)."""
+    if d <= 0:
+        c = n0 // 2 - n1 // 2 + d
+        return n1 + min(c, 0), -min(c, 0), max(0, c)
+    return n1 - max(n0 // 2 + n1 // 2 + d - n0, 0), 0, n0 // 2 - n1 // 2 + d
+
+
+def correl_direct(im0, im1, disp_max, normalisation="pythran"):
+    """correl_numpy (src/fluidimage/calcul/correl.py:284-381, "pythran") and
+    correl_pycuda (src/fluidimage/calcul/correl_pycuda.py:26-240, "pycuda"):
+    map[dmax + dy, dmax + dx] = sum of im1[y, x] im0[y + oy, x + ox] over the
+    overlap; "pythran" divides by the overlap area and multiplies by
+    im1.size, norm sqrt(sum im1^2 sum im0^2); "pycuda" keeps the sum, norm
+    times im0.size. FP64 sums, the map rounded to float32 (the reference
+    sums in float32)."""
+    a = np.asarray(im0, np.float64)
+    b = np.asarray(im1, np.float64)
+    (ny0, nx0), (ny1, nx1) = a.shape, b.shape
+    n = 2 * int(disp_max) + 1
+    out = np.empty((n, n), np.float64)
+    for i in range(n):
+        ly, y1, y0 = _direct_overlap(ny0, ny1, i - disp_max)
+        for j in range(n):
+            lx, x1, x0 = _direct_overlap(nx0, nx1, j - disp_max)
+            s = np.sum(b[y1:y1 + ly, x1:x1 + lx] * a[y0:y0 + ly, x0:x0 + lx])
+            out[i, j] = s / (lx * ly) * b.size if normalisation == "pythran" else s
+    norm = sqrt(np.sum(b**2) * np.sum(a**2))
+    if normalisation == "pycuda":
+        norm *= a.size
+    return out.astype(np.float32), norm
+
+
+class OracleCorrelDirect(OracleCorrelFFT):
+    """CorrelPythran / CorrelPyCuda restated: direct maps of (2 dmax + 1)^2
+    with the zero displacement at (dmax, dmax) (correl.py:409-410,444-445)
+    and CorrelBase's search without mask (correl.py:226-281)."""
+
+    def __init__(self, im0_shape, im1_shape=None, method_subpix="centroid", nsubpix=1,
+                 displacement_max=None, particle_radius=3, nb_peaks_to_search=1, mode=None,
+                 kind="pythran"):
+        im0_shape = tuple(int(n) for n in im0_shape)
+        im1_shape = im0_shape if im1_shape is None else tuple(int(n) for n in im1_shape)
+        self.kind = kind
+        self.mode = mode
+        self.subpix = SubPix(method=method_subpix, nsubpix=nsubpix)
+        self.im0_shape, self.im1_shape = im0_shape, im1_shape
+        dmax = parse_displacement_max(displacement_max, im0_shape)
+        if dmax is None:
+            dmax = default_displacement_max_direct(im0_shape, im1_shape, kind)
+        if dmax <= 0:
+            raise ValueError("displacement_max <= 0 : problem with images shapes?")
+        self.displacement_max = int(dmax)
+        self.ix0 = self.iy0 = self.displacement_max
+        self.particle_radius = particle_radius
+        self.nb_peaks_to_search = nb_peaks_to_search
+        self.start_stop_for_search0 = [0, None]
+        self.start_stop_for_search1 = [0, None]
+
+    def __call__(self, im0, im1):
+        return correl_direct(im0, im1, self.displacement_max, self.kind)
+
+    def compute_displacements_from_correl(self, correl, norm=1.0):
+        """CorrelBase.compute_displacements_from_correl (no search mask: the
+        second-chance square lands in the caller's array)."""
+        dmax, self.displacement_max = self.displacement_max, None
+        try:
+            return super().compute_displacements_from_correl(correl, norm)
+        finally:
+            self.displacement_max = dmax

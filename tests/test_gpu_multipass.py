@@ -112,3 +112,31 @@ def test_deferred_failure_reruns_the_pair(monkeypatch):
             assert pa.errors == pb.errors
         assert np.array_equal(a.passes[-1].deltaxs_final, b.passes[-1].deltaxs_final)
         assert np.array_equal(a.passes[-1].deltays_smooth, b.passes[-1].deltays_smooth)
+
+
+def test_strcrop_mask_vs_oracle():
+    """params.mask.strcrop (src/fluidimage/works/with_mask.py:15-47): the
+    images are cropped before the passes and the positions are reported in
+    the coordinates of the original images."""
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+    from oracle.piv import OracleWorkPIV
+
+    print("case: strcrop", flush=True)
+    im0, im1, p, _ = _setup(True, number=2, window=32, shape=(448, 512))
+    p.mask.strcrop = "20:430, 37:"
+    from tests.test_gpu_piv import _oracle_params
+
+    op = _oracle_params(p)
+    gpu = WorkPIV(p).calcul_from_arrays(im0, im1)
+    ref = OracleWorkPIV(op, precision="f64").calcul(im0, im1)
+    for g, e in zip(gpu.passes, ref):
+        assert np.array_equal(np.asarray(g.xs, float), np.asarray(e["xs"], float))
+        assert np.array_equal(np.asarray(g.ys, float), np.asarray(e["ys"], float))
+        assert float(np.min(g.xs)) >= 37 and float(np.min(g.ys)) >= 20
+        both_nan = np.isnan(g.deltaxs) & np.isnan(e["deltaxs"])
+        ok = both_nan | ((np.abs(g.deltaxs - e["deltaxs"]) <= 1e-3)
+                         & (np.abs(g.deltays - e["deltays"]) <= 1e-3))
+        print(f"strcrop pass: {ok.mean() * 100:.3f} % within 1e-3 px", flush=True)
+        assert ok.mean() >= 0.995
+    df = np.abs(gpu.passes[-1].deltaxs_final - ref[-1]["deltaxs_final"])
+    assert (df <= 1e-3).mean() >= 0.99
