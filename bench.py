@@ -18,7 +18,10 @@ value = final-pass vectors (65,025 per pair) x P per second. The same
 pipeline on one pair (the plain per-pair path) is timed in the same run.
 
 Workload "single": the single-pass correlation of the same kind of pairs
-(32x32 windows, displacement_max 8, 16,129 windows per pair).
+(32x32 windows, displacement_max 8, 16,129 windows per pair). Workload
+"direct": the direct spatial correlation (CorrelPythran) of the 32x32
+windows of such pairs. Workload "refine": the differentiable refinement of
+1024x1024 float32 pairs.
 
 The roofline object is the longest correlation launch of a step (the stage
 whose bytes are the algorithmic HBM figure: unique image bytes + maps +
@@ -56,7 +59,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=32)
-    ap.add_argument("--workload", default="multipass", choices=["multipass", "single", "refine"])
+    ap.add_argument("--workload", default="multipass",
+                    choices=["multipass", "single", "refine", "direct"])
     ap.add_argument("--no-maps", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the one-pair (plain per-pair path) measurement")
@@ -323,16 +327,126 @@ def bench_refine(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_direct(args):
+    """Direct (spatial) correlation, the reference's CorrelPythran: the 32x32
+    windows (50 % overlap, 16,129 per pair) of P seeded 2048x2048 uint16
+    pairs, displacement_max None (15: 31x31 maps), stacked on the device
+    before the timed region; a step = maps + norms of every window, then
+    the peak search and 2d_gaussian2 fit. value = windows/s. FP64 sums:
+    2 x (sum over the map of the overlap areas) flops per window, against
+    the FP64 vector peak."""
+    import torch
+
+    from fluidimage_amd.calcul.correl import CorrelPythranHIP
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+
+    npairs = max(1, args.pairs)
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    print(f"bench: generating {npairs} 2048x2048 uint16 pairs (config 2)", file=sys.stderr,
+          flush=True)
+    pairs = [config_pair(2, pair_idx=k) for k in range(npairs)]
+    p = WorkPIV.create_default_params()
+    p.piv0.shape_crop_im0 = 32
+    work = WorkPIV(p).works_piv[0]
+    work._prepare_with_image(imshape=pairs[0][0].shape)
+    n = 32
+    iy = torch.from_numpy(work.iyvecs_grid.astype("int64")).to(dev) - n // 2
+    ix = torch.from_numpy(work.ixvecs_grid.astype("int64")).to(dev) - n // 2
+    rows = (iy[:, None] + torch.arange(n, device=dev))[:, :, None]
+    cols = (ix[:, None] + torch.arange(n, device=dev))[:, None, :]
+    w0, w1 = [], []
+    for a, b in pairs:
+        for im, out in ((a, w0), (b, w1)):
+            t = torch.from_numpy(im.astype("float32")).to(dev)
+            t = t - t.min()
+            out.append(t[rows.clamp(0, t.shape[0] - 1), cols.clamp(0, t.shape[1] - 1)])
+    w0 = torch.cat(w0).contiguous()
+    w1 = torch.cat(w1).contiguous()
+    nwin = int(w0.shape[0])
+    op = CorrelPythranHIP((n, n), (n, n), method_subpix="2d_gaussian2", nsubpix=None)
+    d = op.displacement_max
+    area = sum(n - abs(k) for k in range(-d, d + 1)) ** 2
+    flops_per_window = 2 * area + 4 * n * n
+
+    def step():
+        maps, norms = op.correlate_stack(w0, w1)
+        return op.peaks_stack(maps, norms, apply_subpix=True)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    e0.record()
+    op.correlate_stack(w0, w1)
+    e1.record()
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1)
+    nerr = int((out["err"] != 0).sum().item())
+    achieved = flops_per_window * nwin / (k_ms / 1e3) / 1e12
+    line = {
+        "metric": "windows/s, direct correlation (CorrelPythran) of 2048² pairs, 32x32 windows",
+        "value": round(nwin / (ms / 1e3), 1),
+        "unit": "windows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"direct spatial correlation of the 32x32 windows of {npairs} "
+                               "2048x2048 uint16 pairs (50% overlap, displacement_max 15, "
+                               "31x31 maps), peak search and 2d_gaussian2",
+                   "pairs": npairs, "windows": nwin, "window_errors": nerr},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": round(achieved / 78.6, 4), "traffic": None,
+                     "kernel": "k_correlate_direct", "kernel_ms": round(k_ms, 4),
+                     "flops_per_window": flops_per_window},
+    }
+    if not args.no_cpu_baseline:
+        from threadpoolctl import threadpool_limits
+
+        from oracle.correl import OracleCorrelDirect
+
+        ref = OracleCorrelDirect((n, n), (n, n), method_subpix="2d_gaussian2", nsubpix=None)
+        h0 = w0[:3000].cpu().numpy()  # about 10 s of oracle work
+        h1 = w1[:3000].cpu().numpy()
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            for k in range(len(h0)):
+                c, norm = ref(h0[k], h1[k])
+                try:
+                    ref.compute_displacements_from_correl(c, norm)
+                except Exception:
+                    pass
+            dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(len(h0) / dt, 1), "unit": "windows/s",
+                                "cores": 1, "kind": "port",
+                                "sample": f"{len(h0)} windows of the workload, oracle "
+                                          f"(numpy per displacement), {dt:.1f} s on 1 thread"}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
         return
-    if args.workload == "refine":
+    if args.workload in ("refine", "direct"):
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
-        bench_refine(args)
+        (bench_refine if args.workload == "refine" else bench_direct)(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
