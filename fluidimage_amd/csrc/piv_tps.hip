@@ -425,6 +425,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   __shared__ double rdiag[kNB];              // 1 / L_ii
   __shared__ double colj[64];                // pivot column of the diagonal factorisation
   __shared__ double tbuf[16][17];            // L21 X11
+  __shared__ double ptile[kCholWaves][16][kNB + 1];  // per-wavefront strip transposition
   __shared__ int bad;
   __shared__ int next_pair;                  // look-ahead work counter
   if (tid == 0) bad = 0;
@@ -543,17 +544,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
   };
   auto npairs_of = [&](int Jt) { return ((m - Jt * kNB + 15) / 16 + 1) / 2; };
+  if (tid == 0) next_pair = 0;
+  __syncthreads();
   for (int J = 0; J < nblk; ++J) {
     const int r0 = J * kNB;
     const int bs = min(kNB, m - r0);
     const int nstrip = (m - r0 + 15) / 16;
-    if (tid == 0) next_pair = 0;
-    // a) finish block column J: its rank-32 update with block J - 1 (the
-    //    earlier blocks were applied while the previous diagonal block was
-    //    factorised), or for J = 0 the rank-6 term alone
-    for (int sp = wave; sp < npairs_of(J); sp += kCholWaves)
-      update_pair(J, sp, J > 0 ? r0 - kNB : 0, r0, J == 0, true);
-    __syncthreads();
+    // a) the diagonal block of column J finished by wavefront 0: its rank-32
+    //    update with block J - 1 (the earlier blocks were applied while the
+    //    previous diagonal block was factorised), or for J = 0 the rank-6
+    //    term alone; the rows below are finished in the fused panel (c)
     // b) diagonal block by wavefront 0 (C_JJ is in lbuf, written there by
     //    this wavefront in a): right-looking factorisation in registers
     //    (lane i < 32 holds row i; column j of the trailing matrix is
@@ -564,6 +564,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     //    column c of the lower one), the off-diagonal block -X22 (L21 X11)
     //    on the matrix cores
     if (wave == 0) {
+      update_pair(J, 0, J > 0 ? r0 - kNB : 0, r0, J == 0, true);
+      wave_sync();
       // the lane index through an opaque move: the lane-dependent selects
       // below stay in this block instead of being hoisted out of the J loop
       int dl;
@@ -661,25 +663,94 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       }
     }
     __syncthreads();
-    // c) panel rows below the diagonal block: L(rows, J) = C(rows, J) inv^T,
-    //    D[i][j] = sum_k C[i][k] inv[j][k] (k <= j), k = k0 + 4 lk + q
+    if (tid == 0) next_pair = 0;  // the next look-ahead's counter (read after the barrier below)
+    // c) rows below the diagonal block, by strips of 16: the finishing
+    //    update of a) for these rows (rank 32 with block J - 1, or the rank-6
+    //    term for J = 0) in registers, transposed through the wavefront's LDS
+    //    tile, then L(rows, J) = C(rows, J) inv^T, D[i][j] = sum_k C[i][k]
+    //    inv[j][k] (k <= j): block column J is read and written once here
     if (bs == kNB) {
-      // each wavefront requests the next strip's rows before the matrix-core
-      // steps of the current one
-      auto load_strip = [&](double* a8, int st) {
-        const int ra = min(r0 + 16 * st + li, m - 1);
-        const double* pa = A + int64_t(ra) * n + r0 + 4 * lk;
+      const int kb = J > 0 ? r0 - kNB : 0;
+      double(*tile)[kNB + 1] = ptile[wave];
+      // the update's right operand (block row J) is the same for every strip
+      double ob0[8], ob1[8];
+      if (J > 0) {
+        const double* pb0 = A + int64_t(r0 + li) * n + kb + 4 * lk;
+        const double* pb1 = A + int64_t(r0 + 16 + li) * n + kb + 4 * lk;
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) a8[4 * h + q] = pa[16 * h + q];
-      };
-      double a8[8], nx8[8];
-      int st = 2 + wave;
-      if (st < nstrip) load_strip(a8, st);
-      for (; st < nstrip; st += kCholWaves) {
+          for (int q = 0; q < 4; ++q) {
+            ob0[4 * h + q] = pb0[16 * h + q];
+            ob1[4 * h + q] = pb1[16 * h + q];
+          }
+      } else {
+        // rank-6 term V Z^T + Z V^T as two k-steps: b_j = [Z_j, V_j]
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+          const int k = 4 * s2 + lk;
+          const bool kin = k < 6;
+          const int kbb = kin ? (k + 3) % 6 : 0;
+          ob0[s2] = kin ? VC[6 * (r0 + li) + kbb] : 0.0;
+          ob1[s2] = kin ? VC[6 * (r0 + 16 + li) + kbb] : 0.0;
+        }
+      }
+      // a strip's current values and update operand (rows clamped below m)
+      auto load_strip = [&](int st, double* cur, double* oa) {
         const int row0 = r0 + 16 * st;
-        if (st + kCholWaves < nstrip) load_strip(nx8, st + kCholWaves);
+        const int ra = min(row0 + li, m - 1);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double* src = A + int64_t(min(row0 + 4 * v + lk, m - 1)) * n + r0;
+          cur[v] = src[li];
+          cur[4 + v] = src[16 + li];
+        }
+        if (J > 0) {
+          const double* pa = A + int64_t(ra) * n + kb + 4 * lk;
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) oa[4 * h + q] = pa[16 * h + q];
+        } else {
+#pragma unroll
+          for (int s2 = 0; s2 < 2; ++s2) {
+            const int k = 4 * s2 + lk;
+            oa[s2] = k < 6 ? VC[6 * ra + k] : 0.0;
+          }
+        }
+      };
+      for (int st = 2 + wave; st < nstrip; st += kCholWaves) {
+        const int row0 = r0 + 16 * st;
+        double cur[8], oa[8];
+        load_strip(st, cur, oa);
+        dvec4 u0 = {0.0, 0.0, 0.0, 0.0}, u1 = {0.0, 0.0, 0.0, 0.0};
+        if (J > 0) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            u0 = mfma_f64(oa[e], ob0[e], u0);
+            u1 = mfma_f64(oa[e], ob1[e], u1);
+          }
+        } else {
+#pragma unroll
+          for (int s2 = 0; s2 < 2; ++s2) {
+            u0 = mfma_f64(oa[s2], ob0[s2], u0);
+            u1 = mfma_f64(oa[s2], ob1[s2], u1);
+          }
+        }
+        // the finished rows (layout D[4 v + lk][li]) into the tile, read
+        // back as the panel's left operand (row li, column 16 h + 4 lk + q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          tile[4 * v + lk][li] = cur[v] - u0[v];
+          tile[4 * v + lk][16 + li] = cur[4 + v] - u1[v];
+        }
+        wave_sync();
+        double a8[8];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) a8[4 * h + q] = tile[li][16 * h + 4 * lk + q];
+        wave_sync();  // the tile is read before the next strip writes it
         dvec4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int h = 0; h < 2; ++h)
@@ -700,11 +771,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             dst[16 + li] = c1[v];
           }
         }
-#pragma unroll
-        for (int e = 0; e < 8; ++e) a8[e] = nx8[e];
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
   if (tid == 0 && bad) status[d] = 1;
 }
