@@ -240,9 +240,10 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
     vx = torch.empty(ntot, dtype=torch.float32, device=dev)
     vy = torch.empty(ntot, dtype=torch.float32, device=dev)
     idx = torch.empty(ntot, **i32)
-    info = torch.empty((len(fields), 6), **f64)
+    info = torch.empty((len(fields), 7), **f64)
     _native.check(lib.piv_tps_select(len(fields), n, _p(dx), _p(dy), _p(xs), _p(ys), _p(cx),
-                                     _p(cy), _p(vx), _p(vy), _p(idx), _p(info), sp),
+                                     _p(cy), _p(vx), _p(vy), _p(idx), _p(info),
+                                     float(lattice or 0.0), sp),
                   "piv_tps_select")
     info_h = info.cpu().numpy()
     # subdomain grids (host scalars, all fields at once)
@@ -252,6 +253,7 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
     for f, g in zip(flist, grids):
         f.count = int(info_h[f.k, 0])
         f.sorted_y = int(info_h[f.k, 5])
+        f.on_lattice = bool(info_h[f.k, 6])
         if f.count == 0:
             f.error = SingularSystemError("Only nan.")
         elif isinstance(g, SingularSystemError):
@@ -415,11 +417,14 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     slot_xy = torch.empty((2, nslots), **f64)
     lat, lat_table = _lattice(lib, lattice, active, dev, sp)
     lat_p = None if lat is None else ctypes.byref(lat)
+    # every center of every field on the lattice: the factorisation generates
+    # the matrix from the table instead of reading it from the build
+    generated = int(lat is not None and all(f.on_lattice for f in active))
     _native.check(
         lib.piv_tps_smooth(nd, n_max, _p(cx), _p(cy), _p(vx), _p(vy), _p(dom_off), _p(slot_pt),
                            _p(mat_off), _p(smoothing), thr, _p(mats), _p(refl), _p(vwork),
                            _p(slot_xy), _p(smooth_slots), nslots, _p(weights), wstride,
-                           _p(stats), _p(status), lat_p, sp),
+                           _p(stats), _p(status), lat_p, generated, sp),
         "piv_tps_smooth")
     smoothed = torch.empty((2, ntot), **f64)
     _native.check(lib.piv_tps_blend(ntot, _p(c_slot), _p(c_coef), _p(c_norm), _p(smooth_slots),

@@ -116,6 +116,16 @@ __device__ __forceinline__ double tps_phi_at(double dx, double dy, const PhiLatt
   return tps_phi(tps_r2(dx, dy));
 }
 
+// the table entry of a difference known to lie on the lattice (every
+// center a multiple of the step, differences within a_max): the index
+// tps_phi_at computes, without its check (clamped for safety)
+__device__ __forceinline__ double tps_phi_on_lattice(double dx, double dy, const PhiLattice& lat,
+                                                    const double* table) {
+  const int a = min(int(rint(fabs(dx) * lat.inv_step)), lat.a_max);
+  const int b = min(int(rint(fabs(dy) * lat.inv_step)), lat.a_max);
+  return table[a * (lat.a_max + 1) + b];
+}
+
 // the lattice table in LDS at `lds` when it was sized for it (else the
 // global table); call before a barrier of the whole workgroup
 __device__ __forceinline__ const double* stage_lattice(const PhiLattice& lat, double* lds) {
@@ -182,7 +192,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
     const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
     double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
     double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm,
-    PhiLattice lat) {
+    PhiLattice lat, bool store_full) {
   constexpr int kWaves = NT / 64;
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
@@ -284,13 +294,15 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
     const bool singular = n < 3 || PM(0, 0) == 0.0 || PM(1, 1) == 0.0 || PM(2, 2) == 0.0;
     if (singular) status[d] = 1;
   }
-  // ---- lower triangle of B = K + s I and Y = B V: row i by one
-  //      wavefront, its lanes over all columns j (B_ij = B_ji bit for bit:
-  //      the kernel function sees the squares of the differences), so that
-  //      Y_i is a wavefront reduction and nothing is scattered; only the
-  //      lower triangle j <= i is stored (it is all the factorisation
-  //      reads). Y goes to the P slots (P is no longer used: R and the
-  //      reflectors are saved above) ----
+  // ---- B = K + s I and Y = B V: row i by one wavefront, its lanes over
+  //      all columns j (B_ij = B_ji bit for bit: the kernel function sees
+  //      the squares of the differences), so that Y_i is a wavefront
+  //      reduction and nothing is scattered. The lower triangle is stored,
+  //      or (store_full false) only B[:, :3], the projection terms of the
+  //      solves, when the factorisation generates the other elements with
+  //      the same values where it first reads them.
+  //      Y goes to the P slots (P is no longer used: R and the reflectors
+  //      are saved above) ----
   double m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int lane = tid & 63, wave = tid >> 6;
   __syncthreads();  // thread 0 has read R out of P
@@ -301,7 +313,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
     for (int j = lane; j < n; j += 64) {
       double b = tps_phi_at(cx[j] - xi, cy[j] - yi, lat, ltab);
       if (i == j) b += smoothing;
-      if (j <= i) Bi[j] = b;
+      if (j <= i && (store_full || j < 3)) Bi[j] = b;
       r0 += b * VM(0, j);
       r1 += b * VM(1, j);
       r2 += b * VM(2, j);
@@ -410,11 +422,30 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                            const int64_t* __restrict__ mat_off,
                                                            double* __restrict__ mats,
                                                            const double* __restrict__ vwork,
-                                                           int32_t* __restrict__ status) {
+                                                           int32_t* __restrict__ status,
+                                                           const double* __restrict__ slot_xy,
+                                                           int64_t nslots,
+                                                           const double* __restrict__ smoothing_d,
+                                                           PhiLattice lat, bool gen) {
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
   double* B = mats + mat_off[d];
+  // C before any update is B'[3:, 3:] without the rank-6 term, i.e. the
+  // kernel function of two centers plus s on the diagonal. With `gen` (every
+  // center on the lattice of the table) it is generated where it is first
+  // read and the matrix build stores only B[:, :3]; else it is read from
+  // the matrix the build stored
+  extern __shared__ double chol_dyn[];  // the lattice table, when staged
+  const double* ltab = stage_lattice(lat, chol_dyn);
+  const double* SX = slot_xy + s0 + 3;
+  const double* SY = slot_xy + nslots + s0 + 3;
+  const double smoothing = smoothing_d[d];
+  auto fresh = [&](int row, int col) {
+    double b = tps_phi_on_lattice(SX[col] - SX[row], SY[col] - SY[row], lat, ltab);
+    if (row == col) b += smoothing;
+    return b;
+  };
   const double* VW = vwork + 6 * s0;
   constexpr int kCholThreads = NT, kCholWaves = NT / 64;
   const int tid = threadIdx.x;
@@ -441,7 +472,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   // C(r0t:m, Jt) -= [rank-6 term] + L(r0t:m, kb:ke) L(Jt, kb:ke)^T for the
   // strip pair sp (rows r0t + 32 sp .. + 31) on the FP64 matrix cores; with
   // keep_diag, pair 0 also leaves the updated diagonal block in lbuf
-  auto update_pair = [&](int Jt, int sp, int kb, int ke, bool with_wz, bool keep_diag) {
+  auto update_pair = [&](int Jt, int sp, int kb, int ke, bool with_wz, bool keep_diag,
+                         bool first_touch) {
     const int r0t = Jt * kNB;
     const int bst = min(kNB, m - r0t);
     {
@@ -462,9 +494,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const double* src = A + int64_t(min(row0 + 16 * h + 4 * v + lk, m - 1)) * n + r0t;
-          cur[h][v][0] = src[min(li, bst - 1)];
-          cur[h][v][1] = src[min(16 + li, bst - 1)];
+          const int row = min(row0 + 16 * h + 4 * v + lk, m - 1);
+          if (first_touch && gen) {
+            cur[h][v][0] = fresh(row, r0t + min(li, bst - 1));
+            cur[h][v][1] = fresh(row, r0t + min(16 + li, bst - 1));
+          } else {
+            const double* src = A + int64_t(row) * n + r0t;
+            cur[h][v][0] = src[min(li, bst - 1)];
+            cur[h][v][1] = src[min(16 + li, bst - 1)];
+          }
         }
       dvec4 c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
       // the rank-6 term V Z^T + Z V^T of B' as two more k-steps:
@@ -564,7 +602,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     //    column c of the lower one), the off-diagonal block -X22 (L21 X11)
     //    on the matrix cores
     if (wave == 0) {
-      update_pair(J, 0, J > 0 ? r0 - kNB : 0, r0, J == 0, true);
+      update_pair(J, 0, J > 0 ? r0 - kNB : 0, r0, J == 0, true, J == 0);
       wave_sync();
       // the lane index through an opaque move: the lane-dependent selects
       // below stay in this block instead of being hoisted out of the J loop
@@ -659,7 +697,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         if (lane == 0) sp = atomicAdd(&next_pair, 1);
         sp = __builtin_amdgcn_readfirstlane(sp);
         if (sp >= np) break;
-        update_pair(J + 1, sp, 0, r0, true, false);
+        update_pair(J + 1, sp, 0, r0, true, false, true);
       }
     }
     __syncthreads();
@@ -701,9 +739,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int ra = min(row0 + li, m - 1);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const double* src = A + int64_t(min(row0 + 4 * v + lk, m - 1)) * n + r0;
-          cur[v] = src[li];
-          cur[4 + v] = src[16 + li];
+          const int row = min(row0 + 4 * v + lk, m - 1);
+          if (J == 0 && gen) {  // column 0 has had no look-ahead: first read
+            cur[v] = fresh(row, r0 + li);
+            cur[4 + v] = fresh(row, r0 + 16 + li);
+          } else {
+            const double* src = A + int64_t(row) * n + r0;
+            cur[v] = src[li];
+            cur[4 + v] = src[16 + li];
+          }
         }
         if (J > 0) {
           const double* pa = A + int64_t(ra) * n + kb + 4 * lk;
@@ -1118,7 +1162,8 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
                                                      const double* __restrict__ xs,
                                                      const double* __restrict__ ys,
                                                      double* cx, double* cy, float* vx,
-                                                     float* vy, int32_t* idx, double* info) {
+                                                     float* vy, int32_t* idx, double* info,
+                                                     double lat_step) {
   // field blockIdx.x: elements [f n, (f + 1) n) in and out
   {
     const int64_t o = int64_t(blockIdx.x) * n;
@@ -1131,13 +1176,15 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
     vx += o;
     vy += o;
     idx += o;
-    info += 6 * blockIdx.x;
+    info += 7 * blockIdx.x;
   }
+  const double inv_step = lat_step > 0.0 ? 1.0 / lat_step : 0.0;
   __shared__ int tmp[32];
   __shared__ double red[4][16];
   double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
   int base = 0;
-  int unsorted = 0;  // some ys[i] > ys[i + 1]
+  int unsorted = 0;   // some ys[i] > ys[i + 1]
+  int off_lattice = lat_step > 0.0 ? 0 : 1;  // some selected position not a multiple of the step
   for (int t0 = 0; t0 < n; t0 += 1024) {
     const int i = t0 + threadIdx.x;
     float a = 0.f, b = 0.f;
@@ -1158,6 +1205,8 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
       vx[pos] = a;
       vy[pos] = b;
       idx[pos] = i;
+      if (rint(x * inv_step) * lat_step != x || rint(y * inv_step) * lat_step != y)
+        off_lattice = 1;
       xmin = fmin(xmin, x);
       xmax = fmax(xmax, x);
       ymin = fmin(ymin, y);
@@ -1166,6 +1215,7 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
     base += total;
   }
   unsorted = __syncthreads_or(unsorted);
+  off_lattice = __syncthreads_or(off_lattice);
   for (int off = 32; off > 0; off >>= 1) {
     xmin = fmin(xmin, __shfl_xor(xmin, off, 64));
     xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
@@ -1192,6 +1242,7 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
     info[3] = red[2][0];
     info[4] = red[3][0];
     info[5] = unsorted ? 0.0 : 1.0;
+    info[6] = off_lattice ? 0.0 : 1.0;
   }
 }
 
@@ -1621,7 +1672,7 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
                    double threshold, double* mats, double* refl, double* vwork, double* slot_xy,
                    double* smooth_slots, int64_t nslots, double* weights, int64_t wstride,
                    double* stats, int32_t* status, const piv_tps_lattice* lattice,
-                   void* stream) {
+                   int32_t generated, void* stream) {
   if (nd <= 0) return PIV_OK;
   if (n_max > kMaxN)
     return tps_fail(PIV_EUNSUPPORTED, "more than 2048 centers in a TPS subdomain");
@@ -1631,15 +1682,16 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
   hipStream_t s = static_cast<hipStream_t>(stream);
   // up to one subdomain per CU: 16 wavefronts per matrix build, else 4
   const PhiLattice lat = make_lattice(lattice);
+  const bool gen = generated != 0 && lat.table != nullptr;
   const size_t prep_lds = size_t(8) * n_max * sizeof(double) + size_t(lat.lds_words) * sizeof(double);
   if (nd <= cu_count())
     hipLaunchKernelGGL(k_tps_prep<1024>, dim3(nd), dim3(1024), prep_lds, s, xs, ys, dom_off,
                        slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots, status,
-                       int(n_max), lat);
+                       int(n_max), lat, !gen);
   else
     hipLaunchKernelGGL(k_tps_prep<kThreads>, dim3(nd), dim3(kThreads), prep_lds, s, xs, ys,
                        dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots,
-                       status, int(n_max), lat);
+                       status, int(n_max), lat, !gen);
   // 8 wavefronts per subdomain (one workgroup per CU) or 4 (two per CU).
   // Up to two subdomains per CU the launch is one round either way if the
   // width fits: 8 wavefronts up to one subdomain per CU, 4 up to two; beyond,
@@ -1649,12 +1701,13 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
   if (nd <= cus) ct = 512;
   else if (nd <= 2 * cus) ct = 256;
   else if (ct == 0) ct = 256;
+  const size_t chol_lds = size_t(lat.lds_words) * sizeof(double);
   if (ct == 512)
-    hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), 0, s, dom_off, mat_off, mats, vwork,
-                       status);
+    hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), chol_lds, s, dom_off, mat_off, mats,
+                       vwork, status, slot_xy, nslots, smoothing, lat, gen);
   else
-    hipLaunchKernelGGL(k_tps_chol<256>, dim3(nd), dim3(256), 0, s, dom_off, mat_off, mats, vwork,
-                       status);
+    hipLaunchKernelGGL(k_tps_chol<256>, dim3(nd), dim3(256), chol_lds, s, dom_off, mat_off, mats,
+                       vwork, status, slot_xy, nslots, smoothing, lat, gen);
   hipLaunchKernelGGL(k_tps_solve, dim3(nd), dim3(kThreads),
                      size_t(3) * ((n_max + 1) & ~1) * sizeof(double), s, vx, vy, dom_off, slot_pt,
                      mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,
@@ -1665,13 +1718,13 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
 
 int piv_tps_select(int32_t nfields, int32_t n, const float* dx, const float* dy,
                    const double* xs, const double* ys, double* cx, double* cy, float* vx,
-                   float* vy, int32_t* idx, double* info, void* stream) {
+                   float* vy, int32_t* idx, double* info, double lattice_step, void* stream) {
   if (nfields <= 0 || n <= 0) return tps_fail(PIV_EINVAL, "no vector");
   if (!dx || !dy || !xs || !ys || !cx || !cy || !vx || !vy || !idx || !info)
     return tps_fail(PIV_EINVAL, "NULL argument");
   hipLaunchKernelGGL(k_tps_select, dim3(nfields), dim3(1024), 0,
                      static_cast<hipStream_t>(stream), n, dx, dy, xs, ys, cx, cy, vx, vy, idx,
-                     info);
+                     info, lattice_step);
   return tps_check("piv_tps_select");
 }
 

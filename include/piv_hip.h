@@ -272,7 +272,13 @@ typedef struct piv_tps_lattice {
  * the device array of the (rescaled) smoothing coefficient of every
  * subdomain, so that the subdomains of several vector fields, each with its
  * own grid, go through one call. threshold < 0 disables the outlier
- * iterations. lattice: see piv_tps_lattice (NULL: direct evaluation). */
+ * iterations. lattice: see piv_tps_lattice (NULL: direct evaluation).
+ * generated (with a lattice): every center is a multiple of the lattice
+ * step (info[f][6] of piv_tps_select for every field) and the table covers
+ * the differences within a subdomain; the matrix build then stores only
+ * B[:, :3] and the factorisation generates the other elements from the
+ * table where it first reads them (the same values; mats still needs its
+ * full size for the factor). 0: the build stores the lower triangle. */
 int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    const double* ys, const float* vx, const float* vy,
                    const int64_t* dom_off, const int32_t* slot_pt,
@@ -280,7 +286,7 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    double* mats, double* refl, double* vwork, double* slot_xy,
                    double* smooth_slots, int64_t nslots, double* weights,
                    int64_t wstride, double* stats, int32_t* status,
-                   const piv_tps_lattice* lattice, void* stream);
+                   const piv_tps_lattice* lattice, int32_t generated, void* stream);
 
 /* One vector field (one image pair) of a TPS batch: its subdomain grid
  * (nsx x nsy, limits at lim + lim_off: lmin_x[nsx], lmax_x[nsx],
@@ -302,8 +308,9 @@ typedef struct piv_tps_field {
  * [f n, (f + 1) n) of dx, dy, xs, ys), ordered compaction of the vectors
  * whose dx and dy are finite (singlepass.py:450-463) into cx, cy
  * (positions), vx, vy (values) and idx (index in the field), starting at
- * element f n; info[f][6] = count, x_min, x_max, y_min, y_max, and 1.0
- * when ys is nondecreasing over the field (0.0 otherwise).
+ * element f n; info[f][7] = count, x_min, x_max, y_min, y_max, 1.0
+ * when ys is nondecreasing over the field (0.0 otherwise), and 1.0 when
+ * lattice_step > 0 and every selected position is a multiple of it.
  * piv_tps_count / piv_tps_fill: members of every subdomain d of the batch
  * (field dom_field[d], local index d - dom_base = iy * nsx + ix) with the
  * half-open limits of its field, in increasing center order
@@ -321,7 +328,7 @@ typedef struct piv_tps_field {
 int piv_tps_select(int32_t nfields, int32_t n, const float* dx, const float* dy,
                    const double* xs, const double* ys, double* cx, double* cy,
                    float* vx, float* vy, int32_t* idx, double* info,
-                   void* stream);
+                   double lattice_step, void* stream);
 int piv_tps_count(int32_t nd, const piv_tps_field* fields,
                   const int32_t* dom_field, const double* lim, const double* cx,
                   const double* cy, int32_t* counts, void* stream);
