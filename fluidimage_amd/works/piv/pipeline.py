@@ -393,6 +393,52 @@ class MultipassPipeline:
         return results
 
 
+class GraphedPasses:
+    """The passes of a single-pass ``WorkPIV`` over a batch of pairs (image
+    minima, window records, one fused correlation launch, WorkFIX) captured
+    once in a HIP graph and replayed: a replay re-runs every kernel on the
+    images currently in ``imageset`` (same tensors, new contents) and writes
+    into the same result tensors (``passes``).
+
+    Only passes without an interpolation between them can be captured: the
+    TPS stage sizes its buffers from device-to-host reads (the valid-vector
+    count and bounds, the subdomain member counts), and griddata runs on the
+    host."""
+
+    def __init__(self, work_piv, imageset, npairs, keep_maps=False, warmup=2):
+        import torch
+
+        if len(work_piv.works_piv) != 1:
+            raise NotImplementedError(
+                "graph capture of the multipass: the interpolation between passes sizes its "
+                "buffers on the host")
+        if work_piv.params.piv0.nb_peaks_to_search > 1:
+            raise NotImplementedError("secondary-peak replacement runs on the host")
+        self.pipe = MultipassPipeline(work_piv)
+        self.imageset = imageset
+        self.npairs = npairs
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):  # plans, grids and caches outside the graph
+            for _ in range(warmup):
+                self._step(keep_maps)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.passes = self._step(keep_maps)
+
+    def _step(self, keep_maps):
+        self.imageset.recompute_min()
+        return self.pipe.run(self.imageset, self.npairs, keep_maps=keep_maps, final_interp=False)
+
+    def replay(self):
+        """Enqueue the captured step on the current stream; returns the
+        passes whose tensors receive the results."""
+        self.graph.replay()
+        return self.passes
+
+
 def run_pairs(work_piv, pairs, keep_maps=False, stream=None):
     """WorkPIV on a list of (im0, im1) host or device arrays of one shape."""
     import torch
