@@ -84,6 +84,11 @@ class ImageSet:
             raise TypeError("all images of a set must share one dtype")
         self.dtype = dts.pop()
         n = len(self.images)
+        # the flat (vectorised) minimum when every image is contiguous and
+        # 16-byte aligned
+        flat = all(t.stride(1) == 1 and t.stride(0) == t.shape[1] and t.data_ptr() % 16 == 0
+                   for t in self.images)
+        self._flat_dtype = self.dtype if flat else -1
         self.mins = torch.empty(n, dtype=torch.float32, device=device)
         host = np.zeros(n, dtype=IMAGE_DTYPE)
         for i, t in enumerate(self.images):
@@ -103,7 +108,7 @@ class ImageSet:
             lib = _native.load()
             _native.check(
                 lib.piv_images_min(ctypes.c_void_p(self.table.data_ptr()), n,
-                                   ctypes.c_void_p(self.mins.data_ptr()),
+                                   ctypes.c_void_p(self.mins.data_ptr()), self._flat_dtype,
                                    _native.stream_ptr(stream)),
                 "piv_images_min",
             )
@@ -112,7 +117,7 @@ class ImageSet:
         lib = _native.load()
         _native.check(
             lib.piv_images_min(ctypes.c_void_p(self.table.data_ptr()), len(self.images),
-                               ctypes.c_void_p(self.mins.data_ptr()),
+                               ctypes.c_void_p(self.mins.data_ptr()), self._flat_dtype,
                                _native.stream_ptr(stream)),
             "piv_images_min",
         )

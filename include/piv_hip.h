@@ -144,9 +144,11 @@ int piv_plan_create(piv_plan** plan, int32_t n0, int32_t n1,
 int piv_plan_destroy(piv_plan* plan);
 
 /* out_min[i] = min over image i (device float per image). `images` is a
- * device array of nimages descriptors. */
+ * device array of nimages descriptors. flat_dtype: the images' dtype when
+ * every image is contiguous (pitch == width) with 16-byte aligned data (a
+ * vectorised pass over the flat images), else -1 (row by row). */
 int piv_images_min(const piv_image* images, int32_t nimages, float* out_min,
-                   void* stream);
+                   int32_t flat_dtype, void* stream);
 
 /* Fused crop -> mean removal -> FFT cross-correlation -> normalisation ->
  * masked peak search (second chance, secondary peaks) -> optional sub-pixel
