@@ -89,8 +89,9 @@ __device__ __forceinline__ double tps_phi(double r2) {
 __device__ __forceinline__ double tps_r2(double dx, double dy) { return fma(dy, dy, dx * dx); }
 
 // The kernel function at offsets that are multiples of a lattice step:
-// table[(a_max + 1) a + b] = tps_phi(tps_r2(a step, b step)) for
-// 0 <= a, b <= a_max. Where both |dx| and |dy| are exactly a step, b step
+// table[(a_max + 1) b + a] = tps_phi(tps_r2(a step, b step)) for
+// 0 <= a, b <= a_max (a from |dx|, b from |dy|: neighbouring points along a
+// row read neighbouring entries). Where both |dx| and |dy| are exactly a step, b step
 // with a, b <= a_max, the table entry is the value the direct evaluation
 // gives (same operands, same expression; the signs drop out of the
 // squares); any other difference is evaluated directly.
@@ -111,7 +112,7 @@ __device__ __forceinline__ double tps_phi_at(double dx, double dy, const PhiLatt
     const double fa = rint(ax * lat.inv_step), fb = rint(ay * lat.inv_step);
     if (fa <= double(lat.a_max) && fb <= double(lat.a_max) && fa * lat.step == ax &&
         fb * lat.step == ay)
-      return table[int(fa) * (lat.a_max + 1) + int(fb)];
+      return table[int(fb) * (lat.a_max + 1) + int(fa)];
   }
   return tps_phi(tps_r2(dx, dy));
 }
@@ -123,7 +124,7 @@ __device__ __forceinline__ double tps_phi_on_lattice(double dx, double dy, const
                                                     const double* table) {
   const int a = min(int(rint(fabs(dx) * lat.inv_step)), lat.a_max);
   const int b = min(int(rint(fabs(dy) * lat.inv_step)), lat.a_max);
-  return table[a * (lat.a_max + 1) + b];
+  return table[b * (lat.a_max + 1) + a];
 }
 
 // the lattice table in LDS at `lds` when it was sized for it (else the
@@ -138,7 +139,7 @@ __global__ void k_tps_phi_table(double step, int a_max, double* __restrict__ tab
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int w = a_max + 1;
   if (e >= w * w) return;
-  const int a = e / w, b = e % w;
+  const int b = e / w, a = e % w;
   table[e] = tps_phi(tps_r2(double(a) * step, double(b) * step));
 }
 

@@ -375,7 +375,7 @@ int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
 int64_t piv_tps_interpolate_workspace(int32_t npts, int32_t nd);
 
 /* Table of the kernel function r^2 ln(r^2) / 2 on a lattice of step
- * `step`: table[(a_max + 1) a + b] at r^2 = (a step)^2 + (b step)^2,
+ * `step`: table[(a_max + 1) b + a] at r^2 = (a step)^2 + (b step)^2,
  * 0 <= a, b <= a_max (at most 4095), (a_max + 1)^2 doubles. No counterpart
  * in the reference, which evaluates the logarithm for every pair of points
  * (thin_plate_spline.py:81-94); here it serves piv_tps_smooth and
