@@ -102,6 +102,15 @@ def subdomain_grids(counts, x_min, x_max, y_min, y_max, subdom_size, smoothing_c
     return out
 
 
+def matrix_sizes(sizes):
+    """Doubles of the matrix region of subdomains of `sizes` centers (as
+    piv_tps_matrix_size): (n-3) x 3 then (n-3) rows of the factorised matrix,
+    both padded to multiples of 16 doubles (128-byte rows)."""
+    m = np.maximum(np.asarray(sizes, dtype=np.int64) - 3, 0)
+    pad = lambda v: (v + 15) // 16 * 16  # noqa: E731
+    return pad(3 * m) + m * pad(m)
+
+
 def _p(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
@@ -382,7 +391,7 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
                              new_x, new_y, thr, dev, sp, keep, own, lattice, defer, stream)
     n_max = int(sizes.max())
     dom_off_h = np.concatenate([[0], np.cumsum(sizes)])
-    mat_off_h = np.concatenate([[0], np.cumsum(sizes**2)])
+    mat_off_h = np.concatenate([[0], np.cumsum(matrix_sizes(sizes))])
     nslots = int(dom_off_h[-1])
     dom_off, mat_off = _upload([dom_off_h.astype(np.int64), mat_off_h.astype(np.int64)], dev)
     slot_pt = torch.empty(nslots, **i32)

@@ -88,6 +88,19 @@ __device__ __forceinline__ double tps_phi(double r2) {
 // and the direct evaluation round alike)
 __device__ __forceinline__ double tps_r2(double dx, double dy) { return fma(dy, dy, dx * dx); }
 
+// Matrix region of a subdomain of n centers (m = n - 3): E = B'[3:, :3]
+// (m x 3, row-major), then C = B'[3:, 3:] with rows of tps_mat_ld(m)
+// doubles (m rounded up to 16) from a 128-byte boundary, the region itself
+// starting on one: a 32-column block of a row of C is two cache lines.
+__host__ __device__ __forceinline__ int64_t tps_mat_ld(int m) { return (int64_t(m) + 15) & ~int64_t(15); }
+__host__ __device__ __forceinline__ int64_t tps_mat_e_size(int m) {
+  return (3 * int64_t(m) + 15) & ~int64_t(15);
+}
+__host__ __device__ __forceinline__ int64_t tps_mat_size(int n) {
+  const int m = n > 3 ? n - 3 : 0;
+  return tps_mat_e_size(m) + int64_t(m) * tps_mat_ld(m);
+}
+
 // The kernel function at offsets that are multiples of a lattice step:
 // table[(a_max + 1) b + a] = tps_phi(tps_r2(a step, b step)) for
 // 0 <= a, b <= a_max (a from |dx|, b from |dy|: neighbouring points along a
@@ -198,7 +211,9 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
-  double* B = mats + mat_off[d];
+  double* E = mats + mat_off[d];
+  const int64_t ld = tps_mat_ld(n - 3);
+  double* C = E + tps_mat_e_size(n - 3);
   double* RF = refl + 16 * d;
   double* VW = vwork + 6 * s0;
   // dynamic LDS, nm >= n doubles per array: cy, cx, P (3 columns, then
@@ -309,12 +324,16 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   __syncthreads();  // thread 0 has read R out of P
   for (int i = wave; i < n; i += kWaves) {
     const double yi = cy[i], xi = cx[i];
-    double* Bi = B + int64_t(i) * n;
+    double* Ei = E + 3 * int64_t(i - 3);
+    double* Ci = C + int64_t(i - 3) * ld - 3;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
     for (int j = lane; j < n; j += 64) {
       double b = tps_phi_at(cx[j] - xi, cy[j] - yi, lat, ltab);
       if (i == j) b += smoothing;
-      if (j <= i && (store_full || j < 3)) Bi[j] = b;
+      if (i >= 3) {  // rows 0..2 of B are not used by the solves
+        if (j < 3) Ei[j] = b;
+        else if (store_full && j <= i) Ci[j] = b;
+      }
       r0 += b * VM(0, j);
       r1 += b * VM(1, j);
       r2 += b * VM(2, j);
@@ -431,7 +450,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
-  double* B = mats + mat_off[d];
+  const int m = n - 3;
+  double* E = mats + mat_off[d];
+  const int64_t ld = tps_mat_ld(m);
+  double* A = E + tps_mat_e_size(m);  // C(i, j) = A[i ld + j]
   // C before any update is B'[3:, 3:] without the rank-6 term, i.e. the
   // kernel function of two centers plus s on the diagonal. With `gen` (every
   // center on the lattice of the table) it is generated where it is first
@@ -462,12 +484,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   __shared__ int next_pair;                  // look-ahead work counter
   if (tid == 0) bad = 0;
   // B'[3:, :3] (kept for the solves)
-  for (int e = tid; e < (n - 3) * 3; e += kCholThreads) {
-    const int i = 3 + e / 3, k = e % 3;
-    B[int64_t(i) * n + k] -= wz_corr(VW, i, k);
-  }
-  const int m = n - 3;
-  double* A = B + 3 * int64_t(n) + 3;  // C(i, j) = A[i n + j]
+  for (int e = tid; e < m * 3; e += kCholThreads) E[e] -= wz_corr(VW, 3 + e / 3, e % 3);
   const double* VC = VW + 18;          // V, Z rows of C's index space
   const int nblk = (m + kNB - 1) / kNB;
   // C(r0t:m, Jt) -= [rank-6 term] + L(r0t:m, kb:ke) L(Jt, kb:ke)^T for the
@@ -483,10 +500,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       const int ra1 = min(row0 + 16 + li, m - 1);
       const int rb0 = min(r0t + li, m - 1);        // B operand rows: block Jt
       const int rb1 = min(r0t + 16 + li, m - 1);
-      const double* pa0 = A + int64_t(ra0) * n + 4 * lk;
-      const double* pa1 = A + int64_t(ra1) * n + 4 * lk;
-      const double* pb0 = A + int64_t(rb0) * n + 4 * lk;
-      const double* pb1 = A + int64_t(rb1) * n + 4 * lk;
+      const double* pa0 = A + int64_t(ra0) * ld + 4 * lk;
+      const double* pa1 = A + int64_t(ra1) * ld + 4 * lk;
+      const double* pb0 = A + int64_t(rb0) * ld + 4 * lk;
+      const double* pb1 = A + int64_t(rb1) * ld + 4 * lk;
       // the current values of the strip pair's block (rows 4 v + lk (+16),
       // columns li (+16)), requested with the first operands: this
       // wavefront alone writes them during the call
@@ -500,7 +517,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             cur[h][v][0] = fresh(row, r0t + min(li, bst - 1));
             cur[h][v][1] = fresh(row, r0t + min(16 + li, bst - 1));
           } else {
-            const double* src = A + int64_t(row) * n + r0t;
+            const double* src = A + int64_t(row) * ld + r0t;
             cur[h][v][0] = src[min(li, bst - 1)];
             cur[h][v][1] = src[min(16 + li, bst - 1)];
           }
@@ -563,7 +580,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         for (int v = 0; v < 4; ++v) {
           const int row = row0 + 16 * h + 4 * v + lk;
           if (row < m) {
-            double* dst = A + int64_t(row) * n + r0t;
+            double* dst = A + int64_t(row) * ld + r0t;
             const int col0 = li, col1 = 16 + li;
             const double u0 = h ? c10[v] : c00[v];
             const double u1 = h ? c11[v] : c01[v];
@@ -684,7 +701,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       if (dl < bs) {
         // L_JJ (lower with the diagonal) and inv(L_JJ)^T (strictly upper)
         // into the matrix, row `dl`
-        double* blk = A + int64_t(r0 + dl) * n + r0;
+        double* blk = A + int64_t(r0 + dl) * ld + r0;
         for (int c = 0; c < bs; ++c) blk[c] = c <= dl ? lbuf[dl][c] : inv[c][dl];
       }
     }
@@ -714,8 +731,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
       // the update's right operand (block row J) is the same for every strip
       double ob0[8], ob1[8];
       if (J > 0) {
-        const double* pb0 = A + int64_t(r0 + li) * n + kb + 4 * lk;
-        const double* pb1 = A + int64_t(r0 + 16 + li) * n + kb + 4 * lk;
+        const double* pb0 = A + int64_t(r0 + li) * ld + kb + 4 * lk;
+        const double* pb1 = A + int64_t(r0 + 16 + li) * ld + kb + 4 * lk;
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -745,13 +762,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             cur[v] = fresh(row, r0 + li);
             cur[4 + v] = fresh(row, r0 + 16 + li);
           } else {
-            const double* src = A + int64_t(row) * n + r0;
+            const double* src = A + int64_t(row) * ld + r0;
             cur[v] = src[li];
             cur[4 + v] = src[16 + li];
           }
         }
         if (J > 0) {
-          const double* pa = A + int64_t(ra) * n + kb + 4 * lk;
+          const double* pa = A + int64_t(ra) * ld + kb + 4 * lk;
 #pragma unroll
           for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -811,7 +828,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         for (int v = 0; v < 4; ++v) {
           const int row = row0 + 4 * v + lk;
           if (row < m) {
-            double* dst = A + int64_t(row) * n + r0;
+            double* dst = A + int64_t(row) * ld + r0;
             dst[li] = c0[v];
             dst[16 + li] = c1[v];
           }
@@ -857,15 +874,15 @@ __device__ void apply_q(double* x0, double* x1, int n, const double* VW, const d
 constexpr int kBatch = 16;
 
 // the diagonal block J of L into LDS (zero outside the bs x bs block)
-__device__ __forceinline__ void load_diag_block(double (*dblk)[kNB + 1], const double* A, int n,
-                                                int r0, int bs) {
+__device__ __forceinline__ void load_diag_block(double (*dblk)[kNB + 1], const double* A,
+                                                int64_t ld, int r0, int bs) {
   double v[kNB * kNB / kThreads];
 #pragma unroll
   for (int t = 0; t < kNB * kNB / kThreads; ++t) {
     const int e = threadIdx.x + kThreads * t;
     const int i = e / kNB, c = e % kNB;
     const int ii = min(i, bs - 1), cc = min(c, bs - 1);
-    v[t] = A[int64_t(r0 + ii) * n + r0 + cc];
+    v[t] = A[int64_t(r0 + ii) * ld + r0 + cc];
   }
 #pragma unroll
   for (int t = 0; t < kNB * kNB / kThreads; ++t) {
@@ -875,7 +892,7 @@ __device__ __forceinline__ void load_diag_block(double (*dblk)[kNB + 1], const d
   }
 }
 
-__device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m) {
+__device__ void chol_solve(double* xa, double* xb, const double* A, int64_t ld, int m) {
   static_assert(kThreads == 256 && kNB == 32, "thread layout of chol_solve");
   __shared__ double dblk[kNB][kNB + 1];
   __shared__ double tmp[2][kNB];
@@ -888,14 +905,14 @@ __device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m
   for (int J = 0; J < nblk; ++J) {
     const int r0 = J * kNB;
     const int bs = min(kNB, m - r0);
-    load_diag_block(dblk, A, n, r0, bs);
+    load_diag_block(dblk, A, ld, r0, bs);
     {
       // thread = (row i, part p): columns 32 q + 4 p .. + 3 of row r0 + i,
       // kBatch / 4 blocks q per batch of loads
       const int i = tid >> 3, p = tid & 7;
       double s0 = 0.0, s1 = 0.0;
       if (i < bs) {
-        const double* Lr = A + int64_t(r0 + i) * n + 4 * p;
+        const double* Lr = A + int64_t(r0 + i) * ld + 4 * p;
         constexpr int QB = kBatch / 4;
         for (int q0 = 0; q0 < J; q0 += QB) {
           double l[kBatch];
@@ -946,7 +963,7 @@ __device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m
   for (int J = nblk - 1; J >= 0; --J) {
     const int r0 = J * kNB;
     const int bs = min(kNB, m - r0);
-    load_diag_block(dblk, A, n, r0, bs);
+    load_diag_block(dblk, A, ld, r0, bs);
     {
       // thread = (column c, row group g): rows r0 + bs + g, + 8, ...,
       // kBatch rows per batch of loads
@@ -957,7 +974,7 @@ __device__ void chol_solve(double* xa, double* xb, const double* A, int n, int m
         for (int i0 = r0 + bs + g; i0 < m; i0 += 8 * kBatch) {
           double l[kBatch];
 #pragma unroll
-          for (int u = 0; u < kBatch; ++u) l[u] = Lc[int64_t(min(i0 + 8 * u, m - 1)) * n];
+          for (int u = 0; u < kBatch; ++u) l[u] = Lc[int64_t(min(i0 + 8 * u, m - 1)) * ld];
 #pragma unroll
           for (int u = 0; u < kBatch; ++u) {
             const int i = i0 + 8 * u;
@@ -1009,8 +1026,9 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
   const int m = n - 3;
-  const double* B = mats + mat_off[d];
-  const double* A = B + 3 * int64_t(n) + 3;
+  const double* E = mats + mat_off[d];
+  const int64_t ld = tps_mat_ld(m);
+  const double* A = E + tps_mat_e_size(m);
   const double* RF = refl + 16 * d;
   const double* VW = vwork + 6 * s0;
   const double smoothing = smoothing_d[d];
@@ -1050,13 +1068,13 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
     for (int c = 0; c < 2; ++c)
       for (int k = 0; k < 3; ++k) vt[c][k] = x[c][k];
     __syncthreads();
-    if (m > 0) chol_solve(x[0], x[1], A, n, m);
-    // a = R^-1 (vt - B'[:3, 3:] z), B'[:3, 3 + j] = B'[3 + j][k] (lower)
+    if (m > 0) chol_solve(x[0], x[1], A, ld, m);
+    // a = R^-1 (vt - B'[:3, 3:] z), B'[:3, 3 + j] = B'[3 + j][k] = E[j][k]
     double bz[2][3];
     for (int k = 0; k < 3; ++k) {
       double p0 = 0.0, p1 = 0.0;
       for (int j = tid; j < m; j += kThreads) {
-        const double b = B[int64_t(3 + j) * n + k];
+        const double b = E[3 * int64_t(j) + k];
         p0 += b * x[0][3 + j];
         p1 += b * x[1][3 + j];
       }
@@ -1775,6 +1793,11 @@ int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef, const 
                      static_cast<hipStream_t>(stream), npts, slots, coef, norm, smooth_slots,
                      nslots, out);
   return tps_check("piv_tps_blend");
+}
+
+int64_t piv_tps_matrix_size(int32_t n) {
+  if (n < 0 || n > kMaxN) return -1;
+  return tps_mat_size(n);
 }
 
 int64_t piv_tps_interpolate_workspace(int32_t npts, int32_t nd) {

@@ -263,7 +263,9 @@ typedef struct piv_tps_lattice {
  *      compute_tps_weights (thin_plate_spline.py:168-216) ----
  * Subdomain d holds the slots [dom_off[d], dom_off[d+1]) (at most 2048);
  * slot s is center slot_pt[s] of xs/ys (float64) with values vx/vy
- * (float32). mats: workspace of sum_d n_d^2 doubles at mat_off[d]; refl:
+ * (float32). mats: workspace of sum_d piv_tps_matrix_size(n_d) doubles,
+ * subdomain d at mat_off[d] (offsets multiples of 16 doubles, mats aligned
+ * to 128 bytes); refl:
  * 16 doubles per subdomain; vwork: 6 doubles per slot; slot_xy receives the
  * center coordinates by slot (x[nslots] then y[nslots]). Outputs: the
  * smoothed value of every slot (smooth_slots[2][nslots]), the weights of
@@ -281,6 +283,13 @@ typedef struct piv_tps_lattice {
  * B[:, :3] and the factorisation generates the other elements from the
  * table where it first reads them (the same values; mats still needs its
  * full size for the factor). 0: the build stores the lower triangle. */
+/* Doubles of the matrix region of a subdomain of n centers (no reference
+ * counterpart: the reference's LAPACK call takes a dense n x n array):
+ * B'[3:, :3] as (n-3) x 3, padded to a multiple of 16, then the (n-3) x
+ * (n-3) matrix that is factorised in place, rows padded to a multiple of 16
+ * doubles. -1 for n outside [0, 2048]. */
+int64_t piv_tps_matrix_size(int32_t n);
+
 int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    const double* ys, const float* vx, const float* vy,
                    const int64_t* dom_off, const int32_t* slot_pt,

@@ -123,3 +123,22 @@ def test_tuning_table_is_valid_for_the_library():
         n0, n1 = (int(v) for v in key.split("x"))
         assert wpb in tuning.candidates(tuning.CORRELATE_WPB, n0, n1), key
     assert table["tps_chol_threads"] in tuning.candidates(tuning.TPS_CHOL_THREADS)
+
+
+def test_tps_matrix_size_matches_host_offsets():
+    """The host's TPS matrix offsets use the library's region size: (n-3) x 3
+    then (n-3) rows of (n-3) doubles, each padded to 16 doubles."""
+    import numpy as np
+
+    from fluidimage_amd.calcul.tps import matrix_sizes
+
+    lib = _native.load()
+    ns = np.arange(0, 2049)
+    lib_sizes = np.array([lib.piv_tps_matrix_size(int(n)) for n in ns])
+    assert np.array_equal(lib_sizes, matrix_sizes(ns))
+    m = np.maximum(ns - 3, 0)
+    ld = -(-m // 16) * 16
+    assert np.all(lib_sizes % 16 == 0) and np.all(ld >= m) and np.all(ld - m < 16)
+    assert np.all(lib_sizes >= 3 * m + m * m)
+    assert lib.piv_tps_matrix_size(373) == 1120 + 370 * 384
+    assert lib.piv_tps_matrix_size(-1) < 0 and lib.piv_tps_matrix_size(2049) < 0
