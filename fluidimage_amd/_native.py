@@ -122,6 +122,10 @@ SIGNATURES = {
         [ctypes.POINTER(_P), _I, _I, ctypes.c_double, _P, _P, _I, _I, _I, _I],
     ),
     "piv_plan_destroy": (ctypes.c_int, [_P]),
+    "piv_graph_begin": (ctypes.c_int, [_P]),
+    "piv_graph_end": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "piv_graph_launch": (ctypes.c_int, [_P, _P]),
+    "piv_graph_destroy": (ctypes.c_int, [_P]),
     "piv_images_min": (ctypes.c_int, [_P, _I, _P, _I, _P]),
     "piv_correlate_batch": (
         ctypes.c_int,

@@ -15,7 +15,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 SOURCES = [HERE / "csrc" / "piv_hip.hip", HERE / "csrc" / "piv_tps.hip",
            HERE / "csrc" / "piv_deform.hip", HERE / "csrc" / "piv_refine.hip",
-           HERE / "csrc" / "piv_direct.hip"]
+           HERE / "csrc" / "piv_direct.hip", HERE / "csrc" / "piv_graph.hip"]
 OUTPUT = HERE / "libpiv_hip.so"
 ARCH = os.environ.get("PIV_HIP_ARCH", "gfx950")
 

@@ -449,6 +449,29 @@ int piv_refine_backward(int32_t dtype, const piv_image* images, int32_t npairs,
                         const double* grad_out, double* gdx, double* gdy,
                         void* stream);
 
+/* ---- capture and replay as one HIP graph (no reference counterpart: the
+ *      reference's per-window loop, src/fluidimage/works/piv/singlepass.py:
+ *      262-361, pays its host cost per window; a replayed graph pays the
+ *      launch cost of a whole batch once) ----
+ * The calls issued on `stream` (not NULL) by the capturing thread between
+ * piv_graph_begin and piv_graph_end are recorded, not run. Every compute
+ * entry point can be captured (piv_images_min, piv_make_windows,
+ * piv_correlate_batch, piv_peaks_from_maps, piv_subpix_at, piv_fix,
+ * piv_correlate_direct, piv_peaks_any, piv_subpix_any, piv_deform_images,
+ * piv_add_predictor, piv_refine_*, piv_tps_*): each only launches kernels.
+ * piv_plan_create / piv_plan_destroy allocate and free: call them outside
+ * the capture, as every allocation of the buffers a graph uses. A replay
+ * (piv_graph_launch, on any stream) runs the recorded launches with the
+ * recorded pointers and sizes: change buffer contents between replays, not
+ * addresses or shapes; buffers must outlive the graph. A capture whose
+ * calls failed still has to be ended; piv_graph_end then fails and returns
+ * no graph. */
+typedef struct piv_graph piv_graph;
+int piv_graph_begin(void* stream);
+int piv_graph_end(void* stream, piv_graph** graph);
+int piv_graph_launch(piv_graph* graph, void* stream);
+int piv_graph_destroy(piv_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,3 +142,14 @@ def test_tps_matrix_size_matches_host_offsets():
     assert np.all(lib_sizes >= 3 * m + m * m)
     assert lib.piv_tps_matrix_size(373) == 1120 + 370 * 384
     assert lib.piv_tps_matrix_size(-1) < 0 and lib.piv_tps_matrix_size(2049) < 0
+
+
+def test_graph_argument_checks_without_gpu():
+    """The capture entry points reject what cannot be captured or replayed
+    before any runtime call."""
+    lib = _native.load()
+    assert lib.piv_graph_begin(None) == 1  # PIV_EINVAL: null stream
+    assert "null stream" in lib.piv_last_error().decode()
+    assert lib.piv_graph_end(None, None) == 1
+    assert lib.piv_graph_launch(None, None) == 1
+    assert lib.piv_graph_destroy(None) == 0
