@@ -207,6 +207,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
     double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
     double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm,
     PhiLattice lat, bool store_full) {
+  static_assert(NT >= kThreads && NT % 64 == 0, "block sums over the first kThreads threads");
   constexpr int kWaves = NT / 64;
   const int d = blockIdx.x;
   const int64_t s0 = dom_off[d];
@@ -227,6 +228,10 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   __shared__ double red[NT / 64];
   __shared__ double sh[8];
   const int tid = threadIdx.x;
+  // every block-wide sum takes its terms over the first kThreads threads
+  // (the others add zeros after them): the sums, hence the results, are the
+  // same for every NT, i.e. for every launch size
+  const bool rt = tid < kThreads;
   const double smoothing = smoothing_d[d];
   const double* ltab = stage_lattice(lat, dyn + 8 * nm);
   for (int j = tid; j < n; j += NT) {
@@ -245,7 +250,8 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   double tau[3];
   for (int k = 0; k < 3; ++k) {
     double part = 0.0;
-    for (int j = k + 1 + tid; j < n; j += NT) part += PM(k, j) * PM(k, j);
+    if (rt)
+      for (int j = k + 1 + tid; j < n; j += kThreads) part += PM(k, j) * PM(k, j);
     const double sig = block_sum<NT>(part, red);
     const double alpha = PM(k, k);
     double beta, t, scale;
@@ -270,7 +276,8 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
     // apply H_k to the remaining columns of P
     for (int c = k + 1; c < 3; ++c) {
       double pp = 0.0;
-      for (int j = k + tid; j < n; j += NT) pp += VM(k, j) * PM(c, j);
+      if (rt)
+        for (int j = k + tid; j < n; j += kThreads) pp += VM(k, j) * PM(c, j);
       const double dot = block_sum<NT>(pp, red);
       for (int j = k + tid; j < n; j += NT) PM(c, j) -= t * VM(k, j) * dot;
       __syncthreads();
@@ -278,7 +285,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   }
   // T of the compact WY form (dlarft, forward, columnwise)
   double p01 = 0.0, p02 = 0.0, p12 = 0.0;
-  for (int j = tid; j < n; j += NT) {
+  for (int j = tid; rt && j < n; j += kThreads) {
     p01 += VM(0, j) * VM(1, j);
     p02 += VM(0, j) * VM(2, j);
     p12 += VM(1, j) * VM(2, j);
@@ -351,7 +358,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   }
   __syncthreads();
   // ---- M = V^T Y, Z = Y T - 1/2 V T^T M T ----
-  for (int j = tid; j < n; j += NT) {
+  for (int j = tid; rt && j < n; j += kThreads) {
     const double y0 = PM(0, j), y1 = PM(1, j), y2 = PM(2, j);
     const double v0 = VM(0, j), v1 = VM(1, j), v2 = VM(2, j);
     m9[0] += v0 * y0; m9[1] += v0 * y1; m9[2] += v0 * y2;

@@ -162,3 +162,28 @@ def test_tps_deferred_check_matches_eager():
         assert isinstance(eager[k], dict) and deferred_error(deferred[k]) is None
         assert torch.equal(eager[k]["approx"], deferred[k]["approx"])
         assert torch.equal(eager[k]["smoothed"], deferred[k]["smoothed"])
+
+
+@pytest.mark.parametrize("lattice", [True, False])
+def test_tps_results_independent_of_batch_size(lattice):
+    """A field's TPS results do not depend on the fields batched with it: one
+    field alone (few subdomains: the small-launch widths of the matrix build
+    and factorisation) and the same field among 16 (about 1,300 subdomains:
+    the large-launch widths) give bit-identical outputs."""
+    import torch
+
+    from fluidimage_amd import tuning
+    from fluidimage_amd.calcul.tps import lattice_step, tps_device_batch
+
+    print("case: tps batch-size independence, lattice", lattice, flush=True)
+    dev = torch.device("cuda")
+    fields, xs, ys = tuning._grid_fields(16, (1024, 1024), 8, dev, seed=13)
+    lat = lattice_step(xs, ys) if lattice else None
+    gx, gy = fields[0][2], fields[0][3]
+    many = tps_device_batch(fields, gx, gy, 200, 2.0, 1.5, lattice=lat)
+    for k in (0, 9):
+        one = tps_device_batch([fields[k]], gx, gy, 200, 2.0, 1.5, lattice=lat)[0]
+        for key in ("smoothed", "approx", "stats"):
+            a, b = one[key], many[k][key]
+            assert torch.equal(torch.nan_to_num(a, nan=1e300), torch.nan_to_num(b, nan=1e300)), \
+                (k, key)
