@@ -11,7 +11,10 @@
 // grid's bounding box). The deformed crops are im0(y - v/2, x - u/2) and
 // im1(y + v/2, x + u/2), sampled by Keys cubic convolution (a = -0.5; R. Keys,
 // IEEE TASSP 29(6), 1981) on the image minus its minimum, zero outside the
-// image; positions, weights and sums are FP64. The pixel's predictor does not
+// image; positions, weights and sums are FP64. A workgroup deforms one row:
+// the predictor is first interpolated along y at every grid column (LDS),
+// then along x per pixel; inside the image the minimum is taken out of the
+// sample once, as m times the sum of the 16 weights. The pixel's predictor does not
 // depend on the window, so the images are deformed once (float32) and the
 // windows are cropped from them by the fused correlation.
 #include <hip/hip_runtime.h>
@@ -25,6 +28,7 @@
 namespace {
 
 constexpr int kDeformThreads = 256;
+constexpr int kMaxGridColumns = 4096;  // the row's predictor in LDS: 64 KB
 
 template <int DT>
 __device__ __forceinline__ double pixel(const void* data, int64_t idx);
@@ -61,16 +65,17 @@ __device__ double bicubic(const piv_image& im, double m, double py, double px) {
   const int iy = int(fy) - 1, ix = int(fx) - 1;
   double acc = 0.0;
   if (iy >= 0 && iy <= im.height - 4 && ix >= 0 && ix <= im.width - 4) {
-    // interior: the 16 taps inside the image, the same sums without checks
+    // interior: the 16 taps inside the image, sum w p - m sum w
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const int64_t row = int64_t(iy + a) * im.pitch + ix;
       double ra = 0.0;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) ra += wx[b] * (pixel<DT>(im.data, row + b) - m);
+      for (int b = 0; b < 4; ++b) ra += wx[b] * pixel<DT>(im.data, row + b);
       acc += wy[a] * ra;
     }
-    return acc;
+    const double sw = ((wy[0] + wy[1]) + (wy[2] + wy[3])) * ((wx[0] + wx[1]) + (wx[2] + wx[3]));
+    return acc - m * sw;
   }
   for (int a = 0; a < 4; ++a) {
     const int yy = iy + a;
@@ -121,17 +126,21 @@ __global__ __launch_bounds__(kDeformThreads) void k_deform_images(
   const int jy = g.cell_y[y];
   const double ty = g.t_y[y];
   const int jy1 = min(jy + 1, g.ny - 1);
+  // the predictor interpolated along y at every grid column of this row
+  extern __shared__ double rowuv[];  // u then v, g.nx each
+  for (int j = threadIdx.x; j < g.nx; j += kDeformThreads) {
+    const int64_t i0 = int64_t(jy) * g.nx + j, i1 = int64_t(jy1) * g.nx + j;
+    rowuv[j] = (1.0 - ty) * U[i0] + ty * U[i1];
+    rowuv[g.nx + j] = (1.0 - ty) * V[i0] + ty * V[i1];
+  }
+  __syncthreads();
   for (int x = threadIdx.x; x < W; x += kDeformThreads) {
-    // bilinear predictor at the pixel
+    // bilinear predictor at the pixel: along x between the row's values
     const int jx = g.cell_x[x];
     const double tx = g.t_x[x];
     const int jx1 = min(jx + 1, g.nx - 1);
-    const int64_t i00 = int64_t(jy) * g.nx + jx, i01 = int64_t(jy) * g.nx + jx1;
-    const int64_t i10 = int64_t(jy1) * g.nx + jx, i11 = int64_t(jy1) * g.nx + jx1;
-    const double uu = (1.0 - ty) * ((1.0 - tx) * U[i00] + tx * U[i01]) +
-                      ty * ((1.0 - tx) * U[i10] + tx * U[i11]);
-    const double vv = (1.0 - ty) * ((1.0 - tx) * V[i00] + tx * V[i01]) +
-                      ty * ((1.0 - tx) * V[i10] + tx * V[i11]);
+    const double uu = (1.0 - tx) * rowuv[jx] + tx * rowuv[jx1];
+    const double vv = (1.0 - tx) * rowuv[g.nx + jx] + tx * rowuv[g.nx + jx1];
     out0[x] = float(bicubic<DT>(im0, m0, y - 0.5 * vv, x - 0.5 * uu));
     out1[x] = float(bicubic<DT>(im1, m1, y + 0.5 * vv, x + 0.5 * uu));
   }
@@ -172,22 +181,25 @@ int piv_deform_images(int32_t dtype, const piv_image* images, int32_t npairs,
   if (!grid->iyvecs || !grid->ixvecs || !grid->cell_y || !grid->t_y || !grid->cell_x ||
       !grid->t_x)
     return deform_fail(PIV_EINVAL, "NULL grid table");
+  if (grid->nx > kMaxGridColumns)
+    return deform_fail(PIV_EINVAL, "deformation grid wider than 4096 nodes");
   const int nvec = grid->ny * grid->nx;
+  const size_t lds = size_t(2) * grid->nx * sizeof(double);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const dim3 gr(static_cast<unsigned>(int64_t(height) * npairs));
   const dim3 bl(kDeformThreads);
   const int64_t stride = int64_t(height) * width;
   switch (dtype) {
     case PIV_U8:
-      hipLaunchKernelGGL(k_deform_images<PIV_U8>, gr, bl, 0, s, images, *grid, u, v, nvec, out,
+      hipLaunchKernelGGL(k_deform_images<PIV_U8>, gr, bl, lds, s, images, *grid, u, v, nvec, out,
                          stride, int(height));
       break;
     case PIV_U16:
-      hipLaunchKernelGGL(k_deform_images<PIV_U16>, gr, bl, 0, s, images, *grid, u, v, nvec, out,
+      hipLaunchKernelGGL(k_deform_images<PIV_U16>, gr, bl, lds, s, images, *grid, u, v, nvec, out,
                          stride, int(height));
       break;
     case PIV_F32:
-      hipLaunchKernelGGL(k_deform_images<PIV_F32>, gr, bl, 0, s, images, *grid, u, v, nvec, out,
+      hipLaunchKernelGGL(k_deform_images<PIV_F32>, gr, bl, lds, s, images, *grid, u, v, nvec, out,
                          stride, int(height));
       break;
     default:
