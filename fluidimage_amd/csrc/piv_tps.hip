@@ -199,14 +199,17 @@ __device__ double block_max(double v, double* red) {
 // ---------------------------------------------------------------------------
 // 1. Householder QR of P, matrix B = K + sI, Y = B V, Z
 // ---------------------------------------------------------------------------
-template <int NT>
+// GEN: every center on the lattice of the table, which covers the
+// differences within a subdomain (the table entry without the lattice
+// check); only B[:, :3] is stored, the factorisation generates the rest
+template <int NT, bool GEN>
 __global__ __launch_bounds__(NT) void k_tps_prep(
     const double* __restrict__ xs, const double* __restrict__ ys,
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
     const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
     double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
     double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm,
-    PhiLattice lat, bool store_full) {
+    PhiLattice lat) {
   static_assert(NT >= kThreads && NT % 64 == 0, "block sums over the first kThreads threads");
   constexpr int kWaves = NT / 64;
   const int d = blockIdx.x;
@@ -321,9 +324,9 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   //      all columns j (B_ij = B_ji bit for bit: the kernel function sees
   //      the squares of the differences), so that Y_i is a wavefront
   //      reduction and nothing is scattered. The lower triangle is stored,
-  //      or (store_full false) only B[:, :3], the projection terms of the
-  //      solves, when the factorisation generates the other elements with
-  //      the same values where it first reads them.
+  //      or (GEN) only B[:, :3], the projection terms of the solves, when
+  //      the factorisation generates the other elements with the same
+  //      values where it first reads them.
   //      Y goes to the P slots (P is no longer used: R and the reflectors
   //      are saved above) ----
   double m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -335,11 +338,12 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
     double* Ci = C + int64_t(i - 3) * ld - 3;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
     for (int j = lane; j < n; j += 64) {
-      double b = tps_phi_at(cx[j] - xi, cy[j] - yi, lat, ltab);
+      double b = GEN ? tps_phi_on_lattice(cx[j] - xi, cy[j] - yi, lat, ltab)
+                     : tps_phi_at(cx[j] - xi, cy[j] - yi, lat, ltab);
       if (i == j) b += smoothing;
       if (i >= 3) {  // rows 0..2 of B are not used by the solves
         if (j < 3) Ei[j] = b;
-        else if (store_full && j <= i) Ci[j] = b;
+        else if (!GEN && j <= i) Ci[j] = b;
       }
       r0 += b * VM(0, j);
       r1 += b * VM(1, j);
@@ -1710,14 +1714,16 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
   const PhiLattice lat = make_lattice(lattice);
   const bool gen = generated != 0 && lat.table != nullptr;
   const size_t prep_lds = size_t(8) * n_max * sizeof(double) + size_t(lat.lds_words) * sizeof(double);
-  if (nd <= cu_count())
-    hipLaunchKernelGGL(k_tps_prep<1024>, dim3(nd), dim3(1024), prep_lds, s, xs, ys, dom_off,
-                       slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots, status,
-                       int(n_max), lat, !gen);
-  else
-    hipLaunchKernelGGL(k_tps_prep<kThreads>, dim3(nd), dim3(kThreads), prep_lds, s, xs, ys,
-                       dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots,
-                       status, int(n_max), lat, !gen);
+  const bool wide = nd <= cu_count();
+#define PIV_PREP(NT, GEN)                                                                    \
+  hipLaunchKernelGGL((k_tps_prep<NT, GEN>), dim3(nd), dim3(NT), prep_lds, s, xs, ys, dom_off, \
+                     slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots, status,   \
+                     int(n_max), lat)
+  if (wide && gen) PIV_PREP(1024, true);
+  else if (wide) PIV_PREP(1024, false);
+  else if (gen) PIV_PREP(kThreads, true);
+  else PIV_PREP(kThreads, false);
+#undef PIV_PREP
   // 8 wavefronts per subdomain (one workgroup per CU) or 4 (two per CU).
   // Up to two subdomains per CU the launch is one round either way if the
   // width fits: 8 wavefronts up to one subdomain per CU, 4 up to two; beyond,
