@@ -130,6 +130,11 @@ __device__ __forceinline__ double tps_phi_at(double dx, double dy, const PhiLatt
   return tps_phi(tps_r2(dx, dy));
 }
 
+// a coordinate on the lattice (the test tps_phi_at applies to differences)
+__device__ __forceinline__ bool on_lattice(double v, const PhiLattice& lat) {
+  return rint(fabs(v) * lat.inv_step) * lat.step == fabs(v);
+}
+
 // the table entry of a difference known to lie on the lattice (every
 // center a multiple of the step, differences within a_max): the index
 // tps_phi_at computes, without its check (clamped for safety)
@@ -1585,25 +1590,68 @@ __global__ __launch_bounds__(256) void k_tps_eval_dom(
   const double* W0 = weights + s0 + 3 * int64_t(d);
   const double* W1 = W0 + wstride;
   extern __shared__ double2 cen[];  // (x, y) then (w0, w1) of center j at 2 j, 2 j + 1
+  __shared__ double box[4][kThreads / 64];  // bounding box of the centers, by wavefront
   const double* ltab = stage_lattice(lat, reinterpret_cast<double*>(cen + 2 * nmax));
+  // with a table: are all centers on its lattice, and where are they
+  bool on = ltab != nullptr;
+  double bx0 = INFINITY, bx1 = -INFINITY, by0 = INFINITY, by1 = -INFINITY;
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    cen[2 * j] = make_double2(slot_xy[s0 + j], slot_xy[nslots + s0 + j]);
+    const double cx = slot_xy[s0 + j], cy = slot_xy[nslots + s0 + j];
+    cen[2 * j] = make_double2(cx, cy);
     cen[2 * j + 1] = make_double2(W0[j], W1[j]);
+    on = on && on_lattice(cx, lat) && on_lattice(cy, lat);
+    bx0 = fmin(bx0, cx);
+    bx1 = fmax(bx1, cx);
+    by0 = fmin(by0, cy);
+    by1 = fmax(by1, cy);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    bx0 = fmin(bx0, __shfl_xor(bx0, off, 64));
+    bx1 = fmax(bx1, __shfl_xor(bx1, off, 64));
+    by0 = fmin(by0, __shfl_xor(by0, off, 64));
+    by1 = fmax(by1, __shfl_xor(by1, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    box[0][threadIdx.x >> 6] = bx0;
+    box[1][threadIdx.x >> 6] = bx1;
+    box[2][threadIdx.x >> 6] = by0;
+    box[3][threadIdx.x >> 6] = by1;
   }
   const double a0 = W0[n], a1 = W0[n + 1], a2 = W0[n + 2];
   const double b0 = W1[n], b1 = W1[n + 1], b2 = W1[n + 2];
-  __syncthreads();
+  const bool all_on = __syncthreads_and(on);
+  for (int k = 0; k < kThreads / 64; ++k) {
+    bx0 = fmin(bx0, box[0][k]);
+    bx1 = fmax(bx1, box[1][k]);
+    by0 = fmin(by0, box[2][k]);
+    by1 = fmax(by1, box[3][k]);
+  }
+  const double reach = all_on ? double(lat.a_max) * lat.step : -1.0;
   for (int t = first + int(threadIdx.x); t < e1; t += step) {
     const int e = list[t];
     const int p = e / kMaxSlots;
     const double x = px[p], y = py[p];
     double v0 = 0.0, v1 = 0.0;
-    for (int j = 0; j < n; ++j) {
-      const double2 c = cen[2 * j];
-      const double2 w = cen[2 * j + 1];
-      const double ph = tps_phi_at(x - c.x, y - c.y, lat, ltab);
-      v0 += w.x * ph;
-      v1 += w.y * ph;
+    // a point on the lattice within the table's reach of every center: the
+    // table entry of each difference without the check (the entry the check
+    // would find), else the checked lookup
+    if (all_on && on_lattice(x, lat) && on_lattice(y, lat) && x - bx0 <= reach &&
+        bx1 - x <= reach && y - by0 <= reach && by1 - y <= reach) {
+      for (int j = 0; j < n; ++j) {
+        const double2 c = cen[2 * j];
+        const double2 w = cen[2 * j + 1];
+        const double ph = tps_phi_on_lattice(x - c.x, y - c.y, lat, ltab);
+        v0 += w.x * ph;
+        v1 += w.y * ph;
+      }
+    } else {
+      for (int j = 0; j < n; ++j) {
+        const double2 c = cen[2 * j];
+        const double2 w = cen[2 * j + 1];
+        const double ph = tps_phi_at(x - c.x, y - c.y, lat, ltab);
+        v0 += w.x * ph;
+        v1 += w.y * ph;
+      }
     }
     v0 += a0 + a1 * y + a2 * x;
     v1 += b0 + b1 * y + b2 * x;
