@@ -78,16 +78,20 @@ def subdomain_grids(counts, x_min, x_max, y_min, y_max, subdom_size, smoothing_c
         nb_subdom = counts / subdom_size
         nsx = np.floor(np.sqrt(nb_subdom / aspect_ratio))
         nsy = np.ceil(np.sqrt(nb_subdom * aspect_ratio))
-    out = [SingularSystemError("degenerate set of centers") if b else None for b in bad]
+    out = [SingularSystemError("degenerate set of centers") if b else None
+           for b in bad.tolist()]
     ok = ~bad
     nsx = np.where(ok & (nsx > 0), nsx, 1).astype(np.int64)
     nsy = np.where(ok & (nsy > 0), nsy, 1).astype(np.int64)
     coef_buffer = percent_buffer_area / 100
-    keys = {}
-    for k in np.flatnonzero(ok):
-        keys.setdefault((int(nsx[k]), int(nsy[k])), []).append(k)
-    for (sx, sy), ks in keys.items():
-        ks = np.asarray(ks)
+    idx_ok = np.flatnonzero(ok)
+    if idx_ok.size == 0:
+        return out
+    # fields grouped by their (nsx, nsy): one vectorised pass per group
+    keys, inverse = np.unique(nsx[idx_ok] * (1 << 32) + nsy[idx_ok], return_inverse=True)
+    for g, key in enumerate(keys.tolist()):
+        sx, sy = key >> 32, key & 0xFFFFFFFF
+        ks = idx_ok[inverse.ravel() == g]
         x_dom = np.linspace(x_min[ks], x_max[ks], sx + 1, axis=1)
         y_dom = np.linspace(y_min[ks], y_max[ks], sy + 1, axis=1)
         num_vectors_per_subdom = counts[ks] / (sx * sy)
@@ -97,8 +101,8 @@ def subdomain_grids(counts, x_min, x_max, y_min, y_max, subdom_size, smoothing_c
         by = (coef_buffer * range_y[ks] / sy)[:, None]
         lim = np.concatenate([x_dom[:, :-1] - bx, x_dom[:, 1:] + bx, y_dom[:, :-1] - by,
                               y_dom[:, 1:] + by], axis=1)
-        for j, k in enumerate(ks):
-            out[k] = (sx, sy, float(smoothing[j]), lim[j])
+        for k, sm, row in zip(ks.tolist(), smoothing.tolist(), lim):
+            out[k] = (sx, sy, sm, row)
     return out
 
 
@@ -241,7 +245,16 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
 
     def stacked(i):
         t = [f[i] for f in fields]
-        return t[0].contiguous() if len(t) == 1 else torch.cat(t)
+        if len(t) == 1:
+            return t[0].contiguous()
+        # consecutive slices of one tensor (the pairs of a batch): a view
+        t0 = t[0]
+        if all(x.is_contiguous() and x.dtype == t0.dtype and x.device == t0.device
+               and x.untyped_storage().data_ptr() == t0.untyped_storage().data_ptr()
+               and x.storage_offset() == t0.storage_offset() + k * n
+               for k, x in enumerate(t)):
+            return torch.as_strided(t0, (len(t) * n,), (1,), t0.storage_offset())
+        return torch.cat(t)
 
     dx, dy, xs, ys = (stacked(i) for i in range(4))
     cx = torch.empty(ntot, **f64)
@@ -360,31 +373,33 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     i32 = dict(dtype=torch.int32, device=dev)
     if not active:
         return {}
-    nd_base = 0
-    for f in active:
-        f.nd_base = nd_base
-        nd_base += f.nd
-    nd = nd_base
+    nds = np.array([f.nd for f in active], dtype=np.int64)
+    nd_bases = np.concatenate([[0], np.cumsum(nds)])
+    for f, b in zip(active, nd_bases.tolist()):
+        f.nd_base = b
+    nd = int(nd_bases[-1])
     lim_h = np.concatenate([f.lim_h for f in active])
     lim_off = np.concatenate([[0], np.cumsum([f.lim_h.size for f in active])])
-    table = np.array([[f.nsx, f.nsy, f.count, f.base, f.nd_base, lim_off[j], f.sorted_y]
-                      for j, f in enumerate(active)], dtype=np.int32)
-    dom_field_h = np.concatenate([np.full(f.nd, j, np.int32) for j, f in enumerate(active)])
-    smoothing_h = np.concatenate([np.full(f.nd, f.smoothing) for f in active])
+    cols = [[f.nsx for f in active], [f.nsy for f in active], [f.count for f in active],
+            [f.base for f in active], nd_bases[:-1], lim_off[:-1], [f.sorted_y for f in active]]
+    table = np.array(cols, dtype=np.int32).T.copy()
+    dom_field_h = np.repeat(np.arange(len(active), dtype=np.int32), nds)
+    smoothing_h = np.repeat(np.array([f.smoothing for f in active], dtype=np.float64), nds)
     lim, ftab, dom_field, smoothing = _upload([lim_h, table, dom_field_h, smoothing_h], dev)
     counts = torch.empty(nd, **i32)
     _native.check(lib.piv_tps_count(nd, _p(ftab), _p(dom_field), _p(lim), _p(cx), _p(cy),
                                     _p(counts), sp), "piv_tps_count")
     sizes = counts.cpu().numpy().astype(np.int64)
+    smin = np.minimum.reduceat(sizes, nd_bases[:-1]).tolist()
+    smax = np.maximum.reduceat(sizes, nd_bases[:-1]).tolist()
     retry = False
-    for f in active:
-        sz = sizes[f.nd_base:f.nd_base + f.nd]
-        if sz.min() < 1:
+    for f, lo, hi in zip(active, smin, smax):
+        if lo < 1:
             f.error = SingularSystemError("empty subdomain")
             retry = True
-        elif int(sz.max()) > MAX_CENTERS_PER_SUBDOM:
+        elif hi > MAX_CENTERS_PER_SUBDOM:
             raise NotImplementedError(
-                f"{int(sz.max())} centers in a TPS subdomain (limit {MAX_CENTERS_PER_SUBDOM}): "
+                f"{hi} centers in a TPS subdomain (limit {MAX_CENTERS_PER_SUBDOM}): "
                 "use a smaller params.multipass.subdom_size")
     if retry:
         return _solve_active(lib, [f for f in active if f.error is None], cx, cy, vx, vy,
