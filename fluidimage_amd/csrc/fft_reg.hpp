@@ -204,4 +204,93 @@ __host__ __device__ __forceinline__ void fft(float2 (&x)[N]) {
   }
 }
 
+// ---------------------------------------------------------------- in-place FFTs
+// The same transform on x[O + S j], j = 0..N-1, without copies (one array of
+// N values plus R temporaries live). fft_dif takes the input in natural
+// order and leaves X[k] at j = fft_pos(N, k); fft_dit takes x[n] at
+// j = fft_pos(N, n) and leaves X[k] at j = k. With N = R M (R =
+// pick_radix(N)): fft_pos(N, q + R k') = q M + fft_pos(M, k'), the identity
+// for the base lengths (2, 4, primes).
+__host__ __device__ constexpr int fft_pos(int n, int k) {
+  if (n <= 4 || pick_radix(n) == n) return k;
+  return (k % pick_radix(n)) * (n / pick_radix(n)) + fft_pos(n / pick_radix(n), k / pick_radix(n));
+}
+
+template <int N, int DIR, int O, int S, int L>
+__host__ __device__ __forceinline__ void fft_base_at(float2 (&x)[L]) {
+  float2 t[N];
+  sfor<0, N>([&](auto j_) {
+    constexpr int j = decltype(j_)::value;
+    t[j] = x[O + S * j];
+  });
+  fft<N, DIR>(t);
+  sfor<0, N>([&](auto j_) {
+    constexpr int j = decltype(j_)::value;
+    x[O + S * j] = t[j];
+  });
+}
+
+// decimation in frequency: R-point DFTs over x[m + r M], twiddles
+// W_N^{q m}, then the M-point transforms of the R blocks q M + m
+template <int N, int DIR, int O, int S, int L>
+__host__ __device__ __forceinline__ void fft_dif(float2 (&x)[L]) {
+  if constexpr (N == 1) {
+    return;
+  } else if constexpr (N <= 4 || pick_radix(N) == N) {
+    fft_base_at<N, DIR, O, S, L>(x);
+  } else {
+    constexpr int R = pick_radix(N);
+    constexpr int M = N / R;
+    sfor<0, M>([&](auto m_) {
+      constexpr int m = decltype(m_)::value;
+      float2 t[R];
+      sfor<0, R>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        t[r] = x[O + S * (m + r * M)];
+      });
+      fft<R, DIR>(t);
+      sfor<0, R>([&](auto q_) {
+        constexpr int q = decltype(q_)::value;
+        x[O + S * (m + q * M)] = twmul<q * m, N, DIR>(t[q]);
+      });
+    });
+    sfor<0, R>([&](auto q_) {
+      constexpr int q = decltype(q_)::value;
+      fft_dif<M, DIR, O + S * q * M, S, L>(x);
+    });
+  }
+}
+
+// decimation in time: the M-point transforms of the R blocks (block q holds
+// the sub-sequence x[m R + q]), then twiddles W_N^{q k} and R-point DFTs
+// over the blocks, X[k + r M] at k + r M
+template <int N, int DIR, int O, int S, int L>
+__host__ __device__ __forceinline__ void fft_dit(float2 (&x)[L]) {
+  if constexpr (N == 1) {
+    return;
+  } else if constexpr (N <= 4 || pick_radix(N) == N) {
+    fft_base_at<N, DIR, O, S, L>(x);
+  } else {
+    constexpr int R = pick_radix(N);
+    constexpr int M = N / R;
+    sfor<0, R>([&](auto q_) {
+      constexpr int q = decltype(q_)::value;
+      fft_dit<M, DIR, O + S * q * M, S, L>(x);
+    });
+    sfor<0, M>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      float2 t[R];
+      sfor<0, R>([&](auto q_) {
+        constexpr int q = decltype(q_)::value;
+        t[q] = twmul<q * k, N, DIR>(x[O + S * (q * M + k)]);
+      });
+      fft<R, DIR>(t);
+      sfor<0, R>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        x[O + S * (r * M + k)] = t[r];
+      });
+    });
+  }
+}
+
 }  // namespace fia

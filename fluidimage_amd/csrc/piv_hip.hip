@@ -88,15 +88,14 @@ struct Shape {
   static constexpr int NN = N0 * N1;
 };
 
-// rotated LDS layout of the complex N0 x N1 buffer: element (r, c) lives at
-// r * N1 + (c + r) mod N1, so both the column pass (lanes over c) and the
-// row pass (lanes over r) touch distinct banks
+// padded LDS layout of the complex N0 x N1 buffer: element (r, c) lives at
+// r * (N1 + 1) + c. The column pass (lanes over c) reads consecutive
+// elements; in the row pass (lanes over r) lane r starts 2 (N1 + 1) dwords
+// after lane r - 1, an even number coprime to 32 halves, so the 64-bit
+// accesses of 32 lanes cover distinct bank pairs. Every access is one base
+// register plus a compile-time offset.
 template <int N1>
-__device__ __forceinline__ int rot(int c, int r) {
-  int k = c + r;
-  while (k >= N1) k -= N1;
-  return k;
-}
+constexpr int kLd = N1 + 1;
 
 template <int N0, int N1, int DT, int WPB>
 __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correlate(
@@ -105,7 +104,9 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   constexpr int T = Shape<N0, N1>::T;
   const int apply_subpix = options & PIV_OPT_SUBPIX;
   constexpr int NN = N0 * N1;
-  __shared__ float2 smem[WPB * NN];
+  constexpr int LD = kLd<N1>, NB = N0 * LD;  // padded buffer of one window
+  static_assert(NB >= NN, "the map and its search copy (2 NN floats) fit in the buffer");
+  __shared__ float2 smem[WPB * NB];
   __shared__ uint64_t scol[N1];  // bit i0 of scol[i1]: where_large_displacement[i0, i1]
 
   const int lane = threadIdx.x & (T - 1);
@@ -121,7 +122,7 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
     }
   }
 
-  float2* buf = smem + wl * NN;
+  float2* buf = smem + wl * NB;
   piv_window w;
   const bool live = win < nwin;
   if (live) {
@@ -193,37 +194,46 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
     }
   }
   // ---- 3. forward 2-D FFT of z = im0' + i im1' -------------------------
-  fft<N0, -1>(z);
+  // (in place: the forward transforms leave frequency k at fft_pos(n, k),
+  // the inverse ones take it there; the loops below index accordingly)
+  fft_dif<N0, -1, 0, 1, N0>(z);
   if (lane < N1) {
-#pragma unroll
-    for (int r = 0; r < N0; ++r) buf[r * N1 + rot<N1>(lane, r)] = z[r];
+    sfor<0, N0>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      constexpr int pk = fft_pos(N0, k);
+      buf[k * LD + lane] = z[pk];
+    });
   }
   __syncthreads();
   float2 y[N1];
   const int r = lane;  // row owned in the row pass
   if (r < N0) {
 #pragma unroll
-    for (int c = 0; c < N1; ++c) y[c] = buf[r * N1 + rot<N1>(c, r)];
+    for (int c = 0; c < N1; ++c) y[c] = buf[r * LD + c];
   } else {
 #pragma unroll
     for (int c = 0; c < N1; ++c) y[c] = make_float2(0.f, 0.f);
   }
-  fft<N1, -1>(y);
+  fft_dif<N1, -1, 0, 1, N1>(y);
   __syncthreads();
   if (r < N0) {
-#pragma unroll
-    for (int c = 0; c < N1; ++c) buf[r * N1 + rot<N1>(c, r)] = y[c];
+    sfor<0, N1>([&](auto c_) {
+      constexpr int c = decltype(c_)::value;
+      constexpr int pc = fft_pos(N1, c);
+      buf[r * LD + c] = y[pc];
+    });
   }
   __syncthreads();
   // ---- 4. F0, F1 from Z(k), Z(-k); energies; cross spectrum ---------------
   float e0f = 0.f, e1f = 0.f;
   if (r < N0) {
     const int pr = r == 0 ? 0 : N0 - r;
-#pragma unroll
-    for (int c = 0; c < N1; ++c) {
-      const int cm = c == 0 ? 0 : N1 - c;
-      const float2 zm = buf[pr * N1 + rot<N1>(cm, pr)];
-      const float2 a = y[c];
+    sfor<0, N1>([&](auto c_) {
+      constexpr int c = decltype(c_)::value;
+      constexpr int cm = c == 0 ? 0 : N1 - c;
+      constexpr int pc = fft_pos(N1, c);
+      const float2 zm = buf[pr * LD + cm];
+      const float2 a = y[pc];
       const float2 b = make_float2(zm.x, -zm.y);
       float2 f0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
       const float2 d = make_float2(a.x - b.x, a.y - b.y);
@@ -238,25 +248,28 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
         e1f += wgt * (f1.x * f1.x + f1.y * f1.y);
       }
       // conj(f0) * f1
-      y[c] = make_float2(f0.x * f1.x + f0.y * f1.y, f0.x * f1.y - f0.y * f1.x);
-    }
+      y[pc] = make_float2(f0.x * f1.x + f0.y * f1.y, f0.x * f1.y - f0.y * f1.x);
+    });
   }
   const double e0 = group_sum(double(e0f), T);
   const double e1 = group_sum(double(e1f), T);
   const double norm = sqrt(e0 * e1);
   // ---- 5. inverse 2-D FFT (rows, then columns) ------------------------
-  fft<N1, +1>(y);
+  fft_dit<N1, +1, 0, 1, N1>(y);
   __syncthreads();
   if (r < N0) {
 #pragma unroll
-    for (int c = 0; c < N1; ++c) buf[r * N1 + rot<N1>(c, r)] = y[c];
+    for (int c = 0; c < N1; ++c) buf[r * LD + c] = y[c];
   }
   __syncthreads();
   if (lane < N1) {
-#pragma unroll
-    for (int rr = 0; rr < N0; ++rr) z[rr] = buf[rr * N1 + rot<N1>(lane, rr)];
+    sfor<0, N0>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      constexpr int pk = fft_pos(N0, k);
+      z[pk] = buf[k * LD + lane];
+    });
   }
-  fft<N0, +1>(z);
+  fft_dit<N0, +1, 0, 1, N0>(z);
   __syncthreads();
   // ---- 6. map in the reference layout: arr[a0, a1] -> out[g(a0), g(a1)],
   //         g(a) = (n - 1 - a + n // 2) mod n  (inverse of _like_fftshift)
