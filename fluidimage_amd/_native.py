@@ -145,7 +145,7 @@ SIGNATURES = {
     "piv_tps_smooth": (
         ctypes.c_int,
         [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P, _P, _P, _P, _P,
-         ctypes.c_int64, _P, ctypes.c_int64, _P, _P, _P, _I, _P],
+         ctypes.c_int64, _P, ctypes.c_int64, _P, _P, _P, _P, _I, _P],
     ),
     "piv_tps_phi_table": (ctypes.c_int, [ctypes.c_double, _I, _P, _P]),
     "piv_tune_candidates": (ctypes.c_int, [_I, _I, _I, _P, _I]),

@@ -408,7 +408,11 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     dom_off_h = np.concatenate([[0], np.cumsum(sizes)])
     mat_off_h = np.concatenate([[0], np.cumsum(matrix_sizes(sizes))])
     nslots = int(dom_off_h[-1])
-    dom_off, mat_off = _upload([dom_off_h.astype(np.int64), mat_off_h.astype(np.int64)], dev)
+    # per-subdomain work launched largest first (its cost grows like n^3):
+    # the last round of a launch is then made of the small subdomains
+    order_h = np.argsort(-sizes, kind="stable").astype(np.int32)
+    dom_off, mat_off, order = _upload([dom_off_h.astype(np.int64), mat_off_h.astype(np.int64),
+                                       order_h], dev)
     slot_pt = torch.empty(nslots, **i32)
     status = torch.zeros(nd + 1, **i32)
     ntot = int(cx.numel())
@@ -448,7 +452,7 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
         lib.piv_tps_smooth(nd, n_max, _p(cx), _p(cy), _p(vx), _p(vy), _p(dom_off), _p(slot_pt),
                            _p(mat_off), _p(smoothing), thr, _p(mats), _p(refl), _p(vwork),
                            _p(slot_xy), _p(smooth_slots), nslots, _p(weights), wstride,
-                           _p(stats), _p(status), lat_p, generated, sp),
+                           _p(stats), _p(status), _p(order), lat_p, generated, sp),
         "piv_tps_smooth")
     smoothed = torch.empty((2, ntot), **f64)
     _native.check(lib.piv_tps_blend(ntot, _p(c_slot), _p(c_coef), _p(c_norm), _p(smooth_slots),
@@ -474,7 +478,8 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
         if stream is not None and stream != torch.cuda.current_stream():
             keep_alive = (mats, refl, vwork, smooth_slots, weights, stats, slot_xy, dom_off,
                           mat_off, slot_pt, c_slot, c_coef, c_norm, n_dom, n_coef, n_norm,
-                          lat_table, iwork, lim, ftab, dom_field, smoothing, counts, px, py)
+                          lat_table, iwork, lim, ftab, dom_field, smoothing, counts, px, py,
+                          order)
         pending = PendingStatus(status, nd, keep_alive)
         st = None
     else:

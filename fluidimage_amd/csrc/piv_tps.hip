@@ -214,10 +214,10 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
     const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
     double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
     double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm,
-    PhiLattice lat) {
+    PhiLattice lat, const int32_t* __restrict__ order) {
   static_assert(NT >= kThreads && NT % 64 == 0, "block sums over the first kThreads threads");
   constexpr int kWaves = NT / 64;
-  const int d = blockIdx.x;
+  const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
   double* E = mats + mat_off[d];
@@ -462,8 +462,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                            const double* __restrict__ slot_xy,
                                                            int64_t nslots,
                                                            const double* __restrict__ smoothing_d,
-                                                           PhiLattice lat, bool gen) {
-  const int d = blockIdx.x;
+                                                           PhiLattice lat, bool gen,
+                                                           const int32_t* __restrict__ order) {
+  const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
   const int m = n - 3;
@@ -1036,8 +1037,8 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
     const double* __restrict__ smoothing_d,
     double threshold, double* __restrict__ smooth_slots, int64_t nslots,
     double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
-    const int32_t* __restrict__ status, int nm) {
-  const int d = blockIdx.x;
+    const int32_t* __restrict__ status, int nm, const int32_t* __restrict__ order) {
+  const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
   if (status[d]) return;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
@@ -1749,8 +1750,8 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
                    const int32_t* slot_pt, const int64_t* mat_off, const double* smoothing,
                    double threshold, double* mats, double* refl, double* vwork, double* slot_xy,
                    double* smooth_slots, int64_t nslots, double* weights, int64_t wstride,
-                   double* stats, int32_t* status, const piv_tps_lattice* lattice,
-                   int32_t generated, void* stream) {
+                   double* stats, int32_t* status, const int32_t* order,
+                   const piv_tps_lattice* lattice, int32_t generated, void* stream) {
   if (nd <= 0) return PIV_OK;
   if (n_max > kMaxN)
     return tps_fail(PIV_EUNSUPPORTED, "more than 2048 centers in a TPS subdomain");
@@ -1766,7 +1767,7 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
 #define PIV_PREP(NT, GEN)                                                                    \
   hipLaunchKernelGGL((k_tps_prep<NT, GEN>), dim3(nd), dim3(NT), prep_lds, s, xs, ys, dom_off, \
                      slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots, status,   \
-                     int(n_max), lat)
+                     int(n_max), lat, order)
   if (wide && gen) PIV_PREP(1024, true);
   else if (wide) PIV_PREP(1024, false);
   else if (gen) PIV_PREP(kThreads, true);
@@ -1784,15 +1785,15 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
   const size_t chol_lds = size_t(lat.lds_words) * sizeof(double);
   if (ct == 512)
     hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), chol_lds, s, dom_off, mat_off, mats,
-                       vwork, status, slot_xy, nslots, smoothing, lat, gen);
+                       vwork, status, slot_xy, nslots, smoothing, lat, gen, order);
   else
     hipLaunchKernelGGL(k_tps_chol<256>, dim3(nd), dim3(256), chol_lds, s, dom_off, mat_off, mats,
-                       vwork, status, slot_xy, nslots, smoothing, lat, gen);
+                       vwork, status, slot_xy, nslots, smoothing, lat, gen, order);
   hipLaunchKernelGGL(k_tps_solve, dim3(nd), dim3(kThreads),
                      size_t(3) * ((n_max + 1) & ~1) * sizeof(double), s, vx, vy, dom_off, slot_pt,
                      mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,
                      weights, wstride, stats, status,
-                     (n_max + 1) & ~1);
+                     (n_max + 1) & ~1, order);
   return tps_check("piv_tps_smooth");
 }
 

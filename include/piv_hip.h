@@ -276,7 +276,11 @@ typedef struct piv_tps_lattice {
  * the device array of the (rescaled) smoothing coefficient of every
  * subdomain, so that the subdomains of several vector fields, each with its
  * own grid, go through one call. threshold < 0 disables the outlier
- * iterations. lattice: see piv_tps_lattice (NULL: direct evaluation).
+ * iterations. order (optional, nd entries): a permutation of the
+ * subdomains giving the order in which the per-subdomain work is launched
+ * (largest first shortens the tail of a launch; results do not depend on
+ * it); NULL: 0..nd-1. lattice: see piv_tps_lattice (NULL: direct
+ * evaluation).
  * generated (with a lattice): every center is a multiple of the lattice
  * step (info[f][6] of piv_tps_select for every field) and the table covers
  * the differences within a subdomain; the matrix build then stores only
@@ -297,7 +301,8 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs,
                    double* mats, double* refl, double* vwork, double* slot_xy,
                    double* smooth_slots, int64_t nslots, double* weights,
                    int64_t wstride, double* stats, int32_t* status,
-                   const piv_tps_lattice* lattice, int32_t generated, void* stream);
+                   const int32_t* order, const piv_tps_lattice* lattice,
+                   int32_t generated, void* stream);
 
 /* One vector field (one image pair) of a TPS batch: its subdomain grid
  * (nsx x nsy, limits at lim + lim_off: lmin_x[nsx], lmax_x[nsx],
