@@ -142,6 +142,33 @@ def make_pair(shape, field, rng, density=0.02, dtype=np.uint8, noise=3.0,
     return im0, im1
 
 
+def _field_and_boxes(config_id, rng, shape):
+    """Displacement field and noise boxes of configs 2 and 3 (first draws of
+    the pair's generator)."""
+    field = fourier_modes(shape, rng)
+    boxes = None
+    if config_id == 3:
+        ny, nx = shape
+        step = 32
+        cand = [(y, x) for y in range(0, ny - 64 + 1, step)
+                for x in range(0, nx - 64 + 1, step)]
+        nsel = max(1, int(0.02 * len(cand)))
+        sel = rng.choice(len(cand), nsel, replace=False)
+        boxes = [(cand[i][0], cand[i][1], 64, 64) for i in sel]
+    return field, boxes
+
+
+def config_truth(config_id, pair_idx=0, shape=None):
+    """(field, boxes): the displacement field ``field(x, y) -> (ux, uy)`` the
+    pair ``config_pair(config_id, pair_idx, shape)`` was rendered with, and the
+    (y0, x0, h, w) boxes of its second image replaced by noise (None).
+    Configs 2 and 3."""
+    if config_id not in (2, 3):
+        raise ValueError("truth fields are kept for configs 2 and 3")
+    rng = np.random.default_rng(SEED_BASE + 1000 * config_id + pair_idx)
+    return _field_and_boxes(config_id, rng, shape or (2048, 2048))
+
+
 def config_pair(config_id, pair_idx=0, shape=None):
     """Seeded pairs of the benchmark workloads.
 
@@ -159,16 +186,7 @@ def config_pair(config_id, pair_idx=0, shape=None):
         return make_pair(shape, lamb_oseen(shape), rng, density=0.02, dtype=np.uint8)
     if config_id in (2, 3):
         shape = shape or (2048, 2048)
-        field = fourier_modes(shape, rng)
-        boxes = None
-        if config_id == 3:
-            ny, nx = shape
-            step = 32
-            cand = [(y, x) for y in range(0, ny - 64 + 1, step)
-                    for x in range(0, nx - 64 + 1, step)]
-            nsel = max(1, int(0.02 * len(cand)))
-            sel = rng.choice(len(cand), nsel, replace=False)
-            boxes = [(cand[i][0], cand[i][1], 64, 64) for i in sel]
+        field, boxes = _field_and_boxes(config_id, rng, shape)
         return make_pair(shape, field, rng, density=0.03, dtype=np.uint16,
                          outlier_windows=boxes)
     if config_id == 5:
