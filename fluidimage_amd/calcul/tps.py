@@ -30,6 +30,10 @@ from .. import _native
 
 MAX_SLOTS = 4
 MAX_CENTERS_PER_SUBDOM = 2048
+# order in which the per-subdomain kernels take the subdomains:
+# "largest_first" (cost grows like n^3: the last round of a launch is made of
+# the small ones) or "natural"; results do not depend on it
+LAUNCH_ORDER = "largest_first"
 MAX_LATTICE_A = 511  # largest lattice table: 512 x 512 doubles (2 MiB)
 
 
@@ -410,9 +414,17 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     nslots = int(dom_off_h[-1])
     # per-subdomain work launched largest first (its cost grows like n^3):
     # the last round of a launch is then made of the small subdomains
-    order_h = np.argsort(-sizes, kind="stable").astype(np.int32)
-    dom_off, mat_off, order = _upload([dom_off_h.astype(np.int64), mat_off_h.astype(np.int64),
-                                       order_h], dev)
+    if LAUNCH_ORDER == "largest_first":
+        order_h = np.argsort(-sizes, kind="stable").astype(np.int32)
+    elif LAUNCH_ORDER == "natural":
+        order_h = None
+    else:
+        raise ValueError(f"unknown LAUNCH_ORDER {LAUNCH_ORDER!r}")
+    uploads = [dom_off_h.astype(np.int64), mat_off_h.astype(np.int64)]
+    if order_h is not None:
+        uploads.append(order_h)
+    dom_off, mat_off, *rest = _upload(uploads, dev)
+    order = rest[0] if rest else None
     slot_pt = torch.empty(nslots, **i32)
     status = torch.zeros(nd + 1, **i32)
     ntot = int(cx.numel())

@@ -187,3 +187,27 @@ def test_tps_results_independent_of_batch_size(lattice):
             a, b = one[key], many[k][key]
             assert torch.equal(torch.nan_to_num(a, nan=1e300), torch.nan_to_num(b, nan=1e300)), \
                 (k, key)
+
+
+def test_tps_launch_order_does_not_change_results(monkeypatch):
+    """The per-subdomain kernels taken largest first (the default) or in
+    natural order (NULL order argument) give bit-identical results."""
+    import torch
+
+    from fluidimage_amd import tuning
+    from fluidimage_amd.calcul import tps
+    from fluidimage_amd.calcul.tps import lattice_step, tps_device_batch
+
+    print("case: tps launch order", flush=True)
+    dev = torch.device("cuda")
+    fields, xs, ys = tuning._grid_fields(4, (1024, 1024), 8, dev, seed=17)
+    lat = lattice_step(xs, ys)
+    gx, gy = fields[0][2], fields[0][3]
+    outs = {}
+    for order in ("largest_first", "natural"):
+        monkeypatch.setattr(tps, "LAUNCH_ORDER", order)
+        outs[order] = tps_device_batch(fields, gx, gy, 200, 2.0, 1.5, lattice=lat)
+    for a, b in zip(outs["largest_first"], outs["natural"]):
+        for key in ("smoothed", "approx", "stats"):
+            assert torch.equal(torch.nan_to_num(a[key], nan=1e300),
+                               torch.nan_to_num(b[key], nan=1e300)), key
