@@ -429,10 +429,18 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     status = torch.zeros(nd + 1, **i32)
     ntot = int(cx.numel())
     # slots of every center of the batch (-1: none, so that the centers
-    # beyond a field's count blend to nothing in the one blend launch below)
-    c_slot = torch.full((ntot * MAX_SLOTS,), -1, **i32)
-    c_coef = torch.zeros(ntot * MAX_SLOTS, **f64)
-    c_norm = torch.ones(ntot, **f64)
+    # beyond a field's count blend to nothing in the one blend launch below):
+    # the centers mode of piv_tps_point_slots writes every entry of an active
+    # field (empty ones beyond its count); the fields left out are prefilled
+    n_field = own[3]
+    if ntot == n_field * len(active):
+        c_slot = torch.empty(ntot * MAX_SLOTS, **i32)
+        c_coef = torch.empty(ntot * MAX_SLOTS, **f64)
+        c_norm = torch.empty(ntot, **f64)
+    else:
+        c_slot = torch.full((ntot * MAX_SLOTS,), -1, **i32)
+        c_coef = torch.zeros(ntot * MAX_SLOTS, **f64)
+        c_norm = torch.ones(ntot, **f64)
     m = int(new_x.numel())
     na = len(active)
     n_dom = torch.empty(na * m * MAX_SLOTS, **i32)
@@ -441,7 +449,7 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     _native.check(lib.piv_tps_fill(nd, _p(ftab), _p(dom_field), _p(lim), _p(cx), _p(cy),
                                    _p(dom_off), _p(slot_pt), sp), "piv_tps_fill")
     _native.check(lib.piv_tps_point_slots(
-        na, _p(ftab), max(f.count for f in active), _p(lim), _p(cx), _p(cy), _p(dom_off),
+        na, _p(ftab), n_field, _p(lim), _p(cx), _p(cy), _p(dom_off),
         _p(slot_pt), _p(c_slot), _p(c_coef), _p(c_norm), _p(status[nd:]), sp),
         "piv_tps_point_slots")
     _native.check(lib.piv_tps_point_slots(

@@ -1395,7 +1395,8 @@ __global__ __launch_bounds__(1024) void k_tps_fill(const piv_tps_field* __restri
 // weights and the sum of the weights. Centers mode (dom_off != NULL): the
 // slot is the position of the point in the member list of the subdomain.
 // field blockIdx.y; centers mode (dom_off != NULL): its centers
-// cx[center_base + p], p < count, outputs at center_base + p; new-position
+// cx[center_base + p], p < count, outputs at center_base + p, and for
+// count <= p < m an empty entry (no slot, weight 0, norm 1); new-position
 // mode: the m positions px[p], outputs at field * m + p
 __global__ void k_tps_point_slots(const piv_tps_field* __restrict__ fields,
                                   const double* __restrict__ lim, int m,
@@ -1407,15 +1408,22 @@ __global__ void k_tps_point_slots(const piv_tps_field* __restrict__ fields,
   const piv_tps_field f = fields[blockIdx.y];
   const SubGrid g = field_grid(f, lim);
   const bool centers = dom_off != nullptr;
-  const int npts = centers ? f.count : m;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npts) return;
+  if (p >= m) return;
   const int64_t in = centers ? int64_t(f.center_base) + p : p;
   const int64_t o = centers ? int64_t(f.center_base) + p : int64_t(blockIdx.y) * m + p;
   const int center_base = f.center_base, dom_base = f.dom_base;
   out_slot += kMaxSlots * o;
   out_coef += kMaxSlots * o;
   out_norm += o;
+  if (centers && p >= f.count) {  // beyond the field's valid vectors
+    for (int k = 0; k < kMaxSlots; ++k) {
+      out_slot[k] = -1;
+      out_coef[k] = 0.0;
+    }
+    out_norm[0] = 1.0;
+    return;
+  }
   const double x = px_all[in], y = py_all[in];
   int xs_[kMaxSlots], ys_[kMaxSlots];
   int nx = 0, ny = 0;

@@ -337,7 +337,10 @@ typedef struct piv_tps_field {
  * weights (thin_plate_spline_subdom.py:118-126,187-196,208-222). With
  * dom_off/slot_pt (centers): the points are the centers of each field
  * (px = cx), outputs at the center's batch index, the slot being its
- * position in the member list of the subdomain; without: the max_points
+ * position in the member list of the subdomain; the entries of a field from
+ * its count up to max_points (at most the distance between the fields'
+ * first centers) are written empty (slots -1, weights 0, norm 1); without:
+ * the max_points
  * new positions (px, py) shared by every field, outputs at
  * f * max_points + p, the slot being the batch subdomain index.
  * status[0] is set when a point lies in more than 4 subdomains. */
