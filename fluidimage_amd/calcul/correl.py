@@ -346,8 +346,16 @@ class CorrelHIP:
         return self.compute_displacement_from_indices(float(ox.item()), float(oy.item()))
 
 
-# alias under the plural name some callers use for this entry point
-CorrelHIP.compute_displacements_from_indices = CorrelHIP.compute_displacements_from_correl
+def _displacements_from_indices(self, ix, iy):
+    """compute_displacement_from_indices (correl.py:209-211) under the plural
+    name some callers use, for scalars or arrays of indices (ix, iy):
+    returns (ix0 - ix, iy0 - iy)."""
+    if np.ndim(ix) or np.ndim(iy):
+        return self.ix0 - np.asarray(ix), self.iy0 - np.asarray(iy)
+    return self.compute_displacement_from_indices(ix, iy)
+
+
+CorrelHIP.compute_displacements_from_indices = _displacements_from_indices
 
 
 class CorrelDirectHIP:

@@ -886,6 +886,9 @@ __global__ void k_min_final(uint32_t* keys, float* out, int n) {
 // pair blockIdx.y of a batch: its n vectors at offset blockIdx.y * n of every array
 __global__ void k_fix_mark(float* dx, float* dy, const float* cmax, const int32_t* err, int n,
                            float correl_min, float dmax2, int use_dmax, int32_t* flags) {
+  // numpy's deltaxs**2 + deltays**2 (fix.py:99) rounds the squares and the
+  // sum separately: no fused multiply-add here
+#pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   i += blockIdx.y * n;
@@ -959,6 +962,8 @@ __device__ __forceinline__ void neigh_sum(const float* f, const float* g, int ny
 __global__ void k_fix_neigh(const float* dx, const float* dy, int ny, int nx, float thr2,
                             int use_thr, int32_t* flags, float* diff2, float* mean_x,
                             float* mean_y) {
+  // diff_x**2 + diff_y**2 (fix.py:121-123): separately rounded, no FMA
+#pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ny * nx) return;
   const int i0 = i / nx, i1 = i % nx;
@@ -1159,6 +1164,11 @@ int piv_plan_create(piv_plan** plan, int32_t n0, int32_t n1, double displacement
     return fail(PIV_EINVAL, "bad subpix method");
   if (nsubpix < 1 || 2 * nsubpix + 1 > (n0 < n1 ? n0 : n1))
     return fail(PIV_EINVAL, "bad nsubpix");
+  // SubPix.__init__ (subpix.py:76-81): the 3x3 fit takes no nsubpix
+  if (subpix_method == PIV_SUBPIX_2D_GAUSSIAN2 && nsubpix != 1)
+    return fail(PIV_EINVAL,
+                "Subpixel method '2d_gaussian2' doesn't require nsubpix. "
+                "In this case, nsubpix has to be equal to None.");
   if (nb_peaks_to_search < 1 || nb_peaks_to_search > kMaxPeaks)
     return fail(PIV_EUNSUPPORTED, "nb_peaks_to_search must be in [1, 7]");
   if (particle_radius < 0) return fail(PIV_EINVAL, "bad particle_radius");

@@ -5,6 +5,7 @@ import ctypes
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from fluidimage_amd import _native
@@ -82,6 +83,14 @@ def test_plan_validation_without_gpu():
     assert lib.piv_plan_destroy(plan) == 0
     st = lib.piv_plan_create(ctypes.byref(plan), 32, 32, -1.0, None, None, 0, 1, 3, 9)
     assert st == 2
+    # 2d_gaussian2 takes no nsubpix (subpix.py:76-81): any value but 1 is refused
+    for nsub in (2, 10):
+        st = lib.piv_plan_create(ctypes.byref(plan), 32, 32, -1.0, None, None, 0, nsub, 3, 1)
+        assert st == 1 and b"2d_gaussian2" in lib.piv_last_error()
+    # centroid does take it
+    st = lib.piv_plan_create(ctypes.byref(plan), 32, 32, -1.0, None, None, 1, 2, 3, 1)
+    assert st == 0
+    assert lib.piv_plan_destroy(plan) == 0
 
 
 def test_tuning_interface_without_gpu():
@@ -153,3 +162,18 @@ def test_graph_argument_checks_without_gpu():
     assert lib.piv_graph_end(None, None) == 1
     assert lib.piv_graph_launch(None, None) == 1
     assert lib.piv_graph_destroy(None) == 0
+
+
+def test_displacements_from_indices_plural_name():
+    """compute_displacements_from_indices takes (ix, iy), scalars or arrays,
+    and gives (ix0 - ix, iy0 - iy) (correl.py:209-211); a 32x32 plan without
+    displacement_max needs no GPU."""
+    from fluidimage_amd.calcul.correl import CorrelHIP
+
+    c = CorrelHIP((32, 32), method_subpix="2d_gaussian2", nsubpix=None)
+    assert (c.iy0, c.ix0) == (15, 15)
+    assert c.compute_displacements_from_indices(10, 20) == (5, -5)
+    dx, dy = c.compute_displacements_from_indices(np.array([15, 0]), np.array([15, 31]))
+    assert dx.tolist() == [0, 15] and dy.tolist() == [0, -16]
+    with pytest.raises(ValueError, match="2d_gaussian2"):
+        CorrelHIP((32, 32), method_subpix="2d_gaussian2", nsubpix=10)
