@@ -163,6 +163,11 @@ SIGNATURES = {
         [_I, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _I, _I, _P, _P, ctypes.c_int64, _P, _P,
          _P, _P, _P],
     ),
+    "piv_tps_interpolate_noalloc": (
+        ctypes.c_int,
+        [_I, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _I, _I, _P, _P, ctypes.c_int64, _P, _P,
+         _P, _P],
+    ),
     "piv_tps_matrix_size": (ctypes.c_int64, [_I]),
     "piv_tps_interpolate_workspace": (ctypes.c_int64, [_I, _I]
     ),

@@ -35,6 +35,12 @@ MAX_CENTERS_PER_SUBDOM = 2048
 # the small ones) or "natural"; results do not depend on it
 LAUNCH_ORDER = "largest_first"
 MAX_LATTICE_A = 511  # largest lattice table: 512 x 512 doubles (2 MiB)
+# entry point of the evaluation at the new positions: "workspace"
+# (piv_tps_interpolate with a workspace allocated here) or "noalloc"
+# (piv_tps_interpolate_noalloc, the library's fixed scratch; at most
+# NOALLOC_SCRATCH_BYTES of workspace); the results are the same
+INTERPOLATE_ENTRY = "workspace"
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class SingularSystemError(ArithmeticError):
@@ -482,13 +488,22 @@ def _solve_active(lib, active, cx, cy, vx, vy, new_x, new_y, thr, dev, sp, keep,
     approx, skip = _at_centers(active, own, smoothed, new_x, new_y, dev)
     if approx is None:
         approx = torch.empty((2, na * m), **f64)
-    wbytes = int(lib.piv_tps_interpolate_workspace(na * m, nd))
-    iwork = torch.empty(max(wbytes, 8), dtype=torch.uint8, device=dev)
-    _native.check(lib.piv_tps_interpolate(na * m, _p(px), _p(py), _p(n_dom), _p(n_coef),
-                                          _p(n_norm), _p(slot_xy), nslots, nd, n_max,
-                                          _p(dom_off), _p(weights), wstride, _p(skip),
-                                          _p(approx), _p(iwork), lat_p, sp),
-                  "piv_tps_interpolate")
+    if INTERPOLATE_ENTRY == "noalloc":
+        iwork = None
+        _native.check(lib.piv_tps_interpolate_noalloc(
+            na * m, _p(px), _p(py), _p(n_dom), _p(n_coef), _p(n_norm), _p(slot_xy), nslots, nd,
+            n_max, _p(dom_off), _p(weights), wstride, _p(skip), _p(approx), lat_p, sp),
+            "piv_tps_interpolate_noalloc")
+    elif INTERPOLATE_ENTRY == "workspace":
+        wbytes = int(lib.piv_tps_interpolate_workspace(na * m, nd))
+        iwork = torch.empty(max(wbytes, 8), dtype=torch.uint8, device=dev)
+        _native.check(lib.piv_tps_interpolate(na * m, _p(px), _p(py), _p(n_dom), _p(n_coef),
+                                              _p(n_norm), _p(slot_xy), nslots, nd, n_max,
+                                              _p(dom_off), _p(weights), wstride, _p(skip),
+                                              _p(approx), _p(iwork), lat_p, sp),
+                      "piv_tps_interpolate")
+    else:
+        raise ValueError(f"unknown INTERPOLATE_ENTRY {INTERPOLATE_ENTRY!r}")
     if defer:
         # no synchronisation here: the caller checks the status later
         # (deferred_error); the workspaces stay referenced until then when

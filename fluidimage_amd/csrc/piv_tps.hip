@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 
 #include "../../include/piv_hip.h"
 #include "last_error.hpp"
@@ -1744,6 +1745,74 @@ int tps_check(const char* what) {
   return PIV_OK;
 }
 
+// Scratch of piv_tps_interpolate_noalloc: the evaluation's point lists and
+// partial sums (the workspace of piv_tps_interpolate) in a fixed device array,
+// one copy per GPU, held until the process ends. Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in): the address of
+// the current device's copy comes from hipGetSymbolAddress, once per device.
+constexpr size_t kNoAllocScratchBytes = size_t(PIV_TPS_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the scratch array on the current device (looked up on the first
+// call on that device and kept).
+int scratch_on_current_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return tps_fail(PIV_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return tps_fail(PIV_EUNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return tps_fail(PIV_EHIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                    hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PIV_OK;
+}
+
+int tps_interpolate_impl(const char* name, int32_t npts, const double* px, const double* py,
+                         const int32_t* doms,
+                        const double* coef, const double* norm, const double* slot_xy,
+                        int64_t nslots, int32_t nd, int32_t n_max, const int64_t* dom_off,
+                        const double* weights, int64_t wstride, const uint8_t* skip,
+                        double* out, void* work, const piv_tps_lattice* lattice,
+                        void* stream) {
+  if (npts <= 0 || nd <= 0) return PIV_OK;
+  if (n_max <= 0 || n_max > kMaxN)
+    return tps_fail(PIV_EINVAL, "n_max of piv_tps_interpolate must be in [1, 2048]");
+  if (int64_t(npts) * kMaxSlots > INT32_MAX)
+    return tps_fail(PIV_EUNSUPPORTED, "too many TPS evaluation points");
+  if (!px || !py || !doms || !coef || !norm || !slot_xy || !dom_off || !weights || !out ||
+      !work)
+    return tps_fail(PIV_EINVAL, std::string("NULL argument of ") + name);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const InterpWork w = interp_work(work, npts, nd);
+  const unsigned ge = unsigned((npts + 255) / 256);
+  hipLaunchKernelGGL(k_tps_list_zero, dim3((nd + 256) / 256), dim3(256), 0, s, int(nd), w.cnt,
+                     w.cursor);
+  hipLaunchKernelGGL(k_tps_list_count, dim3(ge), dim3(256), 0, s, int(npts), doms, skip, w.cnt);
+  hipLaunchKernelGGL(k_tps_list_scan, dim3(1), dim3(1024), 0, s, int(nd), w.cnt);
+  hipLaunchKernelGGL(k_tps_list_fill, dim3(ge), dim3(256), 0, s, int(npts), doms, skip, w.cnt,
+                     w.cursor, w.list);
+  // about 8 workgroups per CU in all: few subdomains split their lists
+  const int splits = std::min(64, std::max(1, (8 * cu_count() + int(nd) - 1) / int(nd)));
+  const PhiLattice lat = make_lattice(lattice);
+  hipLaunchKernelGGL(k_tps_eval_dom, dim3(nd, splits), dim3(256),
+                     size_t(2) * n_max * sizeof(double2) + size_t(lat.lds_words) * sizeof(double),
+                     s, int(npts), int(n_max), w.cnt, w.list, px, py, slot_xy, nslots, dom_off, weights,
+                     wstride, w.partial, lat);
+  hipLaunchKernelGGL(k_tps_blend_partials, dim3((npts + 255) / 256), dim3(256), 0, s, int(npts),
+                     doms, coef, norm, skip, w.partial, out);
+  return tps_check(name);
+}
+
 }  // namespace
 
 namespace fia {
@@ -1881,33 +1950,34 @@ int piv_tps_interpolate(int32_t npts, const double* px, const double* py, const 
                         const double* weights, int64_t wstride, const uint8_t* skip,
                         double* out, void* work, const piv_tps_lattice* lattice,
                         void* stream) {
+  return tps_interpolate_impl("piv_tps_interpolate", npts, px, py, doms, coef, norm, slot_xy,
+                              nslots, nd, n_max, dom_off, weights, wstride, skip, out, work,
+                              lattice, stream);
+}
+
+int piv_tps_interpolate_noalloc(int32_t npts, const double* px, const double* py,
+                                const int32_t* doms, const double* coef, const double* norm,
+                                const double* slot_xy, int64_t nslots, int32_t nd,
+                                int32_t n_max, const int64_t* dom_off, const double* weights,
+                                int64_t wstride, const uint8_t* skip, double* out,
+                                const piv_tps_lattice* lattice, void* stream) {
   if (npts <= 0 || nd <= 0) return PIV_OK;
-  if (n_max <= 0 || n_max > kMaxN)
-    return tps_fail(PIV_EINVAL, "n_max of piv_tps_interpolate must be in [1, 2048]");
-  if (int64_t(npts) * kMaxSlots > INT32_MAX)
+  if (npts > INT32_MAX / kMaxSlots)
     return tps_fail(PIV_EUNSUPPORTED, "too many TPS evaluation points");
-  if (!px || !py || !doms || !coef || !norm || !slot_xy || !dom_off || !weights || !out ||
-      !work)
-    return tps_fail(PIV_EINVAL, "NULL argument");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const InterpWork w = interp_work(work, npts, nd);
-  const unsigned ge = unsigned((npts + 255) / 256);
-  hipLaunchKernelGGL(k_tps_list_zero, dim3((nd + 256) / 256), dim3(256), 0, s, int(nd), w.cnt,
-                     w.cursor);
-  hipLaunchKernelGGL(k_tps_list_count, dim3(ge), dim3(256), 0, s, int(npts), doms, skip, w.cnt);
-  hipLaunchKernelGGL(k_tps_list_scan, dim3(1), dim3(1024), 0, s, int(nd), w.cnt);
-  hipLaunchKernelGGL(k_tps_list_fill, dim3(ge), dim3(256), 0, s, int(npts), doms, skip, w.cnt,
-                     w.cursor, w.list);
-  // about 8 workgroups per CU in all: few subdomains split their lists
-  const int splits = std::min(64, std::max(1, (8 * cu_count() + int(nd) - 1) / int(nd)));
-  const PhiLattice lat = make_lattice(lattice);
-  hipLaunchKernelGGL(k_tps_eval_dom, dim3(nd, splits), dim3(256),
-                     size_t(2) * n_max * sizeof(double2) + size_t(lat.lds_words) * sizeof(double),
-                     s, int(npts), int(n_max), w.cnt, w.list, px, py, slot_xy, nslots, dom_off, weights,
-                     wstride, w.partial, lat);
-  hipLaunchKernelGGL(k_tps_blend_partials, dim3((npts + 255) / 256), dim3(256), 0, s, int(npts),
-                     doms, coef, norm, skip, w.partial, out);
-  return tps_check("piv_tps_interpolate");
+  const size_t need = interp_work(nullptr, npts, nd).bytes;
+  if (need > kNoAllocScratchBytes)
+    return tps_fail(PIV_EUNSUPPORTED,
+                    "piv_tps_interpolate_noalloc: " + std::to_string(npts) + " points and " +
+                        std::to_string(nd) + " subdomains need " + std::to_string(need) +
+                        " bytes of scratch, more than its " +
+                        std::to_string(kNoAllocScratchBytes) +
+                        " (use piv_tps_interpolate with a workspace)");
+  void* work = nullptr;
+  const int st = scratch_on_current_device(&work);
+  if (st != PIV_OK) return st;
+  return tps_interpolate_impl("piv_tps_interpolate_noalloc", npts, px, py, doms, coef, norm,
+                              slot_xy, nslots, nd, n_max, dom_off, weights, wstride, skip, out,
+                              work, lattice, stream);
 }
 
 int piv_tps_phi_table(double step, int32_t a_max, double* table, void* stream) {

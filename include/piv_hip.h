@@ -391,6 +391,23 @@ int piv_tps_interpolate(int32_t npts, const double* px, const double* py,
                         void* stream);
 int64_t piv_tps_interpolate_workspace(int32_t npts, int32_t nd);
 
+/* piv_tps_interpolate without the workspace argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * It allocates nothing: its scratch (the point lists and partial sums) is a
+ * fixed device array of PIV_TPS_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a call whose piv_tps_interpolate_workspace(npts, nd) exceeds
+ * it fails with PIV_EUNSUPPORTED. Every call on one GPU shares that array:
+ * calls that overlap on one GPU (concurrent streams, threads) race. Same
+ * arguments and results as piv_tps_interpolate otherwise. */
+#define PIV_TPS_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int piv_tps_interpolate_noalloc(int32_t npts, const double* px, const double* py,
+                                const int32_t* doms, const double* coef, const double* norm,
+                                const double* slot_xy, int64_t nslots, int32_t nd,
+                                int32_t n_max, const int64_t* dom_off, const double* weights,
+                                int64_t wstride, const uint8_t* skip, double* out,
+                                const piv_tps_lattice* lattice, void* stream);
+
 /* Table of the kernel function r^2 ln(r^2) / 2 on a lattice of step
  * `step`: table[(a_max + 1) b + a] at r^2 = (a step)^2 + (b step)^2,
  * 0 <= a, b <= a_max (at most 4095), (a_max + 1)^2 doubles. No counterpart
