@@ -236,7 +236,18 @@ def tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshol
     finite vector). ``lattice``: as in ``tps_device``. With ``defer`` the
     call does not wait for the solves: the check for singular systems is
     left to ``deferred_error`` on each returned dict (its "pending" entry).
+    With a ``stream``, every allocation, copy, launch and host read of the
+    call is ordered on it (results are ready on that stream).
     """
+    from ..device import on_stream
+
+    with on_stream(stream):
+        return _tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshold,
+                                 percent_buffer_area, stream, keep, lattice, defer)
+
+
+def _tps_device_batch(fields, new_x, new_y, subdom_size, smoothing_coef, threshold,
+                      percent_buffer_area, stream, keep, lattice, defer):
     import torch
 
     lib = _native.load()

@@ -5,6 +5,7 @@ descriptors (``piv_image``) and the window records (``piv_window``) are
 built here, as small device arrays.
 """
 
+import contextlib
 import ctypes
 
 import numpy as np
@@ -30,6 +31,28 @@ WINDOW_FIELDS = ("image0", "image1", "y0", "x0", "y1", "x1", "dx_in", "dy_in", "
 assert len(WINDOW_FIELDS) * 4 == ctypes.sizeof(_native.PivWindow)
 
 _TORCH_TO_PIV = {}
+
+
+@contextlib.contextmanager
+def on_stream(stream):
+    """Run the body with ``stream`` as torch's current stream, after the
+    work already queued on the caller's current stream: the library's
+    launches, torch's allocations and copies of the body and its device-to-
+    host reads are then all ordered on ``stream``. Results are ready on
+    ``stream`` (the caller orders its own stream after it). No-op when
+    ``stream`` is None or already current."""
+    if stream is None:
+        yield
+        return
+    import torch
+
+    cur = torch.cuda.current_stream()
+    if stream == cur:
+        yield
+        return
+    stream.wait_stream(cur)
+    with torch.cuda.stream(stream):
+        yield
 
 
 def piv_dtype_of(tensor):

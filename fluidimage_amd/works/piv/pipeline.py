@@ -127,7 +127,14 @@ class MultipassPipeline:
             timers=None):
         """All passes for pairs (2k, 2k+1) of ``imageset``; returns the list
         of PassOutput. ``timers`` (dict) collects event pairs around the
-        correlation launches and the TPS calls."""
+        correlation launches and the TPS calls. With a ``stream``, all work
+        (launches, allocations, copies, host reads) is ordered on it."""
+        from ...device import on_stream
+
+        with on_stream(stream):
+            return self._run(imageset, npairs, keep_maps, stream, final_interp, timers)
+
+    def _run(self, imageset, npairs, keep_maps, stream, final_interp, timers):
         import torch
 
         from .deform import DeformedPass, add_predictor, check_method
@@ -454,9 +461,12 @@ def run_pairs(work_piv, pairs, keep_maps=False, stream=None):
     for w in work_piv.works_piv:
         if getattr(w, "imshape0", None) != shape:
             w._prepare_with_image(imshape=shape)
-    imageset = ImageSet(imgs, stream=stream)
-    pipe = MultipassPipeline(work_piv)
-    passes = pipe.run(imageset, len(pairs), keep_maps=keep_maps, stream=stream)
+    from ...device import on_stream
+
+    with on_stream(stream):
+        imageset = ImageSet(imgs, stream=stream)
+        pipe = MultipassPipeline(work_piv)
+        passes = pipe.run(imageset, len(pairs), keep_maps=keep_maps, stream=stream)
     return pipe, passes
 
 

@@ -24,8 +24,22 @@ Parity pinning (what fixes each function independently of this code):
   quadratic fitted by ``numpy.linalg.lstsq``; centroid by direct sums.
 * ``piv`` grid/positions: hand-computed vector counts and offsets.
 * ``fix.mean_neighbors_xy``: ``scipy.ndimage.correlate`` formulation.
+* ``fix.work_fix``: hand-built 3 x 3 cases with literal outcomes for every
+  branch (strict thresholds on squares, the " + " error composition, the
+  secondary-peak ratio, correl_min, tie and NaN rules and the unsquared
+  displacement_max quirk of fix.py:192) (tests/test_oracle_pins.py).
+* ``correl`` secondary peaks: planted maps with literal peak lists, including
+  the empty NaN square of a negative start.
 * ``tps``: interpolation at s = 0, affine reproduction for any s,
-  partition-of-unity blending of constants.
+  partition-of-unity blending of constants; ``compute_tps_weights`` and the
+  per-subdomain evaluation against scipy's ``RBFInterpolator``
+  (thin_plate_spline kernel, smoothing, degree 1); subdomain counts, limits,
+  buffers, rescaled smoothing and half-open membership against
+  hand-computed literals (2048 x 2048 at 16 px: 18 x 19; 64,180 vectors:
+  17 x 18); the outlier loop's cap (11 solves) and spike replacement.
+* ``piv.OracleWorkPIVPass.apply_interp``: the use_tps selector against
+  scipy's griddata and ``RBFInterpolator`` on one-subdomain fields; the
+  multipass window schedule against literals.
 
 Parity unpinned (no independent check available in this environment): the
 exact FFTW float32 bits of the reference maps and the Pythran accumulation

@@ -1,13 +1,15 @@
 """GPU multipass pipeline against the oracle's multipass chain.
 
 Bars: every pass's displacements within 1e-3 px and identical error
-strings, except windows whose FP64 map has a near-tie (listed) and the
-windows downstream of them; final interpolated fields within 1e-3 px."""
+strings, and the final interpolated fields within 1e-3 px, except the
+listed windows whose FP64 map has a near-tie and the vectors downstream of
+them (their WorkFIX neighbourhood, the interpolation's reach of them in the
+next pass): every miss must be one of those."""
 
 import numpy as np
 import pytest
 
-from tests.helpers import near_tie_windows
+from tests.helpers import allowed_mismatches, near_tie_windows
 
 pytestmark = pytest.mark.gpu
 
@@ -28,6 +30,46 @@ def _setup(use_tps, number=3, window=64, shape=(512, 512), nb_peaks=1):
     return im0, im1, p, _oracle_params(p)
 
 
+def oracle_ties(passes, rel=2e-5):
+    """Near-tie windows of every pass of the oracle's run (FP64 maps; a
+    window without a map, a bad crop, has no tie)."""
+    out = []
+    for e in passes:
+        maps = e["correls"]
+        shape = next(m.shape for m in maps if m is not None)
+        full = np.stack([m if m is not None else np.zeros(shape) for m in maps])
+        out.append(near_tie_windows(full, rel=rel))
+    return out
+
+
+def check_passes_against_oracle(gpu_passes, ref, works, deform=False, label=""):
+    """Every vector of every pass within 1e-3 px of the oracle with the same
+    error string, and the final field within 1e-3 px, except the listed
+    near-tie windows and what lies downstream of them (allowed_mismatches);
+    prints the ties and the misses."""
+    ties = oracle_ties(ref)
+    allowed, allowed_final = allowed_mismatches(ref, works, ties, deform=deform)
+    for ip, (g, e) in enumerate(zip(gpu_passes, ref)):
+        both_nan = np.isnan(g.deltaxs) & np.isnan(e["deltaxs"])
+        close = (np.abs(g.deltaxs - e["deltaxs"]) <= 1e-3) & (
+            np.abs(g.deltays - e["deltays"]) <= 1e-3)
+        bad = set(np.flatnonzero(~(both_nan | close)).tolist())
+        err_bad = {i for i in range(len(g.deltaxs)) if g.errors.get(i) != e["errors"].get(i)}
+        print(f"{label} pass {ip}: {len(g.deltaxs)} vectors, near-ties {sorted(ties[ip])}, "
+              f"allowed {len(allowed[ip])}, misses {sorted(bad)[:20]}, "
+              f"error-string misses {sorted(err_bad)[:20]}", flush=True)
+        assert bad <= allowed[ip], sorted(bad - allowed[ip])[:20]
+        assert err_bad <= allowed[ip], sorted(err_bad - allowed[ip])[:20]
+    gf, ef = gpu_passes[-1], ref[-1]
+    if getattr(gf, "deltaxs_final", None) is not None and "deltaxs_final" in ef:
+        df = np.maximum(np.abs(gf.deltaxs_final - ef["deltaxs_final"]),
+                        np.abs(gf.deltays_final - ef["deltays_final"]))
+        fbad = set(np.flatnonzero(~(df <= 1e-3)).tolist())
+        print(f"{label} final field: max |diff| {np.nanmax(df):.3g}, misses {sorted(fbad)[:20]}, "
+              f"allowed {len(allowed_final)}", flush=True)
+        assert fbad <= allowed_final, sorted(fbad - allowed_final)[:20]
+
+
 @pytest.mark.parametrize("use_tps", [True, "last", False])
 def test_multipass_vs_oracle(use_tps):
     from fluidimage_amd.works.piv.multipass import WorkPIV
@@ -36,29 +78,10 @@ def test_multipass_vs_oracle(use_tps):
     print("case: multipass", use_tps, flush=True)
     im0, im1, p, op = _setup(use_tps)
     gpu = WorkPIV(p).calcul_from_arrays(im0, im1)
-    ref = OracleWorkPIV(op, precision="f64").calcul(im0, im1)
+    ow = OracleWorkPIV(op, precision="f64")
+    ref = ow.calcul(im0, im1)
     assert len(gpu.passes) == len(ref) == 3
-    frac_ok = []
-    for ip, (g, e) in enumerate(zip(gpu.passes, ref)):
-        maps = [m for m in e["correls"] if m is not None]
-        ties = near_tie_windows(np.stack(maps)) if maps else []
-        both_nan = np.isnan(g.deltaxs) & np.isnan(e["deltaxs"])
-        close = np.abs(g.deltaxs - e["deltaxs"]) <= 1e-3
-        closey = np.abs(g.deltays - e["deltays"]) <= 1e-3
-        ok = both_nan | (close & closey)
-        frac = ok.mean()
-        frac_ok.append(frac)
-        bad = np.flatnonzero(~ok)
-        print(f"pass {ip}: {len(ok)} vectors, {100 * frac:.3f} % within 1e-3 px, "
-              f"near-ties {list(ties)[:10]}, mismatches {list(bad)[:10]}", flush=True)
-        assert frac >= 0.995
-        same_err = sum(g.errors.get(i) == e["errors"].get(i) for i in range(len(ok)))
-        assert same_err >= 0.995 * len(ok)
-    gf = gpu.passes[-1]
-    ef = ref[-1]
-    df = np.abs(gf.deltaxs_final - ef["deltaxs_final"])
-    print("final field: max |diff|", df.max(), "frac > 1e-3:", (df > 1e-3).mean(), flush=True)
-    assert (df <= 1e-3).mean() >= 0.99
+    check_passes_against_oracle(gpu.passes, ref, ow.works_piv, label=f"use_tps={use_tps}")
 
 
 def test_batch_equals_single_pairs():
@@ -140,3 +163,32 @@ def test_strcrop_mask_vs_oracle():
         assert ok.mean() >= 0.995
     df = np.abs(gpu.passes[-1].deltaxs_final - ref[-1]["deltaxs_final"])
     assert (df <= 1e-3).mean() >= 0.99
+
+
+def test_side_stream_matches_default_stream():
+    """run_pairs on a fresh side stream (allocations, uploads, launches and
+    host reads all ordered on it) gives bit for bit the default stream's
+    results."""
+    import torch
+
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+    from fluidimage_amd.works.piv.pipeline import run_pairs
+
+    print("case: side stream", flush=True)
+    im0, im1, p, _ = _setup(True, number=3, window=64, shape=(384, 384))
+    a0, a1 = config_pair(2, shape=(384, 384))
+    pairs = [(im0, im1), (a0, a1)]
+    w = WorkPIV(p)
+    pipe, ref = run_pairs(w, pairs)
+    expected = [pipe.to_host(ref, k) for k in range(2)]
+    side = torch.cuda.Stream()
+    pipe2, got = run_pairs(WorkPIV(p), pairs, stream=side)
+    side.synchronize()
+    for k in range(2):
+        r = pipe2.to_host(got, k)
+        for pa, pb in zip(expected[k].passes, r.passes):
+            assert np.array_equal(pa.deltaxs, pb.deltaxs, equal_nan=True)
+            assert np.array_equal(pa.deltays, pb.deltays, equal_nan=True)
+            assert pa.errors == pb.errors
+        assert np.array_equal(expected[k].passes[-1].deltaxs_final, r.passes[-1].deltaxs_final)

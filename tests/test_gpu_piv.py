@@ -189,3 +189,41 @@ def test_fix_vs_oracle(nb_peaks):
     for i, e in exp["errors"].items():
         if i not in allowed:
             assert res.errors.get(i) == e, (i, res.errors.get(i), e)
+
+
+def test_fix_thresholds_round_like_numpy():
+    """Planted vector whose |d|^2 is 9.0 in numpy float32 (squares and sum
+    rounded separately) but 9.000001 with a fused multiply-add: with
+    threshold_diff_neighbour 3 and displacement_max 3 it is kept, as the
+    oracle (and the reference) keep it."""
+    import torch
+
+    from fluidimage_amd.params import Params
+    from fluidimage_amd.works.piv.fix import run_fix_device
+    from oracle.fix import work_fix
+
+    x, y = np.float32(2.1672993), np.float32(2.0743227)
+    assert np.float32(x * x) + np.float32(y * y) == np.float32(9.0)
+    dx = np.zeros(9, np.float32)
+    dy = np.zeros(9, np.float32)
+    dx[4], dy[4] = x, y
+    for dmax, thr in ((3.0, None), (None, 3.0)):
+        print("case: fix rounding", dmax, thr, flush=True)
+        dev = torch.device("cuda")
+        out = {"dx": torch.from_numpy(dx.copy()).to(dev), "dy": torch.from_numpy(dy.copy()).to(dev),
+               "correl_max": torch.full((9,), 0.9, device=dev),
+               "err": torch.zeros(9, dtype=torch.int32, device=dev)}
+        pf = Params(tag="fix")
+        pf._set_attribs({"correl_min": 0.2, "threshold_diff_neighbour": thr,
+                         "displacement_max": dmax})
+        run_fix_device(out, 3, 3, pf)
+        got = out["dx"].cpu().numpy()
+        exp = work_fix({"deltaxs": dx.copy(), "deltays": dy.copy(),
+                        "correls_max": np.full(9, 0.9, np.float32), "errors": {},
+                        "secondary_peaks": [None] * 9},
+                       {"correl_min": 0.2, "threshold_diff_neighbour": thr,
+                        "displacement_max": dmax}, 3, 3, 1, None)
+        assert not np.isnan(exp["deltaxs"][4])
+        assert np.array_equal(np.isnan(got), np.isnan(exp["deltaxs"]))
+        if thr is not None:
+            assert out["diff2"].cpu().numpy()[4] == np.float32(9.0)

@@ -211,3 +211,42 @@ def test_tps_launch_order_does_not_change_results(monkeypatch):
         for key in ("smoothed", "approx", "stats"):
             assert torch.equal(torch.nan_to_num(a[key], nan=1e300),
                                torch.nan_to_num(b[key], nan=1e300)), key
+
+
+@pytest.mark.parametrize("defer", [False, True])
+def test_tps_batch_with_failed_fields_keeps_the_others_exact(defer):
+    """A batch holding an all-NaN field ("Only nan.") and a field with an
+    empty subdomain (its valid vectors in two far strips: the retry without
+    it) beside normal fields: the normal fields give bit for bit what they
+    give alone (the prefilled slot arrays of the batch with fields left out)."""
+    import torch
+
+    from fluidimage_amd.calcul.tps import SingularSystemError, lattice_step, tps_device_batch
+
+    print("case: tps batch with failed fields, defer", defer, flush=True)
+    rng = np.random.default_rng(47)
+    dev = torch.device("cuda")
+
+    def t(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    fields = []
+    for kind in ("normal", "all_nan", "normal", "empty_subdomain", "normal"):
+        xs, ys, u, v = _grid_field(rng, 8, (640, 720), holes=0.01)
+        if kind == "all_nan":
+            u[:] = np.nan
+            v[:] = np.nan
+        elif kind == "empty_subdomain":
+            gap = (xs > 100) & (xs < 620)
+            u[gap] = np.nan
+            v[gap] = np.nan
+        fields.append((t(u), t(v), t(xs), t(ys)))
+    gx, gy = fields[0][2], fields[0][3]
+    step = lattice_step(fields[0][2].cpu().numpy(), fields[0][3].cpu().numpy())
+    batch = tps_device_batch(fields, gx, gy, 200, 2.0, 1.5, lattice=step, defer=defer)
+    assert isinstance(batch[1], SingularSystemError) and "Only nan" in str(batch[1])
+    assert isinstance(batch[3], SingularSystemError)
+    for k in (0, 2, 4):
+        alone = tps_device_batch([fields[k]], gx, gy, 200, 2.0, 1.5, lattice=step)[0]
+        for key in ("smoothed", "approx", "stats"):
+            assert torch.equal(batch[k][key], alone[key]), (k, key)
