@@ -199,15 +199,18 @@ def config_pair(config_id, pair_idx=0, shape=None):
 UNEVEN_SIZES = [(1024, 1024), (1200, 1600), (2048, 2048), (2160, 2560)]
 
 
-def uneven_batch(npairs=32, seed_offset=0, sizes=UNEVEN_SIZES):
+def uneven_batch(npairs=32, seed_offset=0, sizes=UNEVEN_SIZES, return_fields=False):
     """Config-4 style batch: random sizes, schedules and Oseen + shear fields.
 
     Returns a list of (im0, im1, windows_schedule) with uint16 images
-    (``sizes`` overrides the image sizes, for smaller test cases).
+    (``sizes`` overrides the image sizes, for smaller test cases), and with
+    ``return_fields`` also the list of the displacement fields (callables
+    (x, y) -> (u, v)) the pairs were rendered with.
     """
     rng = np.random.default_rng(SEED_BASE + 4000 + seed_offset)
     schedules = [(128, 64, 32), (64, 32, 16), (48, 24)]
     out = []
+    fields = []
     for _ in range(npairs):
         shape = sizes[rng.integers(len(sizes))]
         sched = schedules[rng.integers(len(schedules))]
@@ -225,4 +228,7 @@ def uneven_batch(npairs=32, seed_offset=0, sizes=UNEVEN_SIZES):
 
         im0, im1 = make_pair(shape, field, rng, density=0.03, dtype=np.uint16)
         out.append((im0, im1, sched))
+        fields.append(field)
+    if return_fields:
+        return out, fields
     return out

@@ -79,6 +79,8 @@ def compare_pass(gpu, exp, correl, label):
     [
         dict(config=1, shape=None, window=48, dmax=None),
         dict(config=2, shape=(512, 512), window=32, dmax=8),
+        # the stated workload: one 2048 x 2048 uint16 pair, 16,129 windows
+        dict(config=2, shape=None, window=32, dmax=8),
         dict(config=2, shape=(384, 320), window=24, dmax=None),
         dict(config=2, shape=(512, 512), window=64, dmax="30%"),
         dict(config=2, shape=(300, 300), window=16, dmax=None),
