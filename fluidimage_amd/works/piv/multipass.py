@@ -1,13 +1,13 @@
 """Multipass PIV (src/fluidimage/works/piv/multipass.py:18-198) on the GPU."""
 
 import time
-from copy import copy
 
 from ...data_objects import MultipassPIVResults
 from ...params import Params
 from ..with_mask import BaseWorkWithMask
 from .fix import WorkFIX
 from .interp import InterpError
+from .schedule import pass_schedule
 from .singlepass import FirstWorkPIV, WorkPIVFromDisplacement
 
 
@@ -44,48 +44,13 @@ class WorkPIV:
         )
 
     def __init__(self, params=None):
-        if params is None:
-            params = self.create_default_params()
-        self.params = params
-        work_piv = FirstWorkPIV(params)
-        self.works_piv = [work_piv]
-        self.works_fix = [WorkFIX(params.fix, work_piv)]
-        coeff_zoom = params.multipass.coeff_zoom
-        if isinstance(coeff_zoom, int):
-            coeffs_zoom = [coeff_zoom] * (params.multipass.number - 1)
-        elif len(coeff_zoom) == params.multipass.number - 1:
-            coeffs_zoom = coeff_zoom
-        else:
-            raise ValueError(
-                "params.multipass.coeff_zoom has to be an integer or "
-                "an iterable of length params.multipass.number - 1"
-            )
-        shape_crop_im0 = copy(params.piv0.shape_crop_im0)
-        shape_crop_im1 = copy(params.piv0.shape_crop_im1)
-        if shape_crop_im1 is None:
-            shape_crop_im1 = shape_crop_im0
-        if isinstance(shape_crop_im0, int):
-            shape_crop_im0 = (shape_crop_im0, shape_crop_im0)
-        elif isinstance(shape_crop_im0, (list, tuple)) and len(shape_crop_im0) == 2:
-            shape_crop_im0 = tuple(shape_crop_im0)
-        else:
-            raise NotImplementedError("For now, shape_crop_im0 has to be one or two integer!")
-        if isinstance(shape_crop_im1, int):
-            shape_crop_im1 = (shape_crop_im1, shape_crop_im1)
-        elif isinstance(shape_crop_im1, (list, tuple)) and len(shape_crop_im1) == 2:
-            shape_crop_im1 = tuple(shape_crop_im1)
-        else:
-            raise NotImplementedError("For now, shape_crop_im1 has to be one or two integer!")
-        for i in range(1, params.multipass.number):
-            shape_crop_im0 = (shape_crop_im0[0] / coeffs_zoom[i - 1],
-                              shape_crop_im0[1] / coeffs_zoom[i - 1])
-            shape_crop_im1 = (shape_crop_im1[0] / coeffs_zoom[i - 1],
-                              shape_crop_im1[1] / coeffs_zoom[i - 1])
-            work_piv = WorkPIVFromDisplacement(params, index_pass=i,
-                                               shape_crop_im0=shape_crop_im0,
-                                               shape_crop_im1=shape_crop_im1)
-            self.works_piv.append(work_piv)
-            self.works_fix.append(WorkFIX(params.fix, work_piv))
+        self.params = params = params if params is not None else self.create_default_params()
+        sizes = pass_schedule(params.piv0.shape_crop_im0, params.piv0.shape_crop_im1,
+                              params.multipass.number, params.multipass.coeff_zoom)
+        self.works_piv = [FirstWorkPIV(params)] + [
+            WorkPIVFromDisplacement(params, index_pass=k, shape_crop_im0=s0, shape_crop_im1=s1)
+            for k, (s0, s1) in enumerate(sizes[1:], start=1)]
+        self.works_fix = [WorkFIX(params.fix, w) for w in self.works_piv]
 
     keep_maps = True  # results carry every correlation map, like the reference
 
@@ -114,16 +79,16 @@ class WorkPIV:
         return [pipe.to_host(passes, k) for k in range(len(pairs))]
 
     def calcul_passes_per_work(self, couple):
-        """The reference's pass-by-pass chain (FirstWorkPIV, WorkFIX,
-        WorkPIVFromDisplacement ... with host results between passes)."""
+        """The per-work chain of the reference (multipass.py:172-190): each
+        pass's work and WorkFIX with host results in between, then the final
+        interpolation of the last result."""
         results = MultipassPIVResults()
-        piv_result = couple
-        for work_piv, work_fix in zip(self.works_piv, self.works_fix):
-            piv_result = work_piv.calcul(piv_result)
-            piv_result = work_fix.calcul(piv_result)
-            results.append(piv_result)
+        state = couple
+        for stage in zip(self.works_piv, self.works_fix):
+            state = stage[1].calcul(stage[0].calcul(state))
+            results.append(state)
         try:
-            work_piv.apply_interp(piv_result, last=True)
+            self.works_piv[-1].apply_interp(state, last=True)
         except InterpError as e:
             print("Warning: InterpError at the end of the last piv pass:", e)
         return results
@@ -137,10 +102,9 @@ class WorkPIV:
         return result
 
     def _prepare_with_image(self, im=None, imshape=None):
-        if imshape is None:
-            imshape = im.shape
-        for work_piv in self.works_piv:
-            work_piv._prepare_with_image(imshape=imshape)
+        shape = im.shape if imshape is None else imshape
+        for w in self.works_piv:
+            w._prepare_with_image(imshape=shape)
 
 
 Work = WorkPIV

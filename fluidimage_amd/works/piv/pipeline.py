@@ -480,17 +480,14 @@ def run_uneven(items, params):
     from copy import deepcopy
 
     from .multipass import WorkPIV
+    from .schedule import schedule_params
 
     groups = {}
     for i, (im0, im1, sched) in enumerate(items):
         groups.setdefault((tuple(im0.shape), tuple(sched)), []).append(i)
     out = [None] * len(items)
     for (shape, sched), idx in groups.items():
-        p = deepcopy(params)
-        p.piv0.shape_crop_im0 = int(sched[0])
-        p.multipass.number = len(sched)
-        p.multipass.coeff_zoom = [sched[k] // sched[k + 1] for k in range(len(sched) - 1)]
-        work = WorkPIV(p)
+        work = WorkPIV(schedule_params(deepcopy(params), sched))
         res = work.calcul_batch([(items[i][0], items[i][1]) for i in idx])
         for i, r in zip(idx, res):
             out[i] = r

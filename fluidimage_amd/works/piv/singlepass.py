@@ -21,14 +21,11 @@ from ...calcul.correl import correlation_classes
 from ...data_objects import ArrayCouple, HeavyPIVResults, error_strings
 from ...device import ImageSet, piv_dtype_of, to_device_image
 from ..with_mask import BaseWorkWithMask
+from .schedule import grid_positions, grid_step, window_shapes
 
 
 class InterpError(ValueError):
     pass
-
-
-def _isint(obj):
-    return isinstance(obj, (int, np.integer))
 
 
 def _torch():
@@ -45,49 +42,28 @@ class BaseWorkPIV(BaseWorkWithMask):
     index_pass = 0
 
     def _init_shape_crop(self, shape_crop_im0, shape_crop_im1):
-        """singlepass.py:52-87"""
-        if shape_crop_im1 is None:
-            shape_crop_im1 = shape_crop_im0
-        if _isint(shape_crop_im0):
-            shape_crop_im0 = (shape_crop_im0, shape_crop_im0)
-        elif isinstance(shape_crop_im0, (tuple, list)) and len(shape_crop_im0) == 2:
-            shape_crop_im0 = tuple(shape_crop_im0)
-        else:
-            raise NotImplementedError("For now, shape_crop_im0 has to be one or two integer!")
-        if _isint(shape_crop_im1):
-            shape_crop_im1 = (shape_crop_im1, shape_crop_im1)
-        elif isinstance(shape_crop_im1, (tuple, list)) and len(shape_crop_im1) == 2:
-            shape_crop_im1 = tuple(shape_crop_im1)
-        else:
-            raise NotImplementedError("For now, shape_crop_im1 has to be one or two integer!")
-        if shape_crop_im1[0] > shape_crop_im0[0] or shape_crop_im1[1] > shape_crop_im0[1]:
-            raise ValueError("shape_crop_im1 must be inferior or equal to shape_crop_im0")
-        self.shape_crop_im0 = tuple(int(n) for n in shape_crop_im0)
-        self.shape_crop_im1 = tuple(int(n) for n in shape_crop_im1)
+        """Integer window shapes of this pass (singlepass.py:52-87)."""
+        self.shape_crop_im0, self.shape_crop_im1 = window_shapes(shape_crop_im0, shape_crop_im1)
 
     def _init_correl(self):
-        """singlepass.py:89-106"""
-        try:
-            correl_cls = correlation_classes[self.params.piv0.method_correl]
-        except KeyError:
+        """The correlation object of this pass, chosen in the registry by
+        params.piv0.method_correl (singlepass.py:89-106)."""
+        p0 = self.params.piv0
+        cls = correlation_classes.get(p0.method_correl)
+        if cls is None:
             raise ValueError(
-                "params.piv0.method_correl should be in " + str(list(correlation_classes.keys()))
-            )
-        if not hasattr(correl_cls, "correlate_windows"):
+                "params.piv0.method_correl should be in " + str(list(correlation_classes.keys())))
+        if not hasattr(cls, "correlate_windows"):
             raise NotImplementedError(
-                f"method_correl {self.params.piv0.method_correl!r}: the batched WorkPIV runs "
+                f"method_correl {p0.method_correl!r}: the batched WorkPIV runs "
                 "the fused FFT correlation ('fftw' / 'hip'); the direct correlation is "
                 "available through its per-window and window-stack API "
-                f"({correl_cls.__name__})")
-        self.correl = correl_cls(
-            im0_shape=self.shape_crop_im0,
-            im1_shape=self.shape_crop_im1,
-            method_subpix=self.params.piv0.method_subpix,
-            nsubpix=self.params.piv0.nsubpix,
-            displacement_max=self.params.piv0.displacement_max,
-            particle_radius=self.params.piv0.particle_radius,
-            nb_peaks_to_search=self.params.piv0.nb_peaks_to_search,
-        )
+                f"({cls.__name__})")
+        options = {key: getattr(p0, key) for key in (
+            "method_subpix", "nsubpix", "displacement_max", "particle_radius",
+            "nb_peaks_to_search")}
+        self.correl = cls(im0_shape=self.shape_crop_im0, im1_shape=self.shape_crop_im1,
+                          **options)
 
     def _init_crop(self):
         """singlepass.py:363-389"""
@@ -111,28 +87,17 @@ class BaseWorkPIV(BaseWorkWithMask):
         return self.index_pass == self.params.multipass.number - 1
 
     def _prepare_with_image(self, im0=None, imshape=None):
-        """Grid of vectors (singlepass.py:108-141)."""
-        if imshape is None:
-            imshape = im0.shape
-        self.imshape0 = (len_y, len_x) = tuple(int(n) for n in imshape)
-        scim = self.shape_crop_im0
-        stepy = scim[0] - int(np.round(self.overlap * scim[0]))
-        stepx = scim[1] - int(np.round(self.overlap * scim[1]))
-        assert stepy >= 1
-        assert stepx >= 1
-        ixvec_max = len_x - self._stop_for_crop0[1]
-        ixvecs = np.arange(self._start_for_crop0[1], ixvec_max, stepx, dtype=int)
-        iyvec_max = len_y - self._stop_for_crop0[0]
-        iyvecs = np.arange(self._start_for_crop0[0], iyvec_max, stepy, dtype=int)
-        if ixvec_max - ixvecs[-1] > stepx // 2.5:
-            ixvecs = np.append(ixvecs, ixvec_max)
-        if iyvec_max - iyvecs[-1] > stepy // 2.5:
-            iyvecs = np.append(iyvecs, iyvec_max)
-        self.ixvecs = ixvecs
-        self.iyvecs = iyvecs
-        gx, gy = np.meshgrid(ixvecs, iyvecs)
-        self.ixvecs_grid = gx.flatten()
-        self.iyvecs_grid = gy.flatten()
+        """Grid of vectors (singlepass.py:108-141): window centres along y
+        and x, flattened row-major (ivec = iy_index nx + ix_index)."""
+        shape = im0.shape if imshape is None else imshape
+        self.imshape0 = tuple(int(n) for n in shape)
+        axes = [grid_positions(self.imshape0[a], self._start_for_crop0[a],
+                               self._stop_for_crop0[a],
+                               grid_step(self.shape_crop_im0[a], self.overlap))
+                for a in (0, 1)]
+        self.iyvecs, self.ixvecs = axes
+        self.iyvecs_grid = np.repeat(self.iyvecs, len(self.ixvecs))
+        self.ixvecs_grid = np.tile(self.ixvecs, len(self.iyvecs))
         self._dev_grid = None
         self._dev_grid_full = None
 
