@@ -63,7 +63,8 @@ def parse():
                     choices=["multipass", "single", "refine", "direct"])
     ap.add_argument("--no-maps", action="store_true")
     ap.add_argument("--graph", action="store_true",
-                    help="single workload: replay the step captured in a HIP graph")
+                    help="replay the step captured in one HIP graph (its TPS checks are read "
+                         "after each replay)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the one-pair (plain per-pair path) measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -250,10 +251,11 @@ def run_bench(args, pairs, keep_maps, steps, warmup):
     images = ImageSet(ims)
     state = {}
 
-    if getattr(args, "graph", False) and args.workload == "single":
+    if getattr(args, "graph", False):
         from fluidimage_amd.works.piv.pipeline import GraphedPasses
 
-        graphed = GraphedPasses(work, images, len(pairs), keep_maps=keep_maps)
+        graphed = GraphedPasses(work, images, len(pairs), keep_maps=keep_maps,
+                                final_interp=args.workload == "multipass")
 
         def step(timers):
             state["passes"] = graphed.replay()
@@ -557,7 +559,7 @@ def main():
             "stage_ms_in_step": {k: round(v, 4) for k, v in sorted(avg.items())},
             "correlation_passes": per_pass,
             "tuning": tuning_in_use(),
-            "hip_graph": bool(args.graph and args.workload == "single"),
+            "hip_graph": bool(args.graph),
         },
         "roofline": {
             "bound": "hbm",
@@ -575,8 +577,10 @@ def main():
 
     if not args.no_plain and npairs > 1:
         print("bench: plain per-pair steps", file=sys.stderr, flush=True)
-        _, _, _, ms1, _ = run_bench(args, pairs[:1], keep_maps, max(3, args.steps // 2),
-                                    args.warmup)
+        # the plain per-pair path: the same kernels on one pair, eager
+        plain_args = argparse.Namespace(**{**vars(args), "graph": False})
+        _, _, _, ms1, _ = run_bench(plain_args, pairs[:1], keep_maps,
+                                    max(3, args.steps // 2), args.warmup)
         plain = nvec / (ms1 / 1e3)
         line["config"]["plain_per_pair"] = {"ms_per_step": round(ms1, 3),
                                             "vectors/s": round(plain, 1)}

@@ -221,6 +221,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
+  if (n == 0) return;  // a subdomain slot without centers (device-sized batches)
   double* E = mats + mat_off[d];
   const int64_t ld = tps_mat_ld(n - 3);
   double* C = E + tps_mat_e_size(n - 3);
@@ -468,6 +469,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
+  if (n == 0) return;  // a subdomain slot without centers (device-sized batches)
   const int m = n - 3;
   double* E = mats + mat_off[d];
   const int64_t ld = tps_mat_ld(m);
@@ -1043,6 +1045,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
   if (status[d]) return;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
+  if (n == 0) return;  // a subdomain slot without centers (device-sized batches)
   const int m = n - 3;
   const double* E = mats + mat_off[d];
   const int64_t ld = tps_mat_ld(m);
@@ -1348,7 +1351,12 @@ __global__ __launch_bounds__(256) void k_tps_count(const piv_tps_field* __restri
                                                    const double* __restrict__ cx_all,
                                                    const double* __restrict__ cy_all,
                                                    int32_t* counts) {
-  const piv_tps_field f = fields[dom_field[blockIdx.x]];
+  const int fi = dom_field[blockIdx.x];
+  if (fi < 0) {  // a subdomain slot beyond its field's grid (device-sized batches)
+    if (threadIdx.x == 0) counts[blockIdx.x] = 0;
+    return;
+  }
+  const piv_tps_field f = fields[fi];
   const SubGrid g = field_grid(f, lim);
   const int d = blockIdx.x - f.dom_base;
   const double* cx = cx_all + f.center_base;
@@ -1373,7 +1381,10 @@ __global__ __launch_bounds__(1024) void k_tps_fill(const piv_tps_field* __restri
                                                    const int64_t* __restrict__ dom_off,
                                                    int32_t* slot_pt) {
   __shared__ int tmp[32];
-  const piv_tps_field f = fields[dom_field[blockIdx.x]];
+  const int fi = dom_field[blockIdx.x];
+  // inactive subdomain, or one of a field the offsets left out (no slots)
+  if (fi < 0 || dom_off[blockIdx.x + 1] == dom_off[blockIdx.x]) return;
+  const piv_tps_field f = fields[fi];
   const SubGrid g = field_grid(f, lim);
   const int d = blockIdx.x - f.dom_base;
   const int center_base = f.center_base;
@@ -1417,7 +1428,8 @@ __global__ void k_tps_point_slots(const piv_tps_field* __restrict__ fields,
   out_slot += kMaxSlots * o;
   out_coef += kMaxSlots * o;
   out_norm += o;
-  if (centers && p >= f.count) {  // beyond the field's valid vectors
+  // beyond the field's valid vectors, or a field left out (no grid): empty
+  if ((centers && p >= f.count) || g.nsx == 0) {
     for (int k = 0; k < kMaxSlots; ++k) {
       out_slot[k] = -1;
       out_coef[k] = 0.0;
@@ -1467,6 +1479,311 @@ __global__ void k_tps_point_slots(const piv_tps_field* __restrict__ fields,
     out_coef[k] = 0.0;
   }
   out_norm[0] = norm;
+}
+
+// ---------------------------------------------------------------------------
+// 4b. subdomain grids and offsets on the device: a batch sized by
+//     capacities instead of host reads (thin_plate_spline_subdom.py:53-126)
+// ---------------------------------------------------------------------------
+// One workgroup per field: the subdomain counts of the reference from the
+// count and bounds of its selected centers (info[f][7] of k_tps_select), the
+// np.linspace limits (y[i] = i step + start, the last one = stop, rounded
+// as numpy rounds them), the 20 % buffers and the rescaled smoothing; field
+// records, limits, per-subdomain smoothing and subdomain-to-field map of
+// dom_cap subdomain slots per field (slots beyond the field's grid: -1).
+__global__ __launch_bounds__(256) void k_tps_grid(
+    const double* __restrict__ info, int n, double subdom_size, double smoothing_coef,
+    double coef_buffer, int dom_cap, double gen_width, int require_lattice,
+    piv_tps_field* __restrict__ fields, double* __restrict__ lim,
+    int32_t* __restrict__ dom_field, double* __restrict__ smoothing_dom,
+    int32_t* __restrict__ fstat) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.x;
+  const double* in = info + 7 * f;
+  const int count = int(in[0]);
+  const double x_min = in[1], x_max = in[2], y_min = in[3], y_max = in[4];
+  __shared__ int sh_nsx, sh_nsy, sh_st;
+  __shared__ double sh_smooth;
+  if (threadIdx.x == 0) {
+    int st = PIV_TPS_FIELD_OK, nsx = 0, nsy = 0;
+    double smooth = 0.0;
+    const double range_x = x_max - x_min, range_y = y_max - y_min;
+    if (count == 0) {
+      st = PIV_TPS_FIELD_NO_VECTOR;
+    } else if (range_x == 0.0 || range_y == 0.0) {
+      st = PIV_TPS_FIELD_DEGENERATE;
+    } else {
+      const double aspect = range_y / range_x;
+      const double nb = double(count) / subdom_size;
+      const double fx = floor(sqrt(nb / aspect));
+      const double fy = ceil(sqrt(nb * aspect));
+      // int(...) or 1; a grid beyond the capacity is left to the host
+      if (!(fx <= double(dom_cap)) || !(fy <= double(dom_cap))) {
+        st = PIV_TPS_FIELD_CAPACITY;
+      } else {
+        nsx = fx >= 1.0 ? int(fx) : 1;
+        nsy = fy >= 1.0 ? int(fy) : 1;
+        if (int64_t(nsx) * nsy > dom_cap) {
+          st = PIV_TPS_FIELD_CAPACITY;
+        } else {
+          const double step_x = range_x / double(nsx), step_y = range_y / double(nsy);
+          // x_dom[1] - x_dom[0]: x_dom[0] = 0 step + x_min = x_min
+          const double x1 = nsx > 1 ? 1.0 * step_x + x_min : x_max;
+          const double y1 = nsy > 1 ? 1.0 * step_y + y_min : y_max;
+          const double per = double(count) / double(nsx * nsy);
+          smooth = smoothing_coef * sqrt((x1 - x_min) * (y1 - y_min) / per);
+        }
+      }
+    }
+    if (st == PIV_TPS_FIELD_OK && require_lattice && in[6] == 0.0) st = PIV_TPS_FIELD_OFF_LATTICE;
+    if (st != PIV_TPS_FIELD_OK) nsx = nsy = 0;
+    sh_nsx = nsx;
+    sh_nsy = nsy;
+    sh_st = st;
+    sh_smooth = smooth;
+  }
+  __syncthreads();
+  const int nsx = sh_nsx, nsy = sh_nsy;
+  int st = sh_st;
+  const int64_t lim_off = int64_t(f) * 4 * dom_cap;
+  if (st == PIV_TPS_FIELD_OK) {
+    const double range_x = x_max - x_min, range_y = y_max - y_min;
+    const double step_x = range_x / double(nsx), step_y = range_y / double(nsy);
+    const double bx = coef_buffer * range_x / double(nsx);
+    const double by = coef_buffer * range_y / double(nsy);
+    double* L = lim + lim_off;
+    int wide = 0;
+    for (int i = threadIdx.x; i < nsx; i += blockDim.x) {
+      const double a = double(i) * step_x + x_min;
+      const double b = i + 1 < nsx ? double(i + 1) * step_x + x_min : x_max;
+      L[i] = a - bx;
+      L[nsx + i] = b + bx;
+      if (gen_width > 0.0 && !(L[nsx + i] - L[i] <= gen_width)) wide = 1;
+    }
+    for (int i = threadIdx.x; i < nsy; i += blockDim.x) {
+      const double a = double(i) * step_y + y_min;
+      const double b = i + 1 < nsy ? double(i + 1) * step_y + y_min : y_max;
+      L[2 * nsx + i] = a - by;
+      L[2 * nsx + nsy + i] = b + by;
+      if (gen_width > 0.0 && !(L[2 * nsx + nsy + i] - L[2 * nsx + i] <= gen_width)) wide = 1;
+    }
+    // a subdomain wider than the kernel-function table (generated matrices)
+    if (__syncthreads_or(wide)) st = PIV_TPS_FIELD_OFF_LATTICE;
+  }
+  const bool ok = st == PIV_TPS_FIELD_OK;
+  const int nd = ok ? nsx * nsy : 0;
+  for (int d = threadIdx.x; d < dom_cap; d += blockDim.x) {
+    dom_field[int64_t(f) * dom_cap + d] = d < nd ? f : -1;
+    smoothing_dom[int64_t(f) * dom_cap + d] = sh_smooth;
+  }
+  if (threadIdx.x == 0) {
+    piv_tps_field r;
+    r.nsx = ok ? nsx : 0;
+    r.nsy = ok ? nsy : 0;
+    r.count = ok ? count : 0;
+    r.center_base = f * n;
+    r.dom_base = f * dom_cap;
+    r.lim_off = int(lim_off);
+    r.sorted_y = in[5] != 0.0 ? 1 : 0;
+    fields[f] = r;
+    fstat[f] = st;
+  }
+}
+
+// inclusive prefix sum of int64 over a 1024-thread block
+__device__ int64_t block_scan_1024_64(int64_t v, int64_t* tmp, int64_t& total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  if (lane == 63) tmp[wv] = v;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    int64_t x = tmp[threadIdx.x];
+    for (int off = 1; off < 16; off <<= 1) {
+      const int64_t u = __shfl_up(x, off, 16);
+      if (threadIdx.x >= off) x += u;
+    }
+    tmp[16 + threadIdx.x] = x;
+  }
+  __syncthreads();
+  total = tmp[16 + 15];
+  const int64_t add = wv > 0 ? tmp[16 + wv - 1] : 0;
+  __syncthreads();
+  return v + add;
+}
+
+constexpr int kOffsetsMaxFields = 2048;
+
+// One workgroup of 1024 threads for the whole batch: checks the member
+// counts of every subdomain (an empty subdomain or too many centers fails
+// its field, as the host sizing does), admits the fields in order while
+// their slots and matrices fit the capacities, then writes the prefix sums
+// dom_off / mat_off over the nd = nfields dom_cap subdomain slots (the
+// slots of failed fields get no room) and the order that takes the largest
+// subdomains first. totals: slots, matrix doubles, largest subdomain,
+// subdomains with centers.
+__global__ __launch_bounds__(1024) void k_tps_offsets(
+    int nfields, int dom_cap, piv_tps_field* __restrict__ fields,
+    const int32_t* __restrict__ counts, const int32_t* __restrict__ dom_field,
+    int64_t slots_cap, int64_t mats_cap, int n_max_cap, int32_t* __restrict__ fstat,
+    int64_t* __restrict__ dom_off, int64_t* __restrict__ mat_off, int32_t* __restrict__ order,
+    int64_t* __restrict__ totals) {
+  __shared__ int flags[kOffsetsMaxFields];
+  __shared__ int hist[kMaxN + 2];
+  __shared__ int64_t tmp64[32];
+  __shared__ int64_t fslots[kOffsetsMaxFields];
+  __shared__ int64_t fmats[kOffsetsMaxFields];
+  const int nd = nfields * dom_cap;
+  const int tid = threadIdx.x;
+  for (int f = tid; f < nfields; f += 1024) flags[f] = 0;
+  for (int b = tid; b < kMaxN + 2; b += 1024) hist[b] = 0;
+  __syncthreads();
+  // 1. per-subdomain checks: bit 0 empty, bit 1 more than kMaxN, bit 2 above n_max_cap
+  for (int d = tid; d < nd; d += 1024) {
+    const int f = dom_field[d];
+    if (f < 0 || fstat[f] != PIV_TPS_FIELD_OK) continue;
+    const int c = counts[d];
+    int bits = 0;
+    if (c == 0) bits = 1;
+    else if (c > kMaxN) bits = 2;
+    else if (c > n_max_cap) bits = 4;
+    if (bits) atomicOr(&flags[f], bits);
+  }
+  __syncthreads();
+  // 2. statuses (an empty subdomain first, as the host sizing checks it
+  //    first), then each field's slots and matrix doubles
+  for (int f = tid; f < nfields; f += 1024) {
+    int st = fstat[f];
+    const int b = flags[f];
+    if (st == PIV_TPS_FIELD_OK && b)
+      st = (b & 1) ? PIV_TPS_FIELD_EMPTY_SUBDOMAIN
+                   : ((b & 2) ? PIV_TPS_FIELD_TOO_MANY : PIV_TPS_FIELD_CAPACITY);
+    int64_t sl = 0, ma = 0;
+    if (st == PIV_TPS_FIELD_OK)
+      for (int d = f * dom_cap; d < (f + 1) * dom_cap; ++d) {
+        if (dom_field[d] < 0) continue;
+        sl += counts[d];
+        ma += tps_mat_size(counts[d]);
+      }
+    flags[f] = st;
+    fslots[f] = sl;
+    fmats[f] = ma;
+  }
+  __syncthreads();
+  // 3. admission in field order while the capacities hold
+  if (tid == 0) {
+    int64_t cs = 0, cm = 0;
+    for (int f = 0; f < nfields; ++f) {
+      if (flags[f] != PIV_TPS_FIELD_OK) continue;
+      const int64_t ma = fmats[f];
+      if (cs + fslots[f] > slots_cap || cm + ma > mats_cap) {
+        flags[f] = PIV_TPS_FIELD_CAPACITY;
+        continue;
+      }
+      cs += fslots[f];
+      cm += ma;
+    }
+  }
+  __syncthreads();
+  for (int f = tid; f < nfields; f += 1024) {
+    fstat[f] = flags[f];
+    if (flags[f] != PIV_TPS_FIELD_OK) {  // no grid from here on: points get no slots
+      fields[f].nsx = 0;
+      fields[f].nsy = 0;
+      fields[f].count = 0;
+    }
+  }
+  // 4. prefix sums of the admitted subdomains' slots and matrix sizes; the
+  //    histogram of their sizes
+  int64_t base_s = 0, base_m = 0;
+  int n_max = 0, active = 0;
+  for (int c0 = 0; c0 < nd; c0 += 1024) {
+    const int d = c0 + tid;
+    int c = 0;
+    if (d < nd) {
+      const int f = dom_field[d];
+      if (f >= 0 && flags[f] == PIV_TPS_FIELD_OK) c = counts[d];
+    }
+    int64_t ts, tm;
+    const int64_t is = block_scan_1024_64(int64_t(c), tmp64, ts);
+    const int64_t im = block_scan_1024_64(c > 0 ? tps_mat_size(c) : 0, tmp64, tm);
+    if (d < nd) {
+      dom_off[d] = base_s + is - c;
+      mat_off[d] = base_m + im - (c > 0 ? tps_mat_size(c) : 0);
+      atomicAdd(&hist[c], 1);
+    }
+    base_s += ts;
+    base_m += tm;
+    n_max = max(n_max, c);
+    active += c > 0 ? 1 : 0;
+  }
+  if (tid == 0) {
+    dom_off[nd] = base_s;
+    mat_off[nd] = base_m;
+  }
+  // reductions of n_max and active
+  for (int off = 32; off > 0; off >>= 1) {
+    n_max = max(n_max, __shfl_xor(n_max, off, 64));
+    active += __shfl_xor(active, off, 64);
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    tmp64[tid >> 6] = n_max;
+    tmp64[16 + (tid >> 6)] = active;
+  }
+  __syncthreads();
+  // 5. order: sizes descending (a counting sort; the order within a size
+  //    is free, results do not depend on it)
+  if (tid == 0) {
+    int64_t nm = 0, na = 0;
+    for (int w = 0; w < 16; ++w) {
+      nm = nm > tmp64[w] ? nm : tmp64[w];
+      na += tmp64[16 + w];
+    }
+    totals[0] = base_s;
+    totals[1] = base_m;
+    totals[2] = nm;
+    totals[3] = na;
+    int run = 0;
+    for (int b = kMaxN + 1; b >= 0; --b) {
+      const int h = hist[b];
+      hist[b] = run;
+      run += h;
+    }
+  }
+  __syncthreads();
+  for (int d = tid; d < nd; d += 1024) {
+    const int f = dom_field[d];
+    const int c = (f >= 0 && flags[f] == PIV_TPS_FIELD_OK) ? counts[d] : 0;
+    order[atomicAdd(&hist[c], 1)] = d;
+  }
+}
+
+// The evaluation at new positions that are the field's own centres: the
+// blended smoothed value there ([K P][w; a] = v - s w at the centres). For
+// field f (blockIdx.y) and the rank o < count of a selected vector (vector
+// q = idx[f n + o]): if new position q is that vector's position, out gets
+// the smoothed value and skip[f n + q] = 1 (out and skip start at zero).
+__global__ void k_tps_at_centers(int n, const double* __restrict__ info,
+                                 const int32_t* __restrict__ idx, const double* __restrict__ xs,
+                                 const double* __restrict__ ys, const double* __restrict__ new_x,
+                                 const double* __restrict__ new_y,
+                                 const double* __restrict__ smoothed, int64_t ntot,
+                                 double* __restrict__ out, uint8_t* __restrict__ skip) {
+  const int f = blockIdx.y;
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  const int count = int(info[7 * f]);
+  if (o >= count || o >= n) return;
+  const int64_t base = int64_t(f) * n;
+  const int q = idx[base + o];
+  if (q < 0 || q >= n) return;
+  if (new_x[q] == xs[base + q] && new_y[q] == ys[base + q]) {
+    out[base + q] = smoothed[base + o];
+    out[ntot + base + q] = smoothed[ntot + base + o];
+    skip[base + q] = 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1921,6 +2238,55 @@ int piv_tps_point_slots(int32_t nfields, const piv_tps_field* fields, int32_t ma
                      static_cast<hipStream_t>(stream), fields, lim, max_points, px, py, dom_off,
                      slot_pt, out_slot, out_coef, out_norm, status);
   return tps_check("piv_tps_point_slots");
+}
+
+int piv_tps_grid(int32_t nfields, int32_t n, const double* info, double subdom_size,
+                 double smoothing_coef, double coef_buffer, int32_t dom_cap, double gen_width,
+                 int32_t require_lattice, piv_tps_field* fields, double* lim,
+                 int32_t* dom_field, double* smoothing, int32_t* fstat, void* stream) {
+  if (nfields <= 0 || n <= 0 || dom_cap <= 0) return tps_fail(PIV_EINVAL, "empty TPS batch");
+  if (nfields > kOffsetsMaxFields)
+    return tps_fail(PIV_EUNSUPPORTED, "more than 2048 fields in a device-sized TPS batch");
+  if (int64_t(nfields) * n > INT32_MAX || int64_t(nfields) * dom_cap * 4 > INT32_MAX)
+    return tps_fail(PIV_EUNSUPPORTED, "device-sized TPS batch too large");
+  if (!(subdom_size > 0.0)) return tps_fail(PIV_EINVAL, "subdom_size must be positive");
+  if (!info || !fields || !lim || !dom_field || !smoothing || !fstat)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_tps_grid, dim3(nfields), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     info, int(n), subdom_size, smoothing_coef, coef_buffer, int(dom_cap),
+                     gen_width, int(require_lattice), fields, lim, dom_field, smoothing, fstat);
+  return tps_check("piv_tps_grid");
+}
+
+int piv_tps_offsets(int32_t nfields, int32_t dom_cap, piv_tps_field* fields,
+                    const int32_t* counts, const int32_t* dom_field, int64_t slots_cap,
+                    int64_t mats_cap, int32_t n_max_cap, int32_t* fstat, int64_t* dom_off,
+                    int64_t* mat_off, int32_t* order, int64_t* totals, void* stream) {
+  if (nfields <= 0 || dom_cap <= 0) return tps_fail(PIV_EINVAL, "empty TPS batch");
+  if (nfields > kOffsetsMaxFields)
+    return tps_fail(PIV_EUNSUPPORTED, "more than 2048 fields in a device-sized TPS batch");
+  if (n_max_cap < 4 || n_max_cap > kMaxN)
+    return tps_fail(PIV_EINVAL, "n_max_cap must be in [4, 2048]");
+  if (!fields || !counts || !dom_field || !fstat || !dom_off || !mat_off || !order || !totals)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_tps_offsets, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream),
+                     int(nfields), int(dom_cap), fields, counts, dom_field, slots_cap, mats_cap,
+                     int(n_max_cap), fstat, dom_off, mat_off, order, totals);
+  return tps_check("piv_tps_offsets");
+}
+
+int piv_tps_at_centers(int32_t nfields, int32_t n, const double* info, const int32_t* idx,
+                       const double* xs, const double* ys, const double* new_x,
+                       const double* new_y, const double* smoothed, double* out,
+                       uint8_t* skip, void* stream) {
+  if (nfields <= 0 || n <= 0) return PIV_OK;
+  if (nfields > 65535) return tps_fail(PIV_EINVAL, "too many fields for one launch");
+  if (!info || !idx || !xs || !ys || !new_x || !new_y || !smoothed || !out || !skip)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_tps_at_centers, dim3((n + 255) / 256, nfields), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), int(n), info, idx, xs, ys, new_x, new_y,
+                     smoothed, int64_t(nfields) * n, out, skip);
+  return tps_check("piv_tps_at_centers");
 }
 
 int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef, const double* norm,

@@ -49,6 +49,12 @@ class MultipassPipeline:
         # alone with the checks in place (griddata fallback and all)
         self.defer_checks = True
         self._pending = []
+        # "device": TPS batches sized on the GPU from capacities planned once
+        # per pass (no host read inside a run; a field beyond them is run
+        # again with "host" sizing); "host": sized from device-to-host reads
+        self.sizing = "device"
+        self._caps = {}
+        self._tables = {}
 
     def _event_pair(self, timers, key):
         import torch
@@ -69,6 +75,8 @@ class MultipassPipeline:
         mp = self.params.multipass
         use_tps = mp.use_tps is True or (mp.use_tps == "last" and last)
         out = [None] * res.npairs
+        if use_tps and self.sizing == "device":
+            return self._interp_planned(work_next, res, last, stream, timers)
         if use_tps:
             dev = res.dev
             gx, gy = work_next._device_grid_full(dev)
@@ -102,6 +110,53 @@ class MultipassPipeline:
                 out[k] = self._griddata(work_next, res, k)
         return out
 
+    def _capacity(self, work, work_next, npairs, deformed):
+        """Capacities of the device-sized TPS call from ``work``'s vectors to
+        ``work_next``'s grid (planned once per pair of grids and batch size)."""
+        from ...calcul.tps import lattice_step
+        from ...calcul.tps_planned import make_lattice_table, plan_capacity
+
+        key = (work.index_pass, work_next.index_pass, npairs, deformed, work.imshape0)
+        cap = self._caps.get(key)
+        if cap is None:
+            mp = self.params.multipass
+            step = lattice_step(work.ixvecs, work.iyvecs, work_next.ixvecs, work_next.iyvecs)
+            # the first pass and the deformed passes put their vectors on the grid
+            on_grid = deformed or work.index_pass == 0
+            cap = plan_capacity(work.ixvecs_grid, work.iyvecs_grid, npairs, mp.subdom_size,
+                                mp.smoothing_coef, 20, step=step, generated=on_grid)
+            self._caps[key] = cap
+        if cap not in self._tables:
+            import torch
+
+            self._tables[cap] = make_lattice_table(cap, torch.device("cuda"))
+        return cap, self._tables[cap]
+
+    def _interp_planned(self, work_next, res, last, stream=None, timers=None):
+        """The TPS interpolation of every pair as one device-sized batch (no
+        host read); the fields' checks are read at the end of the run."""
+        from ...calcul.tps_planned import tps_planned
+
+        mp = self.params.multipass
+        deformed = getattr(self.params.multipass, "image_deformation", None) is not None \
+            and res.work.index_pass > 0
+        cap, table = self._capacity(res.work, work_next, res.npairs, deformed)
+        gx, gy = work_next._device_grid_full(res.dev)
+        ev = self._event_pair(timers, f"tps_pass{work_next.index_pass}" + ("_final" if last else ""))
+        if ev:
+            ev[0].record()
+        t = tps_planned(cap, res.out["dx"], res.out["dy"], res.win["xs"], res.win["ys"], gx, gy,
+                        mp.subdom_size, mp.smoothing_coef, mp.threshold_tps, 20, stream=stream,
+                        lattice_table=table)
+        if ev:
+            ev[1].record()
+        self._pending.append(("planned", t))
+        out = []
+        for k in range(res.npairs):
+            res.tps[k] = (t, k)
+            out.append(t.approx(k))
+        return out
+
     def _griddata(self, work_next, res, k):
         import torch
 
@@ -124,17 +179,20 @@ class MultipassPipeline:
                 torch.from_numpy(np.ascontiguousarray(ay)).to(dev))
 
     def run(self, imageset, npairs, keep_maps=False, stream=None, final_interp=True,
-            timers=None):
+            timers=None, check=True):
         """All passes for pairs (2k, 2k+1) of ``imageset``; returns the list
         of PassOutput. ``timers`` (dict) collects event pairs around the
         correlation launches and the TPS calls. With a ``stream``, all work
-        (launches, allocations, copies, host reads) is ordered on it."""
+        (launches, allocations, copies, host reads) is ordered on it.
+        ``check=False`` leaves the TPS checks of the run in ``self._pending``
+        (the run then reads nothing back from the device): the caller checks
+        them later with ``check_pending``."""
         from ...device import on_stream
 
         with on_stream(stream):
-            return self._run(imageset, npairs, keep_maps, stream, final_interp, timers)
+            return self._run(imageset, npairs, keep_maps, stream, final_interp, timers, check)
 
-    def _run(self, imageset, npairs, keep_maps, stream, final_interp, timers):
+    def _run(self, imageset, npairs, keep_maps, stream, final_interp, timers, check=True):
         import torch
 
         from .deform import DeformedPass, add_predictor, check_method
@@ -217,18 +275,31 @@ class MultipassPipeline:
                 steps = int(getattr(self.params.multipass, "refine_steps", 0) or 0)
                 if steps > 0:
                     self._refine(imageset, res, steps, timers)
-        failed = self._check_deferred()
-        if failed:
-            self._rerun_pairs(imageset, passes, failed, keep_maps, stream, final_interp)
+        if check:
+            failed = self._check_deferred()
+            if failed:
+                self._rerun_pairs(imageset, passes, failed, keep_maps, stream, final_interp)
         return passes
 
     def _check_deferred(self):
-        """Pairs whose deferred TPS check found a singular system (one
-        device-to-host read per TPS call of the run)."""
+        """Pairs whose deferred TPS check failed (one device-to-host read per
+        TPS call of the run); clears the pending checks."""
+        failed = self.check_pending(self._pending)
+        self._pending = []
+        return failed
+
+    @staticmethod
+    def check_pending(pending):
+        """Pairs failing any of the given deferred TPS checks (a singular
+        system, a field beyond the device-sized capacities, ...)."""
         from ...calcul.tps import deferred_error
 
-        failed = {k for k, t in self._pending if deferred_error(t) is not None}
-        self._pending = []
+        failed = set()
+        for k, t in pending:
+            if k == "planned":
+                failed |= {j for j, e in enumerate(t.field_errors()) if e is not None}
+            elif deferred_error(t) is not None:
+                failed.add(k)
         return failed
 
     def _rerun_pairs(self, imageset, passes, pairs, keep_maps, stream, final_interp):
@@ -240,6 +311,7 @@ class MultipassPipeline:
         for k in sorted(pairs):
             sub = MultipassPipeline(self.work)
             sub.defer_checks = False
+            sub.sizing = "host"
             single = ImageSet([imageset.images[2 * k], imageset.images[2 * k + 1]],
                               stream=stream)
             one_passes = sub.run(single, 1, keep_maps=keep_maps, stream=stream,
@@ -378,8 +450,11 @@ class MultipassPipeline:
                 r.deltays_final = res.final[k][1].cpu().numpy()
                 t = res.tps[k]
                 if t is not None:
-                    sel = t["index"].cpu().numpy()
-                    sm = t["smoothed"].cpu().numpy()
+                    if isinstance(t, tuple):  # (device-sized batch, field)
+                        sel, sm = t[0].host_field(t[1])
+                    else:
+                        sel = t["index"].cpu().numpy()
+                        sm = t["smoothed"].cpu().numpy()
                     xs_s = r.ixvecs_final.copy().astype(np.float32)
                     ys_s = r.iyvecs_final.copy().astype(np.float32)
                     xs_s[sel] = r.xs[sel]
@@ -401,49 +476,159 @@ class MultipassPipeline:
 
 
 class GraphedPasses:
-    """The passes of a single-pass ``WorkPIV`` over a batch of pairs (image
-    minima, window records, one fused correlation launch, WorkFIX) captured
-    once in a HIP graph and replayed: a replay re-runs every kernel on the
-    images currently in ``imageset`` (same tensors, new contents) and writes
-    into the same result tensors (``passes``).
+    """Every pass of a ``WorkPIV`` over a batch of pairs (image minima,
+    window records, the fused correlation of each pass, WorkFIX, the
+    device-sized TPS interpolation between passes and at the end, the image
+    deformation) captured once in a HIP graph and replayed: a replay re-runs
+    every kernel on the images currently in ``imageset`` (same tensors, new
+    contents) and writes into the same result tensors (``passes``).
 
-    Only passes without an interpolation between them can be captured: the
-    TPS stage sizes its buffers from device-to-host reads (the valid-vector
-    count and bounds, the subdomain member counts), and griddata runs on the
-    host."""
+    Nothing in the captured step reads the device: the TPS checks (singular
+    systems, fields beyond the capacities planned for the batch) are read
+    after each replay, once, and a pair that failed one is run again alone
+    with host sizing (the reference's fallbacks), its results copied into
+    the replay's. Needs TPS for every interpolation (use_tps True; griddata
+    runs on the host), one peak per window and no refinement steps."""
 
-    def __init__(self, work_piv, imageset, npairs, keep_maps=False, warmup=2):
+    def __init__(self, work_piv, imageset, npairs, keep_maps=False, warmup=2,
+                 final_interp=None):
         import torch
 
-        if len(work_piv.works_piv) != 1:
+        p = work_piv.params
+        mp = p.multipass
+        nwork = len(work_piv.works_piv)
+        if final_interp is None:
+            final_interp = nwork > 1
+        needs_tps = nwork > 1 or final_interp
+        use_tps_all = mp.use_tps is True or (mp.use_tps == "last" and nwork == 1)
+        if needs_tps and not use_tps_all:
             raise NotImplementedError(
-                "graph capture of the multipass: the interpolation between passes sizes its "
-                "buffers on the host")
-        if work_piv.params.piv0.nb_peaks_to_search > 1:
+                "graph capture of the multipass: griddata (use_tps False or 'last' between "
+                "passes) runs on the host; use params.multipass.use_tps = True")
+        if p.piv0.nb_peaks_to_search > 1:
             raise NotImplementedError("secondary-peak replacement runs on the host")
+        if int(getattr(mp, "refine_steps", 0) or 0) > 0:
+            raise NotImplementedError("the refinement steps accept or reject on the host")
         self.pipe = MultipassPipeline(work_piv)
+        self.pipe.sizing = "device"
         self.imageset = imageset
         self.npairs = npairs
+        self.keep_maps = keep_maps
+        self.final_interp = final_interp
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):  # plans, grids and caches outside the graph
+        with torch.cuda.stream(side):  # plans, grids, capacities and caches outside the graph
             for _ in range(warmup):
-                self._step(keep_maps)
+                self._step(check=True)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.passes = self._step(keep_maps)
+            self.passes = self._step(check=False)
+        self.pending = list(self.pipe._pending)
+        self.pipe._pending = []
+        # the graph's own output references (a rerun replaces some of them)
+        self._refs = [(list(r.approx), list(r.final), list(r.tps)) for r in self.passes]
+        self.reruns = 0
 
-    def _step(self, keep_maps):
+    def _step(self, check):
         self.imageset.recompute_min()
-        return self.pipe.run(self.imageset, self.npairs, keep_maps=keep_maps, final_interp=False)
+        return self.pipe.run(self.imageset, self.npairs, keep_maps=self.keep_maps,
+                             final_interp=self.final_interp, check=check)
 
     def replay(self):
-        """Enqueue the captured step on the current stream; returns the
-        passes whose tensors receive the results."""
+        """Run the captured step on the current stream, then read its TPS
+        checks (one device-to-host read) and run the pairs that failed one
+        again. Returns the passes holding the results."""
         self.graph.replay()
+        for res, (approx, final, tps) in zip(self.passes, self._refs):
+            res.approx[:], res.final[:], res.tps[:] = approx, final, tps
+        failed = self.pipe.check_pending(self.pending)
+        if failed:
+            self.reruns += len(failed)
+            self.pipe._rerun_pairs(self.imageset, self.passes, failed, self.keep_maps, None,
+                                   self.final_interp)
         return self.passes
+
+
+class UnevenGraph:
+    """The uneven batch of ``run_uneven`` (pairs of different shapes and
+    window schedules) as one captured HIP graph: each (shape, schedule)
+    bucket is a batched multipass, and the buckets are captured one after
+    the other into the same graph; ``replay`` runs them all. ``items``:
+    (im0, im1, schedule) with the images already on the device (their
+    contents may change between replays)."""
+
+    def __init__(self, items, params, keep_maps=False, warmup=2):
+        from copy import deepcopy
+
+        import torch
+
+        from ...device import ImageSet, to_device_image
+        from .multipass import WorkPIV
+        from .schedule import schedule_params
+
+        dev = torch.device("cuda")
+        groups = {}
+        for i, (im0, _im1, sched) in enumerate(items):
+            groups.setdefault((tuple(im0.shape), tuple(sched)), []).append(i)
+        self.groups = []
+        for (shape, sched), idx in groups.items():
+            work = WorkPIV(schedule_params(deepcopy(params), sched))
+            for w in work.works_piv:
+                w._prepare_with_image(imshape=shape)
+            ims = []
+            for i in idx:
+                ims += [to_device_image(items[i][0], dev), to_device_image(items[i][1], dev)]
+            self.groups.append((work, ImageSet(ims), idx))
+        self.pipes = []
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for work, images, idx in self.groups:
+                pipe = MultipassPipeline(work)
+                pipe.sizing = "device"
+                for _ in range(warmup):
+                    images.recompute_min()
+                    pipe.run(images, len(idx), keep_maps=keep_maps)
+                self.pipes.append(pipe)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.keep_maps = keep_maps
+        self.graph = torch.cuda.CUDAGraph()
+        self.passes = []
+        with torch.cuda.graph(self.graph):
+            for (work, images, idx), pipe in zip(self.groups, self.pipes):
+                images.recompute_min()
+                self.passes.append(pipe.run(images, len(idx), keep_maps=keep_maps,
+                                            check=False))
+        self.pending = []
+        for pipe in self.pipes:
+            self.pending.append(list(pipe._pending))
+            pipe._pending = []
+        self._refs = [[(list(r.approx), list(r.final), list(r.tps)) for r in passes]
+                      for passes in self.passes]
+        self.nitems = len(items)
+
+    def replay(self):
+        """Run every bucket of the captured batch; pairs failing a TPS check
+        are run again alone. Returns the passes of each bucket."""
+        self.graph.replay()
+        for g, ((work, images, idx), pipe) in enumerate(zip(self.groups, self.pipes)):
+            for res, (approx, final, tps) in zip(self.passes[g], self._refs[g]):
+                res.approx[:], res.final[:], res.tps[:] = approx, final, tps
+            failed = pipe.check_pending(self.pending[g])
+            if failed:
+                pipe._rerun_pairs(images, self.passes[g], failed, self.keep_maps, None, True)
+        return self.passes
+
+    def results(self):
+        """MultipassPIVResults of every pair, in the order of the items."""
+        out = [None] * self.nitems
+        for (work, images, idx), pipe, passes in zip(self.groups, self.pipes, self.passes):
+            for j, i in enumerate(idx):
+                out[i] = pipe.to_host(passes, j)
+        return out
 
 
 def run_pairs(work_piv, pairs, keep_maps=False, stream=None):

@@ -362,6 +362,63 @@ int piv_tps_point_slots(int32_t nfields, const piv_tps_field* fields,
                         double* out_coef, double* out_norm, int32_t* status,
                         void* stream);
 
+/* ---- device-sized TPS batches (no counterpart in the reference, whose
+ *      ThinPlateSplineSubdom.__init__ sizes each field on the host,
+ *      thin_plate_spline_subdom.py:53-126): the subdomain grids, member
+ *      offsets and launch order of a batch computed on the GPU from
+ *      capacities fixed beforehand, so that a whole multipass can run
+ *      without a device-to-host read and be captured in one graph ----
+ * Field status (fstat[f]): */
+#define PIV_TPS_FIELD_OK 0
+#define PIV_TPS_FIELD_NO_VECTOR 1       /* no finite vector ("Only nan.") */
+#define PIV_TPS_FIELD_DEGENERATE 2      /* centers on one line (zero range) */
+#define PIV_TPS_FIELD_EMPTY_SUBDOMAIN 3 /* a subdomain without center */
+#define PIV_TPS_FIELD_CAPACITY 4        /* beyond the batch's capacities */
+#define PIV_TPS_FIELD_TOO_MANY 5        /* more than 2048 centers in a subdomain */
+#define PIV_TPS_FIELD_OFF_LATTICE 6     /* generated matrices asked for, but a center or a
+                                           subdomain is off the kernel-function table */
+/* piv_tps_grid: from info[f][7] of piv_tps_select (count, bounds, sorted,
+ * on-lattice), for each of nfields fields of n vectors: its subdomain grid
+ * as thin_plate_spline_subdom.py:53-90 computes it (counts floor/ceil of
+ * sqrt, np.linspace limits rounded as numpy rounds them, buffers
+ * coef_buffer * range / count, smoothing rescaled by
+ * sqrt(area / vectors per subdomain)), laid out in dom_cap subdomain slots
+ * per field: fields[f] (dom_base f dom_cap, lim_off 4 f dom_cap, center_base
+ * f n), lim (4 dom_cap doubles per field), dom_field (nfields dom_cap;
+ * -1 beyond the field's grid), smoothing (per subdomain slot) and fstat.
+ * gen_width > 0: a subdomain wider than that (the reach of the lattice
+ * table) fails its field with PIV_TPS_FIELD_OFF_LATTICE, as does a field
+ * not on the lattice when require_lattice. At most 2048 fields.
+ * piv_tps_offsets: after piv_tps_count over the nfields dom_cap slots,
+ * fails the fields with an empty subdomain (first), more than 2048 or more
+ * than n_max_cap centers in one, admits the others in order while their
+ * slots and matrix doubles fit slots_cap and mats_cap (else
+ * PIV_TPS_FIELD_CAPACITY), then writes dom_off and mat_off (nfields dom_cap
+ * + 1 each, the slots of failed fields empty), order (largest subdomain
+ * first) and totals[4] (slots, matrix doubles, largest subdomain,
+ * subdomains with centers). A failed field's record loses its grid, so its
+ * points get no slot (piv_tps_point_slots writes them empty with norm 1)
+ * and its blended and interpolated values are 0. */
+int piv_tps_grid(int32_t nfields, int32_t n, const double* info, double subdom_size,
+                 double smoothing_coef, double coef_buffer, int32_t dom_cap, double gen_width,
+                 int32_t require_lattice, piv_tps_field* fields, double* lim,
+                 int32_t* dom_field, double* smoothing, int32_t* fstat, void* stream);
+int piv_tps_offsets(int32_t nfields, int32_t dom_cap, piv_tps_field* fields,
+                    const int32_t* counts, const int32_t* dom_field, int64_t slots_cap,
+                    int64_t mats_cap, int32_t n_max_cap, int32_t* fstat, int64_t* dom_off,
+                    int64_t* mat_off, int32_t* order, int64_t* totals, void* stream);
+/* The evaluation at the new positions that are a field's own centers (new
+ * positions shared by the nfields fields of n vectors, new_x[q] against
+ * xs[f n + q]): out[c][f n + q] receives the blended smoothed value of the
+ * vector (smoothed[c][f n + rank], the rank of vector q among the field's
+ * selected vectors idx) and skip[f n + q] = 1, so that piv_tps_interpolate
+ * leaves it (it equals the evaluation: [K P][w; a] = v - s w at the
+ * centers). out and skip start at zero; info as piv_tps_select writes it. */
+int piv_tps_at_centers(int32_t nfields, int32_t n, const double* info, const int32_t* idx,
+                       const double* xs, const double* ys, const double* new_x,
+                       const double* new_y, const double* smoothed, double* out,
+                       uint8_t* skip, void* stream);
+
 /* out[c][i] = sum_k coef[4 i + k] * smooth_slots[c][slots[4 i + k]] /
  * norm[i] over the point's slots (-1 terminated): the blend of
  * thin_plate_spline_subdom.py:145-158. out[2][npts]. */
