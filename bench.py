@@ -101,9 +101,10 @@ def multipass_params():
     return p
 
 
-def cpu_baseline_multipass(im0, im1):
+def _oracle_multipass_pair(pair):
     """The oracle (numpy/pocketfft float32 + LAPACK restatement of the
-    reference) on one pair, one thread."""
+    reference) on one pair of the multipass workload, one thread; returns
+    (final vectors, seconds)."""
     from threadpoolctl import threadpool_limits
 
     from oracle.piv import OracleWorkPIV, default_params
@@ -114,20 +115,12 @@ def cpu_baseline_multipass(im0, im1):
                           threshold_tps=1.5, image_deformation="bicubic")
     with threadpool_limits(1):
         t0 = time.perf_counter()
-        res = OracleWorkPIV(p, precision="f32").calcul(im0, im1)
+        res = OracleWorkPIV(p, precision="f32").calcul(*pair)
         dt = time.perf_counter() - t0
-    n = len(res[-1]["deltaxs"])
-    return {
-        "value": round(n / dt, 1),
-        "unit": "vectors/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"one pair of the workload (one 2048x2048 pair, 3 passes + TPS, {n} final "
-        f"vectors), oracle float32 FFT + LAPACK TPS, {dt:.1f} s on 1 thread",
-    }
+    return len(res[-1]["deltaxs"]), dt
 
 
-def cpu_baseline_single(im0, im1):
+def _oracle_single_pair(pair):
     from threadpoolctl import threadpool_limits
 
     from oracle.piv import OracleWorkPIVPass, default_params
@@ -136,15 +129,57 @@ def cpu_baseline_single(im0, im1):
     p["piv0"]["shape_crop_im0"] = 32
     p["piv0"]["displacement_max"] = 8
     w = OracleWorkPIVPass(p, 0, precision="f32")
-    w.prepare_with_image(im0.shape)
+    w.prepare_with_image(pair[0].shape)
     with threadpool_limits(1):
         t0 = time.perf_counter()
-        res = w.loop_vectors(im0, im1, keep_maps=True)
+        res = w.loop_vectors(*pair, keep_maps=True)
         dt = time.perf_counter() - t0
-    n = len(res["deltaxs"])
-    return {"value": round(n / dt, 1), "unit": "vectors/s", "cores": 1, "kind": "port",
-            "sample": f"one pair of the workload ({n} windows), oracle float32, {dt:.1f} s "
-                      "on 1 thread"}
+    return len(res["deltaxs"]), dt
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(pairs, workload, max_cores=16):
+    """The reference's CPU path restated (the oracle), timed on the host's
+    cores as the reference runs it: one process per allowed core
+    (os.sched_getaffinity, at most max_cores, the host's share of a one-GPU
+    box), each on its own pair with one thread (OMP_NUM_THREADS=1), like
+    multi_exec_async's split of a series; plus one core alone. Run before
+    the GPU is touched: the workers are forked."""
+    import multiprocessing as mp
+
+    fn = _oracle_multipass_pair if workload == "multipass" else _oracle_single_pair
+    cores = max(1, min(len(os.sched_getaffinity(0)), max_cores, len(pairs)))
+    n1, dt1 = fn(pairs[0])
+    one = n1 / dt1
+    os.environ["OMP_NUM_THREADS"] = "1"
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        out = pool.map(fn, pairs[:cores], chunksize=1)
+    wall = time.perf_counter() - t0
+    nall = sum(n for n, _ in out)
+    kind = ("one 2048x2048 pair per process, 3 passes + TPS" if workload == "multipass"
+            else "one 2048x2048 pair per process, 16,129 windows")
+    return {
+        "value": round(nall / wall, 1),
+        "unit": "vectors/s",
+        "cores": cores,
+        "kind": "port",
+        "one_core_value": round(one, 1),
+        "cpu": cpu_model(),
+        "sample": f"{cores} pairs of the workload on {cores} processes ({kind}, {nall} final "
+                  f"vectors, {wall:.1f} s wall) and one pair on one core ({n1} vectors, "
+                  f"{dt1:.1f} s); oracle float32 FFT + LAPACK TPS, one thread per process",
+    }
 
 
 def correl_bytes(work, imshape, itemsize, nwin, npairs, maps):
@@ -471,7 +506,6 @@ def main():
 
     from fluidimage_amd.synthetic import config_pair
 
-    torch.cuda.set_device(0)
     keep_maps = not args.no_maps
     cfg = 3 if args.workload == "multipass" else 2
     npairs = max(1, args.pairs)
@@ -479,6 +513,12 @@ def main():
           flush=True)
     pairs = [config_pair(cfg, pair_idx=k) for k in range(npairs)]
     im0 = pairs[0][0]
+    cpu_line = None
+    if not args.no_cpu_baseline:
+        # before the GPU is initialised: the worker processes are forked
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        cpu_line = cpu_baseline(pairs, args.workload)
+    torch.cuda.set_device(0)
 
     print("bench: batched steps", file=sys.stderr, flush=True)
     work, images, passes, ms_step, avg = run_bench(args, pairs, keep_maps, args.steps,
@@ -586,13 +626,8 @@ def main():
                                             "vectors/s": round(plain, 1)}
         line["config"]["batched_over_plain"] = round(value / plain, 2)
 
-    if not args.no_cpu_baseline:
-        print("bench: CPU baseline", file=sys.stderr, flush=True)
-        a, b = pairs[0]
-        if args.workload == "multipass":
-            line["cpu_baseline"] = cpu_baseline_multipass(a, b)
-        else:
-            line["cpu_baseline"] = cpu_baseline_single(a, b)
+    if cpu_line is not None:
+        line["cpu_baseline"] = cpu_line
     print(json.dumps(line), flush=True)
 
 

@@ -11,12 +11,14 @@
 // grid's bounding box). The deformed crops are im0(y - v/2, x - u/2) and
 // im1(y + v/2, x + u/2), sampled by Keys cubic convolution (a = -0.5; R. Keys,
 // IEEE TASSP 29(6), 1981) on the image minus its minimum, zero outside the
-// image; positions, weights and sums are FP64. A workgroup deforms one row:
+// image; the predictor and the positions are FP64, the weights and the tap
+// sums float32 (the images' own precision). A workgroup deforms one row:
 // the predictor is first interpolated along y at every grid column (LDS),
-// then along x per pixel; inside the image the minimum is taken out of the
-// sample once, as m times the sum of the 16 weights. The pixel's predictor does not
-// depend on the window, so the images are deformed once (float32) and the
-// windows are cropped from them by the fused correlation.
+// then along x per pixel; a row of 4 taps is one vector load; inside the
+// image the minimum is taken out of the sample once, as m times the sum of
+// the 16 weights. The pixel's predictor does not depend on the window, so
+// the images are deformed once (float32) and the windows are cropped from
+// them by the fused correlation.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -30,58 +32,104 @@ namespace {
 constexpr int kDeformThreads = 256;
 constexpr int kMaxGridColumns = 4096;  // the row's predictor in LDS: 64 KB
 
-template <int DT>
-__device__ __forceinline__ double pixel(const void* data, int64_t idx);
-template <>
-__device__ __forceinline__ double pixel<PIV_U8>(const void* data, int64_t idx) {
-  return double(static_cast<const uint8_t*>(data)[idx]);
-}
-template <>
-__device__ __forceinline__ double pixel<PIV_U16>(const void* data, int64_t idx) {
-  return double(static_cast<const uint16_t*>(data)[idx]);
-}
-template <>
-__device__ __forceinline__ double pixel<PIV_F32>(const void* data, int64_t idx) {
-  return double(static_cast<const float*>(data)[idx]);
-}
-
-// Keys weights of the taps at offsets -1, 0, 1, 2 for the fraction t in [0, 1)
-__device__ __forceinline__ void keys_weights(double t, double w[4]) {
-  constexpr double a = -0.5;
-  const double d0 = 1.0 + t, d1 = t, d2 = 1.0 - t, d3 = 2.0 - t;
-  // |d| <= 1: (a + 2) d^3 - (a + 3) d^2 + 1; 1 < |d| < 2: a d^3 - 5 a d^2 + 8 a d - 4 a
-  w[0] = ((a * d0 - 5.0 * a) * d0 + 8.0 * a) * d0 - 4.0 * a;
-  w[1] = ((a + 2.0) * d1 - (a + 3.0)) * d1 * d1 + 1.0;
-  w[2] = ((a + 2.0) * d2 - (a + 3.0)) * d2 * d2 + 1.0;
-  w[3] = ((a * d3 - 5.0 * a) * d3 + 8.0 * a) * d3 - 4.0 * a;
+// a sample position kept within 8 px of the image: every tap of a position
+// beyond that is outside the image (zero) either way, and the tap indices
+// of the clamped value stay far from integer overflow (NaN goes to -8)
+__device__ __forceinline__ double clamp_pos(double p, int n) {
+  return fmin(fmax(p, -8.0), double(n) + 8.0);
 }
 
 template <int DT>
-__device__ double bicubic(const piv_image& im, double m, double py, double px) {
-  const double fy = floor(py), fx = floor(px);
-  double wy[4], wx[4];
-  keys_weights(py - fy, wy);
-  keys_weights(px - fx, wx);
-  const int iy = int(fy) - 1, ix = int(fx) - 1;
-  double acc = 0.0;
+__device__ __forceinline__ float pixel(const void* data, int64_t idx);
+template <>
+__device__ __forceinline__ float pixel<PIV_U8>(const void* data, int64_t idx) {
+  return float(((const __attribute__((address_space(1))) uint8_t*)data)[idx]);
+}
+template <>
+__device__ __forceinline__ float pixel<PIV_U16>(const void* data, int64_t idx) {
+  return float(((const __attribute__((address_space(1))) uint16_t*)data)[idx]);
+}
+template <>
+__device__ __forceinline__ float pixel<PIV_F32>(const void* data, int64_t idx) {
+  return ((const __attribute__((address_space(1))) float*)data)[idx];
+}
+
+// Keys weights of the taps at offsets -1, 0, 1, 2 for the fraction t in
+// [0, 1), a = -0.5: the cubic pieces expanded in t (Catmull-Rom form),
+// w0 = (-t^3 + 2 t^2 - t) / 2, w1 = (3 t^3 - 5 t^2 + 2) / 2,
+// w2 = (-3 t^3 + 4 t^2 + t) / 2, w3 = (t^3 - t^2) / 2
+__device__ __forceinline__ void keys_weights(float t, float w[4]) {
+  const float t2 = t * t, t3 = t2 * t;
+  w[0] = 0.5f * (2.0f * t2 - t3 - t);
+  w[1] = 0.5f * (3.0f * t3 - 5.0f * t2) + 1.0f;
+  w[2] = 0.5f * (4.0f * t2 - 3.0f * t3 + t);
+  w[3] = 0.5f * (t3 - t2);
+}
+
+// the 4 taps of a row of the image starting at element `idx` (in the image,
+// any alignment of the element type) in one vector load
+template <int DT>
+__device__ __forceinline__ void row_taps(const void* data, int64_t idx, float p[4]);
+template <>
+__device__ __forceinline__ void row_taps<PIV_U8>(const void* data, int64_t idx, float p[4]) {
+  typedef uint32_t __attribute__((aligned(1))) u32a1;
+  const uint32_t w = *((const __attribute__((address_space(1))) u32a1*)(
+      (const __attribute__((address_space(1))) uint8_t*)data + idx));
+#pragma unroll
+  for (int b = 0; b < 4; ++b) p[b] = float((w >> (8 * b)) & 0xffu);  // 32-bit lanes
+}
+template <>
+__device__ __forceinline__ void row_taps<PIV_U16>(const void* data, int64_t idx, float p[4]) {
+  typedef uint64_t __attribute__((aligned(2))) u64a2;
+  const uint64_t w = *((const __attribute__((address_space(1))) u64a2*)(
+      (const __attribute__((address_space(1))) uint16_t*)data + idx));
+  // split into 32-bit halves: the conversions stay uint32 -> float
+  const uint32_t lo = uint32_t(w), hi = uint32_t(w >> 32);
+  p[0] = float(lo & 0xffffu);
+  p[1] = float(lo >> 16);
+  p[2] = float(hi & 0xffffu);
+  p[3] = float(hi >> 16);
+}
+template <>
+__device__ __forceinline__ void row_taps<PIV_F32>(const void* data, int64_t idx, float p[4]) {
+  typedef float f4 __attribute__((ext_vector_type(4), aligned(4)));
+  const f4 w = *((const __attribute__((address_space(1))) f4*)(
+      (const __attribute__((address_space(1))) float*)data + idx));
+  p[0] = w[0];
+  p[1] = w[1];
+  p[2] = w[2];
+  p[3] = w[3];
+}
+
+// The sample with tap origin (iy, ix) = floor(p) - 1 and fractions
+// (ty, tx) of the position: float32 weights and 16-tap sum, zero outside
+// the image (the position's split into origin and fraction is done in FP64
+// by the caller; the fraction is then exact in float32 to 2^-24).
+template <int DT>
+__device__ float bicubic_taps(const piv_image& im, float m, int iy, int ix, float ty, float tx) {
+  float wy[4], wx[4];
+  keys_weights(ty, wy);
+  keys_weights(tx, wx);
+  float acc = 0.0f;
   if (iy >= 0 && iy <= im.height - 4 && ix >= 0 && ix <= im.width - 4) {
     // interior: the 16 taps inside the image, sum w p - m sum w
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const int64_t row = int64_t(iy + a) * im.pitch + ix;
-      double ra = 0.0;
+      float p[4];
+      row_taps<DT>(im.data, int64_t(iy + a) * im.pitch + ix, p);
+      float ra = 0.0f;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) ra += wx[b] * pixel<DT>(im.data, row + b);
+      for (int b = 0; b < 4; ++b) ra += wx[b] * p[b];
       acc += wy[a] * ra;
     }
-    const double sw = ((wy[0] + wy[1]) + (wy[2] + wy[3])) * ((wx[0] + wx[1]) + (wx[2] + wx[3]));
+    const float sw = ((wy[0] + wy[1]) + (wy[2] + wy[3])) * ((wx[0] + wx[1]) + (wx[2] + wx[3]));
     return acc - m * sw;
   }
   for (int a = 0; a < 4; ++a) {
     const int yy = iy + a;
     if (yy < 0 || yy >= im.height) continue;
     const int64_t row = int64_t(yy) * im.pitch;
-    double ra = 0.0;
+    float ra = 0.0f;
     for (int b = 0; b < 4; ++b) {
       const int xx = ix + b;
       if (xx < 0 || xx >= im.width) continue;
@@ -116,8 +164,8 @@ __global__ __launch_bounds__(kDeformThreads) void k_deform_images(
   const int lb = xcd_band(int(blockIdx.x), int(gridDim.x));
   const int pair = lb / height, y = lb % height;
   const piv_image im0 = images[2 * pair], im1 = images[2 * pair + 1];
-  const double m0 = im0.minval ? double(*im0.minval) : 0.0;
-  const double m1 = im1.minval ? double(*im1.minval) : 0.0;
+  const float m0 = im0.minval ? *im0.minval : 0.0f;
+  const float m1 = im1.minval ? *im1.minval : 0.0f;
   const double* U = u + int64_t(pair) * nvec;
   const double* V = v + int64_t(pair) * nvec;
   const int W = im0.width;
@@ -134,6 +182,8 @@ __global__ __launch_bounds__(kDeformThreads) void k_deform_images(
     rowuv[g.nx + j] = (1.0 - ty) * V[i0] + ty * V[i1];
   }
   __syncthreads();
+  // independent pixels: unrolled so that their tap loads are in flight together
+#pragma unroll 4
   for (int x = threadIdx.x; x < W; x += kDeformThreads) {
     // bilinear predictor at the pixel: along x between the row's values
     const int jx = g.cell_x[x];
@@ -141,8 +191,14 @@ __global__ __launch_bounds__(kDeformThreads) void k_deform_images(
     const int jx1 = min(jx + 1, g.nx - 1);
     const double uu = (1.0 - tx) * rowuv[jx] + tx * rowuv[jx1];
     const double vv = (1.0 - tx) * rowuv[g.nx + jx] + tx * rowuv[g.nx + jx1];
-    out0[x] = float(bicubic<DT>(im0, m0, y - 0.5 * vv, x - 0.5 * uu));
-    out1[x] = float(bicubic<DT>(im1, m1, y + 0.5 * vv, x + 0.5 * uu));
+    const double py0 = clamp_pos(y - 0.5 * vv, im0.height), px0 = clamp_pos(x - 0.5 * uu, W);
+    const double py1 = clamp_pos(y + 0.5 * vv, im1.height),
+                 px1 = clamp_pos(x + 0.5 * uu, im1.width);
+    const double fy0 = floor(py0), fx0 = floor(px0), fy1 = floor(py1), fx1 = floor(px1);
+    out0[x] = bicubic_taps<DT>(im0, m0, int(fy0) - 1, int(fx0) - 1, float(py0 - fy0),
+                               float(px0 - fx0));
+    out1[x] = bicubic_taps<DT>(im1, m1, int(fy1) - 1, int(fx1) - 1, float(py1 - fy1),
+                               float(px1 - fx1));
   }
 }
 
@@ -181,9 +237,9 @@ int piv_deform_images(int32_t dtype, const piv_image* images, int32_t npairs,
   if (!grid->iyvecs || !grid->ixvecs || !grid->cell_y || !grid->t_y || !grid->cell_x ||
       !grid->t_x)
     return deform_fail(PIV_EINVAL, "NULL grid table");
+  const int nvec = grid->ny * grid->nx;
   if (grid->nx > kMaxGridColumns)
     return deform_fail(PIV_EINVAL, "deformation grid wider than 4096 nodes");
-  const int nvec = grid->ny * grid->nx;
   const size_t lds = size_t(2) * grid->nx * sizeof(double);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const dim3 gr(static_cast<unsigned>(int64_t(height) * npairs));

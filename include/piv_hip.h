@@ -499,8 +499,8 @@ typedef struct piv_deform_grid {
  * npairs x ny x nx float64) on the grid. `out` (device float32,
  * 2 npairs x height x width) receives them in image order; the deformed
  * window of a grid node is the crop of these images around the node.
- * `grid` is a host pointer to a record of device pointers; grid->nx is at
- * most 4096. */
+ * Weights and tap sums are float32, positions FP64. `grid` is a host
+ * pointer to a record of device pointers; grid->nx is at most 4096. */
 int piv_deform_images(int32_t dtype, const piv_image* images, int32_t npairs,
                       int32_t height, int32_t width, const piv_deform_grid* grid,
                       const double* u, const double* v, float* out,

@@ -1,7 +1,7 @@
 """GPU image deformation between passes against the oracle (oracle/deform.py).
 
-Bars: deformed windows within float32 rounding of the FP64 oracle (values
-up to 4095: 1e-3 absolute); multipass displacements within 1e-3 px with
+Bars: deformed windows within float32 rounding of the FP64 oracle (4e-6 of
+the largest value: 0.016 for 12-bit images); multipass displacements within 1e-3 px with
 identical error strings for every vector except the listed near-ties of the
 FP64 maps and the vectors downstream of them; the deformation lowers the
 error against the synthetic truth field."""
@@ -60,9 +60,13 @@ def test_deformed_windows_match_oracle(window, dtype):
     g1 = np.stack([d1[y - n0 // 2:y - n0 // 2 + n0, x - n1 // 2:x - n1 // 2 + n1]
                    for y in w.iyvecs for x in w.ixvecs])
     e0, e1 = deformed_crops(im0, im1, w.iyvecs, w.ixvecs, u, v, (n0, n1))
+    # float32 weights and 16-tap sums against the FP64 oracle: a few
+    # float32 roundings of terms up to the image's largest value
     scale = max(1.0, float(np.abs(e0).max()))
-    assert np.abs(g0 - e0).max() <= 2e-7 * scale + 1e-4
-    assert np.abs(g1 - e1).max() <= 2e-7 * scale + 1e-4
+    err0, err1 = np.abs(g0 - e0).max(), np.abs(g1 - e1).max()
+    print(f"deformed windows: max |diff| {err0:.3g}, {err1:.3g} (scale {scale:.0f})", flush=True)
+    assert err0 <= 4e-6 * scale + 1e-4
+    assert err1 <= 4e-6 * scale + 1e-4
 
 
 def _setup(shape, number=3, window=64):
