@@ -166,6 +166,12 @@ SIGNATURES = {
         ctypes.c_int,
         [_I, _I, _P, _P, _P, ctypes.c_int64, ctypes.c_int64, _I, _P, _P, _P, _P, _P, _P]),
     "piv_tps_at_centers": (ctypes.c_int, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "piv_tps_shared_workspace": (ctypes.c_int64, [_I, _I, _I, _I, _I]),
+    "piv_tps_smooth_shared": (
+        ctypes.c_int,
+        [_I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P,
+         _P, _P, _P, ctypes.c_double, _P, _P, _P, _P, _P, ctypes.c_int64, _P, ctypes.c_int64,
+         _P, _P, _P, _P, _I, _P, _P]),
     "piv_tps_interpolate": (
         ctypes.c_int,
         [_I, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _I, _I, _P, _P, ctypes.c_int64, _P, _P,

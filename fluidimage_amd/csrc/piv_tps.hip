@@ -219,6 +219,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
   static_assert(NT >= kThreads && NT % 64 == 0, "block sums over the first kThreads threads");
   constexpr int kWaves = NT / 64;
   const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
+  if (d < 0) return;  // an order entry for another kernel (solved by patterns)
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
   if (n == 0) return;  // a subdomain slot without centers (device-sized batches)
@@ -467,6 +468,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                            PhiLattice lat, bool gen,
                                                            const int32_t* __restrict__ order) {
   const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
+  if (d < 0) return;  // an order entry for another kernel (solved by patterns)
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
   if (n == 0) return;  // a subdomain slot without centers (device-sized batches)
@@ -1032,6 +1034,256 @@ __device__ void chol_solve(double* xa, double* xb, const double* A, int64_t ld, 
   }
 }
 
+// ---- the same solves for NV right-hand sides at once (x[v nm + j] in LDS):
+// one pass over the factor serves them all ----
+// block sums of NV values (NT = kThreads)
+template <int NV>
+__device__ void block_sum_n(double v[NV], double* red) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[q * (kThreads / 64) + (threadIdx.x >> 6)] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double t = 0.0;
+    for (int k = 0; k < kThreads / 64; ++k) t += red[q * (kThreads / 64) + k];
+    v[q] = t;
+  }
+  __syncthreads();
+}
+
+template <int NV>
+__device__ void apply_q_n(double* x, int nm, int n, const double* VW, const double* tau,
+                          bool transpose, double* red) {
+  for (int kk = 0; kk < 3; ++kk) {
+    const int k = transpose ? kk : 2 - kk;
+    double p[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) p[q] = 0.0;
+    for (int j = threadIdx.x; j < n; j += kThreads) {
+      const double vk = VW[6 * j + k];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) p[q] += vk * x[q * nm + j];
+    }
+    block_sum_n<NV>(p, red);
+    for (int j = threadIdx.x; j < n; j += kThreads) {
+      const double vk = tau[k] * VW[6 * j + k];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) x[q * nm + j] -= vk * p[q];
+    }
+    __syncthreads();
+  }
+}
+
+// LDS scratch of chol_solve_n (doubles): the diagonal block, the block
+// right-hand sides and the partial sums of up to 8 vectors
+constexpr int kSolveScratch = kNB * (kNB + 1) + 8 * kNB + 8 * 8 * kNB;
+
+// C z = b in place for NV vectors (x[q nm + 3 + i]); as chol_solve. scr:
+// kSolveScratch doubles of LDS.
+template <int NV>
+__device__ void chol_solve_n(double* x, int nm, const double* A, int64_t ld, int m, double* scr) {
+  static_assert(kThreads == 256 && kNB == 32 && NV <= 8, "thread layout of chol_solve_n");
+  double(*dblk)[kNB + 1] = reinterpret_cast<double(*)[kNB + 1]>(scr);
+  double(*tmp)[kNB] = reinterpret_cast<double(*)[kNB]>(scr + kNB * (kNB + 1));
+  double(*part)[8][kNB] = reinterpret_cast<double(*)[8][kNB]>(scr + kNB * (kNB + 1) + 8 * kNB);
+  const int tid = threadIdx.x;
+  const int nblk = (m + kNB - 1) / kNB;
+  double* xs = x + 3;
+  for (int J = 0; J < nblk; ++J) {
+    const int r0 = J * kNB;
+    const int bs = min(kNB, m - r0);
+    load_diag_block(dblk, A, ld, r0, bs);
+    {
+      const int i = tid >> 3, p = tid & 7;
+      double sv[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) sv[q] = 0.0;
+      if (i < bs) {
+        const double* Lr = A + int64_t(r0 + i) * ld + 4 * p;
+        constexpr int QB = kBatch / 4;
+        for (int q0 = 0; q0 < J; q0 += QB) {
+          double l[kBatch];
+#pragma unroll
+          for (int qq = 0; qq < QB; ++qq) {
+            const int qb = min(q0 + qq, J - 1);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) l[4 * qq + u] = Lr[32 * qb + u];
+          }
+#pragma unroll
+          for (int qq = 0; qq < QB; ++qq) {
+            if (q0 + qq < J) {
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int k = 32 * (q0 + qq) + 4 * p + u;
+#pragma unroll
+                for (int q = 0; q < NV; ++q) sv[q] += l[4 * qq + u] * xs[q * nm + k];
+              }
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int o = 4; o >= 1; o >>= 1) sv[q] += __shfl_xor(sv[q], o, 64);
+      if (p == 0 && i < bs)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) tmp[q][i] = xs[q * nm + r0 + i] - sv[q];
+    }
+    __syncthreads();
+    if (tid < 32 * NV) {
+      const int c = tid & 31, comp = tid >> 5;
+      if (c < bs) {
+        double sacc = tmp[comp][c] / dblk[c][c];
+#pragma unroll 4
+        for (int k = 0; k < kNB; ++k)
+          if (k < c) sacc += dblk[k][c] * tmp[comp][k];
+        xs[comp * nm + r0 + c] = sacc;
+      }
+    }
+    __syncthreads();
+  }
+  for (int J = nblk - 1; J >= 0; --J) {
+    const int r0 = J * kNB;
+    const int bs = min(kNB, m - r0);
+    load_diag_block(dblk, A, ld, r0, bs);
+    {
+      const int c = tid & 31, g = tid >> 5;
+      double sv[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) sv[q] = 0.0;
+      if (c < bs) {
+        const double* Lc = A + r0 + c;
+        for (int i0 = r0 + bs + g; i0 < m; i0 += 8 * kBatch) {
+          double l[kBatch];
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u) l[u] = Lc[int64_t(min(i0 + 8 * u, m - 1)) * ld];
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u) {
+            const int i = i0 + 8 * u;
+            if (i < m)
+#pragma unroll
+              for (int q = 0; q < NV; ++q) sv[q] += l[u] * xs[q * nm + i];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NV; ++q) part[q][g][c] = sv[q];
+    }
+    __syncthreads();
+    if (tid < 32 * NV) {
+      const int c = tid & 31, comp = tid >> 5;
+      double sacc = 0.0;
+#pragma unroll
+      for (int g = 0; g < 8; ++g) sacc += part[comp][g][c];
+      tmp[comp][c] = c < bs ? xs[comp * nm + r0 + c] - sacc : 0.0;
+    }
+    __syncthreads();
+    if (tid < 32 * NV) {
+      const int c = tid & 31, comp = tid >> 5;
+      if (c < bs) {
+        double sacc = tmp[comp][c] / dblk[c][c];
+#pragma unroll 4
+        for (int k = 0; k < kNB; ++k)
+          if (k > c && k < bs) sacc += dblk[c][k] * tmp[comp][k];
+        xs[comp * nm + r0 + c] = sacc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// saddle_solve for NV right-hand sides x[q nm + j]; the polynomial terms go
+// to aout[q][0..2] (LDS, written by thread 0, read after the call)
+template <int NV>
+__device__ void saddle_solve_n(double* x, int nm, int n, const double* VW, const double* tau,
+                               const double* A, int64_t ld, const double* E, const double* RF,
+                               double* red, double* scr, double (*aout)[3]) {
+  const int tid = threadIdx.x;
+  const int m = n - 3;
+  // Q^T x; the first three entries (the a-part of the rhs) are left alone by the
+  // Cholesky solve of the others
+  apply_q_n<NV>(x, nm, n, VW, tau, true, red);
+  if (m > 0) chol_solve_n<NV>(x, nm, A, ld, m, scr);
+  // a = R^-1 (vt - B'[:3, 3:] z), B'[:3, 3 + j] = B'[3 + j][k] = E[j][k]
+  for (int k = 0; k < 3; ++k) {
+    double bz[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) bz[q] = 0.0;
+    for (int j = tid; j < m; j += kThreads) {
+      const double b = E[3 * int64_t(j) + k];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) bz[q] += b * x[q * nm + 3 + j];
+    }
+    block_sum_n<NV>(bz, red);
+    if (tid == 0)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) aout[q][k] = x[q * nm + k] - bz[q];
+  }
+  if (tid == 0)
+    for (int q = 0; q < NV; ++q) {
+      const double r0 = aout[q][0], r1 = aout[q][1], r2 = aout[q][2];
+      const double a2 = r2 / RF[8];
+      const double a1 = (r1 - RF[7] * a2) / RF[6];
+      aout[q][0] = (r0 - RF[4] * a1 - RF[5] * a2) / RF[3];
+      aout[q][1] = a1;
+      aout[q][2] = a2;
+      x[q * nm] = x[q * nm + 1] = x[q * nm + 2] = 0.0;
+    }
+  __syncthreads();
+  apply_q_n<NV>(x, nm, n, VW, tau, false, red);
+}
+
+// (w, a) = M^-1 [x; 0] for the two right-hand sides x0, x1 (length n, in
+// LDS) of the saddle-point system M = [[K + sI, P], [P^T, 0]] of n centers,
+// by the null-space factorisation of M (VW, tau, RF: the reflectors, R and
+// T; A, ld: the Cholesky factor of C; E = B'[3:, :3]): on return x0, x1 hold
+// w and a[c] the polynomial terms [1, y, x]. Every thread of the block
+// calls it.
+__device__ void saddle_solve(double* x0, double* x1, int n, const double* VW, const double* tau,
+                             const double* A, int64_t ld, const double* E, const double* RF,
+                             double* red, double a[2][3]) {
+  const int tid = threadIdx.x;
+  const int m = n - 3;
+  double* x[2] = {x0, x1};
+  // rhs in the rotated basis; its a-part kept in registers: vt[0..2]
+  apply_q(x0, x1, n, VW, tau, true, red);
+  double vt[2][3];
+  for (int c = 0; c < 2; ++c)
+    for (int k = 0; k < 3; ++k) vt[c][k] = x[c][k];
+  __syncthreads();
+  if (m > 0) chol_solve(x0, x1, A, ld, m);
+  // a = R^-1 (vt - B'[:3, 3:] z), B'[:3, 3 + j] = B'[3 + j][k] = E[j][k]
+  double bz[2][3];
+  for (int k = 0; k < 3; ++k) {
+    double p0 = 0.0, p1 = 0.0;
+    for (int j = tid; j < m; j += kThreads) {
+      const double b = E[3 * int64_t(j) + k];
+      p0 += b * x0[3 + j];
+      p1 += b * x1[3 + j];
+    }
+    bz[0][k] = block_sum(p0, red);
+    bz[1][k] = block_sum(p1, red);
+  }
+  for (int c = 0; c < 2; ++c) {
+    const double r0 = vt[c][0] - bz[c][0], r1 = vt[c][1] - bz[c][1], r2 = vt[c][2] - bz[c][2];
+    a[c][2] = r2 / RF[8];
+    a[c][1] = (r1 - RF[7] * a[c][2]) / RF[6];
+    a[c][0] = (r0 - RF[4] * a[c][1] - RF[5] * a[c][2]) / RF[3];
+  }
+  // w = Q [0; z]
+  if (tid < 2) {
+    x[tid][0] = x[tid][1] = x[tid][2] = 0.0;
+  }
+  __syncthreads();
+  apply_q(x0, x1, n, VW, tau, false, red);
+}
+
 __global__ __launch_bounds__(kThreads) void k_tps_solve(
     const float* __restrict__ vx, const float* __restrict__ vy,
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
@@ -1042,6 +1294,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
     double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
     const int32_t* __restrict__ status, int nm, const int32_t* __restrict__ order) {
   const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
+  if (d < 0) return;  // an order entry for another kernel (solved by patterns)
   if (status[d]) return;
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
@@ -1077,44 +1330,14 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
   int count[2] = {0, 0};
   for (int it = 0; it < 11; ++it) {
     if (!go[0] && !go[1]) break;
-    // rhs in the rotated basis
+    // rhs, then (w, a) = M^-1 [v; 0]
     for (int j = tid; j < n; j += kThreads) {
       x[0][j] = val[0][j];
       x[1][j] = val[1][j];
     }
     __syncthreads();
-    apply_q(x[0], x[1], n, VW, tau, true, red);
-    // a-part rhs kept in registers: vt[0..2]
-    double vt[2][3];
-    for (int c = 0; c < 2; ++c)
-      for (int k = 0; k < 3; ++k) vt[c][k] = x[c][k];
-    __syncthreads();
-    if (m > 0) chol_solve(x[0], x[1], A, ld, m);
-    // a = R^-1 (vt - B'[:3, 3:] z), B'[:3, 3 + j] = B'[3 + j][k] = E[j][k]
-    double bz[2][3];
-    for (int k = 0; k < 3; ++k) {
-      double p0 = 0.0, p1 = 0.0;
-      for (int j = tid; j < m; j += kThreads) {
-        const double b = E[3 * int64_t(j) + k];
-        p0 += b * x[0][3 + j];
-        p1 += b * x[1][3 + j];
-      }
-      bz[0][k] = block_sum(p0, red);
-      bz[1][k] = block_sum(p1, red);
-    }
     double a[2][3];
-    for (int c = 0; c < 2; ++c) {
-      const double r0 = vt[c][0] - bz[c][0], r1 = vt[c][1] - bz[c][1], r2 = vt[c][2] - bz[c][2];
-      a[c][2] = r2 / RF[8];
-      a[c][1] = (r1 - RF[7] * a[c][2]) / RF[6];
-      a[c][0] = (r0 - RF[4] * a[c][1] - RF[5] * a[c][2]) / RF[3];
-    }
-    // w = Q [0; z]
-    if (tid < 2) {
-      x[tid][0] = x[tid][1] = x[tid][2] = 0.0;
-    }
-    __syncthreads();
-    apply_q(x[0], x[1], n, VW, tau, false, red);
+    saddle_solve(x[0], x[1], n, VW, tau, A, ld, E, RF, red, a);
     // smoothed = v - s w, differences, erratic vectors
     for (int c = 0; c < 2; ++c) {
       if (!go[c]) continue;  // uniform across the block
@@ -1787,6 +2010,445 @@ __global__ void k_tps_at_centers(int n, const double* __restrict__ info,
 }
 
 // ---------------------------------------------------------------------------
+// 4c. subdomains sharing one factorisation (centers on a grid)
+// ---------------------------------------------------------------------------
+// When every center of a field is a node of the pass grid gx[ngx] x gy[ngy]
+// (the first pass; passes on deformed images), the members S of a
+// subdomain are the valid nodes of its box F (the grid nodes inside its
+// half-open limits). Boxes with the same numbers of columns and rows (and
+// the same spacings: the grid is regular but for a last column or row
+// appended at a smaller step) hold the same node set up to a translation;
+// with the polynomial part P = [1, y - y0, x - x0] relative to the box's
+// first node their saddle-point matrices M_F are identical, so one
+// factorisation per field and pattern serves every such box. A subdomain
+// missing few nodes R = F \ S (the invalid vectors) is solved with it and a
+// capacitance correction: with v' = v on S, 0 on R,
+//   M_F [w; a] = [v'; 0] + E_R lambda,  w_R = 0,
+// whose rows of S are M_S [w_S; a] = [v_S; 0]. For G = M_F^-1 E_R and
+// H = G_w[R, :] (|R| x |R|, positive definite when M_S is regular),
+// lambda = -H^-1 y_w[R] with y = M_F^-1 [v'; 0], and (w, a) = y + G lambda.
+// The outlier iterations of the reference change v only, so G and H are
+// computed once per subdomain.
+constexpr int kCapMaxR = 12;     // missing nodes of a subdomain solved with a shared factor
+constexpr int kPatMaxPerField = 64;
+
+// first index in [0, n) with a[i] >= v (a ascending), n if none
+__device__ __forceinline__ int lower_bound_d(const double* a, int n, double v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// One workgroup per field: the box of every subdomain on the grid, and the
+// field's patterns. dmode[d]: 1 solved with pattern dpat[d] (global index
+// f npat_cap + id), 0 direct factorisation, -1 no centers. dbox[4 d]:
+// first column, columns, first row, rows. pat_n[p]: nodes of pattern p (0:
+// unused), pat_box[4 p] its reference box, pat_smooth[p] its smoothing.
+__global__ __launch_bounds__(256) void k_tps_patterns(
+    int dom_cap, const piv_tps_field* __restrict__ fields, const double* __restrict__ lim,
+    const int32_t* __restrict__ fstat, const int32_t* __restrict__ dom_field,
+    const int32_t* __restrict__ counts, const double* __restrict__ smoothing,
+    const double* __restrict__ gx, int ngx, int irr_x, const double* __restrict__ gy, int ngy,
+    int irr_y, int cap_r, int n_max_cap, int npat_cap, int32_t* __restrict__ dmode,
+    int32_t* __restrict__ dpat, int32_t* __restrict__ dbox, int32_t* __restrict__ pat_n,
+    int32_t* __restrict__ pat_box, double* __restrict__ pat_smooth) {
+  const int f = blockIdx.x;
+  const int64_t d0 = int64_t(f) * dom_cap;
+  const bool ok = fstat[f] == PIV_TPS_FIELD_OK;
+  const piv_tps_field fr = fields[f];
+  const SubGrid g = field_grid(fr, lim);
+  for (int d = threadIdx.x; d < dom_cap; d += blockDim.x) {
+    const int64_t dd = d0 + d;
+    dmode[dd] = -1;
+    dpat[dd] = -1;
+    if (!ok || dom_field[dd] != f || counts[dd] == 0) continue;
+    const int ix = d % g.nsx, iy = d / g.nsx;
+    const int j0 = lower_bound_d(gx, ngx, g.xlo(ix)), j1 = lower_bound_d(gx, ngx, g.xhi(ix));
+    const int i0 = lower_bound_d(gy, ngy, g.ylo(iy)), i1 = lower_bound_d(gy, ngy, g.yhi(iy));
+    dbox[4 * dd + 0] = j0;
+    dbox[4 * dd + 1] = j1 - j0;
+    dbox[4 * dd + 2] = i0;
+    dbox[4 * dd + 3] = i1 - i0;
+    dmode[dd] = 0;
+  }
+  for (int q = threadIdx.x; q < npat_cap; q += blockDim.x) {
+    pat_n[int64_t(f) * npat_cap + q] = 0;
+    pat_smooth[int64_t(f) * npat_cap + q] = smoothing[d0];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0 || !ok) return;
+  // pattern keys: columns, rows, and whether the box ends on an irregular
+  // last column / row of the grid
+  int keys[kPatMaxPerField];
+  int npat = 0;
+  for (int d = 0; d < dom_cap; ++d) {
+    const int64_t dd = d0 + d;
+    if (dmode[dd] != 0) continue;
+    const int j0 = dbox[4 * dd], nc = dbox[4 * dd + 1], i0 = dbox[4 * dd + 2],
+              nr = dbox[4 * dd + 3];
+    const int nf = nc * nr;
+    if (nf < 4 || nf > n_max_cap || nf - counts[dd] > cap_r) continue;  // direct
+    const int key = ((nc * 4096 + nr) * 2 + ((j0 + nc == ngx && irr_x) ? 1 : 0)) * 2 +
+                    ((i0 + nr == ngy && irr_y) ? 1 : 0);
+    int id = -1;
+    for (int k = 0; k < npat; ++k)
+      if (keys[k] == key) id = k;
+    if (id < 0) {
+      if (npat >= npat_cap || npat >= kPatMaxPerField) continue;  // no room: direct
+      id = npat++;
+      keys[id] = key;
+      const int64_t p = int64_t(f) * npat_cap + id;
+      pat_n[p] = nf;
+      pat_box[4 * p + 0] = j0;
+      pat_box[4 * p + 1] = nc;
+      pat_box[4 * p + 2] = i0;
+      pat_box[4 * p + 3] = nr;
+    }
+    dmode[dd] = 1;
+    dpat[dd] = int(int64_t(f) * npat_cap + id);
+  }
+}
+
+// One workgroup of 1024 threads: offsets of the patterns' centers and
+// matrices (prefix sums of their sizes), their centers (grid nodes relative
+// to the box's first node, row-major: the order of the members of a
+// subdomain), an identity slot list, and the orders of the direct
+// subdomains (from `order`, largest first) and of the shared ones (by
+// index); -1 beyond.
+__global__ __launch_bounds__(1024) void k_tps_pattern_layout(
+    int npat, const int32_t* __restrict__ pat_n, const int32_t* __restrict__ pat_box,
+    const double* __restrict__ gx, const double* __restrict__ gy, int nd,
+    const int32_t* __restrict__ order, const int32_t* __restrict__ dmode,
+    int64_t* __restrict__ pdom_off, int64_t* __restrict__ pmat_off, double* __restrict__ pxy,
+    int32_t* __restrict__ pslot, int64_t pxy_stride, int32_t* __restrict__ order_direct,
+    int32_t* __restrict__ order_shared) {
+  __shared__ int64_t tmp64[32];
+  __shared__ int tmp[32];
+  const int tid = threadIdx.x;
+  int64_t base_s = 0, base_m = 0;
+  for (int c0 = 0; c0 < npat; c0 += 1024) {
+    const int p = c0 + tid;
+    const int c = p < npat ? pat_n[p] : 0;
+    int64_t ts, tm;
+    const int64_t is = block_scan_1024_64(int64_t(c), tmp64, ts);
+    const int64_t im = block_scan_1024_64(c > 0 ? tps_mat_size(c) : 0, tmp64, tm);
+    if (p < npat) {
+      pdom_off[p] = base_s + is - c;
+      pmat_off[p] = base_m + im - (c > 0 ? tps_mat_size(c) : 0);
+    }
+    base_s += ts;
+    base_m += tm;
+  }
+  if (tid == 0) {
+    pdom_off[npat] = base_s;
+    pmat_off[npat] = base_m;
+  }
+  __syncthreads();
+  for (int p = 0; p < npat; ++p) {
+    const int n = pat_n[p];
+    if (n == 0) continue;
+    const int j0 = pat_box[4 * p], nc = pat_box[4 * p + 1], i0 = pat_box[4 * p + 2];
+    const int64_t o = pdom_off[p];
+    for (int k = tid; k < n; k += 1024) {
+      const int r = k / nc, c = k % nc;
+      pxy[o + k] = gx[j0 + c] - gx[j0];
+      pxy[pxy_stride + o + k] = gy[i0 + r] - gy[i0];
+      pslot[o + k] = int(o + k);
+    }
+  }
+  // the direct subdomains in the order of `order`; the shared ones in index
+  // order: a field's subdomains are consecutive and most of them share the
+  // field's interior pattern, so the workgroups running at one time read
+  // few factors (which stay in the L2 cache)
+  int bd = 0, bs = 0;
+  for (int c0 = 0; c0 < nd; c0 += 1024) {
+    const int b = c0 + tid;
+    const int d = b < nd ? order[b] : -1;
+    const int md = d >= 0 ? dmode[d] : -1;
+    const int ms = b < nd ? dmode[b] : -1;
+    int td, tsh;
+    const int idd = block_scan_1024(md == 0 ? 1 : 0, tmp, td);
+    const int ish = block_scan_1024(ms == 1 ? 1 : 0, tmp, tsh);
+    if (md == 0) order_direct[bd + idd - 1] = d;
+    if (ms == 1) order_shared[bs + ish - 1] = b;
+    bd += td;
+    bs += tsh;
+  }
+  for (int b = bd + tid; b < nd; b += 1024) order_direct[b] = -1;
+  for (int b = bs + tid; b < nd; b += 1024) order_shared[b] = -1;
+}
+
+// Solve of a subdomain with its pattern's factorisation and the capacitance
+// correction of its missing nodes, with the reference's outlier iterations
+// (thin_plate_spline_subdom.py:236-280, as k_tps_solve); writes what
+// k_tps_solve and k_tps_prep write for it (smoothed values and weights by
+// slot, the polynomial part in image coordinates, stats, slot_xy). The
+// first pass over the factor solves the two value vectors with up to 6
+// columns of G, a second one the next 6: at most kCapMaxR = 12 missing
+// nodes; then one pass per outlier iteration.
+constexpr int kSharedNV = 8;
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_tps_solve_shared(
+    const int32_t* __restrict__ order, const int32_t* __restrict__ dpat,
+    const int32_t* __restrict__ dbox, const int32_t* __restrict__ pat_n,
+    const int64_t* __restrict__ pdom_off, const int64_t* __restrict__ pmat_off,
+    const double* __restrict__ pmats, const double* __restrict__ prefl,
+    const double* __restrict__ pvwork, const int32_t* __restrict__ pstatus,
+    const double* __restrict__ gx, const double* __restrict__ gy, int ngx,
+    const double* __restrict__ xs, const double* __restrict__ ys,
+    const int32_t* __restrict__ idx, const float* __restrict__ vx,
+    const float* __restrict__ vy, const int64_t* __restrict__ dom_off,
+    const int32_t* __restrict__ slot_pt, const double* __restrict__ smoothing_d,
+    double threshold, double* __restrict__ smooth_slots, int64_t nslots,
+    double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
+    int32_t* __restrict__ status, double* __restrict__ slot_xy, double* __restrict__ gmat,
+    int cap_r, int nm) {
+  const int d = order[blockIdx.x];
+  if (d < 0) return;
+  const int p = dpat[d];
+  const int64_t s0 = dom_off[d];
+  const int n = int(dom_off[d + 1] - s0);
+  const int nf = pat_n[p];
+  const int tid = threadIdx.x;
+  if (pstatus[p]) {
+    if (tid == 0) status[d] = 1;
+    return;
+  }
+  const int j0 = dbox[4 * d], nc = dbox[4 * d + 1], i0 = dbox[4 * d + 2];
+  const int64_t ps0 = pdom_off[p];
+  const int m = nf - 3;
+  const double* E = pmats + pmat_off[p];
+  const int64_t ld = tps_mat_ld(m);
+  const double* A = E + tps_mat_e_size(m);
+  const double* RF = prefl + 16 * p;
+  const double* VW = pvwork + 6 * ps0;
+  const double tau[3] = {RF[0], RF[1], RF[2]};
+  const double smoothing = smoothing_d[d];
+  const double x0 = gx[j0], y0 = gy[i0];
+  double* G = gmat + int64_t(d) * nm * cap_r;  // columns of G_w, nm apart
+  // dynamic LDS: x[kSharedNV][nm] (doubles), val[2][nm] (floats), fidx[nm] (node of a
+  // member), fmem[nm] (member of a node, -1 for a missing node), then the
+  // solve's scratch
+  extern __shared__ double dyn[];
+  double* x = dyn;
+  float* val[2] = {reinterpret_cast<float*>(dyn + kSharedNV * nm),
+                   reinterpret_cast<float*>(dyn + kSharedNV * nm) + nm};
+  int* fidx = reinterpret_cast<int*>(dyn + (kSharedNV + 1) * nm);
+  int* fmem = fidx + nm;
+  double* scr = dyn + (kSharedNV + 2) * nm;
+  __shared__ double red[kSharedNV * (kThreads / 64)];
+  __shared__ int rlist[kCapMaxR];
+  __shared__ int nr_sh;
+  __shared__ double H[kCapMaxR][kCapMaxR + 1];
+  __shared__ double ga[kCapMaxR][3];
+  __shared__ double lam[2][kCapMaxR];
+  __shared__ double a8[kSharedNV][3];  // polynomial terms of the solved vectors
+  __shared__ int go[2];
+  __shared__ int bad;
+  if (tid == 0) {
+    bad = 0;
+    nr_sh = 0;
+  }
+  for (int k = tid; k < nf; k += kThreads) fmem[k] = -1;
+  __syncthreads();
+  // members: their node in the box, their values, their coordinates by slot
+  for (int j = tid; j < n; j += kThreads) {
+    const int c = slot_pt[s0 + j];
+    const int q = idx[c];
+    const int row = q / ngx, col = q % ngx;
+    const int k = (row - i0) * nc + (col - j0);
+    const bool on = col - j0 >= 0 && col - j0 < nc && k >= 0 && k < nf && xs[c] == gx[col] &&
+                    ys[c] == gy[row];
+    if (!on) {
+      bad = 1;  // a center off the grid: the field is solved again by host sizing
+      continue;
+    }
+    fidx[j] = k;
+    fmem[k] = j;
+    val[0][j] = vx[c];
+    val[1][j] = vy[c];
+    slot_xy[s0 + j] = xs[c];
+    slot_xy[nslots + s0 + j] = ys[c];
+  }
+  __syncthreads();
+  if (bad) {
+    if (tid == 0) status[d] = 2;
+    return;
+  }
+  // the missing nodes, in node order
+  if (tid == 0) {
+    int r = 0;
+    for (int k = 0; k < nf && r <= cap_r; ++k)
+      if (fmem[k] < 0) {
+        if (r < kCapMaxR) rlist[r] = k;
+        ++r;
+      }
+    nr_sh = r;
+  }
+  __syncthreads();
+  const int nr = nr_sh;
+  if (nr > cap_r || nr > kCapMaxR || n + nr != nf) {
+    if (tid == 0) status[d] = 2;
+    return;
+  }
+  // pass 0: y = M_F^-1 [v'; 0] (vectors 0, 1) with G columns 0..5 (vectors
+  // 2..7); pass 1: G columns 6..11 (vectors 2..7)
+  for (int pass = 0; pass < 2; ++pass) {
+    const int g0 = 6 * pass;
+    if (pass == 1 && nr <= 6) break;
+    for (int k = tid; k < nf; k += kThreads) {
+      if (pass == 0) {
+        const int j = fmem[k];
+        x[k] = j >= 0 ? double(val[0][j]) : 0.0;
+        x[nm + k] = j >= 0 ? double(val[1][j]) : 0.0;
+      }
+#pragma unroll
+      for (int q = 2; q < kSharedNV; ++q)
+        x[q * nm + k] = (g0 + q - 2 < nr && k == rlist[g0 + q - 2]) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (pass == 0) {
+      saddle_solve_n<kSharedNV>(x, nm, nf, VW, tau, A, ld, E, RF, red, scr, a8);
+    } else {
+      // vectors 0, 1 (y) and a8[0..1] are kept: solve vectors 2..7 only
+      saddle_solve_n<kSharedNV - 2>(x + 2 * nm, nm, nf, VW, tau, A, ld, E, RF, red, scr,
+                                    a8 + 2);
+    }
+    for (int q = 2; q < kSharedNV; ++q) {
+      const int r = g0 + q - 2;
+      if (r >= nr) break;
+      for (int k = tid; k < nf; k += kThreads) G[int64_t(r) * nm + k] = x[q * nm + k];
+      if (tid < nr) H[tid][r] = x[q * nm + rlist[tid]];
+      if (tid == 0)
+        for (int t = 0; t < 3; ++t) ga[r][t] = a8[q][t];
+    }
+    __syncthreads();
+  }
+  // H = L_H L_H^T in place (lower triangle), one thread
+  if (tid == 0) {
+    for (int j = 0; j < nr; ++j) {
+      double dj = H[j][j];
+      for (int k = 0; k < j; ++k) dj -= H[j][k] * H[j][k];
+      if (!(dj > 0.0)) {
+        bad = 1;
+        break;
+      }
+      dj = sqrt(dj);
+      H[j][j] = dj;
+      for (int i = j + 1; i < nr; ++i) {
+        double v = H[i][j];
+        for (int k = 0; k < j; ++k) v -= H[i][k] * H[j][k];
+        H[i][j] = v / dj;
+      }
+    }
+  }
+  __syncthreads();
+  if (bad) {
+    if (tid == 0) status[d] = 1;  // M_S singular, as the direct factorisation reports it
+    return;
+  }
+  if (tid < 2) go[tid] = 1;
+  __syncthreads();
+  double nfixed[2] = {0.0, 0.0}, maxdiff[2] = {0.0, 0.0};
+  int count[2] = {0, 0};
+  for (int it = 0; it < 11; ++it) {
+    if (!go[0] && !go[1]) break;
+    if (it > 0) {  // y for the updated values (the first one came with G)
+      for (int k = tid; k < nf; k += kThreads) {
+        const int j = fmem[k];
+        x[k] = j >= 0 ? double(val[0][j]) : 0.0;
+        x[nm + k] = j >= 0 ? double(val[1][j]) : 0.0;
+      }
+      __syncthreads();
+      saddle_solve_n<2>(x, nm, nf, VW, tau, A, ld, E, RF, red, scr, a8);
+    }
+    // lambda = -H^-1 y_w[R] (two triangular solves with L_H)
+    if (tid < 2) {
+      const int c = tid;
+      double z[kCapMaxR];
+      for (int i = 0; i < nr; ++i) {
+        double v = -x[c * nm + rlist[i]];
+        for (int k = 0; k < i; ++k) v -= H[i][k] * z[k];
+        z[i] = v / H[i][i];
+      }
+      for (int i = nr - 1; i >= 0; --i) {
+        double v = z[i];
+        for (int k = i + 1; k < nr; ++k) v -= H[k][i] * lam[c][k];
+        lam[c][i] = v / H[i][i];
+      }
+    }
+    __syncthreads();
+    // w = y + G lambda at the members; a = y_a + G_a lambda
+    double wa[2][3];
+    for (int c = 0; c < 2; ++c)
+      for (int t = 0; t < 3; ++t) {
+        double v = a8[c][t];
+        for (int k = 0; k < nr; ++k) v += ga[k][t] * lam[c][k];
+        wa[c][t] = v;
+      }
+    for (int j = tid; j < n; j += kThreads) {
+      const int k = fidx[j];
+      double w0 = x[k], w1 = x[nm + k];
+      for (int r = 0; r < nr; ++r) {
+        const double gk = G[int64_t(r) * nm + k];
+        w0 += gk * lam[0][r];
+        w1 += gk * lam[1][r];
+      }
+      x[k] = w0;
+      x[nm + k] = w1;
+    }
+    __syncthreads();
+    for (int c = 0; c < 2; ++c) {
+      if (!go[c]) continue;  // uniform across the block
+      count[c] += 1;
+      double dmax = 0.0;
+      int ne = 0;
+      for (int j = tid; j < n; j += kThreads) {
+        const double v = double(val[c][j]);
+        const double sm = v - smoothing * x[c * nm + fidx[j]];
+        smooth_slots[c * nslots + s0 + j] = sm;
+        const double diff = fabs(sm - v);
+        dmax = fmax(dmax, diff);
+        if (threshold >= 0.0 && diff > threshold) ++ne;
+      }
+      double* W = weights + c * wstride + s0 + 3 * int64_t(d);
+      for (int j = tid; j < n; j += kThreads) W[j] = x[c * nm + fidx[j]];
+      if (tid == 0) {
+        // P = [1, y - y0, x - x0] -> [1, y, x]
+        W[n] = wa[c][0] - wa[c][1] * y0 - wa[c][2] * x0;
+        W[n + 1] = wa[c][1];
+        W[n + 2] = wa[c][2];
+      }
+      const double dm = block_max(dmax, red);
+      const double nesum = block_sum(double(ne), red);
+      if (count[c] == 1) maxdiff[c] = dm;
+      const bool more = threshold >= 0.0 && nesum > 0.0 && count[c] < 11;
+      if (more) {
+        nfixed[c] += nesum;
+        for (int j = tid; j < n; j += kThreads) {
+          const double v = double(val[c][j]);
+          const double sm = v - smoothing * x[c * nm + fidx[j]];
+          if (fabs(sm - v) > threshold) val[c][j] = float(sm);
+        }
+      }
+      __syncthreads();
+      if (tid == 0) go[c] = more ? 1 : 0;
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    for (int c = 0; c < 2; ++c) {
+      stats[6 * d + 3 * c + 0] = nfixed[c];
+      stats[6 * d + 3 * c + 1] = maxdiff[c];
+      stats[6 * d + 3 * c + 2] = count[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 5. blend of the smoothed values, 6. evaluation at new positions
 // ---------------------------------------------------------------------------
 __global__ void k_tps_blend(int npts, const int32_t* __restrict__ slots,
@@ -2130,32 +2792,13 @@ int tps_interpolate_impl(const char* name, int32_t npts, const double* px, const
   return tps_check(name);
 }
 
-}  // namespace
-
-namespace fia {
-int tps_chol_threads() { return g_chol_threads.load(std::memory_order_relaxed); }
-void set_tps_chol_threads(int threads) { g_chol_threads.store(threads, std::memory_order_relaxed); }
-}  // namespace fia
-
-extern "C" {
-
-int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys,
-                   const float* vx, const float* vy, const int64_t* dom_off,
-                   const int32_t* slot_pt, const int64_t* mat_off, const double* smoothing,
-                   double threshold, double* mats, double* refl, double* vwork, double* slot_xy,
-                   double* smooth_slots, int64_t nslots, double* weights, int64_t wstride,
-                   double* stats, int32_t* status, const int32_t* order,
-                   const piv_tps_lattice* lattice, int32_t generated, void* stream) {
-  if (nd <= 0) return PIV_OK;
-  if (n_max > kMaxN)
-    return tps_fail(PIV_EUNSUPPORTED, "more than 2048 centers in a TPS subdomain");
-  if (!xs || !ys || !vx || !vy || !dom_off || !slot_pt || !mat_off || !smoothing || !mats ||
-      !refl || !vwork || !slot_xy || !smooth_slots || !weights || !stats || !status)
-    return tps_fail(PIV_EINVAL, "NULL argument");
-  hipStream_t s = static_cast<hipStream_t>(stream);
+// matrix build and factorisation of nd subdomains (prep + Cholesky)
+void launch_factor(int32_t nd, int32_t n_max, const double* xs, const double* ys,
+                   const int64_t* dom_off, const int32_t* slot_pt, const int64_t* mat_off,
+                   const double* smoothing, double* mats, double* refl, double* vwork,
+                   double* slot_xy, int64_t nslots, int32_t* status, const int32_t* order,
+                   const PhiLattice& lat, bool gen, hipStream_t s) {
   // up to one subdomain per CU: 16 wavefronts per matrix build, else 4
-  const PhiLattice lat = make_lattice(lattice);
-  const bool gen = generated != 0 && lat.table != nullptr;
   const size_t prep_lds = size_t(8) * n_max * sizeof(double) + size_t(lat.lds_words) * sizeof(double);
   const bool wide = nd <= cu_count();
 #define PIV_PREP(NT, GEN)                                                                    \
@@ -2183,12 +2826,164 @@ int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys
   else
     hipLaunchKernelGGL(k_tps_chol<256>, dim3(nd), dim3(256), chol_lds, s, dom_off, mat_off, mats,
                        vwork, status, slot_xy, nslots, smoothing, lat, gen, order);
+}
+
+void launch_solve(int32_t nd, int32_t n_max, const float* vx, const float* vy,
+                  const int64_t* dom_off, const int32_t* slot_pt, const int64_t* mat_off,
+                  const double* mats, const double* refl, const double* vwork,
+                  const double* smoothing, double threshold, double* smooth_slots,
+                  int64_t nslots, double* weights, int64_t wstride, double* stats,
+                  const int32_t* status, const int32_t* order, hipStream_t s) {
   hipLaunchKernelGGL(k_tps_solve, dim3(nd), dim3(kThreads),
                      size_t(3) * ((n_max + 1) & ~1) * sizeof(double), s, vx, vy, dom_off, slot_pt,
                      mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,
-                     weights, wstride, stats, status,
-                     (n_max + 1) & ~1, order);
+                     weights, wstride, stats, status, (n_max + 1) & ~1, order);
+}
+
+// workspace of piv_tps_smooth_shared (offsets in piv_tps_shared_workspace)
+struct SharedWork {
+  int32_t *dmode, *dpat, *dbox, *pat_n, *pat_box, *pslot, *pstatus, *order_direct, *order_shared;
+  int64_t *pdom_off, *pmat_off;
+  double *pat_smooth, *pxy, *pmats, *prefl, *pvwork, *pslot_xy, *gmat;
+  int64_t pxy_stride;
+  size_t bytes;
+};
+
+SharedWork shared_work(void* base, int nfields, int dom_cap, int npat_cap, int n_max, int cap_r) {
+  SharedWork w;
+  char* b = static_cast<char*>(base);
+  size_t o = 0;
+  auto take = [&](size_t n) {
+    const size_t at = o;
+    o += (n + 255) & ~size_t(255);
+    return b + at;
+  };
+  const size_t nd = size_t(nfields) * dom_cap, np = size_t(nfields) * npat_cap;
+  const size_t nm = size_t((n_max + 1) & ~1);
+  const size_t pts = np * size_t(n_max);  // pattern centers, at most n_max each
+  w.dmode = reinterpret_cast<int32_t*>(take(4 * nd));
+  w.dpat = reinterpret_cast<int32_t*>(take(4 * nd));
+  w.dbox = reinterpret_cast<int32_t*>(take(16 * nd));
+  w.order_direct = reinterpret_cast<int32_t*>(take(4 * nd));
+  w.order_shared = reinterpret_cast<int32_t*>(take(4 * nd));
+  w.pat_n = reinterpret_cast<int32_t*>(take(4 * np));
+  w.pat_box = reinterpret_cast<int32_t*>(take(16 * np));
+  w.pstatus = reinterpret_cast<int32_t*>(take(4 * np));
+  w.pslot = reinterpret_cast<int32_t*>(take(4 * pts));
+  w.pdom_off = reinterpret_cast<int64_t*>(take(8 * (np + 1)));
+  w.pmat_off = reinterpret_cast<int64_t*>(take(8 * (np + 1)));
+  w.pat_smooth = reinterpret_cast<double*>(take(8 * np));
+  w.pxy_stride = int64_t(pts);
+  w.pxy = reinterpret_cast<double*>(take(8 * 2 * pts));
+  w.pslot_xy = reinterpret_cast<double*>(take(8 * 2 * pts));
+  w.prefl = reinterpret_cast<double*>(take(8 * 16 * np));
+  w.pvwork = reinterpret_cast<double*>(take(8 * 6 * pts));
+  w.pmats = reinterpret_cast<double*>(take(8 * np * size_t(tps_mat_size(n_max))));
+  w.gmat = reinterpret_cast<double*>(take(8 * nd * nm * size_t(cap_r > 0 ? cap_r : 1)));
+  w.bytes = o;
+  return w;
+}
+
+__global__ void k_tps_zero_i32(int n, int32_t* a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+}  // namespace
+
+namespace fia {
+int tps_chol_threads() { return g_chol_threads.load(std::memory_order_relaxed); }
+void set_tps_chol_threads(int threads) { g_chol_threads.store(threads, std::memory_order_relaxed); }
+}  // namespace fia
+
+extern "C" {
+
+int piv_tps_smooth(int32_t nd, int32_t n_max, const double* xs, const double* ys,
+                   const float* vx, const float* vy, const int64_t* dom_off,
+                   const int32_t* slot_pt, const int64_t* mat_off, const double* smoothing,
+                   double threshold, double* mats, double* refl, double* vwork, double* slot_xy,
+                   double* smooth_slots, int64_t nslots, double* weights, int64_t wstride,
+                   double* stats, int32_t* status, const int32_t* order,
+                   const piv_tps_lattice* lattice, int32_t generated, void* stream) {
+  if (nd <= 0) return PIV_OK;
+  if (n_max > kMaxN)
+    return tps_fail(PIV_EUNSUPPORTED, "more than 2048 centers in a TPS subdomain");
+  if (!xs || !ys || !vx || !vy || !dom_off || !slot_pt || !mat_off || !smoothing || !mats ||
+      !refl || !vwork || !slot_xy || !smooth_slots || !weights || !stats || !status)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const PhiLattice lat = make_lattice(lattice);
+  const bool gen = generated != 0 && lat.table != nullptr;
+  launch_factor(nd, n_max, xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork,
+                slot_xy, nslots, status, order, lat, gen, s);
+  launch_solve(nd, n_max, vx, vy, dom_off, slot_pt, mat_off, mats, refl, vwork, smoothing,
+               threshold, smooth_slots, nslots, weights, wstride, stats, status, order, s);
   return tps_check("piv_tps_smooth");
+}
+
+int64_t piv_tps_shared_workspace(int32_t nfields, int32_t dom_cap, int32_t npat_cap,
+                                 int32_t n_max, int32_t cap_r) {
+  if (nfields <= 0 || dom_cap <= 0 || npat_cap <= 0 || n_max <= 0 || cap_r < 0) return -1;
+  return int64_t(shared_work(nullptr, nfields, dom_cap, npat_cap, n_max, cap_r).bytes);
+}
+
+int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field* fields,
+                          const double* lim, const int32_t* fstat, const int32_t* dom_field,
+                          const int32_t* counts, const double* gx, int32_t ngx, int32_t irr_x,
+                          const double* gy, int32_t ngy, int32_t irr_y, int32_t npat_cap,
+                          int32_t cap_r, int32_t n_max, const double* xs, const double* ys,
+                          const int32_t* idx, const float* vx, const float* vy,
+                          const int64_t* dom_off, const int32_t* slot_pt, const int64_t* mat_off,
+                          const double* smoothing, double threshold, double* mats, double* refl,
+                          double* vwork, double* slot_xy, double* smooth_slots, int64_t nslots,
+                          double* weights, int64_t wstride, double* stats, int32_t* status,
+                          const int32_t* order, const piv_tps_lattice* lattice,
+                          int32_t generated, void* work, void* stream) {
+  if (nfields <= 0 || dom_cap <= 0) return PIV_OK;
+  if (n_max < 4 || n_max > kMaxN) return tps_fail(PIV_EINVAL, "n_max must be in [4, 2048]");
+  if (npat_cap <= 0 || npat_cap > kPatMaxPerField)
+    return tps_fail(PIV_EINVAL, "npat_cap must be in [1, 64]");
+  if (cap_r < 0 || cap_r > kCapMaxR) return tps_fail(PIV_EINVAL, "cap_r must be in [0, 12]");
+  if ((size_t(kSharedNV + 2) * ((n_max + 1) & ~1) + kSolveScratch) * sizeof(double) > 120 * 1024)
+    return tps_fail(PIV_EUNSUPPORTED, "n_max too large for the shared solve's LDS");
+  if (int64_t(nfields) * dom_cap > INT32_MAX / 4 || int64_t(nfields) * npat_cap > 1 << 20)
+    return tps_fail(PIV_EUNSUPPORTED, "TPS batch too large");
+  if (!fields || !lim || !fstat || !dom_field || !counts || !gx || !gy || !xs || !ys || !idx ||
+      !vx || !vy || !dom_off || !slot_pt || !mat_off || !smoothing || !mats || !refl ||
+      !vwork || !slot_xy || !smooth_slots || !weights || !stats || !status || !order || !work)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nd = nfields * dom_cap;
+  const int npat = nfields * npat_cap;
+  const int nm = (n_max + 1) & ~1;
+  const SharedWork w = shared_work(work, nfields, dom_cap, npat_cap, n_max, cap_r);
+  const PhiLattice lat = make_lattice(lattice);
+  const bool gen = generated != 0 && lat.table != nullptr;
+  hipLaunchKernelGGL(k_tps_patterns, dim3(nfields), dim3(256), 0, s, int(dom_cap), fields, lim,
+                     fstat, dom_field, counts, smoothing, gx, int(ngx), int(irr_x), gy, int(ngy),
+                     int(irr_y), int(cap_r), int(n_max), int(npat_cap), w.dmode, w.dpat, w.dbox,
+                     w.pat_n, w.pat_box, w.pat_smooth);
+  hipLaunchKernelGGL(k_tps_pattern_layout, dim3(1), dim3(1024), 0, s, npat, w.pat_n, w.pat_box,
+                     gx, gy, nd, order, w.dmode, w.pdom_off, w.pmat_off, w.pxy, w.pslot,
+                     w.pxy_stride, w.order_direct, w.order_shared);
+  hipLaunchKernelGGL(k_tps_zero_i32, dim3((npat + 255) / 256), dim3(256), 0, s, npat, w.pstatus);
+  // the patterns' factorisations, then the subdomains factorised alone
+  launch_factor(npat, n_max, w.pxy, w.pxy + w.pxy_stride, w.pdom_off, w.pslot, w.pmat_off,
+                w.pat_smooth, w.pmats, w.prefl, w.pvwork, w.pslot_xy, w.pxy_stride, w.pstatus,
+                nullptr, lat, gen, s);
+  launch_factor(nd, n_max, xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork,
+                slot_xy, nslots, status, w.order_direct, lat, gen, s);
+  launch_solve(nd, n_max, vx, vy, dom_off, slot_pt, mat_off, mats, refl, vwork, smoothing,
+               threshold, smooth_slots, nslots, weights, wstride, stats, status, w.order_direct,
+               s);
+  hipLaunchKernelGGL(k_tps_solve_shared, dim3(nd), dim3(kThreads),
+                     (size_t(kSharedNV + 2) * nm + kSolveScratch) * sizeof(double), s,
+                     w.order_shared, w.dpat, w.dbox,
+                     w.pat_n, w.pdom_off, w.pmat_off, w.pmats, w.prefl, w.pvwork, w.pstatus, gx,
+                     gy, int(ngx), xs, ys, idx, vx, vy, dom_off, slot_pt, smoothing, threshold,
+                     smooth_slots, nslots, weights, wstride, stats, status, slot_xy, w.gmat,
+                     int(cap_r), nm);
+  return tps_check("piv_tps_smooth_shared");
 }
 
 int piv_tps_select(int32_t nfields, int32_t n, const float* dx, const float* dy,

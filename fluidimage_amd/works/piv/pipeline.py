@@ -55,6 +55,11 @@ class MultipassPipeline:
         self.sizing = "device"
         self._caps = {}
         self._tables = {}
+        self._grids = {}
+        # device-sized TPS on grid-node centers: subdomains with one node
+        # pattern share a factorisation (a capacitance correction for their
+        # missing nodes); False: every subdomain factorised alone
+        self.share_factors = True
 
     def _event_pair(self, timers, key):
         import torch
@@ -132,6 +137,25 @@ class MultipassPipeline:
             self._tables[cap] = make_lattice_table(cap, torch.device("cuda"))
         return cap, self._tables[cap]
 
+    def _pass_grid(self, work, dev):
+        from ...calcul.tps_planned import PassGrid
+
+        key = (work.index_pass, work.imshape0)
+        g = self._grids.get(key)
+        if g is None:
+            g = self._grids[key] = PassGrid(work.ixvecs, work.iyvecs, dev)
+        return g
+
+    def _share(self, work, cap, deformed):
+        """Whether the TPS call from ``work``'s vectors shares factorisations:
+        only when the vectors sit on the pass grid (the first pass, deformed
+        passes) and the batch has many subdomains, many per field (a pattern
+        then serves many of them)."""
+        from ...calcul import tps_planned as tp
+
+        return (bool(self.share_factors) and (deformed or work.index_pass == 0)
+                and cap.dom_cap >= tp.SHARED_MIN_SUBDOMAINS and cap.nd >= tp.SHARED_MIN_BATCH)
+
     def _interp_planned(self, work_next, res, last, stream=None, timers=None):
         """The TPS interpolation of every pair as one device-sized batch (no
         host read); the fields' checks are read at the end of the run."""
@@ -145,9 +169,13 @@ class MultipassPipeline:
         ev = self._event_pair(timers, f"tps_pass{work_next.index_pass}" + ("_final" if last else ""))
         if ev:
             ev[0].record()
+        grid = None
+        if self._share(res.work, cap, deformed):
+            # the vectors sit on the pass grid: subdomains share factorisations
+            grid = self._pass_grid(res.work, res.dev)
         t = tps_planned(cap, res.out["dx"], res.out["dy"], res.win["xs"], res.win["ys"], gx, gy,
                         mp.subdom_size, mp.smoothing_coef, mp.threshold_tps, 20, stream=stream,
-                        lattice_table=table)
+                        lattice_table=table, grid=grid)
         if ev:
             ev[1].record()
         self._pending.append(("planned", t))

@@ -362,6 +362,38 @@ int piv_tps_point_slots(int32_t nfields, const piv_tps_field* fields,
                         double* out_coef, double* out_norm, int32_t* status,
                         void* stream);
 
+/* Bytes of the workspace of piv_tps_smooth_shared. */
+int64_t piv_tps_shared_workspace(int32_t nfields, int32_t dom_cap, int32_t npat_cap,
+                                 int32_t n_max, int32_t cap_r);
+
+/* piv_tps_smooth for a device-sized batch (see piv_tps_grid below) whose
+ * centers are nodes of the pass grid gx[ngx] x gy[ngy] (the vectors'
+ * positions; irr_x / irr_y: the grid's last column / row is appended at a
+ * smaller step). No counterpart in the reference, which factorises every
+ * subdomain (thin_plate_spline.py:214): subdomains whose boxes hold the
+ * same grid-node pattern share one factorisation per field (the matrix of a
+ * box is the same up to a translation, with the polynomial part relative
+ * to the box's first node), and a subdomain missing at most cap_r (<= 16)
+ * nodes of its box (its invalid vectors) is solved with it and a
+ * capacitance correction; the others are factorised alone as in
+ * piv_tps_smooth. Same outputs as piv_tps_smooth (weights in image
+ * coordinates); status[d] = 2 when a center is off the grid. At most
+ * npat_cap (<= 64) patterns per field. `order`: the order of the nfields
+ * dom_cap subdomain slots (piv_tps_offsets); idx: the vector index of
+ * every center (piv_tps_select); work: piv_tps_shared_workspace bytes. */
+int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field* fields,
+                          const double* lim, const int32_t* fstat, const int32_t* dom_field,
+                          const int32_t* counts, const double* gx, int32_t ngx, int32_t irr_x,
+                          const double* gy, int32_t ngy, int32_t irr_y, int32_t npat_cap,
+                          int32_t cap_r, int32_t n_max, const double* xs, const double* ys,
+                          const int32_t* idx, const float* vx, const float* vy,
+                          const int64_t* dom_off, const int32_t* slot_pt, const int64_t* mat_off,
+                          const double* smoothing, double threshold, double* mats, double* refl,
+                          double* vwork, double* slot_xy, double* smooth_slots, int64_t nslots,
+                          double* weights, int64_t wstride, double* stats, int32_t* status,
+                          const int32_t* order, const piv_tps_lattice* lattice,
+                          int32_t generated, void* work, void* stream);
+
 /* ---- device-sized TPS batches (no counterpart in the reference, whose
  *      ThinPlateSplineSubdom.__init__ sizes each field on the host,
  *      thin_plate_spline_subdom.py:53-126): the subdomain grids, member

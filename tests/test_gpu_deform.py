@@ -87,12 +87,19 @@ def _setup(shape, number=3, window=64):
     return im0, im1, p, op
 
 
-def test_multipass_with_deformation_vs_oracle():
+@pytest.mark.parametrize("share", [False, True])
+def test_multipass_with_deformation_vs_oracle(share, monkeypatch):
+    """share: the TPS calls on the deformed passes' grids share their
+    factorisations between subdomains of one node pattern (forced at this
+    size)."""
+    from fluidimage_amd.calcul import tps_planned
     from fluidimage_amd.works.piv.multipass import WorkPIV
     from oracle.piv import OracleWorkPIV
     from tests.test_gpu_multipass import check_passes_against_oracle
 
-    print("case: multipass with deformation", flush=True)
+    monkeypatch.setattr(tps_planned, "SHARED_MIN_SUBDOMAINS", 0 if share else 1 << 30)
+    monkeypatch.setattr(tps_planned, "SHARED_MIN_BATCH", 0)
+    print("case: multipass with deformation, shared factorisations", share, flush=True)
     im0, im1, p, op = _setup((512, 512))
     gpu = WorkPIV(p).calcul_from_arrays(im0, im1)
     ow = OracleWorkPIV(op, precision="f64")
@@ -100,7 +107,8 @@ def test_multipass_with_deformation_vs_oracle():
     for ip, (g, e) in enumerate(zip(gpu.passes, ref)):
         if ip > 0:
             assert np.allclose(g.xs, e["xs"]) and np.allclose(g.ys, e["ys"])
-    check_passes_against_oracle(gpu.passes, ref, ow.works_piv, deform=True, label="deform")
+    check_passes_against_oracle(gpu.passes, ref, ow.works_piv, deform=True,
+                                label=f"deform, shared {share}")
 
 
 def test_deformation_lowers_error_against_truth():

@@ -94,19 +94,24 @@ def _same(a, b, label):
         assert np.array_equal(ra.passes[-1].deltays_smooth, rb.passes[-1].deltays_smooth), label
 
 
-@pytest.mark.parametrize("deform", [True, False])
-def test_graph_multipass_replays_equal_eager(deform):
+@pytest.mark.parametrize("deform, share", [(True, False), (False, False), (True, True)])
+def test_graph_multipass_replays_equal_eager(deform, share, monkeypatch):
     """The whole 3-pass multipass with TPS between the passes and at the
     end (and image deformation) in one graph: four replays on alternating
-    image contents, with eager work between them, give the eager bits."""
+    image contents, with eager work between them, give the eager bits
+    (share: TPS factorisations shared between subdomains, forced at this
+    size)."""
     import torch
 
+    from fluidimage_amd.calcul import tps_planned
     from fluidimage_amd.device import ImageSet, to_device_image
     from fluidimage_amd.synthetic import config_pair
     from fluidimage_amd.works.piv.multipass import WorkPIV
     from fluidimage_amd.works.piv.pipeline import GraphedPasses, MultipassPipeline
 
-    print("case: graph capture multipass, deformation", deform, flush=True)
+    monkeypatch.setattr(tps_planned, "SHARED_MIN_SUBDOMAINS", 0 if share else 1 << 30)
+    monkeypatch.setattr(tps_planned, "SHARED_MIN_BATCH", 0)
+    print("case: graph capture multipass, deformation", deform, "shared", share, flush=True)
     dev = torch.device("cuda")
     shape = (384, 448)
     pairs = [config_pair(3, shape=shape, pair_idx=k) for k in range(3)]
@@ -161,7 +166,10 @@ def test_graph_reruns_pairs_beyond_capacity(monkeypatch):
     ims = []
     for a, b in pairs:
         ims += [to_device_image(a, dev), to_device_image(b, dev)]
+    # the pairs are run again with host sizing: the eager bits to expect are
+    # those of a host-sized run (every subdomain factorised alone)
     pipe = MultipassPipeline(work)
+    pipe.sizing = "host"
     expect = _host(pipe.run(ImageSet(ims), 2), pipe, 2)
     monkeypatch.setattr(tps_planned, "MARGIN_SLOTS", 0.3)
     imageset = ImageSet(ims)
