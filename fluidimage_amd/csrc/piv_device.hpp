@@ -142,6 +142,109 @@ struct Search {
   }
 };
 
+// The same search on the map itself: an element is skipped when it is NaN,
+// when bit r of scol[c] is set (where_large_displacement, scol == nullptr
+// for none) or when it lies in one of the squares "NaN-ed" so far, which are
+// kept as bounds (wavefront-uniform values) rather than written. Saves the
+// search copy (n0 n1 floats of LDS per window).
+constexpr int kMaxSquares = kMaxPeaks + 1;
+
+template <int T>
+struct MaskedSearch {
+  const float* m;
+  const uint64_t* scol;
+  int n0, n1;
+  int b0s, b0e, b1s, b1e;
+  int nsq = 0;
+  int sa0[kMaxSquares], sa1[kMaxSquares], sc0[kMaxSquares], sc1[kMaxSquares];
+
+  __device__ __forceinline__ bool in_squares(int r, int c) const {
+    bool in = false;
+#pragma unroll
+    for (int k = 0; k < kMaxSquares; ++k)
+      in |= k < nsq && r >= sa0[k] && r < sa1[k] && c >= sc0[k] && c < sc1[k];
+    return in;
+  }
+
+  __device__ __forceinline__ bool skipped(int r, int c, float v) const {
+    if (v != v) return true;
+    if (scol != nullptr && ((scol[c] >> r) & 1)) return true;
+    return nsq > 0 && in_squares(r, c);
+  }
+
+  __device__ __forceinline__ int find(int lane, int& oi0, int& oi1) const {
+    float best = 0.f;
+    int blin = -1;
+    for (int c = b1s + lane; c < b1e; c += T) {
+      const uint64_t bits = scol != nullptr ? scol[c] : uint64_t(0);
+      if (nsq == 0) {
+#pragma unroll 4
+        for (int r = b0s; r < b0e; ++r) {
+          const float v = m[r * n1 + c];
+          if (v != v || ((bits >> r) & 1)) continue;
+          if (blin < 0 || v >= best) {
+            best = v;
+            blin = r * n1 + c;
+          }
+        }
+      } else {
+        for (int r = b0s; r < b0e; ++r) {
+          const float v = m[r * n1 + c];
+          if (v != v || ((bits >> r) & 1) || in_squares(r, c)) continue;
+          if (blin < 0 || v >= best) {
+            best = v;
+            blin = r * n1 + c;
+          }
+        }
+      }
+    }
+    for (int off = T / 2; off > 0; off >>= 1) {
+      const float ob = __shfl_xor(best, off, T);
+      const int ol = __shfl_xor(blin, off, T);
+      if (ol >= 0 && (blin < 0 || ob > best || (ob == best && ol > blin))) {
+        best = ob;
+        blin = ol;
+      }
+    }
+    if (blin < 0) {
+      oi0 = oi1 = 0;
+      return PIV_WIN_NO_PEAK;
+    }
+    const int i0 = blin / n1, i1 = blin % n1;
+    oi0 = i0;
+    oi1 = i1;
+    if (i0 == 0 || i0 == n0 - 1 || i1 == 0 || i1 == n1 - 1) return PIV_WIN_PEAK_BOUNDARY;
+    bool nan = false;
+#pragma unroll
+    for (int d0 = -1; d0 <= 1; ++d0)
+#pragma unroll
+      for (int d1 = -1; d1 <= 1; ++d1)
+        nan |= skipped(i0 + d0, i1 + d1, m[(i0 + d0) * n1 + i1 + d1]);
+    return nan ? PIV_WIN_PEAK_NAN : PIV_WIN_OK;
+  }
+
+  __device__ __forceinline__ void nan_square(int lane, int i0, int i1, int r, int* q) {
+    int a0, a1, c0, c1;
+    py_slice(i0 - r, i0 + r + 1, n0, a0, a1);
+    py_slice(i1 - r, i1 + r + 1, n1, c0, c1);
+#pragma unroll
+    for (int k = 0; k < kMaxSquares; ++k)
+      if (k == nsq) {
+        sa0[k] = a0;
+        sa1[k] = a1;
+        sc0[k] = c0;
+        sc1[k] = c1;
+      }
+    if (nsq < kMaxSquares) ++nsq;
+    if (q != nullptr) {
+      q[0] = a0;
+      q[1] = a1;
+      q[2] = c0;
+      q[3] = c1;
+    }
+  }
+};
+
 // correl_max = correl[iy, ix] / norm with numpy-2 float32 promotion
 // (correl.py:153-157); 0 when norm == 0.
 __device__ __forceinline__ float peak_value(float v, double norm) {
@@ -227,6 +330,13 @@ __device__ __forceinline__ int subpix(const float* m, int n0, int n1, int method
 // with the zero displacement at (iy0, ix0): (n0 // 2 - 1, n1 // 2 - 1) for
 // the FFT maps (correl.py:189), (dmax, dmax) for the direct maps
 // (correl.py:409-410,444-445).
+template <int T, class S>
+__device__ __forceinline__ void finish_window_with(S& s, const float* m, int n0, int n1, int iy0,
+                                                   int ix0, const PlanArgs& pa, int lane,
+                                                   int win, double norm, int apply_subpix,
+                                                   int dx_in, int dy_in, const piv_results& res,
+                                                   int* sq);
+
 template <int T>
 __device__ __forceinline__ void finish_window(const float* m, float* srch, int n0, int n1,
                                               int iy0, int ix0, const PlanArgs& pa, int lane,
@@ -240,6 +350,37 @@ __device__ __forceinline__ void finish_window(const float* m, float* srch, int n
   s.b0e = pa.b0e;
   s.b1s = pa.b1s;
   s.b1e = pa.b1e;
+  finish_window_with<T>(s, m, n0, n1, iy0, ix0, pa, lane, win, norm, apply_subpix, dx_in, dy_in,
+                        res, sq);
+}
+
+// finish_window without a search copy: the mask as column bits (bit i0 of
+// scol[i1], nullptr for none)
+template <int T>
+__device__ __forceinline__ void finish_window_masked(const float* m, const uint64_t* scol, int n0,
+                                                     int n1, int iy0, int ix0, const PlanArgs& pa,
+                                                     int lane, int win, double norm,
+                                                     int apply_subpix, int dx_in, int dy_in,
+                                                     const piv_results& res, int* sq) {
+  MaskedSearch<T> s;
+  s.m = m;
+  s.scol = scol;
+  s.n0 = n0;
+  s.n1 = n1;
+  s.b0s = pa.b0s;
+  s.b0e = pa.b0e;
+  s.b1s = pa.b1s;
+  s.b1e = pa.b1e;
+  finish_window_with<T>(s, m, n0, n1, iy0, ix0, pa, lane, win, norm, apply_subpix, dx_in, dy_in,
+                        res, sq);
+}
+
+template <int T, class S>
+__device__ __forceinline__ void finish_window_with(S& s, const float* m, int n0, int n1, int iy0,
+                                                   int ix0, const PlanArgs& pa, int lane,
+                                                   int win, double norm, int apply_subpix,
+                                                   int dx_in, int dy_in, const piv_results& res,
+                                                   int* sq) {
   sq[0] = sq[1] = sq[2] = sq[3] = 0;
   int i0, i1;
   int code = s.find(lane, i0, i1);

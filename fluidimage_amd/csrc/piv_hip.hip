@@ -88,14 +88,39 @@ struct Shape {
   static constexpr int NN = N0 * N1;
 };
 
-// padded LDS layout of the complex N0 x N1 buffer: element (r, c) lives at
-// r * (N1 + 1) + c. The column pass (lanes over c) reads consecutive
-// elements; in the row pass (lanes over r) lane r starts 2 (N1 + 1) dwords
-// after lane r - 1, an even number coprime to 32 halves, so the 64-bit
-// accesses of 32 lanes cover distinct bank pairs. Every access is one base
-// register plus a compile-time offset.
+// F0(k), F1(k) of the two real windows from Z(k) = a and Z(-k) = (zmx, zmy)
+// of their complex FFT: F0 = (Z(k) + conj Z(-k)) / 2, F1 = (Z(k) - conj
+// Z(-k)) / 2i; zero at the DC bin (dc)
+struct SplitSpectra {
+  float2 f0, f1;
+};
+
+__device__ __forceinline__ SplitSpectra split_spectra(float2 a, float zmx, float zmy, bool dc) {
+  const float2 b = make_float2(zmx, -zmy);
+  const float2 d = make_float2(a.x - b.x, a.y - b.y);
+  SplitSpectra s;
+  s.f0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+  s.f1 = make_float2(0.5f * d.y, -0.5f * d.x);
+  if (dc) {
+    s.f0 = make_float2(0.f, 0.f);
+    s.f1 = make_float2(0.f, 0.f);
+  }
+  return s;
+}
+
+// LDS layout of one component (real or imaginary parts) of the complex
+// N0 x N1 buffer: element (r, c) lives at r * LD + c with an odd row stride
+// LD. The transposes between the column and the row passes move the real
+// parts, then the imaginary parts, so that no more than one full complex
+// array per lane is live at a time and the buffer is N0 LD floats. The
+// column pass (lanes over c) accesses consecutive elements; in the row pass
+// (lanes over r) the odd stride puts the lanes' 32-bit accesses in distinct
+// banks. Every access is one base register plus a compile-time offset. The
+// buffer then holds the map (N0 N1 floats); the peak search reads the mask
+// bits and its NaN squares directly (no search copy). A 64 x 64 window takes
+// 17 KB of LDS, so that VGPRs, not LDS, bound the occupancy.
 template <int N1>
-constexpr int kLd = N1 + 1;
+constexpr int kLd = N1 % 2 == 0 ? N1 + 1 : N1;
 
 template <int N0, int N1, int DT, int WPB>
 __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correlate(
@@ -104,9 +129,10 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   constexpr int T = Shape<N0, N1>::T;
   const int apply_subpix = options & PIV_OPT_SUBPIX;
   constexpr int NN = N0 * N1;
-  constexpr int LD = kLd<N1>, NB = N0 * LD;  // padded buffer of one window
-  static_assert(NB >= NN, "the map and its search copy (2 NN floats) fit in the buffer");
-  __shared__ float2 smem[WPB * NB];
+  constexpr int LD = kLd<N1>;
+  constexpr int NB = N0 * LD;  // floats of one window's buffer
+  static_assert(NB >= NN, "the map (NN floats) fits in the buffer");
+  __shared__ float smem[WPB * NB];
   __shared__ uint64_t scol[N1];  // bit i0 of scol[i1]: where_large_displacement[i0, i1]
 
   const int lane = threadIdx.x & (T - 1);
@@ -122,7 +148,7 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
     }
   }
 
-  float2* buf = smem + wl * NB;
+  float* buf = smem + wl * NB;
   piv_window w;
   const bool live = win < nwin;
   if (live) {
@@ -197,93 +223,100 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   // (in place: the forward transforms leave frequency k at fft_pos(n, k),
   // the inverse ones take it there; the loops below index accordingly)
   fft_dif<N0, -1, 0, 1, N0>(z);
-  if (lane < N1) {
-    sfor<0, N0>([&](auto k_) {
-      constexpr int k = decltype(k_)::value;
-      constexpr int pk = fft_pos(N0, k);
-      buf[k * LD + lane] = z[pk];
-    });
-  }
-  __syncthreads();
   float2 y[N1];
   const int r = lane;  // row owned in the row pass
-  if (r < N0) {
-#pragma unroll
-    for (int c = 0; c < N1; ++c) y[c] = buf[r * LD + c];
-  } else {
+  if (r >= N0) {
 #pragma unroll
     for (int c = 0; c < N1; ++c) y[c] = make_float2(0.f, 0.f);
   }
+  // transpose, real parts then imaginary parts
+  sfor<0, 2>([&](auto part_) {
+    constexpr int part = decltype(part_)::value;
+    if (lane < N1) {
+      sfor<0, N0>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        constexpr int pk = fft_pos(N0, k);
+        buf[k * LD + lane] = part == 0 ? z[pk].x : z[pk].y;
+      });
+    }
+    __syncthreads();
+    if (r < N0) {
+      const float* row = buf + r * LD;
+#pragma unroll
+      for (int c = 0; c < N1; ++c) {
+        if constexpr (part == 0) y[c].x = row[c];
+        else y[c].y = row[c];
+      }
+    }
+    __syncthreads();
+  });
   fft_dif<N1, -1, 0, 1, N1>(y);
-  __syncthreads();
-  if (r < N0) {
-    sfor<0, N1>([&](auto c_) {
-      constexpr int c = decltype(c_)::value;
-      constexpr int pc = fft_pos(N1, c);
-      buf[r * LD + c] = y[pc];
-    });
-  }
-  __syncthreads();
   // ---- 4. F0, F1 from Z(k), Z(-k); energies; cross spectrum ---------------
+  // Z(-r, -c) is element -c of row N0 - r, held by lane N0 - r of the group:
+  // columns c and -c are done together (both read before either is written)
   float e0f = 0.f, e1f = 0.f;
-  if (r < N0) {
-    const int pr = r == 0 ? 0 : N0 - r;
-    sfor<0, N1>([&](auto c_) {
-      constexpr int c = decltype(c_)::value;
-      constexpr int cm = c == 0 ? 0 : N1 - c;
-      constexpr int pc = fft_pos(N1, c);
-      const float2 zm = buf[pr * LD + cm];
-      const float2 a = y[pc];
-      const float2 b = make_float2(zm.x, -zm.y);
-      float2 f0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
-      const float2 d = make_float2(a.x - b.x, a.y - b.y);
-      float2 f1 = make_float2(0.5f * d.y, -0.5f * d.x);
-      if (r == 0 && c == 0) {
-        f0 = make_float2(0.f, 0.f);
-        f1 = make_float2(0.f, 0.f);
-      }
-      if (c <= N1 / 2) {
-        const float wgt = (c == 0 || c == N1 / 2) ? 1.f : 2.f;
-        e0f += wgt * (f0.x * f0.x + f0.y * f0.y);
-        e1f += wgt * (f1.x * f1.x + f1.y * f1.y);
-      }
-      // conj(f0) * f1
-      y[pc] = make_float2(f0.x * f1.x + f0.y * f1.y, f0.x * f1.y - f0.y * f1.x);
-    });
-  }
+  const int pr = (r == 0 || r >= N0) ? r : N0 - r;
+  sfor<0, N1 / 2 + 1>([&](auto c_) {
+    constexpr int c = decltype(c_)::value;
+    constexpr int cm = c == 0 ? 0 : N1 - c;
+    constexpr int pc = fft_pos(N1, c);
+    constexpr int pcm = fft_pos(N1, cm);
+    // partner of c: element -c of row -r; partner of -c: element c of row -r
+    const float ax = __shfl(y[pcm].x, pr, T), ay = __shfl(y[pcm].y, pr, T);
+    float bx = 0.f, by = 0.f;
+    if constexpr (cm != c) {
+      bx = __shfl(y[pc].x, pr, T);
+      by = __shfl(y[pc].y, pr, T);
+    }
+    const SplitSpectra f = split_spectra(y[pc], ax, ay, r == 0 && c == 0);
+    const float2 f0 = f.f0, f1 = f.f1;
+    {
+      const float wgt = (c == 0 || c == N1 / 2) ? 1.f : 2.f;
+      e0f += wgt * (f0.x * f0.x + f0.y * f0.y);
+      e1f += wgt * (f1.x * f1.x + f1.y * f1.y);
+    }
+    if constexpr (cm != c) {
+      const SplitSpectra g = split_spectra(y[pcm], bx, by, false);
+      const float2 g0 = g.f0, g1 = g.f1;
+      y[pcm] = make_float2(g0.x * g1.x + g0.y * g1.y, g0.x * g1.y - g0.y * g1.x);
+    }
+    // conj(f0) * f1
+    y[pc] = make_float2(f0.x * f1.x + f0.y * f1.y, f0.x * f1.y - f0.y * f1.x);
+  });
+  if (r >= N0) e0f = e1f = 0.f;
   const double e0 = group_sum(double(e0f), T);
   const double e1 = group_sum(double(e1f), T);
   const double norm = sqrt(e0 * e1);
   // ---- 5. inverse 2-D FFT (rows, then columns) ------------------------
   fft_dit<N1, +1, 0, 1, N1>(y);
-  __syncthreads();
-  if (r < N0) {
+  sfor<0, 2>([&](auto part_) {
+    constexpr int part = decltype(part_)::value;
+    if (r < N0) {
+      float* row = buf + r * LD;
 #pragma unroll
-    for (int c = 0; c < N1; ++c) buf[r * LD + c] = y[c];
-  }
-  __syncthreads();
-  if (lane < N1) {
-    sfor<0, N0>([&](auto k_) {
-      constexpr int k = decltype(k_)::value;
-      constexpr int pk = fft_pos(N0, k);
-      z[pk] = buf[k * LD + lane];
-    });
-  }
+      for (int c = 0; c < N1; ++c) row[c] = part == 0 ? y[c].x : y[c].y;
+    }
+    __syncthreads();
+    if (lane < N1) {
+      sfor<0, N0>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        constexpr int pk = fft_pos(N0, k);
+        if constexpr (part == 0) z[pk].x = buf[k * LD + lane];
+        else z[pk].y = buf[k * LD + lane];
+      });
+    }
+    __syncthreads();
+  });
   fft_dit<N0, +1, 0, 1, N0>(z);
-  __syncthreads();
   // ---- 6. map in the reference layout: arr[a0, a1] -> out[g(a0), g(a1)],
   //         g(a) = (n - 1 - a + n // 2) mod n  (inverse of _like_fftshift)
-  float* mapf = reinterpret_cast<float*>(buf);
-  float* srch = mapf + NN;  // search copy: masked elements NaN
+  float* mapf = buf;
   if (lane < N1) {
     const int g1 = (N1 - 1 - lane + N1 / 2) % N1;
-    const uint64_t bits = pa.mask != nullptr ? scol[g1] : uint64_t(0);
 #pragma unroll
     for (int a0 = 0; a0 < N0; ++a0) {
       const int g0 = (N0 - 1 - a0 + N0 / 2) % N0;
-      const float v = z[a0].x;
-      mapf[g0 * N1 + g1] = v;
-      srch[g0 * N1 + g1] = ((bits >> g0) & 1) ? __builtin_nanf("") : v;
+      mapf[g0 * N1 + g1] = z[a0].x;
     }
   }
   __syncthreads();
@@ -291,8 +324,9 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   int sq[4] = {0, 0, 0, 0};
   if (compute) {
     if (lane == 0 && res.norm) res.norm[win] = norm;
-    finish_window<T>(mapf, srch, N0, N1, N0 / 2 - 1, N1 / 2 - 1, pa, lane, win, norm,
-                     apply_subpix, w.dx_in, w.dy_in, res, sq);
+    finish_window_masked<T>(mapf, pa.mask != nullptr ? scol : nullptr, N0, N1, N0 / 2 - 1,
+                            N1 / 2 - 1, pa, lane, win, norm, apply_subpix, w.dx_in, w.dy_in,
+                            res, sq);
   } else if (live && lane == 0) {
     const float nan = __builtin_nanf("");
     res.dx[win] = nan;
