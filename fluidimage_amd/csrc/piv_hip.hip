@@ -88,9 +88,14 @@ struct Shape {
   static constexpr int NN = N0 * N1;
 };
 
-// F0(k), F1(k) of the two real windows from Z(k) = a and Z(-k) = (zmx, zmy)
-// of their complex FFT: F0 = (Z(k) + conj Z(-k)) / 2, F1 = (Z(k) - conj
-// Z(-k)) / 2i; zero at the DC bin (dc)
+// 2 F0(k), 2 F1(k) of the two real windows from Z(k) = a and Z(-k) = (zmx,
+// zmy) of their complex FFT: F0 = (Z(k) + conj Z(-k)) / 2, F1 = (Z(k) - conj
+// Z(-k)) / 2i; zero at the DC bin (dc). The factor 2 of each spectrum makes
+// the cross spectrum, the map and each energy 4 times theirs, and the norm
+// too: every product and sum of the scaled values is the scaled result of
+// the same operation exactly (powers of two), so the peaks, their ratios to
+// the norm and the sub-pixel fits are bit for bit those of the unscaled
+// values; the kept map and the norm are scaled back by 1/4 (exact).
 struct SplitSpectra {
   float2 f0, f1;
 };
@@ -99,13 +104,21 @@ __device__ __forceinline__ SplitSpectra split_spectra(float2 a, float zmx, float
   const float2 b = make_float2(zmx, -zmy);
   const float2 d = make_float2(a.x - b.x, a.y - b.y);
   SplitSpectra s;
-  s.f0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
-  s.f1 = make_float2(0.5f * d.y, -0.5f * d.x);
+  s.f0 = make_float2(a.x + b.x, a.y + b.y);
+  s.f1 = make_float2(d.y, -d.x);
   if (dc) {
     s.f0 = make_float2(0.f, 0.f);
     s.f1 = make_float2(0.f, 0.f);
   }
   return s;
+}
+
+// element k of a row-major map of N1 columns lies in the square of rows
+// [sq0, sq1), columns [sq2, sq3)
+template <int N1>
+__device__ __forceinline__ bool in_square(int k, const int* sq) {
+  const int i0 = k / N1, i1 = k % N1;
+  return i0 >= sq[0] && i0 < sq[1] && i1 >= sq[2] && i1 < sq[3];
 }
 
 // LDS layout of one component (real or imaginary parts) of the complex
@@ -132,7 +145,7 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
   constexpr int LD = kLd<N1>;
   constexpr int NB = N0 * LD;  // floats of one window's buffer
   static_assert(NB >= NN, "the map (NN floats) fits in the buffer");
-  __shared__ float smem[WPB * NB];
+  __shared__ __align__(16) float smem[WPB * NB];
   __shared__ uint64_t scol[N1];  // bit i0 of scol[i1]: where_large_displacement[i0, i1]
 
   const int lane = threadIdx.x & (T - 1);
@@ -307,23 +320,51 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
     }
     __syncthreads();
   });
-  fft_dit<N0, +1, 0, 1, N0>(z);
   // ---- 6. map in the reference layout: arr[a0, a1] -> out[g(a0), g(a1)],
   //         g(a) = (n - 1 - a + n // 2) mod n  (inverse of _like_fftshift)
   float* mapf = buf;
-  if (lane < N1) {
-    const int g1 = (N1 - 1 - lane + N1 / 2) % N1;
+  const int g1 = (N1 - 1 - lane + N1 / 2) % N1;
+  if constexpr (N0 % 2 == 0) {
+    // the column transform's output x is real: with M = N0 / 2,
+    // x[2m] + i x[2m + 1] = IFFT_M(E + i O)[m], E[k] = X[k] + X[k + M],
+    // O[k] = (X[k] - X[k + M]) exp(2 pi i k / N0) (half the work of the
+    // N0-point transform)
+    constexpr int M = N0 / 2;
+    float2 h[M];
+    sfor<0, M>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      constexpr int pa = fft_pos(N0, k), pb = fft_pos(N0, k + M), ph = fft_pos(M, k);
+      const float2 a = z[pa];
+      const float2 b = z[pb];
+      const float2 e = cadd(a, b);
+      const float2 o = twmul<k, N0, +1>(csub(a, b));
+      h[ph] = cadd(e, mul_si<+1>(o));
+    });
+    fft_dit<M, +1, 0, 1, M>(h);
+    if (lane < N1) {
+      sfor<0, M>([&](auto m_) {
+        constexpr int m = decltype(m_)::value;
+        constexpr int ga = (N0 - 1 - 2 * m + N0 / 2) % N0;
+        constexpr int gb = (N0 - 2 - 2 * m + N0 / 2) % N0;
+        mapf[ga * N1 + g1] = h[m].x;
+        mapf[gb * N1 + g1] = h[m].y;
+      });
+    }
+  } else {
+    fft_dit<N0, +1, 0, 1, N0>(z);
+    if (lane < N1) {
 #pragma unroll
-    for (int a0 = 0; a0 < N0; ++a0) {
-      const int g0 = (N0 - 1 - a0 + N0 / 2) % N0;
-      mapf[g0 * N1 + g1] = z[a0].x;
+      for (int a0 = 0; a0 < N0; ++a0) {
+        const int g0 = (N0 - 1 - a0 + N0 / 2) % N0;
+        mapf[g0 * N1 + g1] = z[a0].x;
+      }
     }
   }
   __syncthreads();
   // ---- 7. peak search, sub-pixel fit, records ---------------------------
   int sq[4] = {0, 0, 0, 0};
   if (compute) {
-    if (lane == 0 && res.norm) res.norm[win] = norm;
+    if (lane == 0 && res.norm) res.norm[win] = 0.25 * norm;
     finish_window_masked<T>(mapf, pa.mask != nullptr ? scol : nullptr, N0, N1, N0 / 2 - 1,
                             N1 / 2 - 1, pa, lane, win, norm, apply_subpix, w.dx_in, w.dy_in,
                             res, sq);
@@ -337,15 +378,32 @@ __global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correl
     if (res.nsec) res.nsec[win] = 0;
   }
   if (maps != nullptr && live) {
+    // the map, scaled back (x 1/4, exact); four elements per access when the
+    // output is 16-byte aligned (the buffer always is: N0 LD % 4 == 0)
     float* out = maps + size_t(win) * NN;
     const bool side = !pa.has_dmax && (options & PIV_OPT_MAP_SQUARE);
-    for (int k = lane; k < NN; k += T) {
-      float v = compute ? mapf[k] : __builtin_nanf("");
-      if (side) {
-        const int i0 = k / N1, i1 = k % N1;
-        if (i0 >= sq[0] && i0 < sq[1] && i1 >= sq[2] && i1 < sq[3]) v = __builtin_nanf("");
+    const float nan = __builtin_nanf("");
+    if (NN % 4 == 0 && NB % 4 == 0 && (reinterpret_cast<uintptr_t>(maps) & 15) == 0) {
+      for (int k = 4 * lane; k < NN; k += 4 * T) {
+        float4 v = compute ? *reinterpret_cast<const float4*>(mapf + k) : make_float4(nan, nan, nan, nan);
+        v.x *= 0.25f;
+        v.y *= 0.25f;
+        v.z *= 0.25f;
+        v.w *= 0.25f;
+        if (side) {
+          if (in_square<N1>(k, sq)) v.x = nan;
+          if (in_square<N1>(k + 1, sq)) v.y = nan;
+          if (in_square<N1>(k + 2, sq)) v.z = nan;
+          if (in_square<N1>(k + 3, sq)) v.w = nan;
+        }
+        *reinterpret_cast<float4*>(out + k) = v;
       }
-      out[k] = v;
+    } else {
+      for (int k = lane; k < NN; k += T) {
+        float v = compute ? 0.25f * mapf[k] : nan;
+        if (side && in_square<N1>(k, sq)) v = nan;
+        out[k] = v;
+      }
     }
   }
 }
