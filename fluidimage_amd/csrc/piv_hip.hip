@@ -135,8 +135,17 @@ __device__ __forceinline__ bool in_square(int k, const int* sq) {
 template <int N1>
 constexpr int kLd = N1 % 2 == 0 ? N1 + 1 : N1;
 
+// minimum wavefronts per SIMD the register allocation must allow: 2 for
+// the 48-point transforms (whose 264 VGPRs would allow 1; 18 spilled
+// registers cost less than the halved occupancy)
+template <int N0, int N1>
+constexpr int correlate_min_waves() {
+  return (N0 == 48 || N1 == 48) ? 2 : 1;
+}
+
 template <int N0, int N1, int DT, int WPB>
-__global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB)) void k_correlate(
+__global__ __launch_bounds__((lanes_for(N0 > N1 ? N0 : N1) * WPB))
+__attribute__((amdgpu_waves_per_eu(correlate_min_waves<N0, N1>()))) void k_correlate(
     const piv_image* __restrict__ images, const piv_window* __restrict__ windows, int nwin,
     int options, float* __restrict__ maps, piv_results res, PlanArgs pa) {
   constexpr int T = Shape<N0, N1>::T;
