@@ -175,19 +175,40 @@ struct MaskedSearch {
   __device__ __forceinline__ int find(int lane, int& oi0, int& oi1) const {
     float best = 0.f;
     int blin = -1;
-    for (int c = b1s + lane; c < b1e; c += T) {
-      const uint64_t bits = scol != nullptr ? scol[c] : uint64_t(0);
-      if (nsq == 0) {
+    if (nsq == 0) {
+      // with best = -inf, "v >= best" is the non-NaN test and the tie rule
+      // (the last maximum of the lane's column) at once
+      best = -__builtin_inff();
+      for (int c = b1s + lane; c < b1e; c += T) {
+        const float* col = m + c;
+        int br = -1;
+        if (scol == nullptr) {
+#pragma unroll 8
+          for (int r = b0s; r < b0e; ++r) {
+            const float v = col[r * n1];
+            if (v >= best) {
+              best = v;
+              br = r;
+            }
+          }
+        } else {
+          const uint64_t bits = scol[c];
+          const uint32_t lo = uint32_t(bits), hi = uint32_t(bits >> 32);
 #pragma unroll 4
-        for (int r = b0s; r < b0e; ++r) {
-          const float v = m[r * n1 + c];
-          if (v != v || ((bits >> r) & 1)) continue;
-          if (blin < 0 || v >= best) {
-            best = v;
-            blin = r * n1 + c;
+          for (int r = b0s; r < b0e; ++r) {
+            const float v = col[r * n1];
+            const uint32_t masked = ((r < 32 ? lo : hi) >> (r & 31)) & 1u;
+            if (!masked && v >= best) {
+              best = v;
+              br = r;
+            }
           }
         }
-      } else {
+        if (br >= 0) blin = br * n1 + c;
+      }
+    } else {
+      for (int c = b1s + lane; c < b1e; c += T) {
+        const uint64_t bits = scol != nullptr ? scol[c] : uint64_t(0);
         for (int r = b0s; r < b0e; ++r) {
           const float v = m[r * n1 + c];
           if (v != v || ((bits >> r) & 1) || in_squares(r, c)) continue;
