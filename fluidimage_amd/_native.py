@@ -158,6 +158,11 @@ SIGNATURES = {
     "piv_tps_point_slots": (
         ctypes.c_int, [_I, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "piv_tps_blend": (ctypes.c_int, [_I, _P, _P, _P, _P, ctypes.c_int64, _P, _P]),
+    "piv_tps_influence": (ctypes.c_int, [_I, _I, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P]),
+    "piv_tps_node_field": (
+        ctypes.c_int,
+        [_I, _I, _I, _I, _I, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_int64, _P, _P, _P,
+         ctypes.c_int64, _P, _P, _P, _P]),
     "piv_tps_grid": (
         ctypes.c_int,
         [_I, _I, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I, ctypes.c_double, _I,

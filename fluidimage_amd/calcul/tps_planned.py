@@ -319,7 +319,9 @@ def tps_planned(cap, dx, dy, xs, ys, new_x, new_y, subdom_size, smoothing_coef, 
     return PlannedTps(cap, m, {
         "info": info, "idx": idx, "fstat": fstat, "status": status, "smoothed": smoothed,
         "approx_all": approx, "stats": stats, "totals": totals, "counts": counts,
-        "ftab": ftab, "lim": lim, "smoothing": smoothing, "dom_off": dom_off})
+        "ftab": ftab, "lim": lim, "smoothing": smoothing, "dom_off": dom_off,
+        "mat_off": mat_off, "mats": mats, "refl": refl, "vwork": vwork, "slot_pt": slot_pt,
+        "c_slot": c_slot, "c_coef": c_coef, "c_norm": c_norm})
 
 
 def make_lattice_table(cap, device, stream=None):

@@ -2449,6 +2449,178 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 }
 
 // ---------------------------------------------------------------------------
+// 4d. the smoothing as a linear map of the values (differentiable node field)
+// ---------------------------------------------------------------------------
+// Without the outlier iterations, the smoothed values of a subdomain are
+// v - s w with [w; a] = M^-1 [v; 0] (thin_plate_spline.py:168-216), i.e.
+// (I - s Sw) v for the influence matrix Sw = (M^-1)[:n, :n]. Sw is built
+// once per subdomain (column j: the w part of the solution for [e_j; 0],
+// eight columns per pass of the multi-right-hand-side solve); the blend of
+// the subdomains' smoothed values at the centers is the reference's
+// (thin_plate_spline_subdom.py:145-158). The map and its transpose then cost
+// one matrix-vector product per subdomain and field.
+
+// Sw of subdomain blockIdx.x, row-major n x n (ld n) at infl + d stride
+__global__ __launch_bounds__(kThreads) void k_tps_influence(
+    const int64_t* __restrict__ dom_off, const int64_t* __restrict__ mat_off,
+    const double* __restrict__ mats, const double* __restrict__ refl,
+    const double* __restrict__ vwork, const int32_t* __restrict__ status, int nm,
+    double* __restrict__ infl, int64_t stride) {
+  const int d = blockIdx.x;
+  if (status[d]) return;
+  const int64_t s0 = dom_off[d];
+  const int n = int(dom_off[d + 1] - s0);
+  if (n == 0) return;
+  const int m = n - 3;
+  const double* E = mats + mat_off[d];
+  const int64_t ld = tps_mat_ld(m);
+  const double* A = E + tps_mat_e_size(m);
+  const double* RF = refl + 16 * d;
+  const double* VW = vwork + 6 * s0;
+  const double tau[3] = {RF[0], RF[1], RF[2]};
+  const int tid = threadIdx.x;
+  // dynamic LDS: x[8][nm], then the solve's scratch
+  extern __shared__ double dyn[];
+  double* x = dyn;
+  double* scr = dyn + 8 * nm;
+  __shared__ double red[8 * (kThreads / 64)];
+  __shared__ double aout[8][3];
+  double* S = infl + int64_t(d) * stride;
+  for (int j0 = 0; j0 < n; j0 += 8) {
+    for (int k = tid; k < n; k += kThreads)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) x[q * nm + k] = k == j0 + q ? 1.0 : 0.0;
+    __syncthreads();
+    saddle_solve_n<8>(x, nm, n, VW, tau, A, ld, E, RF, red, scr, aout);
+    for (int k = tid; k < n; k += kThreads)
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (j0 + q < n) S[int64_t(k) * n + j0 + q] = x[q * nm + k];
+    __syncthreads();
+  }
+}
+
+// out[c][f nslots + s0 + j] = in_j - s (Sw in)_j (transpose 0) or
+// in_j - s (Sw^T in)_j (transpose 1), in[c][f nslots + s0 + j]; subdomain
+// blockIdx.y, field blockIdx.x (consecutive workgroups read one Sw).
+// cstride: offset of component 1. Sw rows are read by wavefronts (lanes
+// over k), its columns by threads (thread j, loop over k).
+__global__ __launch_bounds__(kThreads) void k_tps_node_apply(
+    const int64_t* __restrict__ dom_off, const int32_t* __restrict__ status,
+    const double* __restrict__ smoothing_d, const double* __restrict__ infl, int64_t stride,
+    const double* __restrict__ in, double* __restrict__ out, int64_t nslots, int64_t cstride,
+    int transpose) {
+  const int d = blockIdx.y;
+  const int f = blockIdx.x;
+  if (status[d]) return;
+  const int64_t s0 = dom_off[d];
+  const int n = int(dom_off[d + 1] - s0);
+  if (n == 0) return;
+  const double sm = smoothing_d[d];
+  const double* S = infl + int64_t(d) * stride;
+  const int64_t base = int64_t(f) * nslots + s0;
+  extern __shared__ double dyn[];
+  double* xv[2] = {dyn, dyn + n};
+  const int tid = threadIdx.x;
+  for (int j = tid; j < n; j += kThreads) {
+    xv[0][j] = in[base + j];
+    xv[1][j] = in[cstride + base + j];
+  }
+  __syncthreads();
+  if (transpose) {
+    for (int j = tid; j < n; j += kThreads) {
+      double a0 = 0.0, a1 = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double w = S[int64_t(k) * n + j];
+        a0 += w * xv[0][k];
+        a1 += w * xv[1][k];
+      }
+      out[base + j] = xv[0][j] - sm * a0;
+      out[cstride + base + j] = xv[1][j] - sm * a1;
+    }
+  } else {
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int j = wave; j < n; j += kThreads / 64) {
+      const double* row = S + int64_t(j) * n;
+      double a0 = 0.0, a1 = 0.0;
+      for (int k = lane; k < n; k += 64) {
+        const double w = row[k];
+        a0 += w * xv[0][k];
+        a1 += w * xv[1][k];
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 64);
+        a1 += __shfl_xor(a1, off, 64);
+      }
+      if (lane == 0) {
+        out[base + j] = xv[0][j] - sm * a0;
+        out[cstride + base + j] = xv[1][j] - sm * a1;
+      }
+    }
+  }
+}
+
+// The per-center steps of the node field, fields f = blockIdx.y, centers i:
+// mode 0 (gather): slots[c][f nslots + s] = v[c][f nc + slot_pt[s]];
+// mode 1 (blend): out[c][f nc + i] = sum_k coef[4 i + k] slots[c][f nslots
+//   + slot_k] / norm[i] (the order of k_tps_blend);
+// mode 2 (blend transposed): slots[c][f nslots + slot_k] = coef[4 i + k] /
+//   norm[i] g[c][f nc + i];
+// mode 3 (gather transposed): out[c][f nc + i] = sum_k slots[c][f nslots +
+//   slot_k].
+// Modes 0 run over slots s < nused, the others over centers i < nc.
+__global__ void k_tps_node_steps(int mode, int nc, int64_t nused, const int32_t* __restrict__ slot_pt,
+                                 const int32_t* __restrict__ c_slot,
+                                 const double* __restrict__ c_coef,
+                                 const double* __restrict__ c_norm, int64_t nslots,
+                                 int64_t scstride, int64_t ncstride,
+                                 const double* __restrict__ src, double* __restrict__ dst) {
+  const int f = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (mode == 0) {
+    if (i >= nused) return;
+    const int64_t q = int64_t(f) * nc + slot_pt[i];
+    dst[int64_t(f) * nslots + i] = src[q];
+    dst[scstride + int64_t(f) * nslots + i] = src[ncstride + q];
+    return;
+  }
+  if (i >= nc) return;
+  const int64_t pi = int64_t(f) * nc + i;
+  const int64_t fs = int64_t(f) * nslots;
+  if (mode == 1) {
+    double a0 = 0.0, a1 = 0.0;
+    for (int k = 0; k < kMaxSlots; ++k) {
+      const int sl = c_slot[kMaxSlots * i + k];
+      if (sl < 0) break;
+      const double c = c_coef[kMaxSlots * i + k];
+      a0 += c * src[fs + sl];
+      a1 += c * src[scstride + fs + sl];
+    }
+    dst[pi] = a0 / c_norm[i];
+    dst[ncstride + pi] = a1 / c_norm[i];
+  } else if (mode == 2) {
+    const double g0 = src[pi] / c_norm[i], g1 = src[ncstride + pi] / c_norm[i];
+    for (int k = 0; k < kMaxSlots; ++k) {
+      const int sl = c_slot[kMaxSlots * i + k];
+      if (sl < 0) break;
+      const double c = c_coef[kMaxSlots * i + k];
+      dst[fs + sl] = c * g0;
+      dst[scstride + fs + sl] = c * g1;
+    }
+  } else {
+    double a0 = 0.0, a1 = 0.0;
+    for (int k = 0; k < kMaxSlots; ++k) {
+      const int sl = c_slot[kMaxSlots * i + k];
+      if (sl < 0) break;
+      a0 += src[fs + sl];
+      a1 += src[scstride + fs + sl];
+    }
+    dst[pi] = a0;
+    dst[ncstride + pi] = a1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 5. blend of the smoothed values, 6. evaluation at new positions
 // ---------------------------------------------------------------------------
 __global__ void k_tps_blend(int npts, const int32_t* __restrict__ slots,
@@ -2984,6 +3156,64 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
                      smooth_slots, nslots, weights, wstride, stats, status, slot_xy, w.gmat,
                      int(cap_r), nm);
   return tps_check("piv_tps_smooth_shared");
+}
+
+int piv_tps_influence(int32_t nd, int32_t n_max, const int64_t* dom_off,
+                      const int64_t* mat_off, const double* mats, const double* refl,
+                      const double* vwork, const int32_t* status, double* infl, int64_t stride,
+                      void* stream) {
+  if (nd <= 0) return PIV_OK;
+  if (n_max < 4 || n_max > kMaxN) return tps_fail(PIV_EINVAL, "n_max must be in [4, 2048]");
+  if (stride < int64_t(n_max) * n_max) return tps_fail(PIV_EINVAL, "stride < n_max^2");
+  if (!dom_off || !mat_off || !mats || !refl || !vwork || !status || !infl)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  const int nm = (n_max + 1) & ~1;
+  const size_t lds = (size_t(8) * nm + kSolveScratch) * sizeof(double);
+  if (lds > 120 * 1024) return tps_fail(PIV_EUNSUPPORTED, "n_max too large for the LDS");
+  hipLaunchKernelGGL(k_tps_influence, dim3(nd), dim3(kThreads), lds,
+                     static_cast<hipStream_t>(stream), dom_off, mat_off, mats, refl, vwork,
+                     status, nm, infl, stride);
+  return tps_check("piv_tps_influence");
+}
+
+int piv_tps_node_field(int32_t transpose, int32_t nfields, int32_t nc, int32_t nd,
+                       int32_t n_max, const int64_t* dom_off, const int32_t* slot_pt,
+                       int64_t nused, const int32_t* status, const double* smoothing,
+                       const double* infl, int64_t stride, const int32_t* c_slot,
+                       const double* c_coef, const double* c_norm, int64_t nslots,
+                       const double* in, double* out, double* work, void* stream) {
+  if (nfields <= 0 || nc <= 0 || nd <= 0) return PIV_OK;
+  if (n_max < 4 || n_max > kMaxN) return tps_fail(PIV_EINVAL, "n_max must be in [4, 2048]");
+  if (nused < 0 || nused > nslots) return tps_fail(PIV_EINVAL, "nused must be in [0, nslots]");
+  if (nfields > 65535 || nd > 65535) return tps_fail(PIV_EUNSUPPORTED, "too many fields or subdomains");
+  if (!dom_off || !slot_pt || !status || !smoothing || !infl || !c_slot || !c_coef || !c_norm ||
+      !in || !out || !work)
+    return tps_fail(PIV_EINVAL, "NULL argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t sc = int64_t(nfields) * nslots;  // component stride of the slot buffers
+  const int64_t ncs = int64_t(nfields) * nc;    // component stride of in / out
+  double* w0 = work;
+  double* w1 = work + 2 * sc;
+  const dim3 gc((nc + 255) / 256, nfields);
+  const size_t lds = size_t(2) * n_max * sizeof(double);
+  if (!transpose) {
+    const dim3 gs(unsigned((nused + 255) / 256), nfields);
+    if (nused > 0)
+      hipLaunchKernelGGL(k_tps_node_steps, gs, dim3(256), 0, s, 0, int(nc), nused, slot_pt,
+                         c_slot, c_coef, c_norm, nslots, sc, ncs, in, w0);
+    hipLaunchKernelGGL(k_tps_node_apply, dim3(nfields, nd), dim3(kThreads), lds, s, dom_off,
+                       status, smoothing, infl, stride, w0, w1, nslots, sc, 0);
+    hipLaunchKernelGGL(k_tps_node_steps, gc, dim3(256), 0, s, 1, int(nc), nused, slot_pt, c_slot,
+                       c_coef, c_norm, nslots, sc, ncs, w1, out);
+  } else {
+    hipLaunchKernelGGL(k_tps_node_steps, gc, dim3(256), 0, s, 2, int(nc), nused, slot_pt, c_slot,
+                       c_coef, c_norm, nslots, sc, ncs, in, w0);
+    hipLaunchKernelGGL(k_tps_node_apply, dim3(nfields, nd), dim3(kThreads), lds, s, dom_off,
+                       status, smoothing, infl, stride, w0, w1, nslots, sc, 1);
+    hipLaunchKernelGGL(k_tps_node_steps, gc, dim3(256), 0, s, 3, int(nc), nused, slot_pt, c_slot,
+                       c_coef, c_norm, nslots, sc, ncs, w1, out);
+  }
+  return tps_check("piv_tps_node_field");
 }
 
 int piv_tps_select(int32_t nfields, int32_t n, const float* dx, const float* dy,

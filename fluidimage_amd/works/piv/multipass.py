@@ -36,10 +36,14 @@ class WorkPIV:
                 "smoothing_coef": 5.0,
                 "threshold_tps": 1.5,
                 # extensions (no counterpart in the reference): image
-                # deformation between passes (None or "bicubic") and gradient
-                # refinement steps after the last pass (0: none)
+                # deformation between passes (None or "bicubic"), gradient
+                # refinement steps after the last pass (0: none) and their
+                # unknowns ("tps": TPS node values smoothed by the TPS of
+                # subdom_size and smoothing_coef; "bilinear": the node
+                # displacements themselves)
                 "image_deformation": None,
                 "refine_steps": 0,
+                "refine_field": "tps",
             },
         )
 

@@ -44,6 +44,7 @@ class MultipassPipeline:
         self.params = work_piv.params
         self._deformed = {}
         self._refine_problem = {}
+        self._refine_fields = {}
         # the TPS calls do not wait for their singular-system check: it runs
         # once at the end of ``run``, and a pair that failed it is run again
         # alone with the checks in place (griddata fallback and all)
@@ -367,19 +368,27 @@ class MultipassPipeline:
         when it lowers its pair's loss)."""
         import torch
 
-        from .refine import RefineProblem, refine
+        from .refine import RefineProblem, node_field, refine
 
         key = id(imageset)
         prob = self._refine_problem.get(key)
         if prob is None or prob.npairs != res.npairs:
             prob = RefineProblem(res.work, imageset, res.npairs)
             self._refine_problem = {key: prob}
+        mp = self.params.multipass
+        field = None
+        if getattr(mp, "refine_field", "tps") == "tps":
+            fkey = (res.work.index_pass, res.work.imshape0, mp.subdom_size, mp.smoothing_coef)
+            field = self._refine_fields.get(fkey)
+            if field is None:
+                field = self._refine_fields[fkey] = node_field(prob, mp.subdom_size,
+                                                               mp.smoothing_coef)
         ev = self._event_pair(timers, "refine")
         if ev:
             ev[0].record()
         dx0 = torch.stack([res.final[k][0] for k in range(res.npairs)])
         dy0 = torch.stack([res.final[k][1] for k in range(res.npairs)])
-        dx, dy, l0, l1, acc = refine(prob, dx0, dy0, steps=steps)
+        dx, dy, l0, l1, acc = refine(prob, dx0, dy0, steps=steps, field=field)
         if ev:
             ev[1].record()
         for k in range(res.npairs):

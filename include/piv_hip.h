@@ -458,6 +458,33 @@ int piv_tps_blend(int32_t npts, const int32_t* slots, const double* coef,
                   const double* norm, const double* smooth_slots,
                   int64_t nslots, double* out, void* stream);
 
+/* Differentiable TPS node field (no counterpart in the reference, which
+ * does not differentiate; the map is its smoothing without the outlier
+ * iterations, thin_plate_spline.py:168-216 per subdomain and the blend of
+ * thin_plate_spline_subdom.py:145-158). The influence matrix of subdomain d,
+ * Sw = (M^-1)[:n, :n] of its factorised saddle-point matrix (from
+ * piv_tps_smooth: dom_off, mat_off, mats, refl, vwork, status), row-major
+ * n x n at infl + d * stride, stride >= n_max^2 doubles. */
+int piv_tps_influence(int32_t nd, int32_t n_max, const int64_t* dom_off,
+                      const int64_t* mat_off, const double* mats, const double* refl,
+                      const double* vwork, const int32_t* status, double* infl,
+                      int64_t stride, void* stream);
+
+/* The smoothed field at the nc centers of one TPS structure (nd subdomains,
+ * members slot_pt, used slots nused, smoothing per subdomain, influence
+ * matrices of piv_tps_influence, the centers' slots / blend coefficients /
+ * norms c_slot, c_coef, c_norm of piv_tps_point_slots) as a linear map of
+ * the values, for nfields fields at once: transpose 0: out = T in (in: the
+ * values, out: the smoothed values, both [2][nfields nc]); transpose 1:
+ * out = T^T in (the gradient of the values from that of the smoothed
+ * values). work: 4 nfields nslots doubles. Every status must be 0. */
+int piv_tps_node_field(int32_t transpose, int32_t nfields, int32_t nc, int32_t nd,
+                       int32_t n_max, const int64_t* dom_off, const int32_t* slot_pt,
+                       int64_t nused, const int32_t* status, const double* smoothing,
+                       const double* infl, int64_t stride, const int32_t* c_slot,
+                       const double* c_coef, const double* c_norm, int64_t nslots,
+                       const double* in, double* out, double* work, void* stream);
+
 /* TPS evaluation at npts new positions (px, py): for each subdomain
  * doms[4 p + k] (-1 terminated) of the point, the weighted sum of
  * w_j r^2 ln(r^2) / 2 over its centers plus the linear part, blended with

@@ -203,3 +203,30 @@ class ThinPlateSplineSubdom:
             )
         values /= self.norm_coefs_new_pos
         return values
+
+
+def smoothing_operator(centers, subdom_size, smoothing_coef, percent_buffer_area=20):
+    """Dense T (num_centers x num_centers) with T v = the smoothed values of
+    ThinPlateSplineSubdom.compute_tps_weights_subdom(v) without outlier
+    iterations (thin_plate_spline_subdom.py:128-159, threshold None): per
+    subdomain, the rows of compute_tps_weights' solve (thin_plate_spline.py:
+    168-216) give smoothed = v - s w with w = inv(M)[:n, :n] v; the blend
+    weights as the reference's."""
+    tps = ThinPlateSplineSubdom(centers, subdom_size, smoothing_coef, None,
+                                percent_buffer_area)
+    num = tps.num_centers
+    out = np.zeros((num, num))
+    s = tps.smoothing_coef
+    for i_dom in range(tps.nb_subdom):
+        ind = tps.indices_domains[i_dom]
+        n = ind.size
+        c = tps.centers[:, ind]
+        em = compute_tps_matrix(c, c).T
+        smoothing = np.hstack([s * np.eye(n), np.zeros([n, 3])])
+        pm = np.hstack([np.ones([n, 1]), c.T])
+        im = np.vstack([em + smoothing, np.hstack([pm.T, np.zeros([3, 3])])])
+        sw = np.linalg.inv(im)[:n, :n]
+        blk = tps.norm_coefs_domains[i_dom][:, None] * (np.eye(n) - s * sw)
+        out[np.ix_(ind, ind)] += blk
+    out /= tps.norm_coefs[:, None]
+    return out

@@ -183,6 +183,26 @@ def test_outlier_loop_without_threshold_and_with_a_large_one():
     assert s["nb_iterations"] == 1 and s["nb_fixed_vectors"] == 0
 
 
+def test_smoothing_operator_is_the_subdomain_smoothing():
+    """The dense operator T of the differentiable node field: T v equals
+    compute_tps_weights_subdom(v)'s smoothed values (threshold None) on a
+    grid of several subdomains, and reproduces an affine field exactly
+    (w = 0 for affine values, whatever s)."""
+    c = _grid_centers(0, 240, 0, 200, 8)
+    t = otps.ThinPlateSplineSubdom(c, 150, 2.0, threshold=None)
+    assert t.nb_subdom >= 4
+    T = otps.smoothing_operator(c, 150, 2.0)
+    rng = np.random.default_rng(12)
+    for _ in range(2):
+        v = rng.normal(0, 2, c.shape[1])
+        sm, _, _ = t.compute_tps_weights_subdom(v)
+        assert np.allclose(T @ v, sm, rtol=0, atol=1e-10 * np.abs(sm).max())
+    aff = 3.0 + 0.25 * c[1] - 0.4 * c[0]
+    assert np.allclose(T @ aff, aff, rtol=0, atol=1e-8)
+    # a value's weight in its own smoothed value: below one for s > 0
+    assert (np.diag(T) < 1).all()
+
+
 def test_outlier_loop_stops_after_eleven_solves():
     """threshold 0 with noisy values and s > 0: the loop never converges and
     stops when its counter passes 10, i.e. after the first solve and ten
