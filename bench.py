@@ -307,6 +307,83 @@ def run_bench(args, pairs, keep_maps, steps, warmup):
     return work, images, state["passes"], wall / steps * 1e3, avg
 
 
+def pcie_inclusive(args, pairs, images, work, passes, keep_maps, nvec, npairs):
+    """The same step with the host transfers a caller holding host arrays
+    pays: the images uploaded from pinned host memory into the device images
+    before the step, and the last pass's vectors (dx, dy, correl_max, errors)
+    and the final interpolated fields copied back after it; separately, the
+    copy of every kept map of the step to pinned host memory (the reference's
+    ``correls``). Timed with HIP events on the launch stream, a few steps."""
+    import torch
+
+    from fluidimage_amd.works.piv.pipeline import MultipassPipeline
+
+    pipe = MultipassPipeline(work)
+    import numpy as np
+
+    host_ims = []
+    for a, b in pairs:
+        for im in (a, b):
+            a16 = im.dtype == np.uint16
+            host_ims.append(torch.from_numpy(im.view(np.int16) if a16 else im).pin_memory())
+    # uint16 device images are filled through an int16 view (same bytes)
+    dev_ims = [d.view(torch.int16) if d.dtype != torch.uint8 and d.element_size() == 2 else d
+               for d in images.images]
+    steps = max(2, args.steps // 2)
+
+    def results_of(ps):
+        last = ps[-1]
+        ts = [last.out[k] for k in ("dx", "dy", "correl_max", "err")]
+        if args.workload == "multipass":
+            ts += [t for f in last.final if f is not None for t in f]
+        return ts
+
+    hosts = None
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    up = comp = down = 0.0
+    for it in range(steps + 1):
+        ev[0].record()
+        for h, d in zip(host_ims, dev_ims):
+            d.copy_(h, non_blocking=True)
+        ev[1].record()
+        images.recompute_min()
+        ps = pipe.run(images, npairs, keep_maps=keep_maps,
+                      final_interp=args.workload == "multipass")
+        ev[2].record()
+        res = results_of(ps)
+        if hosts is None:
+            hosts = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True) for t in res]
+        for h, t in zip(hosts, res):
+            h.copy_(t, non_blocking=True)
+        ev[3].record()
+        torch.cuda.synchronize()
+        if it > 0:  # the first one warms the pinned buffers
+            up += ev[0].elapsed_time(ev[1])
+            comp += ev[1].elapsed_time(ev[2])
+            down += ev[2].elapsed_time(ev[3])
+    up, comp, down = up / steps, comp / steps, down / steps
+    rec = {"upload_ms": round(up, 3), "step_ms": round(comp, 3),
+           "results_download_ms": round(down, 3),
+           "vectors/s_with_upload_and_results": round(nvec * npairs / ((up + comp + down) / 1e3), 1)}
+    if keep_maps:
+        maps = [p.out["maps"] for p in ps if p.out.get("maps") is not None]
+        nbytes = sum(m.numel() * m.element_size() for m in maps)
+        hmaps = [torch.empty(m.shape, dtype=m.dtype, pin_memory=True) for m in maps]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for h, m in zip(hmaps, maps):
+            h.copy_(m, non_blocking=True)
+        ev[1].record()
+        torch.cuda.synchronize()
+        mms = ev[0].elapsed_time(ev[1])
+        rec["maps_download_ms"] = round(mms, 3)
+        rec["maps_GB"] = round(nbytes / 1e9, 3)
+        rec["vectors/s_with_upload_results_and_maps"] = round(
+            nvec * npairs / ((up + comp + down + mms) / 1e3), 1)
+        del hmaps
+    return rec
+
+
 def bench_refine(args):
     """Differentiable refinement: P 1024x1024 float32 pairs, multipass of
     32x32 windows (2 passes 64/32, TPS), then 20 gradient steps over the batch
@@ -618,6 +695,11 @@ def main():
             "profile_tag": ktag,
         },
     }
+
+    if not args.graph:
+        print("bench: host-transfer-inclusive steps", file=sys.stderr, flush=True)
+        line["config"]["pcie"] = pcie_inclusive(args, pairs, images, work, passes, keep_maps,
+                                                nvec, npairs)
 
     if not args.no_plain and npairs > 1:
         print("bench: plain per-pair steps", file=sys.stderr, flush=True)
