@@ -18,7 +18,7 @@ TUNING_PATH = HERE / f"tuning_{ARCH}.json"
 
 PIV_OK = 0
 PIV_U8, PIV_U16, PIV_F32 = 0, 1, 2
-PIV_SUBPIX = {"2d_gaussian2": 0, "centroid": 1, "no_subpix": 2}
+PIV_SUBPIX = {"2d_gaussian2": 0, "centroid": 1, "no_subpix": 2, "2d_gaussian": 3}
 PIV_WINFLAG_INVALID = 1
 PIV_OPT_SUBPIX = 1
 PIV_OPT_MAP_SQUARE = 2

@@ -69,11 +69,6 @@ class SubPixParams:
             )
         if method not in self.methods:
             raise ValueError(f"method has to be in {self.methods}")
-        if method == "2d_gaussian":
-            raise NotImplementedError(
-                "method '2d_gaussian' is not provided by the HIP backend "
-                "(WorkPIV uses '2d_gaussian2')"
-            )
         self.method = method
         self.n = 1 if nsubpix is None else int(nsubpix)
 
@@ -139,7 +134,8 @@ class CorrelHIP:
         n0, n1 = self.im0_shape
         if not lib.piv_shape_supported(n0, n1):
             raise NotImplementedError(
-                f"window shape {self.im0_shape} is not built into libpiv_hip"
+                f"window shape {self.im0_shape} is not supported by libpiv_hip "
+                "(rows and columns of 2 to 256, at most 16384 elements)"
             )
         plan = ctypes.c_void_p()
         if self.displacement_max is None:

@@ -65,6 +65,10 @@ struct PlanArgs {
   int particle_radius;
   int nb_peaks;
   int has_dmax;
+  // the maps the search and fit see are map_scale^-1 times the reference's
+  // (a power of two; 0 reads as 1): the '2d_gaussian' fit, which is not
+  // scale invariant (values <= 0 become 1e-6), scales them back
+  float map_scale;
 };
 
 // Peak search of one window on an LDS search copy of the map in which the
@@ -324,14 +328,58 @@ __device__ __forceinline__ void centroid(const float* m, int n1, int ix, int iy,
   dpy = sy / s;
 }
 
+// The '2d_gaussian' fit of SubPix.compute_subpix (subpix.py:157-181): least
+// squares of log c = a x^2 + b y^2 + c x + d y + e on the (2n+1)^2 crop
+// (values <= 0 taken as 1e-6, float32 log). On the symmetric grid the normal
+// equations decouple (the odd columns x, y are orthogonal to the even ones):
+// with N = 2n + 1, S2 = sum x^2, S4 = sum x^4 over -n..n, u = sum x^2 l,
+// v = sum y^2 l, w = sum l: a = (u - S2 w / N) / (N S4 - S2^2), the same for
+// b with v, c = sum x l / (N S2), d = sum y l / (N S2) -- the reference's
+// pseudo-inverse of M = [X^2, Y^2, X, Y, 1] applied to the logs. Returns
+// false where the reference falls back to the centroid (a > 0 or b > 0, or
+// a NaN displacement); sigma^2 = (1 / sqrt(-2 a))^2 as the reference rounds it.
+__device__ __forceinline__ bool gaussian_ls(const float* m, int n1, int ix, int iy, int n,
+                                            float scale, double& dpx, double& dpy) {
+  double u = 0, v = 0, w = 0, sx = 0, sy = 0;
+  for (int a = -n; a <= n; ++a)
+    for (int b = -n; b <= n; ++b) {
+      float c = m[(iy + a) * n1 + (ix + b)] * scale;
+      if (c <= 0.f) c = 1e-6f;
+      const double l = double(logf(c));
+      u += double(b * b) * l;
+      v += double(a * a) * l;
+      w += l;
+      sx += double(b) * l;
+      sy += double(a) * l;
+    }
+  const double N = 2 * n + 1;
+  const double S2 = double(n) * (n + 1) * (2 * n + 1) / 3.0;
+  double S4 = 0.0;
+  for (int k = 1; k <= n; ++k) S4 += 2.0 * double(k) * k * k * k;
+  const double den = N * S4 - S2 * S2;
+  const double ca = (u - S2 * w / N) / den;
+  const double cb = (v - S2 * w / N) / den;
+  if (ca > 0.0 || cb > 0.0) return false;
+  const double sgx = 1.0 / sqrt(-2.0 * ca), sgy = 1.0 / sqrt(-2.0 * cb);
+  dpx = sx / (N * S2) * (sgx * sgx);
+  dpy = sy / (N * S2) * (sgy * sgy);
+  return !(isnan(dpx) || isnan(dpy));
+}
+
 // SubPix.compute_subpix (subpix.py:107-218) -> PIV_WIN_* code; map n0 x n1
 __device__ __forceinline__ int subpix(const float* m, int n0, int n1, int method, int nsub,
-                                      int ix, int iy, double& dpx, double& dpy) {
+                                      int ix, int iy, double& dpx, double& dpy,
+                                      float map_scale = 0.f) {
   dpx = dpy = 0.0;
   if (iy - nsub < 0 || iy + nsub + 1 > n0 || ix - nsub < 0 || ix + nsub + 1 > n1)
     return PIV_WIN_SUBPIX_BOUNDARY;
   double thr = 2.0 * (0.5 + nsub) * (0.5 + nsub);
   if (method == PIV_SUBPIX_NONE) return PIV_WIN_OK;
+  if (method == PIV_SUBPIX_2D_GAUSSIAN) {
+    // a too large displacement of this fit is an error, not a fallback
+    if (gaussian_ls(m, n1, ix, iy, nsub, map_scale > 0.f ? map_scale : 1.f, dpx, dpy))
+      return dpx * dpx + dpy * dpy > thr ? PIV_WIN_SUBPIX_WRONG : PIV_WIN_OK;
+  }
   if (method == PIV_SUBPIX_2D_GAUSSIAN2) {
     gaussian2(m, n1, ix, iy, dpx, dpy);
     if (dpx * dpx + dpy * dpy > thr) method = PIV_SUBPIX_CENTROID;
@@ -453,7 +501,8 @@ __device__ __forceinline__ void finish_window_with(S& s, const float* m, int n0,
         res.sec[3 * o + 2] = peak_value(m[j0 * n1 + j1], norm);
         if (res.sec_sub) {
           double px, py;
-          const int e = subpix(m, n0, n1, pa.subpix_method, pa.nsubpix, j1, j0, px, py);
+          const int e =
+              subpix(m, n0, n1, pa.subpix_method, pa.nsubpix, j1, j0, px, py, pa.map_scale);
           res.sec_sub[2 * o + 0] = e ? __builtin_nanf("") : float(ix0 - (j1 + px));
           res.sec_sub[2 * o + 1] = e ? __builtin_nanf("") : float(iy0 - (j0 + py));
         }
@@ -466,7 +515,8 @@ __device__ __forceinline__ void finish_window_with(S& s, const float* m, int n0,
     int err = PIV_WIN_OK;
     if (apply_subpix) {
       double px, py;
-      const int e = subpix(m, n0, n1, pa.subpix_method, pa.nsubpix, i1, i0, px, py);
+      const int e =
+          subpix(m, n0, n1, pa.subpix_method, pa.nsubpix, i1, i0, px, py, pa.map_scale);
       if (e == PIV_WIN_OK) {
         dx = float(ix0 - (i1 + px));
         dy = float(iy0 - (i0 + py));

@@ -217,7 +217,7 @@ int piv_peaks_any(int32_t nwin, float* maps, const double* norms, int32_t n0, in
     return direct_fail(PIV_EINVAL, "NULL argument");
   if (n0 < 3 || n1 < 3 || n0 > kMaxAnyMap || n1 > kMaxAnyMap)
     return direct_fail(PIV_EUNSUPPORTED, "maps of 3 x 3 to 127 x 127 only");
-  if (subpix_method < 0 || subpix_method > PIV_SUBPIX_NONE || nsubpix < 1 ||
+  if (subpix_method < 0 || subpix_method > PIV_SUBPIX_2D_GAUSSIAN || nsubpix < 1 ||
       particle_radius < 0 || nb_peaks < 1)
     return direct_fail(PIV_EINVAL, "bad search parameters");
   if (nb_peaks > 1 && (res->sec == nullptr || res->nsec == nullptr))
@@ -246,7 +246,7 @@ int piv_subpix_any(int32_t nwin, const float* maps, int32_t n0, int32_t n1, cons
   if (nwin <= 0) return PIV_OK;
   if (!maps || !ix || !iy || !out_x || !out_y || !err)
     return direct_fail(PIV_EINVAL, "NULL argument");
-  if (n0 < 1 || n1 < 1 || subpix_method < 0 || subpix_method > PIV_SUBPIX_NONE || nsubpix < 1)
+  if (n0 < 1 || n1 < 1 || subpix_method < 0 || subpix_method > PIV_SUBPIX_2D_GAUSSIAN || nsubpix < 1)
     return direct_fail(PIV_EINVAL, "bad sub-pixel parameters");
   hipLaunchKernelGGL(k_subpix_any, dim3((nwin + 255) / 256), dim3(256), 0,
                      static_cast<hipStream_t>(stream), maps, int(n0), int(n1), ix, iy,

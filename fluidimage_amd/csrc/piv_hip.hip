@@ -374,8 +374,10 @@ __attribute__((amdgpu_waves_per_eu(correlate_min_waves<N0, N1>()))) void k_corre
   int sq[4] = {0, 0, 0, 0};
   if (compute) {
     if (lane == 0 && res.norm) res.norm[win] = 0.25 * norm;
+    PlanArgs pl = pa;
+    pl.map_scale = 0.25f;  // the LDS map is 4 times the reference's
     finish_window_masked<T>(mapf, pa.mask != nullptr ? scol : nullptr, N0, N1, N0 / 2 - 1,
-                            N1 / 2 - 1, pa, lane, win, norm, apply_subpix, w.dx_in, w.dy_in,
+                            N1 / 2 - 1, pl, lane, win, norm, apply_subpix, w.dx_in, w.dy_in,
                             res, sq);
   } else if (live && lane == 0) {
     const float nan = __builtin_nanf("");
@@ -705,6 +707,291 @@ __global__ __launch_bounds__(kBigThreads) void k_correlate_big(
       out[k] = v;
     }
   }
+}
+
+// -------------------------------------------------------------------------
+// windows of any other shape
+// -------------------------------------------------------------------------
+// The reference takes any window shape through FFTW (fft.py:205-244,
+// correl.py:596-662). Shapes without a register-FFT kernel run here: one
+// workgroup of 256 threads per window, the window's complex buffer
+// z = im0' + i im1' in LDS (8 n0 n1 bytes), transformed by separable DFTs
+// (each output a sum over its line with a double-precision twiddle table,
+// accumulated in double, stored as float like an FFT stage), then the same
+// steps as the fused kernel: F0 / F1 from Z(k) and Z(-k), DC bins zero,
+// half-spectrum energies with the reference's column weights, conj(F0) F1,
+// inverse DFTs, the map in the _like_fftshift layout, masked peak search,
+// second chance, secondary peaks and sub-pixel fit. Values are unscaled (no
+// factor 2 on the spectra).
+constexpr int kAnyThreads = 256;
+constexpr int kAnyMaxLen = 256;       // rows or columns of a window
+constexpr int kAnyMaxElems = 16384;   // n0 n1: 128 KB of complex buffer
+constexpr int kAnyChunk = 8;          // outputs per thread in one DFT step
+
+// shapes served by k_correlate_any
+bool any_shape_ok(int n0, int n1) {
+  return n0 >= 2 && n1 >= 2 && n0 <= kAnyMaxLen && n1 <= kAnyMaxLen && n0 * n1 <= kAnyMaxElems;
+}
+
+// the complex buffer, then the twiddle tables at a 16-byte boundary
+size_t any_tw_offset(int n0, int n1) { return (size_t(8) * n0 * n1 + 15) / 16 * 16; }
+size_t any_lds_bytes(int n0, int n1) { return any_tw_offset(n0, n1) + size_t(16) * (n0 + n1); }
+
+// tw[k] = (cos, sin)(2 pi k / n)
+__device__ __forceinline__ void any_twiddles(double2* tw, int n) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    double s, c;
+    sincospi(2.0 * double(k) / double(n), &s, &c);
+    tw[k] = make_double2(c, s);
+  }
+}
+
+// In-place DFTs of `nlines` lines of `len` elements (element e of line l at
+// buf[l * lstride + e * estride]): out[k] = sum_e x[e] exp(sign 2 pi i e k /
+// len). Lines are done in chunks of at most kAnyThreads * kAnyChunk
+// outputs whose inputs are all read before any output is written.
+__device__ void any_dft_lines(float2* buf, int nlines, int len, int lstride, int estride,
+                              const double2* tw, double sign) {
+  const int tid = threadIdx.x;
+  const int lines_per_chunk = max(1, (kAnyThreads * kAnyChunk) / len);
+  for (int l0 = 0; l0 < nlines; l0 += lines_per_chunk) {
+    const int nout = (min(nlines, l0 + lines_per_chunk) - l0) * len;
+    float2 out[kAnyChunk];
+#pragma unroll
+    for (int j = 0; j < kAnyChunk; ++j) {
+      const int o = tid + j * kAnyThreads;
+      out[j] = make_float2(0.f, 0.f);
+      if (o < nout) {
+        const int line = l0 + o / len, k = o % len;
+        const float2* x = buf + line * lstride;
+        double ar = 0.0, ai = 0.0;
+        int idx = 0;
+        for (int e = 0; e < len; ++e) {
+          const float2 v = x[e * estride];
+          const double2 w = tw[idx];
+          const double wi = sign * w.y;
+          ar += double(v.x) * w.x - double(v.y) * wi;
+          ai += double(v.x) * wi + double(v.y) * w.x;
+          idx += k;
+          if (idx >= len) idx -= len;
+        }
+        out[j] = make_float2(float(ar), float(ai));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kAnyChunk; ++j) {
+      const int o = tid + j * kAnyThreads;
+      if (o < nout) buf[(l0 + o / len) * lstride + (o % len) * estride] = out[j];
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ double any_block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < kAnyThreads / 64; ++w) s += red[w];
+  return s;
+}
+
+template <int DT>
+__global__ __launch_bounds__(kAnyThreads) void k_correlate_any(
+    int n0, int n1, const piv_image* __restrict__ images, const piv_window* __restrict__ windows,
+    int nwin, int options, float* __restrict__ maps, piv_results res, PlanArgs pa) {
+  extern __shared__ __align__(16) unsigned char any_smem[];
+  const int nn = n0 * n1;
+  float2* buf = reinterpret_cast<float2*>(any_smem);
+  double2* tw0 = reinterpret_cast<double2*>(any_smem + (size_t(8) * nn + 15) / 16 * 16);
+  double2* tw1 = tw0 + n0;
+  __shared__ double red[kAnyThreads / 64];
+  __shared__ int sqs[4];
+  const int tid = threadIdx.x;
+  const int win = blockIdx.x;
+  if (win >= nwin) return;
+  const piv_window w = windows[win];
+  const bool compute = !(w.flags & PIV_WINFLAG_INVALID);
+  const int apply_subpix = options & PIV_OPT_SUBPIX;
+  float* fl = reinterpret_cast<float*>(any_smem);
+  float* mapf = fl + nn;  // the map in the reference layout (after the DFTs)
+  float* srch = fl;       // its search copy, masked elements NaN
+  double norm = 0.0;
+  if (compute) {
+    any_twiddles(tw0, n0);
+    any_twiddles(tw1, n1);
+    // ---- crop (zero padding, min offset) and float32 mean removal ----
+    const piv_image im0 = images[w.image0];
+    const piv_image im1 = images[w.image1];
+    const float m0 = im0.minval ? *im0.minval : 0.f;
+    const float m1 = im1.minval ? *im1.minval : 0.f;
+    float s0 = 0.f, s1 = 0.f;
+    for (int k = tid; k < nn; k += kAnyThreads) {
+      const int r = k / n1, c = k % n1;
+      const int ya = w.y0 + r, xa = w.x0 + c, yb = w.y1 + r, xb = w.x1 + c;
+      float a = 0.f, b = 0.f;
+      if (ya >= 0 && ya < im0.height && xa >= 0 && xa < im0.width)
+        a = gfetch<DT>(im0.data, int64_t(ya) * im0.pitch + xa) - m0;
+      if (yb >= 0 && yb < im1.height && xb >= 0 && xb < im1.width)
+        b = gfetch<DT>(im1.data, int64_t(yb) * im1.pitch + xb) - m1;
+      buf[k] = make_float2(a, b);
+      s0 += a;
+      s1 += b;
+    }
+    const float mean0 = float(any_block_sum(double(s0), red)) / float(nn);
+    const float mean1 = float(any_block_sum(double(s1), red)) / float(nn);
+    for (int k = tid; k < nn; k += kAnyThreads) {
+      buf[k].x -= mean0;
+      buf[k].y -= mean1;
+    }
+    __syncthreads();
+    // ---- forward 2-D DFT of z (rows, then columns) ----
+    any_dft_lines(buf, n0, n1, n1, 1, tw1, -1.0);
+    any_dft_lines(buf, n1, n0, 1, n1, tw0, -1.0);
+    // ---- F0, F1 from Z(k), Z(-k); energies; cross spectrum ----
+    double e0 = 0.0, e1 = 0.0;
+    const int half = n1 / 2;
+    for (int k = tid; k < nn; k += kAnyThreads) {
+      const int k0 = k / n1, k1 = k % n1;
+      const int q0 = k0 ? n0 - k0 : 0, q1 = k1 ? n1 - k1 : 0;
+      const int km = q0 * n1 + q1;
+      if (km < k) continue;  // done with its partner
+      const float2 a = buf[k], b = buf[km];
+      double f0r = 0.5 * (double(a.x) + double(b.x)), f0i = 0.5 * (double(a.y) - double(b.y));
+      double f1r = 0.5 * (double(a.y) + double(b.y)), f1i = -0.5 * (double(a.x) - double(b.x));
+      if (k == 0) f0r = f0i = f1r = f1i = 0.0;  // fft_im[0, 0] = 0 (correl.py:655,657)
+      const double p0 = f0r * f0r + f0i * f0i, p1 = f1r * f1r + f1i * f1i;
+      // rfft2 half spectrum: columns 0..n1 // 2, the first and last weighted 1
+      // (fft.py:72-90), |F(-k)| = |F(k)|
+      if (k1 <= half) {
+        const double wgt = (k1 == 0 || k1 == half) ? 1.0 : 2.0;
+        e0 += wgt * p0;
+        e1 += wgt * p1;
+      }
+      if (km != k && q1 <= half) {
+        const double wgt = (q1 == 0 || q1 == half) ? 1.0 : 2.0;
+        e0 += wgt * p0;
+        e1 += wgt * p1;
+      }
+      const double xr = f0r * f1r + f0i * f1i, xi = f0r * f1i - f0i * f1r;
+      buf[k] = make_float2(float(xr), float(xi));
+      if (km != k) buf[km] = make_float2(float(xr), float(-xi));
+    }
+    e0 = any_block_sum(e0, red);
+    e1 = any_block_sum(e1, red);
+    norm = sqrt(e0 * e1);
+    __syncthreads();
+    // ---- inverse 2-D DFT (rows, then columns): real map ----
+    any_dft_lines(buf, n0, n1, n1, 1, tw1, +1.0);
+    any_dft_lines(buf, n1, n0, 1, n1, tw0, +1.0);
+    // real parts to floats [0, nn) (chunks read before written: float k
+    // comes from float 2k >= k)
+    for (int k0 = 0; k0 < nn; k0 += kAnyThreads * kAnyChunk) {
+      float v[kAnyChunk];
+#pragma unroll
+      for (int j = 0; j < kAnyChunk; ++j) {
+        const int k = k0 + tid + j * kAnyThreads;
+        v[j] = k < nn ? buf[k].x : 0.f;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kAnyChunk; ++j) {
+        const int k = k0 + tid + j * kAnyThreads;
+        if (k < nn) fl[k] = v[j];
+      }
+      __syncthreads();
+    }
+    // arr[a0, a1] -> out[g(a0), g(a1)], g(a) = (n - 1 - a + n // 2) mod n
+    for (int k = tid; k < nn; k += kAnyThreads) {
+      const int a0 = k / n1, a1 = k % n1;
+      const int g0 = (n0 - 1 - a0 + n0 / 2) % n0, g1 = (n1 - 1 - a1 + n1 / 2) % n1;
+      mapf[g0 * n1 + g1] = fl[k];
+    }
+    __syncthreads();
+    for (int k = tid; k < nn; k += kAnyThreads)
+      srch[k] = (pa.mask != nullptr && pa.mask[k]) ? __builtin_nanf("") : mapf[k];
+    __syncthreads();
+  }
+  // ---- peak search, sub-pixel fit, records (one wavefront) ----
+  if (tid < 64) {
+    int sq[4] = {0, 0, 0, 0};
+    if (compute) {
+      if (tid == 0 && res.norm) res.norm[win] = norm;
+      finish_window<64>(mapf, srch, n0, n1, n0 / 2 - 1, n1 / 2 - 1, pa, tid, win, norm,
+                        apply_subpix, w.dx_in, w.dy_in, res, sq);
+    } else if (tid == 0) {
+      const float nan = __builtin_nanf("");
+      res.dx[win] = nan;
+      res.dy[win] = nan;
+      res.correl_max[win] = nan;
+      res.err[win] = PIV_WIN_BAD_CROP;
+      if (res.norm) res.norm[win] = nan;
+      if (res.nsec) res.nsec[win] = 0;
+    }
+    if (tid == 0)
+      for (int i = 0; i < 4; ++i) sqs[i] = sq[i];
+  }
+  __syncthreads();
+  if (maps != nullptr) {
+    float* out = maps + size_t(win) * nn;
+    const bool side = !pa.has_dmax && (options & PIV_OPT_MAP_SQUARE);
+    for (int k = tid; k < nn; k += kAnyThreads) {
+      float v = compute ? mapf[k] : __builtin_nanf("");
+      if (side) {
+        const int i0 = k / n1, i1 = k % n1;
+        if (i0 >= sqs[0] && i0 < sqs[1] && i1 >= sqs[2] && i1 < sqs[3]) v = __builtin_nanf("");
+      }
+      out[k] = v;
+    }
+  }
+}
+
+// peak search (+ sub-pixel) on caller maps of any shape: one wavefront per
+// map, the map and its masked search copy in LDS
+__global__ __launch_bounds__(64) void k_peaks_generic(float* __restrict__ maps,
+                                                      const double* __restrict__ norms, int n0,
+                                                      int n1, int nwin, int apply_subpix,
+                                                      piv_results res, PlanArgs pa) {
+  extern __shared__ __align__(16) unsigned char any_smem[];
+  const int nn = n0 * n1;
+  float* m = reinterpret_cast<float*>(any_smem);
+  float* srch = m + nn;
+  const int win = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (win >= nwin) return;
+  float* g = maps + size_t(win) * nn;
+  for (int k = lane; k < nn; k += 64) {
+    const float v = g[k];
+    m[k] = v;
+    srch[k] = (pa.mask != nullptr && pa.mask[k]) ? __builtin_nanf("") : v;
+  }
+  __syncthreads();
+  int sq[4];
+  finish_window<64>(m, srch, n0, n1, n0 / 2 - 1, n1 / 2 - 1, pa, lane, win, norms[win],
+                    apply_subpix, 0, 0, res, sq);
+  if (!pa.has_dmax) {
+    for (int k = lane; k < nn; k += 64) {
+      const int i0 = k / n1, i1 = k % n1;
+      if (i0 >= sq[0] && i0 < sq[1] && i1 >= sq[2] && i1 < sq[3]) g[k] = __builtin_nanf("");
+    }
+  }
+}
+
+__global__ void k_subpix_generic(const float* __restrict__ maps, int n0, int n1,
+                                 const int32_t* __restrict__ ix, const int32_t* __restrict__ iy,
+                                 int nwin, float* out_x, float* out_y, int32_t* err,
+                                 PlanArgs pa) {
+  const int win = blockIdx.x * blockDim.x + threadIdx.x;
+  if (win >= nwin) return;
+  double px, py;
+  const int x = ix[win], y = iy[win];
+  const int e = subpix(maps + size_t(win) * n0 * n1, n0, n1, pa.subpix_method, pa.nsubpix, x, y,
+                       px, py);
+  err[win] = e;
+  out_x[win] = e ? float(x) : float(x + px);
+  out_y[win] = e ? float(y) : float(y + py);
 }
 
 // -------------------------------------------------------------------------
@@ -1196,7 +1483,9 @@ const char* piv_version(void) { return "fluidimage_amd piv_hip 0.1 (gfx950)"; }
 
 const char* piv_last_error(void) { return fia::last_error().c_str(); }
 
-int piv_shape_supported(int32_t n0, int32_t n1) { return find_shape(n0, n1) != nullptr; }
+int piv_shape_supported(int32_t n0, int32_t n1) {
+  return find_shape(n0, n1) != nullptr || any_shape_ok(n0, n1);
+}
 
 int piv_tune_candidates(int32_t param, int32_t n0, int32_t n1, int32_t* values, int32_t cap) {
   if (values == nullptr && cap > 0) return -PIV_EINVAL;
@@ -1256,12 +1545,15 @@ int piv_plan_create(piv_plan** plan, int32_t n0, int32_t n1, double displacement
                     int32_t nsubpix, int32_t particle_radius, int32_t nb_peaks_to_search) {
   if (plan == nullptr) return fail(PIV_EINVAL, "plan is NULL");
   *plan = nullptr;
+  // a register-FFT kernel of the shape, else the DFT kernel of any shape
   const Launchers* l = find_shape(n0, n1);
-  if (l == nullptr)
+  if (l == nullptr && !any_shape_ok(n0, n1))
     return fail(PIV_EUNSUPPORTED,
                 "window shape " + std::to_string(n0) + "x" + std::to_string(n1) +
-                    " is not built into libpiv_hip");
-  if (subpix_method < 0 || subpix_method > PIV_SUBPIX_NONE)
+                    " is not supported by libpiv_hip (rows and columns 2 to " +
+                    std::to_string(kAnyMaxLen) + ", at most " + std::to_string(kAnyMaxElems) +
+                    " elements)");
+  if (subpix_method < 0 || subpix_method > PIV_SUBPIX_2D_GAUSSIAN)
     return fail(PIV_EINVAL, "bad subpix method");
   if (nsubpix < 1 || 2 * nsubpix + 1 > (n0 < n1 ? n0 : n1))
     return fail(PIV_EINVAL, "bad nsubpix");
@@ -1360,10 +1652,24 @@ int piv_correlate_batch(piv_plan* plan, int32_t dtype, const piv_image* images,
     return fail(PIV_EINVAL, "NULL image, window or result array");
   if (plan->pa.nb_peaks > 1 && (res->sec == nullptr || res->nsec == nullptr))
     return fail(PIV_EINVAL, "secondary peak arrays required when nb_peaks_to_search > 1");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (plan->l == nullptr) {
+    const size_t lds = any_lds_bytes(plan->n0, plan->n1);
+    const dim3 grid(nwin), block(kAnyThreads);
+    if (dtype == PIV_U8)
+      hipLaunchKernelGGL(k_correlate_any<PIV_U8>, grid, block, lds, s, plan->n0, plan->n1,
+                         images, windows, int(nwin), int(options), maps, *res, plan->pa);
+    else if (dtype == PIV_U16)
+      hipLaunchKernelGGL(k_correlate_any<PIV_U16>, grid, block, lds, s, plan->n0, plan->n1,
+                         images, windows, int(nwin), int(options), maps, *res, plan->pa);
+    else
+      hipLaunchKernelGGL(k_correlate_any<PIV_F32>, grid, block, lds, s, plan->n0, plan->n1,
+                         images, windows, int(nwin), int(options), maps, *res, plan->pa);
+    return check_launch("piv_correlate_batch");
+  }
   const CorrVariant& v = variant_of(plan->l);
   dim3 grid((nwin + v.wpb - 1) / v.wpb), block(v.threads);
-  v.correlate[dtype](grid, block, static_cast<hipStream_t>(stream), images, windows, nwin,
-                     options, maps, *res, plan->pa);
+  v.correlate[dtype](grid, block, s, images, windows, nwin, options, maps, *res, plan->pa);
   return check_launch("piv_correlate_batch");
 }
 
@@ -1377,6 +1683,13 @@ int piv_peaks_from_maps(piv_plan* plan, float* maps, const double* norms, int32_
   if (plan->pa.nb_peaks > 1 && (res->sec == nullptr || res->nsec == nullptr))
     return fail(PIV_EINVAL, "secondary peak arrays required when nb_peaks_to_search > 1");
   const Launchers* l = plan->l;
+  if (l == nullptr) {
+    const size_t lds = size_t(8) * plan->n0 * plan->n1;
+    hipLaunchKernelGGL(k_peaks_generic, dim3(nwin), dim3(64), lds,
+                       static_cast<hipStream_t>(stream), maps, norms, plan->n0, plan->n1,
+                       int(nwin), int(apply_subpix), *res, plan->pa);
+    return check_launch("piv_peaks_from_maps");
+  }
   dim3 grid((nwin + l->pwpb - 1) / l->pwpb), block(l->pthreads);
   l->peaks(grid, block, static_cast<hipStream_t>(stream), maps, norms, nwin, apply_subpix, *res,
            plan->pa);
@@ -1388,6 +1701,12 @@ int piv_subpix_at(piv_plan* plan, const float* maps, const int32_t* ix, const in
   if (plan == nullptr) return fail(PIV_EINVAL, "NULL plan");
   if (nwin <= 0) return PIV_OK;
   if (!maps || !ix || !iy || !out_x || !out_y || !err) return fail(PIV_EINVAL, "NULL argument");
+  if (plan->l == nullptr) {
+    hipLaunchKernelGGL(k_subpix_generic, dim3((nwin + 63) / 64), dim3(64), 0,
+                       static_cast<hipStream_t>(stream), maps, plan->n0, plan->n1, ix, iy,
+                       int(nwin), out_x, out_y, err, plan->pa);
+    return check_launch("piv_subpix_at");
+  }
   plan->l->subpix(dim3((nwin + 63) / 64), dim3(64), static_cast<hipStream_t>(stream), maps, ix,
                   iy, nwin, out_x, out_y, err, plan->pa);
   return check_launch("piv_subpix_at");

@@ -42,6 +42,7 @@ extern "C" {
 #define PIV_SUBPIX_2D_GAUSSIAN2 0
 #define PIV_SUBPIX_CENTROID 1
 #define PIV_SUBPIX_NONE 2
+#define PIV_SUBPIX_2D_GAUSSIAN 3 /* least-squares fit on the (2 nsubpix + 1)^2 crop, subpix.py:157-181 */
 
 /* ---- per-window error codes (strings of correl.py:128,130,
  *      subpix.py:154,215, singlepass.py:310) ---- */
@@ -109,7 +110,10 @@ typedef struct piv_plan piv_plan;
 const char* piv_version(void);
 /* Message of the last failed call on this thread ("" if none). */
 const char* piv_last_error(void);
-/* 1 if the library has a kernel for windows of n0 x n1 (rows x cols). */
+/* 1 if the library correlates windows of n0 x n1 (rows x cols): the shapes
+ * of its register-FFT and LDS-staged kernels, and any other shape of 2 to
+ * 256 rows and columns and at most 16384 elements (the DFT kernel of any
+ * shape; the reference takes any shape through FFTW, fft.py:205-244). */
 int piv_shape_supported(int32_t n0, int32_t n1);
 
 /* ---- launch tuning (no counterpart in the reference, whose FFTW plans

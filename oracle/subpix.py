@@ -35,6 +35,42 @@ def subpix_2d_gaussian2(correl, ix, iy):
     return float(deplx), float(deply)
 
 
+def gaussian_fit_matrix(nsubpix):
+    """The least-squares fit matrix of SubPix.prepare_subpix
+    (src/fluidimage/calcul/subpix.py:86-105): the pseudo-inverse of
+    M = [X^2, Y^2, X, Y, 1] on the (2 n + 1)^2 grid, shape (5, (2 n + 1)^2)."""
+    xs = np.arange(-nsubpix, nsubpix + 1, dtype=float)
+    X, Y = np.meshgrid(xs, xs)
+    X = X.ravel()
+    Y = Y.ravel()
+    M = np.stack([X**2, Y**2, X, Y, np.ones_like(X)], axis=1)
+    return np.linalg.pinv(M)
+
+
+def subpix_2d_gaussian(correl, ix, iy, nsubpix=1):
+    """src/fluidimage/calcul/subpix.py:157-181: the log-quadratic least-squares
+    fit on the (2 n + 1)^2 crop (values <= 0 set to 1e-6 in a copy, float32
+    log, float64 fit). Returns (deplx, deply), or None where the reference
+    falls back to the centroid (a coefficient of a square term > 0, or a NaN
+    displacement)."""
+    crop = np.array(
+        correl[iy - nsubpix : iy + nsubpix + 1, ix - nsubpix : ix + nsubpix + 1],
+        dtype=np.float32,
+    ).ravel()
+    crop[crop <= 0.0] = 1e-6
+    coef = gaussian_fit_matrix(nsubpix) @ np.log(crop).astype(np.float64)
+    if coef[0] > 0 or coef[1] > 0:
+        return None
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sigmax = 1 / np.sqrt(-2 * coef[0])
+        sigmay = 1 / np.sqrt(-2 * coef[1])
+        deplx = coef[2] * sigmax**2
+        deply = coef[3] * sigmay**2
+    if np.isnan(deplx) or np.isnan(deply):
+        return None
+    return float(deplx), float(deply)
+
+
 def subpix_centroid(correl, ix, iy, nsubpix=1):
     """src/fluidimage/calcul/subpix.py:189-198."""
     crop = np.array(
@@ -49,8 +85,8 @@ def subpix_centroid(correl, ix, iy, nsubpix=1):
 
 
 class SubPix:
-    """src/fluidimage/calcul/subpix.py:64-218 (methods 2d_gaussian2,
-    centroid and no_subpix)."""
+    """src/fluidimage/calcul/subpix.py:64-218 (methods 2d_gaussian,
+    2d_gaussian2, centroid and no_subpix)."""
 
     methods = ["2d_gaussian", "2d_gaussian2", "centroid", "no_subpix"]
 
@@ -79,8 +115,11 @@ class SubPix:
         ):
             raise SubPixError("close boundary", 3)
         if method == "2d_gaussian":
-            raise NotImplementedError("2d_gaussian is not part of the PIV path")
-        if method == "2d_gaussian2":
+            d = subpix_2d_gaussian(correl, int(ix), int(iy), nsubpix)
+            if d is None:
+                return self.compute_subpix(correl, ix, iy, method="centroid")
+            deplx, deply = d
+        elif method == "2d_gaussian2":
             deplx, deply = subpix_2d_gaussian2(correl, int(ix), int(iy))
         elif method == "centroid":
             deplx, deply = subpix_centroid(correl, ix, iy, nsubpix)

@@ -72,11 +72,18 @@ def test_struct_layouts():
 def test_plan_validation_without_gpu():
     lib = _native.load()
     assert lib.piv_shape_supported(32, 32) == 1
-    assert lib.piv_shape_supported(33, 32) == 0
+    # shapes without a register-FFT kernel take the DFT kernel of any shape
+    for shape in [(33, 32), (7, 9), (12, 13), (21, 32), (96, 96), (256, 64)]:
+        assert lib.piv_shape_supported(*shape) == 1, shape
+    for shape in [(1, 32), (257, 8), (160, 160)]:
+        assert lib.piv_shape_supported(*shape) == 0, shape
     plan = ctypes.c_void_p()
-    # unsupported shape: status PIV_EUNSUPPORTED with a message
     st = lib.piv_plan_create(ctypes.byref(plan), 33, 33, -1.0, None, None, 0, 1, 3, 1)
-    assert st == 2 and b"not built" in lib.piv_last_error()
+    assert st == 0
+    assert lib.piv_plan_destroy(plan) == 0
+    # unsupported shape: status PIV_EUNSUPPORTED with a message
+    st = lib.piv_plan_create(ctypes.byref(plan), 160, 160, -1.0, None, None, 0, 1, 3, 1)
+    assert st == 2 and b"not supported" in lib.piv_last_error()
     # displacement_max None needs no device memory: the plan can be made
     st = lib.piv_plan_create(ctypes.byref(plan), 32, 32, -1.0, None, None, 0, 1, 3, 1)
     assert st == 0

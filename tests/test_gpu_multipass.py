@@ -84,6 +84,21 @@ def test_multipass_vs_oracle(use_tps):
     check_passes_against_oracle(gpu.passes, ref, ow.works_piv, label=f"use_tps={use_tps}")
 
 
+def test_multipass_window_shapes_without_register_kernel():
+    """A 60 -> 30 px multipass (neither size has a register-FFT kernel: the
+    DFT kernel of any shape serves both passes) against the oracle."""
+    from fluidimage_amd.works.piv.multipass import WorkPIV
+    from oracle.piv import OracleWorkPIV
+
+    print("case: multipass 60/30", flush=True)
+    im0, im1, p, op = _setup("last", number=2, window=60, shape=(384, 416))
+    gpu = WorkPIV(p).calcul_from_arrays(im0, im1)
+    ow = OracleWorkPIV(op, precision="f64")
+    ref = ow.calcul(im0, im1)
+    assert [w.shape_crop_im0 for w in ow.works_piv] == [(60, 60), (30, 30)]
+    check_passes_against_oracle(gpu.passes, ref, ow.works_piv, label="60/30")
+
+
 def test_batch_equals_single_pairs():
     """Batching pairs changes the execution, not the arithmetic."""
     from fluidimage_amd.synthetic import config_pair

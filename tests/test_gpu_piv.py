@@ -87,6 +87,10 @@ def compare_pass(gpu, exp, correl, label):
         dict(config=5, shape=(256, 256), window=32, dmax=None),
         dict(config=2, shape=(704, 640), window=128, dmax=None),
         dict(config=2, shape=(640, 640), window=128, dmax="30%"),
+        # window shapes without a register-FFT kernel
+        dict(config=2, shape=(384, 384), window=36, dmax=None),
+        dict(config=2, shape=(512, 448), window=56, dmax="30%"),
+        dict(config=1, shape=None, window=(30, 40), dmax=8),
     ],
 )
 def test_single_pass_vs_oracle(cfg):
@@ -112,6 +116,59 @@ def test_single_pass_vs_oracle(cfg):
             continue
         ref = exp["correls"][i]
         assert np.abs(maps[i] - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("method,nsubpix", [("2d_gaussian", 1), ("2d_gaussian", 2),
+                                            ("centroid", 2)])
+def test_single_pass_subpix_methods_vs_oracle(method, nsubpix):
+    """The batched pass with the other SubPix methods of the reference
+    (params.piv0.method_subpix / nsubpix, subpix.py:107-218). With a 3 x 3
+    fit the displacements follow the FP64 oracle's within 1e-3 px; the
+    5 x 5 fits take the logarithm of the small map values at the crop's
+    edge, whose float32 rounding moves the fit by more, so there the
+    oracle's search and fit run on the GPU's own maps (the maps themselves
+    are checked against the FP64 oracle in the other tests)."""
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.works.piv.singlepass import FirstWorkPIV
+    from oracle import correl as oc
+    from oracle.piv import OracleWorkPIVPass
+
+    print("case: subpix", method, nsubpix, flush=True)
+    im0, im1 = config_pair(2, shape=(384, 384))
+    p = _params(32, 8)
+    p.piv0.method_subpix = method
+    p.piv0.nsubpix = nsubpix
+    work = FirstWorkPIV(p)
+    work._prepare_with_image(im0)
+    gpu = work._loop_vectors(im0, im1)
+    if nsubpix == 1:
+        ow = OracleWorkPIVPass(_oracle_params(p), 0, precision="f64")
+        ow.prepare_with_image(im0.shape)
+        exp = ow.loop_vectors(im0, im1)
+        compare_pass(gpu, exp, work.correl, f"subpix {method} {nsubpix}")
+        return
+    dx, dy, xs, ys, cmax, maps, errors, sec = gpu
+    o = oc.OracleCorrelFFT((32, 32), displacement_max=8, method_subpix=method,
+                           nsubpix=nsubpix)
+    bad, nerr = [], 0
+    for i, m in enumerate(maps):
+        m = np.array(m, dtype=np.float32)
+        try:
+            ddx, ddy, _, _ = o.compute_displacements_from_correl(m.copy(), norm=1.0)
+            exp = o.apply_subpix(ddx, ddy, m)
+            e_err = None
+        except (oc.PIVError, Exception) as e:  # peak or sub-pixel error
+            exp, e_err = None, getattr(e, "explanation", str(e))
+        g_err = errors.get(i)
+        if e_err is not None or g_err is not None:
+            nerr += 1
+            if e_err != g_err:
+                bad.append((i, "err", g_err, e_err))
+            continue
+        if abs(dx[i] - exp[0]) > 1e-4 or abs(dy[i] - exp[1]) > 1e-4:
+            bad.append((i, "d", dx[i], dy[i], exp))
+    print(f"subpix {method} {nsubpix}: {len(maps)} windows, {nerr} with errors", flush=True)
+    assert not bad, bad[:10]
 
 
 def test_later_pass_offsets_padding_and_bad_crops():

@@ -398,3 +398,74 @@ def test_refine_loss_autograd_matches_central_differences():
                                    torch.tensor(d0[1] - e[1])))
             fd = (lp - lm) / (2 * h)
             assert abs(fd - float(g[k])) <= 1e-4 * max(1.0, abs(fd)), (k, comp, fd, float(g[k]))
+
+
+def test_subpix_gaussian_recovers_an_exact_gaussian():
+    """A sampled Gaussian has an exactly quadratic log: the '2d_gaussian' fit
+    (subpix.py:157-181) returns its centre, for every nsubpix."""
+    rng = np.random.default_rng(21)
+    for n in (1, 2, 3):
+        for _ in range(20):
+            x0, y0 = rng.uniform(-0.45, 0.45, 2)
+            sx, sy = rng.uniform(0.8, 2.5, 2)
+            size = 2 * n + 5
+            c = size // 2
+            yy, xx = np.mgrid[0:size, 0:size].astype(np.float64)
+            m = np.exp(-((xx - c - x0) ** 2) / (2 * sx**2) - ((yy - c - y0) ** 2) / (2 * sy**2))
+            d = osub.subpix_2d_gaussian(m.astype(np.float32), c, c, n)
+            assert d is not None
+            assert d[0] == pytest.approx(x0, abs=2e-5)
+            assert d[1] == pytest.approx(y0, abs=2e-5)
+
+
+def test_subpix_gaussian_pinv_is_the_decoupled_least_squares():
+    """The reference's pseudo-inverse of [X^2, Y^2, X, Y, 1] equals the
+    decoupled normal-equation solution of the symmetric grid (the form the
+    device code evaluates)."""
+    for n in (1, 2, 4):
+        P = osub.gaussian_fit_matrix(n)
+        N = 2 * n + 1
+        xs = np.arange(-n, n + 1, dtype=float)
+        S2 = (xs**2).sum()
+        S4 = (xs**4).sum()
+        X, Y = np.meshgrid(xs, xs)
+        X, Y = X.ravel(), Y.ravel()
+        den = N * S4 - S2**2
+        rows = np.stack([(X**2 - S2 / N) / den, (Y**2 - S2 / N) / den,
+                         X / (N * S2), Y / (N * S2)])
+        assert np.allclose(P[:4], rows, atol=1e-12)
+
+
+def test_subpix_gaussian_fallbacks_and_errors():
+    sp = osub.SubPix("2d_gaussian", nsubpix=1)
+    # a valley (positive curvature): the centroid is taken
+    m = np.full((5, 5), 2.0, np.float32)
+    m[2, 2] = 1.0
+    m[2, 3] = 1.5
+    assert osub.subpix_2d_gaussian(m, 2, 2, 1) is None
+    assert sp.compute_subpix(m, 2, 2) == pytest.approx(
+        tuple(np.array(osub.subpix_centroid(m, 2, 2, 1)) + 2))
+    # values <= 0 enter the fit as 1e-6
+    m = np.zeros((5, 5), np.float32)
+    m[2, 2] = 1.0
+    m[2, 3] = 0.5
+    d = osub.subpix_2d_gaussian(m, 2, 2, 1)
+    l = np.log(np.float32(1e-6))
+    logs = np.full((3, 3), l, np.float64)
+    logs[1, 1] = 0.0
+    logs[1, 2] = np.log(np.float32(0.5))
+    xs = np.array([-1.0, 0.0, 1.0])
+    X, Y = np.meshgrid(xs, xs)
+    a = ((X**2 - 2 / 3) * logs).sum() / 2
+    c = (X * logs).sum() / 6
+    assert d[0] == pytest.approx(c / (-2 * a), rel=1e-12)
+    # close boundary, as every method
+    with pytest.raises(osub.SubPixError, match="close boundary"):
+        sp.compute_subpix(m, 0, 2)
+    # a fit whose displacement exceeds 2 (0.5 + n)^2 is an error (no fallback)
+    yy, xx = np.mgrid[-3:4, -3:4].astype(np.float64)
+    m = np.exp(-0.01 * xx**2 - yy**2 + 0.1 * xx).astype(np.float32)  # vertex at x = 5
+    d = osub.subpix_2d_gaussian(m, 3, 3, 1)
+    assert d is not None and d[0] ** 2 + d[1] ** 2 > 4.5
+    with pytest.raises(osub.SubPixError, match="wrong subpix"):
+        sp.compute_subpix(m, 3, 3)
