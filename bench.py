@@ -17,6 +17,7 @@ batched FP64 TPS per pass, final interpolation); the images are resident in HBM 
 value = final-pass vectors (65,025 per pair) x P per second. The same
 pipeline on one pair (the plain per-pair path) is timed in the same run.
 
+Workload "preproc": the sliding median / minima of 2048x2048 uint16 images.
 Workload "single": the single-pass correlation of the same kind of pairs
 (32x32 windows, displacement_max 8, 16,129 windows per pair). Workload
 "direct": the direct spatial correlation (CorrelPythran) of the 32x32
@@ -60,7 +61,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=32)
     ap.add_argument("--workload", default="multipass",
-                    choices=["multipass", "single", "refine", "direct"])
+                    choices=["multipass", "single", "refine", "direct", "preproc"])
     ap.add_argument("--no-maps", action="store_true")
     ap.add_argument("--window", type=int, default=32,
                     help="window size of the single-pass workload (sizes without a "
@@ -461,6 +462,88 @@ def bench_refine(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_preproc(args):
+    """Sliding-window preprocessing (the reference's python toolbox,
+    _toolbox_py.py:66-165): sliding_median and sliding_minima with the
+    reference's default 30 x 30 window and reflect boundaries on P seeded
+    2048x2048 uint16 images (12-bit particle images), resident in HBM; one
+    launch filters the P images. value = images/s of the median. The CPU
+    baseline is scipy.ndimage (what the reference calls) on a 512x512
+    crop of one image, one thread, scaled to a whole image by the pixel
+    count."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from fluidimage_amd import _native
+    from fluidimage_amd.device import ImageSet, to_device_image
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.works.preproc.toolbox import filter_rank
+
+    npairs = max(1, args.pairs // 2)
+    print(f"bench: generating {2 * npairs} 2048x2048 uint16 images", file=sys.stderr, flush=True)
+    ims = [im for k in range(npairs) for im in config_pair(2, pair_idx=k)]
+    cpu = None
+    if not args.no_cpu_baseline:
+        from threadpoolctl import threadpool_limits
+
+        from oracle import preproc as opp
+
+        crop = ims[0][:512, :512]
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            opp.sliding_median(crop)
+            dt = time.perf_counter() - t0
+        scale = crop.size / ims[0].size
+        cpu = {"value": round(scale / dt, 4), "unit": "images/s", "cores": 1, "kind": "port",
+               "sample": f"sliding_median 30x30 of a 512x512 crop of one image with "
+                         f"scipy.ndimage, {dt:.2f} s on one thread, scaled by the pixel count"}
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    dims = [to_device_image(a, dev) for a in ims]
+    images = ImageSet(dims, subtract_min=False)
+    n = len(dims)
+    h, w = ims[0].shape
+    out = torch.empty((n, h, w), dtype=torch.float64, device=dev)
+    lib = _native.load()
+
+    def run(rank):
+        _native.check(lib.piv_sliding_filter(
+            ctypes.c_void_p(images.table.data_ptr()), n, h, w, images.dtype, 30, 30, rank,
+            _native.PIV_BOUNDARY["reflect"], 0.0, 1.0, ctypes.c_void_p(out.data_ptr()),
+            _native.stream_ptr(None)), "piv_sliding_filter")
+
+    for _ in range(max(1, args.warmup)):
+        run(filter_rank("median", 30, 30))
+    med_ms = relaunch_ms(lambda: run(filter_rank("median", 30, 30)), reps=max(2, args.steps))
+    min_ms = relaunch_ms(lambda: run(0), reps=max(2, args.steps))
+    value = n / (med_ms / 1e3)
+    line = {
+        "metric": "images/s, sliding_median 30x30 (reflect) of 2048x2048 uint16 images",
+        "value": round(value, 1),
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": max(2, args.steps),
+        "warmup": args.warmup,
+        "ms_per_step": round(med_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic",
+        "config": {"workload": f"sliding_median and sliding_minima (30x30, reflect, weight 1) "
+                               f"of {n} 2048x2048 uint16 images in one launch each",
+                   "images": n,
+                   "median_ms_per_launch": round(med_ms, 3),
+                   "minima_ms_per_launch": round(min_ms, 3),
+                   "minima_images/s": round(n / (min_ms / 1e3), 1)},
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
 def bench_direct(args):
     """Direct (spatial) correlation, the reference's CorrelPythran: the 32x32
     windows (50 % overlap, 16,129 per pair) of P seeded 2048x2048 uint16
@@ -576,11 +659,12 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    if args.workload in ("refine", "direct"):
+    if args.workload in ("refine", "direct", "preproc"):
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
-        (bench_refine if args.workload == "refine" else bench_direct)(args)
+        {"refine": bench_refine, "direct": bench_direct,
+         "preproc": bench_preproc}[args.workload](args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "

@@ -23,6 +23,8 @@ PIV_WINFLAG_INVALID = 1
 PIV_OPT_SUBPIX = 1
 PIV_OPT_MAP_SQUARE = 2
 PIV_DIRECT_PYTHRAN, PIV_DIRECT_PYCUDA = 0, 1
+# scipy.ndimage boundary modes of the sliding filters
+PIV_BOUNDARY = {"reflect": 0, "constant": 1, "nearest": 2, "mirror": 3, "wrap": 4}
 
 WIN_OK = 0
 WIN_PEAK_BOUNDARY = 1
@@ -127,6 +129,9 @@ SIGNATURES = {
     "piv_graph_launch": (ctypes.c_int, [_P, _P]),
     "piv_graph_destroy": (ctypes.c_int, [_P]),
     "piv_images_min": (ctypes.c_int, [_P, _I, _P, _I, _P]),
+    "piv_sliding_filter": (
+        ctypes.c_int,
+        [_P, _I, _I, _I, _I, _I, _I, _I, _I, ctypes.c_double, ctypes.c_double, _P, _P]),
     "piv_correlate_batch": (
         ctypes.c_int,
         [_P, _I, _P, _P, _I, _I, _P, ctypes.POINTER(PivResults), _P],

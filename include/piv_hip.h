@@ -595,6 +595,28 @@ int piv_refine_backward(int32_t dtype, const piv_image* images, int32_t npairs,
                         const double* grad_out, double* gdx, double* gdy,
                         void* stream);
 
+/* ---- sliding-window preprocessing, replacing the python toolbox's
+ *      sliding_minima / sliding_median / sliding_percentile
+ *      (src/fluidimage/works/preproc/_toolbox_py.py:66-165) ---- */
+/* scipy.ndimage boundary modes (boundary_condition) */
+#define PIV_BOUNDARY_REFLECT 0 /* d c b a | a b c d | d c b a */
+#define PIV_BOUNDARY_CONSTANT 1 /* cval outside (cast to the image type) */
+#define PIV_BOUNDARY_NEAREST 2 /* a a a | a b c d | d d d */
+#define PIV_BOUNDARY_MIRROR 3 /* d c b | a b c d | c b a */
+#define PIV_BOUNDARY_WRAP 4   /* b c d | a b c d | a b c */
+/* out[i] = max(img_i - weight * F(img_i), 0) for the `nimages` images of
+ * the device table `images` (all height x width, element type `dtype`; the
+ * minval fields are not used), F the rank-th smallest value of the
+ * wy x wx box around each pixel (offsets -(w / 2) .. w - 1 - w / 2, as
+ * scipy.ndimage with origin 0): rank 0 is minimum_filter, rank
+ * wy * wx / 2 median_filter, int(wy * wx * percentile / 100)
+ * percentile_filter. `out` is nimages x height x width, float64 for uint8 and
+ * uint16 images and float32 for float32 images, the types and roundings of
+ * the reference's numpy expression. Windows of 1 to 128 rows and columns. */
+int piv_sliding_filter(const piv_image* images, int32_t nimages, int32_t height,
+                       int32_t width, int32_t dtype, int32_t wy, int32_t wx, int32_t rank,
+                       int32_t mode, double cval, double weight, void* out, void* stream);
+
 /* ---- capture and replay as one HIP graph (no reference counterpart: the
  *      reference's per-window loop, src/fluidimage/works/piv/singlepass.py:
  *      262-361, pays its host cost per window; a replayed graph pays the
