@@ -40,6 +40,7 @@ import os
 import re
 import sys
 import time
+from pathlib import Path
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -61,7 +62,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=32)
     ap.add_argument("--workload", default="multipass",
-                    choices=["multipass", "single", "refine", "direct", "preproc"])
+                    choices=["multipass", "single", "refine", "direct", "preproc",
+                             "topology"])
     ap.add_argument("--no-maps", action="store_true")
     ap.add_argument("--window", type=int, default=32,
                     help="window size of the single-pass workload (sizes without a "
@@ -544,6 +546,81 @@ def bench_preproc(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_topology(args):
+    """The GPU-aware TopologyPIV over a series of 2 P seeded 2048x2048
+    uint16 TIFF files on the local disk (P couples 2k, 2k + 1: str_subset
+    "i:i+2", ind_step 2), the multipass
+    workload's parameters, batches of 16 couples: images read by a thread
+    pool while the GPU computes, results saved by another thread. value =
+    final vectors of the series / wall time of compute() (file reading,
+    upload, GPU, download and saving included); one warm-up run on a
+    separate series first."""
+    import shutil
+    import tempfile
+
+    from PIL import Image
+
+    from fluidimage_amd.synthetic import config_pair
+    from fluidimage_amd.topologies.piv import TopologyPIV
+
+    npairs = max(2, args.pairs)
+    root = Path(tempfile.mkdtemp(prefix="piv_topo_"))
+    try:
+        print(f"bench: writing {2 * npairs} 2048x2048 uint16 TIFF files", file=sys.stderr,
+              flush=True)
+        for name, n in (("warm", 4), ("Images", 2 * npairs)):
+            d = root / name
+            d.mkdir()
+            for i in range(n):
+                im = config_pair(3, pair_idx=i // 2)[i % 2]
+                Image.fromarray(im).save(d / f"im_{i:04d}.tif")
+
+        def topo(name):
+            p = TopologyPIV.create_default_params()
+            mp = multipass_params()
+            for sect in ("piv0", "multipass"):
+                for k, v in getattr(mp, sect)._as_dict().items():
+                    if not isinstance(v, dict):
+                        setattr(getattr(p, sect), k, v)
+            p.series.path = str(root / name)
+            p.series.ind_step = 2
+            p.saving.how = "recompute"
+            p.gpu.batch_pairs = 16
+            return TopologyPIV(p, logging_level="warning")
+
+        topo("warm").compute()
+        t = topo("Images")
+        t0 = time.perf_counter()
+        t.compute()
+        wall = time.perf_counter() - t0
+        nvec = 65025
+        line = {
+            "metric": "vectors/s, TopologyPIV over a series of 2048x2048 uint16 TIFF files "
+                      "(multipass, read and save included)",
+            "value": round(nvec * len(t.results) / wall, 1),
+            "unit": "vectors/s",
+            "n_gpus": 1,
+            "steps": 1,
+            "warmup": 1,
+            "ms_per_step": round(wall * 1e3, 1),
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"{len(t.results)} couples of a series of {2 * npairs} "
+                                   "files, batches of 16, 4 reading threads",
+                       "couples": len(t.results),
+                       "ms_per_couple": round(wall * 1e3 / max(1, len(t.results)), 2),
+                       "read_s_summed_over_threads": round(sum(t.durations["read"]), 3),
+                       "compute_s": round(sum(t.durations["compute_piv"]), 3),
+                       "save_s": round(sum(t.durations["save_piv"]), 3)},
+        }
+        print(json.dumps(line), flush=True)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+
+
 def bench_direct(args):
     """Direct (spatial) correlation, the reference's CorrelPythran: the 32x32
     windows (50 % overlap, 16,129 per pair) of P seeded 2048x2048 uint16
@@ -659,12 +736,12 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    if args.workload in ("refine", "direct", "preproc"):
+    if args.workload in ("refine", "direct", "preproc", "topology"):
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
-        {"refine": bench_refine, "direct": bench_direct,
-         "preproc": bench_preproc}[args.workload](args)
+        {"refine": bench_refine, "direct": bench_direct, "preproc": bench_preproc,
+         "topology": bench_topology}[args.workload](args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "

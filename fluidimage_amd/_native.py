@@ -225,6 +225,13 @@ def load(path=None):
             f"{p} not found: build it with `python -m fluidimage_amd.build` "
             "(the HIP library is required; there is no CPU path)"
         )
+    # torch first: its HIP runtime (torch/lib/libamdhip64.so, SONAME
+    # libamdhip64.so.7) must be the process's. Loaded after this library,
+    # which needs libamdhip64.so.7, torch would bring a second runtime
+    # (it asks for libamdhip64.so by another name) and one of the two would
+    # find no device.
+    import torch  # noqa: F401
+
     try:
         lib = ctypes.CDLL(str(p))
     except OSError as exc:
