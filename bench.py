@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--pairs", type=int, default=32)
     ap.add_argument("--workload", default="multipass",
                     choices=["multipass", "single", "refine", "direct", "preproc",
-                             "topology"])
+                             "topology", "uneven"])
     ap.add_argument("--no-maps", action="store_true")
     ap.add_argument("--window", type=int, default=32,
                     help="window size of the single-pass workload (sizes without a "
@@ -621,6 +621,93 @@ def bench_topology(args):
         shutil.rmtree(root, ignore_errors=True)
 
 
+def bench_uneven(args):
+    """The uneven batch: P seeded pairs of sizes drawn from 1024x1024,
+    1600x1200, 2048x2048, 2560x2160 (uint16) with window schedules drawn
+    from 128/64/32, 64/32/16, 48/24 and Oseen + shear fields of 3-12 px,
+    the multipass settings of the default workload otherwise (TPS between
+    passes and at the end, bicubic deformation). The pairs are bucketed by
+    (shape, schedule), each bucket one batched multipass, all buckets
+    captured in one HIP graph; a step = one replay (with the host read of the
+    TPS checks). Beside it: the same buckets run eagerly one after the other,
+    and the pairs one at a time (the plain path). value = final vectors of
+    all pairs per second."""
+    import torch
+
+    from fluidimage_amd.synthetic import uneven_batch
+    from fluidimage_amd.works.piv.pipeline import UnevenGraph
+
+    npairs = max(1, args.pairs)
+    print(f"bench: generating {npairs} uneven pairs", file=sys.stderr, flush=True)
+    items = uneven_batch(npairs)
+    params = multipass_params()
+    torch.cuda.set_device(0)
+    print("bench: capturing the uneven batch", file=sys.stderr, flush=True)
+    g = UnevenGraph(items, params, keep_maps=not args.no_maps)
+    nvec = sum(work.works_piv[-1].nb_vectors * len(idx) for work, _, idx in g.groups)
+
+    def graph_step(_timers):
+        g.replay()
+
+    wall, _ = timed(graph_step, args.steps, args.warmup)
+    ms_graph = wall / args.steps * 1e3
+
+    def eager_step(_timers):
+        for (work, images, idx), pipe in zip(g.groups, g.pipes):
+            images.recompute_min()
+            pipe.run(images, len(idx), keep_maps=not args.no_maps)
+
+    wall, _ = timed(eager_step, max(2, args.steps // 2), 1)
+    ms_eager = wall / max(2, args.steps // 2) * 1e3
+    plain_ms = None
+    if not args.no_plain:
+        from fluidimage_amd.works.piv.pipeline import MultipassPipeline
+
+        singles = []
+        for work, images, idx in g.groups:
+            for j in range(len(idx)):
+                from fluidimage_amd.device import ImageSet
+
+                one = ImageSet(images.images[2 * j:2 * j + 2])
+                singles.append((MultipassPipeline(work), one))
+
+        def plain_step(_timers):
+            for pipe, one in singles:
+                one.recompute_min()
+                pipe.run(one, 1, keep_maps=not args.no_maps)
+
+        wall, _ = timed(plain_step, 2, 1)
+        plain_ms = wall / 2 * 1e3
+    buckets = {f"{tuple(work.works_piv[0].imshape0)} {tuple(w.shape_crop_im0[0] for w in work.works_piv)}":
+               len(idx) for work, _, idx in g.groups}
+    line = {
+        "metric": "vectors/s, uneven batch of 2048-class pairs (mixed sizes and window "
+                  "schedules) in one captured HIP graph",
+        "value": round(nvec / (ms_graph / 1e3), 1),
+        "unit": "vectors/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_graph, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{npairs} pairs of 1024x1024 to 2560x2160 uint16, schedules "
+                               "128/64/32, 64/32/16, 48/24, TPS between passes and at the end, "
+                               "bicubic deformation, maps " +
+                               ("off" if args.no_maps else "materialised"),
+                   "pairs": npairs, "final_vectors": int(nvec), "buckets": buckets,
+                   "eager_ms_per_step": round(ms_eager, 3),
+                   "graph_over_eager": round(ms_eager / ms_graph, 3)},
+    }
+    if plain_ms is not None:
+        line["config"]["plain_per_pair_ms_per_step"] = round(plain_ms, 3)
+        line["config"]["batched_over_plain"] = round(plain_ms / ms_graph, 2)
+    print(json.dumps(line), flush=True)
+
+
 def bench_direct(args):
     """Direct (spatial) correlation, the reference's CorrelPythran: the 32x32
     windows (50 % overlap, 16,129 per pair) of P seeded 2048x2048 uint16
@@ -736,12 +823,12 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    if args.workload in ("refine", "direct", "preproc", "topology"):
+    if args.workload in ("refine", "direct", "preproc", "topology", "uneven"):
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
         {"refine": bench_refine, "direct": bench_direct, "preproc": bench_preproc,
-         "topology": bench_topology}[args.workload](args)
+         "topology": bench_topology, "uneven": bench_uneven}[args.workload](args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "

@@ -2029,6 +2029,17 @@ __global__ void k_tps_at_centers(int n, const double* __restrict__ info,
 // lambda = -H^-1 y_w[R] with y = M_F^-1 [v'; 0], and (w, a) = y + G lambda.
 // The outlier iterations of the reference change v only, so G and H are
 // computed once per subdomain.
+// Rank of workgroup b among nb when the 8 XCDs of the chip (each with its
+// own L2) take workgroups round-robin (b to XCD b % 8): XCD x gets the
+// contiguous ranks [x q + min(x, r), ...), q = nb / 8, r = nb % 8. A
+// bijection of [0, nb).
+__device__ __forceinline__ int xcd_rank(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int x = b % kXcd, j = b / kXcd;
+  const int q = nb / kXcd, r = nb % kXcd;
+  return x * q + min(x, r) + j;
+}
+
 constexpr int kCapMaxR = 12;     // missing nodes of a subdomain solved with a shared factor
 constexpr int kPatMaxPerField = 64;
 
@@ -2180,43 +2191,207 @@ __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
   }
   for (int b = bd + tid; b < nd; b += 1024) order_direct[b] = -1;
   for (int b = bs + tid; b < nd; b += 1024) order_shared[b] = -1;
+  if (tid == 0) order_shared[nd] = bs;
 }
 
-// Solve of a subdomain with its pattern's factorisation and the capacitance
-// correction of its missing nodes, with the reference's outlier iterations
-// (thin_plate_spline_subdom.py:236-280, as k_tps_solve); writes what
-// k_tps_solve and k_tps_prep write for it (smoothed values and weights by
-// slot, the polynomial part in image coordinates, stats, slot_xy). The
-// first pass over the factor solves the two value vectors with up to 6
-// columns of G, a second one the next 6: at most kCapMaxR = 12 missing
-// nodes; then one pass per outlier iteration.
+// Solves of the subdomains that share their pattern's factorisation, with
+// the capacitance correction of their missing nodes and the reference's
+// outlier iterations (thin_plate_spline_subdom.py:236-280, as k_tps_solve),
+// in two kernels:
+//
+//   k_tps_shared_columns: checks every shared subdomain (pattern factor
+//     regular, centers on the grid, at most cap_r missing nodes) and, when
+//     nodes are missing, computes the columns G = M_F^-1 E_R (up to
+//     kSharedNV right-hand sides per pass over the factor), their
+//     polynomial parts and the Cholesky factor of H = G_w[R, :];
+//   k_tps_shared_iter: the outlier iterations, two right-hand sides per pass
+//     over the factor, in a fifth of the LDS of the first kernel so that
+//     five workgroups share a CU (the passes are bound by the latency of
+//     the triangular solves' chain of block rows, so more of them in flight
+//     is what speeds them up); it writes what k_tps_solve and k_tps_prep
+//     write for a subdomain (smoothed values and weights by slot, the
+//     polynomial part in image coordinates, stats, slot_xy).
+//
+// Both map workgroup b to the shared order by xcd_rank (order[gridDim.x]:
+// the number of shared subdomains).
 constexpr int kSharedNV = 8;
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_tps_solve_shared(
-    const int32_t* __restrict__ order, const int32_t* __restrict__ dpat,
-    const int32_t* __restrict__ dbox, const int32_t* __restrict__ pat_n,
-    const int64_t* __restrict__ pdom_off, const int64_t* __restrict__ pmat_off,
-    const double* __restrict__ pmats, const double* __restrict__ prefl,
-    const double* __restrict__ pvwork, const int32_t* __restrict__ pstatus,
-    const double* __restrict__ gx, const double* __restrict__ gy, int ngx,
-    const double* __restrict__ xs, const double* __restrict__ ys,
-    const int32_t* __restrict__ idx, const float* __restrict__ vx,
-    const float* __restrict__ vy, const int64_t* __restrict__ dom_off,
-    const int32_t* __restrict__ slot_pt, const double* __restrict__ smoothing_d,
-    double threshold, double* __restrict__ smooth_slots, int64_t nslots,
-    double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
-    int32_t* __restrict__ status, double* __restrict__ slot_xy, double* __restrict__ gmat,
-    int cap_r, int nm) {
-  const int d = order[blockIdx.x];
+
+// LDS scratch of chol_solve_n<NV> (doubles): the diagonal block, the block
+// right-hand sides and the partial sums of NV vectors
+constexpr int solve_scratch(int nv) { return kNB * (kNB + 1) + 8 * kNB + nv * 8 * kNB; }
+
+// members of subdomain d on its box: fidx[j] (node of member j), fmem[k]
+// (member of node k, -1 for a missing node); false when a center is off the
+// grid. Every thread of the block calls it (one barrier inside).
+__device__ bool shared_members(int n, int nf, int64_t s0, const int32_t* slot_pt,
+                               const int32_t* idx, const double* xs, const double* ys,
+                               const double* gx, const double* gy, int ngx, int j0, int nc,
+                               int i0, int* fidx, int* fmem, int* bad) {
+  for (int k = threadIdx.x; k < nf; k += kThreads) fmem[k] = -1;
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += kThreads) {
+    const int c = slot_pt[s0 + j];
+    const int q = idx[c];
+    const int row = q / ngx, col = q % ngx;
+    const int k = (row - i0) * nc + (col - j0);
+    const bool on = col - j0 >= 0 && col - j0 < nc && k >= 0 && k < nf && xs[c] == gx[col] &&
+                    ys[c] == gy[row];
+    if (!on) {
+      *bad = 1;
+      continue;
+    }
+    fidx[j] = k;
+    fmem[k] = j;
+  }
+  __syncthreads();
+  return *bad == 0;
+}
+
+// the missing nodes in node order (at most kCapMaxR listed); their number
+__device__ int shared_missing(int nf, const int* fmem, int cap_r, int* rlist, int* nr_sh) {
+  if (threadIdx.x == 0) {
+    int r = 0;
+    for (int k = 0; k < nf && r <= cap_r; ++k)
+      if (fmem[k] < 0) {
+        if (r < kCapMaxR) rlist[r] = k;
+        ++r;
+      }
+    *nr_sh = r;
+  }
+  __syncthreads();
+  return *nr_sh;
+}
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void
+k_tps_shared_columns(const int32_t* __restrict__ order, const int32_t* __restrict__ dpat,
+                     const int32_t* __restrict__ dbox, const int32_t* __restrict__ pat_n,
+                     const int64_t* __restrict__ pdom_off, const int64_t* __restrict__ pmat_off,
+                     const double* __restrict__ pmats, const double* __restrict__ prefl,
+                     const double* __restrict__ pvwork, const int32_t* __restrict__ pstatus,
+                     const double* __restrict__ gx, const double* __restrict__ gy, int ngx,
+                     const double* __restrict__ xs, const double* __restrict__ ys,
+                     const int32_t* __restrict__ idx, const int64_t* __restrict__ dom_off,
+                     const int32_t* __restrict__ slot_pt, int32_t* __restrict__ status,
+                     double* __restrict__ gmat, double* __restrict__ hfac,
+                     double* __restrict__ gpoly, int cap_r, int nm) {
+  const int nvalid = order[gridDim.x];
+  if (int(blockIdx.x) >= nvalid) return;
+  const int d = order[xcd_rank(blockIdx.x, nvalid)];
   if (d < 0) return;
   const int p = dpat[d];
-  const int64_t s0 = dom_off[d];
-  const int n = int(dom_off[d + 1] - s0);
-  const int nf = pat_n[p];
   const int tid = threadIdx.x;
   if (pstatus[p]) {
     if (tid == 0) status[d] = 1;
     return;
   }
+  const int64_t s0 = dom_off[d];
+  const int n = int(dom_off[d + 1] - s0);
+  const int nf = pat_n[p];
+  const int j0 = dbox[4 * d], nc = dbox[4 * d + 1], i0 = dbox[4 * d + 2];
+  const int64_t ps0 = pdom_off[p];
+  const double* E = pmats + pmat_off[p];
+  const int m = nf - 3;
+  const int64_t ld = tps_mat_ld(m);
+  const double* A = E + tps_mat_e_size(m);
+  const double* RF = prefl + 16 * p;
+  const double* VW = pvwork + 6 * ps0;
+  const double tau[3] = {RF[0], RF[1], RF[2]};
+  // dynamic LDS: x[kSharedNV][nm] (doubles), fidx[nm], fmem[nm], then the
+  // solve's scratch
+  extern __shared__ double dyn[];
+  double* x = dyn;
+  int* fidx = reinterpret_cast<int*>(dyn + kSharedNV * nm);
+  int* fmem = fidx + nm;
+  double* scr = dyn + (kSharedNV + 1) * nm;
+  __shared__ double red[kSharedNV * (kThreads / 64)];
+  __shared__ int rlist[kCapMaxR];
+  __shared__ int nr_sh;
+  __shared__ double H[kCapMaxR][kCapMaxR + 1];
+  __shared__ double a8[kSharedNV][3];
+  __shared__ int bad;
+  if (tid == 0) bad = 0;
+  if (!shared_members(n, nf, s0, slot_pt, idx, xs, ys, gx, gy, ngx, j0, nc, i0, fidx, fmem,
+                      &bad)) {
+    if (tid == 0) status[d] = 2;  // a center off the grid: solved again by host sizing
+    return;
+  }
+  const int nr = shared_missing(nf, fmem, cap_r, rlist, &nr_sh);
+  if (nr > cap_r || nr > kCapMaxR || n + nr != nf) {
+    if (tid == 0) status[d] = 2;
+    return;
+  }
+  if (nr == 0) return;
+  // G columns g0 .. g0 + kSharedNV - 1 per pass over the factor
+  double* G = gmat + int64_t(d) * nm * cap_r;  // columns of G_w, nm apart
+  double* GA = gpoly + int64_t(d) * kCapMaxR * 3;
+  for (int g0 = 0; g0 < nr; g0 += kSharedNV) {
+    for (int k = tid; k < nf; k += kThreads)
+#pragma unroll
+      for (int q = 0; q < kSharedNV; ++q)
+        x[q * nm + k] = (g0 + q < nr && k == rlist[g0 + q]) ? 1.0 : 0.0;
+    __syncthreads();
+    saddle_solve_n<kSharedNV>(x, nm, nf, VW, tau, A, ld, E, RF, red, scr, a8);
+    for (int q = 0; q < kSharedNV; ++q) {
+      const int r = g0 + q;
+      if (r >= nr) break;
+      for (int k = tid; k < nf; k += kThreads) G[int64_t(r) * nm + k] = x[q * nm + k];
+      if (tid < nr) H[tid][r] = x[q * nm + rlist[tid]];
+      if (tid < 3) GA[3 * r + tid] = a8[q][tid];
+    }
+    __syncthreads();
+  }
+  // H = L_H L_H^T in place (lower triangle), one thread
+  if (tid == 0) {
+    int ok = 1;
+    for (int j = 0; j < nr && ok; ++j) {
+      double dj = H[j][j];
+      for (int k = 0; k < j; ++k) dj -= H[j][k] * H[j][k];
+      if (!(dj > 0.0)) {
+        ok = 0;
+        break;
+      }
+      dj = sqrt(dj);
+      H[j][j] = dj;
+      for (int i = j + 1; i < nr; ++i) {
+        double v = H[i][j];
+        for (int k = 0; k < j; ++k) v -= H[i][k] * H[j][k];
+        H[i][j] = v / dj;
+      }
+    }
+    if (!ok) status[d] = 1;  // M_S singular, as the direct factorisation reports it
+  }
+  __syncthreads();
+  double* HF = hfac + int64_t(d) * kCapMaxR * kCapMaxR;
+  for (int e = tid; e < nr * nr; e += kThreads) HF[e] = H[e / nr][e % nr];
+}
+
+constexpr int kIterNV = 2;
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void
+k_tps_shared_iter(const int32_t* __restrict__ order, const int32_t* __restrict__ dpat,
+                  const int32_t* __restrict__ dbox, const int32_t* __restrict__ pat_n,
+                  const int64_t* __restrict__ pdom_off, const int64_t* __restrict__ pmat_off,
+                  const double* __restrict__ pmats, const double* __restrict__ prefl,
+                  const double* __restrict__ pvwork, const double* __restrict__ gx,
+                  const double* __restrict__ gy, int ngx, const double* __restrict__ xs,
+                  const double* __restrict__ ys, const int32_t* __restrict__ idx,
+                  const float* __restrict__ vx, const float* __restrict__ vy,
+                  const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
+                  const double* __restrict__ smoothing_d, double threshold,
+                  double* __restrict__ smooth_slots, int64_t nslots, double* __restrict__ weights,
+                  int64_t wstride, double* __restrict__ stats, const int32_t* __restrict__ status,
+                  double* __restrict__ slot_xy, const double* __restrict__ gmat,
+                  const double* __restrict__ hfac, const double* __restrict__ gpoly, int cap_r,
+                  int nm) {
+  const int nvalid = order[gridDim.x];
+  if (int(blockIdx.x) >= nvalid) return;
+  const int d = order[xcd_rank(blockIdx.x, nvalid)];
+  if (d < 0 || status[d]) return;
+  const int p = dpat[d];
+  const int64_t s0 = dom_off[d];
+  const int n = int(dom_off[d + 1] - s0);
+  const int nf = pat_n[p];
+  const int tid = threadIdx.x;
   const int j0 = dbox[4 * d], nc = dbox[4 * d + 1], i0 = dbox[4 * d + 2];
   const int64_t ps0 = pdom_off[p];
   const int m = nf - 3;
@@ -2228,143 +2403,54 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   const double tau[3] = {RF[0], RF[1], RF[2]};
   const double smoothing = smoothing_d[d];
   const double x0 = gx[j0], y0 = gy[i0];
-  double* G = gmat + int64_t(d) * nm * cap_r;  // columns of G_w, nm apart
-  // dynamic LDS: x[kSharedNV][nm] (doubles), val[2][nm] (floats), fidx[nm] (node of a
-  // member), fmem[nm] (member of a node, -1 for a missing node), then the
-  // solve's scratch
+  const double* G = gmat + int64_t(d) * nm * cap_r;
+  // dynamic LDS: x[2][nm] (doubles), val[2][nm] (floats), fidx[nm],
+  // fmem[nm], then the solve's scratch
   extern __shared__ double dyn[];
   double* x = dyn;
-  float* val[2] = {reinterpret_cast<float*>(dyn + kSharedNV * nm),
-                   reinterpret_cast<float*>(dyn + kSharedNV * nm) + nm};
-  int* fidx = reinterpret_cast<int*>(dyn + (kSharedNV + 1) * nm);
+  float* val[2] = {reinterpret_cast<float*>(dyn + kIterNV * nm),
+                   reinterpret_cast<float*>(dyn + kIterNV * nm) + nm};
+  int* fidx = reinterpret_cast<int*>(dyn + (kIterNV + 1) * nm);
   int* fmem = fidx + nm;
-  double* scr = dyn + (kSharedNV + 2) * nm;
-  __shared__ double red[kSharedNV * (kThreads / 64)];
+  double* scr = dyn + (kIterNV + 2) * nm;
+  __shared__ double red[kIterNV * (kThreads / 64)];
   __shared__ int rlist[kCapMaxR];
   __shared__ int nr_sh;
   __shared__ double H[kCapMaxR][kCapMaxR + 1];
   __shared__ double ga[kCapMaxR][3];
   __shared__ double lam[2][kCapMaxR];
-  __shared__ double a8[kSharedNV][3];  // polynomial terms of the solved vectors
+  __shared__ double a2[kIterNV][3];  // polynomial terms of the solved vectors
   __shared__ int go[2];
   __shared__ int bad;
-  if (tid == 0) {
-    bad = 0;
-    nr_sh = 0;
-  }
-  for (int k = tid; k < nf; k += kThreads) fmem[k] = -1;
-  __syncthreads();
-  // members: their node in the box, their values, their coordinates by slot
+  if (tid == 0) bad = 0;
+  // the members (the first kernel checked them); values; coordinates by slot
+  shared_members(n, nf, s0, slot_pt, idx, xs, ys, gx, gy, ngx, j0, nc, i0, fidx, fmem, &bad);
   for (int j = tid; j < n; j += kThreads) {
     const int c = slot_pt[s0 + j];
-    const int q = idx[c];
-    const int row = q / ngx, col = q % ngx;
-    const int k = (row - i0) * nc + (col - j0);
-    const bool on = col - j0 >= 0 && col - j0 < nc && k >= 0 && k < nf && xs[c] == gx[col] &&
-                    ys[c] == gy[row];
-    if (!on) {
-      bad = 1;  // a center off the grid: the field is solved again by host sizing
-      continue;
-    }
-    fidx[j] = k;
-    fmem[k] = j;
     val[0][j] = vx[c];
     val[1][j] = vy[c];
     slot_xy[s0 + j] = xs[c];
     slot_xy[nslots + s0 + j] = ys[c];
   }
-  __syncthreads();
-  if (bad) {
-    if (tid == 0) status[d] = 2;
-    return;
-  }
-  // the missing nodes, in node order
-  if (tid == 0) {
-    int r = 0;
-    for (int k = 0; k < nf && r <= cap_r; ++k)
-      if (fmem[k] < 0) {
-        if (r < kCapMaxR) rlist[r] = k;
-        ++r;
-      }
-    nr_sh = r;
-  }
-  __syncthreads();
-  const int nr = nr_sh;
-  if (nr > cap_r || nr > kCapMaxR || n + nr != nf) {
-    if (tid == 0) status[d] = 2;
-    return;
-  }
-  // pass 0: y = M_F^-1 [v'; 0] (vectors 0, 1) with G columns 0..5 (vectors
-  // 2..7); pass 1: G columns 6..11 (vectors 2..7)
-  for (int pass = 0; pass < 2; ++pass) {
-    const int g0 = 6 * pass;
-    if (pass == 1 && nr <= 6) break;
-    for (int k = tid; k < nf; k += kThreads) {
-      if (pass == 0) {
-        const int j = fmem[k];
-        x[k] = j >= 0 ? double(val[0][j]) : 0.0;
-        x[nm + k] = j >= 0 ? double(val[1][j]) : 0.0;
-      }
-#pragma unroll
-      for (int q = 2; q < kSharedNV; ++q)
-        x[q * nm + k] = (g0 + q - 2 < nr && k == rlist[g0 + q - 2]) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    if (pass == 0) {
-      saddle_solve_n<kSharedNV>(x, nm, nf, VW, tau, A, ld, E, RF, red, scr, a8);
-    } else {
-      // vectors 0, 1 (y) and a8[0..1] are kept: solve vectors 2..7 only
-      saddle_solve_n<kSharedNV - 2>(x + 2 * nm, nm, nf, VW, tau, A, ld, E, RF, red, scr,
-                                    a8 + 2);
-    }
-    for (int q = 2; q < kSharedNV; ++q) {
-      const int r = g0 + q - 2;
-      if (r >= nr) break;
-      for (int k = tid; k < nf; k += kThreads) G[int64_t(r) * nm + k] = x[q * nm + k];
-      if (tid < nr) H[tid][r] = x[q * nm + rlist[tid]];
-      if (tid == 0)
-        for (int t = 0; t < 3; ++t) ga[r][t] = a8[q][t];
-    }
-    __syncthreads();
-  }
-  // H = L_H L_H^T in place (lower triangle), one thread
-  if (tid == 0) {
-    for (int j = 0; j < nr; ++j) {
-      double dj = H[j][j];
-      for (int k = 0; k < j; ++k) dj -= H[j][k] * H[j][k];
-      if (!(dj > 0.0)) {
-        bad = 1;
-        break;
-      }
-      dj = sqrt(dj);
-      H[j][j] = dj;
-      for (int i = j + 1; i < nr; ++i) {
-        double v = H[i][j];
-        for (int k = 0; k < j; ++k) v -= H[i][k] * H[j][k];
-        H[i][j] = v / dj;
-      }
-    }
-  }
-  __syncthreads();
-  if (bad) {
-    if (tid == 0) status[d] = 1;  // M_S singular, as the direct factorisation reports it
-    return;
-  }
+  const int nr = shared_missing(nf, fmem, cap_r, rlist, &nr_sh);
+  const double* HF = hfac + int64_t(d) * kCapMaxR * kCapMaxR;
+  const double* GA = gpoly + int64_t(d) * kCapMaxR * 3;
+  for (int e = tid; e < nr * nr; e += kThreads) H[e / nr][e % nr] = HF[e];
+  for (int e = tid; e < 3 * nr; e += kThreads) ga[e / 3][e % 3] = GA[e];
   if (tid < 2) go[tid] = 1;
   __syncthreads();
   double nfixed[2] = {0.0, 0.0}, maxdiff[2] = {0.0, 0.0};
   int count[2] = {0, 0};
   for (int it = 0; it < 11; ++it) {
     if (!go[0] && !go[1]) break;
-    if (it > 0) {  // y for the updated values (the first one came with G)
-      for (int k = tid; k < nf; k += kThreads) {
-        const int j = fmem[k];
-        x[k] = j >= 0 ? double(val[0][j]) : 0.0;
-        x[nm + k] = j >= 0 ? double(val[1][j]) : 0.0;
-      }
-      __syncthreads();
-      saddle_solve_n<2>(x, nm, nf, VW, tau, A, ld, E, RF, red, scr, a8);
+    // y = M_F^-1 [v'; 0] for the current values (v' = 0 at the missing nodes)
+    for (int k = tid; k < nf; k += kThreads) {
+      const int j = fmem[k];
+      x[k] = j >= 0 ? double(val[0][j]) : 0.0;
+      x[nm + k] = j >= 0 ? double(val[1][j]) : 0.0;
     }
+    __syncthreads();
+    saddle_solve_n<kIterNV>(x, nm, nf, VW, tau, A, ld, E, RF, red, scr, a2);
     // lambda = -H^-1 y_w[R] (two triangular solves with L_H)
     if (tid < 2) {
       const int c = tid;
@@ -2385,7 +2471,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     double wa[2][3];
     for (int c = 0; c < 2; ++c)
       for (int t = 0; t < 3; ++t) {
-        double v = a8[c][t];
+        double v = a2[c][t];
         for (int k = 0; k < nr; ++k) v += ga[k][t] * lam[c][k];
         wa[c][t] = v;
       }
@@ -3016,7 +3102,7 @@ void launch_solve(int32_t nd, int32_t n_max, const float* vx, const float* vy,
 struct SharedWork {
   int32_t *dmode, *dpat, *dbox, *pat_n, *pat_box, *pslot, *pstatus, *order_direct, *order_shared;
   int64_t *pdom_off, *pmat_off;
-  double *pat_smooth, *pxy, *pmats, *prefl, *pvwork, *pslot_xy, *gmat;
+  double *pat_smooth, *pxy, *pmats, *prefl, *pvwork, *pslot_xy, *gmat, *hfac, *gpoly;
   int64_t pxy_stride;
   size_t bytes;
 };
@@ -3037,7 +3123,7 @@ SharedWork shared_work(void* base, int nfields, int dom_cap, int npat_cap, int n
   w.dpat = reinterpret_cast<int32_t*>(take(4 * nd));
   w.dbox = reinterpret_cast<int32_t*>(take(16 * nd));
   w.order_direct = reinterpret_cast<int32_t*>(take(4 * nd));
-  w.order_shared = reinterpret_cast<int32_t*>(take(4 * nd));
+  w.order_shared = reinterpret_cast<int32_t*>(take(4 * (nd + 1)));  // + their count
   w.pat_n = reinterpret_cast<int32_t*>(take(4 * np));
   w.pat_box = reinterpret_cast<int32_t*>(take(16 * np));
   w.pstatus = reinterpret_cast<int32_t*>(take(4 * np));
@@ -3052,6 +3138,8 @@ SharedWork shared_work(void* base, int nfields, int dom_cap, int npat_cap, int n
   w.pvwork = reinterpret_cast<double*>(take(8 * 6 * pts));
   w.pmats = reinterpret_cast<double*>(take(8 * np * size_t(tps_mat_size(n_max))));
   w.gmat = reinterpret_cast<double*>(take(8 * nd * nm * size_t(cap_r > 0 ? cap_r : 1)));
+  w.hfac = reinterpret_cast<double*>(take(8 * nd * kCapMaxR * kCapMaxR));
+  w.gpoly = reinterpret_cast<double*>(take(8 * nd * kCapMaxR * 3));
   w.bytes = o;
   return w;
 }
@@ -3116,7 +3204,8 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   if (npat_cap <= 0 || npat_cap > kPatMaxPerField)
     return tps_fail(PIV_EINVAL, "npat_cap must be in [1, 64]");
   if (cap_r < 0 || cap_r > kCapMaxR) return tps_fail(PIV_EINVAL, "cap_r must be in [0, 12]");
-  if ((size_t(kSharedNV + 2) * ((n_max + 1) & ~1) + kSolveScratch) * sizeof(double) > 120 * 1024)
+  if ((size_t(kSharedNV + 1) * ((n_max + 1) & ~1) + solve_scratch(kSharedNV)) * sizeof(double) >
+      120 * 1024)
     return tps_fail(PIV_EUNSUPPORTED, "n_max too large for the shared solve's LDS");
   if (int64_t(nfields) * dom_cap > INT32_MAX / 4 || int64_t(nfields) * npat_cap > 1 << 20)
     return tps_fail(PIV_EUNSUPPORTED, "TPS batch too large");
@@ -3148,13 +3237,17 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   launch_solve(nd, n_max, vx, vy, dom_off, slot_pt, mat_off, mats, refl, vwork, smoothing,
                threshold, smooth_slots, nslots, weights, wstride, stats, status, w.order_direct,
                s);
-  hipLaunchKernelGGL(k_tps_solve_shared, dim3(nd), dim3(kThreads),
-                     (size_t(kSharedNV + 2) * nm + kSolveScratch) * sizeof(double), s,
-                     w.order_shared, w.dpat, w.dbox,
-                     w.pat_n, w.pdom_off, w.pmat_off, w.pmats, w.prefl, w.pvwork, w.pstatus, gx,
-                     gy, int(ngx), xs, ys, idx, vx, vy, dom_off, slot_pt, smoothing, threshold,
-                     smooth_slots, nslots, weights, wstride, stats, status, slot_xy, w.gmat,
-                     int(cap_r), nm);
+  hipLaunchKernelGGL(k_tps_shared_columns, dim3(nd), dim3(kThreads),
+                     (size_t(kSharedNV + 1) * nm + solve_scratch(kSharedNV)) * sizeof(double), s,
+                     w.order_shared, w.dpat, w.dbox, w.pat_n, w.pdom_off, w.pmat_off, w.pmats,
+                     w.prefl, w.pvwork, w.pstatus, gx, gy, int(ngx), xs, ys, idx, dom_off,
+                     slot_pt, status, w.gmat, w.hfac, w.gpoly, int(cap_r), nm);
+  hipLaunchKernelGGL(k_tps_shared_iter, dim3(nd), dim3(kThreads),
+                     (size_t(kIterNV + 2) * nm + solve_scratch(kIterNV)) * sizeof(double), s,
+                     w.order_shared, w.dpat, w.dbox, w.pat_n, w.pdom_off, w.pmat_off, w.pmats,
+                     w.prefl, w.pvwork, gx, gy, int(ngx), xs, ys, idx, vx, vy, dom_off, slot_pt,
+                     smoothing, threshold, smooth_slots, nslots, weights, wstride, stats, status,
+                     slot_xy, w.gmat, w.hfac, w.gpoly, int(cap_r), nm);
   return tps_check("piv_tps_smooth_shared");
 }
 
