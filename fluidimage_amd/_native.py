@@ -25,6 +25,7 @@ PIV_OPT_MAP_SQUARE = 2
 PIV_DIRECT_PYTHRAN, PIV_DIRECT_PYCUDA = 0, 1
 # scipy.ndimage boundary modes of the sliding filters
 PIV_BOUNDARY = {"reflect": 0, "constant": 1, "nearest": 2, "mirror": 3, "wrap": 4}
+PIV_TPS_PART_PREP, PIV_TPS_PART_SHARED, PIV_TPS_PART_DIRECT, PIV_TPS_PART_ALL = 1, 2, 4, 7
 
 WIN_OK = 0
 WIN_PEAK_BOUNDARY = 1
@@ -181,7 +182,7 @@ SIGNATURES = {
         ctypes.c_int,
         [_I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P,
          _P, _P, _P, ctypes.c_double, _P, _P, _P, _P, _P, ctypes.c_int64, _P, ctypes.c_int64,
-         _P, _P, _P, _P, _I, _P, _P]),
+         _P, _P, _P, _P, _I, _I, _P, _P]),
     "piv_tps_interpolate": (
         ctypes.c_int,
         [_I, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _I, _I, _P, _P, ctypes.c_int64, _P, _P,

@@ -120,6 +120,21 @@ def _p(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+_SIDE_STREAMS = {}
+
+
+def _side_stream(dev):
+    """One side stream per device (default priority) for the branch of a
+    TPS call that runs beside the main stream's work."""
+    import torch
+
+    key = torch.device(dev).index
+    s = _SIDE_STREAMS.get(key)
+    if s is None:
+        s = _SIDE_STREAMS[key] = torch.cuda.Stream(device=dev)
+    return s
+
+
 class PlannedTps:
     """Device results of one device-sized TPS batch (see the module
     docstring); nothing is read back until ``field_errors``."""
@@ -288,16 +303,30 @@ def tps_planned(cap, dx, dy, xs, ys, new_x, new_y, subdom_size, smoothing_coef, 
         wbytes = int(lib.piv_tps_shared_workspace(nf, cap.dom_cap, SHARED_PATTERNS,
                                                   int(cap.n_max), SHARED_MAX_MISSING))
         swork = torch.empty(max(wbytes, 8), dtype=torch.uint8, device=dev)
-        _native.check(
-            lib.piv_tps_smooth_shared(
-                nf, cap.dom_cap, _p(ftab), _p(lim), _p(fstat), _p(dom_field), _p(counts),
-                _p(grid.gx), int(grid.gx.numel()), grid.irr_x, _p(grid.gy),
-                int(grid.gy.numel()), grid.irr_y, SHARED_PATTERNS, SHARED_MAX_MISSING,
-                int(cap.n_max), _p(cx), _p(cy), _p(idx), _p(vx), _p(vy), _p(dom_off),
-                _p(slot_pt), _p(mat_off), _p(smoothing), thr, _p(mats), _p(refl), _p(vwork),
-                _p(slot_xy), _p(smooth_slots), nslots, _p(weights), wstride, _p(stats),
-                _p(status), _p(order), lat_p, int(gen), _p(swork), sp),
-            "piv_tps_smooth_shared")
+
+        def shared(parts, s):
+            _native.check(
+                lib.piv_tps_smooth_shared(
+                    nf, cap.dom_cap, _p(ftab), _p(lim), _p(fstat), _p(dom_field), _p(counts),
+                    _p(grid.gx), int(grid.gx.numel()), grid.irr_x, _p(grid.gy),
+                    int(grid.gy.numel()), grid.irr_y, SHARED_PATTERNS, SHARED_MAX_MISSING,
+                    int(cap.n_max), _p(cx), _p(cy), _p(idx), _p(vx), _p(vy), _p(dom_off),
+                    _p(slot_pt), _p(mat_off), _p(smoothing), thr, _p(mats), _p(refl),
+                    _p(vwork), _p(slot_xy), _p(smooth_slots), nslots, _p(weights), wstride,
+                    _p(stats), _p(status), _p(order), lat_p, int(gen), parts, _p(swork), s),
+                "piv_tps_smooth_shared")
+
+        # the subdomains factorised alone on a side stream, beside the
+        # patterns' factorisations (one workgroup per pattern, a small launch)
+        # and the shared solves (bound by the latency of their triangular
+        # solves): the two touch disjoint subdomains
+        shared(_native.PIV_TPS_PART_PREP, sp)
+        main = stream if stream is not None else torch.cuda.current_stream()
+        side = _side_stream(dev)
+        side.wait_stream(main)
+        shared(_native.PIV_TPS_PART_DIRECT, _native.stream_ptr(side))
+        shared(_native.PIV_TPS_PART_SHARED, sp)
+        main.wait_stream(side)
     smoothed = torch.empty((2, ntot), **f64)
     _native.check(lib.piv_tps_blend(ntot, _p(c_slot), _p(c_coef), _p(c_norm), _p(smooth_slots),
                                     nslots, _p(smoothed), sp), "piv_tps_blend")

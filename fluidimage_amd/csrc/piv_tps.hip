@@ -3198,8 +3198,9 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
                           double* vwork, double* slot_xy, double* smooth_slots, int64_t nslots,
                           double* weights, int64_t wstride, double* stats, int32_t* status,
                           const int32_t* order, const piv_tps_lattice* lattice,
-                          int32_t generated, void* work, void* stream) {
+                          int32_t generated, int32_t parts, void* work, void* stream) {
   if (nfields <= 0 || dom_cap <= 0) return PIV_OK;
+  if (parts <= 0 || parts > PIV_TPS_PART_ALL) return tps_fail(PIV_EINVAL, "bad parts");
   if (n_max < 4 || n_max > kMaxN) return tps_fail(PIV_EINVAL, "n_max must be in [4, 2048]");
   if (npat_cap <= 0 || npat_cap > kPatMaxPerField)
     return tps_fail(PIV_EINVAL, "npat_cap must be in [1, 64]");
@@ -3220,23 +3221,30 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   const SharedWork w = shared_work(work, nfields, dom_cap, npat_cap, n_max, cap_r);
   const PhiLattice lat = make_lattice(lattice);
   const bool gen = generated != 0 && lat.table != nullptr;
-  hipLaunchKernelGGL(k_tps_patterns, dim3(nfields), dim3(256), 0, s, int(dom_cap), fields, lim,
-                     fstat, dom_field, counts, smoothing, gx, int(ngx), int(irr_x), gy, int(ngy),
-                     int(irr_y), int(cap_r), int(n_max), int(npat_cap), w.dmode, w.dpat, w.dbox,
-                     w.pat_n, w.pat_box, w.pat_smooth);
-  hipLaunchKernelGGL(k_tps_pattern_layout, dim3(1), dim3(1024), 0, s, npat, w.pat_n, w.pat_box,
-                     gx, gy, nd, order, w.dmode, w.pdom_off, w.pmat_off, w.pxy, w.pslot,
-                     w.pxy_stride, w.order_direct, w.order_shared);
-  hipLaunchKernelGGL(k_tps_zero_i32, dim3((npat + 255) / 256), dim3(256), 0, s, npat, w.pstatus);
-  // the patterns' factorisations, then the subdomains factorised alone
+  if (parts & PIV_TPS_PART_PREP) {
+    hipLaunchKernelGGL(k_tps_patterns, dim3(nfields), dim3(256), 0, s, int(dom_cap), fields,
+                       lim, fstat, dom_field, counts, smoothing, gx, int(ngx), int(irr_x), gy,
+                       int(ngy), int(irr_y), int(cap_r), int(n_max), int(npat_cap), w.dmode,
+                       w.dpat, w.dbox, w.pat_n, w.pat_box, w.pat_smooth);
+    hipLaunchKernelGGL(k_tps_pattern_layout, dim3(1), dim3(1024), 0, s, npat, w.pat_n,
+                       w.pat_box, gx, gy, nd, order, w.dmode, w.pdom_off, w.pmat_off, w.pxy,
+                       w.pslot, w.pxy_stride, w.order_direct, w.order_shared);
+    hipLaunchKernelGGL(k_tps_zero_i32, dim3((npat + 255) / 256), dim3(256), 0, s, npat,
+                       w.pstatus);
+  }
+  // the subdomains factorised alone
+  if (parts & PIV_TPS_PART_DIRECT) {
+    launch_factor(nd, n_max, xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork,
+                  slot_xy, nslots, status, w.order_direct, lat, gen, s);
+    launch_solve(nd, n_max, vx, vy, dom_off, slot_pt, mat_off, mats, refl, vwork, smoothing,
+                 threshold, smooth_slots, nslots, weights, wstride, stats, status,
+                 w.order_direct, s);
+  }
+  if (!(parts & PIV_TPS_PART_SHARED)) return tps_check("piv_tps_smooth_shared");
+  // the patterns' factorisations, then the subdomains that share them
   launch_factor(npat, n_max, w.pxy, w.pxy + w.pxy_stride, w.pdom_off, w.pslot, w.pmat_off,
                 w.pat_smooth, w.pmats, w.prefl, w.pvwork, w.pslot_xy, w.pxy_stride, w.pstatus,
                 nullptr, lat, gen, s);
-  launch_factor(nd, n_max, xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork,
-                slot_xy, nslots, status, w.order_direct, lat, gen, s);
-  launch_solve(nd, n_max, vx, vy, dom_off, slot_pt, mat_off, mats, refl, vwork, smoothing,
-               threshold, smooth_slots, nslots, weights, wstride, stats, status, w.order_direct,
-               s);
   hipLaunchKernelGGL(k_tps_shared_columns, dim3(nd), dim3(kThreads),
                      (size_t(kSharedNV + 1) * nm + solve_scratch(kSharedNV)) * sizeof(double), s,
                      w.order_shared, w.dpat, w.dbox, w.pat_n, w.pdom_off, w.pmat_off, w.pmats,

@@ -594,9 +594,13 @@ class UnevenGraph:
     bucket is a batched multipass, and the buckets are captured one after
     the other into the same graph; ``replay`` runs them all. ``items``:
     (im0, im1, schedule) with the images already on the device (their
-    contents may change between replays)."""
+    contents may change between replays). With ``concurrent`` the buckets
+    are captured on streams of their own, forked from and joined to the
+    capture stream, so the graph holds them as parallel branches: the
+    GPU runs the small buckets' kernels side by side instead of one bucket
+    after the other (each bucket's own kernels keep their order)."""
 
-    def __init__(self, items, params, keep_maps=False, warmup=2):
+    def __init__(self, items, params, keep_maps=False, warmup=2, concurrent=True, split=1):
         from copy import deepcopy
 
         import torch
@@ -609,8 +613,16 @@ class UnevenGraph:
         groups = {}
         for i, (im0, _im1, sched) in enumerate(items):
             groups.setdefault((tuple(im0.shape), tuple(sched)), []).append(i)
+        # ``split``: each bucket as that many sub-batches (parallel branches
+        # with ``concurrent``: one sub-batch's latency-bound kernels then run
+        # beside another's compute-bound ones)
+        chunks = []
+        for key, idx in groups.items():
+            k = max(1, min(int(split), len(idx)))
+            bounds = [len(idx) * j // k for j in range(k + 1)]
+            chunks += [(key, idx[bounds[j]:bounds[j + 1]]) for j in range(k)]
         self.groups = []
-        for (shape, sched), idx in groups.items():
+        for (shape, sched), idx in chunks:
             work = WorkPIV(schedule_params(deepcopy(params), sched))
             for w in work.works_piv:
                 w._prepare_with_image(imshape=shape)
@@ -634,11 +646,19 @@ class UnevenGraph:
         self.keep_maps = keep_maps
         self.graph = torch.cuda.CUDAGraph()
         self.passes = []
+        # default-priority streams, one per bucket
+        self.streams = [torch.cuda.Stream() for _ in self.groups] if concurrent else []
         with torch.cuda.graph(self.graph):
-            for (work, images, idx), pipe in zip(self.groups, self.pipes):
-                images.recompute_min()
-                self.passes.append(pipe.run(images, len(idx), keep_maps=keep_maps,
-                                            check=False))
+            cap = torch.cuda.current_stream()
+            for g, ((work, images, idx), pipe) in enumerate(zip(self.groups, self.pipes)):
+                branch = self.streams[g] if concurrent else cap
+                branch.wait_stream(cap)
+                with torch.cuda.stream(branch):
+                    images.recompute_min()
+                    self.passes.append(pipe.run(images, len(idx), keep_maps=keep_maps,
+                                                check=False))
+            for branch in self.streams:
+                cap.wait_stream(branch)
         self.pending = []
         for pipe in self.pipes:
             self.pending.append(list(pipe._pending))

@@ -384,7 +384,16 @@ int64_t piv_tps_shared_workspace(int32_t nfields, int32_t dom_cap, int32_t npat_
  * coordinates); status[d] = 2 when a center is off the grid. At most
  * npat_cap (<= 64) patterns per field. `order`: the order of the nfields
  * dom_cap subdomain slots (piv_tps_offsets); idx: the vector index of
- * every center (piv_tps_select); work: piv_tps_shared_workspace bytes. */
+ * every center (piv_tps_select); work: piv_tps_shared_workspace bytes.
+ * `parts`: the PIV_TPS_PART_* steps to launch: PREP (patterns and orders)
+ * first, then SHARED (the patterns' factorisations and the shared solves)
+ * and DIRECT (the subdomains factorised alone), which touch disjoint
+ * subdomains and may run on two streams after PREP (the caller orders
+ * them); PIV_TPS_PART_ALL launches the three in order on `stream`. */
+#define PIV_TPS_PART_PREP 1
+#define PIV_TPS_PART_SHARED 2
+#define PIV_TPS_PART_DIRECT 4
+#define PIV_TPS_PART_ALL 7
 int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field* fields,
                           const double* lim, const int32_t* fstat, const int32_t* dom_field,
                           const int32_t* counts, const double* gx, int32_t ngx, int32_t irr_x,
@@ -396,7 +405,7 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
                           double* vwork, double* slot_xy, double* smooth_slots, int64_t nslots,
                           double* weights, int64_t wstride, double* stats, int32_t* status,
                           const int32_t* order, const piv_tps_lattice* lattice,
-                          int32_t generated, void* work, void* stream);
+                          int32_t generated, int32_t parts, void* work, void* stream);
 
 /* ---- device-sized TPS batches (no counterpart in the reference, whose
  *      ThinPlateSplineSubdom.__init__ sizes each field on the host,

@@ -193,8 +193,9 @@ def test_uneven_graph_equals_run_uneven():
     p.multipass.use_tps = True
     p.multipass.smoothing_coef = 2.0
     expect = run_uneven(items, p)
-    g = UnevenGraph(items, p)
-    for rep in range(2):
-        g.replay()
-        got = g.results()
-        _same(got, expect, f"uneven replay {rep}")
+    for concurrent in (False, True):
+        g = UnevenGraph(items, p, concurrent=concurrent)
+        for rep in range(3):
+            g.replay()
+            got = g.results()
+            _same(got, expect, f"uneven replay {rep} (concurrent {concurrent})")
