@@ -2262,7 +2262,11 @@ __device__ int shared_missing(int nf, const int* fmem, int cap_r, int* rlist, in
   return *nr_sh;
 }
 
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void
+// Launched twice: NV = 4 checks every shared subdomain and computes the
+// columns of those missing at most 4 nodes; NV = kSharedNV those of the
+// others (checked already: status 0, more than 4 missing nodes).
+template <int NV>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NV <= 4 ? 3 : 2))) void
 k_tps_shared_columns(const int32_t* __restrict__ order, const int32_t* __restrict__ dpat,
                      const int32_t* __restrict__ dbox, const int32_t* __restrict__ pat_n,
                      const int64_t* __restrict__ pdom_off, const int64_t* __restrict__ pmat_off,
@@ -2278,8 +2282,10 @@ k_tps_shared_columns(const int32_t* __restrict__ order, const int32_t* __restric
   if (int(blockIdx.x) >= nvalid) return;
   const int d = order[xcd_rank(blockIdx.x, nvalid)];
   if (d < 0) return;
+  constexpr bool kFirst = NV <= 4;  // the launch that checks every subdomain
   const int p = dpat[d];
   const int tid = threadIdx.x;
+  if (!kFirst && status[d]) return;
   if (pstatus[p]) {
     if (tid == 0) status[d] = 1;
     return;
@@ -2296,18 +2302,18 @@ k_tps_shared_columns(const int32_t* __restrict__ order, const int32_t* __restric
   const double* RF = prefl + 16 * p;
   const double* VW = pvwork + 6 * ps0;
   const double tau[3] = {RF[0], RF[1], RF[2]};
-  // dynamic LDS: x[kSharedNV][nm] (doubles), fidx[nm], fmem[nm], then the
-  // solve's scratch
+  // dynamic LDS: x[NV][nm] (doubles), fidx[nm], fmem[nm], then the solve's
+  // scratch
   extern __shared__ double dyn[];
   double* x = dyn;
-  int* fidx = reinterpret_cast<int*>(dyn + kSharedNV * nm);
+  int* fidx = reinterpret_cast<int*>(dyn + NV * nm);
   int* fmem = fidx + nm;
-  double* scr = dyn + (kSharedNV + 1) * nm;
-  __shared__ double red[kSharedNV * (kThreads / 64)];
+  double* scr = dyn + (NV + 1) * nm;
+  __shared__ double red[NV * (kThreads / 64)];
   __shared__ int rlist[kCapMaxR];
   __shared__ int nr_sh;
   __shared__ double H[kCapMaxR][kCapMaxR + 1];
-  __shared__ double a8[kSharedNV][3];
+  __shared__ double a8[NV][3];
   __shared__ int bad;
   if (tid == 0) bad = 0;
   if (!shared_members(n, nf, s0, slot_pt, idx, xs, ys, gx, gy, ngx, j0, nc, i0, fidx, fmem,
@@ -2320,18 +2326,18 @@ k_tps_shared_columns(const int32_t* __restrict__ order, const int32_t* __restric
     if (tid == 0) status[d] = 2;
     return;
   }
-  if (nr == 0) return;
-  // G columns g0 .. g0 + kSharedNV - 1 per pass over the factor
+  if (nr == 0 || (kFirst ? nr > NV : nr <= 4)) return;  // the other launch's
+  // G columns g0 .. g0 + NV - 1 per pass over the factor
   double* G = gmat + int64_t(d) * nm * cap_r;  // columns of G_w, nm apart
   double* GA = gpoly + int64_t(d) * kCapMaxR * 3;
-  for (int g0 = 0; g0 < nr; g0 += kSharedNV) {
+  for (int g0 = 0; g0 < nr; g0 += NV) {
     for (int k = tid; k < nf; k += kThreads)
 #pragma unroll
-      for (int q = 0; q < kSharedNV; ++q)
+      for (int q = 0; q < NV; ++q)
         x[q * nm + k] = (g0 + q < nr && k == rlist[g0 + q]) ? 1.0 : 0.0;
     __syncthreads();
-    saddle_solve_n<kSharedNV>(x, nm, nf, VW, tau, A, ld, E, RF, red, scr, a8);
-    for (int q = 0; q < kSharedNV; ++q) {
+    saddle_solve_n<NV>(x, nm, nf, VW, tau, A, ld, E, RF, red, scr, a8);
+    for (int q = 0; q < NV; ++q) {
       const int r = g0 + q;
       if (r >= nr) break;
       for (int k = tid; k < nf; k += kThreads) G[int64_t(r) * nm + k] = x[q * nm + k];
@@ -2367,7 +2373,7 @@ k_tps_shared_columns(const int32_t* __restrict__ order, const int32_t* __restric
 
 constexpr int kIterNV = 2;
 
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void
 k_tps_shared_iter(const int32_t* __restrict__ order, const int32_t* __restrict__ dpat,
                   const int32_t* __restrict__ dbox, const int32_t* __restrict__ pat_n,
                   const int64_t* __restrict__ pdom_off, const int64_t* __restrict__ pmat_off,
@@ -2419,6 +2425,7 @@ k_tps_shared_iter(const int32_t* __restrict__ order, const int32_t* __restrict__
   __shared__ double H[kCapMaxR][kCapMaxR + 1];
   __shared__ double ga[kCapMaxR][3];
   __shared__ double lam[2][kCapMaxR];
+  __shared__ double zl[2][kCapMaxR];
   __shared__ double a2[kIterNV][3];  // polynomial terms of the solved vectors
   __shared__ int go[2];
   __shared__ int bad;
@@ -2451,30 +2458,23 @@ k_tps_shared_iter(const int32_t* __restrict__ order, const int32_t* __restrict__
     }
     __syncthreads();
     saddle_solve_n<kIterNV>(x, nm, nf, VW, tau, A, ld, E, RF, red, scr, a2);
-    // lambda = -H^-1 y_w[R] (two triangular solves with L_H)
+    // lambda = -H^-1 y_w[R] (two triangular solves with L_H, the forward
+    // one's result in zl)
     if (tid < 2) {
       const int c = tid;
-      double z[kCapMaxR];
       for (int i = 0; i < nr; ++i) {
         double v = -x[c * nm + rlist[i]];
-        for (int k = 0; k < i; ++k) v -= H[i][k] * z[k];
-        z[i] = v / H[i][i];
+        for (int k = 0; k < i; ++k) v -= H[i][k] * zl[c][k];
+        zl[c][i] = v / H[i][i];
       }
       for (int i = nr - 1; i >= 0; --i) {
-        double v = z[i];
+        double v = zl[c][i];
         for (int k = i + 1; k < nr; ++k) v -= H[k][i] * lam[c][k];
         lam[c][i] = v / H[i][i];
       }
     }
     __syncthreads();
-    // w = y + G lambda at the members; a = y_a + G_a lambda
-    double wa[2][3];
-    for (int c = 0; c < 2; ++c)
-      for (int t = 0; t < 3; ++t) {
-        double v = a2[c][t];
-        for (int k = 0; k < nr; ++k) v += ga[k][t] * lam[c][k];
-        wa[c][t] = v;
-      }
+    // w = y + G lambda at the members (a = y_a + G_a lambda: below)
     for (int j = tid; j < n; j += kThreads) {
       const int k = fidx[j];
       double w0 = x[k], w1 = x[nm + k];
@@ -2503,10 +2503,16 @@ k_tps_shared_iter(const int32_t* __restrict__ order, const int32_t* __restrict__
       double* W = weights + c * wstride + s0 + 3 * int64_t(d);
       for (int j = tid; j < n; j += kThreads) W[j] = x[c * nm + fidx[j]];
       if (tid == 0) {
+        double wa[3];
+        for (int t = 0; t < 3; ++t) {
+          double v = a2[c][t];
+          for (int k = 0; k < nr; ++k) v += ga[k][t] * lam[c][k];
+          wa[t] = v;
+        }
         // P = [1, y - y0, x - x0] -> [1, y, x]
-        W[n] = wa[c][0] - wa[c][1] * y0 - wa[c][2] * x0;
-        W[n + 1] = wa[c][1];
-        W[n + 2] = wa[c][2];
+        W[n] = wa[0] - wa[1] * y0 - wa[2] * x0;
+        W[n + 1] = wa[1];
+        W[n + 2] = wa[2];
       }
       const double dm = block_max(dmax, red);
       const double nesum = block_sum(double(ne), red);
@@ -3245,7 +3251,13 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   launch_factor(npat, n_max, w.pxy, w.pxy + w.pxy_stride, w.pdom_off, w.pslot, w.pmat_off,
                 w.pat_smooth, w.pmats, w.prefl, w.pvwork, w.pslot_xy, w.pxy_stride, w.pstatus,
                 nullptr, lat, gen, s);
-  hipLaunchKernelGGL(k_tps_shared_columns, dim3(nd), dim3(kThreads),
+  // subdomains missing at most 4 nodes (and every check), then the others
+  hipLaunchKernelGGL(k_tps_shared_columns<4>, dim3(nd), dim3(kThreads),
+                     (size_t(4 + 1) * nm + solve_scratch(4)) * sizeof(double), s,
+                     w.order_shared, w.dpat, w.dbox, w.pat_n, w.pdom_off, w.pmat_off, w.pmats,
+                     w.prefl, w.pvwork, w.pstatus, gx, gy, int(ngx), xs, ys, idx, dom_off,
+                     slot_pt, status, w.gmat, w.hfac, w.gpoly, int(cap_r), nm);
+  hipLaunchKernelGGL(k_tps_shared_columns<kSharedNV>, dim3(nd), dim3(kThreads),
                      (size_t(kSharedNV + 1) * nm + solve_scratch(kSharedNV)) * sizeof(double), s,
                      w.order_shared, w.dpat, w.dbox, w.pat_n, w.pdom_off, w.pmat_off, w.pmats,
                      w.prefl, w.pvwork, w.pstatus, gx, gy, int(ngx), xs, ys, idx, dom_off,
