@@ -261,6 +261,9 @@ def _apply_tuning(lib):
     if "tps_chol_threads" in table:
         check(lib.piv_tune_set(2, 0, 0, int(table["tps_chol_threads"])),
               f"tuning table {choice}: tps_chol_threads", lib)
+    if "tps_shared_solve" in table:
+        check(lib.piv_tune_set(3, 0, 0, int(table["tps_shared_solve"])),
+              f"tuning table {choice}: tps_shared_solve", lib)
 
 
 def check(status, what="", lib=None):

@@ -1507,6 +1507,11 @@ int piv_tune_candidates(int32_t param, int32_t n0, int32_t n1, int32_t* values, 
     put(512);
     return count;
   }
+  if (param == PIV_TUNE_TPS_SHARED_SOLVE) {
+    put(0);
+    put(1);
+    return count;
+  }
   return -fail(PIV_EINVAL, "unknown tuning parameter");
 }
 
@@ -1517,6 +1522,7 @@ int piv_tune_get(int32_t param, int32_t n0, int32_t n1) {
     return variant_of(l).wpb;
   }
   if (param == PIV_TUNE_TPS_CHOL_THREADS) return fia::tps_chol_threads();
+  if (param == PIV_TUNE_TPS_SHARED_SOLVE) return fia::tps_shared_solve();
   return -fail(PIV_EINVAL, "unknown tuning parameter");
 }
 
@@ -1535,6 +1541,11 @@ int piv_tune_set(int32_t param, int32_t n0, int32_t n1, int32_t value) {
     if (value != 0 && value != 256 && value != 512)
       return fail(PIV_EINVAL, "TPS Cholesky threads must be 0, 256 or 512");
     fia::set_tps_chol_threads(value);
+    return PIV_OK;
+  }
+  if (param == PIV_TUNE_TPS_SHARED_SOLVE) {
+    if (value != 0 && value != 1) return fail(PIV_EINVAL, "TPS shared solve must be 0 or 1");
+    fia::set_tps_shared_solve(value);
     return PIV_OK;
   }
   return fail(PIV_EINVAL, "unknown tuning parameter");

@@ -16,6 +16,7 @@
 // One workgroup per subdomain; matrices live in a caller workspace.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <string>
 
@@ -100,6 +101,13 @@ __host__ __device__ __forceinline__ int64_t tps_mat_e_size(int m) {
 __host__ __device__ __forceinline__ int64_t tps_mat_size(int n) {
   const int m = n > 3 ? n - 3 : 0;
   return tps_mat_e_size(m) + int64_t(m) * tps_mat_ld(m);
+}
+
+// Explicit saddle-point inverse of a shared pattern of n nodes: X (n x n) in
+// rows of tps_x_ld(n) doubles, then the three rows of Y^T.
+__host__ __device__ __forceinline__ int64_t tps_x_ld(int n) { return (int64_t(n) + 15) & ~int64_t(15); }
+__host__ __device__ __forceinline__ int64_t tps_x_size(int n) {
+  return n > 0 ? (int64_t(n) + 3) * tps_x_ld(n) : 0;
 }
 
 // The kernel function at offsets that are multiples of a lattice step:
@@ -1200,6 +1208,8 @@ __device__ void chol_solve_n(double* x, int nm, const double* A, int64_t ld, int
 
 // saddle_solve for NV right-hand sides x[q nm + j]; the polynomial terms go
 // to aout[q][0..2] (LDS, written by thread 0, read after the call)
+// (w, a) = M^-1 [x; 0] for NV right-hand sides (see saddle_solve); scr:
+// solve_scratch(NV) doubles of LDS.
 template <int NV>
 __device__ void saddle_solve_n(double* x, int nm, int n, const double* VW, const double* tau,
                                const double* A, int64_t ld, const double* E, const double* RF,
@@ -1209,7 +1219,9 @@ __device__ void saddle_solve_n(double* x, int nm, int n, const double* VW, const
   // Q^T x; the first three entries (the a-part of the rhs) are left alone by the
   // Cholesky solve of the others
   apply_q_n<NV>(x, nm, n, VW, tau, true, red);
-  if (m > 0) chol_solve_n<NV>(x, nm, A, ld, m, scr);
+  if (m > 0) {
+    chol_solve_n<NV>(x, nm, A, ld, m, scr);
+  }
   // a = R^-1 (vt - B'[:3, 3:] z), B'[:3, 3 + j] = B'[3 + j][k] = E[j][k]
   for (int k = 0; k < 3; ++k) {
     double bz[NV];
@@ -2124,39 +2136,77 @@ __global__ __launch_bounds__(256) void k_tps_patterns(
   }
 }
 
-// One workgroup of 1024 threads: offsets of the patterns' centers and
-// matrices (prefix sums of their sizes), their centers (grid nodes relative
-// to the box's first node, row-major: the order of the members of a
-// subdomain), an identity slot list, and the orders of the direct
+// inclusive max-scan over a block of 1024 threads (tmp: 32 ints of LDS)
+__device__ int block_scan_max_1024(int v, int* tmp) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v = max(v, u);
+  }
+  if (lane == 63) tmp[wv] = v;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    int x = tmp[threadIdx.x];
+    for (int off = 1; off < 16; off <<= 1) {
+      const int u = __shfl_up(x, off, 16);
+      if (threadIdx.x >= off) x = max(x, u);
+    }
+    tmp[16 + threadIdx.x] = x;
+  }
+  __syncthreads();
+  const int before = wv > 0 ? tmp[16 + wv - 1] : INT_MIN;
+  __syncthreads();
+  return max(v, before);
+}
+
+// One workgroup of 1024 threads: offsets of the patterns' centers, matrices
+// and explicit inverses (prefix sums of their sizes), their centers (grid
+// nodes relative to the box's first node, row-major: the order of the
+// members of a subdomain), an identity slot list, the orders of the direct
 // subdomains (from `order`, largest first) and of the shared ones (by
-// index); -1 beyond.
+// index; -1 beyond), and the groups of k_tps_shared_x: runs of one pattern
+// in the shared order cut every `group` subdomains (groups[g] the first
+// entry of group g, groups[nd + 1 + g] its size, groups[nd] their number).
 __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
     int npat, const int32_t* __restrict__ pat_n, const int32_t* __restrict__ pat_box,
     const double* __restrict__ gx, const double* __restrict__ gy, int nd,
     const int32_t* __restrict__ order, const int32_t* __restrict__ dmode,
-    int64_t* __restrict__ pdom_off, int64_t* __restrict__ pmat_off, double* __restrict__ pxy,
+    const int32_t* __restrict__ dpat, int group, int64_t* __restrict__ pdom_off,
+    int64_t* __restrict__ pmat_off, int64_t* __restrict__ pxo, double* __restrict__ pxy,
     int32_t* __restrict__ pslot, int64_t pxy_stride, int32_t* __restrict__ order_direct,
-    int32_t* __restrict__ order_shared) {
+    int32_t* __restrict__ order_shared, int32_t* __restrict__ groups,
+    int32_t* __restrict__ plist) {
   __shared__ int64_t tmp64[32];
   __shared__ int tmp[32];
+  __shared__ int last_run;
   const int tid = threadIdx.x;
-  int64_t base_s = 0, base_m = 0;
+  int64_t base_s = 0, base_m = 0, base_x = 0;
+  int base_u = 0;
   for (int c0 = 0; c0 < npat; c0 += 1024) {
     const int p = c0 + tid;
     const int c = p < npat ? pat_n[p] : 0;
-    int64_t ts, tm;
+    int tu;
+    const int iu = block_scan_1024(c > 0 ? 1 : 0, tmp, tu);
+    if (c > 0) plist[base_u + iu - 1] = p;
+    base_u += tu;
+    int64_t ts, tm, tx;
     const int64_t is = block_scan_1024_64(int64_t(c), tmp64, ts);
     const int64_t im = block_scan_1024_64(c > 0 ? tps_mat_size(c) : 0, tmp64, tm);
+    const int64_t ix = block_scan_1024_64(tps_x_size(c), tmp64, tx);
     if (p < npat) {
       pdom_off[p] = base_s + is - c;
       pmat_off[p] = base_m + im - (c > 0 ? tps_mat_size(c) : 0);
+      pxo[p] = base_x + ix - tps_x_size(c);
     }
     base_s += ts;
     base_m += tm;
+    base_x += tx;
   }
   if (tid == 0) {
     pdom_off[npat] = base_s;
     pmat_off[npat] = base_m;
+    pxo[npat] = base_x;
+    plist[npat] = base_u;
   }
   __syncthreads();
   for (int p = 0; p < npat; ++p) {
@@ -2192,6 +2242,30 @@ __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
   for (int b = bd + tid; b < nd; b += 1024) order_direct[b] = -1;
   for (int b = bs + tid; b < nd; b += 1024) order_shared[b] = -1;
   if (tid == 0) order_shared[nd] = bs;
+  int32_t* gcnt = groups + nd + 1;
+  for (int b = tid; b < nd; b += 1024) gcnt[b] = 0;
+  __syncthreads();  // the shared order and the zeroed sizes, for the whole block
+  int run = -1, ng = 0;
+  for (int c0 = 0; c0 < bs; c0 += 1024) {
+    const int b = c0 + tid;
+    const bool valid = b < bs;
+    const int pb = valid ? dpat[order_shared[b]] : -1;
+    const int pp = valid && b > 0 ? dpat[order_shared[b - 1]] : -1;
+    const bool first = valid && (b == 0 || pb != pp);
+    const int rs = max(run, block_scan_max_1024(first ? b : -1, tmp));
+    const bool gs = valid && (b - rs) % group == 0;
+    int tg;
+    const int ig = block_scan_1024(gs ? 1 : 0, tmp, tg);
+    const int gi = ng + ig - 1;
+    if (gs) groups[gi] = b;
+    if (valid) atomicAdd(&gcnt[gi], 1);
+    if (tid == 1023) last_run = rs;
+    __syncthreads();
+    run = last_run;
+    ng += tg;
+    __syncthreads();
+  }
+  if (tid == 0) groups[nd] = ng;
 }
 
 // Solves of the subdomains that share their pattern's factorisation, with
@@ -2219,6 +2293,14 @@ constexpr int kSharedNV = 8;
 // LDS scratch of chol_solve_n<NV> (doubles): the diagonal block, the block
 // right-hand sides and the partial sums of NV vectors
 constexpr int solve_scratch(int nv) { return kNB * (kNB + 1) + 8 * kNB + nv * 8 * kNB; }
+
+// dynamic LDS bytes of k_tps_shared_columns<nv> and k_tps_shared_iter
+constexpr size_t shared_columns_lds(int nv, int nm) {
+  return (size_t(nv + 1) * nm + solve_scratch(nv)) * sizeof(double);
+}
+constexpr size_t shared_iter_lds(int nv, int nm) {
+  return (size_t(nv + 2) * nm + solve_scratch(nv)) * sizeof(double);
+}
 
 // members of subdomain d on its box: fidx[j] (node of member j), fmem[k]
 // (member of node k, -1 for a missing node); false when a center is off the
@@ -2538,6 +2620,749 @@ k_tps_shared_iter(const int32_t* __restrict__ order, const int32_t* __restrict__
       stats[6 * d + 3 * c + 2] = count[c];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// 4c. the shared solves by the explicit inverse of each pattern's saddle-
+//     point matrix
+// ---------------------------------------------------------------------------
+// For a pattern of n nodes, M^-1 [v; 0] = [X v; Y^T v] with
+//   X = Q [[0, 0], [0, C^-1]] Q^T (n x n, symmetric),
+//   Y = Q [I_3; -C^-1 E] R^-T (n x 3),
+// from the null-space factorisation (k_tps_prep, k_tps_chol): the
+// reflectors Q = H_0 H_1 H_2 = I - V T V^T of P = Q [R; 0], C = L L^T and
+// E = B'[3:, :3]. They are built once per pattern:
+//   k_tps_tri_inverse  Linv = L^-1, block columns in parallel;
+//   k_tps_inv_gram     C^-1 = Linv^T Linv, over L in the pattern's matrix
+//                      region (E is kept);
+//   k_tps_xprep        U = C^-1 [V[3:, :] | E] (one pass over C^-1), then the
+//                      rank-6 form X = Z - A Cc^T - Cc A^T of the two-sided
+//                      reflection (Z = [[0, 0], [0, C^-1]], A = V T,
+//                      Cc = U' - A S / 2, U' = Z V = [0; C^-1 V[3:, :]],
+//                      S = V^T U') and Y = F - A (V^T F),
+//                      F = [R^-T; -C^-1 E R^-T];
+//   k_tps_xfill        X, rows of tps_x_ld(n) doubles, Y^T in the three rows
+//                      after it.
+// A subdomain missing the nodes R of its pattern then needs no solve: its
+// capacitance columns are rows of X (G_w = X[:, R], G_a = Y[R, :],
+// H = X[R, R]), and each outlier iteration is one product with X
+// (k_tps_shared_x: up to four subdomains of one pattern per workgroup, one
+// read of X serving all of them).
+
+// Launch shape of these kernels: a fixed grid whose workgroups stride over
+// the items (pattern, column or tile or rows) of the patterns in use
+// (plist[0 .. plist[npat]), written by k_tps_pattern_layout), most work
+// first, instead of a grid over every pattern slot, most of them empty.
+
+// Linv of pattern p, block column J: Linv_JJ = inv(L_JJ) (the factorisation keeps
+// inv(L_JJ)^T in the strict upper triangle of the diagonal block, L_JJ's
+// diagonal on it), then for I > J, Linv_IJ = -inv(L_II) sum_{K=J}^{I-1}
+// L_IK Linv_KJ, the blocks of each term staged in LDS with the next term's
+// loads in flight, 4 outputs per thread. Linv_IJ goes to the rows of I and
+// the columns of J of Ainv (the layout of the factor's matrix region), its
+// transpose to the rows of J and the columns of I; the workgroup reads its
+// own earlier blocks back after a barrier.
+__device__ void tri_inverse_column(int p, int J, const int32_t* __restrict__ pat_n,
+                                   const int64_t* __restrict__ pmat_off,
+                                   const double* __restrict__ pmats,
+                                   const int32_t* __restrict__ pstatus,
+                                   double* __restrict__ pinv) {
+  static_assert(kThreads == 256 && kNB == 32, "thread layout of tri_inverse_column");
+  const int n = pat_n[p];
+  if (n < 4 || pstatus[p]) return;
+  const int m = n - 3;
+  const int nblk = (m + kNB - 1) / kNB;
+  if (J >= nblk) return;
+  const int64_t ld = tps_mat_ld(m);
+  const double* A = pmats + pmat_off[p] + tps_mat_e_size(m);
+  double* Inv = pinv + pmat_off[p] + tps_mat_e_size(m);
+  __shared__ double Lb[kNB][kNB + 1];  // L_IK, then inv(L_II)
+  __shared__ double Xb[kNB][kNB + 1];  // Linv_KJ, then the sum
+  const int tid = threadIdx.x;
+  const int r0J = J * kNB, bsJ = min(kNB, m - r0J);
+  const int ti = tid >> 3, tc = tid & 7;   // outputs (ti, tc + 8 v), v < 4
+  const int er = tid >> 5, ec = tid & 31;  // staged elements (er + 8 t, ec), t < 4
+  // inv(L_II) of diagonal block I into Lb (zero beyond the matrix)
+  auto stage_inv_diag = [&](int r0, int bs) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int i = er + 8 * t;
+      double v = 0.0;
+      if (i < bs && ec < bs) {
+        if (ec == i) v = 1.0 / A[int64_t(r0 + i) * ld + r0 + i];
+        else if (ec < i) v = A[int64_t(r0 + ec) * ld + r0 + i];
+      }
+      Lb[i][ec] = v;
+    }
+  };
+  // L_IK and Linv_KJ (its lower triangle only for K = J) into registers
+  auto load_term = [&](int r0I, int bsI, int K, double* la, double* lx) {
+    const int r0K = K * kNB, bsK = min(kNB, m - r0K);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int i = er + 8 * t;
+      la[t] = (i < bsI && ec < bsK) ? A[int64_t(r0I + i) * ld + r0K + ec] : 0.0;
+      lx[t] = (i < bsK && ec < bsJ && (K > J || ec <= i))
+                  ? Inv[int64_t(r0K + i) * ld + r0J + ec]
+                  : 0.0;
+    }
+  };
+  stage_inv_diag(r0J, bsJ);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int i = er + 8 * t;
+    if (i < bsJ && ec <= i) {
+      const double v = Lb[i][ec];
+      Inv[int64_t(r0J + i) * ld + r0J + ec] = v;
+      if (ec < i) Inv[int64_t(r0J + ec) * ld + r0J + i] = v;
+    }
+  }
+  __syncthreads();
+  for (int I = J + 1; I < nblk; ++I) {
+    const int r0I = I * kNB, bsI = min(kNB, m - r0I);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double la[4], lx[4];
+    load_term(r0I, bsI, J, la, lx);
+    for (int K = J; K < I; ++K) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        Lb[er + 8 * t][ec] = la[t];
+        Xb[er + 8 * t][ec] = lx[t];
+      }
+      __syncthreads();
+      if (K + 1 < I) load_term(r0I, bsI, K + 1, la, lx);
+#pragma unroll 4
+      for (int k = 0; k < kNB; ++k) {
+        const double l = Lb[ti][k];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] += l * Xb[k][tc + 8 * v];
+      }
+      __syncthreads();
+    }
+    // Linv_IJ = -inv(L_II) sum
+    stage_inv_diag(r0I, bsI);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Xb[ti][tc + 8 * v] = acc[v];
+    __syncthreads();
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k = 0; k < kNB; ++k) {
+      const double l = Lb[ti][k];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) x[v] -= l * Xb[k][tc + 8 * v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int c = tc + 8 * v;
+      if (ti < bsI && c < bsJ) {
+        Inv[int64_t(r0I + ti) * ld + r0J + c] = x[v];
+        Inv[int64_t(r0J + c) * ld + r0I + ti] = x[v];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// items: block column J = w / npu (the long columns first) of pattern
+// plist[w % npu]
+__global__ __launch_bounds__(kThreads) void k_tps_tri_inverse(
+    const int32_t* __restrict__ plist, int npat, int nblk_max, const int32_t* __restrict__ pat_n,
+    const int64_t* __restrict__ pmat_off, const double* __restrict__ pmats,
+    const int32_t* __restrict__ pstatus, double* __restrict__ pinv) {
+  const int npu = plist[npat];
+  for (int w = blockIdx.x; w < npu * nblk_max; w += gridDim.x)
+    tri_inverse_column(plist[w % npu], w / npu, pat_n, pmat_off, pmats, pstatus, pinv);
+}
+
+// C^-1 = Linv^T Linv of pattern p: tile (I, J), I >= J, number
+// t = I (I + 1) / 2 + J: sum_{K >= I} Linv_KI^T Linv_KJ, written at (I, J)
+// and mirrored at (J, I) over L (E is kept). The two threads of a mirrored
+// pair inside a diagonal tile add the same products in the same order: C^-1
+// is exactly symmetric.
+__device__ void inv_gram_tile(int p, int t, const int32_t* __restrict__ pat_n,
+                              const int64_t* __restrict__ pmat_off, double* __restrict__ pmats,
+                              const int32_t* __restrict__ pstatus,
+                              const double* __restrict__ pinv) {
+  static_assert(kThreads == 256 && kNB == 32, "thread layout of inv_gram_tile");
+  const int n = pat_n[p];
+  if (n < 4 || pstatus[p]) return;
+  const int m = n - 3;
+  const int nblk = (m + kNB - 1) / kNB;
+  int I = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  while (I * (I + 1) / 2 > t) --I;
+  const int J = t - I * (I + 1) / 2;
+  if (I >= nblk) return;
+  const int64_t ld = tps_mat_ld(m);
+  const double* Li = pinv + pmat_off[p] + tps_mat_e_size(m);
+  double* Xc = pmats + pmat_off[p] + tps_mat_e_size(m);
+  __shared__ double Ab[kNB][kNB + 1];  // Ab[k][a] = Linv[r0K + k][r0I + a]
+  __shared__ double Bb[kNB][kNB + 1];  // Bb[k][b] = Linv[r0K + k][r0J + b]
+  const int tid = threadIdx.x;
+  const int r0I = I * kNB, bsI = min(kNB, m - r0I);
+  const int r0J = J * kNB, bsJ = min(kNB, m - r0J);
+  const int ta = tid >> 3, tc = tid & 7;
+  const int er = tid >> 5, ec = tid & 31;
+  auto load = [&](int K, double* la, double* lb) {
+    const int r0K = K * kNB, bsK = min(kNB, m - r0K);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = er + 8 * u;
+      la[u] = (k < bsK && ec < bsI && (K > I || ec <= k)) ? Li[int64_t(r0K + k) * ld + r0I + ec]
+                                                          : 0.0;
+      lb[u] = (k < bsK && ec < bsJ && (K > J || ec <= k)) ? Li[int64_t(r0K + k) * ld + r0J + ec]
+                                                          : 0.0;
+    }
+  };
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double la[4], lb[4];
+  load(I, la, lb);
+  for (int K = I; K < nblk; ++K) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      Ab[er + 8 * u][ec] = la[u];
+      Bb[er + 8 * u][ec] = lb[u];
+    }
+    __syncthreads();
+    if (K + 1 < nblk) load(K + 1, la, lb);
+#pragma unroll 4
+    for (int k = 0; k < kNB; ++k) {
+      const double a = Ab[k][ta];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] = fma(a, Bb[k][tc + 8 * v], acc[v]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int b = tc + 8 * v;
+    if (ta < bsI && b < bsJ) {
+      Xc[int64_t(r0I + ta) * ld + r0J + b] = acc[v];
+      Xc[int64_t(r0J + b) * ld + r0I + ta] = acc[v];
+    }
+  }
+}
+
+// items: tile t = w / npu (the tiles of most terms first) of pattern
+// plist[w % npu]
+__global__ __launch_bounds__(kThreads) void k_tps_inv_gram(
+    const int32_t* __restrict__ plist, int npat, int ntile_max, const int32_t* __restrict__ pat_n,
+    const int64_t* __restrict__ pmat_off, double* __restrict__ pmats,
+    const int32_t* __restrict__ pstatus, const double* __restrict__ pinv) {
+  const int npu = plist[npat];
+  for (int w = blockIdx.x; w < npu * ntile_max; w += gridDim.x)
+    inv_gram_tile(plist[w % npu], w / npu, pat_n, pmat_off, pmats, pstatus, pinv);
+}
+
+// per pattern: U' = [0; C^-1 V[3:, :]] and C^-1 E by rows (lanes over rows,
+// reading the symmetric C^-1 by columns), into pac (6 doubles per node);
+// then A = V T and Cc = U' - A S / 2 over them, and Y^T after X. Dynamic
+// LDS: V[3:, :] and E, 6 nm doubles.
+__device__ void xprep_pattern(int p, const int32_t* __restrict__ pat_n,
+                              const int64_t* __restrict__ pmat_off,
+                              const double* __restrict__ pmats,
+                              const int64_t* __restrict__ pdom_off,
+                              const double* __restrict__ prefl,
+                              const double* __restrict__ pvwork,
+                              const int32_t* __restrict__ pstatus,
+                              const int64_t* __restrict__ pxo, double* __restrict__ pinv,
+                              double* __restrict__ pac) {
+  const int n = pat_n[p];
+  if (n < 4 || pstatus[p]) return;
+  const int m = n - 3;
+  const int64_t ld = tps_mat_ld(m);
+  const double* E = pmats + pmat_off[p];
+  const double* Xc = E + tps_mat_e_size(m);
+  const double* RF = prefl + 16 * p;
+  const double* VW = pvwork + 6 * pdom_off[p];
+  double* AC = pac + 6 * pdom_off[p];
+  const int64_t ldx = tps_x_ld(n);
+  double* Yt = pinv + pxo[p] + int64_t(n) * ldx;
+  extern __shared__ double dyn[];
+  double* VE = dyn;  // [m][6]: V[3 + j][0..2], E[j][0..2]
+  __shared__ double red[18 * (kThreads / 64)];
+  const int tid = threadIdx.x;
+  for (int j = tid; j < m; j += kThreads)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      VE[6 * j + k] = VW[6 * (3 + j) + k];
+      VE[6 * j + 3 + k] = E[3 * int64_t(j) + k];
+    }
+  for (int j = tid; j < 3; j += kThreads)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) AC[6 * j + k] = 0.0;
+  __syncthreads();
+  // U rows (slot j = 3 + i)
+  for (int i = tid; i < m; i += kThreads) {
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double* col = Xc + i;
+    constexpr int U8 = 8;
+    int j = 0;
+    for (; j + U8 <= m; j += U8) {
+      double a[U8];
+#pragma unroll
+      for (int u = 0; u < U8; ++u) a[u] = col[int64_t(j + u) * ld];
+#pragma unroll
+      for (int u = 0; u < U8; ++u)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] += a[u] * VE[6 * (j + u) + k];
+    }
+    for (; j < m; ++j) {
+      const double a = col[int64_t(j) * ld];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[k] += a * VE[6 * j + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) AC[6 * (3 + i) + k] = acc[k];
+  }
+  __syncthreads();
+  const double T[3][3] = {{RF[9], RF[10], RF[11]}, {0.0, RF[12], RF[13]}, {0.0, 0.0, RF[14]}};
+  // R^-1 (R upper)
+  double Ri[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  Ri[0][0] = 1.0 / RF[3];
+  Ri[1][1] = 1.0 / RF[6];
+  Ri[2][2] = 1.0 / RF[8];
+  Ri[0][1] = -RF[4] * Ri[1][1] / RF[3];
+  Ri[1][2] = -RF[7] * Ri[2][2] / RF[6];
+  Ri[0][2] = -(RF[4] * Ri[1][2] + RF[5] * Ri[2][2]) / RF[3];
+  // row j of F = [R^-T; -C^-1 E R^-T] (u: slot j of AC, the C^-1 E part)
+  auto f_row = [&](int j, const double* u, double* f) {
+    if (j < 3) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) f[k] = Ri[k][j];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) f[k] = -(u[3] * Ri[k][0] + u[4] * Ri[k][1] + u[5] * Ri[k][2]);
+    }
+  };
+  // S = V^T U', G3 = V^T F
+  double part[18];
+#pragma unroll
+  for (int e = 0; e < 18; ++e) part[e] = 0.0;
+  for (int j = tid; j < n; j += kThreads) {
+    const double v[3] = {VW[6 * j], VW[6 * j + 1], VW[6 * j + 2]};
+    const double* u = AC + 6 * j;
+    double f[3];
+    f_row(j, u, f);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        part[3 * k + l] += v[k] * u[l];
+        part[9 + 3 * k + l] += v[k] * f[l];
+      }
+  }
+  block_sum_n<18>(part, red);
+  // Q Z Q^T = Z - A U'^T - U' A^T + A S A^T, S = part[0..8]
+  for (int j = tid; j < n; j += kThreads) {
+    double* u = AC + 6 * j;
+    const double v[3] = {VW[6 * j], VW[6 * j + 1], VW[6 * j + 2]};
+    double a[3], f[3], c[3], y[3];
+    f_row(j, u, f);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a[k] = v[0] * T[0][k] + v[1] * T[1][k] + v[2] * T[2][k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      c[k] = u[k] - 0.5 * (a[0] * part[k] + a[1] * part[3 + k] + a[2] * part[6 + k]);
+      y[k] = f[k] - (a[0] * part[9 + k] + a[1] * part[12 + k] + a[2] * part[15 + k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      u[k] = a[k];
+      u[3 + k] = c[k];
+      Yt[k * ldx + j] = y[k];
+    }
+  }
+  __syncthreads();  // LDS reused by the next pattern
+}
+
+__global__ __launch_bounds__(kThreads) void k_tps_xprep(
+    const int32_t* __restrict__ plist, int npat, const int32_t* __restrict__ pat_n,
+    const int64_t* __restrict__ pmat_off, const double* __restrict__ pmats,
+    const int64_t* __restrict__ pdom_off, const double* __restrict__ prefl,
+    const double* __restrict__ pvwork, const int32_t* __restrict__ pstatus,
+    const int64_t* __restrict__ pxo, double* __restrict__ pinv, double* __restrict__ pac) {
+  const int npu = plist[npat];
+  for (int w = blockIdx.x; w < npu; w += gridDim.x)
+    xprep_pattern(plist[w], pat_n, pmat_off, pmats, pdom_off, prefl, pvwork, pstatus, pxo, pinv,
+                  pac);
+}
+
+// X = Z - A Cc^T - Cc A^T, rows 8 c .. 8 c + 7 (c = w / npu) of pattern
+// plist[w % npu]; X[i][j] and X[j][i] add the same two dot products: X is
+// exactly symmetric
+#pragma clang fp contract(off)
+__global__ __launch_bounds__(kThreads) void k_tps_xfill(
+    const int32_t* __restrict__ plist, int npat, int nchunk, const int32_t* __restrict__ pat_n,
+    const int64_t* __restrict__ pmat_off, const double* __restrict__ pmats,
+    const int64_t* __restrict__ pdom_off, const int32_t* __restrict__ pstatus,
+    const int64_t* __restrict__ pxo, const double* __restrict__ pac, double* __restrict__ pinv) {
+  const int npu = plist[npat];
+  for (int w = blockIdx.x; w < npu * nchunk; w += gridDim.x) {
+  const int p = plist[w % npu];
+  const int n = pat_n[p];
+  const int i0 = 8 * (w / npu);
+  if (n < 4 || pstatus[p] || i0 >= n) continue;
+  const int m = n - 3;
+  const int64_t ld = tps_mat_ld(m);
+  const double* Xc = pmats + pmat_off[p] + tps_mat_e_size(m);
+  const double* AC = pac + 6 * pdom_off[p];
+  const int64_t ldx = tps_x_ld(n);
+  double* X = pinv + pxo[p];
+  for (int i = i0; i < min(i0 + 8, n); ++i) {
+    const double* ri = AC + 6 * i;
+    const double a0 = ri[0], a1 = ri[1], a2 = ri[2], c0 = ri[3], c1 = ri[4], c2 = ri[5];
+    for (int j = threadIdx.x; j < n; j += kThreads) {
+      const double* rj = AC + 6 * j;
+      const double z = (i >= 3 && j >= 3) ? Xc[int64_t(i - 3) * ld + (j - 3)] : 0.0;
+      const double s1 = fma(a2, rj[5], fma(a1, rj[4], a0 * rj[3]));
+      const double s2 = fma(c2, rj[2], fma(c1, rj[1], c0 * rj[0]));
+      X[int64_t(i) * ldx + j] = z - (s1 + s2);
+    }
+  }
+  }
+}
+#pragma clang fp contract(on)
+
+// dynamic LDS bytes of k_tps_shared_x<g>: x (floats) [nm][2g], y [nm][2g],
+// values (floats) [g][2][nm], member nodes [g][nm]
+constexpr size_t shared_x_lds(int g, int nm) {
+  return size_t(nm) * (2 * g * 4 + 2 * g * 8 + 2 * g * 4 + g * 4);
+}
+// subdomains per workgroup of k_tps_shared_x: 4 while two workgroups fit a
+// CU's LDS, else 2, else 1
+constexpr int shared_x_group(int nm) {
+  return shared_x_lds(4, nm) <= 80 * 1024 ? 4 : shared_x_lds(2, nm) <= 80 * 1024 ? 2 : 1;
+}
+
+// The shared subdomains of group gi (up to G of one pattern, consecutive in
+// the shared order) with the explicit inverse: the checks of
+// k_tps_shared_columns, then the outlier iterations of k_tps_shared_iter
+// (thin_plate_spline_subdom.py:236-280), whose solve is y = X v' (v' the
+// values, 0 at the missing nodes), lambda = -H^-1 y[R], w = y + X[:, R]
+// lambda, a = Y^T v' + Y[R, :]^T lambda, for the 2 G components at once.
+// Each component's arithmetic is independent of the others in the group.
+template <int G>
+__device__ void shared_x_solve(
+    int gi, const int32_t* __restrict__ order, const int32_t* __restrict__ groups,
+    const int32_t* __restrict__ dpat, const int32_t* __restrict__ dbox,
+    const int32_t* __restrict__ pat_n, const int32_t* __restrict__ pstatus,
+    const int64_t* __restrict__ pxo, const double* __restrict__ pinv,
+    const double* __restrict__ gx, const double* __restrict__ gy, int ngx,
+    const double* __restrict__ xs, const double* __restrict__ ys,
+    const int32_t* __restrict__ idx, const float* __restrict__ vx,
+    const float* __restrict__ vy, const int64_t* __restrict__ dom_off,
+    const int32_t* __restrict__ slot_pt, const double* __restrict__ smoothing_d,
+    double threshold, double* __restrict__ smooth_slots, int64_t nslots,
+    double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
+    int32_t* __restrict__ status, double* __restrict__ slot_xy, int cap_r, int nm, int nd) {
+  static_assert(G >= 1 && G <= kThreads / 64, "one wavefront per subdomain for its H");
+  constexpr int NV = 2 * G;
+  const int b0 = groups[gi], cnt = groups[nd + 1 + gi];
+  const int tid = threadIdx.x;
+  const int p = dpat[order[b0]];
+  if (pstatus[p]) {
+    if (tid < cnt) status[order[b0 + tid]] = 1;  // as the direct factorisation reports it
+    return;
+  }
+  const int nf = pat_n[p];
+  const int64_t ldx = tps_x_ld(nf);
+  const double* X = pinv + pxo[p];
+  const double* Yt = X + int64_t(nf) * ldx;
+  extern __shared__ double dyn[];
+  float* xv = reinterpret_cast<float*>(dyn);  // [nm][NV]
+  double* yv = dyn + G * nm;                  // [nm][NV]
+  float* val = reinterpret_cast<float*>(dyn + 3 * G * nm);  // [G][2][nm]
+  int* fidx = reinterpret_cast<int*>(dyn + 4 * G * nm);     // [G][nm]
+  int* fmem = reinterpret_cast<int*>(yv);  // [nm], during the checks
+  __shared__ double red[3 * NV * (kThreads / 64)];
+  __shared__ int rl[G][kCapMaxR];
+  __shared__ int nr_sh[G];
+  __shared__ double H[G][kCapMaxR][kCapMaxR + 1];
+  __shared__ double ga[G][kCapMaxR][3];
+  __shared__ double lam[NV][kCapMaxR];
+  __shared__ double wa[NV][3];
+  __shared__ int go[NV];
+  __shared__ int act[G];
+  __shared__ int bad[G];
+  __shared__ double nfixed[NV], maxdiff[NV];
+  __shared__ int count[NV];
+  for (int s = 0; s < G; ++s) {
+    const int d = s < cnt ? order[b0 + s] : -1;
+    bool ok = d >= 0 && status[d] == 0;
+    if (ok) {
+      const int64_t s0 = dom_off[d];
+      const int n = int(dom_off[d + 1] - s0);
+      const int j0 = dbox[4 * d], nc = dbox[4 * d + 1], i0 = dbox[4 * d + 2];
+      if (tid == 0) bad[s] = 0;
+      ok = shared_members(n, nf, s0, slot_pt, idx, xs, ys, gx, gy, ngx, j0, nc, i0,
+                          fidx + s * nm, fmem, &bad[s]);
+      if (ok) {
+        const int nr = shared_missing(nf, fmem, cap_r, rl[s], &nr_sh[s]);
+        ok = nr <= cap_r && nr <= kCapMaxR && n + nr == nf;
+      }
+      if (!ok && tid == 0) status[d] = 2;  // solved again by host sizing
+      if (ok) {
+        float* v0 = val + 2 * s * nm;
+        for (int j = tid; j < n; j += kThreads) {
+          const int c = slot_pt[s0 + j];
+          v0[j] = vx[c];
+          v0[nm + j] = vy[c];
+          slot_xy[s0 + j] = xs[c];
+          slot_xy[nslots + s0 + j] = ys[c];
+        }
+      }
+    }
+    if (tid == 0) act[s] = ok ? 1 : 0;
+    __syncthreads();  // fmem is reused by the next subdomain
+  }
+  // H = X[R, R], G_a = Y[R, :]
+  for (int s = 0; s < G; ++s) {
+    if (!act[s]) continue;
+    const int nr = nr_sh[s];
+    for (int e = tid; e < nr * nr; e += kThreads)
+      H[s][e / nr][e % nr] = X[int64_t(rl[s][e / nr]) * ldx + rl[s][e % nr]];
+    for (int e = tid; e < 3 * nr; e += kThreads)
+      ga[s][e / 3][e % 3] = Yt[int64_t(e % 3) * ldx + rl[s][e / 3]];
+  }
+  __syncthreads();
+  // H = L_H L_H^T in place (lower triangle): wavefront s, one lane
+  if ((tid & 63) == 0 && (tid >> 6) < G && act[tid >> 6]) {
+    const int s = tid >> 6;
+    const int nr = nr_sh[s];
+    bool okh = true;
+    for (int j = 0; j < nr && okh; ++j) {
+      double dj = H[s][j][j];
+      for (int k = 0; k < j; ++k) dj -= H[s][j][k] * H[s][j][k];
+      if (!(dj > 0.0)) {
+        okh = false;
+        break;
+      }
+      dj = sqrt(dj);
+      H[s][j][j] = dj;
+      for (int i = j + 1; i < nr; ++i) {
+        double v = H[s][i][j];
+        for (int k = 0; k < j; ++k) v -= H[s][i][k] * H[s][j][k];
+        H[s][i][j] = v / dj;
+      }
+    }
+    if (!okh) {
+      status[order[b0 + s]] = 1;  // M_S singular, as the direct factorisation reports it
+      act[s] = 0;
+    }
+  }
+  __syncthreads();
+  if (tid < NV) {
+    go[tid] = act[tid >> 1];
+    nfixed[tid] = 0.0;
+    maxdiff[tid] = 0.0;
+    count[tid] = 0;
+  }
+  __syncthreads();
+  for (int it = 0; it < 11; ++it) {
+    bool any = false;
+    for (int q = 0; q < NV; ++q) any = any || go[q] != 0;
+    if (!any) break;
+    // v' of every running component (0 elsewhere)
+    for (int e = tid; e < nf * NV; e += kThreads) xv[e] = 0.0f;
+    __syncthreads();
+    for (int q = 0; q < NV; ++q) {
+      if (!go[q]) continue;
+      const int s = q >> 1;
+      const int d = order[b0 + s];
+      const int n = int(dom_off[d + 1] - dom_off[d]);
+      for (int j = tid; j < n; j += kThreads) xv[fidx[s * nm + j] * NV + q] = val[q * nm + j];
+    }
+    __syncthreads();
+    // y = X v' (X symmetric: lanes over rows i read X by columns)
+    for (int i = tid; i < nf; i += kThreads) {
+      double acc[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+      const double* col = X + i;
+      constexpr int U4 = 4;
+      int j = 0;
+      for (; j + U4 <= nf; j += U4) {
+        double a[U4];
+#pragma unroll
+        for (int u = 0; u < U4; ++u) a[u] = col[int64_t(j + u) * ldx];
+#pragma unroll
+        for (int u = 0; u < U4; ++u)
+#pragma unroll
+          for (int q = 0; q < NV; ++q) acc[q] += a[u] * double(xv[(j + u) * NV + q]);
+      }
+      for (; j < nf; ++j) {
+        const double a = col[int64_t(j) * ldx];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] += a * double(xv[j * NV + q]);
+      }
+#pragma unroll
+      for (int q = 0; q < NV; ++q) yv[i * NV + q] = acc[q];
+    }
+    // Y^T v'
+    for (int k = 0; k < 3; ++k) {
+      double pa[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) pa[q] = 0.0;
+      for (int i = tid; i < nf; i += kThreads) {
+        const double t = Yt[k * ldx + i];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) pa[q] += t * double(xv[i * NV + q]);
+      }
+      block_sum_n<NV>(pa, red);
+      if (tid == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) wa[q][k] = pa[q];
+    }
+    __syncthreads();
+    // lambda = -H^-1 y[R] (forward then backward substitution in place), a
+    if (tid < NV) {
+      const int q = tid, s = q >> 1;
+      if (go[q]) {
+        const int nr = nr_sh[s];
+        for (int i = 0; i < nr; ++i) {
+          double v = -yv[rl[s][i] * NV + q];
+          for (int k = 0; k < i; ++k) v -= H[s][i][k] * lam[q][k];
+          lam[q][i] = v / H[s][i][i];
+        }
+        for (int i = nr - 1; i >= 0; --i) {
+          double v = lam[q][i];
+          for (int k = i + 1; k < nr; ++k) v -= H[s][k][i] * lam[q][k];
+          lam[q][i] = v / H[s][i][i];
+        }
+        for (int t = 0; t < 3; ++t) {
+          double v = wa[q][t];
+          for (int r = 0; r < nr; ++r) v += ga[s][r][t] * lam[q][r];
+          wa[q][t] = v;
+        }
+      }
+    }
+    __syncthreads();
+    // w = y + X[:, R] lambda
+    for (int s = 0; s < G; ++s) {
+      if (!go[2 * s] && !go[2 * s + 1]) continue;
+      const int nr = nr_sh[s];
+      for (int k = tid; k < nf; k += kThreads) {
+        double w0 = yv[k * NV + 2 * s], w1 = yv[k * NV + 2 * s + 1];
+        for (int r = 0; r < nr; ++r) {
+          const double g = X[int64_t(rl[s][r]) * ldx + k];
+          w0 += g * lam[2 * s][r];
+          w1 += g * lam[2 * s + 1][r];
+        }
+        yv[k * NV + 2 * s] = w0;
+        yv[k * NV + 2 * s + 1] = w1;
+      }
+    }
+    __syncthreads();
+    // smoothed values, weights and outliers of every running component:
+    // per-wavefront maxima and counts, combined by thread 0
+    for (int q = 0; q < NV; ++q) {
+      if (!go[q]) continue;
+      const int s = q >> 1, c = q & 1;
+      const int d = order[b0 + s];
+      const int64_t s0 = dom_off[d];
+      const int n = int(dom_off[d + 1] - s0);
+      const double smoothing = smoothing_d[d];
+      double* W = weights + c * wstride + s0 + 3 * int64_t(d);
+      double dm = 0.0, ne = 0.0;
+      for (int j = tid; j < n; j += kThreads) {
+        const double v = double(val[q * nm + j]);
+        const double wq = yv[fidx[s * nm + j] * NV + q];
+        const double sm = v - smoothing * wq;
+        smooth_slots[c * nslots + s0 + j] = sm;
+        const double diff = fabs(sm - v);
+        dm = fmax(dm, diff);
+        if (threshold >= 0.0 && diff > threshold) ne += 1.0;
+        W[j] = wq;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        dm = fmax(dm, __shfl_xor(dm, off, 64));
+        ne += __shfl_xor(ne, off, 64);
+      }
+      if ((tid & 63) == 0) {
+        red[q * (kThreads / 64) + (tid >> 6)] = dm;
+        red[(NV + q) * (kThreads / 64) + (tid >> 6)] = ne;
+      }
+      if (tid == 0) {
+        // P = [1, y - y0, x - x0] -> [1, y, x]
+        const double x0 = gx[dbox[4 * d]], y0 = gy[dbox[4 * d + 2]];
+        W[n] = wa[q][0] - wa[q][1] * y0 - wa[q][2] * x0;
+        W[n + 1] = wa[q][1];
+        W[n + 2] = wa[q][2];
+      }
+    }
+    __syncthreads();
+    // go[q] becomes whether component q iterates again
+    if (tid == 0)
+      for (int q = 0; q < NV; ++q) {
+        if (!go[q]) continue;
+        double dm = red[q * (kThreads / 64)], ne = red[(NV + q) * (kThreads / 64)];
+        for (int k = 1; k < kThreads / 64; ++k) {
+          dm = fmax(dm, red[q * (kThreads / 64) + k]);
+          ne += red[(NV + q) * (kThreads / 64) + k];
+        }
+        count[q] += 1;
+        if (count[q] == 1) maxdiff[q] = dm;
+        const bool more = threshold >= 0.0 && ne > 0.0 && count[q] < 11;
+        if (more) nfixed[q] += ne;
+        go[q] = more ? 1 : 0;
+      }
+    __syncthreads();
+    // the outliers take their smoothed values
+    for (int q = 0; q < NV; ++q) {
+      if (!go[q]) continue;
+      const int s = q >> 1;
+      const int d = order[b0 + s];
+      const int n = int(dom_off[d + 1] - dom_off[d]);
+      const double smoothing = smoothing_d[d];
+      for (int j = tid; j < n; j += kThreads) {
+        const double v = double(val[q * nm + j]);
+        const double sm = v - smoothing * yv[fidx[s * nm + j] * NV + q];
+        if (fabs(sm - v) > threshold) val[q * nm + j] = float(sm);
+      }
+    }
+  }
+  if (tid == 0) {
+#pragma unroll
+    for (int s = 0; s < G; ++s) {
+      if (!act[s]) continue;
+      const int d = order[b0 + s];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        stats[6 * d + 3 * c + 0] = nfixed[2 * s + c];
+        stats[6 * d + 3 * c + 1] = maxdiff[2 * s + c];
+        stats[6 * d + 3 * c + 2] = count[2 * s + c];
+      }
+    }
+  }
+  __syncthreads();  // LDS reused by the next group
+}
+
+// the groups (groups[nd] of them), XCD-contiguous ranges of them per
+// workgroup stride: a fixed grid instead of one workgroup per subdomain slot
+template <int G>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_tps_shared_x(
+    int nd, const int32_t* __restrict__ order, const int32_t* __restrict__ groups,
+    const int32_t* __restrict__ dpat, const int32_t* __restrict__ dbox,
+    const int32_t* __restrict__ pat_n, const int32_t* __restrict__ pstatus,
+    const int64_t* __restrict__ pxo, const double* __restrict__ pinv,
+    const double* __restrict__ gx, const double* __restrict__ gy, int ngx,
+    const double* __restrict__ xs, const double* __restrict__ ys,
+    const int32_t* __restrict__ idx, const float* __restrict__ vx,
+    const float* __restrict__ vy, const int64_t* __restrict__ dom_off,
+    const int32_t* __restrict__ slot_pt, const double* __restrict__ smoothing_d,
+    double threshold, double* __restrict__ smooth_slots, int64_t nslots,
+    double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
+    int32_t* __restrict__ status, double* __restrict__ slot_xy, int cap_r, int nm) {
+  const int ng = groups[nd];
+  const int ngrid = int(gridDim.x) & ~7;  // a multiple of the XCD count
+  if (int(blockIdx.x) >= ngrid) return;
+  for (int w = blockIdx.x; w < ng; w += ngrid)
+    shared_x_solve<G>(xcd_rank(w, ng), order, groups, dpat, dbox, pat_n, pstatus, pxo, pinv, gx,
+                      gy, ngx, xs, ys, idx, vx, vy, dom_off, slot_pt, smoothing_d, threshold,
+                      smooth_slots, nslots, weights, wstride, stats, status, slot_xy, cap_r, nm,
+                      nd);
 }
 
 // ---------------------------------------------------------------------------
@@ -2961,6 +3786,7 @@ InterpWork interp_work(void* base, int npts, int nd) {
 }
 
 std::atomic<int> g_chol_threads{0};
+std::atomic<int> g_shared_solve{0};
 
 // compute units of the current device (cached per device)
 int cu_count() {
@@ -3106,9 +3932,10 @@ void launch_solve(int32_t nd, int32_t n_max, const float* vx, const float* vy,
 
 // workspace of piv_tps_smooth_shared (offsets in piv_tps_shared_workspace)
 struct SharedWork {
-  int32_t *dmode, *dpat, *dbox, *pat_n, *pat_box, *pslot, *pstatus, *order_direct, *order_shared;
-  int64_t *pdom_off, *pmat_off;
-  double *pat_smooth, *pxy, *pmats, *prefl, *pvwork, *pslot_xy, *gmat, *hfac, *gpoly;
+  int32_t *dmode, *dpat, *dbox, *pat_n, *pat_box, *pslot, *pstatus, *order_direct, *order_shared,
+      *groups, *plist;
+  int64_t *pdom_off, *pmat_off, *pxo;
+  double *pat_smooth, *pxy, *pmats, *pinv, *pac, *prefl, *pvwork, *pslot_xy, *gmat, *hfac, *gpoly;
   int64_t pxy_stride;
   size_t bytes;
 };
@@ -3130,12 +3957,15 @@ SharedWork shared_work(void* base, int nfields, int dom_cap, int npat_cap, int n
   w.dbox = reinterpret_cast<int32_t*>(take(16 * nd));
   w.order_direct = reinterpret_cast<int32_t*>(take(4 * nd));
   w.order_shared = reinterpret_cast<int32_t*>(take(4 * (nd + 1)));  // + their count
+  w.groups = reinterpret_cast<int32_t*>(take(4 * 2 * (nd + 1)));
   w.pat_n = reinterpret_cast<int32_t*>(take(4 * np));
   w.pat_box = reinterpret_cast<int32_t*>(take(16 * np));
   w.pstatus = reinterpret_cast<int32_t*>(take(4 * np));
   w.pslot = reinterpret_cast<int32_t*>(take(4 * pts));
   w.pdom_off = reinterpret_cast<int64_t*>(take(8 * (np + 1)));
   w.pmat_off = reinterpret_cast<int64_t*>(take(8 * (np + 1)));
+  w.pxo = reinterpret_cast<int64_t*>(take(8 * (np + 1)));
+  w.plist = reinterpret_cast<int32_t*>(take(4 * (np + 1)));
   w.pat_smooth = reinterpret_cast<double*>(take(8 * np));
   w.pxy_stride = int64_t(pts);
   w.pxy = reinterpret_cast<double*>(take(8 * 2 * pts));
@@ -3143,6 +3973,10 @@ SharedWork shared_work(void* base, int nfields, int dom_cap, int npat_cap, int n
   w.prefl = reinterpret_cast<double*>(take(8 * 16 * np));
   w.pvwork = reinterpret_cast<double*>(take(8 * 6 * pts));
   w.pmats = reinterpret_cast<double*>(take(8 * np * size_t(tps_mat_size(n_max))));
+  // the patterns' Linv (matrix layout), then their X and Y^T
+  w.pinv = reinterpret_cast<double*>(
+      take(8 * np * size_t(std::max(tps_mat_size(n_max), tps_x_size(n_max)))));
+  w.pac = reinterpret_cast<double*>(take(8 * 6 * pts));
   w.gmat = reinterpret_cast<double*>(take(8 * nd * nm * size_t(cap_r > 0 ? cap_r : 1)));
   w.hfac = reinterpret_cast<double*>(take(8 * nd * kCapMaxR * kCapMaxR));
   w.gpoly = reinterpret_cast<double*>(take(8 * nd * kCapMaxR * 3));
@@ -3160,6 +3994,8 @@ __global__ void k_tps_zero_i32(int n, int32_t* a) {
 namespace fia {
 int tps_chol_threads() { return g_chol_threads.load(std::memory_order_relaxed); }
 void set_tps_chol_threads(int threads) { g_chol_threads.store(threads, std::memory_order_relaxed); }
+int tps_shared_solve() { return g_shared_solve.load(std::memory_order_relaxed); }
+void set_tps_shared_solve(int mode) { g_shared_solve.store(mode, std::memory_order_relaxed); }
 }  // namespace fia
 
 extern "C" {
@@ -3211,8 +4047,7 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   if (npat_cap <= 0 || npat_cap > kPatMaxPerField)
     return tps_fail(PIV_EINVAL, "npat_cap must be in [1, 64]");
   if (cap_r < 0 || cap_r > kCapMaxR) return tps_fail(PIV_EINVAL, "cap_r must be in [0, 12]");
-  if ((size_t(kSharedNV + 1) * ((n_max + 1) & ~1) + solve_scratch(kSharedNV)) * sizeof(double) >
-      120 * 1024)
+  if (shared_columns_lds(kSharedNV, (n_max + 1) & ~1) > 120 * 1024)
     return tps_fail(PIV_EUNSUPPORTED, "n_max too large for the shared solve's LDS");
   if (int64_t(nfields) * dom_cap > INT32_MAX / 4 || int64_t(nfields) * npat_cap > 1 << 20)
     return tps_fail(PIV_EUNSUPPORTED, "TPS batch too large");
@@ -3233,8 +4068,9 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
                        int(ngy), int(irr_y), int(cap_r), int(n_max), int(npat_cap), w.dmode,
                        w.dpat, w.dbox, w.pat_n, w.pat_box, w.pat_smooth);
     hipLaunchKernelGGL(k_tps_pattern_layout, dim3(1), dim3(1024), 0, s, npat, w.pat_n,
-                       w.pat_box, gx, gy, nd, order, w.dmode, w.pdom_off, w.pmat_off, w.pxy,
-                       w.pslot, w.pxy_stride, w.order_direct, w.order_shared);
+                       w.pat_box, gx, gy, nd, order, w.dmode, w.dpat, shared_x_group(nm),
+                       w.pdom_off, w.pmat_off, w.pxo, w.pxy, w.pslot, w.pxy_stride,
+                       w.order_direct, w.order_shared, w.groups, w.plist);
     hipLaunchKernelGGL(k_tps_zero_i32, dim3((npat + 255) / 256), dim3(256), 0, s, npat,
                        w.pstatus);
   }
@@ -3251,23 +4087,54 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   launch_factor(npat, n_max, w.pxy, w.pxy + w.pxy_stride, w.pdom_off, w.pslot, w.pmat_off,
                 w.pat_smooth, w.pmats, w.prefl, w.pvwork, w.pslot_xy, w.pxy_stride, w.pstatus,
                 nullptr, lat, gen, s);
-  // subdomains missing at most 4 nodes (and every check), then the others
+  if (g_shared_solve.load(std::memory_order_relaxed) != 1) {
+    // the explicit inverses of the patterns, then the shared subdomains in
+    // groups of one pattern
+    const int nblk = (n_max - 3 + kNB - 1) / kNB;
+    const int64_t nitem = int64_t(npat) * nblk * (nblk + 1) / 2;
+    auto grid = [&](int64_t per_cu) { return dim3(unsigned(std::min<int64_t>(nitem, 256 * per_cu))); };
+    hipLaunchKernelGGL(k_tps_tri_inverse, grid(4), dim3(kThreads), 0, s, w.plist, npat, nblk,
+                       w.pat_n, w.pmat_off, w.pmats, w.pstatus, w.pinv);
+    hipLaunchKernelGGL(k_tps_inv_gram, grid(6), dim3(kThreads), 0, s, w.plist, npat,
+                       nblk * (nblk + 1) / 2, w.pat_n, w.pmat_off, w.pmats, w.pstatus, w.pinv);
+    hipLaunchKernelGGL(k_tps_xprep, dim3(unsigned(std::min(npat, 256))), dim3(kThreads),
+                       size_t(6) * nm * sizeof(double), s, w.plist, npat, w.pat_n, w.pmat_off,
+                       w.pmats, w.pdom_off, w.prefl, w.pvwork, w.pstatus, w.pxo, w.pinv, w.pac);
+    hipLaunchKernelGGL(k_tps_xfill, grid(8), dim3(kThreads), 0, s, w.plist, npat,
+                       (n_max + 7) / 8, w.pat_n, w.pmat_off, w.pmats, w.pdom_off, w.pstatus, w.pxo,
+                       w.pac, w.pinv);
+#define FIA_SHARED_X(G)                                                                           \
+  hipLaunchKernelGGL(k_tps_shared_x<G>, dim3(std::max(8, std::min(nd, 2048))), dim3(kThreads),    \
+                     shared_x_lds(G, nm), s, nd, w.order_shared, w.groups, w.dpat, w.dbox,        \
+                     w.pat_n, w.pstatus, w.pxo, w.pinv,                                           \
+                     gx, gy, int(ngx), xs, ys, idx, vx, vy, dom_off, slot_pt, smoothing,          \
+                     threshold, smooth_slots, nslots, weights, wstride, stats, status, slot_xy,   \
+                     int(cap_r), nm)
+    switch (shared_x_group(nm)) {
+      case 4: FIA_SHARED_X(4); break;
+      case 2: FIA_SHARED_X(2); break;
+      default: FIA_SHARED_X(1); break;
+    }
+#undef FIA_SHARED_X
+    return tps_check("piv_tps_smooth_shared");
+  }
+  // by blocked substitution with the patterns' factors: subdomains missing
+  // at most 4 nodes (and every check), then the others
   hipLaunchKernelGGL(k_tps_shared_columns<4>, dim3(nd), dim3(kThreads),
-                     (size_t(4 + 1) * nm + solve_scratch(4)) * sizeof(double), s,
-                     w.order_shared, w.dpat, w.dbox, w.pat_n, w.pdom_off, w.pmat_off, w.pmats,
-                     w.prefl, w.pvwork, w.pstatus, gx, gy, int(ngx), xs, ys, idx, dom_off,
-                     slot_pt, status, w.gmat, w.hfac, w.gpoly, int(cap_r), nm);
+                     shared_columns_lds(4, nm), s, w.order_shared, w.dpat, w.dbox, w.pat_n,
+                     w.pdom_off, w.pmat_off, w.pmats, w.prefl, w.pvwork, w.pstatus, gx, gy,
+                     int(ngx), xs, ys, idx, dom_off, slot_pt, status, w.gmat, w.hfac, w.gpoly,
+                     int(cap_r), nm);
   hipLaunchKernelGGL(k_tps_shared_columns<kSharedNV>, dim3(nd), dim3(kThreads),
-                     (size_t(kSharedNV + 1) * nm + solve_scratch(kSharedNV)) * sizeof(double), s,
-                     w.order_shared, w.dpat, w.dbox, w.pat_n, w.pdom_off, w.pmat_off, w.pmats,
-                     w.prefl, w.pvwork, w.pstatus, gx, gy, int(ngx), xs, ys, idx, dom_off,
-                     slot_pt, status, w.gmat, w.hfac, w.gpoly, int(cap_r), nm);
-  hipLaunchKernelGGL(k_tps_shared_iter, dim3(nd), dim3(kThreads),
-                     (size_t(kIterNV + 2) * nm + solve_scratch(kIterNV)) * sizeof(double), s,
-                     w.order_shared, w.dpat, w.dbox, w.pat_n, w.pdom_off, w.pmat_off, w.pmats,
-                     w.prefl, w.pvwork, gx, gy, int(ngx), xs, ys, idx, vx, vy, dom_off, slot_pt,
-                     smoothing, threshold, smooth_slots, nslots, weights, wstride, stats, status,
-                     slot_xy, w.gmat, w.hfac, w.gpoly, int(cap_r), nm);
+                     shared_columns_lds(kSharedNV, nm), s, w.order_shared, w.dpat, w.dbox,
+                     w.pat_n, w.pdom_off, w.pmat_off, w.pmats, w.prefl, w.pvwork, w.pstatus, gx,
+                     gy, int(ngx), xs, ys, idx, dom_off, slot_pt, status, w.gmat, w.hfac,
+                     w.gpoly, int(cap_r), nm);
+  hipLaunchKernelGGL(k_tps_shared_iter, dim3(nd), dim3(kThreads), shared_iter_lds(kIterNV, nm),
+                     s, w.order_shared, w.dpat, w.dbox, w.pat_n, w.pdom_off, w.pmat_off,
+                     w.pmats, w.prefl, w.pvwork, gx, gy, int(ngx), xs, ys, idx, vx, vy, dom_off,
+                     slot_pt, smoothing, threshold, smooth_slots, nslots, weights, wstride, stats,
+                     status, slot_xy, w.gmat, w.hfac, w.gpoly, int(cap_r), nm);
   return tps_check("piv_tps_smooth_shared");
 }
 

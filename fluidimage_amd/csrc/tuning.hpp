@@ -7,5 +7,7 @@ namespace fia {
 
 int tps_chol_threads();
 void set_tps_chol_threads(int threads);
+int tps_shared_solve();
+void set_tps_shared_solve(int mode);
 
 }  // namespace fia

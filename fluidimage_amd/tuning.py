@@ -35,6 +35,7 @@ from . import _native
 
 CORRELATE_WPB = 1
 TPS_CHOL_THREADS = 2
+TPS_SHARED_SOLVE = 3
 DEFAULT_SHAPES = (16, 24, 32, 48, 64)
 
 
@@ -70,11 +71,15 @@ def apply(table, lib=None):
     if "tps_chol_threads" in table:
         _native.check(lib.piv_tune_set(TPS_CHOL_THREADS, 0, 0, int(table["tps_chol_threads"])),
                       "piv_tune_set")
+    if "tps_shared_solve" in table:
+        _native.check(lib.piv_tune_set(TPS_SHARED_SOLVE, 0, 0, int(table["tps_shared_solve"])),
+                      "piv_tune_set")
 
 
 def current():
     """The values in use, in the table format."""
-    table = {"correlate_wpb": {}, "tps_chol_threads": get(TPS_CHOL_THREADS)}
+    table = {"correlate_wpb": {}, "tps_chol_threads": get(TPS_CHOL_THREADS),
+             "tps_shared_solve": get(TPS_SHARED_SOLVE)}
     for n in DEFAULT_SHAPES:
         table["correlate_wpb"][f"{n}x{n}"] = get(CORRELATE_WPB, n, n)
     return table

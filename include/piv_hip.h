@@ -119,19 +119,27 @@ int piv_shape_supported(int32_t n0, int32_t n1);
 /* ---- launch tuning (no counterpart in the reference, whose FFTW plans
  *      are chosen by FFTW_MEASURE, fft.py:221-234) ----
  * Tunable launch parameters: they change how work is mapped onto the GPU,
- * never the results.
+ * not the results (but for PIV_TUNE_TPS_SHARED_SOLVE, in rounding).
  * PIV_TUNE_CORRELATE_WPB: windows per workgroup of the fused correlation
  *   kernel of window n0 x n1 (the candidates are the variants built);
  * PIV_TUNE_TPS_CHOL_THREADS: threads per subdomain of the TPS Cholesky
  *   for launches of more than two subdomains per compute unit, 256 or 512
  *   (0: 256); smaller launches take 512 up to one subdomain per CU and 256
- *   up to two, so that they run in one round (n0, n1 ignored).
+ *   up to two, so that they run in one round (n0, n1 ignored);
+ * PIV_TUNE_TPS_SHARED_SOLVE: how the subdomains that share a pattern's
+ *   factorisation are solved: 0 with the explicit inverse of the pattern's
+ *   saddle-point matrix (built once per pattern, then one product per
+ *   outlier iteration for up to four subdomains at once), 1 by blocked
+ *   triangular substitution with the pattern's Cholesky factor (n0, n1
+ *   ignored). The two differ in rounding only, the one tunable whose
+ *   results are not bit-identical.
  * piv_tune_candidates writes at most cap values and returns their number
  * (negative status on error); piv_tune_get returns the value in use
  * (negative status on error); piv_tune_set returns a status. The setting is
  * process-wide and read at each launch. */
 #define PIV_TUNE_CORRELATE_WPB 1
 #define PIV_TUNE_TPS_CHOL_THREADS 2
+#define PIV_TUNE_TPS_SHARED_SOLVE 3
 int piv_tune_candidates(int32_t param, int32_t n0, int32_t n1, int32_t* values, int32_t cap);
 int piv_tune_get(int32_t param, int32_t n0, int32_t n1);
 int piv_tune_set(int32_t param, int32_t n0, int32_t n1, int32_t value);

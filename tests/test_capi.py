@@ -121,6 +121,9 @@ def test_tuning_interface_without_gpu():
     assert tuning.candidates(tuning.TPS_CHOL_THREADS) == [0, 256, 512]
     with pytest.raises(RuntimeError, match="0, 256 or 512"):
         tuning.set_value(tuning.TPS_CHOL_THREADS, 0, 0, 128)
+    assert tuning.candidates(tuning.TPS_SHARED_SOLVE) == [0, 1]
+    with pytest.raises(RuntimeError, match="0 or 1"):
+        tuning.set_value(tuning.TPS_SHARED_SOLVE, 0, 0, 2)
     lib = _native.load()
     assert lib.piv_tune_get(99, 0, 0) < 0
 

@@ -188,14 +188,28 @@ def _host_sized_init(cls):
     return init
 
 
+@pytest.mark.parametrize("solve", [0, 1], ids=["explicit", "substitution"])
 @pytest.mark.parametrize("holes, big_hole", [(0.01, False), (0.03, True)])
-def test_shared_factorisations_match_oracle(holes, big_hole):
+def test_shared_factorisations_match_oracle(holes, big_hole, solve):
     """Centers on a grid with missing nodes: subdomains of one node pattern
     share a factorisation and correct for their missing nodes (a large hole
     sends its subdomains to their own factorisation); smoothed and
     interpolated values within 1e-6 px of the oracle's LAPACK solves, outlier
     iteration counts identical, and within 1e-9 px of every subdomain
-    factorised alone."""
+    factorised alone; with the shared solves by the explicit inverse of the
+    pattern's saddle-point matrix (the default) and by blocked
+    substitution."""
+    from fluidimage_amd import tuning
+
+    before = tuning.get(tuning.TPS_SHARED_SOLVE)
+    tuning.set_value(tuning.TPS_SHARED_SOLVE, 0, 0, solve)
+    try:
+        _check_shared_factorisations(holes, big_hole, solve)
+    finally:
+        tuning.set_value(tuning.TPS_SHARED_SOLVE, 0, 0, before)
+
+
+def _check_shared_factorisations(holes, big_hole, solve):
     import torch
 
     from fluidimage_amd.calcul.tps import lattice_step
@@ -204,7 +218,7 @@ def test_shared_factorisations_match_oracle(holes, big_hole):
     from oracle.tps import ThinPlateSplineSubdom
     from tests.test_gpu_tps import _grid_field
 
-    print("case: shared factorisations", holes, big_hole, flush=True)
+    print("case: shared factorisations", holes, big_hole, solve, flush=True)
     rng = np.random.default_rng(61)
     dev = torch.device("cuda")
     shape, step = (640, 720), 8
