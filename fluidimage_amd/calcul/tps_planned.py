@@ -55,7 +55,7 @@ MARGIN_EXTENT = 1.35
 # per field, and missing nodes of a box solved by the capacitance correction
 # (beyond: the subdomain is factorised alone)
 SHARED_PATTERNS = 32
-SHARED_MAX_MISSING = 12
+SHARED_MAX_MISSING = 32
 # sharing pays where a field has many subdomains per pattern and the batch
 # many more subdomains than the GPU runs at once (the patterns' factorisations
 # come first, one workgroup each): measured on fields of 2048 x 2048 px

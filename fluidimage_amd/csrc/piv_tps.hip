@@ -2052,7 +2052,8 @@ __device__ __forceinline__ int xcd_rank(int b, int nb) {
   return x * q + min(x, r) + j;
 }
 
-constexpr int kCapMaxR = 12;     // missing nodes of a subdomain solved with a shared factor
+constexpr int kCapMaxR = 12;  // missing nodes of a subdomain solved with a shared factor
+constexpr int kCapX = 32;     // ... with a pattern's explicit inverse
 constexpr int kPatMaxPerField = 64;
 
 // first index in [0, n) with a[i] >= v (a ascending), n if none
@@ -2329,13 +2330,15 @@ __device__ bool shared_members(int n, int nf, int64_t s0, const int32_t* slot_pt
   return *bad == 0;
 }
 
-// the missing nodes in node order (at most kCapMaxR listed); their number
-__device__ int shared_missing(int nf, const int* fmem, int cap_r, int* rlist, int* nr_sh) {
+// the missing nodes in node order (at most `list` of them listed); their
+// number
+__device__ int shared_missing(int nf, const int* fmem, int cap_r, int* rlist, int* nr_sh,
+                              int list = kCapMaxR) {
   if (threadIdx.x == 0) {
     int r = 0;
     for (int k = 0; k < nf && r <= cap_r; ++k)
       if (fmem[k] < 0) {
-        if (r < kCapMaxR) rlist[r] = k;
+        if (r < list) rlist[r] = k;
         ++r;
       }
     *nr_sh = r;
@@ -3025,10 +3028,14 @@ __global__ __launch_bounds__(kThreads) void k_tps_xfill(
 }
 #pragma clang fp contract(on)
 
-// dynamic LDS bytes of k_tps_shared_x<g>: x (floats) [nm][2g], y [nm][2g],
-// values (floats) [g][2][nm], member nodes [g][nm]
+// dynamic LDS of k_tps_shared_x<g>: x (floats) [nm][2g] and y [nm][2g]
+// (first the matrices H of the subdomains, kCapX (kCapX + 1) doubles each),
+// then the values (floats) [g][2][nm] and member nodes [g][nm]
+constexpr int shared_x_head(int g, int nm) {
+  return 3 * g * nm > g * kCapX * (kCapX + 1) ? 3 * g * nm : g * kCapX * (kCapX + 1);
+}
 constexpr size_t shared_x_lds(int g, int nm) {
-  return size_t(nm) * (2 * g * 4 + 2 * g * 8 + 2 * g * 4 + g * 4);
+  return (size_t(shared_x_head(g, nm)) + size_t(g) * nm) * sizeof(double) + size_t(g) * nm * 4;
 }
 // subdomains per workgroup of k_tps_shared_x: 4 while two workgroups fit a
 // CU's LDS, else 2, else 1
@@ -3045,7 +3052,8 @@ constexpr int shared_x_group(int nm) {
 // Each component's arithmetic is independent of the others in the group.
 template <int G>
 __device__ void shared_x_solve(
-    int gi, const int32_t* __restrict__ order, const int32_t* __restrict__ groups,
+    int gi, double* __restrict__ hinv, const int32_t* __restrict__ order,
+    const int32_t* __restrict__ groups,
     const int32_t* __restrict__ dpat, const int32_t* __restrict__ dbox,
     const int32_t* __restrict__ pat_n, const int32_t* __restrict__ pstatus,
     const int64_t* __restrict__ pxo, const double* __restrict__ pinv,
@@ -3073,15 +3081,15 @@ __device__ void shared_x_solve(
   extern __shared__ double dyn[];
   float* xv = reinterpret_cast<float*>(dyn);  // [nm][NV]
   double* yv = dyn + G * nm;                  // [nm][NV]
-  float* val = reinterpret_cast<float*>(dyn + 3 * G * nm);  // [G][2][nm]
-  int* fidx = reinterpret_cast<int*>(dyn + 4 * G * nm);     // [G][nm]
+  float* val = reinterpret_cast<float*>(dyn + shared_x_head(G, nm));   // [G][2][nm]
+  int* fidx = reinterpret_cast<int*>(dyn + shared_x_head(G, nm) + G * nm);  // [G][nm]
   int* fmem = reinterpret_cast<int*>(yv);  // [nm], during the checks
   __shared__ double red[3 * NV * (kThreads / 64)];
-  __shared__ int rl[G][kCapMaxR];
+  __shared__ int rl[G][kCapX];
   __shared__ int nr_sh[G];
-  __shared__ double H[G][kCapMaxR][kCapMaxR + 1];
-  __shared__ double ga[G][kCapMaxR][3];
-  __shared__ double lam[NV][kCapMaxR];
+  __shared__ int dom[G];
+  __shared__ double ga[G][kCapX][3];
+  __shared__ double lam[NV][kCapX];
   __shared__ double wa[NV][3];
   __shared__ int go[NV];
   __shared__ int act[G];
@@ -3099,8 +3107,8 @@ __device__ void shared_x_solve(
       ok = shared_members(n, nf, s0, slot_pt, idx, xs, ys, gx, gy, ngx, j0, nc, i0,
                           fidx + s * nm, fmem, &bad[s]);
       if (ok) {
-        const int nr = shared_missing(nf, fmem, cap_r, rl[s], &nr_sh[s]);
-        ok = nr <= cap_r && nr <= kCapMaxR && n + nr == nf;
+        const int nr = shared_missing(nf, fmem, cap_r, rl[s], &nr_sh[s], kCapX);
+        ok = nr <= cap_r && nr <= kCapX && n + nr == nf;
       }
       if (!ok && tid == 0) status[d] = 2;  // solved again by host sizing
       if (ok) {
@@ -3114,41 +3122,61 @@ __device__ void shared_x_solve(
         }
       }
     }
-    if (tid == 0) act[s] = ok ? 1 : 0;
+    if (tid == 0) {
+      act[s] = ok ? 1 : 0;
+      dom[s] = d;
+    }
     __syncthreads();  // fmem is reused by the next subdomain
   }
-  // H = X[R, R], G_a = Y[R, :]
+  // H = X[R, R] (into the free LDS of x and y, nr x (kCapX + 1) per
+  // subdomain), G_a = Y[R, :]
+  double* Hs = dyn;
   for (int s = 0; s < G; ++s) {
     if (!act[s]) continue;
     const int nr = nr_sh[s];
+    double* Hm = Hs + s * kCapX * (kCapX + 1);
     for (int e = tid; e < nr * nr; e += kThreads)
-      H[s][e / nr][e % nr] = X[int64_t(rl[s][e / nr]) * ldx + rl[s][e % nr]];
+      Hm[(e / nr) * (kCapX + 1) + e % nr] = X[int64_t(rl[s][e / nr]) * ldx + rl[s][e % nr]];
     for (int e = tid; e < 3 * nr; e += kThreads)
       ga[s][e / 3][e % 3] = Yt[int64_t(e % 3) * ldx + rl[s][e / 3]];
   }
   __syncthreads();
-  // H = L_H L_H^T in place (lower triangle): wavefront s, one lane
-  if ((tid & 63) == 0 && (tid >> 6) < G && act[tid >> 6]) {
-    const int s = tid >> 6;
+  // H^-1 by Gauss-Jordan elimination in place (H is symmetric positive
+  // definite exactly when every pivot is positive), wavefront s, lanes over
+  // rows; then to hinv (rows of kCapX doubles)
+  if ((tid >> 6) < G && act[tid >> 6]) {
+    const int s = tid >> 6, lane = tid & 63;
     const int nr = nr_sh[s];
+    double* Hm = Hs + s * kCapX * (kCapX + 1);
     bool okh = true;
-    for (int j = 0; j < nr && okh; ++j) {
-      double dj = H[s][j][j];
-      for (int k = 0; k < j; ++k) dj -= H[s][j][k] * H[s][j][k];
-      if (!(dj > 0.0)) {
+    for (int k = 0; k < nr; ++k) {
+      const double piv = Hm[k * (kCapX + 1) + k];
+      if (!(piv > 0.0)) {
         okh = false;
         break;
       }
-      dj = sqrt(dj);
-      H[s][j][j] = dj;
-      for (int i = j + 1; i < nr; ++i) {
-        double v = H[s][i][j];
-        for (int k = 0; k < j; ++k) v -= H[s][i][k] * H[s][j][k];
-        H[s][i][j] = v / dj;
+      const double rp = 1.0 / piv;
+      double rk = 0.0;
+      if (lane < nr) rk = lane == k ? rp : Hm[k * (kCapX + 1) + lane] * rp;
+      wave_sync();
+      if (lane < nr) Hm[k * (kCapX + 1) + lane] = rk;
+      wave_sync();
+      if (lane < nr && lane != k) {
+        double* hi = Hm + lane * (kCapX + 1);
+        const double f = hi[k];
+        for (int j = 0; j < nr; ++j)
+          hi[j] = j == k ? -f * rp : hi[j] - f * Hm[k * (kCapX + 1) + j];
       }
+      wave_sync();
     }
-    if (!okh) {
-      status[order[b0 + s]] = 1;  // M_S singular, as the direct factorisation reports it
+    if (okh) {
+      double* hv = hinv + int64_t(dom[s]) * kCapX * kCapX;
+      for (int e = lane; e < nr * kCapX; e += 64) {
+        const int i = e / kCapX, j = e % kCapX;
+        if (j < nr) hv[e] = Hm[i * (kCapX + 1) + j];
+      }
+    } else if (lane == 0) {
+      status[dom[s]] = 1;  // M_S singular, as the direct factorisation reports it
       act[s] = 0;
     }
   }
@@ -3175,30 +3203,44 @@ __device__ void shared_x_solve(
       for (int j = tid; j < n; j += kThreads) xv[fidx[s * nm + j] * NV + q] = val[q * nm + j];
     }
     __syncthreads();
-    // y = X v' (X symmetric: lanes over rows i read X by columns)
-    for (int i = tid; i < nf; i += kThreads) {
-      double acc[NV];
+    // y = X v' (X symmetric: lanes over row pairs i, i + 1 read X by
+    // columns, 16 bytes a load, kU loads in flight; ldx is even and rows
+    // i >= nf of the pair read the padding of X's rows)
+    for (int i = 2 * tid; i < nf; i += 2 * kThreads) {
+      double acc0[NV], acc1[NV];
 #pragma unroll
-      for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+      for (int q = 0; q < NV; ++q) acc0[q] = acc1[q] = 0.0;
       const double* col = X + i;
-      constexpr int U4 = 4;
+      constexpr int kU = 8;
       int j = 0;
-      for (; j + U4 <= nf; j += U4) {
-        double a[U4];
+      for (; j + kU <= nf; j += kU) {
+        double2 a[kU];
 #pragma unroll
-        for (int u = 0; u < U4; ++u) a[u] = col[int64_t(j + u) * ldx];
+        for (int u = 0; u < kU; ++u)
+          a[u] = *reinterpret_cast<const double2*>(col + int64_t(j + u) * ldx);
 #pragma unroll
-        for (int u = 0; u < U4; ++u)
+        for (int u = 0; u < kU; ++u)
 #pragma unroll
-          for (int q = 0; q < NV; ++q) acc[q] += a[u] * double(xv[(j + u) * NV + q]);
+          for (int q = 0; q < NV; ++q) {
+            const double xq = double(xv[(j + u) * NV + q]);
+            acc0[q] += a[u].x * xq;
+            acc1[q] += a[u].y * xq;
+          }
       }
       for (; j < nf; ++j) {
-        const double a = col[int64_t(j) * ldx];
+        const double2 a = *reinterpret_cast<const double2*>(col + int64_t(j) * ldx);
 #pragma unroll
-        for (int q = 0; q < NV; ++q) acc[q] += a * double(xv[j * NV + q]);
+        for (int q = 0; q < NV; ++q) {
+          const double xq = double(xv[j * NV + q]);
+          acc0[q] += a.x * xq;
+          acc1[q] += a.y * xq;
+        }
       }
 #pragma unroll
-      for (int q = 0; q < NV; ++q) yv[i * NV + q] = acc[q];
+      for (int q = 0; q < NV; ++q) yv[i * NV + q] = acc0[q];
+      if (i + 1 < nf)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) yv[(i + 1) * NV + q] = acc1[q];
     }
     // Y^T v'
     for (int k = 0; k < 3; ++k) {
@@ -3216,26 +3258,25 @@ __device__ void shared_x_solve(
         for (int q = 0; q < NV; ++q) wa[q][k] = pa[q];
     }
     __syncthreads();
-    // lambda = -H^-1 y[R] (forward then backward substitution in place), a
-    if (tid < NV) {
+    // lambda = -H^-1 y[R]: thread (q, r)
+    for (int e = tid; e < NV * kCapX; e += kThreads) {
+      const int q = e / kCapX, r = e % kCapX, s = q >> 1;
+      const int nr = nr_sh[s];
+      if (!go[q] || r >= nr) continue;
+      const double* hr = hinv + int64_t(dom[s]) * kCapX * kCapX + r * kCapX;
+      double v = 0.0;
+      for (int k = 0; k < nr; ++k) v += hr[k] * yv[rl[s][k] * NV + q];
+      lam[q][r] = -v;
+    }
+    __syncthreads();
+    // a = Y^T v' + G_a lambda
+    if (tid < NV && go[tid]) {
       const int q = tid, s = q >> 1;
-      if (go[q]) {
-        const int nr = nr_sh[s];
-        for (int i = 0; i < nr; ++i) {
-          double v = -yv[rl[s][i] * NV + q];
-          for (int k = 0; k < i; ++k) v -= H[s][i][k] * lam[q][k];
-          lam[q][i] = v / H[s][i][i];
-        }
-        for (int i = nr - 1; i >= 0; --i) {
-          double v = lam[q][i];
-          for (int k = i + 1; k < nr; ++k) v -= H[s][k][i] * lam[q][k];
-          lam[q][i] = v / H[s][i][i];
-        }
-        for (int t = 0; t < 3; ++t) {
-          double v = wa[q][t];
-          for (int r = 0; r < nr; ++r) v += ga[s][r][t] * lam[q][r];
-          wa[q][t] = v;
-        }
+      const int nr = nr_sh[s];
+      for (int t = 0; t < 3; ++t) {
+        double v = wa[q][t];
+        for (int r = 0; r < nr; ++r) v += ga[s][r][t] * lam[q][r];
+        wa[q][t] = v;
       }
     }
     __syncthreads();
@@ -3343,7 +3384,8 @@ __device__ void shared_x_solve(
 // workgroup stride: a fixed grid instead of one workgroup per subdomain slot
 template <int G>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_tps_shared_x(
-    int nd, const int32_t* __restrict__ order, const int32_t* __restrict__ groups,
+    int nd, double* __restrict__ hinv, const int32_t* __restrict__ order,
+    const int32_t* __restrict__ groups,
     const int32_t* __restrict__ dpat, const int32_t* __restrict__ dbox,
     const int32_t* __restrict__ pat_n, const int32_t* __restrict__ pstatus,
     const int64_t* __restrict__ pxo, const double* __restrict__ pinv,
@@ -3359,7 +3401,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   const int ngrid = int(gridDim.x) & ~7;  // a multiple of the XCD count
   if (int(blockIdx.x) >= ngrid) return;
   for (int w = blockIdx.x; w < ng; w += ngrid)
-    shared_x_solve<G>(xcd_rank(w, ng), order, groups, dpat, dbox, pat_n, pstatus, pxo, pinv, gx,
+    shared_x_solve<G>(xcd_rank(w, ng), hinv, order, groups, dpat, dbox, pat_n, pstatus, pxo, pinv, gx,
                       gy, ngx, xs, ys, idx, vx, vy, dom_off, slot_pt, smoothing_d, threshold,
                       smooth_slots, nslots, weights, wstride, stats, status, slot_xy, cap_r, nm,
                       nd);
@@ -3977,8 +4019,11 @@ SharedWork shared_work(void* base, int nfields, int dom_cap, int npat_cap, int n
   w.pinv = reinterpret_cast<double*>(
       take(8 * np * size_t(std::max(tps_mat_size(n_max), tps_x_size(n_max)))));
   w.pac = reinterpret_cast<double*>(take(8 * 6 * pts));
-  w.gmat = reinterpret_cast<double*>(take(8 * nd * nm * size_t(cap_r > 0 ? cap_r : 1)));
-  w.hfac = reinterpret_cast<double*>(take(8 * nd * kCapMaxR * kCapMaxR));
+  // capacitance columns of the substitution path (at most kCapMaxR), the
+  // factors of H (kCapMaxR^2) or H^-1 (kCapX^2) per subdomain
+  w.gmat = reinterpret_cast<double*>(
+      take(8 * nd * nm * size_t(std::max(1, std::min(cap_r, kCapMaxR)))));
+  w.hfac = reinterpret_cast<double*>(take(8 * nd * kCapX * kCapX));
   w.gpoly = reinterpret_cast<double*>(take(8 * nd * kCapMaxR * 3));
   w.bytes = o;
   return w;
@@ -4046,7 +4091,10 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   if (n_max < 4 || n_max > kMaxN) return tps_fail(PIV_EINVAL, "n_max must be in [4, 2048]");
   if (npat_cap <= 0 || npat_cap > kPatMaxPerField)
     return tps_fail(PIV_EINVAL, "npat_cap must be in [1, 64]");
-  if (cap_r < 0 || cap_r > kCapMaxR) return tps_fail(PIV_EINVAL, "cap_r must be in [0, 12]");
+  if (cap_r < 0 || cap_r > kCapX) return tps_fail(PIV_EINVAL, "cap_r must be in [0, 32]");
+  // the substitution path corrects for at most kCapMaxR missing nodes
+  const bool subst = g_shared_solve.load(std::memory_order_relaxed) == 1;
+  if (subst) cap_r = std::min(cap_r, kCapMaxR);
   if (shared_columns_lds(kSharedNV, (n_max + 1) & ~1) > 120 * 1024)
     return tps_fail(PIV_EUNSUPPORTED, "n_max too large for the shared solve's LDS");
   if (int64_t(nfields) * dom_cap > INT32_MAX / 4 || int64_t(nfields) * npat_cap > 1 << 20)
@@ -4087,7 +4135,7 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   launch_factor(npat, n_max, w.pxy, w.pxy + w.pxy_stride, w.pdom_off, w.pslot, w.pmat_off,
                 w.pat_smooth, w.pmats, w.prefl, w.pvwork, w.pslot_xy, w.pxy_stride, w.pstatus,
                 nullptr, lat, gen, s);
-  if (g_shared_solve.load(std::memory_order_relaxed) != 1) {
+  if (!subst) {
     // the explicit inverses of the patterns, then the shared subdomains in
     // groups of one pattern
     const int nblk = (n_max - 3 + kNB - 1) / kNB;
@@ -4105,7 +4153,7 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
                        w.pac, w.pinv);
 #define FIA_SHARED_X(G)                                                                           \
   hipLaunchKernelGGL(k_tps_shared_x<G>, dim3(std::max(8, std::min(nd, 2048))), dim3(kThreads),    \
-                     shared_x_lds(G, nm), s, nd, w.order_shared, w.groups, w.dpat, w.dbox,        \
+                     shared_x_lds(G, nm), s, nd, w.hfac, w.order_shared, w.groups, w.dpat, w.dbox,\
                      w.pat_n, w.pstatus, w.pxo, w.pinv,                                           \
                      gx, gy, int(ngx), xs, ys, idx, vx, vy, dom_off, slot_pt, smoothing,          \
                      threshold, smooth_slots, nslots, weights, wstride, stats, status, slot_xy,   \

@@ -385,10 +385,10 @@ int64_t piv_tps_shared_workspace(int32_t nfields, int32_t dom_cap, int32_t npat_
  * subdomain (thin_plate_spline.py:214): subdomains whose boxes hold the
  * same grid-node pattern share one factorisation per field (the matrix of a
  * box is the same up to a translation, with the polynomial part relative
- * to the box's first node), and a subdomain missing at most cap_r (<= 16)
- * nodes of its box (its invalid vectors) is solved with it and a
- * capacitance correction; the others are factorised alone as in
- * piv_tps_smooth. Same outputs as piv_tps_smooth (weights in image
+ * to the box's first node), and a subdomain missing at most cap_r (<= 32;
+ * <= 12 when PIV_TUNE_TPS_SHARED_SOLVE selects the substitution) nodes of
+ * its box (its invalid vectors) is solved with it and a capacitance
+ * correction; the others are factorised alone as in piv_tps_smooth. Same outputs as piv_tps_smooth (weights in image
  * coordinates); status[d] = 2 when a center is off the grid. At most
  * npat_cap (<= 64) patterns per field. `order`: the order of the nfields
  * dom_cap subdomain slots (piv_tps_offsets); idx: the vector index of
