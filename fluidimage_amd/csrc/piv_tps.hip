@@ -217,8 +217,8 @@ __device__ double block_max(double v, double* red) {
 // differences within a subdomain (the table entry without the lattice
 // check); only B[:, :3] is stored, the factorisation generates the rest
 template <int NT, bool GEN>
-__global__ __launch_bounds__(NT) void k_tps_prep(
-    const double* __restrict__ xs, const double* __restrict__ ys,
+__device__ __forceinline__ void tps_prep_one(
+    int b, const double* __restrict__ xs, const double* __restrict__ ys,
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
     const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
     double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
     PhiLattice lat, const int32_t* __restrict__ order) {
   static_assert(NT >= kThreads && NT % 64 == 0, "block sums over the first kThreads threads");
   constexpr int kWaves = NT / 64;
-  const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
+  const int d = order != nullptr ? order[b] : b;
   if (d < 0) return;  // an order entry for another kernel (solved by patterns)
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
@@ -423,6 +423,32 @@ __global__ __launch_bounds__(NT) void k_tps_prep(
 #undef VM
 }
 
+// the subdomain order[b] (b if order is NULL) for b = blockIdx.x, or with
+// STRIDE for b = blockIdx.x, + gridDim.x, ... < nitems (an order lists its
+// subdomains first, then -1: for launches over slots most of which are
+// unused; the loop costs registers, so the one-slot form stays for the
+// others)
+template <int NT, bool GEN, bool STRIDE>
+__global__ __launch_bounds__(NT) void k_tps_prep(
+    const double* __restrict__ xs, const double* __restrict__ ys,
+    const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
+    const int64_t* __restrict__ mat_off, const double* __restrict__ smoothing_d,
+    double* __restrict__ mats, double* __restrict__ refl, double* __restrict__ vwork,
+    double* __restrict__ slot_xy, int64_t nslots, int32_t* __restrict__ status, int nm,
+    PhiLattice lat, const int32_t* __restrict__ order, int nitems) {
+  if constexpr (!STRIDE) {
+    tps_prep_one<NT, GEN>(blockIdx.x, xs, ys, dom_off, slot_pt, mat_off, smoothing_d, mats, refl,
+                          vwork, slot_xy, nslots, status, nm, lat, order);
+  } else {
+  for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
+    if (order != nullptr && order[b] < 0) break;
+    tps_prep_one<NT, GEN>(b, xs, ys, dom_off, slot_pt, mat_off, smoothing_d, mats, refl, vwork,
+                          slot_xy, nslots, status, nm, lat, order);
+    __syncthreads();  // LDS reused by the next subdomain
+  }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 2. B' = B - V Z^T - Z V^T applied on the fly, C = B'[3:, 3:] = L L^T by a
 //    left-looking blocked Cholesky (block columns of kNB = 32). For block
@@ -465,17 +491,16 @@ __device__ __forceinline__ dvec4 mfma_f64(double a, double b, dvec4 c) {
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_tps_chol(const int64_t* __restrict__ dom_off,
-                                                           const int64_t* __restrict__ mat_off,
-                                                           double* __restrict__ mats,
-                                                           const double* __restrict__ vwork,
-                                                           int32_t* __restrict__ status,
-                                                           const double* __restrict__ slot_xy,
-                                                           int64_t nslots,
-                                                           const double* __restrict__ smoothing_d,
-                                                           PhiLattice lat, bool gen,
-                                                           const int32_t* __restrict__ order) {
-  const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
+__device__ __forceinline__ void tps_chol_one(int b, const int64_t* __restrict__ dom_off,
+                                             const int64_t* __restrict__ mat_off,
+                                             double* __restrict__ mats,
+                                             const double* __restrict__ vwork,
+                                             int32_t* __restrict__ status,
+                                             const double* __restrict__ slot_xy, int64_t nslots,
+                                             const double* __restrict__ smoothing_d,
+                                             PhiLattice lat, bool gen,
+                                             const int32_t* __restrict__ order) {
+  const int d = order != nullptr ? order[b] : b;
   if (d < 0) return;  // an order entry for another kernel (solved by patterns)
   const int64_t s0 = dom_off[d];
   const int n = int(dom_off[d + 1] - s0);
@@ -868,6 +893,25 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __syncthreads();
   }
   if (tid == 0 && bad) status[d] = 1;
+}
+
+template <int NT, bool STRIDE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_tps_chol(
+    const int64_t* __restrict__ dom_off, const int64_t* __restrict__ mat_off,
+    double* __restrict__ mats, const double* __restrict__ vwork, int32_t* __restrict__ status,
+    const double* __restrict__ slot_xy, int64_t nslots, const double* __restrict__ smoothing_d,
+    PhiLattice lat, bool gen, const int32_t* __restrict__ order, int nitems) {
+  if constexpr (!STRIDE) {
+    tps_chol_one<NT>(blockIdx.x, dom_off, mat_off, mats, vwork, status, slot_xy, nslots,
+                     smoothing_d, lat, gen, order);
+  } else {
+  for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
+    if (order != nullptr && order[b] < 0) break;
+    tps_chol_one<NT>(b, dom_off, mat_off, mats, vwork, status, slot_xy, nslots, smoothing_d, lat,
+                     gen, order);
+    __syncthreads();  // LDS reused by the next subdomain
+  }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1296,8 +1340,8 @@ __device__ void saddle_solve(double* x0, double* x1, int n, const double* VW, co
   apply_q(x0, x1, n, VW, tau, false, red);
 }
 
-__global__ __launch_bounds__(kThreads) void k_tps_solve(
-    const float* __restrict__ vx, const float* __restrict__ vy,
+__device__ __forceinline__ void tps_solve_one(
+    int b, const float* __restrict__ vx, const float* __restrict__ vy,
     const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
     const int64_t* __restrict__ mat_off, const double* __restrict__ mats,
     const double* __restrict__ refl, const double* __restrict__ vwork,
@@ -1305,7 +1349,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
     double threshold, double* __restrict__ smooth_slots, int64_t nslots,
     double* __restrict__ weights, int64_t wstride, double* __restrict__ stats,
     const int32_t* __restrict__ status, int nm, const int32_t* __restrict__ order) {
-  const int d = order != nullptr ? order[blockIdx.x] : int(blockIdx.x);
+  const int d = order != nullptr ? order[b] : b;
   if (d < 0) return;  // an order entry for another kernel (solved by patterns)
   if (status[d]) return;
   const int64_t s0 = dom_off[d];
@@ -1395,6 +1439,29 @@ __global__ __launch_bounds__(kThreads) void k_tps_solve(
       stats[6 * d + 3 * c + 1] = maxdiff[c];
       stats[6 * d + 3 * c + 2] = count[c];
     }
+  }
+}
+
+template <bool STRIDE>
+__global__ __launch_bounds__(kThreads) void k_tps_solve(
+    const float* __restrict__ vx, const float* __restrict__ vy,
+    const int64_t* __restrict__ dom_off, const int32_t* __restrict__ slot_pt,
+    const int64_t* __restrict__ mat_off, const double* __restrict__ mats,
+    const double* __restrict__ refl, const double* __restrict__ vwork,
+    const double* __restrict__ smoothing_d, double threshold,
+    double* __restrict__ smooth_slots, int64_t nslots, double* __restrict__ weights,
+    int64_t wstride, double* __restrict__ stats, const int32_t* __restrict__ status, int nm,
+    const int32_t* __restrict__ order, int nitems) {
+  if constexpr (!STRIDE) {
+    tps_solve_one(blockIdx.x, vx, vy, dom_off, slot_pt, mat_off, mats, refl, vwork, smoothing_d,
+                  threshold, smooth_slots, nslots, weights, wstride, stats, status, nm, order);
+  } else {
+  for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
+    if (order != nullptr && order[b] < 0) break;
+    tps_solve_one(b, vx, vy, dom_off, slot_pt, mat_off, mats, refl, vwork, smoothing_d,
+                  threshold, smooth_slots, nslots, weights, wstride, stats, status, nm, order);
+    __syncthreads();  // LDS reused by the next subdomain
+  }
   }
 }
 
@@ -2167,7 +2234,8 @@ __device__ int block_scan_max_1024(int v, int* tmp) {
 // subdomains (from `order`, largest first) and of the shared ones (by
 // index; -1 beyond), and the groups of k_tps_shared_x: runs of one pattern
 // in the shared order cut every `group` subdomains (groups[g] the first
-// entry of group g, groups[nd + 1 + g] its size, groups[nd] their number).
+// entry of group g, groups[nd + 1 + g] its size, groups[nd] their number),
+// and the patterns in use (plist[0 .. plist[npat]), -1 beyond).
 __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
     int npat, const int32_t* __restrict__ pat_n, const int32_t* __restrict__ pat_box,
     const double* __restrict__ gx, const double* __restrict__ gy, int nd,
@@ -2209,6 +2277,7 @@ __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
     pxo[npat] = base_x;
     plist[npat] = base_u;
   }
+  for (int p = base_u + tid; p < npat; p += 1024) plist[p] = -1;
   __syncthreads();
   for (int p = 0; p < npat; ++p) {
     const int n = pat_n[p];
@@ -3924,23 +3993,34 @@ int tps_interpolate_impl(const char* name, int32_t npts, const double* px, const
   return tps_check(name);
 }
 
-// matrix build and factorisation of nd subdomains (prep + Cholesky)
+// matrix build and factorisation of nd subdomains (prep + Cholesky); grid:
+// the workgroups of each launch, striding over the subdomains (0: nd, one
+// each; fewer where an order lists few of them, the rest -1)
 void launch_factor(int32_t nd, int32_t n_max, const double* xs, const double* ys,
                    const int64_t* dom_off, const int32_t* slot_pt, const int64_t* mat_off,
                    const double* smoothing, double* mats, double* refl, double* vwork,
                    double* slot_xy, int64_t nslots, int32_t* status, const int32_t* order,
-                   const PhiLattice& lat, bool gen, hipStream_t s) {
+                   const PhiLattice& lat, bool gen, hipStream_t s, int grid = 0) {
+  const dim3 g(unsigned(grid > 0 ? std::min(grid, int(nd)) : nd));
+  const bool stride = grid > 0;
   // up to one subdomain per CU: 16 wavefronts per matrix build, else 4
   const size_t prep_lds = size_t(8) * n_max * sizeof(double) + size_t(lat.lds_words) * sizeof(double);
   const bool wide = nd <= cu_count();
-#define PIV_PREP(NT, GEN)                                                                    \
-  hipLaunchKernelGGL((k_tps_prep<NT, GEN>), dim3(nd), dim3(NT), prep_lds, s, xs, ys, dom_off, \
+#define PIV_PREP(NT, GEN, ST)                                                                \
+  hipLaunchKernelGGL((k_tps_prep<NT, GEN, ST>), g, dim3(NT), prep_lds, s, xs, ys, dom_off,  \
                      slot_pt, mat_off, smoothing, mats, refl, vwork, slot_xy, nslots, status,   \
-                     int(n_max), lat, order)
-  if (wide && gen) PIV_PREP(1024, true);
-  else if (wide) PIV_PREP(1024, false);
-  else if (gen) PIV_PREP(kThreads, true);
-  else PIV_PREP(kThreads, false);
+                     int(n_max), lat, order, int(nd))
+  if (stride) {
+    if (wide && gen) PIV_PREP(1024, true, true);
+    else if (wide) PIV_PREP(1024, false, true);
+    else if (gen) PIV_PREP(kThreads, true, true);
+    else PIV_PREP(kThreads, false, true);
+  } else {
+    if (wide && gen) PIV_PREP(1024, true, false);
+    else if (wide) PIV_PREP(1024, false, false);
+    else if (gen) PIV_PREP(kThreads, true, false);
+    else PIV_PREP(kThreads, false, false);
+  }
 #undef PIV_PREP
   // 8 wavefronts per subdomain (one workgroup per CU) or 4 (two per CU).
   // Up to two subdomains per CU the launch is one round either way if the
@@ -3952,12 +4032,14 @@ void launch_factor(int32_t nd, int32_t n_max, const double* xs, const double* ys
   else if (nd <= 2 * cus) ct = 256;
   else if (ct == 0) ct = 256;
   const size_t chol_lds = size_t(lat.lds_words) * sizeof(double);
-  if (ct == 512)
-    hipLaunchKernelGGL(k_tps_chol<512>, dim3(nd), dim3(512), chol_lds, s, dom_off, mat_off, mats,
-                       vwork, status, slot_xy, nslots, smoothing, lat, gen, order);
-  else
-    hipLaunchKernelGGL(k_tps_chol<256>, dim3(nd), dim3(256), chol_lds, s, dom_off, mat_off, mats,
-                       vwork, status, slot_xy, nslots, smoothing, lat, gen, order);
+#define PIV_CHOL(NT, ST)                                                                       \
+  hipLaunchKernelGGL((k_tps_chol<NT, ST>), g, dim3(NT), chol_lds, s, dom_off, mat_off, mats,  \
+                     vwork, status, slot_xy, nslots, smoothing, lat, gen, order, int(nd))
+  if (ct == 512 && stride) PIV_CHOL(512, true);
+  else if (ct == 512) PIV_CHOL(512, false);
+  else if (stride) PIV_CHOL(256, true);
+  else PIV_CHOL(256, false);
+#undef PIV_CHOL
 }
 
 void launch_solve(int32_t nd, int32_t n_max, const float* vx, const float* vy,
@@ -3965,11 +4047,17 @@ void launch_solve(int32_t nd, int32_t n_max, const float* vx, const float* vy,
                   const double* mats, const double* refl, const double* vwork,
                   const double* smoothing, double threshold, double* smooth_slots,
                   int64_t nslots, double* weights, int64_t wstride, double* stats,
-                  const int32_t* status, const int32_t* order, hipStream_t s) {
-  hipLaunchKernelGGL(k_tps_solve, dim3(nd), dim3(kThreads),
-                     size_t(3) * ((n_max + 1) & ~1) * sizeof(double), s, vx, vy, dom_off, slot_pt,
-                     mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,
-                     weights, wstride, stats, status, (n_max + 1) & ~1, order);
+                  const int32_t* status, const int32_t* order, hipStream_t s, int grid = 0) {
+  const dim3 g(unsigned(grid > 0 ? std::min(grid, int(nd)) : nd));
+  const size_t lds = size_t(3) * ((n_max + 1) & ~1) * sizeof(double);
+  if (grid > 0)
+    hipLaunchKernelGGL(k_tps_solve<true>, g, dim3(kThreads), lds, s, vx, vy, dom_off, slot_pt,
+                       mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,
+                       weights, wstride, stats, status, (n_max + 1) & ~1, order, int(nd));
+  else
+    hipLaunchKernelGGL(k_tps_solve<false>, g, dim3(kThreads), lds, s, vx, vy, dom_off, slot_pt,
+                       mat_off, mats, refl, vwork, smoothing, threshold, smooth_slots, nslots,
+                       weights, wstride, stats, status, (n_max + 1) & ~1, order, int(nd));
 }
 
 // workspace of piv_tps_smooth_shared (offsets in piv_tps_shared_workspace)
@@ -4124,17 +4212,19 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   }
   // the subdomains factorised alone
   if (parts & PIV_TPS_PART_DIRECT) {
+    // few of the slots: workgroups striding over the direct order
+    const int grid = 4 * cu_count();
     launch_factor(nd, n_max, xs, ys, dom_off, slot_pt, mat_off, smoothing, mats, refl, vwork,
-                  slot_xy, nslots, status, w.order_direct, lat, gen, s);
+                  slot_xy, nslots, status, w.order_direct, lat, gen, s, grid);
     launch_solve(nd, n_max, vx, vy, dom_off, slot_pt, mat_off, mats, refl, vwork, smoothing,
                  threshold, smooth_slots, nslots, weights, wstride, stats, status,
-                 w.order_direct, s);
+                 w.order_direct, s, grid);
   }
   if (!(parts & PIV_TPS_PART_SHARED)) return tps_check("piv_tps_smooth_shared");
   // the patterns' factorisations, then the subdomains that share them
   launch_factor(npat, n_max, w.pxy, w.pxy + w.pxy_stride, w.pdom_off, w.pslot, w.pmat_off,
                 w.pat_smooth, w.pmats, w.prefl, w.pvwork, w.pslot_xy, w.pxy_stride, w.pstatus,
-                nullptr, lat, gen, s);
+                w.plist, lat, gen, s, 2 * cu_count());
   if (!subst) {
     // the explicit inverses of the patterns, then the shared subdomains in
     // groups of one pattern
