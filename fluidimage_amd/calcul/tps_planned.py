@@ -57,14 +57,13 @@ MARGIN_EXTENT = 1.35
 SHARED_PATTERNS = 32
 SHARED_MAX_MISSING = 32
 # sharing pays where a field has many subdomains per pattern and the batch
-# many more subdomains than the GPU runs at once (the patterns' factorisations
-# come first, one workgroup each): measured on fields of 2048 x 2048 px
-# (subdom_size 200), 32 fields of 420 subdomains take 13.6 ms shared against
-# 21.0 ms alone; 32 fields of 110, 6.7 against 5.6; 32 fields of 42, 3.2
-# against 1.5; a one-pair multipass (one field of 420 at the end) takes
-# 5.4 ms a step sharing at the end against 4.1 ms without
-SHARED_MIN_SUBDOMAINS = 200  # per field
-SHARED_MIN_BATCH = 8192  # subdomains of the batch
+# many more subdomains than the GPU runs at once (the patterns' explicit
+# inverses come first, a short chain of launches): measured in the 32-pair
+# multipass of 2048 x 2048 px images (subdom_size 200), 32 fields of 110
+# subdomains take 4.45 ms shared against 5.70 ms alone, 32 fields of 42
+# take 2.58 ms against 1.54 ms
+SHARED_MIN_SUBDOMAINS = 100  # per field
+SHARED_MIN_BATCH = 2048  # subdomains of the batch
 
 
 @dataclass(frozen=True)

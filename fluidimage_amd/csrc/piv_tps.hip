@@ -2232,22 +2232,18 @@ __device__ int block_scan_max_1024(int v, int* tmp) {
 // nodes relative to the box's first node, row-major: the order of the
 // members of a subdomain), an identity slot list, the orders of the direct
 // subdomains (from `order`, largest first) and of the shared ones (by
-// index; -1 beyond), and the groups of k_tps_shared_x: runs of one pattern
-// in the shared order cut every `group` subdomains (groups[g] the first
-// entry of group g, groups[nd + 1 + g] its size, groups[nd] their number),
-// and the patterns in use (plist[0 .. plist[npat]), -1 beyond).
+// index; -1 beyond), and the patterns in use (plist[0 .. plist[npat]), -1
+// beyond).
 __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
     int npat, const int32_t* __restrict__ pat_n, const int32_t* __restrict__ pat_box,
     const double* __restrict__ gx, const double* __restrict__ gy, int nd,
     const int32_t* __restrict__ order, const int32_t* __restrict__ dmode,
-    const int32_t* __restrict__ dpat, int group, int64_t* __restrict__ pdom_off,
-    int64_t* __restrict__ pmat_off, int64_t* __restrict__ pxo, double* __restrict__ pxy,
-    int32_t* __restrict__ pslot, int64_t pxy_stride, int32_t* __restrict__ order_direct,
-    int32_t* __restrict__ order_shared, int32_t* __restrict__ groups,
+    int64_t* __restrict__ pdom_off, int64_t* __restrict__ pmat_off, int64_t* __restrict__ pxo,
+    double* __restrict__ pxy, int32_t* __restrict__ pslot, int64_t pxy_stride,
+    int32_t* __restrict__ order_direct, int32_t* __restrict__ order_shared,
     int32_t* __restrict__ plist) {
   __shared__ int64_t tmp64[32];
   __shared__ int tmp[32];
-  __shared__ int last_run;
   const int tid = threadIdx.x;
   int64_t base_s = 0, base_m = 0, base_x = 0;
   int base_u = 0;
@@ -2312,15 +2308,85 @@ __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
   for (int b = bd + tid; b < nd; b += 1024) order_direct[b] = -1;
   for (int b = bs + tid; b < nd; b += 1024) order_shared[b] = -1;
   if (tid == 0) order_shared[nd] = bs;
+}
+
+// inclusive scan over a block of 256 threads (tmp: 8 ints of LDS)
+__device__ int block_scan_256(int v, int* tmp, int& total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  if (lane == 63) tmp[wv] = v;
+  __syncthreads();
+  int before = 0;
+  for (int k = 0; k < wv; ++k) before += tmp[k];
+  total = tmp[0] + tmp[1] + tmp[2] + tmp[3];
+  __syncthreads();
+  return v + before;
+}
+
+// Field blockIdx.x (256 threads): its shared subdomains (a segment of the
+// shared order, which is by index) reordered by pattern, stably, into
+// order_sorted: the subdomains of one pattern follow one another, so that
+// the groups of k_tps_shared_x are full and the workgroups running at one
+// time on an XCD read few patterns' inverses (from its L2 cache).
+__global__ __launch_bounds__(256) void k_tps_shared_sort(
+    int dom_cap, int npat_cap, const int32_t* __restrict__ pat_n,
+    const int32_t* __restrict__ dpat, const int32_t* __restrict__ order_shared, int nd,
+    int32_t* __restrict__ order_sorted) {
+  __shared__ int tmp[8];
+  const int f = blockIdx.x, tid = threadIdx.x;
+  const int bs = order_shared[nd];
+  // the field's segment [lo, hi) of the shared order (ascending subdomains)
+  auto first_at_least = [&](int d) {
+    int lo = 0, hi = bs;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (order_shared[mid] < d) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const int lo = first_at_least(f * dom_cap), hi = first_at_least((f + 1) * dom_cap);
+  int pos = lo;
+  for (int q = 0; q < npat_cap; ++q) {
+    const int p = f * npat_cap + q;
+    if (pat_n[p] == 0) continue;
+    for (int c0 = lo; c0 < hi; c0 += 256) {
+      const int b = c0 + tid;
+      const int d = b < hi ? order_shared[b] : -1;
+      const bool in = d >= 0 && dpat[d] == p;
+      int t;
+      const int r = block_scan_256(in ? 1 : 0, tmp, t);
+      if (in) order_sorted[pos + r - 1] = d;
+      pos += t;
+    }
+  }
+  if (f == 0)
+    for (int b = bs + tid; b <= nd; b += 256) order_sorted[b] = b == nd ? bs : -1;
+}
+
+// One workgroup of 1024 threads: the groups of k_tps_shared_x, runs of one
+// pattern in the sorted shared order cut every `group` subdomains
+// (groups[g] the first entry of group g, groups[nd + 1 + g] its size,
+// groups[nd] their number).
+__global__ __launch_bounds__(1024) void k_tps_shared_groups(
+    int nd, const int32_t* __restrict__ dpat, const int32_t* __restrict__ order, int group,
+    int32_t* __restrict__ groups) {
+  __shared__ int tmp[32];
+  __shared__ int last_run;
+  const int tid = threadIdx.x;
+  const int bs = order[nd];
   int32_t* gcnt = groups + nd + 1;
   for (int b = tid; b < nd; b += 1024) gcnt[b] = 0;
-  __syncthreads();  // the shared order and the zeroed sizes, for the whole block
+  __syncthreads();  // the zeroed sizes, for the whole block
   int run = -1, ng = 0;
   for (int c0 = 0; c0 < bs; c0 += 1024) {
     const int b = c0 + tid;
     const bool valid = b < bs;
-    const int pb = valid ? dpat[order_shared[b]] : -1;
-    const int pp = valid && b > 0 ? dpat[order_shared[b - 1]] : -1;
+    const int pb = valid ? dpat[order[b]] : -1;
+    const int pp = valid && b > 0 ? dpat[order[b - 1]] : -1;
     const bool first = valid && (b == 0 || pb != pp);
     const int rs = max(run, block_scan_max_1024(first ? b : -1, tmp));
     const bool gs = valid && (b - rs) % group == 0;
@@ -4063,7 +4129,7 @@ void launch_solve(int32_t nd, int32_t n_max, const float* vx, const float* vy,
 // workspace of piv_tps_smooth_shared (offsets in piv_tps_shared_workspace)
 struct SharedWork {
   int32_t *dmode, *dpat, *dbox, *pat_n, *pat_box, *pslot, *pstatus, *order_direct, *order_shared,
-      *groups, *plist;
+      *order_sorted, *groups, *plist;
   int64_t *pdom_off, *pmat_off, *pxo;
   double *pat_smooth, *pxy, *pmats, *pinv, *pac, *prefl, *pvwork, *pslot_xy, *gmat, *hfac, *gpoly;
   int64_t pxy_stride;
@@ -4087,6 +4153,7 @@ SharedWork shared_work(void* base, int nfields, int dom_cap, int npat_cap, int n
   w.dbox = reinterpret_cast<int32_t*>(take(16 * nd));
   w.order_direct = reinterpret_cast<int32_t*>(take(4 * nd));
   w.order_shared = reinterpret_cast<int32_t*>(take(4 * (nd + 1)));  // + their count
+  w.order_sorted = reinterpret_cast<int32_t*>(take(4 * (nd + 1)));
   w.groups = reinterpret_cast<int32_t*>(take(4 * 2 * (nd + 1)));
   w.pat_n = reinterpret_cast<int32_t*>(take(4 * np));
   w.pat_box = reinterpret_cast<int32_t*>(take(16 * np));
@@ -4204,9 +4271,8 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
                        int(ngy), int(irr_y), int(cap_r), int(n_max), int(npat_cap), w.dmode,
                        w.dpat, w.dbox, w.pat_n, w.pat_box, w.pat_smooth);
     hipLaunchKernelGGL(k_tps_pattern_layout, dim3(1), dim3(1024), 0, s, npat, w.pat_n,
-                       w.pat_box, gx, gy, nd, order, w.dmode, w.dpat, shared_x_group(nm),
-                       w.pdom_off, w.pmat_off, w.pxo, w.pxy, w.pslot, w.pxy_stride,
-                       w.order_direct, w.order_shared, w.groups, w.plist);
+                       w.pat_box, gx, gy, nd, order, w.dmode, w.pdom_off, w.pmat_off, w.pxo, w.pxy,
+                       w.pslot, w.pxy_stride, w.order_direct, w.order_shared, w.plist);
     hipLaunchKernelGGL(k_tps_zero_i32, dim3((npat + 255) / 256), dim3(256), 0, s, npat,
                        w.pstatus);
   }
@@ -4228,6 +4294,10 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
   if (!subst) {
     // the explicit inverses of the patterns, then the shared subdomains in
     // groups of one pattern
+    hipLaunchKernelGGL(k_tps_shared_sort, dim3(nfields), dim3(256), 0, s, int(dom_cap),
+                       int(npat_cap), w.pat_n, w.dpat, w.order_shared, nd, w.order_sorted);
+    hipLaunchKernelGGL(k_tps_shared_groups, dim3(1), dim3(1024), 0, s, nd, w.dpat,
+                       w.order_sorted, shared_x_group(nm), w.groups);
     const int nblk = (n_max - 3 + kNB - 1) / kNB;
     const int64_t nitem = int64_t(npat) * nblk * (nblk + 1) / 2;
     auto grid = [&](int64_t per_cu) { return dim3(unsigned(std::min<int64_t>(nitem, 256 * per_cu))); };
@@ -4243,7 +4313,7 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
                        w.pac, w.pinv);
 #define FIA_SHARED_X(G)                                                                           \
   hipLaunchKernelGGL(k_tps_shared_x<G>, dim3(std::max(8, std::min(nd, 2048))), dim3(kThreads),    \
-                     shared_x_lds(G, nm), s, nd, w.hfac, w.order_shared, w.groups, w.dpat, w.dbox,\
+                     shared_x_lds(G, nm), s, nd, w.hfac, w.order_sorted, w.groups, w.dpat, w.dbox,\
                      w.pat_n, w.pstatus, w.pxo, w.pinv,                                           \
                      gx, gy, int(ngx), xs, ys, idx, vx, vy, dom_off, slot_pt, smoothing,          \
                      threshold, smooth_slots, nslots, weights, wstride, stats, status, slot_xy,   \
