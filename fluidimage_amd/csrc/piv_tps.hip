@@ -2121,6 +2121,7 @@ __device__ __forceinline__ int xcd_rank(int b, int nb) {
 
 constexpr int kCapMaxR = 12;  // missing nodes of a subdomain solved with a shared factor
 constexpr int kCapX = 32;     // ... with a pattern's explicit inverse
+static_assert(kCapX <= 32, "k_tps_shared_x inverts H with two lanes per row");
 constexpr int kPatMaxPerField = 64;
 
 // first index in [0, n) with a[i] >= v (a ascending), n if none
@@ -3218,7 +3219,7 @@ __device__ void shared_x_solve(
   double* yv = dyn + G * nm;                  // [nm][NV]
   float* val = reinterpret_cast<float*>(dyn + shared_x_head(G, nm));   // [G][2][nm]
   int* fidx = reinterpret_cast<int*>(dyn + shared_x_head(G, nm) + G * nm);  // [G][nm]
-  int* fmem = reinterpret_cast<int*>(yv);  // [nm], during the checks
+  int* fmem = reinterpret_cast<int*>(yv);  // [G][nm], during the checks
   __shared__ double red[3 * NV * (kThreads / 64)];
   __shared__ int rl[G][kCapX];
   __shared__ int nr_sh[G];
@@ -3231,38 +3232,105 @@ __device__ void shared_x_solve(
   __shared__ int bad[G];
   __shared__ double nfixed[NV], maxdiff[NV];
   __shared__ int count[NV];
-  for (int s = 0; s < G; ++s) {
+  __shared__ int64_t s0_sh[G];
+  __shared__ int n_sh[G], box_sh[G][3];
+  // the group's subdomains (those of status 0 are checked: the checks of
+  // k_tps_shared_columns, every subdomain at once)
+  if (tid < G) {
+    const int s = tid;
     const int d = s < cnt ? order[b0 + s] : -1;
-    bool ok = d >= 0 && status[d] == 0;
-    if (ok) {
-      const int64_t s0 = dom_off[d];
-      const int n = int(dom_off[d + 1] - s0);
-      const int j0 = dbox[4 * d], nc = dbox[4 * d + 1], i0 = dbox[4 * d + 2];
-      if (tid == 0) bad[s] = 0;
-      ok = shared_members(n, nf, s0, slot_pt, idx, xs, ys, gx, gy, ngx, j0, nc, i0,
-                          fidx + s * nm, fmem, &bad[s]);
-      if (ok) {
-        const int nr = shared_missing(nf, fmem, cap_r, rl[s], &nr_sh[s], kCapX);
-        ok = nr <= cap_r && nr <= kCapX && n + nr == nf;
-      }
-      if (!ok && tid == 0) status[d] = 2;  // solved again by host sizing
-      if (ok) {
-        float* v0 = val + 2 * s * nm;
-        for (int j = tid; j < n; j += kThreads) {
-          const int c = slot_pt[s0 + j];
-          v0[j] = vx[c];
-          v0[nm + j] = vy[c];
-          slot_xy[s0 + j] = xs[c];
-          slot_xy[nslots + s0 + j] = ys[c];
-        }
-      }
-    }
-    if (tid == 0) {
-      act[s] = ok ? 1 : 0;
-      dom[s] = d;
-    }
-    __syncthreads();  // fmem is reused by the next subdomain
+    const bool ok = d >= 0 && status[d] == 0;
+    dom[s] = d;
+    act[s] = ok ? 1 : 0;
+    bad[s] = 0;
+    nr_sh[s] = 0;
+    s0_sh[s] = ok ? dom_off[d] : 0;
+    n_sh[s] = ok ? int(dom_off[d + 1] - dom_off[d]) : 0;
+    box_sh[s][0] = ok ? dbox[4 * d] : 0;
+    box_sh[s][1] = ok ? dbox[4 * d + 1] : 0;
+    box_sh[s][2] = ok ? dbox[4 * d + 2] : 0;
   }
+  for (int e = tid; e < G * nm; e += kThreads) fmem[e] = -1;
+  __syncthreads();
+  // members on the pattern's nodes: fidx[s][j] (node of member j), fmem[s][k]
+  // (member of node k, -1 for a missing node); a center off the grid marks
+  // its subdomain bad. Items e = (s, j) over all subdomains, eight a batch
+  // so that their dependent loads overlap
+  constexpr int kB = 8;
+  for (int e0 = 0; e0 < G * nm; e0 += kB * kThreads) {
+    int c[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int e = e0 + u * kThreads + tid;
+      const int s = e / nm, j = e % nm;
+      c[u] = (s < G && j < n_sh[s]) ? slot_pt[s0_sh[s] + j] : -1;
+    }
+    int q[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) q[u] = c[u] >= 0 ? idx[c[u]] : 0;
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      if (c[u] < 0) continue;
+      const int e = e0 + u * kThreads + tid;
+      const int s = e / nm, j = e % nm;
+      const int j0 = box_sh[s][0], nc = box_sh[s][1], i0 = box_sh[s][2];
+      const int row = q[u] / ngx, col = q[u] % ngx;
+      const int k = (row - i0) * nc + (col - j0);
+      const bool on = col - j0 >= 0 && col - j0 < nc && k >= 0 && k < nf &&
+                      xs[c[u]] == gx[col] && ys[c[u]] == gy[row];
+      if (!on) {
+        bad[s] = 1;
+        continue;
+      }
+      fidx[s * nm + j] = k;
+      fmem[s * nm + k] = j;
+    }
+  }
+  __syncthreads();
+  // the missing nodes in node order (at most kCapX listed), wavefront s
+  if ((tid >> 6) < G && act[tid >> 6] && !bad[tid >> 6]) {
+    const int s = tid >> 6, lane = tid & 63;
+    int nr = 0;
+    for (int k0 = 0; k0 < nf; k0 += 64) {
+      const int k = k0 + lane;
+      const bool miss = k < nf && fmem[s * nm + k] < 0;
+      const unsigned long long m = __ballot(miss);
+      const int rank = nr + __popcll(m & ((1ull << lane) - 1ull));
+      if (miss && rank < kCapX) rl[s][rank] = k;
+      nr += __popcll(m);
+    }
+    if (lane == 0) nr_sh[s] = nr;
+  }
+  __syncthreads();
+  if (tid < G && act[tid]) {
+    const int s = tid, nr = nr_sh[s];
+    if (bad[s] || nr > cap_r || nr > kCapX || n_sh[s] + nr != nf) {
+      status[dom[s]] = 2;  // solved again by host sizing
+      act[s] = 0;
+    }
+  }
+  __syncthreads();
+  // values and coordinates by slot
+  for (int e0 = 0; e0 < G * nm; e0 += kB * kThreads) {
+    int c[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int e = e0 + u * kThreads + tid;
+      const int s = e / nm, j = e % nm;
+      c[u] = (s < G && act[s] && j < n_sh[s]) ? slot_pt[s0_sh[s] + j] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      if (c[u] < 0) continue;
+      const int e = e0 + u * kThreads + tid;
+      const int s = e / nm, j = e % nm;
+      val[2 * s * nm + j] = vx[c[u]];
+      val[(2 * s + 1) * nm + j] = vy[c[u]];
+      slot_xy[s0_sh[s] + j] = xs[c[u]];
+      slot_xy[nslots + s0_sh[s] + j] = ys[c[u]];
+    }
+  }
+  __syncthreads();
   // H = X[R, R] (into the free LDS of x and y, nr x (kCapX + 1) per
   // subdomain), G_a = Y[R, :]
   double* Hs = dyn;
@@ -3296,10 +3364,13 @@ __device__ void shared_x_solve(
       wave_sync();
       if (lane < nr) Hm[k * (kCapX + 1) + lane] = rk;
       wave_sync();
-      if (lane < nr && lane != k) {
-        double* hi = Hm + lane * (kCapX + 1);
+      // row i = lane % 32, columns j = lane / 32, + 2, ...
+      const int i = lane & 31;
+      if (i < nr && i != k) {
+        double* hi = Hm + i * (kCapX + 1);
         const double f = hi[k];
-        for (int j = 0; j < nr; ++j)
+#pragma unroll 4
+        for (int j = lane >> 5; j < nr; j += 2)
           hi[j] = j == k ? -f * rp : hi[j] - f * Hm[k * (kCapX + 1) + j];
       }
       wave_sync();
@@ -3400,7 +3471,14 @@ __device__ void shared_x_solve(
       if (!go[q] || r >= nr) continue;
       const double* hr = hinv + int64_t(dom[s]) * kCapX * kCapX + r * kCapX;
       double v = 0.0;
-      for (int k = 0; k < nr; ++k) v += hr[k] * yv[rl[s][k] * NV + q];
+      for (int k0 = 0; k0 < nr; k0 += 8) {
+        double h[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) h[u] = k0 + u < nr ? hr[k0 + u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (k0 + u < nr) v += h[u] * yv[rl[s][k0 + u] * NV + q];
+      }
       lam[q][r] = -v;
     }
     __syncthreads();
@@ -3421,10 +3499,16 @@ __device__ void shared_x_solve(
       const int nr = nr_sh[s];
       for (int k = tid; k < nf; k += kThreads) {
         double w0 = yv[k * NV + 2 * s], w1 = yv[k * NV + 2 * s + 1];
-        for (int r = 0; r < nr; ++r) {
-          const double g = X[int64_t(rl[s][r]) * ldx + k];
-          w0 += g * lam[2 * s][r];
-          w1 += g * lam[2 * s + 1][r];
+        for (int r0 = 0; r0 < nr; r0 += 8) {
+          double g[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) g[u] = r0 + u < nr ? X[int64_t(rl[s][r0 + u]) * ldx + k] : 0.0;
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (r0 + u < nr) {
+              w0 += g[u] * lam[2 * s][r0 + u];
+              w1 += g[u] * lam[2 * s + 1][r0 + u];
+            }
         }
         yv[k * NV + 2 * s] = w0;
         yv[k * NV + 2 * s + 1] = w1;
