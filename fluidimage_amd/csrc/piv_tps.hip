@@ -104,10 +104,12 @@ __host__ __device__ __forceinline__ int64_t tps_mat_size(int n) {
 }
 
 // Explicit saddle-point inverse of a shared pattern of n nodes: X (n x n) in
-// rows of tps_x_ld(n) doubles, then the three rows of Y^T.
-__host__ __device__ __forceinline__ int64_t tps_x_ld(int n) { return (int64_t(n) + 15) & ~int64_t(15); }
+// rows of tps_x_ld(n) doubles, Y in columns n .. n + 2 of the rows (one
+// product with X's rows extended gives w and a), padded so that a pair of
+// columns from n + 2 stays in the row.
+__host__ __device__ __forceinline__ int64_t tps_x_ld(int n) { return (int64_t(n) + 4 + 15) & ~int64_t(15); }
 __host__ __device__ __forceinline__ int64_t tps_x_size(int n) {
-  return n > 0 ? (int64_t(n) + 3) * tps_x_ld(n) : 0;
+  return n > 0 ? int64_t(n) * tps_x_ld(n) : 0;
 }
 
 // The kernel function at offsets that are multiples of a lattice step:
@@ -2780,8 +2782,8 @@ k_tps_shared_iter(const int32_t* __restrict__ order, const int32_t* __restrict__
 //                      Cc = U' - A S / 2, U' = Z V = [0; C^-1 V[3:, :]],
 //                      S = V^T U') and Y = F - A (V^T F),
 //                      F = [R^-T; -C^-1 E R^-T];
-//   k_tps_xfill        X, rows of tps_x_ld(n) doubles, Y^T in the three rows
-//                      after it.
+//   k_tps_xfill        X, rows of tps_x_ld(n) doubles (Y in columns n .. n + 2,
+//                      written by k_tps_xprep).
 // A subdomain missing the nodes R of its pattern then needs no solve: its
 // capacitance columns are rows of X (G_w = X[:, R], G_a = Y[R, :],
 // H = X[R, R]), and each outlier iteration is one product with X
@@ -3017,7 +3019,7 @@ __device__ void xprep_pattern(int p, const int32_t* __restrict__ pat_n,
   const double* VW = pvwork + 6 * pdom_off[p];
   double* AC = pac + 6 * pdom_off[p];
   const int64_t ldx = tps_x_ld(n);
-  double* Yt = pinv + pxo[p] + int64_t(n) * ldx;
+  double* Xp = pinv + pxo[p];
   extern __shared__ double dyn[];
   double* VE = dyn;  // [m][6]: V[3 + j][0..2], E[j][0..2]
   __shared__ double red[18 * (kThreads / 64)];
@@ -3110,7 +3112,7 @@ __device__ void xprep_pattern(int p, const int32_t* __restrict__ pat_n,
     for (int k = 0; k < 3; ++k) {
       u[k] = a[k];
       u[3 + k] = c[k];
-      Yt[k * ldx + j] = y[k];
+      Xp[int64_t(j) * ldx + n + k] = y[k];
     }
   }
   __syncthreads();  // LDS reused by the next pattern
@@ -3164,11 +3166,12 @@ __global__ __launch_bounds__(kThreads) void k_tps_xfill(
 }
 #pragma clang fp contract(on)
 
-// dynamic LDS of k_tps_shared_x<g>: x (floats) [nm][2g] and y [nm][2g]
+// dynamic LDS of k_tps_shared_x<g>: x (floats) [nm][2g] and y [nm + 4][2g]
 // (first the matrices H of the subdomains, kCapX (kCapX + 1) doubles each),
 // then the values (floats) [g][2][nm] and member nodes [g][nm]
 constexpr int shared_x_head(int g, int nm) {
-  return 3 * g * nm > g * kCapX * (kCapX + 1) ? 3 * g * nm : g * kCapX * (kCapX + 1);
+  return g * nm + 2 * g * (nm + 4) > g * kCapX * (kCapX + 1) ? g * nm + 2 * g * (nm + 4)
+                                                              : g * kCapX * (kCapX + 1);
 }
 constexpr size_t shared_x_lds(int g, int nm) {
   return (size_t(shared_x_head(g, nm)) + size_t(g) * nm) * sizeof(double) + size_t(g) * nm * 4;
@@ -3213,10 +3216,9 @@ __device__ void shared_x_solve(
   const int nf = pat_n[p];
   const int64_t ldx = tps_x_ld(nf);
   const double* X = pinv + pxo[p];
-  const double* Yt = X + int64_t(nf) * ldx;
   extern __shared__ double dyn[];
   float* xv = reinterpret_cast<float*>(dyn);  // [nm][NV]
-  double* yv = dyn + G * nm;                  // [nm][NV]
+  double* yv = dyn + G * nm;                  // [nm + 4][NV]
   float* val = reinterpret_cast<float*>(dyn + shared_x_head(G, nm));   // [G][2][nm]
   int* fidx = reinterpret_cast<int*>(dyn + shared_x_head(G, nm) + G * nm);  // [G][nm]
   int* fmem = reinterpret_cast<int*>(yv);  // [G][nm], during the checks
@@ -3341,7 +3343,7 @@ __device__ void shared_x_solve(
     for (int e = tid; e < nr * nr; e += kThreads)
       Hm[(e / nr) * (kCapX + 1) + e % nr] = X[int64_t(rl[s][e / nr]) * ldx + rl[s][e % nr]];
     for (int e = tid; e < 3 * nr; e += kThreads)
-      ga[s][e / 3][e % 3] = Yt[int64_t(e % 3) * ldx + rl[s][e / 3]];
+      ga[s][e / 3][e % 3] = X[int64_t(rl[s][e / 3]) * ldx + nf + e % 3];
   }
   __syncthreads();
   // H^-1 by Gauss-Jordan elimination in place (H is symmetric positive
@@ -3409,10 +3411,10 @@ __device__ void shared_x_solve(
       for (int j = tid; j < n; j += kThreads) xv[fidx[s * nm + j] * NV + q] = val[q * nm + j];
     }
     __syncthreads();
-    // y = X v' (X symmetric: lanes over row pairs i, i + 1 read X by
-    // columns, 16 bytes a load, kU loads in flight; ldx is even and rows
-    // i >= nf of the pair read the padding of X's rows)
-    for (int i = 2 * tid; i < nf; i += 2 * kThreads) {
+    // [y; Y^T v'] = [X | Y]^T v' (X symmetric: lanes over output pairs i,
+    // i + 1 read the rows of [X | Y] by columns, 16 bytes a load, kU loads
+    // in flight; outputs i >= nf + 3 read the padding)
+    for (int i = 2 * tid; i < nf + 3; i += 2 * kThreads) {
       double acc0[NV], acc1[NV];
 #pragma unroll
       for (int q = 0; q < NV; ++q) acc0[q] = acc1[q] = 0.0;
@@ -3444,25 +3446,14 @@ __device__ void shared_x_solve(
       }
 #pragma unroll
       for (int q = 0; q < NV; ++q) yv[i * NV + q] = acc0[q];
-      if (i + 1 < nf)
+      if (i + 1 < nf + 3)
 #pragma unroll
         for (int q = 0; q < NV; ++q) yv[(i + 1) * NV + q] = acc1[q];
     }
-    // Y^T v'
-    for (int k = 0; k < 3; ++k) {
-      double pa[NV];
+    __syncthreads();
+    if (tid < NV)
 #pragma unroll
-      for (int q = 0; q < NV; ++q) pa[q] = 0.0;
-      for (int i = tid; i < nf; i += kThreads) {
-        const double t = Yt[k * ldx + i];
-#pragma unroll
-        for (int q = 0; q < NV; ++q) pa[q] += t * double(xv[i * NV + q]);
-      }
-      block_sum_n<NV>(pa, red);
-      if (tid == 0)
-#pragma unroll
-        for (int q = 0; q < NV; ++q) wa[q][k] = pa[q];
-    }
+      for (int k = 0; k < 3; ++k) wa[tid][k] = yv[(nf + k) * NV + tid];  // Y^T v'
     __syncthreads();
     // lambda = -H^-1 y[R]: thread (q, r)
     for (int e = tid; e < NV * kCapX; e += kThreads) {
