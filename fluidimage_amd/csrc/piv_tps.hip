@@ -2174,12 +2174,11 @@ __global__ __launch_bounds__(256) void k_tps_patterns(
     pat_smooth[int64_t(f) * npat_cap + q] = smoothing[d0];
   }
   __syncthreads();
-  if (threadIdx.x != 0 || !ok) return;
-  // pattern keys: columns, rows, and whether the box ends on an irregular
-  // last column / row of the grid
-  int keys[kPatMaxPerField];
-  int npat = 0;
-  for (int d = 0; d < dom_cap; ++d) {
+  if (!ok) return;
+  // pattern keys (columns, rows, and whether the box ends on an irregular
+  // last column / row of the grid) of the subdomains that may share, kept as
+  // dpat = -(key + 2) until they are assigned
+  for (int d = threadIdx.x; d < dom_cap; d += blockDim.x) {
     const int64_t dd = d0 + d;
     if (dmode[dd] != 0) continue;
     const int j0 = dbox[4 * dd], nc = dbox[4 * dd + 1], i0 = dbox[4 * dd + 2],
@@ -2188,23 +2187,46 @@ __global__ __launch_bounds__(256) void k_tps_patterns(
     if (nf < 4 || nf > n_max_cap || nf - counts[dd] > cap_r) continue;  // direct
     const int key = ((nc * 4096 + nr) * 2 + ((j0 + nc == ngx && irr_x) ? 1 : 0)) * 2 +
                     ((i0 + nr == ngy && irr_y) ? 1 : 0);
-    int id = -1;
-    for (int k = 0; k < npat; ++k)
-      if (keys[k] == key) id = k;
-    if (id < 0) {
-      if (npat >= npat_cap || npat >= kPatMaxPerField) continue;  // no room: direct
-      id = npat++;
-      keys[id] = key;
-      const int64_t p = int64_t(f) * npat_cap + id;
-      pat_n[p] = nf;
-      pat_box[4 * p + 0] = j0;
-      pat_box[4 * p + 1] = nc;
-      pat_box[4 * p + 2] = i0;
-      pat_box[4 * p + 3] = nr;
-    }
-    dmode[dd] = 1;
-    dpat[dd] = int(int64_t(f) * npat_cap + id);
+    dpat[dd] = -(key + 2);
   }
+  // pattern r: the key of the first unassigned subdomain in index order,
+  // assigned to every subdomain of that key (patterns numbered in order of
+  // first occurrence); beyond the cap the rest are factorised alone
+  __shared__ int first_sh[4];
+  const int cap = min(npat_cap, kPatMaxPerField);
+  for (int r = 0; r < cap; ++r) {
+    __syncthreads();
+    int dmin = INT_MAX;
+    for (int d = threadIdx.x; d < dom_cap; d += blockDim.x)
+      if (dpat[d0 + d] <= -2) {
+        dmin = d;
+        break;
+      }
+    for (int off = 32; off > 0; off >>= 1) dmin = min(dmin, __shfl_xor(dmin, off, 64));
+    if ((threadIdx.x & 63) == 0) first_sh[threadIdx.x >> 6] = dmin;
+    __syncthreads();
+    dmin = min(min(first_sh[0], first_sh[1]), min(first_sh[2], first_sh[3]));
+    if (dmin == INT_MAX) break;
+    const int64_t dm = d0 + dmin;
+    const int code = dpat[dm];
+    const int64_t p = int64_t(f) * npat_cap + r;
+    if (threadIdx.x == 0) {
+      pat_n[p] = dbox[4 * dm + 1] * dbox[4 * dm + 3];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pat_box[4 * p + k] = dbox[4 * dm + k];
+    }
+    __syncthreads();  // every thread read the key before it is replaced
+    for (int d = dmin + threadIdx.x; d < dom_cap; d += blockDim.x) {
+      const int64_t dd = d0 + d;
+      if (dpat[dd] == code) {
+        dmode[dd] = 1;
+        dpat[dd] = int(p);
+      }
+    }
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < dom_cap; d += blockDim.x)
+    if (dpat[d0 + d] <= -2) dpat[d0 + d] = -1;
 }
 
 // inclusive max-scan over a block of 1024 threads (tmp: 32 ints of LDS)
@@ -2231,18 +2253,14 @@ __device__ int block_scan_max_1024(int v, int* tmp) {
 }
 
 // One workgroup of 1024 threads: offsets of the patterns' centers, matrices
-// and explicit inverses (prefix sums of their sizes), their centers (grid
-// nodes relative to the box's first node, row-major: the order of the
-// members of a subdomain), an identity slot list, the orders of the direct
-// subdomains (from `order`, largest first) and of the shared ones (by
+// and explicit inverses (prefix sums of their sizes), the orders of the
+// direct subdomains (from `order`, largest first) and of the shared ones (by
 // index; -1 beyond), and the patterns in use (plist[0 .. plist[npat]), -1
 // beyond).
 __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
-    int npat, const int32_t* __restrict__ pat_n, const int32_t* __restrict__ pat_box,
-    const double* __restrict__ gx, const double* __restrict__ gy, int nd,
+    int npat, const int32_t* __restrict__ pat_n, int nd,
     const int32_t* __restrict__ order, const int32_t* __restrict__ dmode,
     int64_t* __restrict__ pdom_off, int64_t* __restrict__ pmat_off, int64_t* __restrict__ pxo,
-    double* __restrict__ pxy, int32_t* __restrict__ pslot, int64_t pxy_stride,
     int32_t* __restrict__ order_direct, int32_t* __restrict__ order_shared,
     int32_t* __restrict__ plist) {
   __shared__ int64_t tmp64[32];
@@ -2278,18 +2296,6 @@ __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
   }
   for (int p = base_u + tid; p < npat; p += 1024) plist[p] = -1;
   __syncthreads();
-  for (int p = 0; p < npat; ++p) {
-    const int n = pat_n[p];
-    if (n == 0) continue;
-    const int j0 = pat_box[4 * p], nc = pat_box[4 * p + 1], i0 = pat_box[4 * p + 2];
-    const int64_t o = pdom_off[p];
-    for (int k = tid; k < n; k += 1024) {
-      const int r = k / nc, c = k % nc;
-      pxy[o + k] = gx[j0 + c] - gx[j0];
-      pxy[pxy_stride + o + k] = gy[i0 + r] - gy[i0];
-      pslot[o + k] = int(o + k);
-    }
-  }
   // the direct subdomains in the order of `order`; the shared ones in index
   // order: a field's subdomains are consecutive and most of them share the
   // field's interior pattern, so the workgroups running at one time read
@@ -2311,6 +2317,29 @@ __global__ __launch_bounds__(1024) void k_tps_pattern_layout(
   for (int b = bd + tid; b < nd; b += 1024) order_direct[b] = -1;
   for (int b = bs + tid; b < nd; b += 1024) order_shared[b] = -1;
   if (tid == 0) order_shared[nd] = bs;
+}
+
+// The centers of the patterns in use (workgroups striding over plist): grid
+// nodes relative to the box's first node, row-major (the order of the
+// members of a subdomain), and an identity slot list.
+__global__ __launch_bounds__(256) void k_tps_pattern_nodes(
+    int npat, const int32_t* __restrict__ plist, const int32_t* __restrict__ pat_n,
+    const int32_t* __restrict__ pat_box, const int64_t* __restrict__ pdom_off,
+    const double* __restrict__ gx, const double* __restrict__ gy, double* __restrict__ pxy,
+    int32_t* __restrict__ pslot, int64_t pxy_stride) {
+  const int npu = plist[npat];
+  for (int u = blockIdx.x; u < npu; u += gridDim.x) {
+    const int p = plist[u];
+    const int n = pat_n[p];
+    const int j0 = pat_box[4 * p], nc = pat_box[4 * p + 1], i0 = pat_box[4 * p + 2];
+    const int64_t o = pdom_off[p];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const int r = k / nc, c = k % nc;
+      pxy[o + k] = gx[j0 + c] - gx[j0];
+      pxy[pxy_stride + o + k] = gy[i0 + r] - gy[i0];
+      pslot[o + k] = int(o + k);
+    }
+  }
 }
 
 // inclusive scan over a block of 256 threads (tmp: 8 ints of LDS)
@@ -4165,11 +4194,12 @@ void launch_factor(int32_t nd, int32_t n_max, const double* xs, const double* ys
 #undef PIV_PREP
   // 8 wavefronts per subdomain (one workgroup per CU) or 4 (two per CU).
   // Up to two subdomains per CU the launch is one round either way if the
-  // width fits: 8 wavefronts up to one subdomain per CU, 4 up to two; beyond,
+  // width fits: 8 wavefronts up to one subdomain per CU (and for the strided
+  // launches, over orders that list few subdomains), 4 up to two; beyond,
   // the tuned width (default 4)
   const int cus = cu_count();
   int ct = fia::tps_chol_threads();
-  if (nd <= cus) ct = 512;
+  if (nd <= cus || stride) ct = 512;
   else if (nd <= 2 * cus) ct = 256;
   else if (ct == 0) ct = 256;
   const size_t chol_lds = size_t(lat.lds_words) * sizeof(double);
@@ -4345,9 +4375,12 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
                        lim, fstat, dom_field, counts, smoothing, gx, int(ngx), int(irr_x), gy,
                        int(ngy), int(irr_y), int(cap_r), int(n_max), int(npat_cap), w.dmode,
                        w.dpat, w.dbox, w.pat_n, w.pat_box, w.pat_smooth);
-    hipLaunchKernelGGL(k_tps_pattern_layout, dim3(1), dim3(1024), 0, s, npat, w.pat_n,
-                       w.pat_box, gx, gy, nd, order, w.dmode, w.pdom_off, w.pmat_off, w.pxo, w.pxy,
-                       w.pslot, w.pxy_stride, w.order_direct, w.order_shared, w.plist);
+    hipLaunchKernelGGL(k_tps_pattern_layout, dim3(1), dim3(1024), 0, s, npat, w.pat_n, nd, order,
+                       w.dmode, w.pdom_off, w.pmat_off, w.pxo, w.order_direct, w.order_shared,
+                       w.plist);
+    hipLaunchKernelGGL(k_tps_pattern_nodes, dim3(unsigned(std::min(npat, 2 * cu_count()))),
+                       dim3(256), 0, s, npat, w.plist, w.pat_n, w.pat_box, w.pdom_off, gx, gy,
+                       w.pxy, w.pslot, w.pxy_stride);
     hipLaunchKernelGGL(k_tps_zero_i32, dim3((npat + 255) / 256), dim3(256), 0, s, npat,
                        w.pstatus);
   }
