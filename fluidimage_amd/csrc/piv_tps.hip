@@ -3195,7 +3195,7 @@ __global__ __launch_bounds__(kThreads) void k_tps_xfill(
 }
 #pragma clang fp contract(on)
 
-// dynamic LDS of k_tps_shared_x<g>: x (floats) [nm][2g] and y [nm + 4][2g]
+// dynamic LDS of k_tps_shared_x<g>: x (floats) [nm][2g] and y [2g][nm + 4]
 // (first the matrices H of the subdomains, kCapX (kCapX + 1) doubles each),
 // then the values (floats) [g][2][nm] and member nodes [g][nm]
 constexpr int shared_x_head(int g, int nm) {
@@ -3247,7 +3247,8 @@ __device__ void shared_x_solve(
   const double* X = pinv + pxo[p];
   extern __shared__ double dyn[];
   float* xv = reinterpret_cast<float*>(dyn);  // [nm][NV]
-  double* yv = dyn + G * nm;                  // [nm + 4][NV]
+  double* yv = dyn + G * nm;                  // [NV][ny], component-major
+  const int ny = nm + 4;
   float* val = reinterpret_cast<float*>(dyn + shared_x_head(G, nm));   // [G][2][nm]
   int* fidx = reinterpret_cast<int*>(dyn + shared_x_head(G, nm) + G * nm);  // [G][nm]
   int* fmem = reinterpret_cast<int*>(yv);  // [G][nm], during the checks
@@ -3473,16 +3474,19 @@ __device__ void shared_x_solve(
           acc1[q] += a.y * xq;
         }
       }
+      if (i + 1 < nf + 3) {
 #pragma unroll
-      for (int q = 0; q < NV; ++q) yv[i * NV + q] = acc0[q];
-      if (i + 1 < nf + 3)
+        for (int q = 0; q < NV; ++q)
+          *reinterpret_cast<double2*>(yv + q * ny + i) = make_double2(acc0[q], acc1[q]);
+      } else {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) yv[(i + 1) * NV + q] = acc1[q];
+        for (int q = 0; q < NV; ++q) yv[q * ny + i] = acc0[q];
+      }
     }
     __syncthreads();
     if (tid < NV)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) wa[tid][k] = yv[(nf + k) * NV + tid];  // Y^T v'
+      for (int k = 0; k < 3; ++k) wa[tid][k] = yv[tid * ny + nf + k];  // Y^T v'
     __syncthreads();
     // lambda = -H^-1 y[R]: thread (q, r)
     for (int e = tid; e < NV * kCapX; e += kThreads) {
@@ -3497,7 +3501,7 @@ __device__ void shared_x_solve(
         for (int u = 0; u < 8; ++u) h[u] = k0 + u < nr ? hr[k0 + u] : 0.0;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-          if (k0 + u < nr) v += h[u] * yv[rl[s][k0 + u] * NV + q];
+          if (k0 + u < nr) v += h[u] * yv[q * ny + rl[s][k0 + u]];
       }
       lam[q][r] = -v;
     }
@@ -3518,7 +3522,7 @@ __device__ void shared_x_solve(
       if (!go[2 * s] && !go[2 * s + 1]) continue;
       const int nr = nr_sh[s];
       for (int k = tid; k < nf; k += kThreads) {
-        double w0 = yv[k * NV + 2 * s], w1 = yv[k * NV + 2 * s + 1];
+        double w0 = yv[2 * s * ny + k], w1 = yv[(2 * s + 1) * ny + k];
         for (int r0 = 0; r0 < nr; r0 += 8) {
           double g[8];
 #pragma unroll
@@ -3530,8 +3534,8 @@ __device__ void shared_x_solve(
               w1 += g[u] * lam[2 * s + 1][r0 + u];
             }
         }
-        yv[k * NV + 2 * s] = w0;
-        yv[k * NV + 2 * s + 1] = w1;
+        yv[2 * s * ny + k] = w0;
+        yv[(2 * s + 1) * ny + k] = w1;
       }
     }
     __syncthreads();
@@ -3548,7 +3552,7 @@ __device__ void shared_x_solve(
       double dm = 0.0, ne = 0.0;
       for (int j = tid; j < n; j += kThreads) {
         const double v = double(val[q * nm + j]);
-        const double wq = yv[fidx[s * nm + j] * NV + q];
+        const double wq = yv[q * ny + fidx[s * nm + j]];
         const double sm = v - smoothing * wq;
         smooth_slots[c * nslots + s0 + j] = sm;
         const double diff = fabs(sm - v);
@@ -3598,7 +3602,7 @@ __device__ void shared_x_solve(
       const double smoothing = smoothing_d[d];
       for (int j = tid; j < n; j += kThreads) {
         const double v = double(val[q * nm + j]);
-        const double sm = v - smoothing * yv[fidx[s * nm + j] * NV + q];
+        const double sm = v - smoothing * yv[q * ny + fidx[s * nm + j]];
         if (fabs(sm - v) > threshold) val[q * nm + j] = float(sm);
       }
     }
