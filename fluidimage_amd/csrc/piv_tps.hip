@@ -2803,8 +2803,8 @@ k_tps_shared_iter(const int32_t* __restrict__ order, const int32_t* __restrict__
 // reflectors Q = H_0 H_1 H_2 = I - V T V^T of P = Q [R; 0], C = L L^T and
 // E = B'[3:, :3]. They are built once per pattern:
 //   k_tps_tri_inverse  Linv = L^-1, block columns in parallel;
-//   k_tps_inv_gram     C^-1 = Linv^T Linv, over L in the pattern's matrix
-//                      region (E is kept);
+//   k_tps_inv_gram     C^-1 = Linv^T Linv (64 x 64 tiles), over L in the
+//                      pattern's matrix region (E is kept);
 //   k_tps_xprep        U = C^-1 [V[3:, :] | E] (one pass over C^-1), then the
 //                      rank-6 form X = Z - A Cc^T - Cc A^T of the two-sided
 //                      reflection (Z = [[0, 0], [0, C^-1]], A = V T,
@@ -2945,73 +2945,88 @@ __global__ __launch_bounds__(kThreads) void k_tps_tri_inverse(
     tri_inverse_column(plist[w % npu], w / npu, pat_n, pmat_off, pmats, pstatus, pinv);
 }
 
-// C^-1 = Linv^T Linv of pattern p: tile (I, J), I >= J, number
-// t = I (I + 1) / 2 + J: sum_{K >= I} Linv_KI^T Linv_KJ, written at (I, J)
-// and mirrored at (J, I) over L (E is kept). The two threads of a mirrored
-// pair inside a diagonal tile add the same products in the same order: C^-1
-// is exactly symmetric.
+// C^-1 = Linv^T Linv of pattern p: 64 x 64 tile (I, J), I >= J, number
+// t = I (I + 1) / 2 + J: the sum over the 32-row blocks K of Linv_KI^T
+// Linv_KJ (rows 64 I on), written at (I, J) and mirrored at (J, I) over L (E
+// is kept). Each thread has 4 x 4 outputs; the blocks of each term are
+// staged in LDS with the next term's loads in flight. The two threads of a
+// mirrored pair inside a diagonal tile add the same products in the same
+// order: C^-1 is exactly symmetric.
 __device__ void inv_gram_tile(int p, int t, const int32_t* __restrict__ pat_n,
                               const int64_t* __restrict__ pmat_off, double* __restrict__ pmats,
                               const int32_t* __restrict__ pstatus,
                               const double* __restrict__ pinv) {
   static_assert(kThreads == 256 && kNB == 32, "thread layout of inv_gram_tile");
+  constexpr int kT = 64;
   const int n = pat_n[p];
   if (n < 4 || pstatus[p]) return;
   const int m = n - 3;
-  const int nblk = (m + kNB - 1) / kNB;
+  const int nblk = (m + kNB - 1) / kNB, nt = (m + kT - 1) / kT;
   int I = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((I + 1) * (I + 2) / 2 <= t) ++I;
   while (I * (I + 1) / 2 > t) --I;
   const int J = t - I * (I + 1) / 2;
-  if (I >= nblk) return;
+  if (I >= nt) return;
   const int64_t ld = tps_mat_ld(m);
   const double* Li = pinv + pmat_off[p] + tps_mat_e_size(m);
   double* Xc = pmats + pmat_off[p] + tps_mat_e_size(m);
-  __shared__ double Ab[kNB][kNB + 1];  // Ab[k][a] = Linv[r0K + k][r0I + a]
-  __shared__ double Bb[kNB][kNB + 1];  // Bb[k][b] = Linv[r0K + k][r0J + b]
+  __shared__ double Ab[kNB][kT + 1];  // Ab[k][a] = Linv[32 K + k][64 I + a]
+  __shared__ double Bb[kNB][kT + 1];  // Bb[k][b] = Linv[32 K + k][64 J + b]
   const int tid = threadIdx.x;
-  const int r0I = I * kNB, bsI = min(kNB, m - r0I);
-  const int r0J = J * kNB, bsJ = min(kNB, m - r0J);
-  const int ta = tid >> 3, tc = tid & 7;
-  const int er = tid >> 5, ec = tid & 31;
+  const int cI = kT * I, cJ = kT * J;
+  const int ta = tid >> 4, tb = tid & 15;  // outputs (ta + 16 u, tb + 16 v)
+  // staged element e = tid + 256 u: row e / 64, column e % 64 (lower
+  // triangle of Linv only)
   auto load = [&](int K, double* la, double* lb) {
-    const int r0K = K * kNB, bsK = min(kNB, m - r0K);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = er + 8 * u;
-      la[u] = (k < bsK && ec < bsI && (K > I || ec <= k)) ? Li[int64_t(r0K + k) * ld + r0I + ec]
-                                                          : 0.0;
-      lb[u] = (k < bsK && ec < bsJ && (K > J || ec <= k)) ? Li[int64_t(r0K + k) * ld + r0J + ec]
-                                                          : 0.0;
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + kThreads * u;
+      const int R = kNB * K + (e >> 6), c = e & 63;
+      la[u] = (R < m && cI + c <= R) ? Li[int64_t(R) * ld + cI + c] : 0.0;
+      lb[u] = (R < m && cJ + c <= R) ? Li[int64_t(R) * ld + cJ + c] : 0.0;
     }
   };
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  double la[4], lb[4];
-  load(I, la, lb);
-  for (int K = I; K < nblk; ++K) {
+  double acc[4][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      Ab[er + 8 * u][ec] = la[u];
-      Bb[er + 8 * u][ec] = lb[u];
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  double la[8], lb[8];
+  const int K0 = cI / kNB;
+  load(K0, la, lb);
+  for (int K = K0; K < nblk; ++K) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + kThreads * u;
+      Ab[e >> 6][e & 63] = la[u];
+      Bb[e >> 6][e & 63] = lb[u];
     }
     __syncthreads();
     if (K + 1 < nblk) load(K + 1, la, lb);
-#pragma unroll 4
+#pragma unroll 2
     for (int k = 0; k < kNB; ++k) {
-      const double a = Ab[k][ta];
+      double a[4], b[4];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] = fma(a, Bb[k][tc + 8 * v], acc[v]);
+      for (int u = 0; u < 4; ++u) a[u] = Ab[k][ta + 16 * u];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) b[v] = Bb[k][tb + 16 * v];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int b = tc + 8 * v;
-    if (ta < bsI && b < bsJ) {
-      Xc[int64_t(r0I + ta) * ld + r0J + b] = acc[v];
-      Xc[int64_t(r0J + b) * ld + r0I + ta] = acc[v];
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = cI + ta + 16 * u, c = cJ + tb + 16 * v;
+      if (r < m && c < m) {
+        Xc[int64_t(r) * ld + c] = acc[u][v];
+        Xc[int64_t(c) * ld + r] = acc[u][v];
+      }
     }
-  }
 }
 
 // items: tile t = w / npu (the tiles of most terms first) of pattern
@@ -4415,8 +4430,9 @@ int piv_tps_smooth_shared(int32_t nfields, int32_t dom_cap, const piv_tps_field*
     auto grid = [&](int64_t per_cu) { return dim3(unsigned(std::min<int64_t>(nitem, 256 * per_cu))); };
     hipLaunchKernelGGL(k_tps_tri_inverse, grid(4), dim3(kThreads), 0, s, w.plist, npat, nblk,
                        w.pat_n, w.pmat_off, w.pmats, w.pstatus, w.pinv);
-    hipLaunchKernelGGL(k_tps_inv_gram, grid(6), dim3(kThreads), 0, s, w.plist, npat,
-                       nblk * (nblk + 1) / 2, w.pat_n, w.pmat_off, w.pmats, w.pstatus, w.pinv);
+    const int nt = (n_max - 3 + 63) / 64;
+    hipLaunchKernelGGL(k_tps_inv_gram, grid(4), dim3(kThreads), 0, s, w.plist, npat,
+                       nt * (nt + 1) / 2, w.pat_n, w.pmat_off, w.pmats, w.pstatus, w.pinv);
     hipLaunchKernelGGL(k_tps_xprep, dim3(unsigned(std::min(npat, 256))), dim3(kThreads),
                        size_t(6) * nm * sizeof(double), s, w.plist, npat, w.pat_n, w.pmat_off,
                        w.pmats, w.pdom_off, w.prefl, w.pvwork, w.pstatus, w.pxo, w.pinv, w.pac);
