@@ -24,10 +24,11 @@ Workload "single": the single-pass correlation of the same kind of pairs
 windows of such pairs. Workload "refine": the differentiable refinement of
 1024x1024 float32 pairs.
 
-The roofline object is the longest correlation launch of a step (the stage
-whose bytes are the algorithmic HBM figure: unique image bytes + maps +
-16 B peak records), timed as back-to-back relaunches between two HIP events
-on the launch stream. traffic comes from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+The roofline object is the first pass's correlation launch (the one that
+reads the caller's images; its bytes are the algorithmic HBM figure: unique
+image bytes + maps + 16 B peak records), timed as back-to-back relaunches
+between two HIP events on the launch stream. The deformed passes' per-pass
+figures count the float32 images of the deformation as their source. traffic comes from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
 of that kernel when one exists for the workload.
 
 Prints one JSON line. --copy-calibration runs a plain 256 MiB device copy,
@@ -860,20 +861,26 @@ def main():
     nerr = int((passes[-1].out["err"] != 0).sum().item())
 
     # correlation launches: event-bracketed per pass in the steps, and the
-    # longest one relaunched back to back for its clean average duration
+    # first pass's (the one that reads the caller's images; the deformed
+    # passes read the float32 images of the deformation) relaunched back to
+    # back for its clean average duration
+    deformed = getattr(work.params.multipass, "image_deformation", None) is not None
+
+    def src_itemsize(w):
+        return 4 if deformed and w.index_pass > 0 else im0.dtype.itemsize
+
     per_pass = {}
     for ip, w in enumerate(work.works_piv):
         key = f"correlate_pass{ip}"
         if key in avg:
             nwin = w.nb_vectors * npairs
-            b = correl_bytes(w, im0.shape, im0.dtype.itemsize, nwin, npairs, keep_maps)
             per_pass[key] = {"window": list(w.shape_crop_im0), "windows": int(nwin),
                              "ms_in_step": round(avg[key], 4)}
     if not per_pass:  # graph replays: no per-pass events
         w = work.works_piv[0]
         per_pass["correlate_pass0"] = {"window": list(w.shape_crop_im0),
                                        "windows": int(w.nb_vectors * npairs), "ms_in_step": None}
-    ck = max(per_pass, key=lambda k: per_pass[k]["ms_in_step"] or 0.0)
+    ck = "correlate_pass0" if "correlate_pass0" in per_pass else min(per_pass)
     cip = int(ck[len("correlate_pass"):])
     cw = work.works_piv[cip]
     po = passes[cip]
@@ -895,7 +902,7 @@ def main():
         if v["ms_in_step"] is None:
             continue
         w = work.works_piv[int(key[len("correlate_pass"):])]
-        b = correl_bytes(w, im0.shape, im0.dtype.itemsize, v["windows"], npairs, keep_maps)
+        b = correl_bytes(w, im0.shape, src_itemsize(w), v["windows"], npairs, keep_maps)
         v["GB/s_in_step"] = round(b / (v["ms_in_step"] / 1e3) / 1e9, 1)
         v["valu_TFLOP/s_in_step"] = round(
             correl_flops(w, v["windows"]) / (v["ms_in_step"] / 1e3) / 1e12, 2)
