@@ -1616,36 +1616,76 @@ __device__ __forceinline__ SubGrid field_grid(const piv_tps_field& f, const doub
   return SubGrid{f.nsx, f.nsy, lim + f.lim_off};
 }
 
-// first index i in [0, n) with a[i] >= v for a nondecreasing array, by the
-// whole block: the block's samples a[t step] locate the bucket, then the
-// bucket's elements are counted chunk by chunk
-__device__ int block_lower_bound(const double* a, int n, double v) {
-  const int nt = blockDim.x;
-  const int step = (n + nt - 1) / nt;
-  const int i1 = threadIdx.x * step;
-  const int c = __syncthreads_count(i1 < n && a[i1] < v);  // samples below v
-  if (c == 0) return 0;
-  int lo = (c - 1) * step + 1;  // a[lo - 1] < v; the answer is in [lo, hi]
-  const int hi = min(c * step, n);
-  int below = 0;
-  for (int c0 = lo; c0 < hi; c0 += nt) {
-    const int i2 = c0 + threadIdx.x;
-    below += __syncthreads_count(i2 < hi && a[i2] < v);
+// first index in [0, n) with a[i] >= v (a ascending), n if none, by one
+// wavefront: 64 samples a level narrow the range 64-fold, then one ballot
+__device__ int wave_lower_bound(const double* a, int n, double v) {
+  const int lane = threadIdx.x & 63;
+  int lo = 0, hi = n;  // the answer is in [lo, hi]
+  while (hi - lo > 64) {
+    const int s = (hi - lo + 63) / 64;
+    const int j = lo + lane * s;
+    const int c = __popcll(__ballot(j < hi && a[j] < v));
+    if (c == 0) return lo;
+    const int nlo = lo + (c - 1) * s + 1;
+    hi = min(lo + c * s, hi);
+    lo = nlo;
   }
-  return lo + below;
+  const int j = lo + lane;
+  return lo + __popcll(__ballot(j < hi && a[j] < v));
 }
 
-// the range of centers [first, last) of a field that may lie in subdomain d:
-// those with y in [ylo, yhi) when the centers are sorted by y, else all
-__device__ void member_range(const piv_tps_field& f, const SubGrid& g, int d, const double* cy,
-                             int& first, int& last) {
+// the range of centers [first, last) of a field that may lie in subdomain d
+// (those with y in [ylo, yhi) when the centers are sorted by y, else all),
+// by one wavefront
+__device__ void wave_member_range(const piv_tps_field& f, const SubGrid& g, int d,
+                                  const double* cy, int& first, int& last) {
   first = 0;
   last = f.count;
   if (f.sorted_y) {
     const int iy = d / g.nsx;
-    first = block_lower_bound(cy, f.count, g.ylo(iy));
-    last = block_lower_bound(cy, f.count, g.yhi(iy));
+    first = wave_lower_bound(cy, f.count, g.ylo(iy));
+    last = wave_lower_bound(cy, f.count, g.yhi(iy));
   }
+}
+
+// One workgroup of four wavefronts per (global) subdomain slot: the
+// candidates [first, last) split into four contiguous segments, one per
+// wavefront, tested 64 at a time with four chunks' loads in flight and
+// counted by ballots (no barrier inside a segment)
+constexpr int kSlotChunks = 4;
+
+// the members of subdomain d among the candidates [s0, s1) (one wavefront):
+// their number, and with WRITE their center indices from slot_pt[base] on,
+// in increasing order
+template <bool WRITE>
+__device__ int segment_members(const SubGrid& g, int d, const double* cx, const double* cy,
+                               int s0, int s1, int center_base, int32_t* slot_pt,
+                               int64_t base) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int c = 0;
+  for (int c0 = s0; c0 < s1; c0 += 64 * kSlotChunks) {
+    bool in[kSlotChunks];
+#pragma unroll
+    for (int u = 0; u < kSlotChunks; ++u) {
+      const int i = c0 + 64 * u + lane;
+      in[u] = i < s1 && g.inside(d, cx[i], cy[i]);
+    }
+#pragma unroll
+    for (int u = 0; u < kSlotChunks; ++u) {
+      const unsigned long long m = __ballot(in[u]);
+      if (WRITE && in[u]) slot_pt[base + c + __popcll(m & below)] = center_base + c0 + 64 * u + lane;
+      c += __popcll(m);
+    }
+  }
+  return c;
+}
+
+// segment w (of 4) of [first, last), 64-aligned
+__device__ __forceinline__ void segment_of(int first, int last, int w, int& s0, int& s1) {
+  const int len = ((last - first + 255) / 256) * 64;
+  s0 = min(last, first + w * len);
+  s1 = min(last, s0 + len);
 }
 
 // members of (global) subdomain blockIdx.x: count
@@ -1665,11 +1705,10 @@ __global__ __launch_bounds__(256) void k_tps_count(const piv_tps_field* __restri
   const int d = blockIdx.x - f.dom_base;
   const double* cx = cx_all + f.center_base;
   const double* cy = cy_all + f.center_base;
-  int first, last;
-  member_range(f, g, d, cy, first, last);
-  int c = 0;
-  for (int i = first + threadIdx.x; i < last; i += 256) c += g.inside(d, cx[i], cy[i]) ? 1 : 0;
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  int first, last, s0, s1;
+  wave_member_range(f, g, d, cy, first, last);
+  segment_of(first, last, threadIdx.x >> 6, s0, s1);
+  const int c = segment_members<false>(g, d, cx, cy, s0, s1, 0, nullptr, 0);
   __shared__ int part[4];
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
   __syncthreads();
@@ -1677,14 +1716,13 @@ __global__ __launch_bounds__(256) void k_tps_count(const piv_tps_field* __restri
 }
 
 // members of (global) subdomain blockIdx.x in increasing center order (np.where)
-__global__ __launch_bounds__(1024) void k_tps_fill(const piv_tps_field* __restrict__ fields,
-                                                   const int32_t* __restrict__ dom_field,
-                                                   const double* __restrict__ lim,
-                                                   const double* __restrict__ cx_all,
-                                                   const double* __restrict__ cy_all,
-                                                   const int64_t* __restrict__ dom_off,
-                                                   int32_t* slot_pt) {
-  __shared__ int tmp[32];
+__global__ __launch_bounds__(256) void k_tps_fill(const piv_tps_field* __restrict__ fields,
+                                                  const int32_t* __restrict__ dom_field,
+                                                  const double* __restrict__ lim,
+                                                  const double* __restrict__ cx_all,
+                                                  const double* __restrict__ cy_all,
+                                                  const int64_t* __restrict__ dom_off,
+                                                  int32_t* slot_pt) {
   const int fi = dom_field[blockIdx.x];
   // inactive subdomain, or one of a field the offsets left out (no slots)
   if (fi < 0 || dom_off[blockIdx.x + 1] == dom_off[blockIdx.x]) return;
@@ -1694,17 +1732,18 @@ __global__ __launch_bounds__(1024) void k_tps_fill(const piv_tps_field* __restri
   const int center_base = f.center_base;
   const double* cx = cx_all + center_base;
   const double* cy = cy_all + center_base;
+  const int w = threadIdx.x >> 6;
+  int first, last, s0, s1;
+  wave_member_range(f, g, d, cy, first, last);
+  segment_of(first, last, w, s0, s1);
+  // each segment's count, then its members from the segments before it on
+  __shared__ int part[4];
+  const int c = segment_members<false>(g, d, cx, cy, s0, s1, 0, nullptr, 0);
+  if ((threadIdx.x & 63) == 0) part[w] = c;
+  __syncthreads();
   int64_t base = dom_off[blockIdx.x];
-  int first, last;
-  member_range(f, g, d, cy, first, last);
-  for (int c0 = first; c0 < last; c0 += 1024) {
-    const int i = c0 + threadIdx.x;
-    const int in = (i < last && g.inside(d, cx[i], cy[i])) ? 1 : 0;
-    int total;
-    const int incl = block_scan_1024(in, tmp, total);
-    if (in) slot_pt[base + incl - 1] = center_base + i;
-    base += total;
-  }
+  for (int k = 0; k < w; ++k) base += part[k];
+  segment_members<true>(g, d, cx, cy, s0, s1, center_base, slot_pt, base);
 }
 
 // per point: its subdomains in increasing order (at most kMaxSlots), their
@@ -1941,10 +1980,15 @@ __global__ __launch_bounds__(1024) void k_tps_offsets(
   __shared__ int64_t fmats[kOffsetsMaxFields];
   const int nd = nfields * dom_cap;
   const int tid = threadIdx.x;
-  for (int f = tid; f < nfields; f += 1024) flags[f] = 0;
+  for (int f = tid; f < nfields; f += 1024) {
+    flags[f] = 0;
+    fslots[f] = 0;
+    fmats[f] = 0;
+  }
   for (int b = tid; b < kMaxN + 2; b += 1024) hist[b] = 0;
   __syncthreads();
   // 1. per-subdomain checks: bit 0 empty, bit 1 more than kMaxN, bit 2 above n_max_cap
+  //    and each field's slots and matrix doubles (integer sums: any order)
   for (int d = tid; d < nd; d += 1024) {
     const int f = dom_field[d];
     if (f < 0 || fstat[f] != PIV_TPS_FIELD_OK) continue;
@@ -1954,26 +1998,20 @@ __global__ __launch_bounds__(1024) void k_tps_offsets(
     else if (c > kMaxN) bits = 2;
     else if (c > n_max_cap) bits = 4;
     if (bits) atomicOr(&flags[f], bits);
+    atomicAdd(reinterpret_cast<unsigned long long*>(&fslots[f]), (unsigned long long)c);
+    atomicAdd(reinterpret_cast<unsigned long long*>(&fmats[f]),
+              (unsigned long long)tps_mat_size(c));
   }
   __syncthreads();
   // 2. statuses (an empty subdomain first, as the host sizing checks it
-  //    first), then each field's slots and matrix doubles
+  //    first)
   for (int f = tid; f < nfields; f += 1024) {
     int st = fstat[f];
     const int b = flags[f];
     if (st == PIV_TPS_FIELD_OK && b)
       st = (b & 1) ? PIV_TPS_FIELD_EMPTY_SUBDOMAIN
                    : ((b & 2) ? PIV_TPS_FIELD_TOO_MANY : PIV_TPS_FIELD_CAPACITY);
-    int64_t sl = 0, ma = 0;
-    if (st == PIV_TPS_FIELD_OK)
-      for (int d = f * dom_cap; d < (f + 1) * dom_cap; ++d) {
-        if (dom_field[d] < 0) continue;
-        sl += counts[d];
-        ma += tps_mat_size(counts[d]);
-      }
     flags[f] = st;
-    fslots[f] = sl;
-    fmats[f] = ma;
   }
   __syncthreads();
   // 3. admission in field order while the capacities hold
@@ -4561,7 +4599,7 @@ int piv_tps_fill(int32_t nd, const piv_tps_field* fields, const int32_t* dom_fie
   if (nd <= 0) return tps_fail(PIV_EINVAL, "empty subdomain grid");
   if (!fields || !dom_field || !lim || !cx || !cy || !dom_off || !slot_pt)
     return tps_fail(PIV_EINVAL, "NULL argument");
-  hipLaunchKernelGGL(k_tps_fill, dim3(nd), dim3(1024), 0, static_cast<hipStream_t>(stream),
+  hipLaunchKernelGGL(k_tps_fill, dim3(nd), dim3(256), 0, static_cast<hipStream_t>(stream),
                      fields, dom_field, lim, cx, cy, dom_off, slot_pt);
   return tps_check("piv_tps_fill");
 }
