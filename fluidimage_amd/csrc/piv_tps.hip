@@ -3264,6 +3264,57 @@ constexpr int shared_x_group(int nm) {
   return shared_x_lds(4, nm) <= 80 * 1024 ? 4 : shared_x_lds(2, nm) <= 80 * 1024 ? 2 : 1;
 }
 
+// [y; Y^T v'] = [X | Y]^T v' for the K columns of x (v' of component aq[k]
+// in column k of xv, rows of NV floats; aq[k] < 0: a zero column, not
+// stored): X is symmetric, so lanes over output pairs i, i + 1 read the rows
+// of [X | Y] by columns, 16 bytes a load, kU loads in flight; outputs
+// i >= nf + 3 read the padding. y of component q goes to yv[q ny + i].
+// Each component's sums are the same for every K.
+template <int K, int NV>
+__device__ __forceinline__ void x_products(const double* X, int64_t ldx, int nf, const float* xv,
+                                           double* yv, int ny, const int* aq) {
+  for (int i = 2 * int(threadIdx.x); i < nf + 3; i += 2 * kThreads) {
+    double acc0[K], acc1[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc0[k] = acc1[k] = 0.0;
+    const double* col = X + i;
+    constexpr int kU = 8;
+    int j = 0;
+    for (; j + kU <= nf; j += kU) {
+      double2 a[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        a[u] = *reinterpret_cast<const double2*>(col + int64_t(j + u) * ldx);
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const double xq = double(xv[(j + u) * NV + k]);
+          acc0[k] += a[u].x * xq;
+          acc1[k] += a[u].y * xq;
+        }
+    }
+    for (; j < nf; ++j) {
+      const double2 a = *reinterpret_cast<const double2*>(col + int64_t(j) * ldx);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double xq = double(xv[j * NV + k]);
+        acc0[k] += a.x * xq;
+        acc1[k] += a.y * xq;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = aq[k];
+      if (q < 0) continue;
+      if (i + 1 < nf + 3)
+        *reinterpret_cast<double2*>(yv + q * ny + i) = make_double2(acc0[k], acc1[k]);
+      else
+        yv[q * ny + i] = acc0[k];
+    }
+  }
+}
+
 // The shared subdomains of group gi (up to G of one pattern, consecutive in
 // the shared order) with the explicit inverse: the checks of
 // k_tps_shared_columns, then the outlier iterations of k_tps_shared_iter
@@ -3313,6 +3364,8 @@ __device__ void shared_x_solve(
   __shared__ double lam[NV][kCapX];
   __shared__ double wa[NV][3];
   __shared__ int go[NV];
+  __shared__ int aq[NV];
+  __shared__ int na_sh;
   __shared__ int act[G];
   __shared__ int bad[G];
   __shared__ double nfixed[NV], maxdiff[NV];
@@ -3483,57 +3536,33 @@ __device__ void shared_x_solve(
     bool any = false;
     for (int q = 0; q < NV; ++q) any = any || go[q] != 0;
     if (!any) break;
-    // v' of every running component (0 elsewhere)
+    // the running components, compacted: column k of x holds component
+    // aq[k] (k < na; the columns up to the product's even width are zero)
+    if (tid == 0) {
+      int k = 0;
+      for (int q = 0; q < NV; ++q)
+        if (go[q]) aq[k++] = q;
+      na_sh = k;
+      for (; k < NV; ++k) aq[k] = -1;
+    }
+    // v' of every running component (0 at the missing nodes)
     for (int e = tid; e < nf * NV; e += kThreads) xv[e] = 0.0f;
     __syncthreads();
-    for (int q = 0; q < NV; ++q) {
-      if (!go[q]) continue;
-      const int s = q >> 1;
+    const int na = na_sh;
+    for (int k = 0; k < na; ++k) {
+      const int q = aq[k], s = q >> 1;
       const int d = order[b0 + s];
       const int n = int(dom_off[d + 1] - dom_off[d]);
-      for (int j = tid; j < n; j += kThreads) xv[fidx[s * nm + j] * NV + q] = val[q * nm + j];
+      for (int j = tid; j < n; j += kThreads) xv[fidx[s * nm + j] * NV + k] = val[q * nm + j];
     }
     __syncthreads();
-    // [y; Y^T v'] = [X | Y]^T v' (X symmetric: lanes over output pairs i,
-    // i + 1 read the rows of [X | Y] by columns, 16 bytes a load, kU loads
-    // in flight; outputs i >= nf + 3 read the padding)
-    for (int i = 2 * tid; i < nf + 3; i += 2 * kThreads) {
-      double acc0[NV], acc1[NV];
-#pragma unroll
-      for (int q = 0; q < NV; ++q) acc0[q] = acc1[q] = 0.0;
-      const double* col = X + i;
-      constexpr int kU = 8;
-      int j = 0;
-      for (; j + kU <= nf; j += kU) {
-        double2 a[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u)
-          a[u] = *reinterpret_cast<const double2*>(col + int64_t(j + u) * ldx);
-#pragma unroll
-        for (int u = 0; u < kU; ++u)
-#pragma unroll
-          for (int q = 0; q < NV; ++q) {
-            const double xq = double(xv[(j + u) * NV + q]);
-            acc0[q] += a[u].x * xq;
-            acc1[q] += a[u].y * xq;
-          }
-      }
-      for (; j < nf; ++j) {
-        const double2 a = *reinterpret_cast<const double2*>(col + int64_t(j) * ldx);
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-          const double xq = double(xv[j * NV + q]);
-          acc0[q] += a.x * xq;
-          acc1[q] += a.y * xq;
-        }
-      }
-      if (i + 1 < nf + 3) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q)
-          *reinterpret_cast<double2*>(yv + q * ny + i) = make_double2(acc0[q], acc1[q]);
-      } else {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) yv[q * ny + i] = acc0[q];
+    // [y; Y^T v'] = [X | Y]^T v' for the running components only
+    if (na <= 2) x_products<2, NV>(X, ldx, nf, xv, yv, ny, aq);
+    else if constexpr (NV >= 4) {
+      if (na <= 4) x_products<4, NV>(X, ldx, nf, xv, yv, ny, aq);
+      else if constexpr (NV >= 8) {
+        if (na <= 6) x_products<6, NV>(X, ldx, nf, xv, yv, ny, aq);
+        else x_products<8, NV>(X, ldx, nf, xv, yv, ny, aq);
       }
     }
     __syncthreads();
