@@ -3599,11 +3599,12 @@ __device__ void shared_x_solve(
       }
     }
     __syncthreads();
-    // w = y + X[:, R] lambda
-    for (int s = 0; s < G; ++s) {
+    // w = y + X[:, R] lambda (items (s, k) over every subdomain at once)
+    for (int e = tid; e < G * nf; e += kThreads) {
+      const int s = e / nf, k = e - s * nf;
       if (!go[2 * s] && !go[2 * s + 1]) continue;
       const int nr = nr_sh[s];
-      for (int k = tid; k < nf; k += kThreads) {
+      {
         double w0 = yv[2 * s * ny + k], w1 = yv[(2 * s + 1) * ny + k];
         for (int r0 = 0; r0 < nr; r0 += 8) {
           double g[8];
