@@ -276,6 +276,21 @@ __device__ __forceinline__ float peak_value(float v, double norm) {
   return norm == 0.0 ? 0.f : v / float(norm);
 }
 
+// the log-quadratic fit of compute_subpix_2d_gaussian2 (subpix.py:45-61)
+// from the sums of x l, y l, x y l, (3 x^2 - 2) l, (3 y^2 - 2) l over the
+// 3x3 crop (l the logarithms, x, y in -1..1)
+__device__ __forceinline__ void gaussian2_fit(double c10, double c01, double c11, double c20,
+                                              double c02, double& dpx, double& dpy) {
+  c10 /= 6;
+  c01 /= 6;
+  c11 /= 4;
+  c20 /= 6;
+  c02 /= 6;
+  double den = 4 * c20 * c02 - c11 * c11;
+  dpx = (c11 * c01 - 2 * c10 * c02) / den;
+  dpy = (c11 * c10 - 2 * c01 * c20) / den;
+}
+
 // compute_subpix_2d_gaussian2 (subpix.py:16-61): float32 normalisation of
 // the 3x3 crop, then the log-quadratic fit.
 __device__ __forceinline__ void gaussian2(const float* m, int n1, int ix, int iy, double& dpx,
@@ -288,8 +303,7 @@ __device__ __forceinline__ void gaussian2(const float* m, int n1, int ix, int iy
   for (int k = 0; k < 9; ++k) c[k] -= mn;
   float mx = c[0];
   for (int k = 1; k < 9; ++k) mx = fmaxf(mx, c[k]);
-  const float inv = 1.f / mx;
-  for (int k = 0; k < 9; ++k) c[k] *= inv;
+  for (int k = 0; k < 9; ++k) c[k] /= mx;  // float32 division, as the reference's /=
   double c10 = 0, c01 = 0, c11 = 0, c20 = 0, c02 = 0;
   for (int a = 0; a < 3; ++a) {
     for (int b = 0; b < 3; ++b) {
@@ -304,14 +318,71 @@ __device__ __forceinline__ void gaussian2(const float* m, int n1, int ix, int iy
       c02 += (3 * y * y - 2) * l;
     }
   }
-  c10 /= 6;
-  c01 /= 6;
-  c11 /= 4;
-  c20 /= 6;
-  c02 /= 6;
-  double den = 4 * c20 * c02 - c11 * c11;
-  dpx = (c11 * c01 - 2 * c10 * c02) / den;
-  dpy = (c11 * c10 - 2 * c01 * c20) / den;
+  gaussian2_fit(c10, c01, c11, c20, c02, dpx, dpy);
+}
+
+// gaussian2 with the lanes of one window (T >= 16 of one wavefront, all
+// active): lane k < 9 takes element k = 3 a + b of the 3x3 crop through the
+// normalisation and the logarithm, the other lanes of the first 16 hold
+// element 0 (neutral for the minimum and the maximum, which are exact in any
+// order), and every lane then forms the five sums in gaussian2's order. The
+// coefficients x = b - 1, y = a - 1, x y, 3 x^2 - 2 and 3 y^2 - 2 are 0, +-1
+// or -2, so each product is exact and each term one rounded addition; the
+// zero terms, which add 0 to a sum of finite logarithms, are left out. The
+// result is gaussian2's bit for bit.
+template <int T>
+__device__ __forceinline__ void gaussian2_lanes(const float* m, int n1, int ix, int iy, int lane,
+                                                double& dpx, double& dpy) {
+  static_assert(T >= 16 && T <= 64, "one wavefront, at least 16 lanes per window");
+  const int k = lane < 9 ? lane : 0;
+  const int a = k / 3, b = k - 3 * a;
+  float c = m[(iy - 1 + a) * n1 + (ix - 1 + b)];
+  float mn = c;
+  for (int off = 8; off > 0; off >>= 1) mn = fminf(mn, __shfl_xor(mn, off, T));
+  c -= mn;
+  float mx = c;
+  for (int off = 8; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, T));
+  c /= mx;
+  if (c == 0.f) c = 1e-8f;
+  const float lf = logf(c);
+  double l[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) l[j] = double(__shfl(lf, j, T));
+  double c10 = -l[0];
+  c10 += l[2];
+  c10 -= l[3];
+  c10 += l[5];
+  c10 -= l[6];
+  c10 += l[8];
+  double c01 = -l[0];
+  c01 -= l[1];
+  c01 -= l[2];
+  c01 += l[6];
+  c01 += l[7];
+  c01 += l[8];
+  double c11 = l[0];
+  c11 -= l[2];
+  c11 -= l[6];
+  c11 += l[8];
+  double c20 = l[0];
+  c20 += -2.0 * l[1];
+  c20 += l[2];
+  c20 += l[3];
+  c20 += -2.0 * l[4];
+  c20 += l[5];
+  c20 += l[6];
+  c20 += -2.0 * l[7];
+  c20 += l[8];
+  double c02 = l[0];
+  c02 += l[1];
+  c02 += l[2];
+  c02 += -2.0 * l[3];
+  c02 += -2.0 * l[4];
+  c02 += -2.0 * l[5];
+  c02 += l[6];
+  c02 += l[7];
+  c02 += l[8];
+  gaussian2_fit(c10, c01, c11, c20, c02, dpx, dpy);
 }
 
 __device__ __forceinline__ void centroid(const float* m, int n1, int ix, int iy, int n,
@@ -388,6 +459,29 @@ __device__ __forceinline__ int subpix(const float* m, int n0, int n1, int method
   centroid(m, n1, ix, iy, nsub, dpx, dpy);
   if (dpx * dpx + dpy * dpy > thr) return PIV_WIN_SUBPIX_WRONG;
   return PIV_WIN_OK;
+}
+
+// subpix() with the lanes of one window (all T lanes active, every lane
+// gets the result): the '2d_gaussian2' fit by gaussian2_lanes when T lanes
+// of one wavefront are at least 16, the serial fit on every lane otherwise
+template <int T>
+__device__ __forceinline__ int subpix_lanes(const float* m, int n0, int n1, int method, int nsub,
+                                            int ix, int iy, int lane, double& dpx, double& dpy,
+                                            float map_scale) {
+  if constexpr (T >= 16 && T <= 64) {
+    if (method == PIV_SUBPIX_2D_GAUSSIAN2) {
+      dpx = dpy = 0.0;
+      if (iy - nsub < 0 || iy + nsub + 1 > n0 || ix - nsub < 0 || ix + nsub + 1 > n1)
+        return PIV_WIN_SUBPIX_BOUNDARY;
+      const double thr = 2.0 * (0.5 + nsub) * (0.5 + nsub);
+      gaussian2_lanes<T>(m, n1, ix, iy, lane, dpx, dpy);
+      if (!(dpx * dpx + dpy * dpy > thr)) return PIV_WIN_OK;
+      centroid(m, n1, ix, iy, nsub, dpx, dpy);
+      if (dpx * dpx + dpy * dpy > thr) return PIV_WIN_SUBPIX_WRONG;
+      return PIV_WIN_OK;
+    }
+  }
+  return subpix(m, n0, n1, method, nsub, ix, iy, dpx, dpy, map_scale);
 }
 
 // Everything that follows the correlation map of one window: peak search
@@ -510,13 +604,16 @@ __device__ __forceinline__ void finish_window_with(S& s, const float* m, int n0,
       ++nsec;
     }
   }
+  // the fit on the window's lanes, the records by the writer
+  double px = 0.0, py = 0.0;
+  int e = PIV_WIN_OK;
+  if (apply_subpix)
+    e = subpix_lanes<T>(m, n0, n1, pa.subpix_method, pa.nsubpix, i1, i0, lane, px, py,
+                        pa.map_scale);
   if (writer) {
     float dx = float(ix0 - i1), dy = float(iy0 - i0);
     int err = PIV_WIN_OK;
     if (apply_subpix) {
-      double px, py;
-      const int e =
-          subpix(m, n0, n1, pa.subpix_method, pa.nsubpix, i1, i0, px, py, pa.map_scale);
       if (e == PIV_WIN_OK) {
         dx = float(ix0 - (i1 + px));
         dy = float(iy0 - (i0 + py));
