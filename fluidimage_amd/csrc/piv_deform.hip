@@ -67,24 +67,37 @@ __device__ __forceinline__ void keys_weights(float t, float w[4]) {
 }
 
 // the 4 taps of a row of the image starting at element `idx` (in the image,
-// any alignment of the element type) in one vector load
+// element-aligned data)
 template <int DT>
 __device__ __forceinline__ void row_taps(const void* data, int64_t idx, float p[4]);
+// The loads are aligned: a tap row at an arbitrary element is read as the
+// aligned dwords that hold it and shifted into place (unaligned multi-dword
+// loads cost the memory pipeline several passes). They read up to 3 bytes
+// before the taps (in the same dword, so the same page) and up to 4 bytes
+// after them (the next row, or kOverread more elements of the last row: the
+// caller keeps the last columns of the last row out of this path).
+template <int DT>
+constexpr int kOverread = DT == PIV_U8 ? 4 : DT == PIV_U16 ? 2 : 0;
+
 template <>
 __device__ __forceinline__ void row_taps<PIV_U8>(const void* data, int64_t idx, float p[4]) {
-  typedef uint32_t __attribute__((aligned(1))) u32a1;
-  const uint32_t w = *((const __attribute__((address_space(1))) u32a1*)(
-      (const __attribute__((address_space(1))) uint8_t*)data + idx));
+  const uintptr_t byte = reinterpret_cast<uintptr_t>(data) + uintptr_t(idx);
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2), aligned(4)));
+  const u32x2 d = *((const __attribute__((address_space(1))) u32x2*)(byte & ~uintptr_t(3)));
+  const uint32_t w = __builtin_amdgcn_alignbit(d[1], d[0], uint32_t(byte & 3) * 8);
 #pragma unroll
   for (int b = 0; b < 4; ++b) p[b] = float((w >> (8 * b)) & 0xffu);  // 32-bit lanes
 }
 template <>
 __device__ __forceinline__ void row_taps<PIV_U16>(const void* data, int64_t idx, float p[4]) {
-  typedef uint64_t __attribute__((aligned(2))) u64a2;
-  const uint64_t w = *((const __attribute__((address_space(1))) u64a2*)(
-      (const __attribute__((address_space(1))) uint16_t*)data + idx));
-  // split into 32-bit halves: the conversions stay uint32 -> float
-  const uint32_t lo = uint32_t(w), hi = uint32_t(w >> 32);
+  // three aligned dwords holding the 8 bytes, which start at byte 0 or 2 of
+  // the first one
+  const uintptr_t byte = reinterpret_cast<uintptr_t>(data) + uintptr_t(idx) * 2;
+  typedef uint32_t u32x3 __attribute__((ext_vector_type(3), aligned(4)));
+  const u32x3 w = *((const __attribute__((address_space(1))) u32x3*)(byte & ~uintptr_t(3)));
+  const uint32_t sh = uint32_t(byte & 3) * 8;
+  const uint32_t lo = __builtin_amdgcn_alignbit(w[1], w[0], sh);
+  const uint32_t hi = __builtin_amdgcn_alignbit(w[2], w[1], sh);
   p[0] = float(lo & 0xffffu);
   p[1] = float(lo >> 16);
   p[2] = float(hi & 0xffffu);
@@ -111,7 +124,8 @@ __device__ float bicubic_taps(const piv_image& im, float m, int iy, int ix, floa
   keys_weights(ty, wy);
   keys_weights(tx, wx);
   float acc = 0.0f;
-  if (iy >= 0 && iy <= im.height - 4 && ix >= 0 && ix <= im.width - 4) {
+  if (iy >= 0 && iy <= im.height - 4 && ix >= 0 && ix <= im.width - 4 &&
+      (iy < im.height - 4 || ix <= im.width - 4 - kOverread<DT>)) {
     // interior: the 16 taps inside the image, sum w p - m sum w
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
