@@ -16,6 +16,10 @@ the windows of every pair, deformed windows in passes 2 and 3, fix, one
 batched FP64 TPS per pass, final interpolation); the images are resident in HBM before the timed region.
 value = final-pass vectors (65,025 per pair) x P per second. The same
 pipeline on one pair (the plain per-pair path) is timed in the same run.
+The TPS checks of a step (singular systems, capacities) are read while the
+next step runs, and those of the last step after the timed region; a failed
+check fails the bench. The per-stage times come from two more steps with
+HIP events around the stages, run after the timed ones.
 
 Workload "preproc": the sliding median / minima of 2048x2048 uint16 images.
 Workload "single": the single-pass correlation of the same kind of pairs
@@ -231,18 +235,24 @@ def tuning_in_use():
     return t
 
 
-def timed(step, steps, warmup):
+def timed(step, steps, warmup, breakdown=2):
+    """Wall time of `steps` steps after `warmup` ones, and the per-stage
+    averages (ms) of `breakdown` more steps run after the timed ones with
+    HIP events around their stages (the timed steps record no events)."""
     import torch
 
     for _ in range(warmup):
         step(None)
     torch.cuda.synchronize()
-    timers = {}
     t0 = time.perf_counter()
     for _ in range(steps):
-        step(timers)
+        step(None)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    timers = {}
+    for _ in range(breakdown):
+        step(timers)
+    torch.cuda.synchronize()
     avg = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in timers.items()}
     return wall, avg
 
@@ -305,12 +315,25 @@ def run_bench(args, pairs, keep_maps, steps, warmup):
         wall, avg = timed(step, steps, warmup)
         return work, images, state["passes"], wall / steps * 1e3, avg
 
+    def check_tps(pending):
+        failed = pipe.check_pending(pending)
+        if failed:
+            raise RuntimeError(f"bench: TPS checks failed for pairs {sorted(failed)}")
+
     def step(timers):
+        # the TPS checks of a step are read while the next step runs (as a
+        # caller processing a series would: pairs failing one would be run
+        # again), so the host stays ahead of the GPU
         images.recompute_min()
         state["passes"] = pipe.run(images, len(pairs), keep_maps=keep_maps, timers=timers,
-                                   final_interp=args.workload == "multipass")
+                                   final_interp=args.workload == "multipass", check=False)
+        prev = state.get("pending")
+        state["pending"] = list(pipe._pending)
+        if prev is not None:
+            check_tps(prev)
 
     wall, avg = timed(step, steps, warmup)
+    check_tps(state["pending"])
     return work, images, state["passes"], wall / steps * 1e3, avg
 
 

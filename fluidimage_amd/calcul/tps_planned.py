@@ -139,9 +139,23 @@ class PlannedTps:
     docstring); nothing is read back until ``field_errors``."""
 
     def __init__(self, cap, m, tensors):
+        import torch
+
         self.cap = cap
         self.m = m
         self.__dict__.update(tensors)
+        # the checks copied to pinned host memory behind the call's kernels,
+        # with an event: field_errors then waits for this call alone, not
+        # for work queued on the stream after it (not under graph capture,
+        # where field_errors reads the device tensors)
+        self._checks = None
+        if not torch.cuda.is_current_stream_capturing():
+            flat = torch.cat([self.fstat.to(torch.int64), self.status.to(torch.int64)])
+            host = torch.empty(flat.shape, dtype=torch.int64, pin_memory=True)
+            host.copy_(flat, non_blocking=True)
+            done = torch.cuda.Event()
+            done.record()
+            self._checks = (host, done)
 
     def approx(self, k):
         """(x, y) float64 device tensors of field k at the new positions."""
@@ -155,7 +169,13 @@ class PlannedTps:
         4 subdomains. One device-to-host read."""
         import torch
 
-        host = torch.cat([self.fstat.to(torch.int64), self.status.to(torch.int64)]).cpu().numpy()
+        if self._checks is not None:
+            pinned, done = self._checks
+            done.synchronize()
+            host = pinned.numpy()
+        else:
+            host = torch.cat([self.fstat.to(torch.int64),
+                              self.status.to(torch.int64)]).cpu().numpy()
         nf, nd = self.cap.nfields, self.cap.nd
         fstat, status = host[:nf], host[nf:]
         if status[nd]:
