@@ -1500,7 +1500,8 @@ __device__ int block_scan_1024(int v, int* tmp, int& total) {
 
 // one block of 1024 threads: ordered compaction of the vectors whose dx and
 // dy are finite (singlepass.py:450-463) and the bounds of their positions;
-// consecutive threads read consecutive vectors, tile by tile
+// consecutive threads read consecutive vectors, four tiles of 1024 per
+// barrier round (wavefront ballots, per-wavefront counts in LDS)
 __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restrict__ dx,
                                                      const float* __restrict__ dy,
                                                      const double* __restrict__ xs,
@@ -1523,40 +1524,64 @@ __global__ __launch_bounds__(1024) void k_tps_select(int n, const float* __restr
     info += 7 * blockIdx.x;
   }
   const double inv_step = lat_step > 0.0 ? 1.0 / lat_step : 0.0;
-  __shared__ int tmp[32];
+  constexpr int kTiles = 4;  // tiles of 1024 vectors per barrier round
+  __shared__ int wcount[2][kTiles][16];  // valid vectors per tile and wavefront (double-buffered)
   __shared__ double red[4][16];
   double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
   int base = 0;
   int unsorted = 0;   // some ys[i] > ys[i + 1]
   int off_lattice = lat_step > 0.0 ? 0 : 1;  // some selected position not a multiple of the step
-  for (int t0 = 0; t0 < n; t0 += 1024) {
-    const int i = t0 + threadIdx.x;
-    float a = 0.f, b = 0.f;
-    int f = 0;
-    if (i < n) {
-      a = dx[i];
-      b = dy[i];
-      f = (a == a && b == b) ? 1 : 0;
-      if (i + 1 < n && !(ys[i] <= ys[i + 1])) unsorted = 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  for (int t0 = 0, round = 0; t0 < n; t0 += 1024 * kTiles, round ^= 1) {
+    float a[kTiles], b[kTiles];
+    uint64_t mask[kTiles];
+#pragma unroll
+    for (int s = 0; s < kTiles; ++s) {
+      const int i = t0 + 1024 * s + int(threadIdx.x);
+      a[s] = 0.f;
+      b[s] = 0.f;
+      bool f = false;
+      if (i < n) {
+        a[s] = dx[i];
+        b[s] = dy[i];
+        f = a[s] == a[s] && b[s] == b[s];
+        if (i + 1 < n && !(ys[i] <= ys[i + 1])) unsorted = 1;
+      }
+      mask[s] = __ballot(f);
+      if (lane == 0) wcount[round][s][wv] = int(__popcll(mask[s]));
     }
-    int total;
-    const int incl = block_scan_1024(f, tmp, total);
-    if (f) {
-      const int pos = base + incl - 1;
-      const double x = xs[i], y = ys[i];
-      cx[pos] = x;
-      cy[pos] = y;
-      vx[pos] = a;
-      vy[pos] = b;
-      idx[pos] = i;
-      if (rint(x * inv_step) * lat_step != x || rint(y * inv_step) * lat_step != y)
-        off_lattice = 1;
-      xmin = fmin(xmin, x);
-      xmax = fmax(xmax, x);
-      ymin = fmin(ymin, y);
-      ymax = fmax(ymax, y);
+    __syncthreads();
+    // vector (tile s, lane of wavefront wv) goes after the valid vectors of
+    // the tiles before s and of the wavefronts before wv in tile s
+    int start = base;
+#pragma unroll
+    for (int s = 0; s < kTiles; ++s) {
+      int in_tile = 0, tile_total = 0;
+      for (int w = 0; w < 16; ++w) {
+        const int c = wcount[round][s][w];
+        in_tile += w < wv ? c : 0;
+        tile_total += c;
+      }
+      const int i = t0 + 1024 * s + int(threadIdx.x);
+      if ((mask[s] >> lane) & 1) {
+        const int pos = start + in_tile + int(__popcll(mask[s] & below));
+        const double x = xs[i], y = ys[i];
+        cx[pos] = x;
+        cy[pos] = y;
+        vx[pos] = a[s];
+        vy[pos] = b[s];
+        idx[pos] = i;
+        if (rint(x * inv_step) * lat_step != x || rint(y * inv_step) * lat_step != y)
+          off_lattice = 1;
+        xmin = fmin(xmin, x);
+        xmax = fmax(xmax, x);
+        ymin = fmin(ymin, y);
+        ymax = fmax(ymax, y);
+      }
+      start += tile_total;
     }
-    base += total;
+    base = start;
   }
   unsorted = __syncthreads_or(unsorted);
   off_lattice = __syncthreads_or(off_lattice);
