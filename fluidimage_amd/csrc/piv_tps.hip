@@ -2001,6 +2001,7 @@ __global__ __launch_bounds__(1024) void k_tps_offsets(
   __shared__ int flags[kOffsetsMaxFields];
   __shared__ int hist[kMaxN + 2];
   __shared__ int64_t tmp64[32];
+  __shared__ int tmpi[32];
   __shared__ int64_t fslots[kOffsetsMaxFields];
   __shared__ int64_t fmats[kOffsetsMaxFields];
   const int nd = nfields * dom_cap;
@@ -2113,11 +2114,26 @@ __global__ __launch_bounds__(1024) void k_tps_offsets(
     totals[1] = base_m;
     totals[2] = nm;
     totals[3] = na;
-    int run = 0;
-    for (int b = kMaxN + 1; b >= 0; --b) {
-      const int h = hist[b];
-      hist[b] = run;
-      run += h;
+  }
+  {
+    // hist[b] <- the subdomains of sizes above b (an exclusive scan of the
+    // bins from the largest size down, kPer bins a thread)
+    constexpr int kBins = kMaxN + 2, kPer = (kBins + 1023) / 1024;
+    int h[kPer];
+    int local = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int r = kPer * tid + k;
+      h[k] = r < kBins ? hist[kBins - 1 - r] : 0;
+      local += h[k];
+    }
+    int total;
+    int run = block_scan_1024(local, tmpi, total) - local;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int r = kPer * tid + k;
+      if (r < kBins) hist[kBins - 1 - r] = run;
+      run += h[k];
     }
   }
   __syncthreads();
