@@ -2065,27 +2065,44 @@ __global__ __launch_bounds__(1024) void k_tps_offsets(
   }
   // 4. prefix sums of the admitted subdomains' slots and matrix sizes; the
   //    histogram of their sizes
+  //    (kPer consecutive subdomains a thread, so one round covers 8192)
   int64_t base_s = 0, base_m = 0;
   int n_max = 0, active = 0;
-  for (int c0 = 0; c0 < nd; c0 += 1024) {
-    const int d = c0 + tid;
-    int c = 0;
-    if (d < nd) {
-      const int f = dom_field[d];
-      if (f >= 0 && flags[f] == PIV_TPS_FIELD_OK) c = counts[d];
+  constexpr int kPer = 8;
+  for (int c0 = 0; c0 < nd; c0 += 1024 * kPer) {
+    int cs[kPer];
+    int64_t ls = 0, lm = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int d = c0 + kPer * tid + k;
+      int c = 0;
+      if (d < nd) {
+        const int f = dom_field[d];
+        if (f >= 0 && flags[f] == PIV_TPS_FIELD_OK) c = counts[d];
+      }
+      cs[k] = c;
+      ls += c;
+      lm += c > 0 ? tps_mat_size(c) : 0;
     }
     int64_t ts, tm;
-    const int64_t is = block_scan_1024_64(int64_t(c), tmp64, ts);
-    const int64_t im = block_scan_1024_64(c > 0 ? tps_mat_size(c) : 0, tmp64, tm);
-    if (d < nd) {
-      dom_off[d] = base_s + is - c;
-      mat_off[d] = base_m + im - (c > 0 ? tps_mat_size(c) : 0);
-      atomicAdd(&hist[c], 1);
+    int64_t os = base_s + block_scan_1024_64(ls, tmp64, ts) - ls;
+    int64_t om = base_m + block_scan_1024_64(lm, tmp64, tm) - lm;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int d = c0 + kPer * tid + k;
+      const int c = cs[k];
+      if (d < nd) {
+        dom_off[d] = os;
+        mat_off[d] = om;
+        atomicAdd(&hist[c], 1);
+      }
+      os += c;
+      om += c > 0 ? tps_mat_size(c) : 0;
+      n_max = max(n_max, c);
+      active += c > 0 ? 1 : 0;
     }
     base_s += ts;
     base_m += tm;
-    n_max = max(n_max, c);
-    active += c > 0 ? 1 : 0;
   }
   if (tid == 0) {
     dom_off[nd] = base_s;
